@@ -1,0 +1,196 @@
+/* C ABI of the MI355X projective-dynamics step.
+ *
+ * This is the seam between the C++20 silk::World front end and the HIP device
+ * library.  It replaces the reference's internal backend interface
+ * (silk/src/common/backend.hpp:9-45, implemented by CudaBackend in
+ * silk/src/backend/cuda/cuda_backend.cu:64-352): every World call maps to one
+ * function here.  Signatures use plain pointers, sizes and POD structs only.
+ *
+ * Conventions
+ *  - Every function returns an int status: SILK_HIP_OK (0) or an error code
+ *    whose value equals the silk::ErrorCode enumerator + 1.
+ *  - Caller buffers are read (or written) during the call only and are never
+ *    retained.
+ *  - Calls are synchronous: the device stream is drained before returning.
+ *  - One context is driven from one thread at a time.
+ *  - Device and runtime faults never unwind across the ABI: they return
+ *    SILK_HIP_ERR_UNKNOWN and silk_hip_last_error() describes them.
+ *  - Handles are non-zero, unique within a context (reference:
+ *    silk/src/ecs/registry.hpp:70-85).
+ *  - Device work is deferred to the first silk_hip_step after a change, as
+ *    in the reference (silk/src/backend/cuda/main_loop.cu:58-110).
+ */
+#ifndef SILK_HIP_H_
+#define SILK_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SILK_HIP_OK = 0,
+  SILK_HIP_ERR_UNKNOWN = 1,
+  SILK_HIP_ERR_NOT_INITIALIZED = 2,
+  SILK_HIP_ERR_INVALID_CONFIG = 3,
+  SILK_HIP_ERR_TOO_MANY_BODY = 4,
+  SILK_HIP_ERR_INVALID_HANDLE = 5,
+  SILK_HIP_ERR_INVALID_MESH = 6,
+  SILK_HIP_ERR_INVALID_PIN = 7,
+  SILK_HIP_ERR_INVALID_POSITION = 8,
+  SILK_HIP_ERR_CHOLESKY_FAIL = 9,
+  SILK_HIP_ERR_NEED_INIT_SOLVER = 10,
+  SILK_HIP_ERR_ITERATIVE_SOLVE_FAIL = 11,
+  SILK_HIP_ERR_NO_DEVICE = 12
+};
+
+typedef struct silk_hip_ctx silk_hip_ctx;
+
+/* silk::ClothConfig (silk/include/silk/silk.hpp:37-47). */
+typedef struct {
+  float elastic_stiffness;
+  float bending_stiffness;
+  float density;
+  float damping;
+} silk_hip_cloth_params;
+
+/* silk::CollisionConfig (silk/include/silk/silk.hpp:28-34). */
+typedef struct {
+  int32_t is_collision_on;
+  int32_t is_self_collision_on;
+  int32_t group;
+  float restitution;
+  float friction;
+} silk_hip_collision_params;
+
+/* silk::GlobalConfig (silk/include/silk/silk.hpp:50-77), plus solver knobs
+ * that the reference keeps as hidden constants (main_loop.cuh:21-26,
+ * contact_constraints.cuh:89, mas_cg_solver.cuh:22).  Fields after
+ * admm_rel_tol are extensions: zero selects the reference value. */
+typedef struct {
+  float acceleration[3];
+  int32_t max_outer_iteration;
+  int32_t max_inner_iteration;
+  float dt;
+  float linear_solver_abs_tol;
+  float linear_solver_rel_tol_min;
+  float linear_solver_rel_tol_max;
+  float initial_linear_rel_tol;
+  float admm_abs_tol;
+  float admm_rel_tol;
+  /* Extensions (0 = reference default). */
+  int32_t pcg_max_iter;           /* 1000 */
+  int32_t contact_sweeps;         /* 10 */
+  int32_t ccd_max_iter;           /* 4096 */
+  float ccd_tolerance;            /* 1e-6 */
+  float ccd_min_separation_scale; /* 0.2 */
+  float dcd_activation_scale;     /* 1.0 */
+  int32_t mas_precision;          /* 0 = fp32 blocks, 1 = fp16 blocks */
+  int32_t reserved[7];
+} silk_hip_global;
+
+/* Per-step counters returned by silk_hip_step. */
+typedef struct {
+  int32_t outer_iterations;  /* CCD segments taken */
+  int32_t admm_iterations;   /* summed over segments */
+  int32_t pcg_iterations;    /* summed over all PCG solves */
+  int32_t pcg_solves;
+  int32_t contacts;          /* active Daviet contacts at step end */
+  int32_t contact_colors;
+  int32_t ccd_candidates;    /* summed over segments */
+  int32_t dcd_candidates;
+  float min_toi;             /* earliest accepted TOI fraction (1 = none) */
+  float last_pcg_rel_residual; /* sqrt(rr/rr0) of the last PCG solve */
+  int32_t reserved[6];
+} silk_hip_step_stats;
+
+/* Number of visible HIP devices (0 when none).  Never fails. */
+int silk_hip_device_count(int32_t* count);
+
+/* Create a context bound to `device`.  No device call is made until the first
+ * silk_hip_step; a context without a usable device reports
+ * SILK_HIP_ERR_NO_DEVICE from silk_hip_step. */
+int silk_hip_create(int32_t device, silk_hip_ctx** out);
+int silk_hip_destroy(silk_hip_ctx* ctx);
+
+/* Human readable description of the last error on this context ("" if none).
+ * The string stays valid until the next call on the context. */
+const char* silk_hip_last_error(const silk_hip_ctx* ctx);
+
+/* World::set_global_config (cuda_backend.cu:68-85). */
+int silk_hip_set_global(silk_hip_ctx* ctx, const silk_hip_global* global);
+
+/* World::add_cloth (cuda_backend.cu:110-151).  verts: 3*nv floats, faces:
+ * 3*nf ints, pins: npin vertex indices.  On an invalid mesh *handle = 0 and
+ * SILK_HIP_ERR_INVALID_MESH is returned. */
+int silk_hip_add_cloth(silk_hip_ctx* ctx, const silk_hip_cloth_params* cloth,
+                       const silk_hip_collision_params* collision,
+                       const float* verts, int64_t nv, const int32_t* faces,
+                       int64_t nf, const int32_t* pins, int64_t npin,
+                       uint32_t* handle);
+
+/* World::add_obstacle (cuda_backend.cu:269-299). */
+int silk_hip_add_obstacle(silk_hip_ctx* ctx,
+                          const silk_hip_collision_params* collision,
+                          const float* verts, int64_t nv,
+                          const int32_t* faces, int64_t nf, uint32_t* handle);
+
+/* World::remove_cloth / remove_obstacle (cuda_backend.cu:153-160,301-307).
+ * is_cloth selects which kind `handle` must be. */
+int silk_hip_remove(silk_hip_ctx* ctx, uint32_t handle, int32_t is_cloth);
+
+/* World::set_cloth_config (cuda_backend.cu:197-204). */
+int silk_hip_set_cloth_params(silk_hip_ctx* ctx, uint32_t handle,
+                              const silk_hip_cloth_params* cloth);
+
+/* World::set_cloth_collision_config / set_obstacle_collision_config
+ * (cuda_backend.cu:206-222,309-325). */
+int silk_hip_set_collision_params(silk_hip_ctx* ctx, uint32_t handle,
+                                  int32_t is_cloth,
+                                  const silk_hip_collision_params* collision);
+
+/* World::set_cloth_pin_index (cuda_backend.cu:224-241). */
+int silk_hip_set_pin_index(silk_hip_ctx* ctx, uint32_t handle,
+                           const int32_t* pins, int64_t npin);
+
+/* World::set_cloth_pin_position (cuda_backend.cu:243-267): 3*npin floats. */
+int silk_hip_set_pin_positions(silk_hip_ctx* ctx, uint32_t handle,
+                               const float* positions, int64_t n);
+
+/* World::set_obstacle_position (cuda_backend.cu:327-352): 3*nv floats. */
+int silk_hip_set_obstacle_positions(silk_hip_ctx* ctx, uint32_t handle,
+                                    const float* positions, int64_t n);
+
+/* World::solver_step (cuda_backend.cu:87-96).  stats may be NULL. */
+int silk_hip_step(silk_hip_ctx* ctx, silk_hip_step_stats* stats);
+
+/* World::solver_reset (cuda_backend.cu:98-108). */
+int silk_hip_reset(silk_hip_ctx* ctx);
+
+/* World::get_cloth_position (cuda_backend.cu:162-195): n must equal 3*nv.
+ * Positions are returned in the caller's original vertex order. */
+int silk_hip_get_positions(const silk_hip_ctx* ctx, uint32_t handle,
+                           float* out, int64_t n);
+
+/* ---- Introspection (no reference counterpart; used by tests and bench) ---- */
+
+/* Vertex partition used by the MAS preconditioner for one cloth: perm maps
+ * original vertex -> solver order (nv ints); offsets holds nparts+1 ints.
+ * Pass NULL buffers to query sizes. */
+int silk_hip_get_partition(const silk_hip_ctx* ctx, uint32_t handle,
+                           int32_t* perm, int64_t nv, int32_t* offsets,
+                           int64_t offsets_cap, int32_t* nparts);
+
+/* Elapsed device time (ms) of the named kernel class accumulated since the
+ * last call with reset != 0.  Classes: 0 = whole step, 1 = PCG iterations,
+ * 2 = local projection, 3 = contact solve, 4 = broadphase, 5 = CCD. */
+int silk_hip_get_timing(silk_hip_ctx* ctx, int32_t cls, double* ms,
+                        int64_t* launches, int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SILK_HIP_H_ */

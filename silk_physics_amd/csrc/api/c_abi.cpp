@@ -1,0 +1,336 @@
+// extern "C" entry points of include/silk_hip.h.  Every function converts
+// exceptions into status codes; nothing unwinds across the ABI.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "api/context.hpp"
+#include "silk_hip.h"
+
+using silk::amd::ClothEntity;
+using silk::amd::Context;
+using silk::amd::ObstacleEntity;
+using silk::amd::TriMesh;
+
+struct silk_hip_ctx {
+  Context impl;
+  explicit silk_hip_ctx(int d) : impl(d) {}
+};
+
+namespace {
+
+template <typename F>
+int guarded(silk_hip_ctx* ctx, F&& f) {
+  if (ctx == nullptr) return SILK_HIP_ERR_NOT_INITIALIZED;
+  ctx->impl.last_error_.clear();
+  try {
+    return f(ctx->impl);
+  } catch (const std::bad_alloc&) {
+    ctx->impl.last_error_ = "out of memory";
+  } catch (const std::exception& e) {
+    ctx->impl.last_error_ = e.what();
+    if (std::string(e.what()) == "no HIP device") return SILK_HIP_ERR_NO_DEVICE;
+  } catch (...) {
+    ctx->impl.last_error_ = "unknown exception";
+  }
+  return SILK_HIP_ERR_UNKNOWN;
+}
+
+bool valid_pins(const int32_t* pins, int64_t npin, int nv) {
+  if (npin < 0 || npin > nv) return false;
+  if (npin > 0 && pins == nullptr) return false;
+  for (int64_t i = 0; i < npin; ++i)
+    if (pins[i] < 0 || pins[i] >= nv) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int silk_hip_device_count(int32_t* count) {
+  if (count == nullptr) return SILK_HIP_ERR_INVALID_CONFIG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    n = 0;
+  }
+  *count = n;
+  return SILK_HIP_OK;
+}
+
+int silk_hip_create(int32_t device, silk_hip_ctx** out) {
+  if (out == nullptr) return SILK_HIP_ERR_INVALID_CONFIG;
+  *out = nullptr;
+  try {
+    *out = new silk_hip_ctx(device);
+  } catch (...) {
+    return SILK_HIP_ERR_UNKNOWN;
+  }
+  return SILK_HIP_OK;
+}
+
+int silk_hip_destroy(silk_hip_ctx* ctx) {
+  try {
+    delete ctx;
+  } catch (...) {
+    return SILK_HIP_ERR_UNKNOWN;
+  }
+  return SILK_HIP_OK;
+}
+
+const char* silk_hip_last_error(const silk_hip_ctx* ctx) {
+  if (ctx == nullptr) return "null context";
+  return ctx->impl.last_error_.c_str();
+}
+
+int silk_hip_set_global(silk_hip_ctx* ctx, const silk_hip_global* g) {
+  return guarded(ctx, [&](Context& c) {
+    if (g == nullptr) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    c.global_ = *g;
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_add_cloth(silk_hip_ctx* ctx, const silk_hip_cloth_params* cloth,
+                       const silk_hip_collision_params* collision,
+                       const float* verts, int64_t nv, const int32_t* faces,
+                       int64_t nf, const int32_t* pins, int64_t npin,
+                       uint32_t* handle) {
+  return guarded(ctx, [&](Context& c) {
+    if (handle == nullptr || cloth == nullptr || collision == nullptr)
+      return int{SILK_HIP_ERR_INVALID_CONFIG};
+    TriMesh mesh;
+    std::string why;
+    if (!silk::amd::make_cloth_mesh(verts, 3 * nv, faces, 3 * nf, mesh, why)) {
+      *handle = 0;
+      c.last_error_ = why;
+      return int{SILK_HIP_ERR_INVALID_MESH};
+    }
+    if (!valid_pins(pins, npin, mesh.nv())) {
+      c.last_error_ = "pin index out of range";
+      return int{SILK_HIP_ERR_INVALID_PIN};
+    }
+    c.invalidate_scene();
+    ClothEntity e;
+    e.handle = c.new_handle();
+    e.cloth = *cloth;
+    e.coll = *collision;
+    e.mesh = std::move(mesh);
+    e.pins.assign(pins, pins + npin);
+    c.cloths_.push_back(std::move(e));
+    *handle = c.cloths_.back().handle;
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_add_obstacle(silk_hip_ctx* ctx,
+                          const silk_hip_collision_params* collision,
+                          const float* verts, int64_t nv,
+                          const int32_t* faces, int64_t nf, uint32_t* handle) {
+  return guarded(ctx, [&](Context& c) {
+    if (handle == nullptr || collision == nullptr)
+      return int{SILK_HIP_ERR_INVALID_CONFIG};
+    TriMesh mesh;
+    std::string why;
+    if (!silk::amd::make_obstacle_mesh(verts, 3 * nv, faces, 3 * nf, mesh,
+                                       why)) {
+      *handle = 0;
+      c.last_error_ = why;
+      return int{SILK_HIP_ERR_INVALID_MESH};
+    }
+    c.invalidate_scene();
+    ObstacleEntity o;
+    o.handle = c.new_handle();
+    o.coll = *collision;
+    o.mesh = std::move(mesh);
+    c.obstacles_.push_back(std::move(o));
+    *handle = c.obstacles_.back().handle;
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_remove(silk_hip_ctx* ctx, uint32_t handle, int32_t is_cloth) {
+  return guarded(ctx, [&](Context& c) {
+    if (is_cloth) {
+      if (!c.find_cloth(handle)) return int{SILK_HIP_ERR_INVALID_HANDLE};
+      c.invalidate_scene();
+      c.cloths_.erase(std::find_if(
+          c.cloths_.begin(), c.cloths_.end(),
+          [&](const ClothEntity& e) { return e.handle == handle; }));
+    } else {
+      if (!c.find_obstacle(handle)) return int{SILK_HIP_ERR_INVALID_HANDLE};
+      c.invalidate_scene();
+      c.obstacles_.erase(std::find_if(
+          c.obstacles_.begin(), c.obstacles_.end(),
+          [&](const ObstacleEntity& e) { return e.handle == handle; }));
+    }
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_set_cloth_params(silk_hip_ctx* ctx, uint32_t handle,
+                              const silk_hip_cloth_params* cloth) {
+  return guarded(ctx, [&](Context& c) {
+    ClothEntity* e = c.find_cloth(handle);
+    if (!e) return int{SILK_HIP_ERR_INVALID_HANDLE};
+    if (cloth == nullptr) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    c.invalidate_scene();
+    e = c.find_cloth(handle);
+    e->cloth = *cloth;
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_set_collision_params(silk_hip_ctx* ctx, uint32_t handle,
+                                  int32_t is_cloth,
+                                  const silk_hip_collision_params* coll) {
+  return guarded(ctx, [&](Context& c) {
+    if (coll == nullptr) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    if (is_cloth) {
+      if (!c.find_cloth(handle)) return int{SILK_HIP_ERR_INVALID_HANDLE};
+      c.invalidate_scene();
+      c.find_cloth(handle)->coll = *coll;
+    } else {
+      if (!c.find_obstacle(handle)) return int{SILK_HIP_ERR_INVALID_HANDLE};
+      c.invalidate_scene();
+      c.find_obstacle(handle)->coll = *coll;
+    }
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_set_pin_index(silk_hip_ctx* ctx, uint32_t handle,
+                           const int32_t* pins, int64_t npin) {
+  return guarded(ctx, [&](Context& c) {
+    ClothEntity* e = c.find_cloth(handle);
+    if (!e) return int{SILK_HIP_ERR_INVALID_HANDLE};
+    for (int64_t i = 0; i < npin; ++i)
+      if (pins == nullptr || pins[i] < 0 || pins[i] >= e->mesh.nv())
+        return int{SILK_HIP_ERR_INVALID_PIN};
+    c.invalidate_scene();
+    e = c.find_cloth(handle);
+    e->pins.assign(pins, pins + npin);
+    e->has_pin_pos = false;  // re-seeded from the initial mesh
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_set_pin_positions(silk_hip_ctx* ctx, uint32_t handle,
+                               const float* positions, int64_t n) {
+  return guarded(ctx, [&](Context& c) {
+    ClothEntity* e = c.find_cloth(handle);
+    if (!e) return int{SILK_HIP_ERR_INVALID_HANDLE};
+    if (n != 3 * static_cast<int64_t>(e->pins.size()) ||
+        (n > 0 && positions == nullptr))
+      return int{SILK_HIP_ERR_INVALID_PIN};
+    e->pin_curr.assign(positions, positions + n);
+    e->has_pin_pos = true;
+    c.update_pin_targets(*e);
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_set_obstacle_positions(silk_hip_ctx* ctx, uint32_t handle,
+                                    const float* positions, int64_t n) {
+  return guarded(ctx, [&](Context& c) {
+    ObstacleEntity* o = c.find_obstacle(handle);
+    if (!o) return int{SILK_HIP_ERR_INVALID_HANDLE};
+    if (n != static_cast<int64_t>(o->mesh.V.size()) || positions == nullptr)
+      return int{SILK_HIP_ERR_INVALID_POSITION};
+    if (o->has_pos) {
+      o->is_static = false;
+      o->is_static_twice = false;
+      std::swap(o->curr, o->prev);
+      o->curr.assign(positions, positions + n);
+    } else {
+      // A fresh record is static until the next change
+      // (cuda_backend.cu:339-349, pin.cpp:337-345).
+      o->curr.assign(positions, positions + n);
+      o->prev = o->curr;
+      o->is_static = true;
+      o->is_static_twice = true;
+      o->has_pos = true;
+    }
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_step(silk_hip_ctx* ctx, silk_hip_step_stats* stats) {
+  return guarded(ctx, [&](Context& c) { return c.step(stats); });
+}
+
+int silk_hip_reset(silk_hip_ctx* ctx) {
+  return guarded(ctx, [&](Context& c) {
+    c.scene_.reset();
+    c.scene_dirty_ = true;
+    for (auto& e : c.cloths_) {
+      e.has_state = false;
+      e.has_pin_pos = false;
+    }
+    for (auto& o : c.obstacles_) {
+      o.has_pos = false;
+      o.is_static = true;
+      o.is_static_twice = true;
+    }
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_get_positions(const silk_hip_ctx* cctx, uint32_t handle,
+                           float* out, int64_t n) {
+  silk_hip_ctx* ctx = const_cast<silk_hip_ctx*>(cctx);
+  return guarded(ctx, [&](Context& c) {
+    ClothEntity* e = c.find_cloth(handle);
+    if (!e) return int{SILK_HIP_ERR_INVALID_HANDLE};
+    if (out == nullptr || n != static_cast<int64_t>(e->mesh.V.size()))
+      return int{SILK_HIP_ERR_INVALID_POSITION};
+    c.get_positions(*e, out);
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_get_partition(const silk_hip_ctx* cctx, uint32_t handle,
+                           int32_t* perm, int64_t nv, int32_t* offsets,
+                           int64_t offsets_cap, int32_t* nparts) {
+  silk_hip_ctx* ctx = const_cast<silk_hip_ctx*>(cctx);
+  return guarded(ctx, [&](Context& c) {
+    ClothEntity* e = c.find_cloth(handle);
+    if (!e) return int{SILK_HIP_ERR_INVALID_HANDLE};
+    if (!e->part_valid) {
+      e->part = silk::amd::partition_vertices(e->mesh.V,
+                                              silk::amd::MasTopology::kBank);
+      e->pmesh = silk::amd::permute_mesh(e->mesh, e->part);
+      e->part_valid = true;
+      e->geom_valid = false;
+    }
+    const int np = static_cast<int>(e->part.offsets.size()) - 1;
+    if (nparts) *nparts = np;
+    if (perm != nullptr) {
+      if (nv != e->mesh.nv()) return int{SILK_HIP_ERR_INVALID_POSITION};
+      std::copy(e->part.perm.begin(), e->part.perm.end(), perm);
+    }
+    if (offsets != nullptr) {
+      if (offsets_cap < np + 1) return int{SILK_HIP_ERR_INVALID_POSITION};
+      std::copy(e->part.offsets.begin(), e->part.offsets.end(), offsets);
+    }
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_get_timing(silk_hip_ctx* ctx, int32_t cls, double* ms,
+                        int64_t* launches, int32_t reset) {
+  return guarded(ctx, [&](Context& c) {
+    if (cls < 0 || cls >= 8) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    if (ms) *ms = c.timing_ms_[cls];
+    if (launches) *launches = c.timing_launch_[cls];
+    if (reset) {
+      c.timing_ms_[cls] = 0.0;
+      c.timing_launch_[cls] = 0;
+    }
+    return int{SILK_HIP_OK};
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// Collision subsystem of the step: colliders, broadphase, DCD contacts, CCD
+// and the Daviet contact projection (reference: collision/*.cu,
+// solver/contact_constraints.cu, solver/contact_coloring.cu).
+#pragma once
+
+#include <memory>
+
+#include "silk_hip.h"
+
+namespace silk::amd {
+
+class Context;
+struct Scene;
+
+/// Device pointers the ADMM kernels need from the contact set.  All null when
+/// no contact is active.
+struct ContactView {
+  const float* w = nullptr;     // nv: active per-vertex contact weight
+  const float* p = nullptr;     // 3*nv: projected position
+  const float* eta = nullptr;   // 3*nv: multiplier
+  const float* dual = nullptr;  // 3*nv: w (p - p_prev)
+};
+
+class CollisionWorld {
+ public:
+  virtual ~CollisionWorld() = default;
+  /// Frame start: refit obstacles and cloth colliders, minimum separation and
+  /// activation distance, initial DCD contact set (main_loop.cu:186-216).
+  virtual void begin_step(const float* outer_x, float* ms, float* dhat,
+                          silk_hip_step_stats* stats) = 0;
+  /// Earliest TOI of the segment outer_x -> inner_x in frame-time units,
+  /// `interval` when none (find_collision.cu:140-190).
+  virtual float min_toi(const float* outer_x, const float* inner_x,
+                        float time_start, float interval, float ms,
+                        silk_hip_step_stats* stats) = 0;
+  /// After a rollback: refit at outer_x and append DCD contacts at `time`
+  /// (main_loop.cu:267-277).
+  virtual void rollback_contacts(const float* outer_x, float time, float dhat,
+                                 silk_hip_step_stats* stats) = 0;
+  /// ADMM contact update for the current x (contact_constraints.cu:246-264).
+  virtual void contact_update(const float* x) = 0;
+  virtual ContactView contact_view() const = 0;
+  virtual int active_dof() const = 0;
+  virtual void end_step(silk_hip_step_stats* stats) = 0;
+};
+
+/// nullptr when no object has collision enabled.
+std::unique_ptr<CollisionWorld> make_collision_world(Context& ctx, Scene& S);
+
+}  // namespace silk::amd

@@ -1,0 +1,707 @@
+// Runtime context: registry, device scene assembly and the PD step.
+#include "api/context.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "api/collision.hpp"
+
+namespace silk::amd {
+
+namespace {
+
+uint64_t splitmix64(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+template <typename T>
+void download(std::vector<T>& h, const T* d, size_t n, hipStream_t s) {
+  h.resize(n);
+  if (n > 0)
+    hip_check(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost,
+                             s),
+              "download");
+}
+
+}  // namespace
+
+Context::Context(int device) : device_(device) {
+  global_.acceleration[0] = 0.0f;
+  global_.acceleration[1] = 0.0f;
+  global_.acceleration[2] = -10.0f;
+  global_.max_outer_iteration = 100;
+  global_.max_inner_iteration = 100;
+  global_.dt = 1.0f / 60.0f;
+  global_.linear_solver_abs_tol = 1e-7f;
+  global_.linear_solver_rel_tol_min = 1e-6f;
+  global_.linear_solver_rel_tol_max = 1e-3f;
+  global_.initial_linear_rel_tol = 1e-2f;
+  global_.admm_abs_tol = 1e-5f;
+  global_.admm_rel_tol = 1e-3f;
+}
+
+Context::~Context() {
+  scene_.reset();
+  if (dev_ready_) {
+    for (auto& e : ev_)
+      if (e) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+}
+
+uint32_t Context::new_handle() {
+  for (;;) {
+    uint32_t h = static_cast<uint32_t>(splitmix64(rng_state_));
+    if (h == 0) continue;
+    if (find_cloth(h) || find_obstacle(h)) continue;
+    return h;
+  }
+}
+
+ClothEntity* Context::find_cloth(uint32_t h) {
+  for (auto& c : cloths_)
+    if (c.handle == h) return &c;
+  return nullptr;
+}
+
+ObstacleEntity* Context::find_obstacle(uint32_t h) {
+  for (auto& o : obstacles_)
+    if (o.handle == h) return &o;
+  return nullptr;
+}
+
+void Context::ensure_device() {
+  if (dev_ready_) return;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device_ || device_ < 0) {
+    (void)hipGetLastError();
+    throw std::runtime_error("no HIP device");
+  }
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking),
+            "hipStreamCreate");
+  for (auto& e : ev_) hip_check(hipEventCreate(&e), "hipEventCreate");
+  dev_ready_ = true;
+}
+
+void Context::invalidate_scene() {
+  if (scene_ && dev_ready_) {
+    // Persist cloth states (solver order on device -> caller order on host).
+    std::vector<float> hx, hv;
+    download(hx, scene_->x.get(), 3 * static_cast<size_t>(scene_->nv), stream_);
+    download(hv, scene_->v.get(), 3 * static_cast<size_t>(scene_->nv), stream_);
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    for (size_t ci = 0; ci < cloths_.size(); ++ci) {
+      ClothEntity& c = cloths_[ci];
+      if (ci + 1 >= scene_->vert_off.size()) break;
+      const int off = scene_->vert_off[ci];
+      const int nv = c.mesh.nv();
+      c.x.assign(3 * nv, 0.0f);
+      c.v.assign(3 * nv, 0.0f);
+      for (int v = 0; v < nv; ++v)
+        for (int d = 0; d < 3; ++d) {
+          c.x[3 * v + d] = hx[3 * (off + c.part.perm[v]) + d];
+          c.v[3 * v + d] = hv[3 * (off + c.part.perm[v]) + d];
+        }
+      c.has_state = true;
+    }
+  }
+  scene_.reset();
+  scene_dirty_ = true;
+}
+
+void Context::ensure_cloth_derived(ClothEntity& c) {
+  if (!c.part_valid) {
+    c.part = partition_vertices(c.mesh.V, MasTopology::kBank);
+    c.pmesh = permute_mesh(c.mesh, c.part);
+    c.part_valid = true;
+    c.geom_valid = false;
+  }
+  if (!c.geom_valid) {
+    c.geom = build_cloth_geometry(c.pmesh);
+    c.geom_valid = true;
+  }
+  if (!c.has_pin_pos) {
+    c.pin_curr.assign(3 * c.pins.size(), 0.0f);
+    for (size_t i = 0; i < c.pins.size(); ++i)
+      for (int d = 0; d < 3; ++d)
+        c.pin_curr[3 * i + d] = c.mesh.V[3 * c.pins[i] + d];
+    c.has_pin_pos = true;
+  }
+  if (!c.has_state) {
+    c.x = c.mesh.V;
+    c.v.assign(c.mesh.V.size(), 0.0f);
+    c.has_state = true;
+  }
+}
+
+void Context::build_scene() {
+  ensure_device();
+  auto sc = std::make_unique<Scene>();
+  Scene& S = *sc;
+  hipStream_t st = stream_;
+  const float h = global_.dt;
+  S.dt = h;
+  S.nsys = static_cast<int>(cloths_.size());
+  S.vert_off.assign(S.nsys + 1, 0);
+  S.face_off.assign(S.nsys + 1, 0);
+  for (int ci = 0; ci < S.nsys; ++ci) {
+    ensure_cloth_derived(cloths_[ci]);
+    S.vert_off[ci + 1] = S.vert_off[ci] + cloths_[ci].mesh.nv();
+    S.face_off[ci + 1] = S.face_off[ci] + cloths_[ci].mesh.nf();
+  }
+  S.nv = S.vert_off[S.nsys];
+  S.nf = S.face_off[S.nsys];
+  const int nv = S.nv, nf = S.nf;
+
+  std::vector<float> hx(3 * nv), hv(3 * nv), hfparam(4 * nf), hfstiff(2 * nf),
+      hmass(nv), hdamp(nv), hbend(3 * nv), hpin_t(3 * nv, 0.0f), hcw(nv);
+  std::vector<int> hF(3 * nf), hpinned(nv, 0);
+  std::vector<std::vector<int>> vf(nv);
+
+  // PCG / MAS host tables.
+  std::vector<int> blk_sys, blk_row0, blk_rows, sys_blk0{0};
+  std::vector<int> slice_off, slice_width, ell_col;
+  std::vector<float> ell_val;
+  std::vector<int> bank_sys, bank_level, bank_member;
+  std::vector<float> bank_static;
+  // coarse nodes: level-1 nodes of all systems first, then deeper ones.
+  struct SysMas {
+    MasTopology topo;
+    int c1_off = 0;             // global id of first level-1 node
+    std::vector<int> deep_off;  // per matrix level >= 2
+  };
+  std::vector<SysMas> smas(S.nsys);
+  int n1 = 0;
+  std::vector<CsrMatrix> Ks(S.nsys);
+
+  for (int ci = 0; ci < S.nsys; ++ci) {
+    ClothEntity& c = cloths_[ci];
+    const int vo = S.vert_off[ci], fo = S.face_off[ci];
+    const int cnv = c.mesh.nv(), cnf = c.mesh.nf();
+    const double k = c.cloth.elastic_stiffness, kb = c.cloth.bending_stiffness;
+    const double dens = c.cloth.density;
+    // state (caller order -> solver order)
+    for (int v = 0; v < cnv; ++v)
+      for (int d = 0; d < 3; ++d) {
+        hx[3 * (vo + c.part.perm[v]) + d] = c.x[3 * v + d];
+        hv[3 * (vo + c.part.perm[v]) + d] = c.v[3 * v + d];
+      }
+    for (int f = 0; f < cnf; ++f) {
+      for (int j = 0; j < 3; ++j) {
+        const int lv = c.pmesh.F[3 * f + j];
+        hF[3 * (fo + f) + j] = vo + lv;
+        vf[vo + lv].push_back(4 * (fo + f) + j);
+      }
+      for (int j = 0; j < 4; ++j)
+        hfparam[4 * (fo + f) + j] = c.geom.face_param[4 * f + j];
+      hfstiff[2 * (fo + f)] = c.cloth.elastic_stiffness;
+      hfstiff[2 * (fo + f) + 1] = c.cloth.elastic_stiffness;  // ADMM penalty
+    }
+    Ks[ci] = combine_operator(c.geom, k, kb);
+    const CsrMatrix& K = Ks[ci];
+    for (int v = 0; v < cnv; ++v) {
+      const double m = dens * c.geom.mass[v];
+      hmass[vo + v] = static_cast<float>(m);
+      hdamp[vo + v] = 1.0f - c.cloth.damping;
+      for (int d = 0; d < 3; ++d)
+        hbend[3 * (vo + v) + d] =
+            static_cast<float>(kb * c.geom.bending_rest[3 * v + d]);
+      // Daviet contact weight (cloth_assembly_l1_cache.cu:303-321): the 3x3
+      // diagonal block is (K_vv + m/h^2) I3, so its smallest eigenvalue is
+      // that scalar.
+      double kvv = 0.0;
+      for (int a = K.row_ptr[v]; a < K.row_ptr[v + 1]; ++a)
+        if (K.col[a] == v) kvv = K.val[a];
+      const double inertia = m / (static_cast<double>(h) * h);
+      const double eta = kvv + inertia;
+      hcw[vo + v] = static_cast<float>(
+          std::max(0.001 * eta, std::min(25.0 * inertia, eta)));
+    }
+    for (size_t i = 0; i < c.pins.size(); ++i) {
+      const int lv = c.part.perm[c.pins[i]];
+      if (!hpinned[vo + lv]) S.pinned_dof += 3;
+      hpinned[vo + lv] = 1;
+      for (int d = 0; d < 3; ++d)
+        hpin_t[3 * (vo + lv) + d] = c.pin_curr[3 * i + d];
+    }
+
+    // Row blocks and sliced ELL.
+    for (int r0 = 0; r0 < cnv; r0 += kBlock) {
+      const int rows = std::min(kBlock, cnv - r0);
+      blk_sys.push_back(ci);
+      blk_row0.push_back(vo + r0);
+      blk_rows.push_back(rows);
+      for (int sl = 0; sl < kBlock / kSlice; ++sl) {
+        const int s0 = r0 + sl * kSlice;
+        const int srows = std::max(0, std::min(kSlice, cnv - s0));
+        int width = 0;
+        for (int rr = 0; rr < srows; ++rr)
+          width = std::max(width, K.row_ptr[s0 + rr + 1] - K.row_ptr[s0 + rr]);
+        slice_off.push_back(static_cast<int>(ell_col.size()));
+        slice_width.push_back(width);
+        const size_t base = ell_col.size();
+        ell_col.resize(base + static_cast<size_t>(width) * kSlice);
+        ell_val.resize(base + static_cast<size_t>(width) * kSlice);
+        for (int lane = 0; lane < kSlice; ++lane) {
+          const int row = s0 + lane;
+          const bool real = lane < srows;
+          const int a0 = real ? K.row_ptr[row] : 0;
+          const int len = real ? K.row_ptr[row + 1] - a0 : 0;
+          for (int kk = 0; kk < width; ++kk) {
+            const size_t e = base + static_cast<size_t>(kk) * kSlice + lane;
+            if (kk < len) {
+              ell_col[e] = vo + K.col[a0 + kk];
+              ell_val[e] = static_cast<float>(K.val[a0 + kk]);
+            } else {
+              ell_col[e] = real ? vo + row : vo;
+              ell_val[e] = 0.0f;
+            }
+          }
+        }
+      }
+    }
+    sys_blk0.push_back(static_cast<int>(blk_sys.size()));
+
+    // MAS hierarchy.
+    SysMas& sm = smas[ci];
+    sm.topo = build_mas_topology(pattern_graph(K), c.part.offsets);
+    sm.c1_off = n1;
+    n1 += sm.topo.coarse_levels > 0 ? sm.topo.level_nodes[1] : 0;
+  }
+  // deep-node offsets
+  int ncn = n1;
+  for (int ci = 0; ci < S.nsys; ++ci) {
+    SysMas& sm = smas[ci];
+    for (int l = 2; l <= sm.topo.coarse_levels; ++l) {
+      sm.deep_off.push_back(ncn);
+      ncn += sm.topo.level_nodes[l];
+    }
+  }
+  S.n1node = n1;
+  S.ncnode = ncn;
+  std::vector<int> cnode_ptr(ncn + 1, 0), cnode_level(ncn, 0),
+      row_cnode(4 * static_cast<size_t>(nv), -1);
+  std::vector<std::vector<int>> cmembers(ncn);
+  for (int ci = 0; ci < S.nsys; ++ci) {
+    const SysMas& sm = smas[ci];
+    const MasTopology& t = sm.topo;
+    const int vo = S.vert_off[ci];
+    auto gid = [&](int mlevel, int local) {
+      return mlevel == 1 ? sm.c1_off + local : sm.deep_off[mlevel - 2] + local;
+    };
+    for (int row = 0; row < t.n; ++row) {
+      for (int l = 1; l <= t.coarse_levels; ++l) {
+        const int g = gid(l, t.cmap[l - 1][row]);
+        row_cnode[4 * static_cast<size_t>(vo + row) + (l - 1)] = g;
+        cnode_level[g] = l;
+        if (l == 1) cmembers[g].push_back(vo + row);
+      }
+    }
+    // deep nodes list their level-1 descendants (each exactly once)
+    for (int l = 2; l <= t.coarse_levels; ++l) {
+      for (int row = 0; row < t.n; ++row)
+        cmembers[gid(l, t.cmap[l - 1][row])].push_back(gid(1, t.cmap[0][row]));
+      for (int j = 0; j < t.level_nodes[l]; ++j) {
+        auto& m = cmembers[gid(l, j)];
+        std::sort(m.begin(), m.end());
+        m.erase(std::unique(m.begin(), m.end()), m.end());
+      }
+    }
+    // banks
+    std::vector<double> stat = mas_static_blocks(t, Ks[ci]);
+    size_t sb = 0;
+    for (int l = 0; l <= t.coarse_levels; ++l) {
+      for (int bk = 0; bk < t.level_banks[l]; ++bk, ++sb) {
+        bank_sys.push_back(ci);
+        bank_level.push_back(l);
+        for (int s = 0; s < kBankSlots; ++s) {
+          const int id = bk * kBankSlots + s;
+          int m = -1;
+          if (l == 0) {
+            // padded slot -> row
+            const auto& po = cloths_[ci].part.offsets;
+            const int r0 = po[bk], r1 = po[bk + 1];
+            m = (r0 + s < r1) ? vo + r0 + s : -1;
+          } else if (id < t.level_nodes[l]) {
+            m = gid(l, id);
+          }
+          bank_member.push_back(m);
+        }
+        for (int e = 0; e < 1024; ++e)
+          bank_static.push_back(static_cast<float>(stat[sb * 1024 + e]));
+      }
+    }
+  }
+  for (int g = 0; g < ncn; ++g)
+    cnode_ptr[g + 1] = cnode_ptr[g] + static_cast<int>(cmembers[g].size());
+  std::vector<int> cnode_list;
+  cnode_list.reserve(cnode_ptr[ncn]);
+  for (auto& m : cmembers) cnode_list.insert(cnode_list.end(), m.begin(), m.end());
+
+  std::vector<int> vf_ptr(nv + 1, 0), vf_list;
+  for (int v = 0; v < nv; ++v) {
+    vf_ptr[v + 1] = vf_ptr[v] + static_cast<int>(vf[v].size());
+    vf_list.insert(vf_list.end(), vf[v].begin(), vf[v].end());
+  }
+
+  // ---- upload
+  S.x.upload(hx, st);
+  S.v.upload(hv, st);
+  for (DevBuf<float>* b : {&S.prev_x, &S.prev_v, &S.outer_x, &S.outer_v,
+                           &S.inner_x, &S.minert, &S.elastic_rhs,
+                           &S.pin_lambda, &S.b})
+    b->resize(3 * static_cast<size_t>(nv));
+  S.d.resize(nv);
+  for (int i = 0; i < 2; ++i) {
+    S.z[i].resize(6 * static_cast<size_t>(nf));
+    S.u[i].resize(6 * static_cast<size_t>(nf));
+  }
+  S.F.upload(hF, st);
+  S.vf_ptr.upload(vf_ptr, st);
+  S.vf_list.upload(vf_list, st);
+  S.pinned.upload(hpinned, st);
+  S.fparam.upload(hfparam, st);
+  S.fstiff.upload(hfstiff, st);
+  S.mass.upload(hmass, st);
+  S.damp.upload(hdamp, st);
+  S.bend_rhs.upload(hbend, st);
+  S.pin_target.upload(hpin_t, st);
+  S.contact_base_w.upload(hcw, st);
+  S.h_mass = hmass;
+  S.h_pinned = hpinned;
+  S.admm_part.resize(kAdmmPartials * static_cast<size_t>(local_step_grid(nv)));
+
+  S.nblk = static_cast<int>(blk_sys.size());
+  S.nbank = static_cast<int>(bank_sys.size());
+  S.blk_sys.upload(blk_sys, st);
+  S.blk_row0.upload(blk_row0, st);
+  S.blk_rows.upload(blk_rows, st);
+  S.sys_blk0.upload(sys_blk0, st);
+  S.h_blk0 = sys_blk0;
+  S.slice_off.upload(slice_off, st);
+  S.slice_width.upload(slice_width, st);
+  if (ell_col.empty()) {
+    ell_col.push_back(0);
+    ell_val.push_back(0.0f);
+  }
+  S.ell_col.upload(ell_col, st);
+  S.ell_val.upload(ell_val, st);
+  S.bank_sys.upload(bank_sys, st);
+  S.bank_level.upload(bank_level, st);
+  S.bank_member.upload(bank_member, st);
+  S.bank_static.upload(bank_static, st);
+  S.bank_inv.resize(std::max<size_t>(1, bank_static.size()));
+  if (cnode_list.empty()) cnode_list.push_back(0);
+  S.cnode_ptr.upload(cnode_ptr, st);
+  S.cnode_list.upload(cnode_list, st);
+  S.cnode_level.upload(cnode_level.empty() ? std::vector<int>{0} : cnode_level,
+                       st);
+  S.row_cnode.upload(row_cnode, st);
+  S.r.resize(6 * static_cast<size_t>(nv));
+  S.p.resize(3 * static_cast<size_t>(nv));
+  S.q.resize(3 * static_cast<size_t>(nv));
+  S.z_pcg.resize(3 * static_cast<size_t>(nv));
+  S.zl0.resize(3 * static_cast<size_t>(nv));
+  S.cr.resize(3 * static_cast<size_t>(std::max(1, ncn)));
+  S.cz.resize(3 * static_cast<size_t>(std::max(1, ncn)));
+  S.cd.resize(std::max(1, n1));
+  S.part.resize(2 * static_cast<size_t>(std::max(1, S.nblk)));
+  S.state.resize(2 * static_cast<size_t>(std::max(1, S.nsys)));
+  S.coll = make_collision_world(*this, S);
+  hip_check(hipStreamSynchronize(st), "scene upload");
+  scene_ = std::move(sc);
+  scene_dirty_ = false;
+}
+
+ClothDev Context::cloth_dev() {
+  Scene& S = *scene_;
+  ClothDev c;
+  c.nv = S.nv;
+  c.nf = S.nf;
+  c.h = S.dt;
+  for (int d = 0; d < 3; ++d) c.acc[d] = global_.acceleration[d];
+  c.F = S.F.get();
+  c.fparam = S.fparam.get();
+  c.fstiff = S.fstiff.get();
+  c.vf_ptr = S.vf_ptr.get();
+  c.vf_list = S.vf_list.get();
+  c.mass = S.mass.get();
+  c.damp = S.damp.get();
+  c.bend_rhs = S.bend_rhs.get();
+  c.pinned = S.pinned.get();
+  c.pin_target = S.pin_target.get();
+  return c;
+}
+
+PcgDev Context::pcg_dev() {
+  Scene& S = *scene_;
+  PcgDev p;
+  p.nsys = S.nsys;
+  p.nv = S.nv;
+  p.nblk = S.nblk;
+  p.blk_sys = S.blk_sys.get();
+  p.blk_row0 = S.blk_row0.get();
+  p.blk_rows = S.blk_rows.get();
+  p.sys_blk0 = S.sys_blk0.get();
+  p.slice_off = S.slice_off.get();
+  p.slice_width = S.slice_width.get();
+  p.ell_col = S.ell_col.get();
+  p.ell_val = S.ell_val.get();
+  p.nbank = S.nbank;
+  p.bank_sys = S.bank_sys.get();
+  p.bank_level = S.bank_level.get();
+  p.bank_member = S.bank_member.get();
+  p.bank_static = S.bank_static.get();
+  p.bank_inv = S.bank_inv.get();
+  p.ncnode = S.ncnode;
+  p.cnode_ptr = S.cnode_ptr.get();
+  p.cnode_list = S.cnode_list.get();
+  p.cnode_level = S.cnode_level.get();
+  p.row_cnode = S.row_cnode.get();
+  p.n1node = S.n1node;
+  p.x = S.inner_x.get();
+  p.r = S.r.get();
+  p.p = S.p.get();
+  p.q = S.q.get();
+  p.z = S.z_pcg.get();
+  p.zl0 = S.zl0.get();
+  p.cr = S.cr.get();
+  p.cz = S.cz.get();
+  p.d = S.d.get();
+  p.b = S.b.get();
+  p.part = S.part.get();
+  p.state = S.state.get();
+  p.abs_tol = global_.linear_solver_abs_tol;
+  p.max_iter = pcg_max_iter();
+  return p;
+}
+
+int Context::pcg_solve(float rel_tol, silk_hip_step_stats* stats) {
+  Scene& S = *scene_;
+  PcgDev p = pcg_dev();
+  launch_pcg_init(stream_, p, rel_tol);
+  int k = 0;
+  int chunk = 8;
+  const int max_iter = pcg_max_iter();
+  for (;;) {
+    launch_pcg_iters(stream_, p, k, chunk, rel_tol);
+    k += chunk;
+    S.h_state.resize(S.nsys);
+    hip_check(hipMemcpyAsync(S.h_state.data(), S.state.get() + (k & 1) * S.nsys,
+                             S.nsys * sizeof(PcgState), hipMemcpyDeviceToHost,
+                             stream_),
+              "pcg state");
+    hip_check(hipStreamSynchronize(stream_), "pcg sync");
+    bool all_done = true;
+    for (const auto& s : S.h_state) all_done = all_done && s.done;
+    if (all_done || k > max_iter + 2) break;
+    chunk = 4;
+  }
+  int iters = 0;
+  double rel = 0.0;
+  for (const auto& s : S.h_state) {
+    iters += s.iter;
+    if (s.rr0 > 0) rel = std::max(rel, std::sqrt(s.rr / s.rr0));
+  }
+  if (stats) {
+    stats->pcg_iterations += iters;
+    stats->pcg_solves += 1;
+    stats->last_pcg_rel_residual = static_cast<float>(rel);
+  }
+  return iters;
+}
+
+bool Context::admm_solve(silk_hip_step_stats* stats) {
+  Scene& S = *scene_;
+  ClothDev c = cloth_dev();
+  const size_t n3 = 3 * static_cast<size_t>(S.nv);
+  hip_check(hipMemsetAsync(S.pin_lambda.get(), 0, n3 * sizeof(float), stream_),
+            "pin lambda reset");
+  ContactView cv = S.coll ? S.coll->contact_view() : ContactView{};
+  const int grid = local_step_grid(S.nv);
+  double init_primal = 0.0, init_dual = 0.0, ratio = 1.0;
+  int primal_dof = 6 * S.nf + S.pinned_dof;
+  const auto& g = global_;
+  for (int it = 0; it < g.max_inner_iteration; ++it) {
+    if (it != 0) {
+      launch_assemble(stream_, c, S.minert.get(), S.elastic_rhs.get(),
+                      S.pin_lambda.get(), cv.w, cv.p, cv.eta, S.d.get(),
+                      S.b.get());
+      const float rel = std::clamp(
+          static_cast<float>(g.initial_linear_rel_tol * ratio),
+          g.linear_solver_rel_tol_min, g.linear_solver_rel_tol_max);
+      if (it == 1) launch_mas_factor(stream_, pcg_dev(), S.cd.get());
+      pcg_solve(rel, stats);
+    }
+    if (S.coll) S.coll->contact_update(S.inner_x.get());
+    LocalStepArgs a;
+    a.x = S.inner_x.get();
+    a.z_in = S.z[S.aux_cur].get();
+    a.u_in = S.u[S.aux_cur].get();
+    a.z_out = S.z[S.aux_cur ^ 1].get();
+    a.u_out = S.u[S.aux_cur ^ 1].get();
+    a.elastic_rhs = S.elastic_rhs.get();
+    a.pin_lambda = S.pin_lambda.get();
+    a.contact_dual = cv.dual;
+    a.contact_eta = cv.eta;
+    a.contact_w = cv.w;
+    a.contact_p = cv.p;
+    a.partials = S.admm_part.get();
+    launch_local_step(stream_, c, a);
+    S.aux_cur ^= 1;
+    if (stats) stats->admm_iterations += 1;
+    if (it == 0) continue;
+
+    download(S.h_admm_part, S.admm_part.get(),
+             static_cast<size_t>(kAdmmPartials) * grid, stream_);
+    hip_check(hipStreamSynchronize(stream_), "admm sync");
+    double sum[kAdmmPartials] = {};
+    for (int b = 0; b < grid; ++b)
+      for (int j = 0; j < kAdmmPartials; ++j)
+        sum[j] += S.h_admm_part[kAdmmPartials * b + j];
+    if (sum[6] > 0.0) return false;  // non-finite state: diverged
+    const int dof_p = primal_dof + (S.coll ? S.coll->active_dof() : 0);
+    const double primal = std::sqrt(sum[0]);
+    const double dual = std::sqrt(sum[3]);
+    if (it == 1) {
+      init_primal = primal;
+      init_dual = dual;
+    }
+    const double eps_p = std::sqrt(static_cast<double>(dof_p)) * g.admm_abs_tol +
+                         std::max(std::sqrt(sum[1]), std::sqrt(sum[2])) *
+                             g.admm_rel_tol;
+    const double eps_d =
+        std::sqrt(static_cast<double>(3 * S.nv)) * g.admm_abs_tol +
+        std::max(std::sqrt(sum[4]), std::sqrt(sum[5])) * g.admm_rel_tol;
+    ratio = std::max(primal / std::max(init_primal, double(g.admm_abs_tol)),
+                     dual / std::max(init_dual, double(g.admm_abs_tol)));
+    if (primal <= eps_p && dual <= eps_d) break;
+  }
+  launch_enforce(stream_, c, cv.w, cv.p, S.inner_x.get());
+  return true;
+}
+
+int Context::step(silk_hip_step_stats* stats) {
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (cloths_.empty()) {
+    last_error_ = "nothing to solve";
+    return SILK_HIP_ERR_UNKNOWN;
+  }
+  if (scene_ && scene_->dt != global_.dt) invalidate_scene();
+  if (scene_dirty_ || !scene_) build_scene();
+  Scene& S = *scene_;
+  ClothDev c = cloth_dev();
+  const size_t n3 = 3 * static_cast<size_t>(S.nv);
+  hip_check(hipEventRecord(ev_[0], stream_), "event");
+
+  launch_step_init(stream_, c, S.x.get(), S.v.get(), S.prev_x.get(),
+                   S.prev_v.get(), S.outer_x.get(), S.outer_v.get(),
+                   S.minert.get());
+  // Aux variables restart every step (cloth_assembler.cu:137-142).
+  S.z[S.aux_cur].zero(stream_);
+  S.u[S.aux_cur].zero(stream_);
+
+  float ms = 0.0f, dhat = 0.0f;
+  if (S.coll) S.coll->begin_step(S.outer_x.get(), &ms, &dhat, stats);
+
+  float remaining = 1.0f;
+  float accepted_toi = 1.0f;
+  int outer = 0;
+  for (; outer < global_.max_outer_iteration; ++outer) {
+    launch_predict(stream_, c, S.outer_x.get(), S.outer_v.get(),
+                   S.inner_x.get());
+    if (!admm_solve(stats)) {
+      last_error_ = "solver diverged";
+      return SILK_HIP_ERR_UNKNOWN;
+    }
+    float min_toi = remaining;
+    if (S.coll) {
+      min_toi = S.coll->min_toi(S.outer_x.get(), S.inner_x.get(),
+                                1.0f - remaining, remaining, ms, stats);
+    }
+    launch_velocity(stream_, static_cast<int>(n3), S.dt, S.outer_x.get(),
+                    S.inner_x.get(), S.outer_v.get());
+    if (min_toi >= remaining) {
+      hip_check(hipMemcpyAsync(S.outer_x.get(), S.inner_x.get(),
+                               n3 * sizeof(float), hipMemcpyDeviceToDevice,
+                               stream_),
+                "accept");
+      break;
+    }
+    const float rollback = 0.8f * min_toi;
+    accepted_toi = std::min(accepted_toi, 1.0f - remaining + min_toi);
+    const float frac = rollback / remaining;
+    launch_mix(stream_, static_cast<int>(n3), frac, S.inner_x.get(),
+               S.outer_x.get(), S.outer_x.get());
+    remaining -= rollback;
+    if (S.coll) S.coll->rollback_contacts(S.outer_x.get(), 1.0f - remaining,
+                                          dhat, stats);
+  }
+  // Velocity from the whole accepted displacement (main_loop.cu:281-296).
+  launch_velocity(stream_, static_cast<int>(n3), S.dt, S.prev_x.get(),
+                  S.outer_x.get(), S.v.get());
+  hip_check(hipMemcpyAsync(S.x.get(), S.outer_x.get(), n3 * sizeof(float),
+                           hipMemcpyDeviceToDevice, stream_),
+            "commit");
+  if (S.coll) S.coll->end_step(stats);
+  hip_check(hipEventRecord(ev_[1], stream_), "event");
+  hip_check(hipStreamSynchronize(stream_), "step sync");
+  float ms_step = 0.0f;
+  hip_check(hipEventElapsedTime(&ms_step, ev_[0], ev_[1]), "elapsed");
+  timing_ms_[0] += ms_step;
+  timing_launch_[0] += 1;
+  // Obstacle / pin static flags (main_loop.cu:298-306).
+  for (auto& o : obstacles_) {
+    if (o.has_pos) o.prev = o.curr;
+    if (o.is_static) o.is_static_twice = true;
+    else o.is_static = true;
+  }
+  if (stats) {
+    stats->outer_iterations = std::min(outer + 1, global_.max_outer_iteration);
+    stats->min_toi = accepted_toi;
+  }
+  return SILK_HIP_OK;
+}
+
+void Context::update_pin_targets(const ClothEntity& c) {
+  if (!scene_) return;
+  const size_t ci = static_cast<size_t>(&c - cloths_.data());
+  const int off = scene_->vert_off[ci];
+  std::vector<float> t(3 * static_cast<size_t>(c.mesh.nv()), 0.0f);
+  for (size_t i = 0; i < c.pins.size(); ++i) {
+    const int lv = c.part.perm[c.pins[i]];
+    for (int d = 0; d < 3; ++d) t[3 * lv + d] = c.pin_curr[3 * i + d];
+  }
+  hip_check(hipMemcpyAsync(scene_->pin_target.get() + 3 * static_cast<size_t>(off),
+                           t.data(), t.size() * sizeof(float),
+                           hipMemcpyHostToDevice, stream_),
+            "pin targets");
+  hip_check(hipStreamSynchronize(stream_), "pin targets sync");
+}
+
+void Context::get_positions(const ClothEntity& c, float* out) const {
+  const int nv = c.mesh.nv();
+  if (scene_) {
+    size_t ci = static_cast<size_t>(&c - cloths_.data());
+    const int off = scene_->vert_off[ci];
+    std::vector<float> hx(3 * static_cast<size_t>(nv));
+    hip_check(hipMemcpyAsync(hx.data(), scene_->x.get() + 3 * off,
+                             hx.size() * sizeof(float), hipMemcpyDeviceToHost,
+                             stream_),
+              "get_positions");
+    hip_check(hipStreamSynchronize(stream_), "get_positions sync");
+    for (int v = 0; v < nv; ++v)
+      for (int d = 0; d < 3; ++d) out[3 * v + d] = hx[3 * c.part.perm[v] + d];
+    return;
+  }
+  const std::vector<float>& src = c.has_state ? c.x : c.mesh.V;
+  std::memcpy(out, src.data(), src.size() * sizeof(float));
+}
+
+}  // namespace silk::amd

@@ -1,0 +1,137 @@
+// Runtime context behind the C ABI: entity registry, lazily assembled device
+// scene and the per-step orchestration (reference MainLoop::step,
+// silk/src/backend/cuda/main_loop.cu:174-312, and ADMMSolver::solve,
+// solver/admm_solver.cu:228-397).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "api/collision.hpp"
+#include "api/devbuf.hpp"
+#include "host/geometry.hpp"
+#include "hip/kernels.hpp"
+#include "silk_hip.h"
+
+namespace silk::amd {
+
+struct ClothEntity {
+  uint32_t handle = 0;
+  silk_hip_cloth_params cloth{};
+  silk_hip_collision_params coll{};
+  TriMesh mesh;           // caller order
+  std::vector<int> pins;  // caller vertex indices
+  bool has_pin_pos = false;
+  std::vector<float> pin_curr;  // 3 * pins.size()
+  // derived
+  bool part_valid = false;
+  Partition part;
+  TriMesh pmesh;  // solver order
+  bool geom_valid = false;
+  ClothGeometry geom;
+  // persistent state, caller order (valid when has_state)
+  bool has_state = false;
+  std::vector<float> x, v;
+};
+
+struct ObstacleEntity {
+  uint32_t handle = 0;
+  silk_hip_collision_params coll{};
+  TriMesh mesh;
+  bool has_pos = false;
+  std::vector<float> curr, prev;
+  bool is_static = true;
+  bool is_static_twice = true;
+};
+
+/// Device scene: all cloths concatenated in solver order (one PCG system per
+/// cloth) plus collision data.
+struct Scene {
+  int nv = 0, nf = 0, nsys = 0;
+  float dt = 0.0f;
+  std::vector<int> vert_off, face_off;  // per cloth (+1)
+  int pinned_dof = 0;
+  // ---- state and work vectors (3 * nv)
+  DevBuf<float> x, v, prev_x, prev_v, outer_x, outer_v, inner_x, minert,
+      elastic_rhs, pin_lambda, b;
+  DevBuf<float> d;  // nv
+  DevBuf<float> z[2], u[2];
+  int aux_cur = 0;
+  // ---- static cloth data
+  DevBuf<int> F, vf_ptr, vf_list, pinned;
+  DevBuf<float> fparam, fstiff, mass, damp, bend_rhs, pin_target,
+      contact_base_w;
+  std::vector<float> h_mass;  // nv, density applied
+  std::vector<int> h_pinned;
+  // ---- local-step partials
+  DevBuf<double> admm_part;
+  std::vector<double> h_admm_part;
+  // ---- PCG / MAS
+  DevBuf<int> blk_sys, blk_row0, blk_rows, sys_blk0, slice_off, slice_width,
+      ell_col, bank_sys, bank_level, bank_member, cnode_ptr, cnode_list,
+      cnode_level, row_cnode;
+  DevBuf<float> ell_val, bank_static, bank_inv, r, p, q, z_pcg, zl0, cr, cz,
+      cd;
+  DevBuf<double> part;
+  DevBuf<PcgState> state;
+  int nblk = 0, nbank = 0, ncnode = 0, n1node = 0;
+  std::vector<int> h_blk0;  // sys_blk0 on host
+  std::vector<PcgState> h_state;
+  // ---- collision
+  std::unique_ptr<CollisionWorld> coll;
+};
+
+class Context {
+ public:
+  explicit Context(int device);
+  ~Context();
+
+  int device_ = 0;
+  std::string last_error_;
+  silk_hip_global global_{};
+
+  std::vector<ClothEntity> cloths_;
+  std::vector<ObstacleEntity> obstacles_;
+  uint64_t rng_state_ = 0x5117C0DE2024ULL;
+
+  bool dev_ready_ = false;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev_[2] = {nullptr, nullptr};
+  bool scene_dirty_ = true;
+  std::unique_ptr<Scene> scene_;
+  double timing_ms_[8] = {};
+  int64_t timing_launch_[8] = {};
+
+  uint32_t new_handle();
+  ClothEntity* find_cloth(uint32_t h);
+  ObstacleEntity* find_obstacle(uint32_t h);
+
+  /// Download cloth states from the device scene, then drop it.
+  void invalidate_scene();
+  void ensure_device();
+  void ensure_cloth_derived(ClothEntity& c);
+  void build_scene();
+  int step(silk_hip_step_stats* stats);
+  int pcg_solve(float rel_tol, silk_hip_step_stats* st);
+  bool admm_solve(silk_hip_step_stats* st);
+  void get_positions(const ClothEntity& c, float* out) const;
+  /// Push one cloth's pin targets to the device scene (if built).
+  void update_pin_targets(const ClothEntity& c);
+
+  // global config with defaults resolved
+  int pcg_max_iter() const { return global_.pcg_max_iter > 0 ? global_.pcg_max_iter : 1000; }
+  int contact_sweeps() const { return global_.contact_sweeps > 0 ? global_.contact_sweeps : 10; }
+  int ccd_max_iter() const { return global_.ccd_max_iter > 0 ? global_.ccd_max_iter : 4096; }
+  float ccd_tolerance() const { return global_.ccd_tolerance > 0 ? global_.ccd_tolerance : 1e-6f; }
+  float ccd_ms_scale() const { return global_.ccd_min_separation_scale > 0 ? global_.ccd_min_separation_scale : 0.2f; }
+  float dcd_scale() const { return global_.dcd_activation_scale > 0 ? global_.dcd_activation_scale : 1.0f; }
+
+  ClothDev cloth_dev();
+  PcgDev pcg_dev();
+};
+
+}  // namespace silk::amd

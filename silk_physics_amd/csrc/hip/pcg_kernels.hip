@@ -1,0 +1,397 @@
+// MAS-preconditioned conjugate gradient for the PD global step.
+//
+// The reference operator is A = k JWJ + kb CWC (x) I3 + diag(d) stored as a
+// 3x3 BSR (cloth_assembly_l1_cache.cu:270-278, mas_cg_solver.cu:127-150).
+// Every block of that matrix is a scalar times I3 and d is the same for the
+// three coordinates of a vertex (mass, pin penalty, contact weight), so here
+// the operator is stored once as a vertex-scalar sliced-ELL matrix and applied
+// to the three coordinates together.  The MAS banks of the reference are
+// 96x96 (32 vertices x 3); with the same structure they are 32x32 here and
+// their inverse applies to the 32x3 bank residual.
+//
+// One PCG system per cloth, as in the reference (one MASCGSolver per cloth,
+// cloth_admm_helper.cu:367-408).  All systems advance in the same launches;
+// row blocks and MAS banks never straddle systems, and every scalar (rz, rr,
+// pAp, alpha, beta, stopping) is per system.  Dot products are reduced in
+// FP64 from per-block partials in a fixed order.
+#include "hip/device_common.hpp"
+#include "hip/kernels.hpp"
+
+namespace silk::amd {
+
+using namespace dev;
+
+namespace {
+
+__device__ __forceinline__ double sys_sum(const PcgDev& p, int sys, int comp,
+                                          double* lds) {
+  const int b0 = p.sys_blk0[sys], b1 = p.sys_blk0[sys + 1];
+  double v = 0.0;
+  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x)
+    v += p.part[2 * b + comp];
+  double s = block_sum(v, lds);
+  if (threadIdx.x == 0) lds[16] = s;
+  __syncthreads();
+  s = lds[16];
+  __syncthreads();
+  return s;
+}
+
+// q = (K + D) src for the rows of block b; returns src_i . q_i per thread.
+__device__ __forceinline__ double spmv_row(const PcgDev& p, int b,
+                                           const float* __restrict__ src,
+                                           float* __restrict__ out, int row,
+                                           bool valid) {
+  if (!valid) return 0.0;
+  const int g = b * (kBlock / kSlice) + threadIdx.x / kSlice;
+  const int lane = threadIdx.x % kSlice;
+  const int off = p.slice_off[g];
+  const int width = p.slice_width[g];
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  for (int k = 0; k < width; ++k) {
+    const int c = p.ell_col[off + k * kSlice + lane];
+    const float a = p.ell_val[off + k * kSlice + lane];
+    ax += a * src[3 * c];
+    ay += a * src[3 * c + 1];
+    az += a * src[3 * c + 2];
+  }
+  const float dd = p.d[row];
+  const f3 s = ld3(src, row);
+  const f3 q = {ax + dd * s.x, ay + dd * s.y, az + dd * s.z};
+  st3(out, row, q);
+  return static_cast<double>(s.x) * q.x + static_cast<double>(s.y) * q.y +
+         static_cast<double>(s.z) * q.z;
+}
+
+// ---- init: r = b - (K + D) x
+__global__ void __launch_bounds__(kBlock) k_init_residual(PcgDev p,
+                                                          float* r0) {
+  const int b = blockIdx.x;
+  const int row = p.blk_row0[b] + threadIdx.x;
+  const bool valid = threadIdx.x < p.blk_rows[b];
+  spmv_row(p, b, p.x, r0, row, valid);
+  if (valid) {
+    const f3 bb = ld3(p.b, row);
+    const f3 ax = ld3(r0, row);
+    st3(r0, row, bb - ax);
+  }
+}
+
+// ---- level-1 coarse sums of a 3-vector: cr[n] = sum_{rows in n} src
+__global__ void k_coarse_sum(PcgDev p, const float* __restrict__ src,
+                             int k_parity) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= p.n1node) return;
+  (void)k_parity;
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  for (int e = p.cnode_ptr[n]; e < p.cnode_ptr[n + 1]; ++e) {
+    const int row = p.cnode_list[e];
+    sx += src[3 * row];
+    sy += src[3 * row + 1];
+    sz += src[3 * row + 2];
+  }
+  p.cr[3 * n] = sx;
+  p.cr[3 * n + 1] = sy;
+  p.cr[3 * n + 2] = sz;
+}
+
+// ---- level-1 coarse sums of the scalar diagonal
+__global__ void k_coarse_dsum(PcgDev p, float* __restrict__ cd) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= p.n1node) return;
+  float s = 0.0f;
+  for (int e = p.cnode_ptr[n]; e < p.cnode_ptr[n + 1]; ++e)
+    s += p.d[p.cnode_list[e]];
+  cd[n] = s;
+}
+
+// Value of a bank slot: fine row, level-1 node or sum over level-1
+// descendants of a deeper node.  Two threads per slot (t and t+32).
+template <int NC>
+__device__ __forceinline__ void load_bank_slot(const PcgDev& p, int bank,
+                                               int level,
+                                               const float* __restrict__ fine,
+                                               const float* __restrict__ c1,
+                                               float (*out)[NC],
+                                               float (*tmp)[NC]) {
+  const int slot = threadIdx.x & 31;
+  const int half = threadIdx.x >> 5;
+  const int m = p.bank_member[bank * kBankSlots + slot];
+  float acc[NC];
+  for (int c = 0; c < NC; ++c) acc[c] = 0.0f;
+  if (m >= 0) {
+    if (level == 0) {
+      if (half == 0)
+        for (int c = 0; c < NC; ++c) acc[c] = fine[NC * m + c];
+    } else if (level == 1) {
+      if (half == 0)
+        for (int c = 0; c < NC; ++c) acc[c] = c1[NC * m + c];
+    } else {
+      const int e0 = p.cnode_ptr[m], e1 = p.cnode_ptr[m + 1];
+      for (int e = e0 + half; e < e1; e += 2) {
+        const int child = p.cnode_list[e];
+        for (int c = 0; c < NC; ++c) acc[c] += c1[NC * child + c];
+      }
+    }
+  }
+  if (threadIdx.x < 64) {
+    for (int c = 0; c < NC; ++c) tmp[threadIdx.x][c] = acc[c];
+  }
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    for (int c = 0; c < NC; ++c)
+      out[slot][c] = tmp[slot][c] + tmp[slot + 32][c];
+  }
+  __syncthreads();
+}
+
+// ---- MAS factorization: one 64-thread workgroup per bank.
+__global__ void __launch_bounds__(64) k_mas_factor(PcgDev p,
+                                                   const float* __restrict__ cd) {
+  const int bank = blockIdx.x;
+  const int level = p.bank_level[bank];
+  __shared__ float A[kBankSlots][kBankSlots + 1];
+  __shared__ float dg[kBankSlots][1];
+  __shared__ float tmp[64][1];
+  const float* S = p.bank_static + static_cast<size_t>(bank) * 1024;
+  for (int e = threadIdx.x; e < 1024; e += 64) A[e >> 5][e & 31] = S[e];
+  load_bank_slot<1>(p, bank, level, p.d, cd, dg, tmp);
+  if (threadIdx.x < 32) {
+    const int s = threadIdx.x;
+    float v = A[s][s] + dg[s][0];
+    if (v == 0.0f) v = 1.0f;  // padding slot (mas_preconditioner.cu:730-745)
+    A[s][s] = v;
+  }
+  __syncthreads();
+  // In-place Gauss-Jordan inverse (SPD, no pivoting needed).
+  float nv[16];
+  for (int piv = 0; piv < kBankSlots; ++piv) {
+    const float pv = A[piv][piv];
+    const float ipv = 1.0f / pv;
+    for (int m = 0; m < 16; ++m) {
+      const int e = threadIdx.x + 64 * m;
+      const int i = e >> 5, j = e & 31;
+      float val;
+      if (i == piv && j == piv) val = ipv;
+      else if (i == piv) val = A[piv][j] * ipv;
+      else if (j == piv) val = -A[i][piv] * ipv;
+      else val = A[i][j] - A[i][piv] * (A[piv][j] * ipv);
+      nv[m] = val;
+    }
+    __syncthreads();
+    for (int m = 0; m < 16; ++m) {
+      const int e = threadIdx.x + 64 * m;
+      A[e >> 5][e & 31] = nv[m];
+    }
+    __syncthreads();
+  }
+  float* out = p.bank_inv + static_cast<size_t>(bank) * 1024;
+  for (int e = threadIdx.x; e < 1024; e += 64) {
+    // Symmetrize: the exact inverse is symmetric.
+    const int i = e >> 5, j = e & 31;
+    out[e] = 0.5f * (A[i][j] + A[j][i]);
+  }
+}
+
+// ---- MAS bank solve: z_bank = inv(A_bank) r_bank (32 x 3).
+__global__ void __launch_bounds__(64) k_bank_solve(PcgDev p,
+                                                   const float* __restrict__ r,
+                                                   const PcgState* __restrict__ st) {
+  const int bank = blockIdx.x;
+  if (st != nullptr && st[p.bank_sys[bank]].done) return;
+  const int level = p.bank_level[bank];
+  __shared__ float rb[kBankSlots][3];
+  __shared__ float tmp[64][3];
+  load_bank_slot<3>(p, bank, level, r, p.cr, rb, tmp);
+  if (threadIdx.x >= 32) return;
+  const int row = threadIdx.x;
+  const float* inv = p.bank_inv + static_cast<size_t>(bank) * 1024;
+  float zx = 0.0f, zy = 0.0f, zz = 0.0f;
+  for (int k = 0; k < kBankSlots; ++k) {
+    const float a = inv[k * kBankSlots + row];
+    zx += a * rb[k][0];
+    zy += a * rb[k][1];
+    zz += a * rb[k][2];
+  }
+  const int m = p.bank_member[bank * kBankSlots + row];
+  if (m < 0) return;
+  float* dst = (level == 0) ? p.zl0 : p.cz;
+  dst[3 * m] = zx;
+  dst[3 * m + 1] = zy;
+  dst[3 * m + 2] = zz;
+}
+
+// ---- z = sum over levels; partials (r.z, r.r)
+__global__ void __launch_bounds__(kBlock) k_gather_z(PcgDev p,
+                                                     const float* __restrict__ r,
+                                                     const PcgState* __restrict__ st) {
+  __shared__ double lds[17];
+  const int b = blockIdx.x;
+  if (st != nullptr && st[p.blk_sys[b]].done) return;
+  const int row = p.blk_row0[b] + threadIdx.x;
+  const bool valid = threadIdx.x < p.blk_rows[b];
+  double rz = 0.0, rr = 0.0;
+  if (valid) {
+    f3 z = ld3(p.zl0, row);
+    for (int l = 0; l < 4; ++l) {
+      const int n = p.row_cnode[4 * row + l];
+      if (n >= 0) z = z + ld3(p.cz, n);
+    }
+    st3(p.z, row, z);
+    const f3 rv = ld3(r, row);
+    rz = static_cast<double>(rv.x) * z.x + static_cast<double>(rv.y) * z.y +
+         static_cast<double>(rv.z) * z.z;
+    rr = static_cast<double>(rv.x) * rv.x + static_cast<double>(rv.y) * rv.y +
+         static_cast<double>(rv.z) * rv.z;
+  }
+  const double s0 = block_sum(rz, lds);
+  const double s1 = block_sum(rr, lds);
+  if (threadIdx.x == 0) {
+    p.part[2 * b] = s0;
+    p.part[2 * b + 1] = s1;
+  }
+}
+
+// ---- init finalize: rz0, rr0, p = z
+__global__ void __launch_bounds__(kBlock) k_init_final(PcgDev p) {
+  __shared__ double lds[17];
+  const int b = blockIdx.x;
+  const int sys = p.blk_sys[b];
+  const double rz = sys_sum(p, sys, 0, lds);
+  const double rr = sys_sum(p, sys, 1, lds);
+  const int row = p.blk_row0[b] + threadIdx.x;
+  if (threadIdx.x < p.blk_rows[b]) st3(p.p, row, ld3(p.z, row));
+  if (threadIdx.x == 0 && b == p.sys_blk0[sys]) {
+    PcgState s;
+    s.rz = rz;
+    s.rr = rr;
+    s.rr0 = rr;
+    s.pAp = 0.0;
+    s.iter = 0;
+    const bool ok = isfinite(rz);
+    s.done = ok ? 0 : 1;
+    s.status = ok ? -1 : 3;
+    s.pad = 0;
+    p.state[sys] = s;  // parity 0
+  }
+}
+
+// ---- iteration phase 1: q = (K+D) p, partial p.q
+__global__ void __launch_bounds__(kBlock) k_it_spmv(PcgDev p, int k) {
+  __shared__ double lds[17];
+  const int b = blockIdx.x;
+  const PcgState* st = p.state + (k & 1) * p.nsys;
+  if (st[p.blk_sys[b]].done) return;
+  const int row = p.blk_row0[b] + threadIdx.x;
+  const bool valid = threadIdx.x < p.blk_rows[b];
+  const double v = spmv_row(p, b, p.p, p.q, row, valid);
+  const double s = block_sum(v, lds);
+  if (threadIdx.x == 0) p.part[2 * b] = s;
+}
+
+// ---- iteration phase 2: alpha, x += alpha p, r_out = r_in - alpha q
+__global__ void __launch_bounds__(kBlock) k_it_update(PcgDev p, int k,
+                                                      const float* __restrict__ r_in,
+                                                      float* __restrict__ r_out) {
+  __shared__ double lds[17];
+  const int b = blockIdx.x;
+  const int sys = p.blk_sys[b];
+  const PcgState* st = p.state + (k & 1) * p.nsys;
+  if (st[sys].done) return;
+  const double pAp = sys_sum(p, sys, 0, lds);
+  const double alpha_d = (fabs(pAp) < 1e-20) ? 0.0 : st[sys].rz / pAp;
+  const float alpha = static_cast<float>(alpha_d);
+  const int row = p.blk_row0[b] + threadIdx.x;
+  if (threadIdx.x < p.blk_rows[b]) {
+    const f3 pv = ld3(p.p, row);
+    const f3 qv = ld3(p.q, row);
+    st3(p.x, row, ld3(p.x, row) + alpha * pv);
+    st3(r_out, row, ld3(r_in, row) - alpha * qv);
+  }
+}
+
+// ---- iteration phase 4: stopping test, beta, p = z + beta p
+__global__ void __launch_bounds__(kBlock) k_it_final(PcgDev p, int k,
+                                                     float rel_tol) {
+  __shared__ double lds[17];
+  const int b = blockIdx.x;
+  const int sys = p.blk_sys[b];
+  const PcgState* st = p.state + (k & 1) * p.nsys;
+  PcgState* nxt = p.state + ((k + 1) & 1) * p.nsys;
+  const PcgState s = st[sys];
+  if (s.done) {
+    if (threadIdx.x == 0 && b == p.sys_blk0[sys]) nxt[sys] = s;
+    return;
+  }
+  const double rz = sys_sum(p, sys, 0, lds);
+  const double rr = sys_sum(p, sys, 1, lds);
+  const int iter = s.iter + 1;
+  const double abs2 = static_cast<double>(p.abs_tol) * p.abs_tol;
+  const double rel2 = static_cast<double>(rel_tol) * rel_tol;
+  int status = -1;
+  if (iter >= p.max_iter) status = 2;
+  else if (rr <= abs2) status = 0;
+  else if (rr <= rel2 * s.rr0) status = 1;
+  const double beta_d = (fabs(s.rz) < 1e-20) ? 0.0 : rz / s.rz;
+  const float beta = static_cast<float>(beta_d);
+  const int row = p.blk_row0[b] + threadIdx.x;
+  if (status < 0 && threadIdx.x < p.blk_rows[b])
+    st3(p.p, row, ld3(p.z, row) + beta * ld3(p.p, row));
+  if (threadIdx.x == 0 && b == p.sys_blk0[sys]) {
+    PcgState n = s;
+    n.rz = rz;
+    n.rr = rr;
+    n.iter = iter;
+    n.status = status;
+    n.done = status >= 0 ? 1 : 0;
+    nxt[sys] = n;
+  }
+}
+
+inline int grid_for(int n, int bs) { return (n + bs - 1) / bs; }
+
+void mas_apply(hipStream_t s, const PcgDev& p, const float* r,
+               const PcgState* st) {
+  if (p.n1node > 0)
+    k_coarse_sum<<<grid_for(p.n1node, kBlock), kBlock, 0, s>>>(p, r, 0);
+  k_bank_solve<<<p.nbank, 64, 0, s>>>(p, r, st);
+  k_gather_z<<<p.nblk, kBlock, 0, s>>>(p, r, st);
+}
+
+}  // namespace
+
+void launch_mas_factor(hipStream_t s, const PcgDev& p, float* cd_scratch) {
+  if (p.nbank == 0) return;
+  if (p.n1node > 0)
+    k_coarse_dsum<<<grid_for(p.n1node, kBlock), kBlock, 0, s>>>(p, cd_scratch);
+  k_mas_factor<<<p.nbank, 64, 0, s>>>(p, cd_scratch);
+}
+
+void launch_pcg_init(hipStream_t s, const PcgDev& p, float rel_tol) {
+  (void)rel_tol;
+  if (p.nblk == 0) return;
+  k_init_residual<<<p.nblk, kBlock, 0, s>>>(p, p.r);
+  mas_apply(s, p, p.r, nullptr);
+  k_init_final<<<p.nblk, kBlock, 0, s>>>(p);
+}
+
+void launch_pcg_iters(hipStream_t s, const PcgDev& p, int k, int count,
+                      float rel_tol) {
+  if (p.nblk == 0) return;
+  float* r_buf[2] = {p.r, p.r + 3 * static_cast<size_t>(p.nv)};
+  for (int it = k; it < k + count; ++it) {
+    const float* r_in = r_buf[it & 1];
+    float* r_out = r_buf[(it + 1) & 1];
+    const PcgState* st = p.state + (it & 1) * p.nsys;
+    k_it_spmv<<<p.nblk, kBlock, 0, s>>>(p, it);
+    k_it_update<<<p.nblk, kBlock, 0, s>>>(p, it, r_in, r_out);
+    mas_apply(s, p, r_out, st);
+    k_it_final<<<p.nblk, kBlock, 0, s>>>(p, it, rel_tol);
+  }
+}
+
+void launch_mas_apply(hipStream_t s, const PcgDev& p) {
+  mas_apply(s, p, p.r, nullptr);
+}
+
+}  // namespace silk::amd

@@ -1,0 +1,73 @@
+"""Synthetic scenes for tests and benchmarks (seeded, FP32, metres).
+
+grid_sheet:   n x n quads over [0, span]^2 in the XY plane, split along the
+              (a, b, c), (c, b, d) diagonal; optional uniform jitter.
+icosphere:    subdivided icosahedron (subdiv 4 -> 2562 vertices / 5120 faces).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def grid_sheet(n: int, span: float = 1.0, z: float = 0.0, jitter: float = 0.0,
+               seed: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    xs = np.linspace(0.0, span, n + 1, dtype=np.float64)
+    X, Y = np.meshgrid(xs, xs, indexing="xy")
+    V = np.stack([X.ravel(), Y.ravel(), np.full(X.size, z)], axis=1)
+    if jitter > 0.0:
+        rng = np.random.default_rng(seed)
+        V = V + rng.uniform(-jitter, jitter, size=V.shape)
+    a = (np.arange(n)[None, :] + (n + 1) * np.arange(n)[:, None]).ravel()
+    b = a + 1
+    c = a + n + 1
+    d = c + 1
+    F = np.stack([np.stack([a, b, c], 1), np.stack([c, b, d], 1)], 1).reshape(-1, 3)
+    return V.astype(np.float32), F.astype(np.int32)
+
+
+def top_corner_pins(n: int) -> np.ndarray:
+    """The two vertices at y = span, x in {0, span}."""
+    return np.array([n * (n + 1), n * (n + 1) + n], dtype=np.int32)
+
+
+def icosphere(subdiv: int = 4, radius: float = 0.3,
+              center=(0.5, 0.5, 0.0)) -> tuple[np.ndarray, np.ndarray]:
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+             (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9),
+             (5, 11, 4), (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2),
+             (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+             (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache: dict[tuple[int, int], int] = {}
+
+        def mid(i: int, j: int) -> int:
+            key = (min(i, j), max(i, j))
+            if key not in cache:
+                m = verts[i] + verts[j]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.array(verts) * radius + np.asarray(center, dtype=np.float64)
+    return V.astype(np.float32), np.array(faces, dtype=np.int32)
+
+
+def c1_sheet() -> dict:
+    """32x32 quads, 1 m, XY plane, two top corners pinned, g = -9.81 z."""
+    V, F = grid_sheet(32)
+    return {"V": V, "F": F, "pins": top_corner_pins(32)}
+
+
+def c2_scene(n: int = 80) -> dict:
+    """n x n sheet at z = 0.8 dropped onto an r = 0.3 icosphere at (0.5, 0.5, 0)."""
+    V, F = grid_sheet(n, z=0.8)
+    SV, SF = icosphere(4, 0.3, (0.5, 0.5, 0.0))
+    return {"V": V, "F": F, "pins": np.zeros(0, np.int32), "SV": SV, "SF": SF}
