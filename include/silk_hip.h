@@ -88,7 +88,9 @@ typedef struct {
   float ccd_min_separation_scale; /* 0.2 */
   float dcd_activation_scale;     /* 1.0 */
   int32_t mas_precision;          /* 0 = fp32 blocks, 1 = fp16 blocks */
-  int32_t reserved[7];
+  int32_t true_residual_stats;    /* 1 = evaluate the FP64 true residual of
+                                     every PCG solve (reported in stats) */
+  int32_t reserved[6];
 } silk_hip_global;
 
 /* Per-step counters returned by silk_hip_step. */
@@ -103,7 +105,9 @@ typedef struct {
   int32_t dcd_candidates;
   float min_toi;             /* earliest accepted TOI fraction (1 = none) */
   float last_pcg_rel_residual; /* sqrt(rr/rr0) of the last PCG solve */
-  int32_t reserved[6];
+  float max_true_rel_residual; /* max over solves of |b - A x| / |b| (FP64),
+                                  when true_residual_stats is set */
+  int32_t reserved[5];
 } silk_hip_step_stats;
 
 /* Number of visible HIP devices (0 when none).  Never fails. */

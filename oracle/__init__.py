@@ -1,0 +1,293 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the reported CPU
+baseline, never as part of the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "build" / "liboracle.so"
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE), "-j8"], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        d, i, f = C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_float)
+        vp = C.c_void_p
+        sig = {
+            "oracle_create": (vp, []),
+            "oracle_destroy": (None, [vp]),
+            "oracle_set_global": (None, [vp, d]),
+            "oracle_add_cloth": (C.c_int, [vp, d, d, f, C.c_int, i, C.c_int, i,
+                                           C.c_int, i, i, C.c_int]),
+            "oracle_finalize": (None, [vp]),
+            "oracle_add_obstacle": (C.c_int, [vp, d, f, C.c_int, i, C.c_int]),
+            "oracle_set_obstacle_position": (None, [vp, C.c_int, f]),
+            "oracle_step": (C.c_int, [vp, d]),
+            "oracle_get_positions": (None, [vp, C.c_int, d]),
+            "oracle_set_state": (None, [vp, C.c_int, d, d]),
+            "oracle_mass_area": (None, [d, C.c_int, i, C.c_int, d, d]),
+            "oracle_jacobians": (None, [d, C.c_int, i, C.c_int, d]),
+            "oracle_operator_dense": (None, [d, C.c_int, i, C.c_int, C.c_double,
+                                             C.c_double, d, d]),
+            "oracle_project_stretch": (None, [d, C.c_double, C.c_double, d]),
+            "oracle_local_step": (None, [d, C.c_int, i, C.c_int, C.c_double, d, d,
+                                         d, d, d, d, d]),
+            "oracle_mas_apply": (None, [d, C.c_int, i, C.c_int, C.c_double,
+                                        C.c_double, i, C.c_int, d, d, d, i]),
+            "oracle_pcg": (None, [d, C.c_int, i, C.c_int, C.c_double, C.c_double, i,
+                                  C.c_int, d, d, d, C.c_double, C.c_double,
+                                  C.c_int, d]),
+            "oracle_overlap_pairs": (C.c_int, [f, C.c_int, f, C.c_int, C.c_int, i,
+                                               C.c_int]),
+            "oracle_ccd": (C.c_int, [C.c_int, d, d, C.c_double, C.c_double,
+                                     C.c_int, d, C.c_double, C.c_int, d]),
+            "oracle_ccd_f32": (C.c_int, [C.c_int, f, f, C.c_float, C.c_float,
+                                         C.c_int, f, C.c_float, f]),
+            "oracle_ccd_filter": (None, [d, C.c_int, d]),
+            "oracle_coulomb": (None, [d, d, C.c_double, d]),
+            "oracle_ee_uv": (C.c_int, [d, C.c_double, d]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _f(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class OracleWorld:
+    """FP64 restatement of the GPU-path step.  add_cloth needs the solver
+    partition (perm, offsets) used by the library under test."""
+
+    def __init__(self, g: dict):
+        self._w = lib().oracle_create()
+        p = np.array([g.get("ax", 0.0), g.get("ay", 0.0), g.get("az", -10.0),
+                      g.get("max_outer", 100), g.get("max_inner", 100),
+                      g.get("dt", 1 / 60), g.get("lin_abs", 1e-7),
+                      g.get("lin_rel_min", 1e-6), g.get("lin_rel_max", 1e-3),
+                      g.get("lin_rel_init", 1e-2), g.get("admm_abs", 1e-5),
+                      g.get("admm_rel", 1e-3), g.get("pcg_max_iter", 1000),
+                      g.get("sweeps", 10), g.get("ccd_max_iter", 4096),
+                      g.get("ccd_tol", 1e-6), g.get("ms_scale", 0.2),
+                      g.get("dcd_scale", 1.0)], dtype=np.float64)
+        self._keep = [p]
+        lib().oracle_set_global(self._w, _d(p)[1])
+        self.sizes = []
+
+    def __del__(self):
+        if getattr(self, "_w", None):
+            lib().oracle_destroy(self._w)
+            self._w = None
+
+    def add_cloth(self, V, F, pins, perm, offsets, k=100.0, kb=1e-4, density=0.1,
+                  damping=0.01, coll=(1, 1, 0, 0.3, 0.3)):
+        cl, clp = _d([k, kb, density, damping])
+        co, cop = _d(coll)
+        v, vp = _f(V)
+        f, fp = _i(F)
+        pn, pnp = _i(pins if len(pins) else [0])
+        pm, pmp = _i(perm)
+        of, ofp = _i(offsets)
+        idx = lib().oracle_add_cloth(self._w, clp, cop, vp, v.size // 3, fp,
+                                     f.size // 3, pnp, len(pins), pmp, ofp,
+                                     of.size - 1)
+        self.sizes.append(v.size)
+        return idx
+
+    def add_obstacle(self, V, F, coll=(1, 1, 0, 0.3, 0.3)):
+        co, cop = _d(coll)
+        v, vp = _f(V)
+        f, fp = _i(F)
+        return lib().oracle_add_obstacle(self._w, cop, vp, v.size // 3, fp,
+                                         f.size // 3)
+
+    def finalize(self):
+        lib().oracle_finalize(self._w)
+
+    def step(self) -> tuple[int, dict]:
+        st = np.zeros(8)
+        rc = lib().oracle_step(self._w, _d(st)[1] if False else
+                               st.ctypes.data_as(C.POINTER(C.c_double)))
+        keys = ["outer", "admm", "pcg", "contacts", "colors", "ccd_candidates",
+                "min_toi"]
+        return rc, dict(zip(keys, st[:7].tolist()))
+
+    def positions(self, idx: int) -> np.ndarray:
+        out = np.zeros(self.sizes[idx], dtype=np.float64)
+        lib().oracle_get_positions(self._w, idx,
+                                   out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+
+# ------------------------------------------------------------ unit wrappers
+
+def mass_area(V, F):
+    v, vp = _d(V)
+    f, fp = _i(F)
+    m = np.zeros(v.size // 3)
+    a = np.zeros(f.size // 3)
+    lib().oracle_mass_area(vp, v.size // 3, fp, f.size // 3, _d(m)[1] if False else
+                           m.ctypes.data_as(C.POINTER(C.c_double)),
+                           a.ctypes.data_as(C.POINTER(C.c_double)))
+    return m, a
+
+
+def jacobians(V, F):
+    v, vp = _d(V)
+    f, fp = _i(F)
+    J = np.zeros((f.size // 3, 6, 9))
+    lib().oracle_jacobians(vp, v.size // 3, fp, f.size // 3,
+                           J.ctypes.data_as(C.POINTER(C.c_double)))
+    return J
+
+
+def operator_dense(V, F, k, kb):
+    v, vp = _d(V)
+    f, fp = _i(F)
+    n3 = v.size
+    A = np.zeros((n3, n3))
+    b = np.zeros(n3)
+    lib().oracle_operator_dense(vp, v.size // 3, fp, f.size // 3, k, kb,
+                                A.ctypes.data_as(C.POINTER(C.c_double)),
+                                b.ctypes.data_as(C.POINTER(C.c_double)))
+    return A, b
+
+
+def project_stretch(y, lo=0.9, hi=1.1):
+    y, yp = _d(y)
+    out = np.zeros(6)
+    lib().oracle_project_stretch(yp, lo, hi, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def local_step(V, F, k, x, u, z):
+    v, vp = _d(V)
+    f, fp = _i(F)
+    xx, xp = _d(x)
+    uu = np.array(u, dtype=np.float64)
+    zz = np.array(z, dtype=np.float64)
+    nv = v.size // 3
+    sums = np.zeros(3)
+    dual, dcur, dprev = np.zeros(3 * nv), np.zeros(3 * nv), np.zeros(3 * nv)
+    P = C.POINTER(C.c_double)
+    lib().oracle_local_step(vp, nv, fp, f.size // 3, k, xp, uu.ctypes.data_as(P),
+                            zz.ctypes.data_as(P), sums.ctypes.data_as(P),
+                            dual.ctypes.data_as(P), dcur.ctypes.data_as(P),
+                            dprev.ctypes.data_as(P))
+    return uu, zz, sums, dual, dcur, dprev
+
+
+def mas_apply(V, F, k, kb, offsets, diag, r):
+    v, vp = _d(V)
+    f, fp = _i(F)
+    o, op = _i(offsets)
+    dg, dgp = _d(diag)
+    rr, rp = _d(r)
+    z = np.zeros(v.size)
+    lv = C.c_int(0)
+    lib().oracle_mas_apply(vp, v.size // 3, fp, f.size // 3, k, kb, op, o.size - 1,
+                           dgp, rp, z.ctypes.data_as(C.POINTER(C.c_double)),
+                           C.byref(lv))
+    return z, lv.value
+
+
+def pcg(V, F, k, kb, offsets, diag, b, x0, rel_tol, abs_tol, max_iter=1000):
+    v, vp = _d(V)
+    f, fp = _i(F)
+    o, op = _i(offsets)
+    dg, dgp = _d(diag)
+    bb, bp = _d(b)
+    x = np.array(x0, dtype=np.float64)
+    out = np.zeros(4)
+    lib().oracle_pcg(vp, v.size // 3, fp, f.size // 3, k, kb, op, o.size - 1, dgp,
+                     bp, x.ctypes.data_as(C.POINTER(C.c_double)), rel_tol, abs_tol,
+                     max_iter, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return x, {"iters": int(out[0]), "status": int(out[1]), "rr0": out[2],
+               "rr": out[3]}
+
+
+def overlap_pairs(A, B=None):
+    a, ap = _f(np.asarray(A).reshape(-1, 6))
+    self = B is None
+    b, bp = _f(np.asarray(A if self else B).reshape(-1, 6))
+    n = lib().oracle_overlap_pairs(ap, a.size // 6, bp, b.size // 6, int(self),
+                                   None, 0)
+    out = np.zeros((max(n, 1), 2), dtype=np.int32)
+    lib().oracle_overlap_pairs(ap, a.size // 6, bp, b.size // 6, int(self),
+                               out.ctypes.data_as(C.POINTER(C.c_int)), n)
+    return out[:n]
+
+
+def ccd(vf, p0, p1, ms, tolerance, err, max_iter=4096, max_time=1.0,
+        stack_cap=1024):
+    a, ap = _d(p0)
+    b, bp = _d(p1)
+    e, ep = _d(err)
+    t = C.c_double(0)
+    hit = lib().oracle_ccd(int(vf), ap, bp, ms, tolerance, max_iter, ep, max_time,
+                           stack_cap, C.byref(t))
+    return bool(hit), t.value
+
+
+def ccd_f32(vf, p0, p1, ms, tolerance, err, max_iter=4096, max_time=1.0):
+    a, ap = _f(p0)
+    b, bp = _f(p1)
+    e, ep = _f(err)
+    t = C.c_float(0)
+    hit = lib().oracle_ccd_f32(int(vf), ap, bp, ms, tolerance, max_iter, ep,
+                               max_time, C.byref(t))
+    return bool(hit), t.value
+
+
+def ccd_filter(abs_max, vf):
+    a, ap = _d(abs_max)
+    e = np.zeros(3)
+    lib().oracle_ccd_filter(ap, int(vf), e.ctypes.data_as(C.POINTER(C.c_double)))
+    return e
+
+
+def coulomb(u_star, normal, mu):
+    a, ap = _d(u_star)
+    n, np_ = _d(normal)
+    out = np.zeros(3)
+    lib().oracle_coulomb(ap, np_, mu, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def ee_uv(x, eps=1e-6):
+    a, ap = _d(np.asarray(x).reshape(-1))
+    out = np.zeros(2)
+    ok = lib().oracle_ee_uv(ap, eps, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return bool(ok), out

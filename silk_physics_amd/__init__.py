@@ -100,7 +100,8 @@ class SilkHipGlobal(C.Structure):
                 ("ccd_max_iter", C.c_int32), ("ccd_tolerance", C.c_float),
                 ("ccd_min_separation_scale", C.c_float),
                 ("dcd_activation_scale", C.c_float),
-                ("mas_precision", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("mas_precision", C.c_int32), ("true_residual_stats", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
 
 
 class SilkHipStepStats(C.Structure):
@@ -109,7 +110,7 @@ class SilkHipStepStats(C.Structure):
                 ("contacts", C.c_int32), ("contact_colors", C.c_int32),
                 ("ccd_candidates", C.c_int32), ("dcd_candidates", C.c_int32),
                 ("min_toi", C.c_float), ("last_pcg_rel_residual", C.c_float),
-                ("reserved", C.c_int32 * 6)]
+                ("max_true_rel_residual", C.c_float), ("reserved", C.c_int32 * 5)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -202,6 +203,7 @@ class GlobalConfig:
     ccd_min_separation_scale: float = 0.0
     dcd_activation_scale: float = 0.0
     mas_precision: int = 0
+    true_residual_stats: int = 0
 
     def _c(self) -> SilkHipGlobal:
         g = SilkHipGlobal()
@@ -214,7 +216,7 @@ class GlobalConfig:
                      "admm_abs_tol", "admm_rel_tol", "pcg_max_iter",
                      "contact_sweeps", "ccd_max_iter", "ccd_tolerance",
                      "ccd_min_separation_scale", "dcd_activation_scale",
-                     "mas_precision"):
+                     "mas_precision", "true_residual_stats"):
             setattr(g, name, getattr(self, name))
         return g
 

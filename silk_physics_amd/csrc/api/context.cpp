@@ -160,7 +160,8 @@ void Context::build_scene() {
   const int nv = S.nv, nf = S.nf;
 
   std::vector<float> hx(3 * nv), hv(3 * nv), hfparam(4 * nf), hfstiff(2 * nf),
-      hmass(nv), hdamp(nv), hbend(3 * nv), hpin_t(3 * nv, 0.0f), hcw(nv);
+      hmass(nv), hdamp(nv), hpin_t(3 * nv, 0.0f), hcw(nv);
+  std::vector<double> hbend(3 * nv);
   std::vector<int> hF(3 * nf), hpinned(nv, 0);
   std::vector<std::vector<int>> vf(nv);
 
@@ -210,8 +211,7 @@ void Context::build_scene() {
       hmass[vo + v] = static_cast<float>(m);
       hdamp[vo + v] = 1.0f - c.cloth.damping;
       for (int d = 0; d < 3; ++d)
-        hbend[3 * (vo + v) + d] =
-            static_cast<float>(kb * c.geom.bending_rest[3 * v + d]);
+        hbend[3 * (vo + v) + d] = kb * c.geom.bending_rest[3 * v + d];
       // Daviet contact weight (cloth_assembly_l1_cache.cu:303-321): the 3x3
       // diagonal block is (K_vv + m/h^2) I3, so its smallest eigenvalue is
       // that scalar.
@@ -231,7 +231,14 @@ void Context::build_scene() {
         hpin_t[3 * (vo + lv) + d] = c.pin_curr[3 * i + d];
     }
 
-    // Row blocks and sliced ELL.
+    // Row blocks and sliced ELL of the off-diagonal entries (the operator is
+    // applied in difference form, which needs no diagonal).
+    auto offdiag = [&K](int row) {
+      int cnt = 0;
+      for (int a = K.row_ptr[row]; a < K.row_ptr[row + 1]; ++a)
+        cnt += K.col[a] != row;
+      return cnt;
+    };
     for (int r0 = 0; r0 < cnv; r0 += kBlock) {
       const int rows = std::min(kBlock, cnv - r0);
       blk_sys.push_back(ci);
@@ -241,8 +248,7 @@ void Context::build_scene() {
         const int s0 = r0 + sl * kSlice;
         const int srows = std::max(0, std::min(kSlice, cnv - s0));
         int width = 0;
-        for (int rr = 0; rr < srows; ++rr)
-          width = std::max(width, K.row_ptr[s0 + rr + 1] - K.row_ptr[s0 + rr]);
+        for (int rr = 0; rr < srows; ++rr) width = std::max(width, offdiag(s0 + rr));
         slice_off.push_back(static_cast<int>(ell_col.size()));
         slice_width.push_back(width);
         const size_t base = ell_col.size();
@@ -251,17 +257,20 @@ void Context::build_scene() {
         for (int lane = 0; lane < kSlice; ++lane) {
           const int row = s0 + lane;
           const bool real = lane < srows;
-          const int a0 = real ? K.row_ptr[row] : 0;
-          const int len = real ? K.row_ptr[row + 1] - a0 : 0;
-          for (int kk = 0; kk < width; ++kk) {
-            const size_t e = base + static_cast<size_t>(kk) * kSlice + lane;
-            if (kk < len) {
-              ell_col[e] = vo + K.col[a0 + kk];
-              ell_val[e] = static_cast<float>(K.val[a0 + kk]);
-            } else {
-              ell_col[e] = real ? vo + row : vo;
-              ell_val[e] = 0.0f;
+          int kk = 0;
+          if (real) {
+            for (int a = K.row_ptr[row]; a < K.row_ptr[row + 1]; ++a) {
+              if (K.col[a] == row) continue;
+              const size_t e = base + static_cast<size_t>(kk) * kSlice + lane;
+              ell_col[e] = vo + K.col[a];
+              ell_val[e] = static_cast<float>(K.val[a]);
+              ++kk;
             }
+          }
+          for (; kk < width; ++kk) {
+            const size_t e = base + static_cast<size_t>(kk) * kSlice + lane;
+            ell_col[e] = real ? vo + row : vo;
+            ell_val[e] = 0.0f;
           }
         }
       }
@@ -354,9 +363,9 @@ void Context::build_scene() {
   S.x.upload(hx, st);
   S.v.upload(hv, st);
   for (DevBuf<float>* b : {&S.prev_x, &S.prev_v, &S.outer_x, &S.outer_v,
-                           &S.inner_x, &S.minert, &S.elastic_rhs,
-                           &S.pin_lambda, &S.b})
+                           &S.inner_x, &S.pin_lambda, &S.dx})
     b->resize(3 * static_cast<size_t>(nv));
+  S.elastic_rhs.resize(3 * static_cast<size_t>(nv));
   S.d.resize(nv);
   for (int i = 0; i < 2; ++i) {
     S.z[i].resize(6 * static_cast<size_t>(nf));
@@ -412,6 +421,7 @@ void Context::build_scene() {
   S.cz.resize(3 * static_cast<size_t>(std::max(1, ncn)));
   S.cd.resize(std::max(1, n1));
   S.part.resize(2 * static_cast<size_t>(std::max(1, S.nblk)));
+  S.part2.resize(2 * static_cast<size_t>(std::max(1, S.nblk)));
   S.state.resize(2 * static_cast<size_t>(std::max(1, S.nsys)));
   S.coll = make_collision_world(*this, S);
   hip_check(hipStreamSynchronize(st), "scene upload");
@@ -466,6 +476,7 @@ PcgDev Context::pcg_dev() {
   p.row_cnode = S.row_cnode.get();
   p.n1node = S.n1node;
   p.x = S.inner_x.get();
+  p.dx = S.dx.get();
   p.r = S.r.get();
   p.p = S.p.get();
   p.q = S.q.get();
@@ -474,18 +485,21 @@ PcgDev Context::pcg_dev() {
   p.cr = S.cr.get();
   p.cz = S.cz.get();
   p.d = S.d.get();
-  p.b = S.b.get();
   p.part = S.part.get();
+  p.part2 = S.part2.get();
   p.state = S.state.get();
   p.abs_tol = global_.linear_solver_abs_tol;
   p.max_iter = pcg_max_iter();
   return p;
 }
 
-int Context::pcg_solve(float rel_tol, silk_hip_step_stats* stats) {
+int Context::pcg_solve(float rel_tol, bool factor, const RhsArgs& ra,
+                       silk_hip_step_stats* stats) {
   Scene& S = *scene_;
   PcgDev p = pcg_dev();
-  launch_pcg_init(stream_, p, rel_tol);
+  launch_pcg_residual(stream_, p, ra);
+  if (factor) launch_mas_factor(stream_, p, S.cd.get());
+  launch_pcg_start(stream_, p);
   int k = 0;
   int chunk = 8;
   const int max_iter = pcg_max_iter();
@@ -503,6 +517,7 @@ int Context::pcg_solve(float rel_tol, silk_hip_step_stats* stats) {
     if (all_done || k > max_iter + 2) break;
     chunk = 4;
   }
+  launch_pcg_commit(stream_, p);
   int iters = 0;
   double rel = 0.0;
   for (const auto& s : S.h_state) {
@@ -513,6 +528,21 @@ int Context::pcg_solve(float rel_tol, silk_hip_step_stats* stats) {
     stats->pcg_iterations += iters;
     stats->pcg_solves += 1;
     stats->last_pcg_rel_residual = static_cast<float>(rel);
+    if (global_.true_residual_stats) {
+      launch_true_residual(stream_, p, ra);
+      download(S.h_part2, S.part2.get(), 2 * static_cast<size_t>(S.nblk), stream_);
+      hip_check(hipStreamSynchronize(stream_), "true residual");
+      for (int sys = 0; sys < S.nsys; ++sys) {
+        double rr = 0.0, bb = 0.0;
+        for (int b = S.h_blk0[sys]; b < S.h_blk0[sys + 1]; ++b) {
+          rr += S.h_part2[2 * b];
+          bb += S.h_part2[2 * b + 1];
+        }
+        if (bb > 0.0)
+          stats->max_true_rel_residual = std::max(
+              stats->max_true_rel_residual, static_cast<float>(std::sqrt(rr / bb)));
+      }
+    }
   }
   return iters;
 }
@@ -528,16 +558,21 @@ bool Context::admm_solve(silk_hip_step_stats* stats) {
   double init_primal = 0.0, init_dual = 0.0, ratio = 1.0;
   int primal_dof = 6 * S.nf + S.pinned_dof;
   const auto& g = global_;
+  RhsArgs ra;
+  ra.c = c;
+  ra.prev_x = S.prev_x.get();
+  ra.prev_v = S.prev_v.get();
+  ra.elastic_rhs = S.elastic_rhs.get();
+  ra.pin_lambda = S.pin_lambda.get();
+  ra.cw = cv.w;
+  ra.cp = cv.p;
+  ra.ceta = cv.eta;
   for (int it = 0; it < g.max_inner_iteration; ++it) {
     if (it != 0) {
-      launch_assemble(stream_, c, S.minert.get(), S.elastic_rhs.get(),
-                      S.pin_lambda.get(), cv.w, cv.p, cv.eta, S.d.get(),
-                      S.b.get());
       const float rel = std::clamp(
           static_cast<float>(g.initial_linear_rel_tol * ratio),
           g.linear_solver_rel_tol_min, g.linear_solver_rel_tol_max);
-      if (it == 1) launch_mas_factor(stream_, pcg_dev(), S.cd.get());
-      pcg_solve(rel, stats);
+      pcg_solve(rel, it == 1, ra, stats);
     }
     if (S.coll) S.coll->contact_update(S.inner_x.get());
     LocalStepArgs a;
@@ -601,8 +636,7 @@ int Context::step(silk_hip_step_stats* stats) {
   hip_check(hipEventRecord(ev_[0], stream_), "event");
 
   launch_step_init(stream_, c, S.x.get(), S.v.get(), S.prev_x.get(),
-                   S.prev_v.get(), S.outer_x.get(), S.outer_v.get(),
-                   S.minert.get());
+                   S.prev_v.get(), S.outer_x.get(), S.outer_v.get());
   // Aux variables restart every step (cloth_assembler.cu:137-142).
   S.z[S.aux_cur].zero(stream_);
   S.u[S.aux_cur].zero(stream_);

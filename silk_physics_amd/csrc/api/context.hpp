@@ -56,15 +56,16 @@ struct Scene {
   std::vector<int> vert_off, face_off;  // per cloth (+1)
   int pinned_dof = 0;
   // ---- state and work vectors (3 * nv)
-  DevBuf<float> x, v, prev_x, prev_v, outer_x, outer_v, inner_x, minert,
-      elastic_rhs, pin_lambda, b;
+  DevBuf<float> x, v, prev_x, prev_v, outer_x, outer_v, inner_x, pin_lambda,
+      dx;
+  DevBuf<double> elastic_rhs;
   DevBuf<float> d;  // nv
   DevBuf<float> z[2], u[2];
   int aux_cur = 0;
   // ---- static cloth data
   DevBuf<int> F, vf_ptr, vf_list, pinned;
-  DevBuf<float> fparam, fstiff, mass, damp, bend_rhs, pin_target,
-      contact_base_w;
+  DevBuf<float> fparam, fstiff, mass, damp, pin_target, contact_base_w;
+  DevBuf<double> bend_rhs;
   std::vector<float> h_mass;  // nv, density applied
   std::vector<int> h_pinned;
   // ---- local-step partials
@@ -76,7 +77,8 @@ struct Scene {
       cnode_level, row_cnode;
   DevBuf<float> ell_val, bank_static, bank_inv, r, p, q, z_pcg, zl0, cr, cz,
       cd;
-  DevBuf<double> part;
+  DevBuf<double> part, part2;
+  std::vector<double> h_part2;
   DevBuf<PcgState> state;
   int nblk = 0, nbank = 0, ncnode = 0, n1node = 0;
   std::vector<int> h_blk0;  // sys_blk0 on host
@@ -116,7 +118,8 @@ class Context {
   void ensure_cloth_derived(ClothEntity& c);
   void build_scene();
   int step(silk_hip_step_stats* stats);
-  int pcg_solve(float rel_tol, silk_hip_step_stats* st);
+  int pcg_solve(float rel_tol, bool factor, const RhsArgs& ra,
+                silk_hip_step_stats* st);
   bool admm_solve(silk_hip_step_stats* st);
   void get_positions(const ClothEntity& c, float* out) const;
   /// Push one cloth's pin targets to the device scene (if built).

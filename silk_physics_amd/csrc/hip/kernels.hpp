@@ -26,16 +26,15 @@ struct ClothDev {
   const int* vf_list = nullptr;      // face*4 + corner
   const float* mass = nullptr;       // nv (density applied)
   const float* damp = nullptr;       // nv: 1 - damping
-  const float* bend_rhs = nullptr;   // 3*nv
+  const double* bend_rhs = nullptr;  // 3*nv, kb * CWC * V
   const int* pinned = nullptr;       // nv: 1 if pinned
   const float* pin_target = nullptr; // 3*nv
 };
 
-/// prev_x = x; prev_v = damp*v; outer_* = prev_*; minert = m(prev_x/h^2 +
-/// prev_v/h + a)  (main_loop.cu:58-110, admm_solver.cu:141-151).
+/// prev_x = x; prev_v = damp*v; outer_* = prev_*  (main_loop.cu:58-110).
 void launch_step_init(hipStream_t s, const ClothDev& c, const float* x,
                       const float* v, float* prev_x, float* prev_v,
-                      float* outer_x, float* outer_v, float* minert);
+                      float* outer_x, float* outer_v);
 
 /// inner = (outer + h v) + h^2 a  (main_loop.cu:112-132).
 void launch_predict(hipStream_t s, const ClothDev& c, const float* outer_x,
@@ -63,7 +62,7 @@ struct LocalStepArgs {
   const float* u_in = nullptr;
   float* z_out = nullptr;
   float* u_out = nullptr;
-  float* elastic_rhs = nullptr;  // 3*nv, next main rhs elastic part
+  double* elastic_rhs = nullptr;  // 3*nv, next main rhs elastic part
   float* pin_lambda = nullptr;   // 3*nv
   // contact terms accumulated into the dual residual (may be null)
   const float* contact_dual = nullptr;   // 3*nv: w (p - p_prev)
@@ -80,14 +79,6 @@ void launch_local_step(hipStream_t s, const ClothDev& c,
                        const LocalStepArgs& a);
 int local_step_grid(int nv);
 
-/// Main-update system: diag d (per vertex) and rhs b (3 per vertex)
-/// (admm_solver.cu:272-292, cloth_admm_helper.cu:264-332,
-/// equality_constraints.cu:107-120, contact_constraints.cu:299-314).
-void launch_assemble(hipStream_t s, const ClothDev& c, const float* minert,
-                     const float* elastic_rhs, const float* pin_lambda,
-                     const float* contact_w, const float* contact_p,
-                     const float* contact_eta, float* d, float* b);
-
 /// Pins x = t, contact vertices x = p (equality_constraints.cu:156-167,
 /// contact_constraints.cu:367-374).
 void launch_enforce(hipStream_t s, const ClothDev& c, const float* contact_w,
@@ -98,7 +89,7 @@ void launch_enforce(hipStream_t s, const ClothDev& c, const float* contact_w,
 /// Per-system PCG state (one system per cloth).  Double-buffered by iteration
 /// parity: iteration k reads state[k&1] and writes state[(k+1)&1].
 struct PcgState {
-  double rz, rr, rr0, pAp;
+  double rz, rr, rr0, bb;
   int iter;
   int done;     // 1 once converged / stopped
   int status;   // 0 abs, 1 rel, 2 max iter, 3 invalid initial residual, -1 running
@@ -133,7 +124,8 @@ struct PcgDev {
   const int* row_cnode = nullptr;      // 4 per row: coarse node per level, -1
   int n1node = 0;                      // level-1 nodes are ids [0, n1node)
   // vectors (3 per row)
-  float* x = nullptr;
+  float* x = nullptr;   // warm start x0 (updated by launch_pcg_commit)
+  float* dx = nullptr;  // accumulated correction: solution = x0 + dx
   float* r = nullptr;
   float* p = nullptr;
   float* q = nullptr;
@@ -141,11 +133,10 @@ struct PcgDev {
   float* zl0 = nullptr;   // level-0 bank output (3 per row)
   float* cr = nullptr;    // coarse r (3 per coarse node)
   float* cz = nullptr;    // coarse z (3 per coarse node)
-  const float* d = nullptr;  // diag (per row)
-  const float* b = nullptr;  // rhs
+  float* d = nullptr;        // diag (per row), written by launch_pcg_init
   double* part = nullptr;    // 2 doubles per block
+  double* part2 = nullptr;   // 2 doubles per block (residual kernel)
   PcgState* state = nullptr; // 2*nsys
-  const float* rel_tol = nullptr;  // per system? (one value for all)
   float abs_tol = 0.0f;
   int max_iter = 1000;
 };
@@ -154,8 +145,36 @@ struct PcgDev {
 /// (mas_preconditioner.cu:489-598,730-859).
 void launch_mas_factor(hipStream_t s, const PcgDev& p, float* cd_scratch);
 
-/// PCG init: r = b - A x, z = M r, p = z, rz0/rr0  (mas_cg_solver.cu:264-296).
-void launch_pcg_init(hipStream_t s, const PcgDev& p, float rel_tol);
+/// Right-hand side of the x-update (admm_solver.cu:272-292,
+/// cloth_admm_helper.cu:264-332, equality_constraints.cu:107-120,
+/// contact_constraints.cu:299-314), consumed directly by the residual kernel.
+struct RhsArgs {
+  ClothDev c;
+  const float* prev_x = nullptr;       // frame-start state (inertia term)
+  const float* prev_v = nullptr;
+  const double* elastic_rhs = nullptr; // from the local step
+  const float* pin_lambda = nullptr;
+  float pin_rho = 1e4f;
+  const float* cw = nullptr;           // active contact weight (may be null)
+  const float* cp = nullptr;
+  const float* ceta = nullptr;
+};
+
+/// PCG init, part 1 (mas_cg_solver.cu:273-277): d and r0 = b - (K + D) x0
+/// assembled and evaluated in FP64 from FP32 inputs.  Must precede
+/// launch_mas_factor, which reads d.
+void launch_pcg_residual(hipStream_t s, const PcgDev& p, const RhsArgs& ra);
+
+/// PCG init, part 2 (mas_cg_solver.cu:279-296): z = M r0, p = z, rz0/rr0,
+/// dx = 0.
+void launch_pcg_start(hipStream_t s, const PcgDev& p);
+
+/// FP64 true residual b - (K + D) x of the committed solution: per-block
+/// partials (|r|^2, |b|^2) in p.part2; r written to p.q.
+void launch_true_residual(hipStream_t s, const PcgDev& p, const RhsArgs& ra);
+
+/// x0 += dx.
+void launch_pcg_commit(hipStream_t s, const PcgDev& p);
 
 /// PCG iterations k .. k+count-1 (mas_cg_solver.cu:321-357 body).
 void launch_pcg_iters(hipStream_t s, const PcgDev& p, int k, int count,

@@ -38,6 +38,14 @@ __device__ __forceinline__ double sys_sum(const PcgDev& p, int sys, int comp,
 }
 
 // q = (K + D) src for the rows of block b; returns src_i . q_i per thread.
+//
+// K annihilates translations (J and C both map constants to zero:
+// cloth_assembly_l2_cache.cpp:27-69,152-171), i.e. every row sums to zero.
+// It is therefore applied in difference form, K src = sum_{j != i} K_ij
+// (src_j - src_i), with only off-diagonal entries stored.  Rounding K to
+// FP32 then cannot create a net force on rigidly translated positions; the
+// naive form K_ii src_i + sum K_ij src_j does, and the spurious force grows
+// with the absolute coordinates.
 __device__ __forceinline__ double spmv_row(const PcgDev& p, int b,
                                            const float* __restrict__ src,
                                            float* __restrict__ out, int row,
@@ -47,34 +55,96 @@ __device__ __forceinline__ double spmv_row(const PcgDev& p, int b,
   const int lane = threadIdx.x % kSlice;
   const int off = p.slice_off[g];
   const int width = p.slice_width[g];
+  const f3 s = ld3(src, row);
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   for (int k = 0; k < width; ++k) {
     const int c = p.ell_col[off + k * kSlice + lane];
     const float a = p.ell_val[off + k * kSlice + lane];
-    ax += a * src[3 * c];
-    ay += a * src[3 * c + 1];
-    az += a * src[3 * c + 2];
+    ax += a * (src[3 * c] - s.x);
+    ay += a * (src[3 * c + 1] - s.y);
+    az += a * (src[3 * c + 2] - s.z);
   }
   const float dd = p.d[row];
-  const f3 s = ld3(src, row);
   const f3 q = {ax + dd * s.x, ay + dd * s.y, az + dd * s.z};
   st3(out, row, q);
   return static_cast<double>(s.x) * q.x + static_cast<double>(s.y) * q.y +
          static_cast<double>(s.z) * q.z;
 }
 
-// ---- init: r = b - (K + D) x
-__global__ void __launch_bounds__(kBlock) k_init_residual(PcgDev p,
-                                                          float* r0) {
+// ---- FP64 residual r = b - (K + D) x of the x-update system.  Also writes
+// the float diagonal d used by the iterations and per-block partials
+// (|r|^2, |b|^2) in part2.  r and b are formed in double from float inputs,
+// so the cancellation between b and A x (both O(1), their difference small)
+// does not lose the FP32 mantissa.
+__global__ void __launch_bounds__(kBlock) k_residual(PcgDev p, RhsArgs ra,
+                                                     const float* __restrict__ x,
+                                                     float* __restrict__ r_out,
+                                                     int write_d) {
+  __shared__ double lds[17];
   const int b = blockIdx.x;
   const int row = p.blk_row0[b] + threadIdx.x;
   const bool valid = threadIdx.x < p.blk_rows[b];
-  spmv_row(p, b, p.x, r0, row, valid);
+  double rr = 0.0, bb = 0.0;
   if (valid) {
-    const f3 bb = ld3(p.b, row);
-    const f3 ax = ld3(r0, row);
-    st3(r0, row, bb - ax);
+    const ClothDev& c = ra.c;
+    const double h = c.h;
+    const double m = c.mass[row];
+    double dd = m / (h * h);
+    double rhs[3];
+    for (int k = 0; k < 3; ++k) {
+      // -m * inertia_mod, inertia_mod = -x/h^2 - v/h - a (admm_solver.cu:141-151)
+      const double px = ra.prev_x[3 * row + k], pv = ra.prev_v[3 * row + k];
+      rhs[k] = ra.elastic_rhs[3 * row + k] + c.bend_rhs[3 * row + k] +
+               m * (px / (h * h) + pv / h + static_cast<double>(c.acc[k]));
+    }
+    if (c.pinned[row]) {
+      dd += ra.pin_rho;
+      for (int k = 0; k < 3; ++k)
+        rhs[k] += static_cast<double>(ra.pin_rho) * c.pin_target[3 * row + k] -
+                  static_cast<double>(ra.pin_lambda[3 * row + k]);
+    }
+    if (ra.cw != nullptr) {
+      const double w = ra.cw[row];
+      if (w > 0.0) {
+        dd += w;
+        for (int k = 0; k < 3; ++k)
+          rhs[k] += w * ra.cp[3 * row + k] + static_cast<double>(ra.ceta[3 * row + k]);
+      }
+    }
+    if (write_d) p.d[row] = static_cast<float>(dd);
+    const int g = b * (kBlock / kSlice) + threadIdx.x / kSlice;
+    const int lane = threadIdx.x % kSlice;
+    const int off = p.slice_off[g];
+    const int width = p.slice_width[g];
+    // Difference form (see spmv_row): the zero row sum of K is exact.
+    const double xi[3] = {x[3 * row], x[3 * row + 1], x[3 * row + 2]};
+    double ax[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < width; ++k) {
+      const int col = p.ell_col[off + k * kSlice + lane];
+      const double a = p.ell_val[off + k * kSlice + lane];
+      ax[0] += a * (static_cast<double>(x[3 * col]) - xi[0]);
+      ax[1] += a * (static_cast<double>(x[3 * col + 1]) - xi[1]);
+      ax[2] += a * (static_cast<double>(x[3 * col + 2]) - xi[2]);
+    }
+    for (int k = 0; k < 3; ++k) {
+      const double res = rhs[k] - (ax[k] + dd * x[3 * row + k]);
+      r_out[3 * row + k] = static_cast<float>(res);
+      rr += res * res;
+      bb += rhs[k] * rhs[k];
+    }
   }
+  const double s0 = block_sum(rr, lds);
+  const double s1 = block_sum(bb, lds);
+  if (threadIdx.x == 0) {
+    p.part2[2 * b] = s0;
+    p.part2[2 * b + 1] = s1;
+  }
+}
+
+__global__ void k_commit(int n3, float* __restrict__ x,
+                         const float* __restrict__ dx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3) x[i] += dx[i];
 }
 
 // ---- level-1 coarse sums of a 3-vector: cr[n] = sum_{rows in n} src
@@ -259,14 +329,24 @@ __global__ void __launch_bounds__(kBlock) k_init_final(PcgDev p) {
   const int sys = p.blk_sys[b];
   const double rz = sys_sum(p, sys, 0, lds);
   const double rr = sys_sum(p, sys, 1, lds);
+  double bb = 0.0;
+  {
+    const int b0 = p.sys_blk0[sys], b1 = p.sys_blk0[sys + 1];
+    double v = 0.0;
+    for (int q = b0 + threadIdx.x; q < b1; q += blockDim.x) v += p.part2[2 * q + 1];
+    bb = block_sum(v, lds);
+  }
   const int row = p.blk_row0[b] + threadIdx.x;
-  if (threadIdx.x < p.blk_rows[b]) st3(p.p, row, ld3(p.z, row));
+  if (threadIdx.x < p.blk_rows[b]) {
+    st3(p.p, row, ld3(p.z, row));
+    st3(p.dx, row, f3{0.0f, 0.0f, 0.0f});
+  }
   if (threadIdx.x == 0 && b == p.sys_blk0[sys]) {
     PcgState s;
     s.rz = rz;
     s.rr = rr;
     s.rr0 = rr;
-    s.pAp = 0.0;
+    s.bb = bb;
     s.iter = 0;
     const bool ok = isfinite(rz);
     s.done = ok ? 0 : 1;
@@ -305,7 +385,7 @@ __global__ void __launch_bounds__(kBlock) k_it_update(PcgDev p, int k,
   if (threadIdx.x < p.blk_rows[b]) {
     const f3 pv = ld3(p.p, row);
     const f3 qv = ld3(p.q, row);
-    st3(p.x, row, ld3(p.x, row) + alpha * pv);
+    st3(p.dx, row, ld3(p.dx, row) + alpha * pv);
     st3(r_out, row, ld3(r_in, row) - alpha * qv);
   }
 }
@@ -367,12 +447,26 @@ void launch_mas_factor(hipStream_t s, const PcgDev& p, float* cd_scratch) {
   k_mas_factor<<<p.nbank, 64, 0, s>>>(p, cd_scratch);
 }
 
-void launch_pcg_init(hipStream_t s, const PcgDev& p, float rel_tol) {
-  (void)rel_tol;
+void launch_pcg_residual(hipStream_t s, const PcgDev& p, const RhsArgs& ra) {
   if (p.nblk == 0) return;
-  k_init_residual<<<p.nblk, kBlock, 0, s>>>(p, p.r);
+  k_residual<<<p.nblk, kBlock, 0, s>>>(p, ra, p.x, p.r, 1);
+}
+
+void launch_pcg_start(hipStream_t s, const PcgDev& p) {
+  if (p.nblk == 0) return;
   mas_apply(s, p, p.r, nullptr);
   k_init_final<<<p.nblk, kBlock, 0, s>>>(p);
+}
+
+void launch_true_residual(hipStream_t s, const PcgDev& p, const RhsArgs& ra) {
+  if (p.nblk == 0) return;
+  k_residual<<<p.nblk, kBlock, 0, s>>>(p, ra, p.x, p.q, 0);
+}
+
+void launch_pcg_commit(hipStream_t s, const PcgDev& p) {
+  const int n3 = 3 * p.nv;
+  if (n3 == 0) return;
+  k_commit<<<grid_for(n3, kBlock), kBlock, 0, s>>>(n3, p.x, p.dx);
 }
 
 void launch_pcg_iters(hipStream_t s, const PcgDev& p, int k, int count,

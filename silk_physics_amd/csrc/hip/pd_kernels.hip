@@ -19,13 +19,10 @@ __global__ void k_step_init(ClothDev c, const float* __restrict__ x,
                             float* __restrict__ prev_x,
                             float* __restrict__ prev_v,
                             float* __restrict__ outer_x,
-                            float* __restrict__ outer_v,
-                            float* __restrict__ minert) {
+                            float* __restrict__ outer_v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.nv) return;
   const float dmp = c.damp[i];
-  const float m = c.mass[i];
-  const float h = c.h;
   for (int d = 0; d < 3; ++d) {
     const float xi = x[3 * i + d];
     const float vi = dmp * v[3 * i + d];
@@ -33,9 +30,6 @@ __global__ void k_step_init(ClothDev c, const float* __restrict__ x,
     prev_v[3 * i + d] = vi;
     outer_x[3 * i + d] = xi;
     outer_v[3 * i + d] = vi;
-    // rhs inertia term -m * inertia_mod, inertia_mod = -x/h^2 - v/h - a.
-    const float mod = -xi / (h * h) - vi / h - c.acc[d];
-    minert[3 * i + d] = -m * mod;
   }
 }
 
@@ -99,10 +93,12 @@ __device__ void clamp_singular_values(const float y[6], float lo, float hi,
   v1x *= inv_len;
   v1y *= inv_len;
   const float v2x = -v1y, v2y = v1x;
-  const float s1 = sqrtf(l1), s2 = sqrtf(l2);
-  // Left singular directions.
+  (void)l1;
+  (void)l2;
+  // Singular values from |Y v| (accurate even when l2 suffers cancellation).
   f3 yv1 = v1x * c0 + v1y * c1;
   f3 yv2 = v2x * c0 + v2y * c1;
+  const float s1 = sqrtf(dot3(yv1, yv1)), s2 = sqrtf(dot3(yv2, yv2));
   const float tiny = 1e-12f;
   f3 u1, u2;
   if (s1 > tiny) {
@@ -113,12 +109,15 @@ __device__ void clamp_singular_values(const float y[6], float lo, float hi,
   if (s2 > 1e-6f * s1 && s2 > tiny) {
     u2 = (1.0f / s2) * yv2;
   } else {
-    // Completion: the coordinate axis least aligned with u1, orthogonalised.
-    f3 e = {1.0f, 0.0f, 0.0f};
-    float ax = fabsf(u1.x), ay = fabsf(u1.y), az = fabsf(u1.z);
-    if (ay <= ax && ay <= az) e = {0.0f, 1.0f, 0.0f};
-    else if (az <= ax && az <= ay) e = {0.0f, 0.0f, 1.0f};
-    if (ax <= ay && ax <= az) e = {1.0f, 0.0f, 0.0f};
+    // Completion: the coordinate axis least aligned with u1 (lowest index on
+    // ties), orthogonalised against u1.
+    const float ax = fabsf(u1.x), ay = fabsf(u1.y), az = fabsf(u1.z);
+    int axis = 0;
+    float amin = ax;
+    if (ay < amin) { axis = 1; amin = ay; }
+    if (az < amin) { axis = 2; }
+    const f3 e = {axis == 0 ? 1.0f : 0.0f, axis == 1 ? 1.0f : 0.0f,
+                  axis == 2 ? 1.0f : 0.0f};
     f3 t = e - dot3(e, u1) * u1;
     u2 = (1.0f / sqrtf(dot3(t, t))) * t;
   }
@@ -174,7 +173,8 @@ __global__ void __launch_bounds__(kBlock)
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   double s[7] = {0, 0, 0, 0, 0, 0, 0};
   if (v < c.nv) {
-    f3 rhs = {0, 0, 0}, dcur = {0, 0, 0}, dprev = {0, 0, 0};
+    double rhs[3] = {0.0, 0.0, 0.0};
+    f3 dcur = {0, 0, 0}, dprev = {0, 0, 0};
     for (int e = c.vf_ptr[v]; e < c.vf_ptr[v + 1]; ++e) {
       const int f = c.vf_list[e] >> 2;
       const int corner = c.vf_list[e] & 3;
@@ -202,7 +202,9 @@ __global__ void __launch_bounds__(kBlock)
       const f3 sun = g0 * un0 + g1 * un1;
       dcur = dcur + rw2 * szn;
       dprev = dprev + rw2 * szo;
-      rhs = rhs + (rw2 * szn - w * sun);
+      rhs[0] += static_cast<double>(rw2) * szn.x - static_cast<double>(w) * sun.x;
+      rhs[1] += static_cast<double>(rw2) * szn.y - static_cast<double>(w) * sun.y;
+      rhs[2] += static_cast<double>(rw2) * szn.z - static_cast<double>(w) * sun.z;
       if (corner == 0) {
         // Face owner: write the new aux variables and the primal terms.
         for (int j = 0; j < 6; ++j) {
@@ -217,7 +219,9 @@ __global__ void __launch_bounds__(kBlock)
         }
       }
     }
-    st3(a.elastic_rhs, v, rhs);
+    a.elastic_rhs[3 * v] = rhs[0];
+    a.elastic_rhs[3 * v + 1] = rhs[1];
+    a.elastic_rhs[3 * v + 2] = rhs[2];
     const f3 xv = ld3(a.x, v);
     // Pins (equality_constraints.cu:90-154).
     if (c.pinned[v]) {
@@ -262,33 +266,6 @@ __global__ void __launch_bounds__(kBlock)
   if (threadIdx.x == 0) a.partials[kAdmmPartials * blockIdx.x + 7] = 0.0;
 }
 
-__global__ void k_assemble(ClothDev c, const float* __restrict__ minert,
-                           const float* __restrict__ erhs,
-                           const float* __restrict__ lam,
-                           const float* __restrict__ cw,
-                           const float* __restrict__ cp,
-                           const float* __restrict__ ceta, float pin_rho,
-                           float* __restrict__ d, float* __restrict__ b) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= c.nv) return;
-  const float h = c.h;
-  float dv = c.mass[v] / (h * h);
-  f3 bv = ld3(erhs, v) + (ld3(c.bend_rhs, v) + ld3(minert, v));
-  if (c.pinned[v]) {
-    dv += pin_rho;
-    bv = bv + (pin_rho * ld3(c.pin_target, v) - ld3(lam, v));
-  }
-  if (cw != nullptr) {
-    const float w = cw[v];
-    if (w > 0.0f) {
-      dv += w;
-      bv = bv + (w * ld3(cp, v) + ld3(ceta, v));
-    }
-  }
-  d[v] = dv;
-  st3(b, v, bv);
-}
-
 __global__ void k_enforce(ClothDev c, const float* __restrict__ cw,
                           const float* __restrict__ cp, float* __restrict__ x) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -303,10 +280,10 @@ inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
 
 void launch_step_init(hipStream_t s, const ClothDev& c, const float* x,
                       const float* v, float* prev_x, float* prev_v,
-                      float* outer_x, float* outer_v, float* minert) {
+                      float* outer_x, float* outer_v) {
   if (c.nv == 0) return;
   k_step_init<<<grid_for(c.nv), kBlock, 0, s>>>(c, x, v, prev_x, prev_v,
-                                                outer_x, outer_v, minert);
+                                                outer_x, outer_v);
 }
 
 void launch_predict(hipStream_t s, const ClothDev& c, const float* outer_x,
@@ -333,16 +310,6 @@ void launch_local_step(hipStream_t s, const ClothDev& c,
                        const LocalStepArgs& a) {
   if (c.nv == 0) return;
   k_local_step<<<grid_for(c.nv), kBlock, 0, s>>>(c, a);
-}
-
-void launch_assemble(hipStream_t s, const ClothDev& c, const float* minert,
-                     const float* elastic_rhs, const float* pin_lambda,
-                     const float* contact_w, const float* contact_p,
-                     const float* contact_eta, float* d, float* b) {
-  if (c.nv == 0) return;
-  k_assemble<<<grid_for(c.nv), kBlock, 0, s>>>(c, minert, elastic_rhs,
-                                               pin_lambda, contact_w, contact_p,
-                                               contact_eta, 1e4f, d, b);
 }
 
 void launch_enforce(hipStream_t s, const ClothDev& c, const float* contact_w,
