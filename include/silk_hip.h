@@ -193,6 +193,32 @@ int silk_hip_get_partition(const silk_hip_ctx* ctx, uint32_t handle,
 int silk_hip_get_timing(silk_hip_ctx* ctx, int32_t cls, double* ms,
                         int64_t* launches, int32_t reset);
 
+/* Collider vertex count of the current scene: cloth vertices (solver order,
+ * cloths in insertion order) followed by obstacle vertices (insertion order,
+ * caller order).  0 when no object pair can collide.  Builds the device
+ * scene if needed. */
+int silk_hip_collider_vertex_count(silk_hip_ctx* ctx, int64_t* nvt);
+
+/* Broadphase of the current scene on explicit collider positions pos0 -> pos1
+ * (n = 3 * collider vertex count floats each): triangle-point candidates as
+ * (point, tri0, tri1, tri2) and edge-edge candidates as (a0, a1, b0, b1) with
+ * the edge of smaller (min, max) vertex pair first, global collider ids, in
+ * no particular order.  Lists are written up to their capacity (in tuples);
+ * *npt / *nee receive the full counts. */
+int silk_hip_debug_candidates(silk_hip_ctx* ctx, const float* pos0,
+                              const float* pos1, int64_t n, int32_t* pt,
+                              int64_t pt_cap, int64_t* npt, int32_t* ee,
+                              int64_t ee_cap, int64_t* nee);
+
+/* The step's FP64 CCD kernel on n explicit queries: is_vf[i] selects
+ * vertex-face (point, tri0, tri1, tri2) or edge-edge (a0, a1, b0, b1);
+ * p0 / p1 hold 12 doubles per query, err 3 per query.  hit[i] = 1 and toi[i]
+ * (local time in [0, 1]) for a hit, hit[i] = 0 otherwise. */
+int silk_hip_debug_ccd(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
+                       const double* p0, const double* p1, const double* err,
+                       double ms, double tolerance, int32_t max_iter,
+                       int32_t* hit, double* toi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,8 @@ def lib():
             "oracle_ccd_filter": (None, [d, C.c_int, d]),
             "oracle_coulomb": (None, [d, d, C.c_double, d]),
             "oracle_ee_uv": (C.c_int, [d, C.c_double, d]),
+            "oracle_candidates": (C.c_int, [vp, d, d, i, C.c_int, i, i, C.c_int,
+                                            i]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -132,6 +134,29 @@ class OracleWorld:
         f, fp = _i(F)
         return lib().oracle_add_obstacle(self._w, cop, vp, v.size // 3, fp,
                                          f.size // 3)
+
+    def set_obstacle_position(self, idx: int, V):
+        v, vp = _f(V)
+        lib().oracle_set_obstacle_position(self._w, idx, vp)
+
+    def candidates(self, pos0, pos1):
+        """Broadphase candidates for explicit collider positions (global
+        numbering): (PT tuples (p, t0, t1, t2), EE tuples (a0, a1, b0, b1))."""
+        a, ap = _d(pos0)
+        b, bp = _d(pos1)
+        npt, nee = C.c_int(0), C.c_int(0)
+        cap_pt, cap_ee = 1 << 16, 1 << 16
+        while True:
+            pt = np.zeros(4 * cap_pt, dtype=np.int32)
+            ee = np.zeros(4 * cap_ee, dtype=np.int32)
+            rc = lib().oracle_candidates(
+                self._w, ap, bp, pt.ctypes.data_as(C.POINTER(C.c_int)), cap_pt,
+                C.byref(npt), ee.ctypes.data_as(C.POINTER(C.c_int)), cap_ee,
+                C.byref(nee))
+            if rc == 0:
+                return (pt[:4 * npt.value].reshape(-1, 4),
+                        ee[:4 * nee.value].reshape(-1, 4))
+            cap_pt, cap_ee = max(cap_pt, npt.value), max(cap_ee, nee.value)
 
     def finalize(self):
         lib().oracle_finalize(self._w)

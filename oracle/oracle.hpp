@@ -117,6 +117,9 @@ struct CcdConfig {
   int stack_cap = 1024;   // pending-box capacity (overflow returns early)
 };
 
+/// Pending-box capacity of the world-level CCD; equals the HIP kernel's.
+constexpr int kCcdStackCap = 64;
+
 /// Earliest conservative TOI of a vertex-face (vf=true) or edge-edge query,
 /// restating interval_root_finder.cpp:391-510 and ccd.cpp:195-257 in double.
 /// Returns false for no hit; otherwise *toi in [0, max_time].

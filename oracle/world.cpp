@@ -58,6 +58,7 @@ struct OObstacle {
   std::vector<float> curr, prev;
   std::vector<int> F, E;
   bool is_static = true, is_static_twice = true;
+  bool has_pos = false;
   float padding = 0.0f;
 };
 
@@ -558,6 +559,7 @@ static double min_toi(World& w, double time_start, double interval, double ms) {
   cfg.tolerance = w.P.ccd_tol / interval;
   cfg.max_iter = w.P.ccd_max_iter;
   cfg.max_time = 1.0;
+  cfg.stack_cap = kCcdStackCap;
   double best = 1.0;
   auto traj = [&](int gv, double p0[3], double p1[3]) {
     if (gv < w.nv_cloth) {
@@ -1007,6 +1009,16 @@ int oracle_add_obstacle(void* wp, const double* coll, const float* V, int nv,
 void oracle_set_obstacle_position(void* wp, int idx, const float* V) {
   World& w = *static_cast<World*>(wp);
   auto& o = w.obstacles[idx];
+  if (!o.has_pos) {
+    // First position record: static at that position
+    // (cuda_backend.cu:339-349, PinPosition::from_position).
+    o.curr.assign(V, V + 3 * o.nv);
+    o.prev = o.curr;
+    o.is_static = true;
+    o.is_static_twice = true;
+    o.has_pos = true;
+    return;
+  }
   o.is_static = false;
   o.is_static_twice = false;
   o.prev = o.curr;
@@ -1023,6 +1035,53 @@ int oracle_step(void* wp, double* stats) {
     stats[5] = w.stats.ccd_cand; stats[6] = w.stats.min_toi;
   }
   return ok ? 0 : 1;
+}
+
+/// Broadphase candidates of the current objects for explicit segment end
+/// positions pos0 -> pos1 (global numbering: cloths in solver order, then
+/// obstacles).  PT entries are (point, tri0, tri1, tri2), EE entries
+/// (a0, a1, b0, b1) in canonical edge order.  Returns 0, or 1 if a list
+/// exceeded its capacity (counts are still exact).
+int oracle_candidates(void* wp, const double* pos0, const double* pos1,
+                      int* pt, int pt_cap, int* npt, int* ee, int ee_cap,
+                      int* nee) {
+  World& w = *static_cast<World*>(wp);
+  int off = 0;
+  for (auto& c : w.cloths) {
+    c.off = off;
+    off += c.nv;
+  }
+  w.nv_cloth = off;
+  for (auto& o : w.obstacles) {
+    o.off = off;
+    off += o.nv;
+  }
+  w.nv_total = off;
+  w.pos_t0.assign(pos0, pos0 + 3 * off);
+  w.pos_t1.assign(pos1, pos1 + 3 * off);
+  auto objs = oracle::build_objects(w);
+  auto cand = oracle::candidates(w, objs);
+  *npt = static_cast<int>(cand.pt.size());
+  *nee = static_cast<int>(cand.ee.size());
+  int i = 0;
+  for (auto& [t, p] : cand.pt) {
+    if (i < pt_cap) {
+      pt[4 * i] = p->v[0];
+      for (int k = 0; k < 3; ++k) pt[4 * i + 1 + k] = t->v[k];
+    }
+    ++i;
+  }
+  i = 0;
+  for (auto& [a, b] : cand.ee) {
+    if (i < ee_cap) {
+      ee[4 * i] = a->v[0];
+      ee[4 * i + 1] = a->v[1];
+      ee[4 * i + 2] = b->v[0];
+      ee[4 * i + 3] = b->v[1];
+    }
+    ++i;
+  }
+  return (*npt > pt_cap || *nee > ee_cap) ? 1 : 0;
 }
 
 /// Positions of cloth `idx` in caller order (double).

@@ -39,6 +39,7 @@ def load_library():
     vp = C.c_void_p
     fp = C.POINTER(C.c_float)
     ip = C.POINTER(C.c_int32)
+    dp = C.POINTER(C.c_double)
     sig = {
         "silk_hip_device_count": (C.c_int, [ip]),
         "silk_hip_create": (C.c_int, [i32, C.POINTER(vp)]),
@@ -65,6 +66,12 @@ def load_library():
         "silk_hip_get_partition": (C.c_int, [vp, u32, ip, i64, ip, i64, ip]),
         "silk_hip_get_timing": (C.c_int, [vp, i32, C.POINTER(C.c_double),
                                           C.POINTER(C.c_int64), i32]),
+        "silk_hip_collider_vertex_count": (C.c_int, [vp, C.POINTER(i64)]),
+        "silk_hip_debug_candidates": (C.c_int, [vp, fp, fp, i64, ip, i64,
+                                                C.POINTER(i64), ip, i64,
+                                                C.POINTER(i64)]),
+        "silk_hip_debug_ccd": (C.c_int, [vp, i64, ip, dp, dp, dp, C.c_double,
+                                         C.c_double, i32, ip, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -412,6 +419,53 @@ class World:
         st = lib.silk_hip_get_partition(self._ctx, handle, None, 0, _iptr(offsets),
                                         offsets.size, None)
         return self._wrap(st), perm, offsets
+
+    def collider_vertex_count(self) -> int:
+        n = C.c_int64(0)
+        st = load_library().silk_hip_collider_vertex_count(self._ctx, C.byref(n))
+        if st != 0:
+            raise RuntimeError(self._wrap(st).to_string())
+        return n.value
+
+    def debug_candidates(self, pos0, pos1):
+        """Broadphase of the current scene on explicit collider positions:
+        (PT tuples (p, t0, t1, t2), EE tuples (a0, a1, b0, b1))."""
+        a = np.ascontiguousarray(pos0, dtype=np.float32).reshape(-1)
+        b = np.ascontiguousarray(pos1, dtype=np.float32).reshape(-1)
+        lib = load_library()
+        npt, nee = C.c_int64(0), C.c_int64(0)
+        st = lib.silk_hip_debug_candidates(self._ctx, _fptr(a), _fptr(b), a.size,
+                                           None, 0, C.byref(npt), None, 0,
+                                           C.byref(nee))
+        if st != 0:
+            raise RuntimeError(self._wrap(st).to_string())
+        pt = np.zeros((max(1, npt.value), 4), dtype=np.int32)
+        ee = np.zeros((max(1, nee.value), 4), dtype=np.int32)
+        st = lib.silk_hip_debug_candidates(self._ctx, _fptr(a), _fptr(b), a.size,
+                                           _iptr(pt), npt.value, C.byref(npt),
+                                           _iptr(ee), nee.value, C.byref(nee))
+        if st != 0:
+            raise RuntimeError(self._wrap(st).to_string())
+        return pt[:npt.value], ee[:nee.value]
+
+    def debug_ccd(self, is_vf, p0, p1, err, ms, tolerance, max_iter=4096):
+        """The step's FP64 CCD kernel on explicit queries: (hit, toi)."""
+        v = np.ascontiguousarray(is_vf, dtype=np.int32).reshape(-1)
+        a = np.ascontiguousarray(p0, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(p1, dtype=np.float64).reshape(-1)
+        e = np.ascontiguousarray(err, dtype=np.float64).reshape(-1)
+        n = v.size
+        assert a.size == 12 * n and b.size == 12 * n and e.size == 3 * n
+        hit = np.zeros(n, dtype=np.int32)
+        toi = np.zeros(n, dtype=np.float64)
+        dp = C.POINTER(C.c_double)
+        st = load_library().silk_hip_debug_ccd(
+            self._ctx, n, _iptr(v), a.ctypes.data_as(dp), b.ctypes.data_as(dp),
+            e.ctypes.data_as(dp), ms, tolerance, max_iter, _iptr(hit),
+            toi.ctypes.data_as(dp))
+        if st != 0:
+            raise RuntimeError(self._wrap(st).to_string())
+        return hit, toi
 
     def timing(self, cls: int, reset: bool = False) -> tuple[float, int]:
         ms = C.c_double(0)

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "api/context.hpp"
+#include "hip/collision_kernels.hpp"
 #include "silk_hip.h"
 
 using silk::amd::ClothEntity;
@@ -329,6 +330,86 @@ int silk_hip_get_timing(silk_hip_ctx* ctx, int32_t cls, double* ms,
       c.timing_ms_[cls] = 0.0;
       c.timing_launch_[cls] = 0;
     }
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_collider_vertex_count(silk_hip_ctx* ctx, int64_t* nvt) {
+  return guarded(ctx, [&](Context& c) {
+    if (nvt == nullptr) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    if (c.cloths_.empty()) {
+      *nvt = 0;
+      return int{SILK_HIP_OK};
+    }
+    if (c.scene_dirty_ || !c.scene_) c.build_scene();
+    *nvt = c.scene_->coll ? c.scene_->coll->total_vertices() : 0;
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_debug_candidates(silk_hip_ctx* ctx, const float* pos0,
+                              const float* pos1, int64_t n, int32_t* pt,
+                              int64_t pt_cap, int64_t* npt, int32_t* ee,
+                              int64_t ee_cap, int64_t* nee) {
+  return guarded(ctx, [&](Context& c) {
+    if (npt == nullptr || nee == nullptr) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    if (c.cloths_.empty()) return int{SILK_HIP_ERR_NEED_INIT_SOLVER};
+    if (c.scene_dirty_ || !c.scene_) c.build_scene();
+    auto* cw = c.scene_->coll.get();
+    if (cw == nullptr) {
+      *npt = 0;
+      *nee = 0;
+      return int{SILK_HIP_OK};
+    }
+    if (pos0 == nullptr || pos1 == nullptr ||
+        n != 3 * static_cast<int64_t>(cw->total_vertices()))
+      return int{SILK_HIP_ERR_INVALID_POSITION};
+    std::vector<int> vpt, vee;
+    cw->debug_candidates(pos0, pos1, vpt, vee);
+    *npt = static_cast<int64_t>(vpt.size() / 4);
+    *nee = static_cast<int64_t>(vee.size() / 4);
+    if (pt != nullptr)
+      std::copy_n(vpt.begin(), 4 * std::min<int64_t>(pt_cap, *npt), pt);
+    if (ee != nullptr)
+      std::copy_n(vee.begin(), 4 * std::min<int64_t>(ee_cap, *nee), ee);
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_debug_ccd(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
+                       const double* p0, const double* p1, const double* err,
+                       double ms, double tolerance, int32_t max_iter,
+                       int32_t* hit, double* toi) {
+  return guarded(ctx, [&](Context& c) {
+    if (n < 0 || n > (1 << 24)) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    if (n == 0) return int{SILK_HIP_OK};
+    if (!is_vf || !p0 || !p1 || !err || !hit || !toi)
+      return int{SILK_HIP_ERR_INVALID_CONFIG};
+    c.ensure_device();
+    using silk::amd::DevBuf;
+    const size_t m = static_cast<size_t>(n);
+    std::vector<int> hv(is_vf, is_vf + m);
+    std::vector<double> h0(p0, p0 + 12 * m), h1(p1, p1 + 12 * m),
+        he(err, err + 3 * m);
+    DevBuf<int> dv, dhit;
+    DevBuf<double> d0, d1, de, dtoi;
+    dv.upload(hv, c.stream_);
+    d0.upload(h0, c.stream_);
+    d1.upload(h1, c.stream_);
+    de.upload(he, c.stream_);
+    dhit.resize(m);
+    dtoi.resize(m);
+    silk::amd::launch_ccd_each(c.stream_, static_cast<int>(n), dv.get(),
+                               d0.get(), d1.get(), de.get(), ms, tolerance,
+                               max_iter, dhit.get(), dtoi.get());
+    silk::amd::hip_check(hipGetLastError(), "ccd launch");
+    silk::amd::hip_check(hipMemcpyAsync(hit, dhit.get(), m * sizeof(int),
+                                        hipMemcpyDeviceToHost, c.stream_),
+                         "ccd hit");
+    silk::amd::hip_check(hipMemcpyAsync(toi, dtoi.get(), m * sizeof(double),
+                                        hipMemcpyDeviceToHost, c.stream_),
+                         "ccd toi");
+    silk::amd::hip_check(hipStreamSynchronize(c.stream_), "ccd sync");
     return int{SILK_HIP_OK};
   });
 }

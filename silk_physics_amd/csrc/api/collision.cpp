@@ -1,14 +1,807 @@
-// Collision subsystem construction.
+// Collision subsystem of the step on the device.
+//
+// Reference pipeline (silk/src/backend/cuda/main_loop.cu:186-277):
+//   frame start   refit colliders at t0 = t1 = x, DCD, contact set
+//   every segment refit swept colliders x -> x_pred, CCD, earliest TOI
+//   rollback      refit at the rolled-back x, DCD at that time, append
+// and the Daviet contact projection inside every ADMM iteration
+// (solver/contact_constraints.cu:203-411).
+//
+// Device side: per object one implicit BVH over its triangles and one over
+// its edges (Morton-sorted once, refit every call), point and edge queries
+// traverse them; the CCD kernel and the DCD kernel consume the candidate
+// lists straight from device memory.  Host side: the DCD records are keyed on
+// global vertex ids, sorted, de-duplicated against the contacts of the step,
+// turned into Daviet contacts and greedily coloured in contact order, which
+// makes the contact solve independent of the order in which the GPU found the
+// pairs.
 #include "api/collision.hpp"
 
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <set>
+
 #include "api/context.hpp"
+#include "hip/collision_kernels.hpp"
 
 namespace silk::amd {
 
+namespace {
+
+struct ObjDesc {
+  bool physical = false;
+  bool self_on = false;
+  int group = -1;  // -1: collision off (object_collider.cu:135,197)
+  float pad = 0.0f;
+  float friction = 0.0f;
+  int v0 = 0, nv = 0;  // global vertex range
+  int t0 = 0, nt = 0;  // global triangle range
+  int e0 = 0, ne = 0;  // global edge range
+  int tree_t = -1, tree_e = -1;
+};
+
+uint32_t spread10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// Daviet contact in double, as assembled on the host (contact_constraints.cu:
+// 23-150).
+struct HostContact {
+  int state[4];
+  double coef[4], gamma[4], normal[3], affine[3], mu;
+};
+
+// Order-independent identity of a DCD record: type and global vertex ids,
+// triangle corners sorted, each edge (min, max) and the smaller edge first.
+std::array<int, 5> contact_key(const DcdRec& r) {
+  std::array<int, 5> k{r.type, r.gv[0], r.gv[1], r.gv[2], r.gv[3]};
+  if (r.type == 0) {
+    std::sort(k.begin() + 2, k.end());
+  } else {
+    if (k[2] < k[1]) std::swap(k[1], k[2]);
+    if (k[4] < k[3]) std::swap(k[3], k[4]);
+    if (std::make_pair(k[3], k[4]) < std::make_pair(k[1], k[2])) {
+      std::swap(k[1], k[3]);
+      std::swap(k[2], k[4]);
+    }
+  }
+  return k;
+}
+
+class HipCollision final : public CollisionWorld {
+ public:
+  HipCollision(Context& ctx, Scene& S, std::vector<ObjDesc> objs, int nvc,
+               int nvt);
+  ~HipCollision() override;
+
+  void begin_step(const float* outer_x, float* ms, float* dhat,
+                  silk_hip_step_stats* stats) override;
+  float min_toi(const float* outer_x, const float* inner_x, float time_start,
+                float interval, float ms, silk_hip_step_stats* stats) override;
+  void rollback_contacts(const float* outer_x, float time, float dhat,
+                         silk_hip_step_stats* stats) override;
+  void contact_update(const float* x) override;
+  ContactView contact_view() const override;
+  int active_dof() const override { return 3 * n_active_; }
+  void end_step(silk_hip_step_stats* stats) override;
+  void debug_candidates(const float* pos0, const float* pos1,
+                        std::vector<int>& pt, std::vector<int>& ee) override;
+  int total_vertices() const override { return nvt_; }
+
+  // Static tables (filled by make_collision_world).
+  std::vector<int> h_tri, h_edge, h_tri_obj, h_edge_obj, h_point_obj;
+  void finish_setup(const std::vector<float>& pos0);
+
+ private:
+  Context& ctx_;
+  Scene& S_;
+  hipStream_t st_;
+  std::vector<ObjDesc> objs_;
+  int nvc_, nvt_;
+  float min_pad_ = 0.0f;
+  float cur_dhat_ = 0.0f;
+
+  DevBuf<float> p0_, p1_, invm_;
+  DevBuf<int> tri_, edge_, point_, tri_obj_, edge_obj_, point_obj_;
+  DevBuf<float> obj_friction_;
+  std::vector<TreeDev> trees_;
+  DevBuf<TreeDev> trees_dev_;
+  DevBuf<int> tree_prim_, tree_prim_id_;
+  DevBuf<Box6> sbox_, node_;
+
+  // candidates and outputs
+  DevBuf<int2> pt_pairs_, ee_pairs_;
+  int pt_cap_ = 1 << 16, ee_cap_ = 1 << 16;
+  DevBuf<DcdRec> recs_;
+  int rec_cap_ = 1 << 12;
+  DevBuf<int> counters_;  // pt, ee, dcd, pad
+  DevBuf<unsigned long long> toi_bits_;
+  DevBuf<double> err6_;
+  int* h_io_ = nullptr;  // pinned: counters[4] then toi bits (8 bytes)
+
+  // contacts of the step
+  std::vector<HostContact> contacts_;
+  std::set<std::array<int, 5>> keys_;
+  std::vector<int> color_off_h_, color_list_h_;
+  int n_active_ = 0;
+  DevBuf<ContactDev> c_dev_;
+  DevBuf<int> color_off_, color_list_;
+  DevBuf<float> corr_;
+  int corr_cap_ = 0;
+  DevBuf<float> w_, p_, p_prev_, eta_, dual_;
+
+  CollVerts verts() const;
+  PrimTables tables() const;
+  void set_cloth(const float* a, const float* b);
+  void refit();
+  void launch_broadphase();
+  std::vector<DcdRec> dcd(float time, float dhat, silk_hip_step_stats* stats);
+  void append(std::vector<DcdRec> recs);
+  void recolor();
+  void upload_contacts(int n_old);
+  void read_io();
+};
+
+HipCollision::HipCollision(Context& ctx, Scene& S, std::vector<ObjDesc> objs,
+                           int nvc, int nvt)
+    : ctx_(ctx), S_(S), st_(ctx.stream_), objs_(std::move(objs)), nvc_(nvc),
+      nvt_(nvt) {
+  hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_io_), 32), "hipHostMalloc");
+  std::memset(h_io_, 0, 32);
+  min_pad_ = 1e30f;
+  for (const auto& o : objs_) min_pad_ = std::min(min_pad_, o.pad);
+}
+
+HipCollision::~HipCollision() {
+  if (h_io_ != nullptr) (void)hipHostFree(h_io_);
+}
+
+void HipCollision::finish_setup(const std::vector<float>& pos0) {
+  const size_t n3 = 3 * static_cast<size_t>(nvt_);
+  p0_.resize(n3);
+  p1_.resize(n3);
+  std::vector<float> hinv(nvt_, 0.0f);
+  for (int i = 0; i < nvc_; ++i)
+    hinv[i] = S_.h_pinned[i] ? 0.0f : static_cast<float>(1.0 / S_.h_mass[i]);
+  invm_.upload(hinv, st_);
+  std::vector<int> points(nvt_);
+  for (int i = 0; i < nvt_; ++i) points[i] = i;
+  std::vector<float> fr;
+  for (const auto& o : objs_) fr.push_back(o.friction);
+  tri_.upload(h_tri.empty() ? std::vector<int>{0, 0, 0} : h_tri, st_);
+  edge_.upload(h_edge.empty() ? std::vector<int>{0, 0} : h_edge, st_);
+  point_.upload(points, st_);
+  tri_obj_.upload(h_tri_obj.empty() ? std::vector<int>{0} : h_tri_obj, st_);
+  edge_obj_.upload(h_edge_obj.empty() ? std::vector<int>{0} : h_edge_obj, st_);
+  point_obj_.upload(h_point_obj, st_);
+  obj_friction_.upload(fr, st_);
+
+  // Trees: primitives Morton-sorted by their centroid at pos0.
+  std::vector<int> prim, prim_id;
+  size_t nbox = 0, nnode = 0;
+  struct Pending {
+    int nprim, nvert, P;
+    float pad;
+    size_t prim_off, box_off, node_off;
+  };
+  std::vector<Pending> pend;
+  auto add_tree = [&](const int* verts, int nprim, int nvert, int gid0,
+                      float pad) {
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    std::vector<std::array<float, 3>> cen(nprim);
+    for (int i = 0; i < nprim; ++i) {
+      for (int d = 0; d < 3; ++d) {
+        float s = 0.0f;
+        for (int k = 0; k < nvert; ++k) s += pos0[3 * verts[nvert * i + k] + d];
+        cen[i][d] = s / static_cast<float>(nvert);
+        lo[d] = std::min(lo[d], cen[i][d]);
+        hi[d] = std::max(hi[d], cen[i][d]);
+      }
+    }
+    std::vector<std::pair<uint32_t, int>> code(nprim);
+    for (int i = 0; i < nprim; ++i) {
+      uint32_t c = 0;
+      for (int d = 0; d < 3; ++d) {
+        const float ext = hi[d] - lo[d];
+        const float t = ext > 0.0f ? (cen[i][d] - lo[d]) / ext : 0.0f;
+        const uint32_t q = static_cast<uint32_t>(std::clamp(t * 1023.0f, 0.0f, 1023.0f));
+        c |= spread10(q) << (2 - d);
+      }
+      code[i] = {c, i};
+    }
+    std::sort(code.begin(), code.end());
+    Pending p;
+    p.nprim = nprim;
+    p.nvert = nvert;
+    int leaves = std::max(1, (nprim + kLeafPrims - 1) / kLeafPrims);
+    p.P = 1;
+    while (p.P < leaves) p.P <<= 1;
+    p.pad = pad;
+    p.prim_off = prim.size();
+    p.box_off = nbox;
+    p.node_off = nnode;
+    for (const auto& [c, i] : code) {
+      for (int k = 0; k < nvert; ++k) prim.push_back(verts[nvert * i + k]);
+      prim_id.push_back(gid0 + i);
+    }
+    nbox += std::max(1, nprim);
+    nnode += 2 * static_cast<size_t>(p.P);
+    pend.push_back(p);
+    return static_cast<int>(pend.size()) - 1;
+  };
+  for (auto& o : objs_) {
+    o.tree_t = add_tree(h_tri.data() + 3 * static_cast<size_t>(o.t0), o.nt, 3,
+                        o.t0, o.pad);
+    o.tree_e = add_tree(h_edge.data() + 2 * static_cast<size_t>(o.e0), o.ne, 2,
+                        o.e0, o.pad);
+  }
+  if (prim.empty()) prim.push_back(0);
+  if (prim_id.empty()) prim_id.push_back(0);
+  tree_prim_.upload(prim, st_);
+  tree_prim_id_.upload(prim_id, st_);
+  sbox_.resize(std::max<size_t>(1, nbox));
+  node_.resize(std::max<size_t>(1, nnode));
+  size_t id_off = 0;
+  for (const auto& p : pend) {
+    TreeDev t;
+    t.nprim = p.nprim;
+    t.nvert = p.nvert;
+    t.P = p.P;
+    t.pad = p.pad;
+    t.prim = tree_prim_.get() + p.prim_off;
+    t.prim_id = tree_prim_id_.get() + id_off;
+    t.sbox = sbox_.get() + p.box_off;
+    t.node = node_.get() + p.node_off;
+    id_off += p.nprim;
+    trees_.push_back(t);
+  }
+  trees_dev_.upload(trees_, st_);
+
+  pt_pairs_.resize(pt_cap_);
+  ee_pairs_.resize(ee_cap_);
+  recs_.resize(rec_cap_);
+  counters_.resize(4);
+  toi_bits_.resize(1);
+  err6_.resize(6);
+  const size_t nc3 = 3 * static_cast<size_t>(std::max(1, nvc_));
+  w_.resize(std::max(1, nvc_));
+  for (DevBuf<float>* b : {&p_, &p_prev_, &eta_, &dual_}) b->resize(nc3);
+  // Host vectors above are temporaries: drain the uploads before they go.
+  hip_check(hipStreamSynchronize(st_), "collision setup");
+}
+
+CollVerts HipCollision::verts() const {
+  CollVerts v;
+  v.nvc = nvc_;
+  v.nvt = nvt_;
+  v.p0 = p0_.get();
+  v.p1 = p1_.get();
+  v.invm = invm_.get();
+  return v;
+}
+
+PrimTables HipCollision::tables() const {
+  PrimTables t;
+  t.tri = tri_.get();
+  t.edge = edge_.get();
+  t.point = point_.get();
+  t.tri_obj = tri_obj_.get();
+  t.edge_obj = edge_obj_.get();
+  t.point_obj = point_obj_.get();
+  t.obj_friction = obj_friction_.get();
+  return t;
+}
+
+void HipCollision::set_cloth(const float* a, const float* b) {
+  const size_t bytes = 3 * static_cast<size_t>(nvc_) * sizeof(float);
+  hip_check(hipMemcpyAsync(p0_.get(), a, bytes, hipMemcpyDeviceToDevice, st_),
+            "collider t0");
+  hip_check(hipMemcpyAsync(p1_.get(), b, bytes, hipMemcpyDeviceToDevice, st_),
+            "collider t1");
+}
+
+void HipCollision::refit() {
+  launch_refit(st_, verts(), trees_dev_.get(), static_cast<int>(trees_.size()));
+}
+
+// Object pairs and self pairs of find_collision.cu:17-26,45-166.
+void HipCollision::launch_broadphase() {
+  hip_check(hipMemsetAsync(counters_.get(), 0, 4 * sizeof(int), st_), "counters");
+  const CollVerts v = verts();
+  PairOut pt{pt_pairs_.get(), counters_.get(), pt_cap_};
+  PairOut ee{ee_pairs_.get(), counters_.get() + 1, ee_cap_};
+  auto points = [&](const ObjDesc& o) {
+    QuerySet q;
+    q.n = o.nv;
+    q.nvert = 1;
+    q.prim = point_.get() + o.v0;
+    q.pad = o.pad;
+    q.gid0 = o.v0;
+    return q;
+  };
+  auto edges = [&](const ObjDesc& o) {
+    QuerySet q;
+    q.n = o.ne;
+    q.nvert = 2;
+    q.prim = edge_.get() + 2 * static_cast<size_t>(o.e0);
+    q.pad = o.pad;
+    q.gid0 = o.e0;
+    return q;
+  };
+  auto root = [&](const ObjDesc& o) { return trees_[o.tree_t].node + 1; };
+  const size_t n = objs_.size();
+  for (size_t i = 0; i < n; ++i)
+    for (size_t j = i + 1; j < n; ++j) {
+      const ObjDesc& a = objs_[i];
+      const ObjDesc& b = objs_[j];
+      if (a.group == -1 || b.group == -1 || a.group != b.group) continue;
+      if (!(a.physical || b.physical)) continue;
+      launch_traverse(st_, v, trees_[a.tree_t], points(b), 0, 0, root(a), root(b), pt);
+      launch_traverse(st_, v, trees_[b.tree_t], points(a), 0, 0, root(a), root(b), pt);
+      launch_traverse(st_, v, trees_[a.tree_e], edges(b), 1, 0, root(a), root(b), ee);
+    }
+  for (const auto& o : objs_) {
+    if (!o.physical || !o.self_on) continue;
+    launch_traverse(st_, v, trees_[o.tree_t], points(o), 0, 1, nullptr, nullptr, pt);
+    launch_traverse(st_, v, trees_[o.tree_e], edges(o), 1, 1, nullptr, nullptr, ee);
+  }
+}
+
+void HipCollision::read_io() {
+  hip_check(hipMemcpyAsync(h_io_, counters_.get(), 4 * sizeof(int),
+                           hipMemcpyDeviceToHost, st_),
+            "counters");
+  hip_check(hipMemcpyAsync(h_io_ + 4, toi_bits_.get(), sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, st_),
+            "toi");
+  hip_check(hipStreamSynchronize(st_), "collision sync");
+}
+
+static int grow(int need) { return std::max(1024, need + need / 2); }
+
+std::vector<DcdRec> HipCollision::dcd(float time, float dhat,
+                                      silk_hip_step_stats* stats) {
+  for (;;) {
+    launch_broadphase();
+    launch_dcd(st_, verts(), tables(), pt_pairs_.get(), counters_.get(), pt_cap_,
+               ee_pairs_.get(), counters_.get() + 1, ee_cap_, time, dhat,
+               recs_.get(), counters_.get() + 2, rec_cap_);
+    read_io();
+    const int npt = h_io_[0], nee = h_io_[1], nrec = h_io_[2];
+    bool again = false;
+    if (npt > pt_cap_) {
+      pt_cap_ = grow(npt);
+      pt_pairs_.resize(pt_cap_);
+      again = true;
+    }
+    if (nee > ee_cap_) {
+      ee_cap_ = grow(nee);
+      ee_pairs_.resize(ee_cap_);
+      again = true;
+    }
+    if (nrec > rec_cap_) {
+      rec_cap_ = grow(nrec);
+      recs_.resize(rec_cap_);
+      again = true;
+    }
+    if (again) continue;
+    if (stats) stats->dcd_candidates += npt + nee;
+    std::vector<DcdRec> out(nrec);
+    if (nrec > 0) {
+      hip_check(hipMemcpyAsync(out.data(), recs_.get(), nrec * sizeof(DcdRec),
+                               hipMemcpyDeviceToHost, st_),
+                "dcd records");
+      hip_check(hipStreamSynchronize(st_), "dcd sync");
+    }
+    return out;
+  }
+}
+
+// Contact assembly of contact_constraints.cu:23-150, in double.
+void HipCollision::append(std::vector<DcdRec> recs) {
+  std::sort(recs.begin(), recs.end(), [](const DcdRec& a, const DcdRec& b) {
+    return contact_key(a) < contact_key(b);
+  });
+  const int n_old = static_cast<int>(contacts_.size());
+  for (const DcdRec& r : recs) {
+    if (!keys_.insert(contact_key(r)).second) continue;
+    HostContact c;
+    c.mu = r.friction;
+    double ref[3] = {0.0, 0.0, 0.0};
+    const double dhat = static_cast<double>(cur_dhat_);
+    for (int d = 0; d < 3; ++d) {
+      c.normal[d] = r.normal[d];
+      c.affine[d] = -dhat * c.normal[d];
+    }
+    double scatter = 0.0;
+    for (int k = 0; k < 4; ++k) {
+      c.coef[k] = r.coef[k];
+      for (int d = 0; d < 3; ++d) ref[d] += c.coef[k] * r.x[k][d];
+      if (r.gv[k] >= nvc_) {
+        c.state[k] = -1;
+        for (int d = 0; d < 3; ++d) c.affine[d] += c.coef[k] * r.x[k][d];
+        continue;
+      }
+      c.state[k] = r.gv[k];
+      if (r.invm[k] > 0.0f)
+        scatter += c.coef[k] * c.coef[k] / S_.h_contact_w[r.gv[k]];
+    }
+    for (int d = 0; d < 3; ++d) c.affine[d] += r.presc[d];
+    const double rn =
+        ref[0] * c.normal[0] + ref[1] * c.normal[1] + ref[2] * c.normal[2];
+    for (int d = 0; d < 3; ++d) c.affine[d] -= ref[d] - rn * c.normal[d];
+    for (int k = 0; k < 4; ++k) {
+      if (c.state[k] >= 0 && r.invm[k] > 0.0f)
+        c.gamma[k] = c.coef[k] / (scatter * S_.h_contact_w[c.state[k]]);
+      else
+        c.gamma[k] = 0.0;
+    }
+    contacts_.push_back(c);
+  }
+  if (static_cast<int>(contacts_.size()) == n_old) return;
+  recolor();
+  upload_contacts(n_old);
+}
+
+// Greedy first-fit colouring in contact order over shared movable vertices
+// (contact_coloring.cu:241-392 with a deterministic schedule).
+void HipCollision::recolor() {
+  const int n = static_cast<int>(contacts_.size());
+  std::map<int, std::vector<int>> incid;
+  for (int c = 0; c < n; ++c)
+    for (int k = 0; k < 4; ++k)
+      if (contacts_[c].gamma[k] != 0.0) incid[contacts_[c].state[k]].push_back(c);
+  std::vector<int> color(n, 0);
+  int maxc = 0;
+  std::vector<char> used;
+  for (int c = 0; c < n; ++c) {
+    used.assign(maxc + 2, 0);
+    for (int k = 0; k < 4; ++k) {
+      if (contacts_[c].gamma[k] == 0.0) continue;
+      for (int o : incid[contacts_[c].state[k]])
+        if (o < c) used[color[o]] = 1;
+    }
+    int col = 1;
+    while (used[col]) ++col;
+    color[c] = col;
+    maxc = std::max(maxc, col);
+  }
+  color_off_h_.assign(maxc + 1, 0);
+  for (int c = 0; c < n; ++c) color_off_h_[color[c]] += 1;
+  for (int k = 1; k <= maxc; ++k) color_off_h_[k] += color_off_h_[k - 1];
+  color_list_h_.assign(n, 0);
+  std::vector<int> fill(color_off_h_.begin(), color_off_h_.end() - 1);
+  for (int c = 0; c < n; ++c) color_list_h_[fill[color[c] - 1]++] = c;
+  std::vector<float> hw(std::max(1, nvc_), 0.0f);
+  n_active_ = 0;
+  for (const auto& [v, list] : incid) {
+    (void)list;
+    ++n_active_;
+    hw[v] = static_cast<float>(S_.h_contact_w[v]);
+  }
+  hip_check(hipMemcpyAsync(w_.get(), hw.data(), hw.size() * sizeof(float),
+                           hipMemcpyHostToDevice, st_),
+            "contact weights");
+  hip_check(hipStreamSynchronize(st_), "contact weights sync");
+}
+
+void HipCollision::upload_contacts(int n_old) {
+  const int n = static_cast<int>(contacts_.size());
+  std::vector<ContactDev> hc(n);
+  for (int i = 0; i < n; ++i) {
+    const HostContact& c = contacts_[i];
+    ContactDev& d = hc[i];
+    for (int k = 0; k < 4; ++k) {
+      d.state[k] = c.state[k];
+      d.coef[k] = static_cast<float>(c.coef[k]);
+      d.gamma[k] = static_cast<float>(c.gamma[k]);
+    }
+    for (int j = 0; j < 3; ++j) {
+      d.normal[j] = static_cast<float>(c.normal[j]);
+      d.affine[j] = static_cast<float>(c.affine[j]);
+    }
+    d.mu = static_cast<float>(c.mu);
+    d.pad = 0.0f;
+  }
+  c_dev_.upload(hc, st_);
+  color_off_.upload(color_off_h_, st_);
+  color_list_.upload(color_list_h_, st_);
+  // Corrections persist across appends; new contacts start at zero.
+  if (n > corr_cap_) {
+    const int cap = std::max(1024, 2 * n);
+    DevBuf<float> nc(3 * static_cast<size_t>(cap));
+    if (n_old > 0)
+      hip_check(hipMemcpyAsync(nc.get(), corr_.get(), 3 * sizeof(float) * n_old,
+                               hipMemcpyDeviceToDevice, st_),
+                "corr grow");
+    corr_ = std::move(nc);
+    corr_cap_ = cap;
+  }
+  hip_check(hipMemsetAsync(corr_.get() + 3 * static_cast<size_t>(n_old), 0,
+                           3 * sizeof(float) * (n - n_old), st_),
+            "corr tail");
+  hip_check(hipStreamSynchronize(st_), "contacts upload");
+}
+
+void HipCollision::begin_step(const float* outer_x, float* ms, float* dhat,
+                              silk_hip_step_stats* stats) {
+  // Obstacle segment prev -> curr of this frame.
+  const size_t nvo = static_cast<size_t>(nvt_ - nvc_);
+  if (nvo > 0) {
+    std::vector<float> a, b;
+    a.reserve(3 * nvo);
+    b.reserve(3 * nvo);
+    for (const auto& o : ctx_.obstacles_) {
+      const std::vector<float>& cur = o.has_pos ? o.curr : o.mesh.V;
+      const std::vector<float>& prv = o.has_pos ? o.prev : o.mesh.V;
+      a.insert(a.end(), prv.begin(), prv.end());
+      b.insert(b.end(), cur.begin(), cur.end());
+    }
+    hip_check(hipMemcpyAsync(p0_.get() + 3 * static_cast<size_t>(nvc_), a.data(),
+                             a.size() * sizeof(float), hipMemcpyHostToDevice, st_),
+              "obstacle t0");
+    hip_check(hipMemcpyAsync(p1_.get() + 3 * static_cast<size_t>(nvc_), b.data(),
+                             b.size() * sizeof(float), hipMemcpyHostToDevice, st_),
+              "obstacle t1");
+    hip_check(hipStreamSynchronize(st_), "obstacle upload");
+  }
+  // Minimum separation and DCD activation distance from the smallest
+  // collider padding (main_loop.cu:186-200).
+  *ms = ctx_.ccd_ms_scale() * min_pad_;
+  *dhat = ctx_.dcd_scale() * min_pad_;
+  cur_dhat_ = *dhat;
+  // The contact set restarts every step.
+  contacts_.clear();
+  keys_.clear();
+  color_off_h_.clear();
+  color_list_h_.clear();
+  n_active_ = 0;
+  const size_t nc3 = 3 * static_cast<size_t>(nvc_);
+  hip_check(hipMemsetAsync(w_.get(), 0, nvc_ * sizeof(float), st_), "w reset");
+  for (DevBuf<float>* b : {&p_, &p_prev_, &eta_, &dual_})
+    hip_check(hipMemsetAsync(b->get(), 0, nc3 * sizeof(float), st_), "reset");
+  set_cloth(outer_x, outer_x);
+  refit();
+  append(dcd(0.0f, *dhat, stats));
+}
+
+float HipCollision::min_toi(const float* outer_x, const float* inner_x,
+                            float time_start, float interval, float ms,
+                            silk_hip_step_stats* stats) {
+  set_cloth(outer_x, inner_x);
+  refit();
+  launch_ccd_filter(st_, verts(), err6_.get());
+  CcdParams cp;
+  cp.ms = ms;
+  cp.tolerance = static_cast<double>(ctx_.ccd_tolerance()) / interval;
+  cp.max_iter = ctx_.ccd_max_iter();
+  cp.time_start = time_start;
+  cp.interval = interval;
+  cp.err_vf = err6_.get();
+  cp.err_ee = err6_.get() + 3;
+  for (;;) {
+    const double one = 1.0;
+    unsigned long long bits;
+    std::memcpy(&bits, &one, sizeof(bits));
+    // toi bits start at 1.0: write through the pinned staging word.
+    std::memcpy(h_io_ + 6, &bits, sizeof(bits));
+    hip_check(hipMemcpyAsync(toi_bits_.get(), h_io_ + 6, sizeof(bits),
+                             hipMemcpyHostToDevice, st_),
+              "toi init");
+    launch_broadphase();
+    launch_ccd(st_, verts(), tables(), pt_pairs_.get(), counters_.get(), pt_cap_,
+               ee_pairs_.get(), counters_.get() + 1, ee_cap_, cp,
+               toi_bits_.get());
+    read_io();
+    const int npt = h_io_[0], nee = h_io_[1];
+    bool again = false;
+    if (npt > pt_cap_) {
+      pt_cap_ = grow(npt);
+      pt_pairs_.resize(pt_cap_);
+      again = true;
+    }
+    if (nee > ee_cap_) {
+      ee_cap_ = grow(nee);
+      ee_pairs_.resize(ee_cap_);
+      again = true;
+    }
+    if (again) continue;
+    if (stats) stats->ccd_candidates += npt + nee;
+    double toi;
+    std::memcpy(&toi, h_io_ + 4, sizeof(toi));
+    return static_cast<float>(static_cast<double>(interval) * toi);
+  }
+}
+
+void HipCollision::rollback_contacts(const float* outer_x, float time,
+                                     float dhat, silk_hip_step_stats* stats) {
+  set_cloth(outer_x, outer_x);
+  refit();
+  append(dcd(time, dhat, stats));
+}
+
+void HipCollision::contact_update(const float* x) {
+  if (contacts_.empty()) return;
+  ContactSet cs;
+  cs.n = static_cast<int>(contacts_.size());
+  cs.ncolor = static_cast<int>(color_off_h_.size()) - 1;
+  cs.nv = nvc_;
+  cs.c = c_dev_.get();
+  cs.color_off = color_off_.get();
+  cs.color_list = color_list_.get();
+  cs.corr = corr_.get();
+  cs.w = w_.get();
+  cs.p = p_.get();
+  cs.p_prev = p_prev_.get();
+  cs.eta = eta_.get();
+  cs.dual = dual_.get();
+  cs.sweeps = ctx_.contact_sweeps();
+  launch_contact_update(st_, cs, x);
+}
+
+ContactView HipCollision::contact_view() const {
+  ContactView v;
+  if (contacts_.empty()) return v;
+  v.w = w_.get();
+  v.p = p_.get();
+  v.eta = eta_.get();
+  v.dual = dual_.get();
+  return v;
+}
+
+void HipCollision::end_step(silk_hip_step_stats* stats) {
+  if (!stats) return;
+  stats->contacts = static_cast<int>(contacts_.size());
+  stats->contact_colors =
+      color_off_h_.empty() ? 0 : static_cast<int>(color_off_h_.size()) - 1;
+}
+
+void HipCollision::debug_candidates(const float* pos0, const float* pos1,
+                                    std::vector<int>& pt, std::vector<int>& ee) {
+  const size_t bytes = 3 * static_cast<size_t>(nvt_) * sizeof(float);
+  hip_check(hipMemcpyAsync(p0_.get(), pos0, bytes, hipMemcpyHostToDevice, st_),
+            "debug t0");
+  hip_check(hipMemcpyAsync(p1_.get(), pos1, bytes, hipMemcpyHostToDevice, st_),
+            "debug t1");
+  refit();
+  int npt = 0, nee = 0;
+  for (;;) {
+    launch_broadphase();
+    read_io();
+    npt = h_io_[0];
+    nee = h_io_[1];
+    bool again = false;
+    if (npt > pt_cap_) {
+      pt_cap_ = grow(npt);
+      pt_pairs_.resize(pt_cap_);
+      again = true;
+    }
+    if (nee > ee_cap_) {
+      ee_cap_ = grow(nee);
+      ee_pairs_.resize(ee_cap_);
+      again = true;
+    }
+    if (!again) break;
+  }
+  std::vector<int2> hp(npt), he(nee);
+  if (npt > 0)
+    hip_check(hipMemcpyAsync(hp.data(), pt_pairs_.get(), npt * sizeof(int2),
+                             hipMemcpyDeviceToHost, st_),
+              "debug pt");
+  if (nee > 0)
+    hip_check(hipMemcpyAsync(he.data(), ee_pairs_.get(), nee * sizeof(int2),
+                             hipMemcpyDeviceToHost, st_),
+              "debug ee");
+  hip_check(hipStreamSynchronize(st_), "debug sync");
+  pt.clear();
+  ee.clear();
+  for (const int2& q : hp) {
+    pt.push_back(q.y);
+    for (int k = 0; k < 3; ++k) pt.push_back(h_tri[3 * static_cast<size_t>(q.x) + k]);
+  }
+  for (const int2& q : he) {
+    int a0 = h_edge[2 * static_cast<size_t>(q.x)], a1 = h_edge[2 * static_cast<size_t>(q.x) + 1];
+    int b0 = h_edge[2 * static_cast<size_t>(q.y)], b1 = h_edge[2 * static_cast<size_t>(q.y) + 1];
+    const auto ka = std::make_pair(std::min(a0, a1), std::max(a0, a1));
+    const auto kb = std::make_pair(std::min(b0, b1), std::max(b0, b1));
+    if (kb < ka) {
+      std::swap(a0, b0);
+      std::swap(a1, b1);
+    }
+    ee.insert(ee.end(), {a0, a1, b0, b1});
+  }
+}
+
+}  // namespace
+
 std::unique_ptr<CollisionWorld> make_collision_world(Context& ctx, Scene& S) {
-  (void)ctx;
-  (void)S;
-  return nullptr;
+  // Objects: cloths (solver order, scene offsets), then obstacles.
+  std::vector<ObjDesc> objs;
+  std::vector<int> tri, edge, tri_obj, edge_obj, point_obj;
+  int v0 = 0;
+  for (size_t ci = 0; ci < ctx.cloths_.size(); ++ci) {
+    const ClothEntity& c = ctx.cloths_[ci];
+    ObjDesc o;
+    o.physical = true;
+    o.self_on = c.coll.is_self_collision_on != 0;
+    o.group = c.coll.is_collision_on ? c.coll.group : -1;
+    o.pad = 0.05f * c.mesh.avg_edge_length;
+    o.friction = c.coll.friction;
+    o.v0 = S.vert_off[ci];
+    o.nv = c.mesh.nv();
+    o.t0 = static_cast<int>(tri.size() / 3);
+    o.nt = c.pmesh.nf();
+    o.e0 = static_cast<int>(edge.size() / 2);
+    o.ne = c.pmesh.ne();
+    for (int f : c.pmesh.F) tri.push_back(o.v0 + f);
+    for (int e : c.pmesh.E) edge.push_back(o.v0 + e);
+    v0 = o.v0 + o.nv;
+    objs.push_back(o);
+  }
+  const int nvc = v0;
+  for (const auto& ob : ctx.obstacles_) {
+    ObjDesc o;
+    o.physical = false;
+    o.self_on = ob.coll.is_self_collision_on != 0;
+    o.group = ob.coll.is_collision_on ? ob.coll.group : -1;
+    o.pad = 0.05f * ob.mesh.avg_edge_length;
+    o.friction = ob.coll.friction;
+    o.v0 = v0;
+    o.nv = ob.mesh.nv();
+    o.t0 = static_cast<int>(tri.size() / 3);
+    o.nt = ob.mesh.nf();
+    o.e0 = static_cast<int>(edge.size() / 2);
+    o.ne = ob.mesh.ne();
+    for (int f : ob.mesh.F) tri.push_back(o.v0 + f);
+    for (int e : ob.mesh.E) edge.push_back(o.v0 + e);
+    v0 += o.nv;
+    objs.push_back(o);
+  }
+  const int nvt = v0;
+  // Anything to collide at all?
+  bool any = false;
+  for (size_t i = 0; i < objs.size() && !any; ++i) {
+    if (objs[i].physical && objs[i].self_on) any = true;
+    for (size_t j = i + 1; j < objs.size() && !any; ++j)
+      any = objs[i].group != -1 && objs[i].group == objs[j].group &&
+            (objs[i].physical || objs[j].physical);
+  }
+  if (!any) return nullptr;
+  for (size_t k = 0; k < objs.size(); ++k) {
+    for (int t = 0; t < objs[k].nt; ++t) tri_obj.push_back(static_cast<int>(k));
+    for (int e = 0; e < objs[k].ne; ++e) edge_obj.push_back(static_cast<int>(k));
+    for (int v = 0; v < objs[k].nv; ++v) point_obj.push_back(static_cast<int>(k));
+  }
+  // Positions used to order the primitives along a Morton curve.
+  std::vector<float> pos(3 * static_cast<size_t>(nvt));
+  hip_check(hipMemcpyAsync(pos.data(), S.x.get(), 3 * sizeof(float) * nvc,
+                           hipMemcpyDeviceToHost, ctx.stream_),
+            "collider positions");
+  hip_check(hipStreamSynchronize(ctx.stream_), "collider positions sync");
+  {
+    size_t off = 3 * static_cast<size_t>(nvc);
+    for (const auto& ob : ctx.obstacles_) {
+      const std::vector<float>& cur = ob.has_pos ? ob.curr : ob.mesh.V;
+      std::copy(cur.begin(), cur.end(), pos.begin() + off);
+      off += cur.size();
+    }
+  }
+  auto hc = std::make_unique<HipCollision>(ctx, S, std::move(objs), nvc, nvt);
+  hc->h_tri = std::move(tri);
+  hc->h_edge = std::move(edge);
+  hc->h_tri_obj = std::move(tri_obj);
+  hc->h_edge_obj = std::move(edge_obj);
+  hc->h_point_obj = std::move(point_obj);
+  hc->finish_setup(pos);
+  return hc;
 }
 
 }  // namespace silk::amd

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <memory>
+#include <vector>
 
 #include "silk_hip.h"
 
@@ -42,6 +43,12 @@ class CollisionWorld {
   virtual ContactView contact_view() const = 0;
   virtual int active_dof() const = 0;
   virtual void end_step(silk_hip_step_stats* stats) = 0;
+  /// Broadphase on explicit host positions (3 * total collider vertices per
+  /// end): PT tuples (point, tri0, tri1, tri2) and EE tuples (a0, a1, b0, b1)
+  /// in canonical edge order, global vertex ids.
+  virtual void debug_candidates(const float* pos0, const float* pos1,
+                                std::vector<int>& pt, std::vector<int>& ee) = 0;
+  virtual int total_vertices() const = 0;
 };
 
 /// nullptr when no object has collision enabled.

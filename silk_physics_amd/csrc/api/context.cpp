@@ -161,6 +161,7 @@ void Context::build_scene() {
 
   std::vector<float> hx(3 * nv), hv(3 * nv), hfparam(4 * nf), hfstiff(2 * nf),
       hmass(nv), hdamp(nv), hpin_t(3 * nv, 0.0f), hcw(nv);
+  std::vector<double> hcw_d(nv);
   std::vector<double> hbend(3 * nv);
   std::vector<int> hF(3 * nf), hpinned(nv, 0);
   std::vector<std::vector<int>> vf(nv);
@@ -220,8 +221,8 @@ void Context::build_scene() {
         if (K.col[a] == v) kvv = K.val[a];
       const double inertia = m / (static_cast<double>(h) * h);
       const double eta = kvv + inertia;
-      hcw[vo + v] = static_cast<float>(
-          std::max(0.001 * eta, std::min(25.0 * inertia, eta)));
+      hcw_d[vo + v] = std::max(0.001 * eta, std::min(25.0 * inertia, eta));
+      hcw[vo + v] = static_cast<float>(hcw_d[vo + v]);
     }
     for (size_t i = 0; i < c.pins.size(); ++i) {
       const int lv = c.part.perm[c.pins[i]];
@@ -384,6 +385,7 @@ void Context::build_scene() {
   S.contact_base_w.upload(hcw, st);
   S.h_mass = hmass;
   S.h_pinned = hpinned;
+  S.h_contact_w = hcw_d;
   S.admm_part.resize(kAdmmPartials * static_cast<size_t>(local_step_grid(nv)));
 
   S.nblk = static_cast<int>(blk_sys.size());

@@ -68,6 +68,7 @@ struct Scene {
   DevBuf<double> bend_rhs;
   std::vector<float> h_mass;  // nv, density applied
   std::vector<int> h_pinned;
+  std::vector<double> h_contact_w;  // nv, Daviet base weight
   // ---- local-step partials
   DevBuf<double> admm_part;
   std::vector<double> h_admm_part;

@@ -1,0 +1,820 @@
+// Collision kernels of the step.  See collision_kernels.hpp.
+//
+// Reference: collision/object_collider.cu:378-475 (swept, padded boxes),
+// oibvh.cuh:417-516 (stack traversal, closed-interval overlap, self pairs
+// once), find_collision.cu:45-92 (filters), dcd.cu:23-315 (DCD),
+// cpu/collision/interval_root_finder.cpp + ccd.cpp (the CCD restated here in
+// FP64), contact_constraints.cu:203-411 (Daviet Gauss-Seidel).
+#include <cfloat>
+
+#include "hip/collision_kernels.hpp"
+#include "hip/device_common.hpp"
+
+namespace silk::amd {
+
+namespace {
+
+constexpr int kRefitThreads = 1024;
+
+__device__ __forceinline__ void swept_box(const CollVerts& v, const int* gv,
+                                          int n, float pad, Box6& b) {
+  for (int d = 0; d < 3; ++d) {
+    float mn = 1e30f, mx = -1e30f;
+    for (int k = 0; k < n; ++k) {
+      const float a = v.p0[3 * gv[k] + d];
+      const float c = v.p1[3 * gv[k] + d];
+      mn = fminf(mn, fminf(a, c));
+      mx = fmaxf(mx, fmaxf(a, c));
+    }
+    b.mn[d] = mn - pad;
+    b.mx[d] = mx + pad;
+  }
+}
+
+__device__ __forceinline__ bool overlap(const Box6& a, const Box6& b) {
+  for (int d = 0; d < 3; ++d) {
+    const float lo = fmaxf(a.mn[d], b.mn[d]);
+    const float hi = fminf(a.mx[d], b.mx[d]);
+    if (!(lo <= hi)) return false;
+  }
+  return true;
+}
+
+__device__ __forceinline__ Box6 merge(const Box6& a, const Box6& b) {
+  Box6 r;
+  for (int d = 0; d < 3; ++d) {
+    r.mn[d] = fminf(a.mn[d], b.mn[d]);
+    r.mx[d] = fmaxf(a.mx[d], b.mx[d]);
+  }
+  return r;
+}
+
+__device__ __forceinline__ Box6 empty_box() {
+  Box6 r;
+  for (int d = 0; d < 3; ++d) {
+    r.mn[d] = FLT_MAX;
+    r.mx[d] = -FLT_MAX;
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------- refit
+__global__ void __launch_bounds__(kRefitThreads)
+    k_refit(CollVerts v, const TreeDev* __restrict__ trees) {
+  const TreeDev t = trees[blockIdx.x];
+  for (int i = threadIdx.x; i < t.nprim; i += blockDim.x) {
+    int gv[3];
+    for (int k = 0; k < t.nvert; ++k) gv[k] = t.prim[t.nvert * i + k];
+    Box6 b;
+    swept_box(v, gv, t.nvert, t.pad, b);
+    t.sbox[i] = b;
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < t.P; l += blockDim.x) {
+    Box6 b = empty_box();
+    const int j0 = l * kLeafPrims;
+    const int j1 = min(j0 + kLeafPrims, t.nprim);
+    for (int j = j0; j < j1; ++j) b = merge(b, t.sbox[j]);
+    t.node[t.P + l] = b;
+  }
+  __syncthreads();
+  for (int half = t.P >> 1; half >= 1; half >>= 1) {
+    for (int j = half + threadIdx.x; j < 2 * half; j += blockDim.x)
+      t.node[j] = merge(t.node[2 * j], t.node[2 * j + 1]);
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- traversal
+__global__ void __launch_bounds__(256)
+    k_traverse(CollVerts v, TreeDev t, QuerySet q, int kind, int self,
+               const Box6* __restrict__ gate_a, const Box6* __restrict__ gate_b,
+               PairOut out) {
+  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= q.n) return;
+  if (gate_a != nullptr && gate_b != nullptr && !overlap(*gate_a, *gate_b))
+    return;
+  const int qi = q.gid0 + li;
+  int qv[2];
+  for (int k = 0; k < q.nvert; ++k) qv[k] = q.prim[q.nvert * li + k];
+  Box6 qb;
+  swept_box(v, qv, q.nvert, q.pad, qb);
+  float qinv = 0.0f;
+  for (int k = 0; k < q.nvert; ++k) qinv += v.invm[qv[k]];
+  int stack[32];
+  int sp = 0;
+  stack[sp++] = 1;
+  while (sp > 0) {
+    const int node = stack[--sp];
+    if (!overlap(qb, t.node[node])) continue;
+    if (node >= t.P) {
+      const int j0 = (node - t.P) * kLeafPrims;
+      const int j1 = min(j0 + kLeafPrims, t.nprim);
+      for (int j = j0; j < j1; ++j) {
+        if (!overlap(qb, t.sbox[j])) continue;
+        int tv[3];
+        float inv = qinv;
+        for (int k = 0; k < t.nvert; ++k) {
+          tv[k] = t.prim[t.nvert * j + k];
+          inv += v.invm[tv[k]];
+        }
+        if (inv == 0.0f) continue;  // all pinned / kinematic
+        const int tid_orig = t.prim_id[j];
+        if (self) {
+          bool adjacent = false;
+          for (int a = 0; a < t.nvert; ++a)
+            for (int b = 0; b < q.nvert; ++b) adjacent |= (tv[a] == qv[b]);
+          if (adjacent) continue;
+          if (kind == 1 && !(tid_orig < qi)) continue;  // each pair once
+        }
+        const int slot = atomicAdd(out.count, 1);
+        if (slot < out.cap) out.pairs[slot] = make_int2(tid_orig, qi);
+      }
+    } else {
+      stack[sp++] = 2 * node + 1;
+      stack[sp++] = 2 * node;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- DCD
+__device__ __forceinline__ dev::f3 pos_at(const CollVerts& v, int gv,
+                                          float time) {
+  const dev::f3 a = dev::ld3(v.p0, gv), b = dev::ld3(v.p1, gv);
+  return (1.0f - time) * a + time * b;
+}
+
+// exact_pt_uv (dcd.cu:321-351)
+__device__ bool pt_uv(dev::f3 x0, dev::f3 x1, dev::f3 x2, dev::f3 x3, float eps,
+                      float* u, float* v) {
+  using namespace dev;
+  const f3 x21 = x2 - x1, x31 = x3 - x1, x01 = x0 - x1;
+  const float a = dot3(x21, x21), b = dot3(x21, x31), c = dot3(x31, x31);
+  const float d = dot3(x21, x01), e = dot3(x31, x01);
+  const float det = a * c - b * b;
+  const float area_eps = eps * fmaxf(a, c);
+  if (det < area_eps * area_eps) return false;
+  const float b1 = (c * d - b * e) / det;
+  const float b2 = (a * e - b * d) / det;
+  if (b1 < 0.0f || b2 < 0.0f || b1 + b2 > 1.0f) return false;
+  *u = b1;
+  *v = b2;
+  return true;
+}
+
+// exact_ee_uv (dcd.cu:353-394), double inside.
+__device__ bool ee_uv(dev::f3 x0, dev::f3 x1, dev::f3 x2, dev::f3 x3, float eps,
+                      float* uo, float* vo) {
+  const double d1[3] = {double(x1.x) - x0.x, double(x1.y) - x0.y, double(x1.z) - x0.z};
+  const double d2[3] = {double(x3.x) - x2.x, double(x3.y) - x2.y, double(x3.z) - x2.z};
+  const double r[3] = {double(x0.x) - x2.x, double(x0.y) - x2.y, double(x0.z) - x2.z};
+  const double a = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+  const double b = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
+  const double c = d1[0] * r[0] + d1[1] * r[1] + d1[2] * r[2];
+  const double e = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+  const double f = d2[0] * r[0] + d2[1] * r[1] + d2[2] * r[2];
+  const double eps2 = static_cast<double>(eps) * eps;
+  const double ls2 = fmax(a, e);
+  if (a <= eps2 * ls2 || e <= eps2 * ls2) return false;
+  const double den = a * e - b * b;
+  double u = den > eps2 * a * e ? fmin(fmax((b * f - c * e) / den, 0.0), 1.0) : 0.0;
+  double v = (b * u + f) / e;
+  if (v < 0.0) {
+    v = 0.0;
+    u = fmin(fmax(-c / a, 0.0), 1.0);
+  } else if (v > 1.0) {
+    v = 1.0;
+    u = fmin(fmax((b - c) / a, 0.0), 1.0);
+  }
+  *uo = static_cast<float>(u);
+  *vo = static_cast<float>(v);
+  return true;
+}
+
+__device__ void emit(DcdRec* out, int* count, int cap, const DcdRec& r) {
+  const int slot = atomicAdd(count, 1);
+  if (slot < cap) out[slot] = r;
+}
+
+__device__ __forceinline__ float pair_friction(const PrimTables& pt, int oa,
+                                               int ob) {
+  return static_cast<float>(sqrt(static_cast<double>(pt.obj_friction[oa]) *
+                                 static_cast<double>(pt.obj_friction[ob])));
+}
+
+__device__ void finish_rec(const CollVerts& v, DcdRec& r, const dev::f3 x[4],
+                           dev::f3 disp, float dist) {
+  using namespace dev;
+  const f3 n = (-1.0f / dist) * disp;
+  r.normal[0] = n.x;
+  r.normal[1] = n.y;
+  r.normal[2] = n.z;
+  for (int k = 0; k < 4; ++k) {
+    r.x[k][0] = x[k].x;
+    r.x[k][1] = x[k].y;
+    r.x[k][2] = x[k].z;
+  }
+  // Prescribed obstacle displacement coef_k (x_k(t1) - x_k(time))
+  // (dcd.cu:164-174,245-255).
+  f3 pres = {0.0f, 0.0f, 0.0f};
+  for (int k = 0; k < 4; ++k) {
+    if (r.gv[k] < v.nvc) continue;
+    pres = pres + r.coef[k] * (ld3(v.p1, r.gv[k]) - x[k]);
+  }
+  r.presc[0] = pres.x;
+  r.presc[1] = pres.y;
+  r.presc[2] = pres.z;
+}
+
+__global__ void __launch_bounds__(256)
+    k_dcd(CollVerts v, PrimTables pt, const int2* __restrict__ ptp,
+          const int* __restrict__ ptc, int pt_cap, const int2* __restrict__ eep,
+          const int* __restrict__ eec, int ee_cap, float time, float dhat,
+          DcdRec* out, int* out_count, int out_cap) {
+  using namespace dev;
+  const int npt = min(*ptc, pt_cap);
+  const int nee = min(*eec, ee_cap);
+  const float eps = 1e-6f;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npt + nee;
+       i += gridDim.x * blockDim.x) {
+    DcdRec r;
+    f3 x[4];
+    float param[4];
+    f3 disp;
+    if (i < npt) {
+      const int2 pr = ptp[i];
+      const int t = pr.x, p = pr.y;
+      r.type = 0;
+      r.gv[0] = pt.point[p];
+      r.gv[1] = pt.tri[3 * t];
+      r.gv[2] = pt.tri[3 * t + 1];
+      r.gv[3] = pt.tri[3 * t + 2];
+      for (int k = 0; k < 4; ++k) x[k] = pos_at(v, r.gv[k], time);
+      float u, w;
+      if (!pt_uv(x[0], x[1], x[2], x[3], eps, &u, &w)) continue;
+      const f3 tp = ((1.0f - u - w) * x[1] + u * x[2]) + w * x[3];
+      disp = tp - x[0];
+      param[0] = 1.0f;
+      param[1] = 1.0f - u - w;
+      param[2] = u;
+      param[3] = w;
+      r.coef[0] = 1.0f;
+      r.coef[1] = -param[1];
+      r.coef[2] = -param[2];
+      r.coef[3] = -param[3];
+      r.friction = pair_friction(pt, pt.tri_obj[t], pt.point_obj[p]);
+    } else {
+      const int2 pr = eep[i - npt];
+      int a0 = pt.edge[2 * pr.x], a1 = pt.edge[2 * pr.x + 1];
+      int b0 = pt.edge[2 * pr.y], b1 = pt.edge[2 * pr.y + 1];
+      int oa = pt.edge_obj[pr.x], ob = pt.edge_obj[pr.y];
+      // Canonical order: smaller (min, max) vertex pair is edge a.
+      const int ka0 = min(a0, a1), ka1 = max(a0, a1);
+      const int kb0 = min(b0, b1), kb1 = max(b0, b1);
+      if (kb0 < ka0 || (kb0 == ka0 && kb1 < ka1)) {
+        int t0 = a0, t1 = a1, to = oa;
+        a0 = b0; a1 = b1; oa = ob;
+        b0 = t0; b1 = t1; ob = to;
+      }
+      r.type = 1;
+      r.gv[0] = a0;
+      r.gv[1] = a1;
+      r.gv[2] = b0;
+      r.gv[3] = b1;
+      for (int k = 0; k < 4; ++k) x[k] = pos_at(v, r.gv[k], time);
+      float u, w;
+      if (!ee_uv(x[0], x[1], x[2], x[3], eps, &u, &w)) continue;
+      const f3 pa = (1.0f - u) * x[0] + u * x[1];
+      const f3 pb = (1.0f - w) * x[2] + w * x[3];
+      disp = pb - pa;
+      param[0] = 1.0f - u;
+      param[1] = u;
+      param[2] = 1.0f - w;
+      param[3] = w;
+      r.coef[0] = param[0];
+      r.coef[1] = param[1];
+      r.coef[2] = -param[2];
+      r.coef[3] = -param[3];
+      r.friction = pair_friction(pt, oa, ob);
+    }
+    const float d2 = dot3(disp, disp);
+    if (d2 == 0.0f) continue;
+    const float dist = sqrtf(d2);
+    if (dist >= dhat) continue;
+    float den = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+      r.invm[k] = v.invm[r.gv[k]];
+      den += param[k] * param[k] * r.invm[k];
+    }
+    if (den == 0.0f) continue;
+    finish_rec(v, r, x, disp, dist);
+    emit(out, out_count, out_cap, r);
+  }
+}
+
+// ---------------------------------------------------------------- CCD (FP64)
+//
+// Bit-for-bit restatement of the oracle's double instantiation: no
+// floating-point contraction, identical operation order.
+#pragma clang fp contract(off)
+
+
+struct IBox {
+  double lo[3], hi[3];  // t, u, v
+};
+
+// std::min / std::max semantics (first argument wins ties and NaN).
+__device__ __forceinline__ double dmin(double a, double b) { return b < a ? b : a; }
+__device__ __forceinline__ double dmax(double a, double b) { return a < b ? b : a; }
+
+__device__ __forceinline__ double linf3(const double a[3], const double b[3]) {
+  double m = 0.0;
+  for (int i = 0; i < 3; ++i) m = dmax(m, fabs(b[i] - a[i]));
+  return m;
+}
+
+__device__ __forceinline__ double clamp_div(double a, double b) {
+  if (b == 0.0) return __builtin_inf();
+  return dmin(a / b, __builtin_inf());
+}
+
+__device__ void param_tolerance(bool vf, const double p0[4][3],
+                                const double p1[4][3], double dist,
+                                double tol[3]) {
+  double q[2][4][3];
+  for (int t = 0; t < 2; ++t) {
+    const double(*p)[3] = t == 0 ? p0 : p1;
+    for (int d = 0; d < 3; ++d) {
+      if (vf) {
+        q[t][0][d] = p[0][d] - p[1][d];
+        q[t][1][d] = p[0][d] - p[3][d];
+        q[t][2][d] = p[0][d] - (p[2][d] + p[3][d] - p[1][d]);
+        q[t][3][d] = p[0][d] - p[2][d];
+      } else {
+        q[t][0][d] = p[0][d] - p[2][d];
+        q[t][1][d] = p[0][d] - p[3][d];
+        q[t][2][d] = p[1][d] - p[3][d];
+        q[t][3][d] = p[1][d] - p[2][d];
+      }
+    }
+  }
+  auto m4 = [](const double* a1, const double* b1, const double* a2,
+               const double* b2, const double* a3, const double* b3,
+               const double* a4, const double* b4) {
+    return dmax(dmax(linf3(a1, b1), linf3(a2, b2)),
+                dmax(linf3(a3, b3), linf3(a4, b4)));
+  };
+  const double* p000 = q[0][0]; const double* p001 = q[0][1];
+  const double* p011 = q[0][2]; const double* p010 = q[0][3];
+  const double* p100 = q[1][0]; const double* p101 = q[1][1];
+  const double* p111 = q[1][2]; const double* p110 = q[1][3];
+  const double dl = 3 * m4(p000, p100, p001, p101, p011, p111, p010, p110);
+  const double e0 = 3 * m4(p000, p010, p100, p110, p101, p111, p001, p011);
+  const double e1 = 3 * m4(p000, p001, p100, p101, p110, p111, p010, p011);
+  tol[0] = clamp_div(dist, dl);
+  tol[1] = clamp_div(dist, e0);
+  tol[2] = clamp_div(dist, e1);
+}
+
+__device__ bool unit_eval(bool vf, const double p0[4][3], const double p1[4][3],
+                          double eps_ms, int d, bool* in_eps, double* width) {
+  double mn, mx;
+  if (vf) {
+    const double D[6] = {p0[0][d] - p0[1][d], p0[0][d] - p0[2][d],
+                         p0[0][d] - p0[3][d], p1[0][d] - p1[1][d],
+                         p1[0][d] - p1[2][d], p1[0][d] - p1[3][d]};
+    mn = D[0];
+    mx = D[0];
+    for (int i = 1; i < 6; ++i) {
+      if (D[i] < mn) mn = D[i];
+      if (mx < D[i]) mx = D[i];
+    }
+  } else {
+    const double D[8] = {p0[0][d] - p0[2][d], p0[0][d] - p0[3][d],
+                         p0[1][d] - p0[2][d], p0[1][d] - p0[3][d],
+                         p1[0][d] - p1[2][d], p1[0][d] - p1[3][d],
+                         p1[1][d] - p1[2][d], p1[1][d] - p1[3][d]};
+    mn = D[0];
+    mx = D[0];
+    for (int i = 1; i < 8; ++i) {
+      if (D[i] < mn) mn = D[i];
+      if (mx < D[i]) mx = D[i];
+    }
+  }
+  *width = mx - mn;
+  *in_eps = false;
+  if (mn > eps_ms || mx < -eps_ms) return false;
+  if (mn >= -eps_ms && mx <= eps_ms) *in_eps = true;
+  return true;
+}
+
+__device__ bool box_eval(bool vf, const IBox& b, const double k[6][3],
+                         double eps_ms, int d, bool* in_eps, double* width) {
+  const double c0 = k[0][d], c1 = k[1][d], c2 = k[2][d], c3 = k[3][d],
+               c4 = k[4][d], c5 = k[5][d];
+  const double ulo = b.lo[1], uhi = b.hi[1], vlo = b.lo[2], vhi = b.hi[2];
+  double lo_t[2], hi_t[2];
+  for (int e = 0; e < 2; ++e) {
+    const double t = e == 0 ? b.lo[0] : b.hi[0];
+    const double q0 = (c1 - c0) * t + c0;
+    const double q1 = (c3 - c2) * t + c2;
+    const double q2 = (c5 - c4) * t + c4;
+    double lo, hi;
+    if (!vf || uhi + vhi <= 1.0) {
+      const double a = q1 * ulo, bb = q1 * uhi, c = q2 * vlo, dd = q2 * vhi;
+      lo = q0 + dmin(a, bb) + dmin(c, dd);
+      hi = q0 + dmax(a, bb) + dmax(c, dd);
+    } else {
+      lo = __builtin_inf();
+      hi = -__builtin_inf();
+      auto inc = [&](double u, double v) {
+        const double val = q0 + q1 * u + q2 * v;
+        lo = dmin(lo, val);
+        hi = dmax(hi, val);
+      };
+      if (ulo + vlo <= 1.0) inc(ulo, vlo);
+      if (ulo + vhi <= 1.0) inc(ulo, vhi);
+      if (uhi + vlo <= 1.0) inc(uhi, vlo);
+      if (uhi + vhi <= 1.0) inc(uhi, vhi);
+      const double v_at_ulo = 1.0 - ulo;
+      if (v_at_ulo >= vlo && v_at_ulo <= vhi) inc(ulo, v_at_ulo);
+      const double v_at_uhi = 1.0 - uhi;
+      if (v_at_uhi >= vlo && v_at_uhi <= vhi) inc(uhi, v_at_uhi);
+      const double u_at_vlo = 1.0 - vlo;
+      if (u_at_vlo >= ulo && u_at_vlo <= uhi) inc(u_at_vlo, vlo);
+      const double u_at_vhi = 1.0 - vhi;
+      if (u_at_vhi >= ulo && u_at_vhi <= uhi) inc(u_at_vhi, vhi);
+    }
+    lo_t[e] = lo;
+    hi_t[e] = hi;
+  }
+  const double mn = dmin(lo_t[0], lo_t[1]), mx = dmax(hi_t[0], hi_t[1]);
+  *in_eps = false;
+  if (mn > eps_ms || mx < -eps_ms) return false;
+  *width = mx - mn;
+  if (mn >= -eps_ms && mx <= eps_ms) *in_eps = true;
+  return true;
+}
+
+// interval_root_finder_bucket_DFS with a bounded pending list (kCcdStackCap); the
+// pending box with the smallest t.lower is processed first, ties going to
+// the most recently pushed one.
+// Not inlined: one copy of the pending list per lane instead of one per
+// call site keeps the scratch footprint at a single 3 KB stack.
+__device__ __noinline__ bool root_find(bool vf, const double p0[4][3],
+                                       const double p1[4][3],
+                                       const double tol[3], double co_tol,
+                                       const double err[3], double ms,
+                                       int max_iter, double max_time,
+                                       double* t_lo) {
+  double k[6][3];
+  for (int d = 0; d < 3; ++d) {
+    if (vf) {
+      k[0][d] = p0[0][d] - p0[1][d];
+      k[1][d] = p1[0][d] - p1[1][d];
+      k[2][d] = p0[1][d] - p0[2][d];
+      k[3][d] = p1[1][d] - p1[2][d];
+      k[4][d] = p0[1][d] - p0[3][d];
+      k[5][d] = p1[1][d] - p1[3][d];
+    } else {
+      k[0][d] = p0[0][d] - p0[2][d];
+      k[1][d] = p1[0][d] - p1[2][d];
+      k[2][d] = p0[1][d] - p0[0][d];
+      k[3][d] = p1[1][d] - p1[0][d];
+      k[4][d] = p0[2][d] - p0[3][d];
+      k[5][d] = p1[2][d] - p1[3][d];
+    }
+  }
+  IBox stack[kCcdStackCap];
+  int sp = 0;
+  IBox cur;
+  cur.lo[0] = 0.0; cur.hi[0] = max_time;
+  cur.lo[1] = 0.0; cur.hi[1] = 1.0;
+  cur.lo[2] = 0.0; cur.hi[2] = 1.0;
+  bool has_cur = true;
+  bool unit = (max_time == 1.0);
+  int iter = 0;
+  while (has_cur || sp > 0) {
+    if (!has_cur) {
+      int best = sp - 1;
+      for (int i = sp - 2; i >= 0; --i)
+        if (stack[i].lo[0] < stack[best].lo[0]) best = i;
+      cur = stack[best];
+      for (int i = best; i < sp - 1; ++i) stack[i] = stack[i + 1];
+      --sp;
+    }
+    has_cur = false;
+    ++iter;
+    bool in_eps_all = true;
+    bool hit = true;
+    double width[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < 3; ++d) {
+      bool in_eps = false;
+      const double em = err[d] + ms;
+      const bool ok = unit ? unit_eval(vf, p0, p1, err[d] + ms, d, &in_eps, &width[d])
+                           : box_eval(vf, cur, k, em, d, &in_eps, &width[d]);
+      if (!ok) {
+        hit = false;
+        break;
+      }
+      in_eps_all = in_eps_all && in_eps;
+    }
+    unit = false;
+    if (!hit) continue;
+    const bool small = (width[0] <= co_tol && width[1] <= co_tol &&
+                        width[2] <= co_tol) || in_eps_all;
+    if (small || (max_iter > 0 && iter > max_iter)) {
+      *t_lo = cur.lo[0];
+      return true;
+    }
+    int split = 0;
+    double best = -__builtin_inf();
+    for (int d = 0; d < 3; ++d) {
+      const double w = cur.hi[d] - cur.lo[d];
+      const double s = (w > tol[d]) ? w / tol[d] : -__builtin_inf();
+      if (s > best) {
+        best = s;
+        split = d;
+      }
+    }
+    const double mid = (cur.hi[split] + cur.lo[split]) / 2;
+    if (mid < 1e-12) {
+      *t_lo = cur.lo[0];
+      return true;
+    }
+    IBox second = cur, first = cur;
+    second.lo[split] = mid;
+    first.hi[split] = mid;
+    bool push_second = true, push_first = true;
+    if (vf && split != 0) {
+      const double other = (split == 1) ? cur.lo[2] : cur.lo[1];
+      push_second = second.lo[split] + other <= 1.0;
+      push_first = first.lo[split] + other <= 1.0;
+    }
+    const int need = static_cast<int>(push_second) + static_cast<int>(push_first);
+    if (sp + need > kCcdStackCap) {
+      *t_lo = cur.lo[0];
+      return true;
+    }
+    if (push_second) stack[sp++] = second;
+    if (push_first) stack[sp++] = first;
+  }
+  return false;
+}
+
+__device__ bool ccd_query(bool vf, const double p0[4][3], const double p1[4][3],
+                          double ms, double tolerance, int max_iter,
+                          const double err[3], double* toi) {
+  double tol[3];
+  param_tolerance(vf, p0, p1, tolerance, tol);
+  double t;
+  if (!root_find(vf, p0, p1, tol, tolerance, err, ms, max_iter, 1.0, &t))
+    return false;
+  if (t > 0.0) {
+    *toi = t;
+    return true;
+  }
+  double t2;
+  if (!root_find(vf, p0, p1, tol, tolerance, err, ms * 0.5, max_iter, 1.0, &t2)) {
+    *toi = 1.0;
+    return true;
+  }
+  if (t2 > 0.0) {
+    *toi = t2;
+    return true;
+  }
+  return false;
+}
+
+__device__ void trajectory(const CollVerts& v, int gv, const CcdParams& cp,
+                           double p0[3], double p1[3]) {
+  for (int d = 0; d < 3; ++d) {
+    const double a = v.p0[3 * gv + d], b = v.p1[3 * gv + d];
+    if (gv < v.nvc) {
+      p0[d] = a;
+      p1[d] = b;
+    } else {
+      const double delta = b - a;
+      p0[d] = a + cp.time_start * delta;
+      p1[d] = p0[d] + cp.interval * delta;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_ccd_each(int n, const int* __restrict__ is_vf,
+               const double* __restrict__ p0, const double* __restrict__ p1,
+               const double* __restrict__ err, double ms, double tolerance,
+               int max_iter, int* __restrict__ hit, double* __restrict__ toi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a[4][3], b[4][3];
+  for (int k = 0; k < 12; ++k) {
+    a[k / 3][k % 3] = p0[12 * i + k];
+    b[k / 3][k % 3] = p1[12 * i + k];
+  }
+  const double e[3] = {err[3 * i], err[3 * i + 1], err[3 * i + 2]};
+  double t = 0.0;
+  const bool h = ccd_query(is_vf[i] != 0, a, b, ms, tolerance, max_iter, e, &t);
+  hit[i] = h ? 1 : 0;
+  toi[i] = h ? t : 0.0;
+}
+
+#pragma clang fp contract(fast)
+
+__global__ void __launch_bounds__(256)
+    k_ccd(CollVerts v, PrimTables pt, const int2* __restrict__ ptp,
+          const int* __restrict__ ptc, int pt_cap, const int2* __restrict__ eep,
+          const int* __restrict__ eec, int ee_cap, CcdParams cp,
+          unsigned long long* toi_bits) {
+  const int npt = min(*ptc, pt_cap);
+  const int nee = min(*eec, ee_cap);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npt + nee;
+       i += gridDim.x * blockDim.x) {
+    int gv[4];
+    bool vf;
+    if (i < npt) {
+      const int2 pr = ptp[i];
+      vf = true;
+      gv[0] = pt.point[pr.y];
+      gv[1] = pt.tri[3 * pr.x];
+      gv[2] = pt.tri[3 * pr.x + 1];
+      gv[3] = pt.tri[3 * pr.x + 2];
+    } else {
+      const int2 pr = eep[i - npt];
+      vf = false;
+      int a0 = pt.edge[2 * pr.x], a1 = pt.edge[2 * pr.x + 1];
+      int b0 = pt.edge[2 * pr.y], b1 = pt.edge[2 * pr.y + 1];
+      const int ka0 = min(a0, a1), ka1 = max(a0, a1);
+      const int kb0 = min(b0, b1), kb1 = max(b0, b1);
+      if (kb0 < ka0 || (kb0 == ka0 && kb1 < ka1)) {
+        int t0 = a0, t1 = a1;
+        a0 = b0; a1 = b1;
+        b0 = t0; b1 = t1;
+      }
+      gv[0] = a0; gv[1] = a1; gv[2] = b0; gv[3] = b1;
+    }
+    double p0[4][3], p1[4][3];
+    for (int k = 0; k < 4; ++k) trajectory(v, gv[k], cp, p0[k], p1[k]);
+    double toi;
+    if (!ccd_query(vf, p0, p1, cp.ms, cp.tolerance, cp.max_iter,
+                   vf ? cp.err_vf : cp.err_ee, &toi))
+      continue;
+    if (toi == 0.0) continue;
+    atomicMin(toi_bits, static_cast<unsigned long long>(__double_as_longlong(toi)));
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_ccd_filter(CollVerts v, double* err6) {
+  __shared__ float red[3][1024];
+  float m[3] = {0.0f, 0.0f, 0.0f};
+  for (int i = threadIdx.x; i < v.nvt; i += blockDim.x)
+    for (int d = 0; d < 3; ++d)
+      m[d] = fmaxf(m[d], fmaxf(fabsf(v.p0[3 * i + d]), fabsf(v.p1[3 * i + d])));
+  for (int d = 0; d < 3; ++d) red[d][threadIdx.x] = m[d];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int d = 0; d < 3; ++d)
+        red[d][threadIdx.x] = fmaxf(red[d][threadIdx.x], red[d][threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double vf = static_cast<double>(4.053116e-06f);
+    const double ee = static_cast<double>(3.814698e-06f);
+    for (int d = 0; d < 3; ++d) {
+      const double g = fmax(static_cast<double>(red[d][0]), 1.0);
+      err6[d] = vf * g;
+      err6[3 + d] = ee * g;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- contacts
+__global__ void __launch_bounds__(1024)
+    k_contact_update(ContactSet cs, const float* __restrict__ x, int nv) {
+  using namespace dev;
+  // q = x - eta / w for contact vertices, p = x elsewhere
+  // (contact_constraints.cu:203-214); keep p_prev for the dual residual.
+  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+    const float w = cs.w[i];
+    for (int d = 0; d < 3; ++d) {
+      const int k = 3 * i + d;
+      cs.p_prev[k] = cs.p[k];
+      cs.p[k] = w > 0.0f ? x[k] - cs.eta[k] / w : x[k];
+    }
+  }
+  __syncthreads();
+  // Warm start with the stored corrections, colour by colour
+  // (contact_constraints.cu:216-243).
+  for (int col = 0; col < cs.ncolor; ++col) {
+    for (int e = cs.color_off[col] + threadIdx.x; e < cs.color_off[col + 1];
+         e += blockDim.x) {
+      const int ci = cs.color_list[e];
+      const ContactDev& c = cs.c[ci];
+      const f3 corr = ld3(cs.corr, ci);
+      for (int s = 0; s < 4; ++s) {
+        if (c.gamma[s] == 0.0f) continue;
+        st3(cs.p, c.state[s], ld3(cs.p, c.state[s]) + c.gamma[s] * corr);
+      }
+    }
+    __syncthreads();
+  }
+  // Coloured Gauss-Seidel sweeps (contact_constraints.cu:245-297).
+  for (int sw = 0; sw < cs.sweeps; ++sw) {
+    for (int col = 0; col < cs.ncolor; ++col) {
+      for (int e = cs.color_off[col] + threadIdx.x; e < cs.color_off[col + 1];
+           e += blockDim.x) {
+        const int ci = cs.color_list[e];
+        const ContactDev& c = cs.c[ci];
+        f3 u = {c.affine[0], c.affine[1], c.affine[2]};
+        for (int s = 0; s < 4; ++s)
+          if (c.state[s] >= 0) u = u + c.coef[s] * ld3(cs.p, c.state[s]);
+        const f3 old = ld3(cs.corr, ci);
+        const f3 us = u - old;
+        const f3 n = {c.normal[0], c.normal[1], c.normal[2]};
+        const float un = dot3(us, n);
+        f3 unew;
+        if (un >= 0.0f) {
+          unew = us;
+        } else {
+          const f3 ut = us - un * n;
+          const float utn = sqrtf(dot3(ut, ut));
+          const float rad = -c.mu * un;
+          unew = (utn <= rad) ? f3{0.0f, 0.0f, 0.0f} : (1.0f - rad / utn) * ut;
+        }
+        const f3 du = unew - u;
+        st3(cs.corr, ci, old + du);
+        for (int s = 0; s < 4; ++s) {
+          if (c.gamma[s] == 0.0f) continue;
+          st3(cs.p, c.state[s], ld3(cs.p, c.state[s]) + c.gamma[s] * du);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // Multiplier update and dual residual terms (contact_constraints.cu:316-364).
+  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+    const float w = cs.w[i];
+    if (!(w > 0.0f)) continue;
+    for (int d = 0; d < 3; ++d) {
+      const int k = 3 * i + d;
+      cs.eta[k] += w * (cs.p[k] - x[k]);
+      cs.dual[k] = w * (cs.p[k] - cs.p_prev[k]);
+    }
+  }
+}
+
+}  // namespace
+
+void launch_refit(hipStream_t s, const CollVerts& v, const TreeDev* trees_dev,
+                  int ntrees) {
+  if (ntrees == 0) return;
+  k_refit<<<ntrees, kRefitThreads, 0, s>>>(v, trees_dev);
+}
+
+void launch_traverse(hipStream_t s, const CollVerts& v, const TreeDev& tree,
+                     const QuerySet& q, int kind, int self, const Box6* gate_a,
+                     const Box6* gate_b, PairOut out) {
+  if (q.n == 0 || tree.nprim == 0) return;
+  k_traverse<<<(q.n + 255) / 256, 256, 0, s>>>(v, tree, q, kind, self, gate_a,
+                                               gate_b, out);
+}
+
+void launch_dcd(hipStream_t s, const CollVerts& v, const PrimTables& pt,
+                const int2* pt_pairs, const int* pt_count, int pt_cap,
+                const int2* ee_pairs, const int* ee_count, int ee_cap,
+                float time, float dhat, DcdRec* out, int* out_count,
+                int out_cap) {
+  k_dcd<<<1024, 256, 0, s>>>(v, pt, pt_pairs, pt_count, pt_cap, ee_pairs,
+                             ee_count, ee_cap, time, dhat, out, out_count,
+                             out_cap);
+}
+
+void launch_ccd_filter(hipStream_t s, const CollVerts& v, double* err6) {
+  k_ccd_filter<<<1, 1024, 0, s>>>(v, err6);
+}
+
+void launch_ccd(hipStream_t s, const CollVerts& v, const PrimTables& pt,
+                const int2* pt_pairs, const int* pt_count, int pt_cap,
+                const int2* ee_pairs, const int* ee_count, int ee_cap,
+                const CcdParams& cp, unsigned long long* toi_bits) {
+  k_ccd<<<1024, 256, 0, s>>>(v, pt, pt_pairs, pt_count, pt_cap, ee_pairs,
+                             ee_count, ee_cap, cp, toi_bits);
+}
+
+void launch_ccd_each(hipStream_t s, int n, const int* is_vf, const double* p0,
+                     const double* p1, const double* err, double ms,
+                     double tolerance, int max_iter, int* hit, double* toi) {
+  if (n <= 0) return;
+  k_ccd_each<<<(n + 255) / 256, 256, 0, s>>>(n, is_vf, p0, p1, err, ms,
+                                             tolerance, max_iter, hit, toi);
+}
+
+void launch_contact_update(hipStream_t s, const ContactSet& cs,
+                           const float* x) {
+  if (cs.n == 0) return;
+  k_contact_update<<<1, 1024, 0, s>>>(cs, x, cs.nv);
+}
+
+}  // namespace silk::amd

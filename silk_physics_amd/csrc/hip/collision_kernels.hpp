@@ -1,0 +1,162 @@
+// Collision kernels: collider boxes, implicit BVH refit and traversal, DCD,
+// FP64 CCD and the Daviet contact Gauss-Seidel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace silk::amd {
+
+struct Box6 {
+  float mn[3], mx[3];
+};
+
+/// Global collider vertex numbering: cloth vertices [0, nvc) in solver order,
+/// then obstacle vertices.  p0/p1 hold the segment endpoints.
+struct CollVerts {
+  int nvc = 0, nvt = 0;
+  const float* p0 = nullptr;
+  const float* p1 = nullptr;
+  const float* invm = nullptr;  // 0 for pinned / obstacle
+};
+
+/// One implicit BVH over a primitive set (triangles or edges of one object):
+/// primitives sorted once by Morton code, leaves of kLeafPrims consecutive
+/// sorted primitives, complete binary heap (root 1, leaves [P, 2P)).
+constexpr int kLeafPrims = 4;
+struct TreeDev {
+  int nprim = 0;
+  int nvert = 0;           // vertices per primitive (2 or 3)
+  int P = 1;               // padded leaf count (power of two)
+  float pad = 0.0f;
+  const int* prim = nullptr;   // nvert * nprim global vertex ids (sorted order)
+  const int* prim_id = nullptr;// original primitive id per sorted slot
+  Box6* sbox = nullptr;        // nprim, sorted order
+  Box6* node = nullptr;        // 2P
+};
+
+/// Refit all trees: one workgroup per tree.
+void launch_refit(hipStream_t s, const CollVerts& v, const TreeDev* trees_dev,
+                  int ntrees);
+
+struct QuerySet {
+  int n = 0;
+  int nvert = 1;                 // 1 point, 2 edge
+  const int* prim = nullptr;     // nvert * n global vertex ids
+  float pad = 0.0f;
+  int gid0 = 0;                  // global primitive id of query 0
+};
+
+/// Pair output: (tree primitive global id, query primitive global id) plus a
+/// running counter; entries beyond cap are counted but not written.
+struct PairOut {
+  int2* pairs = nullptr;
+  int* count = nullptr;
+  int cap = 0;
+};
+
+/// kind 0: triangle tree x points (PT), 1: edge tree x edges (EE).
+/// self: same object (adjacency filter; EE keeps tree id < query id only).
+/// gate_a / gate_b: object boxes (triangle tree roots); when both are given
+/// and do not overlap the launch reports nothing (find_collision.cu:45-92).
+void launch_traverse(hipStream_t s, const CollVerts& v, const TreeDev& tree,
+                     const QuerySet& q, int kind, int self, const Box6* gate_a,
+                     const Box6* gate_b, PairOut out);
+
+/// DCD record of one active contact (dcd.cu:102-261), global vertex ids.
+struct DcdRec {
+  int type;     // 0 PT, 1 EE
+  int gv[4];
+  float coef[4];
+  float invm[4];
+  float normal[3];
+  float x[4][3];
+  float presc[3];
+  float friction;   // sqrt(mu_a mu_b), rounded once from double
+};
+
+struct PrimTables {
+  const int* tri = nullptr;    // 3 per triangle (global vertex ids)
+  const int* edge = nullptr;   // 2 per edge
+  const int* point = nullptr;  // 1 per point
+  const int* tri_obj = nullptr;
+  const int* edge_obj = nullptr;
+  const int* point_obj = nullptr;
+  const float* obj_friction = nullptr;
+};
+
+/// DCD over PT and EE candidate lists at fixed normalized time.
+void launch_dcd(hipStream_t s, const CollVerts& v, const PrimTables& pt,
+                const int2* pt_pairs, const int* pt_count, int pt_cap,
+                const int2* ee_pairs, const int* ee_count, int ee_cap,
+                float time, float dhat, DcdRec* out, int* out_count,
+                int out_cap);
+
+struct CcdParams {
+  double ms = 0.0;
+  double tolerance = 1e-6;   // already divided by the segment length
+  int max_iter = 4096;
+  double time_start = 0.0;
+  double interval = 1.0;
+  const double* err_vf = nullptr;  // device, 3 doubles
+  const double* err_ee = nullptr;
+};
+
+/// Per-axis max |coordinate| over all collider vertices at t0 and t1, then the
+/// VF / EE numerical filters (interval_root_finder.cpp:512-522) into err[6].
+void launch_ccd_filter(hipStream_t s, const CollVerts& v, double* err6);
+
+/// Earliest TOI over the candidates (local units, written as the bit pattern
+/// of a positive double into *toi_bits via atomicMin; initialise to 1.0).
+void launch_ccd(hipStream_t s, const CollVerts& v, const PrimTables& pt,
+                const int2* pt_pairs, const int* pt_count, int pt_cap,
+                const int2* ee_pairs, const int* ee_count, int ee_cap,
+                const CcdParams& cp, unsigned long long* toi_bits);
+
+/// Pending-box capacity of one CCD query (overflow returns the current box's
+/// t.lower: an earlier, conservative TOI).
+constexpr int kCcdStackCap = 64;
+
+/// Per-query CCD on explicit FP64 end positions (12 doubles each for p0 and
+/// p1, err 3 doubles per query): hit[i] = 1 and toi[i] as the step's CCD
+/// computes them (toi 0 reported as a hit with toi 0).
+void launch_ccd_each(hipStream_t s, int n, const int* is_vf, const double* p0,
+                     const double* p1, const double* err, double ms,
+                     double tolerance, int max_iter, int* hit, double* toi);
+
+// ---------------------------------------------------------------- contacts
+
+/// Daviet contact (contact_constraints.cuh:19-42).
+struct ContactDev {
+  int state[4];     // cloth vertex (solver order, global) or -1
+  float coef[4];
+  float gamma[4];
+  float normal[3];
+  float affine[3];
+  float mu;
+  float pad;
+};
+
+struct ContactSet {
+  int n = 0;
+  int ncolor = 0;
+  int nv = 0;                       // cloth vertices
+  const ContactDev* c = nullptr;
+  const int* color_off = nullptr;   // ncolor + 1
+  const int* color_list = nullptr;  // contact ids grouped by colour
+  float* corr = nullptr;            // 3 per contact
+  const float* w = nullptr;         // nv, active weight (0 = inactive)
+  float* p = nullptr;               // 3 nv
+  float* p_prev = nullptr;
+  float* eta = nullptr;
+  float* dual = nullptr;            // 3 nv, w (p - p_prev)
+  int sweeps = 10;
+};
+
+/// One ADMM contact update (contact_constraints.cu:246-264): single
+/// workgroup, colours separated by workgroup barriers.
+void launch_contact_update(hipStream_t s, const ContactSet& cs,
+                           const float* x);
+
+}  // namespace silk::amd
