@@ -135,6 +135,12 @@ class OracleWorld:
         return lib().oracle_add_obstacle(self._w, cop, vp, v.size // 3, fp,
                                          f.size // 3)
 
+    def set_state(self, idx: int, x, v):
+        """Overwrite cloth `idx`'s position and velocity (caller order)."""
+        a, ap = _d(np.asarray(x).reshape(-1))
+        b, bp = _d(np.asarray(v).reshape(-1))
+        lib().oracle_set_state(self._w, idx, ap, bp)
+
     def set_obstacle_position(self, idx: int, V):
         v, vp = _f(V)
         lib().oracle_set_obstacle_position(self._w, idx, vp)

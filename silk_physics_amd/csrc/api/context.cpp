@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include <numeric>
 
 #include "api/collision.hpp"
@@ -297,6 +298,11 @@ void Context::build_scene() {
   S.ncnode = ncn;
   std::vector<int> cnode_ptr(ncn + 1, 0), cnode_level(ncn, 0),
       row_cnode(4 * static_cast<size_t>(nv), -1);
+  // Persistent-PCG tables, slot order (banks padded to 32 slots).
+  std::vector<PcgSysDev> ps_sys;
+  std::vector<int> ps_slot_row, ps_slot_cn, ps_cb_bank, ps_cb_member,
+      ps_cn_ptr{0}, ps_cn_list, ps_slot_of_row(nv, -1);
+  int ps_wg = 0;
   std::vector<std::vector<int>> cmembers(ncn);
   for (int ci = 0; ci < S.nsys; ++ci) {
     const SysMas& sm = smas[ci];
@@ -324,6 +330,7 @@ void Context::build_scene() {
       }
     }
     // banks
+    const int bank_start = static_cast<int>(bank_sys.size());
     std::vector<double> stat = mas_static_blocks(t, Ks[ci]);
     size_t sb = 0;
     for (int l = 0; l <= t.coarse_levels; ++l) {
@@ -345,6 +352,108 @@ void Context::build_scene() {
         }
         for (int e = 0; e < 1024; ++e)
           bank_static.push_back(static_cast<float>(stat[sb * 1024 + e]));
+      }
+    }
+    // Persistent-PCG view of this system: slots, system-local coarse ids
+    // (level 1 first, then deeper levels), coarse banks and, per coarse
+    // node, the level-1 nodes whose residual sums make up its own.
+    {
+      const auto& po = cloths_[ci].part.offsets;
+      const int nb0 = t.level_banks[0];
+      PcgSysDev sd;
+      sd.slot0 = static_cast<int>(ps_slot_row.size());
+      sd.nslot = kBankSlots * nb0;
+      sd.bank0 = bank_start;
+      sd.cb0 = static_cast<int>(ps_cb_bank.size());
+      sd.cn0 = static_cast<int>(ps_cn_ptr.size()) - 1;
+      std::vector<int> lvl_off(t.coarse_levels + 2, 0);
+      for (int l = 1; l <= t.coarse_levels; ++l)
+        lvl_off[l + 1] = lvl_off[l] + t.level_nodes[l];
+      sd.ncn = lvl_off[t.coarse_levels + 1];
+      for (int bk = 0; bk < nb0; ++bk)
+        for (int s = 0; s < kBankSlots; ++s) {
+          const int lr = po[bk] + s;
+          const bool real = lr < po[bk + 1];
+          const int slot = static_cast<int>(ps_slot_row.size());
+          ps_slot_row.push_back(real ? vo + lr : -1);
+          if (real) ps_slot_of_row[vo + lr] = slot;
+          for (int l = 1; l <= 4; ++l)
+            ps_slot_cn.push_back(real && l <= t.coarse_levels
+                                     ? lvl_off[l] + t.cmap[l - 1][lr]
+                                     : -1);
+        }
+      int bidx = bank_start + nb0;
+      for (int l = 1; l <= t.coarse_levels; ++l)
+        for (int bk = 0; bk < t.level_banks[l]; ++bk, ++bidx) {
+          ps_cb_bank.push_back(bidx);
+          for (int s = 0; s < kBankSlots; ++s) {
+            const int id = bk * kBankSlots + s;
+            ps_cb_member.push_back(id < t.level_nodes[l] ? lvl_off[l] + id : -1);
+          }
+        }
+      sd.ncb = static_cast<int>(ps_cb_bank.size()) - sd.cb0;
+      std::vector<std::vector<int>> desc(sd.ncn);
+      for (int lr = 0; lr < t.n; ++lr)
+        for (int l = 1; l <= t.coarse_levels; ++l)
+          desc[lvl_off[l] + t.cmap[l - 1][lr]].push_back(lvl_off[1] + t.cmap[0][lr]);
+      for (auto& dsc : desc) {
+        std::sort(dsc.begin(), dsc.end());
+        dsc.erase(std::unique(dsc.begin(), dsc.end()), dsc.end());
+        ps_cn_list.insert(ps_cn_list.end(), dsc.begin(), dsc.end());
+        ps_cn_ptr.push_back(static_cast<int>(ps_cn_list.size()));
+      }
+      const int per_wg = kPcgThreads / kBankSlots;
+      sd.nwg = std::max(1, (nb0 + per_wg - 1) / per_wg);
+      sd.wg0 = ps_wg;
+      ps_wg += sd.nwg;
+      ps_sys.push_back(sd);
+    }
+  }
+  // Slot-ordered sliced ELL of the off-diagonal operator (64-slot slices).
+  std::vector<int> ps_sell_off, ps_sell_width, ps_sell_col;
+  std::vector<float> ps_sell_val;
+  {
+    const int nslot = static_cast<int>(ps_slot_row.size());
+    std::vector<int> row_sys(nv, 0);
+    for (int ci = 0; ci < S.nsys; ++ci)
+      for (int r = S.vert_off[ci]; r < S.vert_off[ci + 1]; ++r) row_sys[r] = ci;
+    auto offdiag = [&](int row) {
+      const CsrMatrix& K = Ks[row_sys[row]];
+      const int lr = row - S.vert_off[row_sys[row]];
+      int cnt = 0;
+      for (int a = K.row_ptr[lr]; a < K.row_ptr[lr + 1]; ++a) cnt += K.col[a] != lr;
+      return cnt;
+    };
+    for (int s0 = 0; s0 < nslot; s0 += kSlice) {
+      int width = 0;
+      for (int s = s0; s < std::min(nslot, s0 + kSlice); ++s)
+        if (ps_slot_row[s] >= 0) width = std::max(width, offdiag(ps_slot_row[s]));
+      ps_sell_off.push_back(static_cast<int>(ps_sell_col.size()));
+      ps_sell_width.push_back(width);
+      const size_t base = ps_sell_col.size();
+      ps_sell_col.resize(base + static_cast<size_t>(width) * kSlice);
+      ps_sell_val.resize(base + static_cast<size_t>(width) * kSlice);
+      for (int lane = 0; lane < kSlice; ++lane) {
+        const int s = s0 + lane;
+        const int row = s < nslot ? ps_slot_row[s] : -1;
+        int kk = 0;
+        if (row >= 0) {
+          const int ci = row_sys[row];
+          const CsrMatrix& K = Ks[ci];
+          const int lr = row - S.vert_off[ci];
+          for (int a = K.row_ptr[lr]; a < K.row_ptr[lr + 1]; ++a) {
+            if (K.col[a] == lr) continue;
+            const size_t e = base + static_cast<size_t>(kk) * kSlice + lane;
+            ps_sell_col[e] = ps_slot_of_row[S.vert_off[ci] + K.col[a]];
+            ps_sell_val[e] = static_cast<float>(K.val[a]);
+            ++kk;
+          }
+        }
+        for (; kk < width; ++kk) {
+          const size_t e = base + static_cast<size_t>(kk) * kSlice + lane;
+          ps_sell_col[e] = s < nslot ? s : 0;
+          ps_sell_val[e] = 0.0f;
+        }
       }
     }
   }
@@ -425,6 +534,76 @@ void Context::build_scene() {
   S.part.resize(2 * static_cast<size_t>(std::max(1, S.nblk)));
   S.part2.resize(2 * static_cast<size_t>(std::max(1, S.nblk)));
   S.state.resize(2 * static_cast<size_t>(std::max(1, S.nsys)));
+  // ---- persistent PCG
+  {
+    auto up = [&](DevBuf<int>& b, std::vector<int> v) {
+      if (v.empty()) v.push_back(0);
+      b.upload(v, st);
+    };
+    const int nslot = static_cast<int>(ps_slot_row.size());
+    int max_ncb = 0, max_ncn = 0;
+    for (const auto& sd : ps_sys) {
+      max_ncb = std::max(max_ncb, sd.ncb);
+      max_ncn = std::max(max_ncn, sd.ncn);
+    }
+    S.ps_sys.upload(ps_sys, st);
+    up(S.ps_slot_row, ps_slot_row);
+    up(S.ps_sell_off, ps_sell_off);
+    up(S.ps_sell_width, ps_sell_width);
+    up(S.ps_sell_col, ps_sell_col);
+    if (ps_sell_val.empty()) ps_sell_val.push_back(0.0f);
+    S.ps_sell_val.upload(ps_sell_val, st);
+    up(S.ps_slot_cn, ps_slot_cn);
+    up(S.ps_cb_bank, ps_cb_bank);
+    up(S.ps_cb_member, ps_cb_member);
+    up(S.ps_cn_ptr, ps_cn_ptr);
+    up(S.ps_cn_list, ps_cn_list);
+    const int ncn_all = static_cast<int>(ps_cn_ptr.size()) - 1;
+    S.ps_pbuf.resize(6 * static_cast<size_t>(std::max(1, nslot)));
+    S.ps_zbuf.resize(3 * static_cast<size_t>(std::max(1, nslot)));
+    S.ps_cr.resize(3 * static_cast<size_t>(std::max(1, ncn_all)));
+    S.ps_cz.resize(3 * static_cast<size_t>(std::max(1, ncn_all)));
+    S.ps_part.resize(4 * static_cast<size_t>(std::max(1, ps_wg)));
+    S.ps_bar.resize(kPcgBarrierStride * static_cast<size_t>(std::max(1, S.nsys)));
+    S.ps_bar.zero(st);
+    S.ps_abort.resize(1);
+    S.ps_abort.zero(st);
+    S.ps_out.resize(std::max(1, S.nsys));
+    PersistDev& a = S.pers;
+    a.nsys = S.nsys;
+    a.nslot_total = nslot;
+    a.sys = S.ps_sys.get();
+    a.slot_row = S.ps_slot_row.get();
+    a.sell_off = S.ps_sell_off.get();
+    a.sell_width = S.ps_sell_width.get();
+    a.sell_col = S.ps_sell_col.get();
+    a.sell_val = S.ps_sell_val.get();
+    a.slot_cn = S.ps_slot_cn.get();
+    a.cb_bank = S.ps_cb_bank.get();
+    a.cb_member = S.ps_cb_member.get();
+    a.cn_ptr = S.ps_cn_ptr.get();
+    a.cn_list = S.ps_cn_list.get();
+    a.bank_inv = S.bank_inv.get();
+    a.d = S.d.get();
+    a.r0 = S.r.get();
+    a.x = S.inner_x.get();
+    a.pbuf = S.ps_pbuf.get();
+    a.zbuf = S.ps_zbuf.get();
+    a.cr = S.ps_cr.get();
+    a.cz = S.ps_cz.get();
+    a.part = S.ps_part.get();
+    a.bar = S.ps_bar.get();
+    a.abort_flag = S.ps_abort.get();
+    a.out = S.ps_out.get();
+    a.abs_tol = global_.linear_solver_abs_tol;
+    a.max_iter = pcg_max_iter();
+    a.coarse_lds = (max_ncb <= 64 && max_ncn <= 4096) ? 1 : 0;
+    a.lds_nodes = a.coarse_lds ? std::max(1, max_ncn) : 0;
+    a.grid = ps_wg;
+    const char* mode = std::getenv("SILK_HIP_PCG");
+    const bool want = !(mode && std::strcmp(mode, "launch") == 0);
+    S.use_persistent = want && ps_wg > 0 && pcg_persistent_fits(ps_wg, a.lds_nodes);
+  }
   S.coll = make_collision_world(*this, S);
   hip_check(hipStreamSynchronize(st), "scene upload");
   scene_ = std::move(sc);
@@ -501,6 +680,43 @@ int Context::pcg_solve(float rel_tol, bool factor, const RhsArgs& ra,
   PcgDev p = pcg_dev();
   launch_pcg_residual(stream_, p, ra);
   if (factor) launch_mas_factor(stream_, p, S.cd.get());
+  if (S.use_persistent) {
+    PersistDev a = S.pers;
+    a.abs_tol = global_.linear_solver_abs_tol;
+    a.max_iter = pcg_max_iter();
+    hip_check(launch_pcg_persistent(stream_, a, rel_tol), "persistent PCG launch");
+    S.h_state.resize(S.nsys);
+    int abort_flag = 0;
+    hip_check(hipMemcpyAsync(S.h_state.data(), S.ps_out.get(),
+                             S.nsys * sizeof(PcgState), hipMemcpyDeviceToHost,
+                             stream_),
+              "pcg state");
+    hip_check(hipMemcpyAsync(&abort_flag, S.ps_abort.get(), sizeof(int),
+                             hipMemcpyDeviceToHost, stream_),
+              "pcg abort flag");
+    hip_check(hipStreamSynchronize(stream_), "pcg sync");
+    if (abort_flag != 0) {
+      // A barrier wait gave up; clear the barrier words for the next launch.
+      S.ps_bar.zero(stream_);
+      S.ps_abort.zero(stream_);
+      hip_check(hipStreamSynchronize(stream_), "pcg barrier reset");
+      throw HipError("persistent PCG: grid barrier timed out");
+    }
+  } else {
+    pcg_iterate(rel_tol);
+  }
+  int iters = 0;
+  double rel = 0.0;
+  for (const auto& s : S.h_state) {
+    iters += s.iter;
+    if (s.rr0 > 0) rel = std::max(rel, std::sqrt(s.rr / s.rr0));
+  }
+  return finish_pcg(iters, rel, ra, stats);
+}
+
+void Context::pcg_iterate(float rel_tol) {
+  Scene& S = *scene_;
+  PcgDev p = pcg_dev();
   launch_pcg_start(stream_, p);
   int k = 0;
   int chunk = 8;
@@ -520,12 +736,12 @@ int Context::pcg_solve(float rel_tol, bool factor, const RhsArgs& ra,
     chunk = 4;
   }
   launch_pcg_commit(stream_, p);
-  int iters = 0;
-  double rel = 0.0;
-  for (const auto& s : S.h_state) {
-    iters += s.iter;
-    if (s.rr0 > 0) rel = std::max(rel, std::sqrt(s.rr / s.rr0));
-  }
+}
+
+int Context::finish_pcg(int iters, double rel, const RhsArgs& ra,
+                        silk_hip_step_stats* stats) {
+  Scene& S = *scene_;
+  PcgDev p = pcg_dev();
   if (stats) {
     stats->pcg_iterations += iters;
     stats->pcg_solves += 1;

@@ -84,6 +84,16 @@ struct Scene {
   int nblk = 0, nbank = 0, ncnode = 0, n1node = 0;
   std::vector<int> h_blk0;  // sys_blk0 on host
   std::vector<PcgState> h_state;
+  // ---- persistent PCG (one cooperative launch per solve)
+  bool use_persistent = false;
+  PersistDev pers;  // device pointers into the buffers below
+  DevBuf<PcgSysDev> ps_sys;
+  DevBuf<int> ps_slot_row, ps_sell_off, ps_sell_width, ps_sell_col, ps_slot_cn,
+      ps_cb_bank, ps_cb_member, ps_cn_ptr, ps_cn_list, ps_abort;
+  DevBuf<float> ps_sell_val, ps_pbuf, ps_zbuf, ps_cr, ps_cz;
+  DevBuf<double> ps_part;
+  DevBuf<unsigned> ps_bar;
+  DevBuf<PcgState> ps_out;
   // ---- collision
   std::unique_ptr<CollisionWorld> coll;
 };
@@ -121,6 +131,11 @@ class Context {
   int step(silk_hip_step_stats* stats);
   int pcg_solve(float rel_tol, bool factor, const RhsArgs& ra,
                 silk_hip_step_stats* st);
+  /// Multi-launch PCG iterations (used when the persistent kernel cannot
+  /// keep its grid resident); leaves per-system states in scene h_state.
+  void pcg_iterate(float rel_tol);
+  int finish_pcg(int iters, double rel, const RhsArgs& ra,
+                 silk_hip_step_stats* st);
   bool admm_solve(silk_hip_step_stats* st);
   void get_positions(const ClothEntity& c, float* out) const;
   /// Push one cloth's pin targets to the device scene (if built).

@@ -183,4 +183,67 @@ void launch_pcg_iters(hipStream_t s, const PcgDev& p, int k, int count,
 /// Apply the MAS preconditioner once to p.r -> p.z (tests).
 void launch_mas_apply(hipStream_t s, const PcgDev& p);
 
+// ------------------------------------------------- persistent PCG solve
+//
+// One cooperative launch runs a whole PCG solve (all iterations) for every
+// system.  Each system owns a group of workgroups; a workgroup owns up to
+// eight 32-slot level-0 MAS banks and keeps the rows of those banks (r, dx,
+// z, p, q) in registers for the whole solve.  Only z and p travel through
+// memory, for the neighbour reads of the SpMV, in "slot" order (banks padded
+// to 32 slots).  Workgroups of a group meet at three grid barriers per
+// iteration (p.Ap, level-1 coarse sums, r.z / r.r); the coarse levels are
+// solved redundantly by every workgroup of the group when they are small.
+
+constexpr int kPcgThreads = 256;   // threads per workgroup (8 banks x 32)
+constexpr int kPcgBarrierStride = 64;  // unsigned words per system barrier
+
+struct PcgSysDev {
+  int slot0 = 0, nslot = 0;  // padded slots (32 per level-0 bank)
+  int bank0 = 0;             // global bank id of the first level-0 bank
+  int cb0 = 0, ncb = 0;      // coarse banks in the persistent coarse list
+  int cn0 = 0, ncn = 0;      // coarse nodes (system-local ids cn0 + local)
+  int wg0 = 0, nwg = 0;      // workgroups of the group
+};
+
+struct PersistDev {
+  int nsys = 0;
+  int nslot_total = 0;              // slots of all systems
+  const PcgSysDev* sys = nullptr;
+  const int* slot_row = nullptr;    // global row of a slot, -1 for padding
+  const int* sell_off = nullptr;    // slot-ELL: 64-slot slices
+  const int* sell_width = nullptr;
+  const int* sell_col = nullptr;    // neighbour slot
+  const float* sell_val = nullptr;
+  const int* slot_cn = nullptr;     // 4 per slot: coarse node (cn0-relative) per level
+  const int* cb_bank = nullptr;     // per coarse bank: global bank id (bank_inv)
+  const int* cb_member = nullptr;   // 32 per coarse bank: node (cn0-relative), -1
+  const int* cn_ptr = nullptr;      // per global coarse node: level-1 nodes summed
+  const int* cn_list = nullptr;     //   (cn0-relative ids)
+  const float* bank_inv = nullptr;  // 1024 per bank
+  const float* d = nullptr;         // diag per row
+  const float* r0 = nullptr;        // FP32 initial residual per row
+  float* x = nullptr;               // x += dx at the end (rows)
+  float* pbuf = nullptr;            // 2 x 3 per slot
+  float* zbuf = nullptr;            // 3 per slot
+  float* cr = nullptr;              // 3 per coarse node: level-1 sums
+  float* cz = nullptr;              // 3 per coarse node (distributed mode)
+  double* part = nullptr;           // 4 per workgroup
+  unsigned* bar = nullptr;          // kPcgBarrierStride per system
+  int* abort_flag = nullptr;
+  PcgState* out = nullptr;          // per system
+  float abs_tol = 0.0f;
+  int max_iter = 1000;
+  int coarse_lds = 0;   // 1: coarse levels solved per workgroup in LDS
+  int lds_nodes = 0;    // coarse nodes held in LDS (max over systems)
+  int grid = 0;         // total workgroups
+};
+
+/// True when the device can keep `grid` workgroups of the persistent kernel
+/// resident at once with `lds_nodes` coarse nodes in LDS.
+bool pcg_persistent_fits(int grid, int lds_nodes);
+
+/// One whole PCG solve (mas_cg_solver.cu:193-311) per system, x += solution.
+hipError_t launch_pcg_persistent(hipStream_t s, const PersistDev& a,
+                                 float rel_tol);
+
 }  // namespace silk::amd

@@ -1,0 +1,59 @@
+"""Helpers shared by the test modules (not collected by pytest)."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle as O
+import silk_physics_amd as sp
+from silk_physics_amd import scenes
+
+
+def f32(x: float) -> float:
+    """Round a Python float through FP32, as the library stores its inputs."""
+    return float(np.float32(x))
+
+
+def gpu_world(**global_kw) -> "sp.World":
+    w = sp.World()
+    r = w.init(sp.Backend.GPU)
+    assert r, r.to_string()
+    if global_kw:
+        r = w.set_global_config(sp.GlobalConfig(**global_kw))
+        assert r, r.to_string()
+    return w
+
+
+def oracle_for(global_kw: dict) -> dict:
+    """Oracle global parameters matching GlobalConfig keyword arguments (FP32
+    rounded like the library's copies)."""
+    g = sp.GlobalConfig(**global_kw)
+    return dict(
+        ax=f32(g.acceleration_x), ay=f32(g.acceleration_y), az=f32(g.acceleration_z),
+        max_outer=g.max_outer_iteration, max_inner=g.max_inner_iteration,
+        dt=f32(g.dt), lin_abs=f32(g.linear_solver_abs_tol),
+        lin_rel_min=f32(g.linear_solver_rel_tol_min),
+        lin_rel_max=f32(g.linear_solver_rel_tol_max),
+        lin_rel_init=f32(g.initial_linear_rel_tol), admm_abs=f32(g.admm_abs_tol),
+        admm_rel=f32(g.admm_rel_tol), ms_scale=f32(0.2), ccd_tol=f32(1e-6))
+
+
+def coll_tuple(c: "sp.CollisionConfig") -> tuple:
+    return (int(c.is_collision_on), int(c.is_self_collision_on), c.group,
+            c.restitution, c.friction)
+
+
+def sheet_on_sphere(n: int = 16, subdiv: int = 2, z: float = 0.4,
+                    radius: float = 0.3):
+    V, F = scenes.grid_sheet(n, z=z)
+    SV, SF = scenes.icosphere(subdiv, radius, (0.5, 0.5, 0.0))
+    return V, F, SV, SF
+
+
+def inscribed_radius(SV: np.ndarray, SF: np.ndarray,
+                     center=(0.5, 0.5, 0.0)) -> float:
+    """Smallest distance from the centre to a face plane of a convex mesh."""
+    c = np.asarray(center, dtype=np.float64)
+    a, b, d = (SV[SF[:, k]].astype(np.float64) for k in range(3))
+    n = np.cross(b - a, d - a)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    return float(np.abs(np.einsum("ij,ij->i", n, a - c)).min())

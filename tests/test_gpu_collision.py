@@ -1,0 +1,169 @@
+"""GPU collision path against the oracle: the FP64 CCD kernel bit for bit,
+broadphase candidate sets exactly, a sheet falling on a sphere step by step
+(lockstep from the same state) and free-running without penetration."""
+import numpy as np
+import pytest
+
+import oracle as O
+import silk_physics_amd as sp
+
+from helpers import coll_tuple, gpu_world, inscribed_radius, oracle_for, sheet_on_sphere
+from test_oracle_ccd import MS, TOL, err_for, point_triangle_distance
+
+pytestmark = pytest.mark.gpu
+
+
+def _f32(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
+def _queries(n, seed):
+    rng = np.random.default_rng(seed)
+    vf = rng.integers(0, 2, n).astype(np.int32)
+    p0 = _f32(rng.uniform(-1, 1, (n, 12)))
+    mv = _f32(rng.uniform(-1, 1, (n, 12)) * rng.choice([0.01, 0.3, 1.0], (n, 1)))
+    p1 = _f32(p0 + mv)
+    for i in range(0, n, 2):  # every other query touches at some t* in (0, 1)
+        ts = rng.uniform(0.05, 0.95)
+        x = rng.uniform(-1, 1, (4, 3))
+        if vf[i]:
+            b = rng.dirichlet([1, 1, 1])
+            x[0] = b[0] * x[1] + b[1] * x[2] + b[2] * x[3]
+        else:
+            u = rng.uniform(0.1, 0.9)
+            c = (1 - u) * x[0] + u * x[1]
+            wv = rng.uniform(-0.5, 0.5, 3)
+            x[2] = c + wv
+            x[3] = c - rng.uniform(0.2, 2.0) * wv
+        v = rng.uniform(-1, 1, (4, 3)) * rng.choice([0.05, 0.5])
+        p0[i] = _f32((x - ts * v).reshape(-1))
+        p1[i] = _f32((x + (1 - ts) * v).reshape(-1))
+    err = np.array([err_for(p0[i].reshape(4, 3), p1[i].reshape(4, 3), bool(vf[i]))
+                    for i in range(n)])
+    return vf, p0, p1, err
+
+
+@pytest.mark.parametrize("ms", [MS, 1e-5])
+def test_ccd_kernel_bit_exact_vs_oracle(ms):
+    w = gpu_world()
+    vf, p0, p1, err = _queries(4000, seed=int(ms * 1e6) + 1)
+    hit, toi = w.debug_ccd(vf, p0, p1, err, ms, TOL)
+    ohit = np.zeros_like(hit)
+    otoi = np.zeros_like(toi)
+    for i in range(vf.size):
+        h, t = O.ccd(bool(vf[i]), p0[i].reshape(4, 3), p1[i].reshape(4, 3), ms, TOL,
+                     err[i], stack_cap=64)
+        ohit[i], otoi[i] = h, (t if h else 0.0)
+    print(f"CCD ms={ms}: {int(hit.sum())} hits of {vf.size}, oracle {int(ohit.sum())}")
+    assert ohit.sum() > vf.size // 4
+    np.testing.assert_array_equal(hit, ohit)
+    np.testing.assert_array_equal(toi, otoi)  # bit for bit
+
+
+def test_ccd_kernel_closed_forms():
+    w = gpu_world()
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], dtype=np.float64)
+    vf0 = np.vstack([[0.2, 0.2, 1.0], tri])
+    vf1 = np.vstack([[0.2, 0.2, -1.0], tri])
+    ee0 = np.array([[-1, 0, 1.0], [1, 0, 1.0], [0, -1, 0], [0, 1, 0]], dtype=np.float64)
+    ee1 = ee0.copy()
+    ee1[:2, 2] = -1.0
+    miss0 = vf0.copy()
+    miss1 = vf0.copy()
+    miss1[:, 0] += 0.5
+    p0 = np.stack([vf0, ee0, miss0]).reshape(3, 12)
+    p1 = np.stack([vf1, ee1, miss1]).reshape(3, 12)
+    is_vf = np.array([1, 0, 1], np.int32)
+    err = np.array([err_for(p0[i].reshape(4, 3), p1[i].reshape(4, 3), bool(is_vf[i]))
+                    for i in range(3)])
+    hit, toi = w.debug_ccd(is_vf, p0, p1, err, MS, TOL)
+    t_star = (1 - MS) / 2
+    assert hit.tolist() == [1, 1, 0]
+    assert t_star - 5e-4 <= toi[0] <= t_star and t_star - 5e-4 <= toi[1] <= t_star
+
+
+def _pair_worlds(n=20, subdiv=2, z=0.32, global_kw=None):
+    global_kw = global_kw or {}
+    V, F, SV, SF = sheet_on_sphere(n, subdiv, z)
+    w = gpu_world(**global_kw)
+    cc = sp.CollisionConfig()
+    r, h = w.add_cloth(sp.ClothConfig(), cc, V, F, np.zeros(0, np.int32))
+    assert r, r.to_string()
+    r, ho = w.add_obstacle(cc, SV, SF)
+    assert r, r.to_string()
+    r, perm, offs = w.get_partition(h, V.shape[0])
+    ow = O.OracleWorld(oracle_for(global_kw))
+    ow.add_cloth(V, F, [], perm, offs, coll=coll_tuple(cc))
+    ow.add_obstacle(SV, SF, coll=coll_tuple(cc))
+    ow.finalize()
+    return V, F, SV, SF, w, h, ho, ow, perm
+
+
+@pytest.mark.parametrize("trial", [0, 1, 2])
+def test_broadphase_sets_equal_oracle(trial):
+    V, F, SV, SF, w, h, ho, ow, perm = _pair_worlds()
+    nvt = w.collider_vertex_count()
+    assert nvt == V.shape[0] + SV.shape[0]
+    rng = np.random.default_rng(100 + trial)
+    base = np.zeros((nvt, 3))
+    base[perm] = V
+    base[V.shape[0]:] = SV
+    a = _f32(base + rng.uniform(-0.02, 0.02, base.shape))
+    b = _f32(a + rng.uniform(-0.03, 0.03, base.shape) * (trial + 1))
+    gpt, gee = w.debug_candidates(a, b)
+    opt, oee = ow.candidates(a.reshape(-1), b.reshape(-1))
+    print(f"broadphase trial {trial}: PT {len(gpt)} EE {len(gee)}")
+    assert len(opt) > 0 and len(oee) > 0
+    assert sorted(map(tuple, gpt.tolist())) == sorted(map(tuple, opt.tolist()))
+    assert sorted(map(tuple, gee.tolist())) == sorted(map(tuple, oee.tolist()))
+
+
+def test_lockstep_steps_match_oracle():
+    """Each step starts the oracle from the GPU's own state, so the error
+    measured is that of one step of the collision path, not accumulated
+    divergence of two chaotic trajectories."""
+    kw = dict(acceleration_z=-9.81, dt=1 / 60, max_inner_iteration=50,
+              linear_solver_rel_tol_min=1e-6, linear_solver_rel_tol_max=1e-6,
+              initial_linear_rel_tol=1e-6, admm_rel_tol=1e-5, admm_abs_tol=1e-7)
+    V, F, SV, SF, w, h, ho, ow, perm = _pair_worlds(16, 2, 0.4, kw)
+    dt = float(np.float32(1 / 60))
+    x_prev = V.astype(np.float64).reshape(-1)
+    v_prev = np.zeros_like(x_prev)
+    errs, contact_steps = [], 0
+    for i in range(40):
+        ow.set_state(0, x_prev, v_prev)
+        assert w.solver_step()
+        st = w.last_stats.as_dict()
+        rc, ost = ow.step()
+        assert rc == 0
+        _, xg = w.get_cloth_position(h, n=V.size)
+        xg = xg.astype(np.float64)
+        e = np.abs(xg - ow.positions(0)).max()
+        errs.append(e)
+        contact_steps += st["contacts"] > 0
+        v_prev = (xg - x_prev) / dt
+        x_prev = xg
+    errs = np.array(errs)
+    print("lockstep per-step max error: "
+          f"max {errs.max():.3e} median {np.median(errs):.3e}; "
+          f"{contact_steps} steps with contacts")
+    assert contact_steps >= 10
+    assert np.median(errs) <= 1e-4
+
+
+def test_free_running_sheet_does_not_penetrate_sphere():
+    V, F, SV, SF = sheet_on_sphere(16, 2, 0.4)
+    w = gpu_world(acceleration_z=-9.81, dt=1 / 60)
+    cc = sp.CollisionConfig()
+    r, h = w.add_cloth(sp.ClothConfig(), cc, V, F, np.zeros(0, np.int32))
+    r, ho = w.add_obstacle(cc, SV, SF)
+    r_in = inscribed_radius(SV, SF)
+    c = np.array([0.5, 0.5, 0.0])
+    touched = False
+    for i in range(60):
+        assert w.solver_step()
+        touched |= w.last_stats.contacts > 0
+        _, x = w.get_cloth_position(h, n=V.size)
+        d = np.linalg.norm(x.reshape(-1, 3) - c, axis=1)
+        assert d.min() >= r_in - 1e-3, (i, d.min(), r_in)
+    assert touched

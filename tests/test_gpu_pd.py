@@ -1,0 +1,72 @@
+"""GPU PD step (no collision) against the FP64 oracle on the 32x32 pinned
+sheet: 200 steps, positions within 1e-4 m (the error is printed)."""
+import numpy as np
+import pytest
+
+import oracle as O
+import silk_physics_amd as sp
+from silk_physics_amd import scenes
+
+from helpers import coll_tuple, gpu_world, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+TOL_M = 1e-4
+
+
+def _c1_pair(global_kw):
+    s = scenes.c1_sheet()
+    w = gpu_world(**global_kw)
+    cc = sp.CollisionConfig(is_collision_on=False, is_self_collision_on=False)
+    r, h = w.add_cloth(sp.ClothConfig(), cc, s["V"], s["F"], s["pins"])
+    assert r, r.to_string()
+    r, perm, offs = w.get_partition(h, s["V"].shape[0])
+    ow = O.OracleWorld(oracle_for(global_kw))
+    ow.add_cloth(s["V"], s["F"], s["pins"], perm, offs, coll=coll_tuple(cc))
+    ow.finalize()
+    return s, w, h, ow
+
+
+@pytest.mark.parametrize("mode", ["cg_rel_1e-6", "adaptive"])
+def test_c1_200_steps_match_oracle(mode):
+    kw = dict(acceleration_z=-9.81, max_inner_iteration=10, dt=1 / 60)
+    if mode == "cg_rel_1e-6":
+        kw.update(linear_solver_rel_tol_min=1e-6, linear_solver_rel_tol_max=1e-6,
+                  initial_linear_rel_tol=1e-6)
+    s, w, h, ow = _c1_pair(kw)
+    worst = 0.0
+    for i in range(200):
+        r = w.solver_step()
+        assert r, r.to_string()
+        rc, _ = ow.step()
+        assert rc == 0
+        if i % 20 == 19 or i == 199:
+            _, xg = w.get_cloth_position(h, n=s["V"].size)
+            err = np.abs(xg.astype(np.float64) - ow.positions(0)).max()
+            worst = max(worst, err)
+    print(f"C1 {mode}: max |x_gpu - x_oracle| over 200 steps = {worst:.3e} m")
+    assert worst <= TOL_M
+
+
+def test_pinned_vertices_exact_and_free_fall_closed_form():
+    # one step from rest: pinned vertices stay at their targets bit for bit
+    s = scenes.c1_sheet()
+    w = gpu_world(acceleration_z=-9.81, dt=1 / 60)
+    cc = sp.CollisionConfig(is_collision_on=False, is_self_collision_on=False)
+    r, h = w.add_cloth(sp.ClothConfig(), cc, s["V"], s["F"], s["pins"])
+    for _ in range(5):
+        assert w.solver_step()
+    _, x = w.get_cloth_position(h, n=s["V"].size)
+    x = x.reshape(-1, 3)
+    np.testing.assert_array_equal(x[s["pins"]], s["V"][s["pins"]])
+    # unpinned free fall: rest shape, no elastic force -> implicit Euler
+    # gives x_n = x_0 + g h^2 n (n + 1) / 2 ... (damping 0: use damping 0)
+    w2 = gpu_world(acceleration_z=-9.81, dt=1 / 60)
+    r, h2 = w2.add_cloth(sp.ClothConfig(damping=0.0), cc, s["V"], s["F"], [])
+    g, dt = np.float32(-9.81), np.float32(1 / 60)
+    for n in range(1, 4):
+        assert w2.solver_step()
+        _, x2 = w2.get_cloth_position(h2, n=s["V"].size)
+        dz = x2.reshape(-1, 3)[:, 2] - s["V"][:, 2]
+        want = float(g) * float(dt) ** 2 * n * (n + 1) / 2
+        np.testing.assert_allclose(dz, want, rtol=2e-4, atol=1e-6)
