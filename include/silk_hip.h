@@ -187,11 +187,25 @@ int silk_hip_get_partition(const silk_hip_ctx* ctx, uint32_t handle,
                            int32_t* perm, int64_t nv, int32_t* offsets,
                            int64_t offsets_cap, int32_t* nparts);
 
-/* Elapsed device time (ms) of the named kernel class accumulated since the
- * last call with reset != 0.  Classes: 0 = whole step, 1 = PCG iterations,
- * 2 = local projection, 3 = contact solve, 4 = broadphase, 5 = CCD. */
+/* Elapsed device time (ms) and launch count of a kernel class accumulated
+ * since the last call with reset != 0.  Class 0 (whole step) is always
+ * recorded; the others only while silk_hip_set_timing(ctx, 1) is on, from
+ * HIP events around each launch on the library stream: 1 = PCG solve kernel,
+ * 2 = local projection, 3 = contact solve, 4 = broadphase, 5 = CCD,
+ * 6 = FP64 residual. */
 int silk_hip_get_timing(silk_hip_ctx* ctx, int32_t cls, double* ms,
                         int64_t* launches, int32_t reset);
+
+/* Turn per-kernel event timing on (1) or off (0). */
+int silk_hip_set_timing(silk_hip_ctx* ctx, int32_t on);
+
+/* Sizes of the device scene (builds it if needed), in this order:
+ * 0 cloth vertices, 1 cloth triangles, 2 off-diagonal scalar entries of the
+ * PD operator, 3 MAS banks (all levels), 4 padded slots, 5 persistent PCG
+ * workgroups (0 when the multi-launch PCG is used), 6 coarse MAS banks,
+ * 7 collider vertices.  n: capacity of info (entries beyond it are not
+ * written). */
+int silk_hip_get_scene_info(silk_hip_ctx* ctx, int64_t* info, int32_t n);
 
 /* Collider vertex count of the current scene: cloth vertices (solver order,
  * cloths in insertion order) followed by obstacle vertices (insertion order,

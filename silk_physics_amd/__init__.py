@@ -67,6 +67,8 @@ def load_library():
         "silk_hip_get_timing": (C.c_int, [vp, i32, C.POINTER(C.c_double),
                                           C.POINTER(C.c_int64), i32]),
         "silk_hip_collider_vertex_count": (C.c_int, [vp, C.POINTER(i64)]),
+        "silk_hip_set_timing": (C.c_int, [vp, i32]),
+        "silk_hip_get_scene_info": (C.c_int, [vp, C.POINTER(i64), i32]),
         "silk_hip_debug_candidates": (C.c_int, [vp, fp, fp, i64, ip, i64,
                                                 C.POINTER(i64), ip, i64,
                                                 C.POINTER(i64)]),
@@ -419,6 +421,20 @@ class World:
         st = lib.silk_hip_get_partition(self._ctx, handle, None, 0, _iptr(offsets),
                                         offsets.size, None)
         return self._wrap(st), perm, offsets
+
+    def set_timing(self, on: bool) -> Result:
+        return self._wrap(load_library().silk_hip_set_timing(self._ctx, int(on)))
+
+    SCENE_INFO_KEYS = ("vertices", "triangles", "offdiag_entries", "mas_banks",
+                       "slots", "pcg_workgroups", "coarse_banks",
+                       "collider_vertices")
+
+    def scene_info(self) -> dict:
+        info = (C.c_int64 * 8)()
+        st = load_library().silk_hip_get_scene_info(self._ctx, info, 8)
+        if st != 0:
+            raise RuntimeError(self._wrap(st).to_string())
+        return dict(zip(self.SCENE_INFO_KEYS, [int(v) for v in info]))
 
     def collider_vertex_count(self) -> int:
         n = C.c_int64(0)

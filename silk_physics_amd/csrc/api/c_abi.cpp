@@ -334,6 +334,30 @@ int silk_hip_get_timing(silk_hip_ctx* ctx, int32_t cls, double* ms,
   });
 }
 
+int silk_hip_set_timing(silk_hip_ctx* ctx, int32_t on) {
+  return guarded(ctx, [&](Context& c) {
+    c.timing_on_ = on != 0;
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_get_scene_info(silk_hip_ctx* ctx, int64_t* info, int32_t n) {
+  return guarded(ctx, [&](Context& c) {
+    if (info == nullptr || n < 0) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    if (c.cloths_.empty()) return int{SILK_HIP_ERR_NEED_INIT_SOLVER};
+    if (c.scene_dirty_ || !c.scene_) c.build_scene();
+    const auto& S = *c.scene_;
+    int ncb = 0;
+    for (int b = 0; b < S.nbank; ++b) ncb += S.h_bank_level[b] > 0;
+    const int64_t v[8] = {S.nv, S.nf, S.nnz_off, S.nbank,
+                          S.pers.nslot_total,
+                          S.use_persistent ? S.pers.grid : 0, ncb,
+                          S.coll ? S.coll->total_vertices() : 0};
+    for (int i = 0; i < n && i < 8; ++i) info[i] = v[i];
+    return int{SILK_HIP_OK};
+  });
+}
+
 int silk_hip_collider_vertex_count(silk_hip_ctx* ctx, int64_t* nvt) {
   return guarded(ctx, [&](Context& c) {
     if (nvt == nullptr) return int{SILK_HIP_ERR_INVALID_CONFIG};

@@ -88,7 +88,7 @@ class HipCollision final : public CollisionWorld {
                 float interval, float ms, silk_hip_step_stats* stats) override;
   void rollback_contacts(const float* outer_x, float time, float dhat,
                          silk_hip_step_stats* stats) override;
-  void contact_update(const float* x) override;
+  void contact_update(const float* x, const int* skip) override;
   ContactView contact_view() const override;
   int active_dof() const override { return 3 * n_active_; }
   void end_step(silk_hip_step_stats* stats) override;
@@ -370,11 +370,14 @@ static int grow(int need) { return std::max(1024, need + need / 2); }
 std::vector<DcdRec> HipCollision::dcd(float time, float dhat,
                                       silk_hip_step_stats* stats) {
   for (;;) {
+    ctx_.timed_begin(kTimeBroad);
     launch_broadphase();
+    ctx_.timed_end();
     launch_dcd(st_, verts(), tables(), pt_pairs_.get(), counters_.get(), pt_cap_,
                ee_pairs_.get(), counters_.get() + 1, ee_cap_, time, dhat,
                recs_.get(), counters_.get() + 2, rec_cap_);
     read_io();
+    ctx_.timed_collect();
     const int npt = h_io_[0], nee = h_io_[1], nrec = h_io_[2];
     bool again = false;
     if (npt > pt_cap_) {
@@ -596,11 +599,16 @@ float HipCollision::min_toi(const float* outer_x, const float* inner_x,
     hip_check(hipMemcpyAsync(toi_bits_.get(), h_io_ + 6, sizeof(bits),
                              hipMemcpyHostToDevice, st_),
               "toi init");
+    ctx_.timed_begin(kTimeBroad);
     launch_broadphase();
+    ctx_.timed_end();
+    ctx_.timed_begin(kTimeCcd);
     launch_ccd(st_, verts(), tables(), pt_pairs_.get(), counters_.get(), pt_cap_,
                ee_pairs_.get(), counters_.get() + 1, ee_cap_, cp,
                toi_bits_.get());
+    ctx_.timed_end();
     read_io();
+    ctx_.timed_collect();
     const int npt = h_io_[0], nee = h_io_[1];
     bool again = false;
     if (npt > pt_cap_) {
@@ -628,7 +636,7 @@ void HipCollision::rollback_contacts(const float* outer_x, float time,
   append(dcd(time, dhat, stats));
 }
 
-void HipCollision::contact_update(const float* x) {
+void HipCollision::contact_update(const float* x, const int* skip) {
   if (contacts_.empty()) return;
   ContactSet cs;
   cs.n = static_cast<int>(contacts_.size());
@@ -644,6 +652,7 @@ void HipCollision::contact_update(const float* x) {
   cs.eta = eta_.get();
   cs.dual = dual_.get();
   cs.sweeps = ctx_.contact_sweeps();
+  cs.skip = skip;
   launch_contact_update(st_, cs, x);
 }
 

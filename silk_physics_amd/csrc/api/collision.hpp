@@ -39,7 +39,8 @@ class CollisionWorld {
   virtual void rollback_contacts(const float* outer_x, float time, float dhat,
                                  silk_hip_step_stats* stats) = 0;
   /// ADMM contact update for the current x (contact_constraints.cu:246-264).
-  virtual void contact_update(const float* x) = 0;
+  /// skip: optional device flag; the update is a no-op while *skip != 0.
+  virtual void contact_update(const float* x, const int* skip) = 0;
   virtual ContactView contact_view() const = 0;
   virtual int active_dof() const = 0;
   virtual void end_step(silk_hip_step_stats* stats) = 0;

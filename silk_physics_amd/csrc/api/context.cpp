@@ -31,6 +31,17 @@ void download(std::vector<T>& h, const T* d, size_t n, hipStream_t s) {
 
 }  // namespace
 
+Scene::Scene() {
+  hip_check(hipHostMalloc(reinterpret_cast<void**>(&h_admm_ctl),
+                          2 * sizeof(AdmmCtl)),
+            "hipHostMalloc");
+  std::memset(static_cast<void*>(h_admm_ctl), 0, 2 * sizeof(AdmmCtl));
+}
+
+Scene::~Scene() {
+  if (h_admm_ctl != nullptr) (void)hipHostFree(h_admm_ctl);
+}
+
 Context::Context(int device) : device_(device) {
   global_.acceleration[0] = 0.0f;
   global_.acceleration[1] = 0.0f;
@@ -46,8 +57,40 @@ Context::Context(int device) : device_(device) {
   global_.admm_rel_tol = 1e-3f;
 }
 
+void Context::timed_begin(int cls) {
+  if (!timing_on_) return;
+  const size_t k = ev_cls_.size();
+  if (k >= ev_pool_.size()) {
+    EvPair e;
+    hip_check(hipEventCreate(&e.a), "hipEventCreate");
+    hip_check(hipEventCreate(&e.b), "hipEventCreate");
+    ev_pool_.push_back(e);
+  }
+  ev_cls_.push_back(cls);
+  hip_check(hipEventRecord(ev_pool_[k].a, stream_), "event");
+}
+
+void Context::timed_end() {
+  if (!timing_on_ || ev_cls_.empty()) return;
+  hip_check(hipEventRecord(ev_pool_[ev_cls_.size() - 1].b, stream_), "event");
+}
+
+void Context::timed_collect() {
+  for (size_t k = 0; k < ev_cls_.size(); ++k) {
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, ev_pool_[k].a, ev_pool_[k].b), "elapsed");
+    timing_ms_[ev_cls_[k]] += ms;
+    timing_launch_[ev_cls_[k]] += 1;
+  }
+  ev_cls_.clear();
+}
+
 Context::~Context() {
   scene_.reset();
+  for (auto& e : ev_pool_) {
+    if (e.a) (void)hipEventDestroy(e.a);
+    if (e.b) (void)hipEventDestroy(e.b);
+  }
   if (dev_ready_) {
     for (auto& e : ev_)
       if (e) (void)hipEventDestroy(e);
@@ -499,6 +542,11 @@ void Context::build_scene() {
 
   S.nblk = static_cast<int>(blk_sys.size());
   S.nbank = static_cast<int>(bank_sys.size());
+  S.h_bank_level = bank_level;
+  S.nnz_off = 0;
+  for (const auto& K : Ks)
+    for (int r = 0; r < K.n; ++r)
+      for (int a = K.row_ptr[r]; a < K.row_ptr[r + 1]; ++a) S.nnz_off += K.col[a] != r;
   S.blk_sys.upload(blk_sys, st);
   S.blk_row0.upload(blk_row0, st);
   S.blk_rows.upload(blk_rows, st);
@@ -569,6 +617,7 @@ void Context::build_scene() {
     S.ps_abort.resize(1);
     S.ps_abort.zero(st);
     S.ps_out.resize(std::max(1, S.nsys));
+    S.admm_ctl.resize(1);
     PersistDev& a = S.pers;
     a.nsys = S.nsys;
     a.nslot_total = nslot;
@@ -765,8 +814,111 @@ int Context::finish_pcg(int iters, double rel, const RhsArgs& ra,
   return iters;
 }
 
+// ADMM loop driven from the device: every iteration's kernels are enqueued
+// without waiting; k_admm_check runs the stopping test after each local step
+// and later kernels become no-ops once it passes.  One synchronisation per
+// solve reads the counters back.
+bool Context::admm_solve_device(silk_hip_step_stats* stats) {
+  Scene& S = *scene_;
+  ClothDev c = cloth_dev();
+  const size_t n3 = 3 * static_cast<size_t>(S.nv);
+  hip_check(hipMemsetAsync(S.pin_lambda.get(), 0, n3 * sizeof(float), stream_),
+            "pin lambda reset");
+  ContactView cv = S.coll ? S.coll->contact_view() : ContactView{};
+  const auto& g = global_;
+  AdmmCtl* ctl = S.admm_ctl.get();
+  const int* skip = &ctl->done;
+  launch_admm_init(stream_, ctl, g.initial_linear_rel_tol,
+                   g.linear_solver_rel_tol_min, g.linear_solver_rel_tol_max);
+  RhsArgs ra;
+  ra.c = c;
+  ra.prev_x = S.prev_x.get();
+  ra.prev_v = S.prev_v.get();
+  ra.elastic_rhs = S.elastic_rhs.get();
+  ra.pin_lambda = S.pin_lambda.get();
+  ra.cw = cv.w;
+  ra.cp = cv.p;
+  ra.ceta = cv.eta;
+  ra.skip = skip;
+  PcgDev p = pcg_dev();
+  PersistDev pa = S.pers;
+  pa.abs_tol = g.linear_solver_abs_tol;
+  pa.max_iter = pcg_max_iter();
+  pa.ctl = ctl;
+  AdmmCheckArgs ca;
+  ca.grid = local_step_grid(S.nv);
+  ca.n3 = static_cast<int>(n3);
+  ca.primal_dof = 6 * S.nf + S.pinned_dof + (S.coll ? S.coll->active_dof() : 0);
+  ca.abs_tol = g.admm_abs_tol;
+  ca.rel_tol = g.admm_rel_tol;
+  ca.lin_init = g.initial_linear_rel_tol;
+  ca.lin_min = g.linear_solver_rel_tol_min;
+  ca.lin_max = g.linear_solver_rel_tol_max;
+  for (int it = 0; it < g.max_inner_iteration; ++it) {
+    if (it != 0) {
+      timed_begin(kTimeResidual);
+      launch_pcg_residual(stream_, p, ra);
+      timed_end();
+      if (it == 1) launch_mas_factor(stream_, p, S.cd.get(), skip);
+      timed_begin(kTimePcg);
+      hip_check(launch_pcg_persistent(stream_, pa, 0.0f),
+                "persistent PCG launch");
+      timed_end();
+    }
+    timed_begin(kTimeContact);
+    if (S.coll) S.coll->contact_update(S.inner_x.get(), skip);
+    timed_end();
+    LocalStepArgs a;
+    a.x = S.inner_x.get();
+    a.z_in = S.z[S.aux_cur].get();
+    a.u_in = S.u[S.aux_cur].get();
+    a.z_out = S.z[S.aux_cur ^ 1].get();
+    a.u_out = S.u[S.aux_cur ^ 1].get();
+    a.elastic_rhs = S.elastic_rhs.get();
+    a.pin_lambda = S.pin_lambda.get();
+    a.contact_dual = cv.dual;
+    a.contact_eta = cv.eta;
+    a.contact_w = cv.w;
+    a.contact_p = cv.p;
+    a.partials = S.admm_part.get();
+    a.skip = skip;
+    timed_begin(kTimeLocal);
+    launch_local_step(stream_, c, a);
+    timed_end();
+    S.aux_cur ^= 1;
+    ca.it = it;
+    launch_admm_check(stream_, S.admm_part.get(), ca, ctl);
+  }
+  launch_enforce(stream_, c, cv.w, cv.p, S.inner_x.get());
+  hip_check(hipMemcpyAsync(S.h_admm_ctl, ctl, sizeof(AdmmCtl),
+                           hipMemcpyDeviceToHost, stream_),
+            "admm ctl");
+  hip_check(hipMemcpyAsync(S.h_admm_ctl + 1, S.ps_abort.get(), sizeof(int),
+                           hipMemcpyDeviceToHost, stream_),
+            "pcg abort flag");
+  hip_check(hipStreamSynchronize(stream_), "admm sync");
+  timed_collect();
+  const AdmmCtl& hc = S.h_admm_ctl[0];
+  const int abort_flag = *reinterpret_cast<const int*>(S.h_admm_ctl + 1);
+  if (abort_flag != 0) {
+    S.ps_bar.zero(stream_);
+    S.ps_abort.zero(stream_);
+    hip_check(hipStreamSynchronize(stream_), "pcg barrier reset");
+    throw HipError("persistent PCG: grid barrier timed out");
+  }
+  if (stats) {
+    stats->admm_iterations += hc.admm_iters;
+    stats->pcg_iterations += hc.pcg_iters;
+    stats->pcg_solves += hc.pcg_solves;
+    if (hc.pcg_solves > 0) stats->last_pcg_rel_residual = hc.last_rel;
+  }
+  return hc.diverged == 0;
+}
+
 bool Context::admm_solve(silk_hip_step_stats* stats) {
   Scene& S = *scene_;
+  if (S.use_persistent && !global_.true_residual_stats)
+    return admm_solve_device(stats);
   ClothDev c = cloth_dev();
   const size_t n3 = 3 * static_cast<size_t>(S.nv);
   hip_check(hipMemsetAsync(S.pin_lambda.get(), 0, n3 * sizeof(float), stream_),
@@ -792,7 +944,7 @@ bool Context::admm_solve(silk_hip_step_stats* stats) {
           g.linear_solver_rel_tol_min, g.linear_solver_rel_tol_max);
       pcg_solve(rel, it == 1, ra, stats);
     }
-    if (S.coll) S.coll->contact_update(S.inner_x.get());
+    if (S.coll) S.coll->contact_update(S.inner_x.get(), nullptr);
     LocalStepArgs a;
     a.x = S.inner_x.get();
     a.z_in = S.z[S.aux_cur].get();

@@ -19,6 +19,17 @@
 
 namespace silk::amd {
 
+// Timing classes of silk_hip_get_timing.
+enum : int {
+  kTimeStep = 0,
+  kTimePcg = 1,       // PCG solve kernel (persistent) per launch
+  kTimeLocal = 2,     // local projection
+  kTimeContact = 3,   // contact Gauss-Seidel
+  kTimeBroad = 4,     // refit + traversal
+  kTimeCcd = 5,       // CCD kernel
+  kTimeResidual = 6,  // FP64 residual / RHS kernel
+};
+
 struct ClothEntity {
   uint32_t handle = 0;
   silk_hip_cloth_params cloth{};
@@ -82,6 +93,8 @@ struct Scene {
   std::vector<double> h_part2;
   DevBuf<PcgState> state;
   int nblk = 0, nbank = 0, ncnode = 0, n1node = 0;
+  int64_t nnz_off = 0;              // off-diagonal scalar entries of K
+  std::vector<int> h_bank_level;
   std::vector<int> h_blk0;  // sys_blk0 on host
   std::vector<PcgState> h_state;
   // ---- persistent PCG (one cooperative launch per solve)
@@ -94,6 +107,10 @@ struct Scene {
   DevBuf<double> ps_part;
   DevBuf<unsigned> ps_bar;
   DevBuf<PcgState> ps_out;
+  DevBuf<AdmmCtl> admm_ctl;
+  AdmmCtl* h_admm_ctl = nullptr;  // pinned: AdmmCtl, then the abort flag
+  Scene();
+  ~Scene();
   // ---- collision
   std::unique_ptr<CollisionWorld> coll;
 };
@@ -118,6 +135,17 @@ class Context {
   std::unique_ptr<Scene> scene_;
   double timing_ms_[8] = {};
   int64_t timing_launch_[8] = {};
+  // Per-kernel event timing (off unless enabled): pairs of events recorded
+  // around launches on stream_, resolved after the next synchronisation.
+  bool timing_on_ = false;
+  struct EvPair {
+    hipEvent_t a = nullptr, b = nullptr;
+  };
+  std::vector<EvPair> ev_pool_;
+  std::vector<int> ev_cls_;  // class of each recorded pair
+  void timed_begin(int cls);
+  void timed_end();
+  void timed_collect();  // stream must be synchronised
 
   uint32_t new_handle();
   ClothEntity* find_cloth(uint32_t h);
@@ -137,6 +165,7 @@ class Context {
   int finish_pcg(int iters, double rel, const RhsArgs& ra,
                  silk_hip_step_stats* st);
   bool admm_solve(silk_hip_step_stats* st);
+  bool admm_solve_device(silk_hip_step_stats* st);
   void get_positions(const ClothEntity& c, float* out) const;
   /// Push one cloth's pin targets to the device scene (if built).
   void update_pin_targets(const ClothEntity& c);

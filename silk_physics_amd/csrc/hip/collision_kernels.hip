@@ -694,6 +694,7 @@ __global__ void __launch_bounds__(1024) k_ccd_filter(CollVerts v, double* err6) 
 __global__ void __launch_bounds__(1024)
     k_contact_update(ContactSet cs, const float* __restrict__ x, int nv) {
   using namespace dev;
+  if (cs.skip != nullptr && *cs.skip != 0) return;
   // q = x - eta / w for contact vertices, p = x elsewhere
   // (contact_constraints.cu:203-214); keep p_prev for the dual residual.
   for (int i = threadIdx.x; i < nv; i += blockDim.x) {

@@ -152,6 +152,7 @@ struct ContactSet {
   float* eta = nullptr;
   float* dual = nullptr;            // 3 nv, w (p - p_prev)
   int sweeps = 10;
+  const int* skip = nullptr;        // no-op when *skip != 0
 };
 
 /// One ADMM contact update (contact_constraints.cu:246-264): single

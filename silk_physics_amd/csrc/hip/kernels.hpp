@@ -70,7 +70,43 @@ struct LocalStepArgs {
   const float* contact_w = nullptr;      // nv: active weight
   const float* contact_p = nullptr;      // 3*nv
   double* partials = nullptr;            // kAdmmPartials * grid
+  // When non-null and *skip != 0 the launch only carries z and u over
+  // (z_out = z_in, u_out = u_in): the ADMM loop has already stopped.
+  const int* skip = nullptr;
 };
+
+/// Device-side control of one ADMM solve (admm_solver.cu:209-330): the
+/// stopping test runs on the device after every local step, and every
+/// kernel of a later iteration becomes a no-op once `done` is set, so the
+/// host enqueues all iterations without waiting on any of them.
+struct AdmmCtl {
+  int done = 0;       // stopping test passed, or diverged
+  int diverged = 0;   // non-finite state seen
+  int admm_iters = 0;
+  int pcg_iters = 0;
+  int pcg_solves = 0;
+  int pad = 0;
+  float rel_tol = 0;  // linear tolerance of the next x-update
+  float last_rel = 0; // sqrt(rr / rr0) of the last PCG solve
+  double init_primal = 0, init_dual = 0, ratio = 1;
+};
+
+struct AdmmCheckArgs {
+  int grid = 0;        // local-step workgroups (partials)
+  int n3 = 0;          // 3 * cloth vertices
+  int primal_dof = 0;  // 6 F + pinned dof + active contact dof
+  int it = 0;          // ADMM iteration just finished
+  float abs_tol = 0, rel_tol = 0;
+  float lin_init = 0, lin_min = 0, lin_max = 0;
+};
+
+/// ctl = {not done, rel_tol = clamp(lin_init)} before an ADMM solve.
+void launch_admm_init(hipStream_t s, AdmmCtl* ctl, float lin_init,
+                      float lin_min, float lin_max);
+/// Stopping test of iteration a.it from the local-step partials
+/// (admm_solver.cu:296-330): updates ratio, rel_tol and done.
+void launch_admm_check(hipStream_t s, const double* partials,
+                       const AdmmCheckArgs& a, AdmmCtl* ctl);
 
 /// ADMM aux-variable and multiplier update for cloth, pins and the contact
 /// primal term (cloth_admm_helper.cu:80-262, equality_constraints.cu:90-154,
@@ -143,7 +179,8 @@ struct PcgDev {
 
 /// Factorize: bank matrices = static + aggregated diag, inverted in place
 /// (mas_preconditioner.cu:489-598,730-859).
-void launch_mas_factor(hipStream_t s, const PcgDev& p, float* cd_scratch);
+void launch_mas_factor(hipStream_t s, const PcgDev& p, float* cd_scratch,
+                       const int* skip = nullptr);
 
 /// Right-hand side of the x-update (admm_solver.cu:272-292,
 /// cloth_admm_helper.cu:264-332, equality_constraints.cu:107-120,
@@ -158,6 +195,7 @@ struct RhsArgs {
   const float* cw = nullptr;           // active contact weight (may be null)
   const float* cp = nullptr;
   const float* ceta = nullptr;
+  const int* skip = nullptr;           // no-op when *skip != 0
 };
 
 /// PCG init, part 1 (mas_cg_solver.cu:273-277): d and r0 = b - (K + D) x0
@@ -231,6 +269,7 @@ struct PersistDev {
   unsigned* bar = nullptr;          // kPcgBarrierStride per system
   int* abort_flag = nullptr;
   PcgState* out = nullptr;          // per system
+  AdmmCtl* ctl = nullptr;           // optional: skip flag, tolerance, counters
   float abs_tol = 0.0f;
   int max_iter = 1000;
   int coarse_lds = 0;   // 1: coarse levels solved per workgroup in LDS

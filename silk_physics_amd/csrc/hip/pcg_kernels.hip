@@ -81,6 +81,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(PcgDev p, RhsArgs ra,
                                                      float* __restrict__ r_out,
                                                      int write_d) {
   __shared__ double lds[17];
+  if (ra.skip != nullptr && *ra.skip != 0) return;
   const int b = blockIdx.x;
   const int row = p.blk_row0[b] + threadIdx.x;
   const bool valid = threadIdx.x < p.blk_rows[b];
@@ -166,7 +167,9 @@ __global__ void k_coarse_sum(PcgDev p, const float* __restrict__ src,
 }
 
 // ---- level-1 coarse sums of the scalar diagonal
-__global__ void k_coarse_dsum(PcgDev p, float* __restrict__ cd) {
+__global__ void k_coarse_dsum(PcgDev p, float* __restrict__ cd,
+                              const int* skip) {
+  if (skip != nullptr && *skip != 0) return;
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= p.n1node) return;
   float s = 0.0f;
@@ -217,7 +220,9 @@ __device__ __forceinline__ void load_bank_slot(const PcgDev& p, int bank,
 
 // ---- MAS factorization: one 64-thread workgroup per bank.
 __global__ void __launch_bounds__(64) k_mas_factor(PcgDev p,
-                                                   const float* __restrict__ cd) {
+                                                   const float* __restrict__ cd,
+                                                   const int* skip) {
+  if (skip != nullptr && *skip != 0) return;
   const int bank = blockIdx.x;
   const int level = p.bank_level[bank];
   __shared__ float A[kBankSlots][kBankSlots + 1];
@@ -440,11 +445,13 @@ void mas_apply(hipStream_t s, const PcgDev& p, const float* r,
 
 }  // namespace
 
-void launch_mas_factor(hipStream_t s, const PcgDev& p, float* cd_scratch) {
+void launch_mas_factor(hipStream_t s, const PcgDev& p, float* cd_scratch,
+                       const int* skip) {
   if (p.nbank == 0) return;
   if (p.n1node > 0)
-    k_coarse_dsum<<<grid_for(p.n1node, kBlock), kBlock, 0, s>>>(p, cd_scratch);
-  k_mas_factor<<<p.nbank, 64, 0, s>>>(p, cd_scratch);
+    k_coarse_dsum<<<grid_for(p.n1node, kBlock), kBlock, 0, s>>>(p, cd_scratch,
+                                                                skip);
+  k_mas_factor<<<p.nbank, 64, 0, s>>>(p, cd_scratch, skip);
 }
 
 void launch_pcg_residual(hipStream_t s, const PcgDev& p, const RhsArgs& ra) {

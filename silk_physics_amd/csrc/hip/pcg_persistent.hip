@@ -106,6 +106,10 @@ __global__ void __launch_bounds__(kPcgThreads)
         static_cast<int>(blockIdx.x) < a.sys[s].wg0 + a.sys[s].nwg)
       sys = s;
   if (sys < 0) return;  // spare workgroup: takes no part in any barrier
+  // Stopped ADMM loop: nothing to solve.  ctl is not written before every
+  // workgroup has passed its first barrier, so all read the same value.
+  if (a.ctl != nullptr && a.ctl->done != 0) return;
+  if (a.ctl != nullptr) rel_tol = a.ctl->rel_tol;
   const PcgSysDev S = a.sys[sys];
   unsigned* bcount = a.bar + kPcgBarrierStride * sys;
   unsigned* bgen = bcount + kPcgBarrierStride / 2;
@@ -309,6 +313,13 @@ __global__ void __launch_bounds__(kPcgThreads)
     o.status = status;
     o.pad = 0;
     a.out[sys] = o;
+    if (a.ctl != nullptr) {
+      atomicAdd(&a.ctl->pcg_iters, iter);
+      if (sys == 0) {
+        atomicAdd(&a.ctl->pcg_solves, 1);
+        a.ctl->last_rel = static_cast<float>(rr0 > 0.0 ? sqrt(rr / rr0) : 0.0);
+      }
+    }
   }
 }
 

@@ -171,6 +171,20 @@ __global__ void __launch_bounds__(kBlock)
     k_local_step(ClothDev c, LocalStepArgs a) {
   __shared__ double lds[17];
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a.skip != nullptr && *a.skip != 0) {
+    // The loop has stopped: carry the aux variables over to the buffers the
+    // next iteration (or the next segment) reads.
+    if (v < c.nv)
+      for (int e = c.vf_ptr[v]; e < c.vf_ptr[v + 1]; ++e) {
+        if ((c.vf_list[e] & 3) != 0) continue;
+        const int f = c.vf_list[e] >> 2;
+        for (int j = 0; j < 6; ++j) {
+          a.z_out[6 * f + j] = a.z_in[6 * f + j];
+          a.u_out[6 * f + j] = a.u_in[6 * f + j];
+        }
+      }
+    return;
+  }
   double s[7] = {0, 0, 0, 0, 0, 0, 0};
   if (v < c.nv) {
     double rhs[3] = {0.0, 0.0, 0.0};
@@ -304,7 +318,68 @@ void launch_mix(hipStream_t s, int n3, float w, const float* a, const float* b,
   k_mix<<<grid_for(n3), kBlock, 0, s>>>(n3, w, a, b, out);
 }
 
+// Stopping test of admm_solver.cu:296-330 on the device; one workgroup.
+__global__ void __launch_bounds__(kBlock)
+    k_admm_check(const double* __restrict__ part, AdmmCheckArgs a, AdmmCtl* ctl) {
+  __shared__ double lds[17];
+  __shared__ double tot[kAdmmPartials];
+  if (ctl->done) return;
+  for (int j = 0; j < 7; ++j) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < a.grid; b += blockDim.x)
+      v += part[kAdmmPartials * b + j];
+    const double s = block_sum(v, lds);
+    if (threadIdx.x == 0) tot[j] = s;
+  }
+  if (threadIdx.x != 0) return;
+  AdmmCtl c = *ctl;
+  c.admm_iters += 1;
+  if (a.it == 0) {
+    *ctl = c;
+    return;
+  }
+  if (tot[6] > 0.0) {
+    c.diverged = 1;
+    c.done = 1;
+    *ctl = c;
+    return;
+  }
+  const double primal = sqrt(tot[0]);
+  const double dual = sqrt(tot[3]);
+  if (a.it == 1) {
+    c.init_primal = primal;
+    c.init_dual = dual;
+  }
+  const double at = a.abs_tol, rt = a.rel_tol;
+  const double eps_p = sqrt(static_cast<double>(a.primal_dof)) * at +
+                       fmax(sqrt(tot[1]), sqrt(tot[2])) * rt;
+  const double eps_d = sqrt(static_cast<double>(a.n3)) * at +
+                       fmax(sqrt(tot[4]), sqrt(tot[5])) * rt;
+  c.ratio = fmax(primal / fmax(c.init_primal, at), dual / fmax(c.init_dual, at));
+  const float rel = static_cast<float>(static_cast<double>(a.lin_init) * c.ratio);
+  c.rel_tol = fminf(fmaxf(rel, a.lin_min), a.lin_max);
+  if (primal <= eps_p && dual <= eps_d) c.done = 1;
+  *ctl = c;
+}
+
+__global__ void k_admm_init(AdmmCtl* ctl, float lin_init, float lin_min,
+                            float lin_max) {
+  AdmmCtl c;
+  c.rel_tol = fminf(fmaxf(lin_init, lin_min), lin_max);
+  *ctl = c;
+}
+
 int local_step_grid(int nv) { return grid_for(nv); }
+
+void launch_admm_init(hipStream_t s, AdmmCtl* ctl, float lin_init,
+                      float lin_min, float lin_max) {
+  k_admm_init<<<1, 1, 0, s>>>(ctl, lin_init, lin_min, lin_max);
+}
+
+void launch_admm_check(hipStream_t s, const double* partials,
+                       const AdmmCheckArgs& a, AdmmCtl* ctl) {
+  k_admm_check<<<1, kBlock, 0, s>>>(partials, a, ctl);
+}
 
 void launch_local_step(hipStream_t s, const ClothDev& c,
                        const LocalStepArgs& a) {
