@@ -413,6 +413,7 @@ void Context::build_scene() {
       for (int l = 1; l <= t.coarse_levels; ++l)
         lvl_off[l + 1] = lvl_off[l] + t.level_nodes[l];
       sd.ncn = lvl_off[t.coarse_levels + 1];
+      sd.n1 = t.coarse_levels >= 1 ? t.level_nodes[1] : 0;
       for (int bk = 0; bk < nb0; ++bk)
         for (int s = 0; s < kBankSlots; ++s) {
           const int lr = po[bk] + s;
@@ -435,10 +436,15 @@ void Context::build_scene() {
           }
         }
       sd.ncb = static_cast<int>(ps_cb_bank.size()) - sd.cb0;
+      // Direct children of every coarse node of level >= 2 (level-1 nodes
+      // take their sums from the residual itself).
       std::vector<std::vector<int>> desc(sd.ncn);
       for (int lr = 0; lr < t.n; ++lr)
-        for (int l = 1; l <= t.coarse_levels; ++l)
-          desc[lvl_off[l] + t.cmap[l - 1][lr]].push_back(lvl_off[1] + t.cmap[0][lr]);
+        for (int l = 2; l <= t.coarse_levels; ++l)
+          desc[lvl_off[l] + t.cmap[l - 1][lr]].push_back(lvl_off[l - 1] +
+                                                          t.cmap[l - 2][lr]);
+      for (int l = 0; l < 5; ++l)
+        sd.lvl[l] = l + 1 <= t.coarse_levels + 1 ? lvl_off[l + 1] : sd.ncn;
       for (auto& dsc : desc) {
         std::sort(dsc.begin(), dsc.end());
         dsc.erase(std::unique(dsc.begin(), dsc.end()), dsc.end());
@@ -471,6 +477,8 @@ void Context::build_scene() {
       int width = 0;
       for (int s = s0; s < std::min(nslot, s0 + kSlice); ++s)
         if (ps_slot_row[s] >= 0) width = std::max(width, offdiag(ps_slot_row[s]));
+      // Whole chunks of neighbours: the kernel gathers them 8 at a time.
+      width = (width + kPcgNbrChunk - 1) / kPcgNbrChunk * kPcgNbrChunk;
       ps_sell_off.push_back(static_cast<int>(ps_sell_col.size()));
       ps_sell_width.push_back(width);
       const size_t base = ps_sell_col.size();
@@ -589,10 +597,12 @@ void Context::build_scene() {
       b.upload(v, st);
     };
     const int nslot = static_cast<int>(ps_slot_row.size());
-    int max_ncb = 0, max_ncn = 0;
+    int max_ncb = 0, max_ncn = 0, max_n1 = 0, max_cnl = 0;
     for (const auto& sd : ps_sys) {
       max_ncb = std::max(max_ncb, sd.ncb);
       max_ncn = std::max(max_ncn, sd.ncn);
+      max_n1 = std::max(max_n1, sd.n1);
+      max_cnl = std::max(max_cnl, ps_cn_ptr[sd.cn0 + sd.ncn] - ps_cn_ptr[sd.cn0]);
     }
     S.ps_sys.upload(ps_sys, st);
     up(S.ps_slot_row, ps_slot_row);
@@ -607,10 +617,9 @@ void Context::build_scene() {
     up(S.ps_cn_ptr, ps_cn_ptr);
     up(S.ps_cn_list, ps_cn_list);
     const int ncn_all = static_cast<int>(ps_cn_ptr.size()) - 1;
-    S.ps_pbuf.resize(6 * static_cast<size_t>(std::max(1, nslot)));
-    S.ps_zbuf.resize(3 * static_cast<size_t>(std::max(1, nslot)));
-    S.ps_cr.resize(3 * static_cast<size_t>(std::max(1, ncn_all)));
-    S.ps_cz.resize(3 * static_cast<size_t>(std::max(1, ncn_all)));
+    S.ps_pbuf.resize(8 * static_cast<size_t>(std::max(1, nslot)));
+    S.ps_zbuf.resize(4 * static_cast<size_t>(std::max(1, nslot)));
+    S.ps_cr.resize(4 * static_cast<size_t>(std::max(1, ncn_all)));
     S.ps_part.resize(4 * static_cast<size_t>(std::max(1, ps_wg)));
     S.ps_bar.resize(kPcgBarrierStride * static_cast<size_t>(std::max(1, S.nsys)));
     S.ps_bar.zero(st);
@@ -638,20 +647,22 @@ void Context::build_scene() {
     a.x = S.inner_x.get();
     a.pbuf = S.ps_pbuf.get();
     a.zbuf = S.ps_zbuf.get();
-    a.cr = S.ps_cr.get();
-    a.cz = S.ps_cz.get();
+    a.nsum = S.ps_cr.get();
     a.part = S.ps_part.get();
     a.bar = S.ps_bar.get();
     a.abort_flag = S.ps_abort.get();
     a.out = S.ps_out.get();
     a.abs_tol = global_.linear_solver_abs_tol;
     a.max_iter = pcg_max_iter();
-    a.coarse_lds = (max_ncb <= 64 && max_ncn <= 4096) ? 1 : 0;
-    a.lds_nodes = a.coarse_lds ? std::max(1, max_ncn) : 0;
+    a.n1_max = max_n1;
+    a.ncn_max = max_ncn;
+    a.ncb_max = max_ncb;
+    a.cnl_max = max_cnl;
     a.grid = ps_wg;
     const char* mode = std::getenv("SILK_HIP_PCG");
     const bool want = !(mode && std::strcmp(mode, "launch") == 0);
-    S.use_persistent = want && ps_wg > 0 && pcg_persistent_fits(ps_wg, a.lds_nodes);
+    S.use_persistent = want && ps_wg > 0 &&
+                       pcg_persistent_fits(ps_wg, pcg_persistent_lds(a));
   }
   S.coll = make_collision_world(*this, S);
   hip_check(hipStreamSynchronize(st), "scene upload");

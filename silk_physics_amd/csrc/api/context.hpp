@@ -103,7 +103,7 @@ struct Scene {
   DevBuf<PcgSysDev> ps_sys;
   DevBuf<int> ps_slot_row, ps_sell_off, ps_sell_width, ps_sell_col, ps_slot_cn,
       ps_cb_bank, ps_cb_member, ps_cn_ptr, ps_cn_list, ps_abort;
-  DevBuf<float> ps_sell_val, ps_pbuf, ps_zbuf, ps_cr, ps_cz;
+  DevBuf<float> ps_sell_val, ps_pbuf, ps_zbuf, ps_cr;
   DevBuf<double> ps_part;
   DevBuf<unsigned> ps_bar;
   DevBuf<PcgState> ps_out;

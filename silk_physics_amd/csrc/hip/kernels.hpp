@@ -223,23 +223,30 @@ void launch_mas_apply(hipStream_t s, const PcgDev& p);
 
 // ------------------------------------------------- persistent PCG solve
 //
-// One cooperative launch runs a whole PCG solve (all iterations) for every
-// system.  Each system owns a group of workgroups; a workgroup owns up to
-// eight 32-slot level-0 MAS banks and keeps the rows of those banks (r, dx,
-// z, p, q) in registers for the whole solve.  Only z and p travel through
-// memory, for the neighbour reads of the SpMV, in "slot" order (banks padded
-// to 32 slots).  Workgroups of a group meet at three grid barriers per
-// iteration (p.Ap, level-1 coarse sums, r.z / r.r); the coarse levels are
-// solved redundantly by every workgroup of the group when they are small.
+// One launch runs a whole PCG solve (all iterations) for every system.  Each
+// system owns a group of workgroups; a workgroup owns up to eight 32-slot
+// level-0 MAS banks, keeps the rows of those banks (r, dx, z, p, q) in
+// registers and the inverses of its banks and of all coarse banks of its
+// system in LDS for the whole solve.  Only z, p, the per-node sums of q and
+// the dot-product partials travel through memory, written and read with
+// device-coherent (agent-scope) atomics.  Workgroups of a group meet at two
+// grid barriers per iteration (p.Ap; r.z and r.r); the level-1 residual sums
+// that the coarse levels need are kept current in every workgroup by the
+// recurrence cr -= alpha * (sum of q over the node).
 
-constexpr int kPcgThreads = 256;   // threads per workgroup (8 banks x 32)
+constexpr int kPcgThreads = 256;       // threads per workgroup (8 banks x 32)
 constexpr int kPcgBarrierStride = 64;  // unsigned words per system barrier
+constexpr int kPcgLdsBytes = 150 * 1024;  // dynamic LDS budget per workgroup
+constexpr int kPcgNbr = 32;        // neighbours kept in registers per row
+constexpr int kPcgNbrChunk = 8;    // slot-ELL widths are multiples of this
 
 struct PcgSysDev {
   int slot0 = 0, nslot = 0;  // padded slots (32 per level-0 bank)
   int bank0 = 0;             // global bank id of the first level-0 bank
   int cb0 = 0, ncb = 0;      // coarse banks in the persistent coarse list
   int cn0 = 0, ncn = 0;      // coarse nodes (system-local ids cn0 + local)
+  int n1 = 0;                // level-1 nodes (local ids [0, n1))
+  int lvl[5] = {0, 0, 0, 0, 0};  // level l nodes: local ids [lvl[l-1], lvl[l])
   int wg0 = 0, nwg = 0;      // workgroups of the group
 };
 
@@ -255,16 +262,15 @@ struct PersistDev {
   const int* slot_cn = nullptr;     // 4 per slot: coarse node (cn0-relative) per level
   const int* cb_bank = nullptr;     // per coarse bank: global bank id (bank_inv)
   const int* cb_member = nullptr;   // 32 per coarse bank: node (cn0-relative), -1
-  const int* cn_ptr = nullptr;      // per global coarse node: level-1 nodes summed
-  const int* cn_list = nullptr;     //   (cn0-relative ids)
+  const int* cn_ptr = nullptr;      // per global coarse node: its children at
+  const int* cn_list = nullptr;     //   the level below (cn0-relative ids)
   const float* bank_inv = nullptr;  // 1024 per bank
   const float* d = nullptr;         // diag per row
   const float* r0 = nullptr;        // FP32 initial residual per row
   float* x = nullptr;               // x += dx at the end (rows)
-  float* pbuf = nullptr;            // 2 x 3 per slot
-  float* zbuf = nullptr;            // 3 per slot
-  float* cr = nullptr;              // 3 per coarse node: level-1 sums
-  float* cz = nullptr;              // 3 per coarse node (distributed mode)
+  float* pbuf = nullptr;            // 2 x 4 floats per slot (xyz, pad)
+  float* zbuf = nullptr;            // 4 floats per slot
+  float* nsum = nullptr;            // 4 floats per coarse node: level-1 sums
   double* part = nullptr;           // 4 per workgroup
   unsigned* bar = nullptr;          // kPcgBarrierStride per system
   int* abort_flag = nullptr;
@@ -272,14 +278,19 @@ struct PersistDev {
   AdmmCtl* ctl = nullptr;           // optional: skip flag, tolerance, counters
   float abs_tol = 0.0f;
   int max_iter = 1000;
-  int coarse_lds = 0;   // 1: coarse levels solved per workgroup in LDS
-  int lds_nodes = 0;    // coarse nodes held in LDS (max over systems)
+  int n1_max = 0;       // max level-1 nodes of a system (LDS)
+  int ncn_max = 0;      // max coarse nodes of a system (LDS)
+  int ncb_max = 0;      // max coarse banks of a system (LDS inverses)
+  int cnl_max = 0;      // max coarse-node descendant list length (LDS)
   int grid = 0;         // total workgroups
 };
 
+/// Dynamic LDS bytes of the persistent kernel for the sizes in `a`.
+size_t pcg_persistent_lds(const PersistDev& a);
+
 /// True when the device can keep `grid` workgroups of the persistent kernel
-/// resident at once with `lds_nodes` coarse nodes in LDS.
-bool pcg_persistent_fits(int grid, int lds_nodes);
+/// resident at once with the given dynamic LDS.
+bool pcg_persistent_fits(int grid, size_t lds_bytes);
 
 /// One whole PCG solve (mas_cg_solver.cu:193-311) per system, x += solution.
 hipError_t launch_pcg_persistent(hipStream_t s, const PersistDev& a,
