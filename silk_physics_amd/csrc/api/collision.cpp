@@ -133,7 +133,7 @@ class HipCollision final : public CollisionWorld {
   std::vector<int> color_off_h_, color_list_h_;
   int n_active_ = 0;
   DevBuf<ContactDev> c_dev_;
-  DevBuf<int> color_off_, color_list_;
+  DevBuf<int> color_off_;
   DevBuf<float> corr_;
   int corr_cap_ = 0;
   DevBuf<float> w_, p_, p_prev_, eta_, dual_;
@@ -498,10 +498,13 @@ void HipCollision::recolor() {
 
 void HipCollision::upload_contacts(int n_old) {
   const int n = static_cast<int>(contacts_.size());
+  // Device copy in colour order (ids ascending inside a colour), so that a
+  // colour is one contiguous range.
   std::vector<ContactDev> hc(n);
-  for (int i = 0; i < n; ++i) {
+  for (int e = 0; e < n; ++e) {
+    const int i = color_list_h_[e];
     const HostContact& c = contacts_[i];
-    ContactDev& d = hc[i];
+    ContactDev& d = hc[e];
     for (int k = 0; k < 4; ++k) {
       d.state[k] = c.state[k];
       d.coef[k] = static_cast<float>(c.coef[k]);
@@ -512,11 +515,10 @@ void HipCollision::upload_contacts(int n_old) {
       d.affine[j] = static_cast<float>(c.affine[j]);
     }
     d.mu = static_cast<float>(c.mu);
-    d.pad = 0.0f;
+    d.id = i;
   }
   c_dev_.upload(hc, st_);
   color_off_.upload(color_off_h_, st_);
-  color_list_.upload(color_list_h_, st_);
   // Corrections persist across appends; new contacts start at zero.
   if (n > corr_cap_) {
     const int cap = std::max(1024, 2 * n);
@@ -644,7 +646,6 @@ void HipCollision::contact_update(const float* x, const int* skip) {
   cs.nv = nvc_;
   cs.c = c_dev_.get();
   cs.color_off = color_off_.get();
-  cs.color_list = color_list_.get();
   cs.corr = corr_.get();
   cs.w = w_.get();
   cs.p = p_.get();

@@ -691,79 +691,126 @@ __global__ void __launch_bounds__(1024) k_ccd_filter(CollVerts v, double* err6) 
 }
 
 // ---------------------------------------------------------------- contacts
-__global__ void __launch_bounds__(1024)
+constexpr int kContactThreads = 1024;
+constexpr size_t kContactLdsBytes = 150 * 1024;
+
+// Contact of round r for this thread (contacts are stored colour by colour;
+// rounds run through the colours once for the warm start and then once per
+// sweep).  Returns false when the thread has none in that round.
+__device__ __forceinline__ bool round_contact(const ContactSet& cs, int r,
+                                              int e_extra, ContactDev* out) {
+  const int col = r % cs.ncolor;
+  const int e = cs.color_off[col] + static_cast<int>(threadIdx.x) + e_extra;
+  if (e >= cs.color_off[col + 1]) return false;
+  *out = cs.c[e];
+  return true;
+}
+
+// One Gauss-Seidel projection of contact c (contact_constraints.cu:245-297)
+// against positions P and corrections C (LDS or global).
+__device__ __forceinline__ void gs_project(const ContactDev& c, float* P,
+                                           float* C) {
+  using namespace dev;
+  f3 u = {c.affine[0], c.affine[1], c.affine[2]};
+  for (int s = 0; s < 4; ++s)
+    if (c.state[s] >= 0) u = u + c.coef[s] * ld3(P, c.state[s]);
+  const f3 old = ld3(C, c.id);
+  const f3 us = u - old;
+  const f3 n = {c.normal[0], c.normal[1], c.normal[2]};
+  const float un = dot3(us, n);
+  f3 unew;
+  if (un >= 0.0f) {
+    unew = us;
+  } else {
+    const f3 ut = us - un * n;
+    const float utn = sqrtf(dot3(ut, ut));
+    const float rad = -c.mu * un;
+    unew = (utn <= rad) ? f3{0.0f, 0.0f, 0.0f} : (1.0f - rad / utn) * ut;
+  }
+  const f3 du = unew - u;
+  st3(C, c.id, old + du);
+  for (int s = 0; s < 4; ++s) {
+    if (c.gamma[s] == 0.0f) continue;
+    st3(P, c.state[s], ld3(P, c.state[s]) + c.gamma[s] * du);
+  }
+}
+
+__device__ __forceinline__ void warm_start(const ContactDev& c, float* P,
+                                           const float* C) {
+  using namespace dev;
+  const f3 corr = ld3(C, c.id);
+  for (int s = 0; s < 4; ++s) {
+    if (c.gamma[s] == 0.0f) continue;
+    st3(P, c.state[s], ld3(P, c.state[s]) + c.gamma[s] * corr);
+  }
+}
+
+template <bool kLds>
+__global__ void __launch_bounds__(kContactThreads)
     k_contact_update(ContactSet cs, const float* __restrict__ x, int nv) {
   using namespace dev;
+  extern __shared__ float sh[];
   if (cs.skip != nullptr && *cs.skip != 0) return;
+  float* P = kLds ? sh : cs.p;
+  float* C = kLds ? sh + 3 * static_cast<size_t>(nv) : cs.corr;
   // q = x - eta / w for contact vertices, p = x elsewhere
-  // (contact_constraints.cu:203-214); keep p_prev for the dual residual.
+  // (contact_constraints.cu:203-214); the previous p stays in cs.p (LDS
+  // case) or is copied to p_prev for the dual residual.
   for (int i = threadIdx.x; i < nv; i += blockDim.x) {
     const float w = cs.w[i];
     for (int d = 0; d < 3; ++d) {
       const int k = 3 * i + d;
-      cs.p_prev[k] = cs.p[k];
-      cs.p[k] = w > 0.0f ? x[k] - cs.eta[k] / w : x[k];
+      const float q = w > 0.0f ? x[k] - cs.eta[k] / w : x[k];
+      if (!kLds) cs.p_prev[k] = cs.p[k];
+      P[k] = q;
     }
   }
+  if (kLds)
+    for (int k = threadIdx.x; k < 3 * cs.n; k += blockDim.x) C[k] = cs.corr[k];
   __syncthreads();
-  // Warm start with the stored corrections, colour by colour
-  // (contact_constraints.cu:216-243).
-  for (int col = 0; col < cs.ncolor; ++col) {
-    for (int e = cs.color_off[col] + threadIdx.x; e < cs.color_off[col + 1];
-         e += blockDim.x) {
-      const int ci = cs.color_list[e];
-      const ContactDev& c = cs.c[ci];
-      const f3 corr = ld3(cs.corr, ci);
-      for (int s = 0; s < 4; ++s) {
-        if (c.gamma[s] == 0.0f) continue;
-        st3(cs.p, c.state[s], ld3(cs.p, c.state[s]) + c.gamma[s] * corr);
+  // Warm start (one pass over the colours), then the sweeps; the next
+  // round's contact is fetched before the barrier of the current one.
+  const int rounds = cs.ncolor * (1 + cs.sweeps);
+  ContactDev cur, nxt;
+  bool have = rounds > 0 && round_contact(cs, 0, 0, &cur);
+  for (int r = 0; r < rounds; ++r) {
+    const bool have_next = r + 1 < rounds && round_contact(cs, r + 1, 0, &nxt);
+    const bool warm = r < cs.ncolor;
+    if (have) {
+      if (warm) warm_start(cur, P, C);
+      else gs_project(cur, P, C);
+    }
+    // Colours larger than the workgroup: the remaining contacts of the
+    // round, without prefetch.
+    const int col = r % cs.ncolor;
+    for (int extra = kContactThreads;
+         cs.color_off[col] + extra < cs.color_off[col + 1]; extra += kContactThreads) {
+      ContactDev c;
+      if (round_contact(cs, r, extra, &c)) {
+        if (warm) warm_start(c, P, C);
+        else gs_project(c, P, C);
       }
     }
     __syncthreads();
-  }
-  // Coloured Gauss-Seidel sweeps (contact_constraints.cu:245-297).
-  for (int sw = 0; sw < cs.sweeps; ++sw) {
-    for (int col = 0; col < cs.ncolor; ++col) {
-      for (int e = cs.color_off[col] + threadIdx.x; e < cs.color_off[col + 1];
-           e += blockDim.x) {
-        const int ci = cs.color_list[e];
-        const ContactDev& c = cs.c[ci];
-        f3 u = {c.affine[0], c.affine[1], c.affine[2]};
-        for (int s = 0; s < 4; ++s)
-          if (c.state[s] >= 0) u = u + c.coef[s] * ld3(cs.p, c.state[s]);
-        const f3 old = ld3(cs.corr, ci);
-        const f3 us = u - old;
-        const f3 n = {c.normal[0], c.normal[1], c.normal[2]};
-        const float un = dot3(us, n);
-        f3 unew;
-        if (un >= 0.0f) {
-          unew = us;
-        } else {
-          const f3 ut = us - un * n;
-          const float utn = sqrtf(dot3(ut, ut));
-          const float rad = -c.mu * un;
-          unew = (utn <= rad) ? f3{0.0f, 0.0f, 0.0f} : (1.0f - rad / utn) * ut;
-        }
-        const f3 du = unew - u;
-        st3(cs.corr, ci, old + du);
-        for (int s = 0; s < 4; ++s) {
-          if (c.gamma[s] == 0.0f) continue;
-          st3(cs.p, c.state[s], ld3(cs.p, c.state[s]) + c.gamma[s] * du);
-        }
-      }
-      __syncthreads();
-    }
+    cur = nxt;
+    have = have_next;
   }
   // Multiplier update and dual residual terms (contact_constraints.cu:316-364).
   for (int i = threadIdx.x; i < nv; i += blockDim.x) {
     const float w = cs.w[i];
-    if (!(w > 0.0f)) continue;
     for (int d = 0; d < 3; ++d) {
       const int k = 3 * i + d;
-      cs.eta[k] += w * (cs.p[k] - x[k]);
-      cs.dual[k] = w * (cs.p[k] - cs.p_prev[k]);
+      const float pn = P[k];
+      const float po = kLds ? cs.p[k] : cs.p_prev[k];
+      if (kLds) cs.p[k] = pn;
+      if (w > 0.0f) {
+        cs.eta[k] += w * (pn - x[k]);
+        cs.dual[k] = w * (pn - po);
+      }
     }
   }
+  if (kLds)
+    for (int k = threadIdx.x; k < 3 * cs.n; k += blockDim.x) cs.corr[k] = C[k];
 }
 
 }  // namespace
@@ -812,10 +859,26 @@ void launch_ccd_each(hipStream_t s, int n, const int* is_vf, const double* p0,
                                              tolerance, max_iter, hit, toi);
 }
 
+bool contact_update_fits_lds(int nv, int n) {
+  return 12 * (static_cast<size_t>(nv) + static_cast<size_t>(n)) <= kContactLdsBytes;
+}
+
 void launch_contact_update(hipStream_t s, const ContactSet& cs,
                            const float* x) {
-  if (cs.n == 0) return;
-  k_contact_update<<<1, 1024, 0, s>>>(cs, x, cs.nv);
+  if (cs.n == 0 || cs.ncolor == 0) return;
+  if (contact_update_fits_lds(cs.nv, cs.n)) {
+    const size_t shmem = 12 * (static_cast<size_t>(cs.nv) + cs.n);
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_contact_update<true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(kContactLdsBytes));
+      attr = true;
+    }
+    k_contact_update<true><<<1, kContactThreads, shmem, s>>>(cs, x, cs.nv);
+  } else {
+    k_contact_update<false><<<1, kContactThreads, 0, s>>>(cs, x, cs.nv);
+  }
 }
 
 }  // namespace silk::amd

@@ -135,16 +135,15 @@ struct ContactDev {
   float normal[3];
   float affine[3];
   float mu;
-  float pad;
+  int id;           // contact id (index of its correction)
 };
 
 struct ContactSet {
   int n = 0;
   int ncolor = 0;
   int nv = 0;                       // cloth vertices
-  const ContactDev* c = nullptr;
-  const int* color_off = nullptr;   // ncolor + 1
-  const int* color_list = nullptr;  // contact ids grouped by colour
+  const ContactDev* c = nullptr;    // stored colour by colour
+  const int* color_off = nullptr;   // ncolor + 1 offsets into c
   float* corr = nullptr;            // 3 per contact
   const float* w = nullptr;         // nv, active weight (0 = inactive)
   float* p = nullptr;               // 3 nv
@@ -156,8 +155,12 @@ struct ContactSet {
 };
 
 /// One ADMM contact update (contact_constraints.cu:246-264): single
-/// workgroup, colours separated by workgroup barriers.
+/// workgroup, colours separated by workgroup barriers.  p and the
+/// corrections live in LDS for the whole update when they fit.
 void launch_contact_update(hipStream_t s, const ContactSet& cs,
                            const float* x);
+
+/// Largest (vertices, contacts) pair whose p and corrections fit in LDS.
+bool contact_update_fits_lds(int nv, int n);
 
 }  // namespace silk::amd

@@ -576,22 +576,26 @@ hipError_t launch_pcg_persistent(hipStream_t s, const PersistDev& a,
   float tol = rel_tol;
   void* args[] = {&arg, &tol};
   const size_t shmem = pcg_persistent_lds(a);
-  // SILK_HIP_COOP=0 launches the same grid without the cooperative flag
-  // (profilers that do not follow cooperative launches).  Residency is then
-  // not guaranteed by the runtime, but the grid was sized to fit and every
-  // barrier wait is bounded, so a failure is reported, never a hang.
-  static const bool coop = [] {
+  // The grid was sized to fit the device (pcg_persistent_fits) and every
+  // barrier wait is bounded, so a plain launch is safe as well: it is used
+  // when the cooperative one is refused (the runtime grants cooperative
+  // launches to one process of a device at a time) and with SILK_HIP_COOP=0
+  // (profilers that do not follow cooperative launches).  Without the
+  // runtime's residency guarantee a failure is reported, never a hang.
+  static const bool coop_env = [] {
     const char* e = std::getenv("SILK_HIP_COOP");
     return !(e != nullptr && e[0] == '0');
   }();
-  if (!coop) {
-    k_pcg_persistent<<<dim3(a.grid), dim3(kPcgThreads),
-                       static_cast<unsigned>(shmem), s>>>(arg, tol);
-    return hipGetLastError();
+  if (coop_env) {
+    const hipError_t e = hipLaunchCooperativeKernel(
+        reinterpret_cast<const void*>(k_pcg_persistent), dim3(a.grid),
+        dim3(kPcgThreads), args, static_cast<unsigned>(shmem), s);
+    if (e == hipSuccess) return e;
+    (void)hipGetLastError();
   }
-  return hipLaunchCooperativeKernel(
-      reinterpret_cast<const void*>(k_pcg_persistent), dim3(a.grid),
-      dim3(kPcgThreads), args, static_cast<unsigned>(shmem), s);
+  k_pcg_persistent<<<dim3(a.grid), dim3(kPcgThreads),
+                     static_cast<unsigned>(shmem), s>>>(arg, tol);
+  return hipGetLastError();
 }
 
 }  // namespace silk::amd

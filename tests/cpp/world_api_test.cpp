@@ -98,6 +98,7 @@ int main(int argc, char** argv) {
     r = w.solver_step();
     if (!r) break;
   }
+  if (!r) std::printf("step error: %s\n", r.to_string().c_str());
   check(static_cast<bool>(r), "steps");
   r = w.get_cloth_position(h, pos);
   bool fell = r && pos[2 * 3 + 2] < -1e-4f;  // unpinned bottom vertex falls
