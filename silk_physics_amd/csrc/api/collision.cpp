@@ -122,7 +122,7 @@ class HipCollision final : public CollisionWorld {
   int pt_cap_ = 1 << 16, ee_cap_ = 1 << 16;
   DevBuf<DcdRec> recs_;
   int rec_cap_ = 1 << 12;
-  DevBuf<int> counters_;  // pt, ee, dcd, pad
+  DevBuf<int> counters_;  // pt, ee, dcd, ccd work (zeroed per broadphase)
   DevBuf<unsigned long long> toi_bits_;
   DevBuf<double> err6_;
   int* h_io_ = nullptr;  // pinned: counters[4] then toi bits (8 bytes)
@@ -607,7 +607,7 @@ float HipCollision::min_toi(const float* outer_x, const float* inner_x,
     ctx_.timed_begin(kTimeCcd);
     launch_ccd(st_, verts(), tables(), pt_pairs_.get(), counters_.get(), pt_cap_,
                ee_pairs_.get(), counters_.get() + 1, ee_cap_, cp,
-               toi_bits_.get());
+               counters_.get() + 3, toi_bits_.get());
     ctx_.timed_end();
     read_io();
     ctx_.timed_collect();

@@ -5,6 +5,7 @@
 // once), find_collision.cu:45-92 (filters), dcd.cu:23-315 (DCD),
 // cpu/collision/interval_root_finder.cpp + ccd.cpp (the CCD restated here in
 // FP64), contact_constraints.cu:203-411 (Daviet Gauss-Seidel).
+#include <algorithm>
 #include <cfloat>
 
 #include "hip/collision_kernels.hpp"
@@ -376,8 +377,11 @@ __device__ void param_tolerance(bool vf, const double p0[4][3],
   tol[2] = clamp_div(dist, e1);
 }
 
-__device__ bool unit_eval(bool vf, const double p0[4][3], const double p1[4][3],
-                          double eps_ms, int d, bool* in_eps, double* width) {
+// Range of one coordinate of the distance function over the unit box: the
+// extremes of its values at the box corners (the unit-box evaluation of
+// interval_root_finder.cpp), independent of the minimum separation.
+__device__ void unit_range(bool vf, const double p0[4][3], const double p1[4][3],
+                           int d, double* mn_out, double* mx_out) {
   double mn, mx;
   if (vf) {
     const double D[6] = {p0[0][d] - p0[1][d], p0[0][d] - p0[2][d],
@@ -401,11 +405,8 @@ __device__ bool unit_eval(bool vf, const double p0[4][3], const double p1[4][3],
       if (mx < D[i]) mx = D[i];
     }
   }
-  *width = mx - mn;
-  *in_eps = false;
-  if (mn > eps_ms || mx < -eps_ms) return false;
-  if (mn >= -eps_ms && mx <= eps_ms) *in_eps = true;
-  return true;
+  *mn_out = mn;
+  *mx_out = mx;
 }
 
 __device__ bool box_eval(bool vf, const IBox& b, const double k[6][3],
@@ -456,18 +457,172 @@ __device__ bool box_eval(bool vf, const IBox& b, const double k[6][3],
   return true;
 }
 
-// interval_root_finder_bucket_DFS with a bounded pending list (kCcdStackCap); the
-// pending box with the smallest t.lower is processed first, ties going to
-// the most recently pushed one.
-// Not inlined: one copy of the pending list per lane instead of one per
-// call site keeps the scratch footprint at a single 3 KB stack.
-__device__ __noinline__ bool root_find(bool vf, const double p0[4][3],
-                                       const double p1[4][3],
-                                       const double tol[3], double co_tol,
-                                       const double err[3], double ms,
-                                       int max_iter, double max_time,
-                                       double* t_lo) {
-  double k[6][3];
+// Result of evaluating one box (a pure function of the box for a fixed query
+// and minimum separation): bit 0 the box may hold a root, bit 1 it meets the
+// stopping criterion (all widths within the tolerance, or inside the
+// separation band in every coordinate).
+constexpr int kBoxHit = 1;
+constexpr int kBoxSmall = 2;
+
+__device__ int box_flags(bool vf, const IBox& b, const double k[6][3],
+                         const double err[3], double ms, double co_tol) {
+  bool in_eps_all = true;
+  double width[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < 3; ++d) {
+    bool in_eps = false;
+    const double em = err[d] + ms;
+    if (!box_eval(vf, b, k, em, d, &in_eps, &width[d])) return 0;
+    in_eps_all = in_eps_all && in_eps;
+  }
+  const bool small = (width[0] <= co_tol && width[1] <= co_tol &&
+                      width[2] <= co_tol) || in_eps_all;
+  return small ? (kBoxHit | kBoxSmall) : kBoxHit;
+}
+
+__device__ int unit_flags(const double umn[3], const double umx[3],
+                          const double err[3], double ms, double co_tol) {
+  bool in_eps_all = true;
+  double width[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < 3; ++d) {
+    const double em = err[d] + ms;
+    width[d] = umx[d] - umn[d];
+    if (umn[d] > em || umx[d] < -em) return 0;
+    in_eps_all = in_eps_all && (umn[d] >= -em && umx[d] <= em);
+  }
+  const bool small = (width[0] <= co_tol && width[1] <= co_tol &&
+                      width[2] <= co_tol) || in_eps_all;
+  return small ? (kBoxHit | kBoxSmall) : kBoxHit;
+}
+
+__device__ __forceinline__ double read_lane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = static_cast<unsigned>(
+      __builtin_amdgcn_readlane(static_cast<int>(b), l));
+  const unsigned hi = static_cast<unsigned>(
+      __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l));
+  return __longlong_as_double(static_cast<long long>(
+      (static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+static_assert(kCcdStackCap == 64, "one pending box per lane of a wavefront");
+
+enum : int { kRootFound = 1, kRootNone = 0 };
+
+// interval_root_finder_bucket_DFS (interval_root_finder.cpp:391-510) run by
+// one wavefront.  The pending list (at most kCcdStackCap = 64 boxes) is
+// spread over the lanes: lane i holds one box, whether it is in use, its
+// evaluation (computed when it is pushed; it depends on the box alone) and
+// its rank in processing order.  The order is that of the sequential search:
+// smallest t.lower first, ties to the most recently pushed box.  A pop is a
+// ballot for rank 0 and a read of that lane; a push ranks the new box with
+// one ballot.  Results, including where the list overflows (the current
+// box's t.lower is returned: an earlier, conservative TOI) and the iteration
+// limit, are those of the sequential search.  All lanes must be active.
+// Lane-dependent updates are selects, not branches: a branch on the lane
+// lets the optimiser route lanes around the wavefront operations on their
+// own, which splits the wavefront.
+__device__ int wave_root_find(bool vf, const double umn[3], const double umx[3],
+                              const double k[6][3], const double tol[3],
+                              double co_tol, const double err[3], double ms,
+                              int max_iter, double* t_lo) {
+  const int lane = static_cast<int>(__lane_id());
+  bool valid = false;
+  int rank = 0;
+  int flags = 0;
+  IBox box;
+  for (int d = 0; d < 3; ++d) {
+    box.lo[d] = 0.0;
+    box.hi[d] = 1.0;
+  }
+  IBox cur = box;
+  int cur_flags = __builtin_amdgcn_readfirstlane(unit_flags(umn, umx, err, ms, co_tol));
+  int iter = 0;
+  for (;;) {
+    ++iter;
+    if (cur_flags & kBoxHit) {
+      if ((cur_flags & kBoxSmall) || (max_iter > 0 && iter > max_iter)) {
+        *t_lo = cur.lo[0];
+        return kRootFound;
+      }
+      // find_next_split: the largest width / tolerance among the widths
+      // above their tolerance.
+      int split = 0;
+      double best = -__builtin_inf();
+      for (int d = 0; d < 3; ++d) {
+        const double w = cur.hi[d] - cur.lo[d];
+        const double q = (w > tol[d]) ? w / tol[d] : -__builtin_inf();
+        if (q > best) {
+          best = q;
+          split = d;
+        }
+      }
+      double slo = cur.lo[0], shi = cur.hi[0];
+      for (int d = 1; d < 3; ++d)
+        if (split == d) {
+          slo = cur.lo[d];
+          shi = cur.hi[d];
+        }
+      const double mid = (shi + slo) / 2;
+      if (mid < 1e-12) {
+        *t_lo = cur.lo[0];
+        return kRootFound;
+      }
+      IBox second = cur, first = cur;
+      for (int d = 0; d < 3; ++d)
+        if (split == d) {
+          second.lo[d] = mid;
+          first.hi[d] = mid;
+        }
+      bool push_second = true, push_first = true;
+      if (vf && split != 0) {
+        const double other = (split == 1) ? cur.lo[2] : cur.lo[1];
+        push_second = mid + other <= 1.0;
+        push_first = slo + other <= 1.0;
+      }
+      const unsigned long long used = __ballot(valid);
+      const int need = static_cast<int>(push_second) + static_cast<int>(push_first);
+      if (__popcll(used) + need > kCcdStackCap) {
+        *t_lo = cur.lo[0];
+        return kRootFound;
+      }
+      unsigned long long free_lanes = ~used;
+      bool fresh = false;
+      // Push the second half, then the first (the most recent).
+      for (int c = 0; c < 2; ++c) {
+        if (!(c == 0 ? push_second : push_first)) continue;
+        const IBox& nb = c == 0 ? second : first;
+        const double key = nb.lo[0];
+        const int l = __builtin_ctzll(free_lanes);
+        free_lanes &= free_lanes - 1;
+        const bool below = box.lo[0] < key;
+        const int before = __popcll(__ballot(valid && below));
+        const bool me = lane == l;
+        rank = me ? before : rank + ((valid && !below) ? 1 : 0);
+        valid = valid || me;
+        fresh = fresh || me;
+        for (int d = 0; d < 3; ++d) {
+          box.lo[d] = me ? nb.lo[d] : box.lo[d];
+          box.hi[d] = me ? nb.hi[d] : box.hi[d];
+        }
+      }
+      const int f = box_flags(vf, box, k, err, ms, co_tol);
+      flags = fresh ? f : flags;
+    }
+    const unsigned long long head = __ballot(valid && rank == 0);
+    if (head == 0) return kRootNone;
+    const int l = __builtin_ctzll(head);
+    for (int d = 0; d < 3; ++d) {
+      cur.lo[d] = read_lane(box.lo[d], l);
+      cur.hi[d] = read_lane(box.hi[d], l);
+    }
+    cur_flags = __builtin_amdgcn_readlane(flags, l);
+    rank -= (valid && lane != l) ? 1 : 0;
+    valid = valid && lane != l;
+  }
+}
+
+__device__ void coefficients(bool vf, const double p0[4][3],
+                             const double p1[4][3], double k[6][3]) {
   for (int d = 0; d < 3; ++d) {
     if (vf) {
       k[0][d] = p0[0][d] - p0[1][d];
@@ -485,105 +640,6 @@ __device__ __noinline__ bool root_find(bool vf, const double p0[4][3],
       k[5][d] = p1[2][d] - p1[3][d];
     }
   }
-  IBox stack[kCcdStackCap];
-  int sp = 0;
-  IBox cur;
-  cur.lo[0] = 0.0; cur.hi[0] = max_time;
-  cur.lo[1] = 0.0; cur.hi[1] = 1.0;
-  cur.lo[2] = 0.0; cur.hi[2] = 1.0;
-  bool has_cur = true;
-  bool unit = (max_time == 1.0);
-  int iter = 0;
-  while (has_cur || sp > 0) {
-    if (!has_cur) {
-      int best = sp - 1;
-      for (int i = sp - 2; i >= 0; --i)
-        if (stack[i].lo[0] < stack[best].lo[0]) best = i;
-      cur = stack[best];
-      for (int i = best; i < sp - 1; ++i) stack[i] = stack[i + 1];
-      --sp;
-    }
-    has_cur = false;
-    ++iter;
-    bool in_eps_all = true;
-    bool hit = true;
-    double width[3] = {0.0, 0.0, 0.0};
-    for (int d = 0; d < 3; ++d) {
-      bool in_eps = false;
-      const double em = err[d] + ms;
-      const bool ok = unit ? unit_eval(vf, p0, p1, err[d] + ms, d, &in_eps, &width[d])
-                           : box_eval(vf, cur, k, em, d, &in_eps, &width[d]);
-      if (!ok) {
-        hit = false;
-        break;
-      }
-      in_eps_all = in_eps_all && in_eps;
-    }
-    unit = false;
-    if (!hit) continue;
-    const bool small = (width[0] <= co_tol && width[1] <= co_tol &&
-                        width[2] <= co_tol) || in_eps_all;
-    if (small || (max_iter > 0 && iter > max_iter)) {
-      *t_lo = cur.lo[0];
-      return true;
-    }
-    int split = 0;
-    double best = -__builtin_inf();
-    for (int d = 0; d < 3; ++d) {
-      const double w = cur.hi[d] - cur.lo[d];
-      const double s = (w > tol[d]) ? w / tol[d] : -__builtin_inf();
-      if (s > best) {
-        best = s;
-        split = d;
-      }
-    }
-    const double mid = (cur.hi[split] + cur.lo[split]) / 2;
-    if (mid < 1e-12) {
-      *t_lo = cur.lo[0];
-      return true;
-    }
-    IBox second = cur, first = cur;
-    second.lo[split] = mid;
-    first.hi[split] = mid;
-    bool push_second = true, push_first = true;
-    if (vf && split != 0) {
-      const double other = (split == 1) ? cur.lo[2] : cur.lo[1];
-      push_second = second.lo[split] + other <= 1.0;
-      push_first = first.lo[split] + other <= 1.0;
-    }
-    const int need = static_cast<int>(push_second) + static_cast<int>(push_first);
-    if (sp + need > kCcdStackCap) {
-      *t_lo = cur.lo[0];
-      return true;
-    }
-    if (push_second) stack[sp++] = second;
-    if (push_first) stack[sp++] = first;
-  }
-  return false;
-}
-
-__device__ bool ccd_query(bool vf, const double p0[4][3], const double p1[4][3],
-                          double ms, double tolerance, int max_iter,
-                          const double err[3], double* toi) {
-  double tol[3];
-  param_tolerance(vf, p0, p1, tolerance, tol);
-  double t;
-  if (!root_find(vf, p0, p1, tol, tolerance, err, ms, max_iter, 1.0, &t))
-    return false;
-  if (t > 0.0) {
-    *toi = t;
-    return true;
-  }
-  double t2;
-  if (!root_find(vf, p0, p1, tol, tolerance, err, ms * 0.5, max_iter, 1.0, &t2)) {
-    *toi = 1.0;
-    return true;
-  }
-  if (t2 > 0.0) {
-    *toi = t2;
-    return true;
-  }
-  return false;
 }
 
 __device__ void trajectory(const CollVerts& v, int gv, const CcdParams& cp,
@@ -601,48 +657,34 @@ __device__ void trajectory(const CollVerts& v, int gv, const CcdParams& cp,
   }
 }
 
-__global__ void __launch_bounds__(256)
-    k_ccd_each(int n, const int* __restrict__ is_vf,
-               const double* __restrict__ p0, const double* __restrict__ p1,
-               const double* __restrict__ err, double ms, double tolerance,
-               int max_iter, int* __restrict__ hit, double* __restrict__ toi) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double a[4][3], b[4][3];
-  for (int k = 0; k < 12; ++k) {
-    a[k / 3][k % 3] = p0[12 * i + k];
-    b[k / 3][k % 3] = p1[12 * i + k];
-  }
-  const double e[3] = {err[3 * i], err[3 * i + 1], err[3 * i + 2]};
-  double t = 0.0;
-  const bool h = ccd_query(is_vf[i] != 0, a, b, ms, tolerance, max_iter, e, &t);
-  hit[i] = h ? 1 : 0;
-  toi[i] = h ? t : 0.0;
-}
-
-#pragma clang fp contract(fast)
-
-__global__ void __launch_bounds__(256)
-    k_ccd(CollVerts v, PrimTables pt, const int2* __restrict__ ptp,
-          const int* __restrict__ ptc, int pt_cap, const int2* __restrict__ eep,
-          const int* __restrict__ eec, int ee_cap, CcdParams cp,
-          unsigned long long* toi_bits) {
-  const int npt = min(*ptc, pt_cap);
-  const int nee = min(*eec, ee_cap);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npt + nee;
-       i += gridDim.x * blockDim.x) {
+// Candidate pairs of a step: query q < npt is a point-triangle pair, the
+// rest edge-edge pairs (edges in canonical order by vertex key).
+struct PairQueries {
+  CollVerts v;
+  PrimTables pt;
+  const int2* ptp;
+  const int* ptc;
+  int pt_cap;
+  const int2* eep;
+  const int* eec;
+  int ee_cap;
+  CcdParams cp;
+  __device__ int count() const { return min(*ptc, pt_cap) + min(*eec, ee_cap); }
+  __device__ double ms() const { return cp.ms; }
+  __device__ void load(int q, bool* vf, double p0[4][3], double p1[4][3],
+                       double err[3]) const {
+    const int npt = min(*ptc, pt_cap);
     int gv[4];
-    bool vf;
-    if (i < npt) {
-      const int2 pr = ptp[i];
-      vf = true;
+    if (q < npt) {
+      const int2 pr = ptp[q];
+      *vf = true;
       gv[0] = pt.point[pr.y];
       gv[1] = pt.tri[3 * pr.x];
       gv[2] = pt.tri[3 * pr.x + 1];
       gv[3] = pt.tri[3 * pr.x + 2];
     } else {
-      const int2 pr = eep[i - npt];
-      vf = false;
+      const int2 pr = eep[q - npt];
+      *vf = false;
       int a0 = pt.edge[2 * pr.x], a1 = pt.edge[2 * pr.x + 1];
       int b0 = pt.edge[2 * pr.y], b1 = pt.edge[2 * pr.y + 1];
       const int ka0 = min(a0, a1), ka1 = max(a0, a1);
@@ -654,16 +696,100 @@ __global__ void __launch_bounds__(256)
       }
       gv[0] = a0; gv[1] = a1; gv[2] = b0; gv[3] = b1;
     }
-    double p0[4][3], p1[4][3];
-    for (int k = 0; k < 4; ++k) trajectory(v, gv[k], cp, p0[k], p1[k]);
-    double toi;
-    if (!ccd_query(vf, p0, p1, cp.ms, cp.tolerance, cp.max_iter,
-                   vf ? cp.err_vf : cp.err_ee, &toi))
-      continue;
-    if (toi == 0.0) continue;
+    for (int i = 0; i < 4; ++i) trajectory(v, gv[i], cp, p0[i], p1[i]);
+    const double* e = *vf ? cp.err_vf : cp.err_ee;
+    for (int d = 0; d < 3; ++d) err[d] = e[d];
+  }
+};
+
+// Explicit queries: 12 doubles per end position set, 3 filter values each.
+struct ExplicitQueries {
+  int n;
+  const int* is_vf;
+  const double* p0;
+  const double* p1;
+  const double* err;
+  double sep;
+  __device__ int count() const { return n; }
+  __device__ double ms() const { return sep; }
+  __device__ void load(int q, bool* vf, double a[4][3], double b[4][3],
+                       double e[3]) const {
+    *vf = is_vf[q] != 0;
+    for (int i = 0; i < 12; ++i) {
+      a[i / 3][i % 3] = p0[12 * static_cast<size_t>(q) + i];
+      b[i / 3][i % 3] = p1[12 * static_cast<size_t>(q) + i];
+    }
+    for (int d = 0; d < 3; ++d) e[d] = err[3 * static_cast<size_t>(q) + d];
+  }
+};
+
+// Outputs are written by every lane of the wavefront with the same values.
+// Earliest TOI over the candidates (toi == 0 is not a collision).
+struct MinToiOut {
+  unsigned long long* toi_bits;
+  __device__ void put(int, bool hit, double toi) const {
+    if (!hit || toi == 0.0) return;
     atomicMin(toi_bits, static_cast<unsigned long long>(__double_as_longlong(toi)));
   }
+};
+
+struct EachOut {
+  int* hit;
+  double* toi;
+  __device__ void put(int q, bool h, double t) const {
+    hit[q] = h ? 1 : 0;
+    toi[q] = h ? t : 0.0;
+  }
+};
+
+// CCD, one query per wavefront at a time; wavefronts take queries from the
+// zero-initialised counter *next.  Per query (ccd.cpp:235-256,
+// narrowphase.cu:70-76): the root search at ms; a root at t = 0 is searched
+// again at ms / 2, where no root means a collision at t = 1 and a root at
+// t = 0 means none.
+template <class In, class Out>
+__global__ void __launch_bounds__(256)
+    k_ccd(In in, Out out, int* next, double tolerance, int max_iter) {
+  const int n = in.count();
+  const int lane = static_cast<int>(__lane_id());
+  for (;;) {
+    // Every lane takes part (lane 0 adds one): the query index must not
+    // come from a branch on the lane.
+    const int q = __builtin_amdgcn_readfirstlane(atomicAdd(next, lane == 0 ? 1 : 0));
+    if (q >= n) break;
+    bool vf = false;
+    double p0[4][3], p1[4][3], err[3];
+    in.load(q, &vf, p0, p1, err);
+    vf = __builtin_amdgcn_readfirstlane(static_cast<int>(vf)) != 0;
+    double tol[3], k[6][3], umn[3], umx[3];
+    param_tolerance(vf, p0, p1, tolerance, tol);
+    coefficients(vf, p0, p1, k);
+    for (int d = 0; d < 3; ++d) unit_range(vf, p0, p1, d, &umn[d], &umx[d]);
+    const double ms = in.ms();
+    bool hit = false;
+    double toi = 0.0, t = 0.0;
+    if (wave_root_find(vf, umn, umx, k, tol, tolerance, err, ms, max_iter, &t) ==
+        kRootFound) {
+      if (t > 0.0) {
+        hit = true;
+        toi = t;
+      } else {
+        double t2 = 0.0;
+        if (wave_root_find(vf, umn, umx, k, tol, tolerance, err, ms * 0.5,
+                           max_iter, &t2) == kRootNone) {
+          hit = true;
+          toi = 1.0;
+        } else if (t2 > 0.0) {
+          hit = true;
+          toi = t2;
+        }
+      }
+    }
+    out.put(q, hit, toi);
+  }
 }
+
+#pragma clang fp contract(fast)
 
 __global__ void __launch_bounds__(1024) k_ccd_filter(CollVerts v, double* err6) {
   __shared__ float red[3][1024];
@@ -846,17 +972,20 @@ void launch_ccd_filter(hipStream_t s, const CollVerts& v, double* err6) {
 void launch_ccd(hipStream_t s, const CollVerts& v, const PrimTables& pt,
                 const int2* pt_pairs, const int* pt_count, int pt_cap,
                 const int2* ee_pairs, const int* ee_count, int ee_cap,
-                const CcdParams& cp, unsigned long long* toi_bits) {
-  k_ccd<<<1024, 256, 0, s>>>(v, pt, pt_pairs, pt_count, pt_cap, ee_pairs,
-                             ee_count, ee_cap, cp, toi_bits);
+                const CcdParams& cp, int* work, unsigned long long* toi_bits) {
+  PairQueries in{v, pt, pt_pairs, pt_count, pt_cap, ee_pairs, ee_count, ee_cap, cp};
+  k_ccd<<<kCcdBlocks, 256, 0, s>>>(in, MinToiOut{toi_bits}, work, cp.tolerance,
+                                   cp.max_iter);
 }
 
 void launch_ccd_each(hipStream_t s, int n, const int* is_vf, const double* p0,
                      const double* p1, const double* err, double ms,
-                     double tolerance, int max_iter, int* hit, double* toi) {
+                     double tolerance, int max_iter, int* work, int* hit,
+                     double* toi) {
   if (n <= 0) return;
-  k_ccd_each<<<(n + 255) / 256, 256, 0, s>>>(n, is_vf, p0, p1, err, ms,
-                                             tolerance, max_iter, hit, toi);
+  ExplicitQueries in{n, is_vf, p0, p1, err, ms};
+  const int blocks = std::min(kCcdBlocks, (n + 3) / 4);
+  k_ccd<<<blocks, 256, 0, s>>>(in, EachOut{hit, toi}, work, tolerance, max_iter);
 }
 
 bool contact_update_fits_lds(int nv, int n) {

@@ -107,23 +107,30 @@ struct CcdParams {
 /// VF / EE numerical filters (interval_root_finder.cpp:512-522) into err[6].
 void launch_ccd_filter(hipStream_t s, const CollVerts& v, double* err6);
 
+/// Pending-box capacity of one CCD query (overflow returns the current box's
+/// t.lower: an earlier, conservative TOI).  One box per lane of a wavefront.
+constexpr int kCcdStackCap = 64;
+
+/// Workgroups (of four wavefronts) of a CCD launch; wavefronts take queries
+/// from a shared counter until none are left.
+constexpr int kCcdBlocks = 1024;
+
 /// Earliest TOI over the candidates (local units, written as the bit pattern
 /// of a positive double into *toi_bits via atomicMin; initialise to 1.0).
+/// *work is the query counter and must be zero at launch.
 void launch_ccd(hipStream_t s, const CollVerts& v, const PrimTables& pt,
                 const int2* pt_pairs, const int* pt_count, int pt_cap,
                 const int2* ee_pairs, const int* ee_count, int ee_cap,
-                const CcdParams& cp, unsigned long long* toi_bits);
-
-/// Pending-box capacity of one CCD query (overflow returns the current box's
-/// t.lower: an earlier, conservative TOI).
-constexpr int kCcdStackCap = 64;
+                const CcdParams& cp, int* work, unsigned long long* toi_bits);
 
 /// Per-query CCD on explicit FP64 end positions (12 doubles each for p0 and
-/// p1, err 3 doubles per query): hit[i] = 1 and toi[i] as the step's CCD
-/// computes them (toi 0 reported as a hit with toi 0).
+/// p1, err 3 doubles per query) with the kernel of launch_ccd: hit[i] = 1 and
+/// toi[i] as the step's CCD computes them (a root at t = 0 at both
+/// separations is no hit).  *work must be zero at launch.
 void launch_ccd_each(hipStream_t s, int n, const int* is_vf, const double* p0,
                      const double* p1, const double* err, double ms,
-                     double tolerance, int max_iter, int* hit, double* toi);
+                     double tolerance, int max_iter, int* work, int* hit,
+                     double* toi);
 
 // ---------------------------------------------------------------- contacts
 
