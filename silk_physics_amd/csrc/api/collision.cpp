@@ -52,6 +52,38 @@ uint32_t spread10(uint32_t v) {
   return v;
 }
 
+// Primitives of nvert vertices each, ordered by the Morton code of their
+// centroid (10 bits per axis over the centroid bounds).
+std::vector<int> morton_order(const int* verts, int nprim, int nvert,
+                              const std::vector<float>& pos) {
+  float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+  std::vector<std::array<float, 3>> cen(nprim);
+  for (int i = 0; i < nprim; ++i) {
+    for (int d = 0; d < 3; ++d) {
+      float s = 0.0f;
+      for (int k = 0; k < nvert; ++k) s += pos[3 * verts[nvert * i + k] + d];
+      cen[i][d] = s / static_cast<float>(nvert);
+      lo[d] = std::min(lo[d], cen[i][d]);
+      hi[d] = std::max(hi[d], cen[i][d]);
+    }
+  }
+  std::vector<std::pair<uint32_t, int>> code(nprim);
+  for (int i = 0; i < nprim; ++i) {
+    uint32_t c = 0;
+    for (int d = 0; d < 3; ++d) {
+      const float ext = hi[d] - lo[d];
+      const float t = ext > 0.0f ? (cen[i][d] - lo[d]) / ext : 0.0f;
+      const uint32_t q = static_cast<uint32_t>(std::clamp(t * 1023.0f, 0.0f, 1023.0f));
+      c |= spread10(q) << (2 - d);
+    }
+    code[i] = {c, i};
+  }
+  std::sort(code.begin(), code.end());
+  std::vector<int> order(nprim);
+  for (int i = 0; i < nprim; ++i) order[i] = code[i].second;
+  return order;
+}
+
 // Daviet contact in double, as assembled on the host (contact_constraints.cu:
 // 23-150).
 struct HostContact {
@@ -116,6 +148,11 @@ class HipCollision final : public CollisionWorld {
   DevBuf<TreeDev> trees_dev_;
   DevBuf<int> tree_prim_, tree_prim_id_;
   DevBuf<Box6> sbox_, node_;
+  // Broadphase passes (object pairs, then self) in one launch.
+  DevBuf<int> query_order_;  // per object: point order, then edge order
+  std::vector<TraverseJob> jobs_h_;
+  DevBuf<TraverseJob> jobs_;
+  int job_items_ = 0;
 
   // candidates and outputs
   DevBuf<int2> pt_pairs_, ee_pairs_;
@@ -142,6 +179,7 @@ class HipCollision final : public CollisionWorld {
   PrimTables tables() const;
   void set_cloth(const float* a, const float* b);
   void refit();
+  void build_jobs(const std::vector<float>& pos0);
   void launch_broadphase();
   std::vector<DcdRec> dcd(float time, float dhat, silk_hip_step_stats* stats);
   void append(std::vector<DcdRec> recs);
@@ -195,29 +233,7 @@ void HipCollision::finish_setup(const std::vector<float>& pos0) {
   std::vector<Pending> pend;
   auto add_tree = [&](const int* verts, int nprim, int nvert, int gid0,
                       float pad) {
-    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
-    std::vector<std::array<float, 3>> cen(nprim);
-    for (int i = 0; i < nprim; ++i) {
-      for (int d = 0; d < 3; ++d) {
-        float s = 0.0f;
-        for (int k = 0; k < nvert; ++k) s += pos0[3 * verts[nvert * i + k] + d];
-        cen[i][d] = s / static_cast<float>(nvert);
-        lo[d] = std::min(lo[d], cen[i][d]);
-        hi[d] = std::max(hi[d], cen[i][d]);
-      }
-    }
-    std::vector<std::pair<uint32_t, int>> code(nprim);
-    for (int i = 0; i < nprim; ++i) {
-      uint32_t c = 0;
-      for (int d = 0; d < 3; ++d) {
-        const float ext = hi[d] - lo[d];
-        const float t = ext > 0.0f ? (cen[i][d] - lo[d]) / ext : 0.0f;
-        const uint32_t q = static_cast<uint32_t>(std::clamp(t * 1023.0f, 0.0f, 1023.0f));
-        c |= spread10(q) << (2 - d);
-      }
-      code[i] = {c, i};
-    }
-    std::sort(code.begin(), code.end());
+    const std::vector<int> order = morton_order(verts, nprim, nvert, pos0);
     Pending p;
     p.nprim = nprim;
     p.nvert = nvert;
@@ -228,7 +244,7 @@ void HipCollision::finish_setup(const std::vector<float>& pos0) {
     p.prim_off = prim.size();
     p.box_off = nbox;
     p.node_off = nnode;
-    for (const auto& [c, i] : code) {
+    for (const int i : order) {
       for (int k = 0; k < nvert; ++k) prim.push_back(verts[nvert * i + k]);
       prim_id.push_back(gid0 + i);
     }
@@ -264,6 +280,7 @@ void HipCollision::finish_setup(const std::vector<float>& pos0) {
     trees_.push_back(t);
   }
   trees_dev_.upload(trees_, st_);
+  build_jobs(pos0);
 
   pt_pairs_.resize(pt_cap_);
   ee_pairs_.resize(ee_cap_);
@@ -312,29 +329,69 @@ void HipCollision::refit() {
   launch_refit(st_, verts(), trees_dev_.get(), static_cast<int>(trees_.size()));
 }
 
-// Object pairs and self pairs of find_collision.cu:17-26,45-166.
-void HipCollision::launch_broadphase() {
-  hip_check(hipMemsetAsync(counters_.get(), 0, 4 * sizeof(int), st_), "counters");
-  const CollVerts v = verts();
-  PairOut pt{pt_pairs_.get(), counters_.get(), pt_cap_};
-  PairOut ee{ee_pairs_.get(), counters_.get() + 1, ee_cap_};
-  auto points = [&](const ObjDesc& o) {
+// Object pairs and self pairs of find_collision.cu:17-26,45-166: for each
+// pair of objects in one group with at least one physical, the triangle tree
+// of each against the other's points and the edge tree of the first against
+// the second's edges, gated by the two object boxes; then each physical
+// object with self-collision against itself.
+void HipCollision::build_jobs(const std::vector<float>& pos0) {
+  std::vector<int> order;
+  std::vector<size_t> point_off(objs_.size()), edge_off(objs_.size());
+  std::vector<int> ids;
+  for (size_t i = 0; i < objs_.size(); ++i) {
+    const ObjDesc& o = objs_[i];
+    ids.resize(o.nv);
+    for (int k = 0; k < o.nv; ++k) ids[k] = o.v0 + k;
+    point_off[i] = order.size();
+    const std::vector<int> po = morton_order(ids.data(), o.nv, 1, pos0);
+    order.insert(order.end(), po.begin(), po.end());
+    edge_off[i] = order.size();
+    const std::vector<int> eo = morton_order(
+        h_edge.data() + 2 * static_cast<size_t>(o.e0), o.ne, 2, pos0);
+    order.insert(order.end(), eo.begin(), eo.end());
+  }
+  if (order.empty()) order.push_back(0);
+  query_order_.upload(order, st_);
+  auto points = [&](size_t i) {
+    const ObjDesc& o = objs_[i];
     QuerySet q;
     q.n = o.nv;
     q.nvert = 1;
     q.prim = point_.get() + o.v0;
+    q.order = query_order_.get() + point_off[i];
     q.pad = o.pad;
     q.gid0 = o.v0;
     return q;
   };
-  auto edges = [&](const ObjDesc& o) {
+  auto edges = [&](size_t i) {
+    const ObjDesc& o = objs_[i];
     QuerySet q;
     q.n = o.ne;
     q.nvert = 2;
     q.prim = edge_.get() + 2 * static_cast<size_t>(o.e0);
+    q.order = query_order_.get() + edge_off[i];
     q.pad = o.pad;
     q.gid0 = o.e0;
     return q;
+  };
+  int items = 0;
+  auto add = [&](const TreeDev& t, const QuerySet& q, int kind, int self,
+                 const Box6* ga, const Box6* gb) {
+    if (q.n == 0 || t.nprim == 0) return;
+    TraverseJob j;
+    j.tree = t;
+    j.q = q;
+    j.kind = kind;
+    j.self = self;
+    int height = 0;
+    while ((1 << height) < t.P) ++height;
+    j.depth = std::min(kTraverseSplitDepth, height);
+    j.item0 = items;
+    j.nitem = q.n << j.depth;
+    j.gate_a = ga;
+    j.gate_b = gb;
+    items += (j.nitem + 63) / 64 * 64;
+    jobs_h_.push_back(j);
   };
   auto root = [&](const ObjDesc& o) { return trees_[o.tree_t].node + 1; };
   const size_t n = objs_.size();
@@ -344,15 +401,28 @@ void HipCollision::launch_broadphase() {
       const ObjDesc& b = objs_[j];
       if (a.group == -1 || b.group == -1 || a.group != b.group) continue;
       if (!(a.physical || b.physical)) continue;
-      launch_traverse(st_, v, trees_[a.tree_t], points(b), 0, 0, root(a), root(b), pt);
-      launch_traverse(st_, v, trees_[b.tree_t], points(a), 0, 0, root(a), root(b), pt);
-      launch_traverse(st_, v, trees_[a.tree_e], edges(b), 1, 0, root(a), root(b), ee);
+      add(trees_[a.tree_t], points(j), 0, 0, root(a), root(b));
+      add(trees_[b.tree_t], points(i), 0, 0, root(a), root(b));
+      add(trees_[a.tree_e], edges(j), 1, 0, root(a), root(b));
     }
-  for (const auto& o : objs_) {
+  for (size_t i = 0; i < n; ++i) {
+    const ObjDesc& o = objs_[i];
     if (!o.physical || !o.self_on) continue;
-    launch_traverse(st_, v, trees_[o.tree_t], points(o), 0, 1, nullptr, nullptr, pt);
-    launch_traverse(st_, v, trees_[o.tree_e], edges(o), 1, 1, nullptr, nullptr, ee);
+    add(trees_[o.tree_t], points(i), 0, 1, nullptr, nullptr);
+    add(trees_[o.tree_e], edges(i), 1, 1, nullptr, nullptr);
   }
+  job_items_ = items;
+  if (!jobs_h_.empty()) jobs_.upload(jobs_h_, st_);
+  // `order` is a temporary: drain the uploads before it goes.
+  hip_check(hipStreamSynchronize(st_), "broadphase jobs");
+}
+
+void HipCollision::launch_broadphase() {
+  hip_check(hipMemsetAsync(counters_.get(), 0, 4 * sizeof(int), st_), "counters");
+  PairOut pt{pt_pairs_.get(), counters_.get(), pt_cap_};
+  PairOut ee{ee_pairs_.get(), counters_.get() + 1, ee_cap_};
+  launch_traverse_jobs(st_, verts(), jobs_.get(), static_cast<int>(jobs_h_.size()),
+                       job_items_, pt, ee);
 }
 
 void HipCollision::read_io() {

@@ -87,14 +87,35 @@ __global__ void __launch_bounds__(kRefitThreads)
 }
 
 // ---------------------------------------------------------------- traversal
-__global__ void __launch_bounds__(256)
-    k_traverse(CollVerts v, TreeDev t, QuerySet q, int kind, int self,
-               const Box6* __restrict__ gate_a, const Box6* __restrict__ gate_b,
-               PairOut out) {
-  const int li = blockIdx.x * blockDim.x + threadIdx.x;
-  if (li >= q.n) return;
-  if (gate_a != nullptr && gate_b != nullptr && !overlap(*gate_a, *gate_b))
+constexpr int kTraverseThreads = 256;
+
+// All passes of a broadphase in one launch.  A work item is (job, subtree,
+// query): lane `item` walks the subtree rooted at heap node 2^depth + sub
+// for the query at position li of the job's Morton order.  The subtree walk
+// is stackless on the implicit heap (descend to the left child on overlap,
+// otherwise climb while on a right child and step to the right sibling).
+// Parent boxes contain their children, so the leaves reached over all
+// subtrees are the leaves of a walk from the root (oibvh.cuh:417-516).
+__global__ void __launch_bounds__(kTraverseThreads)
+    k_broadphase(CollVerts v, const TraverseJob* __restrict__ jobs, int njobs,
+                 PairOut pt, PairOut ee) {
+  // Job ranges are whole wavefronts: the job is uniform over the wavefront.
+  const int wave0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  int j = 0;
+  while (j + 1 < njobs && jobs[j + 1].item0 <= wave0) ++j;
+  j = __builtin_amdgcn_readfirstlane(j);
+  const TraverseJob& job = jobs[j];
+  const int local = wave0 + static_cast<int>(__lane_id()) - job.item0;
+  if (local >= job.nitem) return;
+  if (job.gate_a != nullptr && job.gate_b != nullptr &&
+      !overlap(*job.gate_a, *job.gate_b))
     return;
+  const QuerySet q = job.q;
+  const TreeDev t = job.tree;
+  const int kind = job.kind, self = job.self;
+  const PairOut out = kind == 0 ? pt : ee;
+  const int sub = local / q.n;
+  const int li = q.order[local - sub * q.n];
   const int qi = q.gid0 + li;
   int qv[2];
   for (int k = 0; k < q.nvert; ++k) qv[k] = q.prim[q.nvert * li + k];
@@ -102,39 +123,45 @@ __global__ void __launch_bounds__(256)
   swept_box(v, qv, q.nvert, q.pad, qb);
   float qinv = 0.0f;
   for (int k = 0; k < q.nvert; ++k) qinv += v.invm[qv[k]];
-  int stack[32];
-  int sp = 0;
-  stack[sp++] = 1;
-  while (sp > 0) {
-    const int node = stack[--sp];
-    if (!overlap(qb, t.node[node])) continue;
-    if (node >= t.P) {
-      const int j0 = (node - t.P) * kLeafPrims;
-      const int j1 = min(j0 + kLeafPrims, t.nprim);
-      for (int j = j0; j < j1; ++j) {
-        if (!overlap(qb, t.sbox[j])) continue;
-        int tv[3];
-        float inv = qinv;
-        for (int k = 0; k < t.nvert; ++k) {
-          tv[k] = t.prim[t.nvert * j + k];
-          inv += v.invm[tv[k]];
+  const int r = (1 << job.depth) + sub;
+  int node = r;
+  for (;;) {
+    bool down = false;
+    if (overlap(qb, t.node[node])) {
+      if (node < t.P) {
+        down = true;
+      } else {
+        const int j0 = (node - t.P) * kLeafPrims;
+        const int j1 = min(j0 + kLeafPrims, t.nprim);
+        for (int jj = j0; jj < j1; ++jj) {
+          if (!overlap(qb, t.sbox[jj])) continue;
+          int tv[3];
+          float inv = qinv;
+          for (int k = 0; k < t.nvert; ++k) {
+            tv[k] = t.prim[t.nvert * jj + k];
+            inv += v.invm[tv[k]];
+          }
+          if (inv == 0.0f) continue;  // all pinned / kinematic
+          const int tid_orig = t.prim_id[jj];
+          if (self) {
+            bool adjacent = false;
+            for (int a = 0; a < t.nvert; ++a)
+              for (int b = 0; b < q.nvert; ++b) adjacent |= (tv[a] == qv[b]);
+            if (adjacent) continue;
+            if (kind == 1 && !(tid_orig < qi)) continue;  // each pair once
+          }
+          const int slot = atomicAdd(out.count, 1);
+          if (slot < out.cap) out.pairs[slot] = make_int2(tid_orig, qi);
         }
-        if (inv == 0.0f) continue;  // all pinned / kinematic
-        const int tid_orig = t.prim_id[j];
-        if (self) {
-          bool adjacent = false;
-          for (int a = 0; a < t.nvert; ++a)
-            for (int b = 0; b < q.nvert; ++b) adjacent |= (tv[a] == qv[b]);
-          if (adjacent) continue;
-          if (kind == 1 && !(tid_orig < qi)) continue;  // each pair once
-        }
-        const int slot = atomicAdd(out.count, 1);
-        if (slot < out.cap) out.pairs[slot] = make_int2(tid_orig, qi);
       }
-    } else {
-      stack[sp++] = 2 * node + 1;
-      stack[sp++] = 2 * node;
     }
+    if (down) {
+      node <<= 1;
+      continue;
+    }
+    while (node != r && (node & 1)) node >>= 1;
+    if (node == r) break;
+    ++node;
   }
 }
 
@@ -947,12 +974,11 @@ void launch_refit(hipStream_t s, const CollVerts& v, const TreeDev* trees_dev,
   k_refit<<<ntrees, kRefitThreads, 0, s>>>(v, trees_dev);
 }
 
-void launch_traverse(hipStream_t s, const CollVerts& v, const TreeDev& tree,
-                     const QuerySet& q, int kind, int self, const Box6* gate_a,
-                     const Box6* gate_b, PairOut out) {
-  if (q.n == 0 || tree.nprim == 0) return;
-  k_traverse<<<(q.n + 255) / 256, 256, 0, s>>>(v, tree, q, kind, self, gate_a,
-                                               gate_b, out);
+void launch_traverse_jobs(hipStream_t s, const CollVerts& v, const TraverseJob* jobs,
+                          int njobs, int nitems, PairOut pt, PairOut ee) {
+  if (njobs <= 0 || nitems <= 0) return;
+  k_broadphase<<<(nitems + kTraverseThreads - 1) / kTraverseThreads,
+                 kTraverseThreads, 0, s>>>(v, jobs, njobs, pt, ee);
 }
 
 void launch_dcd(hipStream_t s, const CollVerts& v, const PrimTables& pt,

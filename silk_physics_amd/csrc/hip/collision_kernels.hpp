@@ -44,6 +44,7 @@ struct QuerySet {
   int n = 0;
   int nvert = 1;                 // 1 point, 2 edge
   const int* prim = nullptr;     // nvert * n global vertex ids
+  const int* order = nullptr;    // n: visiting order of the queries (Morton)
   float pad = 0.0f;
   int gid0 = 0;                  // global primitive id of query 0
 };
@@ -56,13 +57,32 @@ struct PairOut {
   int cap = 0;
 };
 
+/// One tree x query-set pass of a broadphase.
 /// kind 0: triangle tree x points (PT), 1: edge tree x edges (EE).
 /// self: same object (adjacency filter; EE keeps tree id < query id only).
 /// gate_a / gate_b: object boxes (triangle tree roots); when both are given
-/// and do not overlap the launch reports nothing (find_collision.cu:45-92).
-void launch_traverse(hipStream_t s, const CollVerts& v, const TreeDev& tree,
-                     const QuerySet& q, int kind, int self, const Box6* gate_a,
-                     const Box6* gate_b, PairOut out);
+/// and do not overlap the pass reports nothing (find_collision.cu:45-92).
+/// Each query is walked by 2^depth lanes, one per subtree at that depth;
+/// the job's work items start at item0 (a multiple of the wavefront size).
+struct TraverseJob {
+  TreeDev tree;
+  QuerySet q;
+  int kind = 0;
+  int self = 0;
+  int depth = 0;
+  int item0 = 0;
+  int nitem = 0;   // q.n << depth
+  const Box6* gate_a = nullptr;
+  const Box6* gate_b = nullptr;
+};
+
+/// Subtree split depth of a traversal job (clamped to the tree height).
+constexpr int kTraverseSplitDepth = 3;
+
+/// All passes of a broadphase in one launch (nitems: end of the last job's
+/// padded item range).  Pairs go to pt (kind 0) or ee (kind 1).
+void launch_traverse_jobs(hipStream_t s, const CollVerts& v, const TraverseJob* jobs,
+                          int njobs, int nitems, PairOut pt, PairOut ee);
 
 /// DCD record of one active contact (dcd.cu:102-261), global vertex ids.
 struct DcdRec {
