@@ -233,6 +233,16 @@ int silk_hip_debug_ccd(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
                        double ms, double tolerance, int32_t max_iter,
                        int32_t* hit, double* toi);
 
+/* The step's earliest-TOI CCD kernel on the same explicit queries: the
+ * smallest toi over the hits with toi > 0, or 1 when there is none (the
+ * reduction of find_collision.cu:142-190 and narrowphase.cu:32-80 over one
+ * query list; queries that cannot lower the running minimum are cut short,
+ * which leaves the minimum unchanged). */
+int silk_hip_debug_ccd_min(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
+                           const double* p0, const double* p1,
+                           const double* err, double ms, double tolerance,
+                           int32_t max_iter, double* min_toi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,8 @@ def load_library():
                                                 C.POINTER(i64)]),
         "silk_hip_debug_ccd": (C.c_int, [vp, i64, ip, dp, dp, dp, C.c_double,
                                          C.c_double, i32, ip, dp]),
+        "silk_hip_debug_ccd_min": (C.c_int, [vp, i64, ip, dp, dp, dp, C.c_double,
+                                             C.c_double, i32, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -482,6 +484,23 @@ class World:
         if st != 0:
             raise RuntimeError(self._wrap(st).to_string())
         return hit, toi
+
+    def debug_ccd_min(self, is_vf, p0, p1, err, ms, tolerance, max_iter=4096):
+        """The step's earliest-TOI CCD kernel on explicit queries."""
+        v = np.ascontiguousarray(is_vf, dtype=np.int32).reshape(-1)
+        a = np.ascontiguousarray(p0, dtype=np.float64).reshape(-1)
+        b = np.ascontiguousarray(p1, dtype=np.float64).reshape(-1)
+        e = np.ascontiguousarray(err, dtype=np.float64).reshape(-1)
+        n = v.size
+        assert a.size == 12 * n and b.size == 12 * n and e.size == 3 * n
+        out = C.c_double(0.0)
+        dp = C.POINTER(C.c_double)
+        st = load_library().silk_hip_debug_ccd_min(
+            self._ctx, n, _iptr(v), a.ctypes.data_as(dp), b.ctypes.data_as(dp),
+            e.ctypes.data_as(dp), ms, tolerance, max_iter, C.byref(out))
+        if st != 0:
+            raise RuntimeError(self._wrap(st).to_string())
+        return out.value
 
     def timing(self, cls: int, reset: bool = False) -> tuple[float, int]:
         ms = C.c_double(0)

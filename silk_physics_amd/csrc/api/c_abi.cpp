@@ -441,4 +441,43 @@ int silk_hip_debug_ccd(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
   });
 }
 
+int silk_hip_debug_ccd_min(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
+                           const double* p0, const double* p1,
+                           const double* err, double ms, double tolerance,
+                           int32_t max_iter, double* min_toi) {
+  return guarded(ctx, [&](Context& c) {
+    if (n < 0 || n > (1 << 24)) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    if (min_toi == nullptr) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    *min_toi = 1.0;
+    if (n == 0) return int{SILK_HIP_OK};
+    if (!is_vf || !p0 || !p1 || !err) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    c.ensure_device();
+    using silk::amd::DevBuf;
+    const size_t m = static_cast<size_t>(n);
+    std::vector<int> hv(is_vf, is_vf + m);
+    std::vector<double> h0(p0, p0 + 12 * m), h1(p1, p1 + 12 * m),
+        he(err, err + 3 * m), one{1.0};
+    DevBuf<int> dv, dwork;
+    DevBuf<double> d0, d1, de, dmin;
+    dwork.resize(1);
+    silk::amd::hip_check(hipMemsetAsync(dwork.get(), 0, sizeof(int), c.stream_),
+                         "ccd work");
+    dv.upload(hv, c.stream_);
+    d0.upload(h0, c.stream_);
+    d1.upload(h1, c.stream_);
+    de.upload(he, c.stream_);
+    dmin.upload(one, c.stream_);
+    silk::amd::launch_ccd_min(
+        c.stream_, static_cast<int>(n), dv.get(), d0.get(), d1.get(), de.get(),
+        ms, tolerance, max_iter, dwork.get(),
+        reinterpret_cast<unsigned long long*>(dmin.get()));
+    silk::amd::hip_check(hipGetLastError(), "ccd launch");
+    silk::amd::hip_check(hipMemcpyAsync(min_toi, dmin.get(), sizeof(double),
+                                        hipMemcpyDeviceToHost, c.stream_),
+                         "ccd min");
+    silk::amd::hip_check(hipStreamSynchronize(c.stream_), "ccd sync");
+    return int{SILK_HIP_OK};
+  });
+}
+
 }  // extern "C"

@@ -436,52 +436,45 @@ __device__ void unit_range(bool vf, const double p0[4][3], const double p1[4][3]
   *mx_out = mx;
 }
 
-__device__ bool box_eval(bool vf, const IBox& b, const double k[6][3],
-                         double eps_ms, int d, bool* in_eps, double* width) {
-  const double c0 = k[0][d], c1 = k[1][d], c2 = k[2][d], c3 = k[3][d],
-               c4 = k[4][d], c5 = k[5][d];
-  const double ulo = b.lo[1], uhi = b.hi[1], vlo = b.lo[2], vhi = b.hi[2];
-  double lo_t[2], hi_t[2];
-  for (int e = 0; e < 2; ++e) {
-    const double t = e == 0 ? b.lo[0] : b.hi[0];
-    const double q0 = (c1 - c0) * t + c0;
-    const double q1 = (c3 - c2) * t + c2;
-    const double q2 = (c5 - c4) * t + c4;
-    double lo, hi;
-    if (!vf || uhi + vhi <= 1.0) {
-      const double a = q1 * ulo, bb = q1 * uhi, c = q2 * vlo, dd = q2 * vhi;
-      lo = q0 + dmin(a, bb) + dmin(c, dd);
-      hi = q0 + dmax(a, bb) + dmax(c, dd);
-    } else {
-      lo = __builtin_inf();
-      hi = -__builtin_inf();
-      auto inc = [&](double u, double v) {
-        const double val = q0 + q1 * u + q2 * v;
-        lo = dmin(lo, val);
-        hi = dmax(hi, val);
-      };
-      if (ulo + vlo <= 1.0) inc(ulo, vlo);
-      if (ulo + vhi <= 1.0) inc(ulo, vhi);
-      if (uhi + vlo <= 1.0) inc(uhi, vlo);
-      if (uhi + vhi <= 1.0) inc(uhi, vhi);
-      const double v_at_ulo = 1.0 - ulo;
-      if (v_at_ulo >= vlo && v_at_ulo <= vhi) inc(ulo, v_at_ulo);
-      const double v_at_uhi = 1.0 - uhi;
-      if (v_at_uhi >= vlo && v_at_uhi <= vhi) inc(uhi, v_at_uhi);
-      const double u_at_vlo = 1.0 - vlo;
-      if (u_at_vlo >= ulo && u_at_vlo <= uhi) inc(u_at_vlo, vlo);
-      const double u_at_vhi = 1.0 - vhi;
-      if (u_at_vhi >= ulo && u_at_vhi <= uhi) inc(u_at_vhi, vhi);
-    }
-    lo_t[e] = lo;
-    hi_t[e] = hi;
+// Range [lo, hi] of one coordinate of the distance function over the
+// (u, v) rectangle of a box at one time t (the box evaluation of
+// interval_root_finder.cpp, one end of its t interval); on a VF query the
+// rectangle is clipped to u + v <= 1.  c0..c5: that coordinate's affine
+// coefficients.
+__device__ void end_range(bool vf, double t, double ulo, double uhi, double vlo,
+                          double vhi, const double c[6], double* lo_out,
+                          double* hi_out) {
+  const double q0 = (c[1] - c[0]) * t + c[0];
+  const double q1 = (c[3] - c[2]) * t + c[2];
+  const double q2 = (c[5] - c[4]) * t + c[4];
+  double lo, hi;
+  if (!vf || uhi + vhi <= 1.0) {
+    const double a = q1 * ulo, bb = q1 * uhi, cc = q2 * vlo, dd = q2 * vhi;
+    lo = q0 + dmin(a, bb) + dmin(cc, dd);
+    hi = q0 + dmax(a, bb) + dmax(cc, dd);
+  } else {
+    lo = __builtin_inf();
+    hi = -__builtin_inf();
+    auto inc = [&](double u, double v) {
+      const double val = q0 + q1 * u + q2 * v;
+      lo = dmin(lo, val);
+      hi = dmax(hi, val);
+    };
+    if (ulo + vlo <= 1.0) inc(ulo, vlo);
+    if (ulo + vhi <= 1.0) inc(ulo, vhi);
+    if (uhi + vlo <= 1.0) inc(uhi, vlo);
+    if (uhi + vhi <= 1.0) inc(uhi, vhi);
+    const double v_at_ulo = 1.0 - ulo;
+    if (v_at_ulo >= vlo && v_at_ulo <= vhi) inc(ulo, v_at_ulo);
+    const double v_at_uhi = 1.0 - uhi;
+    if (v_at_uhi >= vlo && v_at_uhi <= vhi) inc(uhi, v_at_uhi);
+    const double u_at_vlo = 1.0 - vlo;
+    if (u_at_vlo >= ulo && u_at_vlo <= uhi) inc(u_at_vlo, vlo);
+    const double u_at_vhi = 1.0 - vhi;
+    if (u_at_vhi >= ulo && u_at_vhi <= uhi) inc(u_at_vhi, vhi);
   }
-  const double mn = dmin(lo_t[0], lo_t[1]), mx = dmax(hi_t[0], hi_t[1]);
-  *in_eps = false;
-  if (mn > eps_ms || mx < -eps_ms) return false;
-  *width = mx - mn;
-  if (mn >= -eps_ms && mx <= eps_ms) *in_eps = true;
-  return true;
+  *lo_out = lo;
+  *hi_out = hi;
 }
 
 // Result of evaluating one box (a pure function of the box for a fixed query
@@ -490,21 +483,6 @@ __device__ bool box_eval(bool vf, const IBox& b, const double k[6][3],
 // separation band in every coordinate).
 constexpr int kBoxHit = 1;
 constexpr int kBoxSmall = 2;
-
-__device__ int box_flags(bool vf, const IBox& b, const double k[6][3],
-                         const double err[3], double ms, double co_tol) {
-  bool in_eps_all = true;
-  double width[3] = {0.0, 0.0, 0.0};
-  for (int d = 0; d < 3; ++d) {
-    bool in_eps = false;
-    const double em = err[d] + ms;
-    if (!box_eval(vf, b, k, em, d, &in_eps, &width[d])) return 0;
-    in_eps_all = in_eps_all && in_eps;
-  }
-  const bool small = (width[0] <= co_tol && width[1] <= co_tol &&
-                      width[2] <= co_tol) || in_eps_all;
-  return small ? (kBoxHit | kBoxSmall) : kBoxHit;
-}
 
 __device__ int unit_flags(const double umn[3], const double umx[3],
                           const double err[3], double ms, double co_tol) {
@@ -531,9 +509,47 @@ __device__ __forceinline__ double read_lane(double v, int l) {
       (static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
+// a, b or c for i = 0, 1, 2, by masks on the bit patterns (a select chain on
+// a lane-dependent index is compiled into an indexed private-memory load).
+__device__ __forceinline__ double pick3(int i, double a, double b, double c) {
+  const unsigned long long ma = 0ull - static_cast<unsigned long long>(i == 0);
+  const unsigned long long mb = 0ull - static_cast<unsigned long long>(i == 1);
+  const unsigned long long mc = 0ull - static_cast<unsigned long long>(i == 2);
+  return __longlong_as_double(static_cast<long long>(
+      (static_cast<unsigned long long>(__double_as_longlong(a)) & ma) |
+      (static_cast<unsigned long long>(__double_as_longlong(b)) & mb) |
+      (static_cast<unsigned long long>(__double_as_longlong(c)) & mc)));
+}
+
+// Value of the neighbouring lane (lanes 2i and 2i + 1 swap; DPP quad_perm
+// [1, 0, 3, 2]).  All lanes must be active.
+__device__ __forceinline__ double swap_pair(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = static_cast<unsigned>(
+      __builtin_amdgcn_update_dpp(0, static_cast<int>(b), 0xB1, 0xF, 0xF, false));
+  const unsigned hi = static_cast<unsigned>(
+      __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), 0xB1, 0xF, 0xF, false));
+  return __longlong_as_double(static_cast<long long>(
+      (static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
 static_assert(kCcdStackCap == 64, "one pending box per lane of a wavefront");
 
-enum : int { kRootFound = 1, kRootNone = 0 };
+enum : int { kRootFound = 1, kRootNone = 0, kRootPruned = 2 };
+
+// Read by every lane at once; taken from the first lane so that the value
+// (and the branch on it) is uniform over the wavefront.
+__device__ __forceinline__ double load_cutoff(const unsigned long long* bits) {
+  const unsigned long long v = __hip_atomic_load(
+      const_cast<unsigned long long*>(bits), __ATOMIC_RELAXED,
+      __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned lo = static_cast<unsigned>(
+      __builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+  const unsigned hi = static_cast<unsigned>(
+      __builtin_amdgcn_readfirstlane(static_cast<int>(v >> 32)));
+  return __longlong_as_double(static_cast<long long>(
+      (static_cast<unsigned long long>(hi) << 32) | lo));
+}
 
 // interval_root_finder_bucket_DFS (interval_root_finder.cpp:391-510) run by
 // one wavefront.  The pending list (at most kCcdStackCap = 64 boxes) is
@@ -542,30 +558,60 @@ enum : int { kRootFound = 1, kRootNone = 0 };
 // its rank in processing order.  The order is that of the sequential search:
 // smallest t.lower first, ties to the most recently pushed box.  A pop is a
 // ballot for rank 0 and a read of that lane; a push ranks the new box with
-// one ballot.  Results, including where the list overflows (the current
-// box's t.lower is returned: an earlier, conservative TOI) and the iteration
-// limit, are those of the sequential search.  All lanes must be active.
+// one ballot.  The two halves of a split box are evaluated by twelve lanes
+// at once, one per (half, coordinate, end of the t interval); a pair swap
+// joins the two ends and three ballots give each half's flags.  Results,
+// including where the list overflows (the current box's t.lower is
+// returned: an earlier, conservative TOI) and the iteration limit, are those
+// of the sequential search.  All lanes must be active.
 // Lane-dependent updates are selects, not branches: a branch on the lane
 // lets the optimiser route lanes around the wavefront operations on their
 // own, which splits the wavefront.
+// cutoff (optional): the bit pattern of a running minimum TOI.  Popped
+// t.lower values never decrease (a half's t.lower is at least its parent's)
+// and the search returns the t.lower of a popped box, so once the popped
+// t.lower reaches the minimum the result cannot lower it: kRootPruned.  The
+// minimum is re-read every 32 passes.
 __device__ int wave_root_find(bool vf, const double umn[3], const double umx[3],
                               const double k[6][3], const double tol[3],
                               double co_tol, const double err[3], double ms,
-                              int max_iter, double* t_lo) {
+                              int max_iter, const unsigned long long* cutoff,
+                              double* t_lo) {
   const int lane = static_cast<int>(__lane_id());
+  // This lane's part of a split evaluation: half h, coordinate d, end e
+  // (lanes 12 and up repeat lanes 0..11 and are ignored).
+  const int part = lane % 12;
+  const int h = part / 6, d = (part % 6) / 2, e = part & 1;
+  double kd[6];
+  for (int i = 0; i < 6; ++i) kd[i] = pick3(d, k[i][0], k[i][1], k[i][2]);
+  const double em = pick3(d, err[0], err[1], err[2]) + ms;
+  const bool lead = lane < 12 && e == 0;
   bool valid = false;
   int rank = 0;
   int flags = 0;
+  double key = 0.0;  // t.lower of this lane's box
   IBox box;
-  for (int d = 0; d < 3; ++d) {
-    box.lo[d] = 0.0;
-    box.hi[d] = 1.0;
+  for (int i = 0; i < 3; ++i) {
+    box.lo[i] = 0.0;
+    box.hi[i] = 1.0;
   }
   IBox cur = box;
   int cur_flags = __builtin_amdgcn_readfirstlane(unit_flags(umn, umx, err, ms, co_tol));
+  double cut = __builtin_inf(), cut_next = __builtin_inf();
+  if (cutoff != nullptr) {
+    cut = load_cutoff(cutoff);
+    cut_next = cut;
+  }
   int iter = 0;
   for (;;) {
     ++iter;
+    if (cutoff != nullptr) {
+      if ((iter & 31) == 0) {
+        cut = cut_next;
+        cut_next = load_cutoff(cutoff);
+      }
+      if (cur.lo[0] >= cut) return kRootPruned;
+    }
     if (cur_flags & kBoxHit) {
       if ((cur_flags & kBoxSmall) || (max_iter > 0 && iter > max_iter)) {
         *t_lo = cur.lo[0];
@@ -575,30 +621,30 @@ __device__ int wave_root_find(bool vf, const double umn[3], const double umx[3],
       // above their tolerance.
       int split = 0;
       double best = -__builtin_inf();
-      for (int d = 0; d < 3; ++d) {
-        const double w = cur.hi[d] - cur.lo[d];
-        const double q = (w > tol[d]) ? w / tol[d] : -__builtin_inf();
+      for (int i = 0; i < 3; ++i) {
+        const double w = cur.hi[i] - cur.lo[i];
+        const double q = (w > tol[i]) ? w / tol[i] : -__builtin_inf();
         if (q > best) {
           best = q;
-          split = d;
+          split = i;
         }
       }
       double slo = cur.lo[0], shi = cur.hi[0];
-      for (int d = 1; d < 3; ++d)
-        if (split == d) {
-          slo = cur.lo[d];
-          shi = cur.hi[d];
+      for (int i = 1; i < 3; ++i)
+        if (split == i) {
+          slo = cur.lo[i];
+          shi = cur.hi[i];
         }
       const double mid = (shi + slo) / 2;
       if (mid < 1e-12) {
         *t_lo = cur.lo[0];
         return kRootFound;
       }
-      IBox second = cur, first = cur;
-      for (int d = 0; d < 3; ++d)
-        if (split == d) {
-          second.lo[d] = mid;
-          first.hi[d] = mid;
+      IBox half[2] = {cur, cur};  // second (upper) half, first (lower) half
+      for (int i = 0; i < 3; ++i)
+        if (split == i) {
+          half[0].lo[i] = mid;
+          half[1].hi[i] = mid;
         }
       bool push_second = true, push_first = true;
       if (vf && split != 0) {
@@ -612,35 +658,59 @@ __device__ int wave_root_find(bool vf, const double umn[3], const double umx[3],
         *t_lo = cur.lo[0];
         return kRootFound;
       }
+      // Evaluate both halves: lane (h, d, e) takes coordinate d at one end
+      // of half h's t interval; the even lane of each pair joins the ends.
+      int f[2];
+      {
+        double blo[3], bhi[3];
+        for (int i = 0; i < 3; ++i) {
+          blo[i] = h == 0 ? half[0].lo[i] : half[1].lo[i];
+          bhi[i] = h == 0 ? half[0].hi[i] : half[1].hi[i];
+        }
+        double lo, hi;
+        end_range(vf, e == 0 ? blo[0] : bhi[0], blo[1], bhi[1], blo[2], bhi[2],
+                  kd, &lo, &hi);
+        const double lo_o = swap_pair(lo), hi_o = swap_pair(hi);
+        const double mn = dmin(lo, lo_o), mx = dmax(hi, hi_o);
+        const bool ok = !(mn > em || mx < -em);
+        const bool in_eps = mn >= -em && mx <= em;
+        const bool thin = (mx - mn) <= co_tol;
+        const unsigned long long okm = __ballot(lead && ok);
+        const unsigned long long epsm = __ballot(lead && in_eps);
+        const unsigned long long thinm = __ballot(lead && thin);
+        for (int c = 0; c < 2; ++c) {
+          const unsigned long long m = 0x15ull << (6 * c);
+          const bool hit = (okm & m) == m;
+          const bool small = (thinm & m) == m || (epsm & m) == m;
+          f[c] = hit ? (small ? (kBoxHit | kBoxSmall) : kBoxHit) : 0;
+        }
+      }
       unsigned long long free_lanes = ~used;
-      bool fresh = false;
       // Push the second half, then the first (the most recent).
       for (int c = 0; c < 2; ++c) {
         if (!(c == 0 ? push_second : push_first)) continue;
-        const IBox& nb = c == 0 ? second : first;
-        const double key = nb.lo[0];
+        const double nkey = half[c].lo[0];
         const int l = __builtin_ctzll(free_lanes);
         free_lanes &= free_lanes - 1;
-        const bool below = box.lo[0] < key;
+        const bool below = key < nkey;
         const int before = __popcll(__ballot(valid && below));
         const bool me = lane == l;
         rank = me ? before : rank + ((valid && !below) ? 1 : 0);
         valid = valid || me;
-        fresh = fresh || me;
-        for (int d = 0; d < 3; ++d) {
-          box.lo[d] = me ? nb.lo[d] : box.lo[d];
-          box.hi[d] = me ? nb.hi[d] : box.hi[d];
+        flags = me ? f[c] : flags;
+        key = me ? nkey : key;
+        for (int i = 0; i < 3; ++i) {
+          box.lo[i] = me ? half[c].lo[i] : box.lo[i];
+          box.hi[i] = me ? half[c].hi[i] : box.hi[i];
         }
       }
-      const int f = box_flags(vf, box, k, err, ms, co_tol);
-      flags = fresh ? f : flags;
     }
     const unsigned long long head = __ballot(valid && rank == 0);
     if (head == 0) return kRootNone;
     const int l = __builtin_ctzll(head);
-    for (int d = 0; d < 3; ++d) {
-      cur.lo[d] = read_lane(box.lo[d], l);
-      cur.hi[d] = read_lane(box.hi[d], l);
+    for (int i = 0; i < 3; ++i) {
+      cur.lo[i] = read_lane(box.lo[i], l);
+      cur.hi[i] = read_lane(box.hi[i], l);
     }
     cur_flags = __builtin_amdgcn_readlane(flags, l);
     rank -= (valid && lane != l) ? 1 : 0;
@@ -751,9 +821,11 @@ struct ExplicitQueries {
 };
 
 // Outputs are written by every lane of the wavefront with the same values.
-// Earliest TOI over the candidates (toi == 0 is not a collision).
+// Earliest TOI over the candidates (toi == 0 is not a collision).  Queries
+// that cannot lower it are cut short.
 struct MinToiOut {
   unsigned long long* toi_bits;
+  __device__ const unsigned long long* cutoff() const { return toi_bits; }
   __device__ void put(int, bool hit, double toi) const {
     if (!hit || toi == 0.0) return;
     atomicMin(toi_bits, static_cast<unsigned long long>(__double_as_longlong(toi)));
@@ -763,6 +835,7 @@ struct MinToiOut {
 struct EachOut {
   int* hit;
   double* toi;
+  __device__ const unsigned long long* cutoff() const { return nullptr; }
   __device__ void put(int q, bool h, double t) const {
     hit[q] = h ? 1 : 0;
     toi[q] = h ? t : 0.0;
@@ -793,17 +866,24 @@ __global__ void __launch_bounds__(256)
     coefficients(vf, p0, p1, k);
     for (int d = 0; d < 3; ++d) unit_range(vf, p0, p1, d, &umn[d], &umx[d]);
     const double ms = in.ms();
-    bool hit = false;
+    const unsigned long long* cut = out.cutoff();
+    bool hit = false, pruned = false;
     double toi = 0.0, t = 0.0;
-    if (wave_root_find(vf, umn, umx, k, tol, tolerance, err, ms, max_iter, &t) ==
-        kRootFound) {
+    const int r1 = wave_root_find(vf, umn, umx, k, tol, tolerance, err, ms,
+                                  max_iter, cut, &t);
+    if (r1 == kRootPruned) {
+      pruned = true;
+    } else if (r1 == kRootFound) {
       if (t > 0.0) {
         hit = true;
         toi = t;
       } else {
         double t2 = 0.0;
-        if (wave_root_find(vf, umn, umx, k, tol, tolerance, err, ms * 0.5,
-                           max_iter, &t2) == kRootNone) {
+        const int r2 = wave_root_find(vf, umn, umx, k, tol, tolerance, err,
+                                      ms * 0.5, max_iter, cut, &t2);
+        if (r2 == kRootPruned) {
+          pruned = true;
+        } else if (r2 == kRootNone) {
           hit = true;
           toi = 1.0;
         } else if (t2 > 0.0) {
@@ -812,7 +892,7 @@ __global__ void __launch_bounds__(256)
         }
       }
     }
-    out.put(q, hit, toi);
+    if (!pruned) out.put(q, hit, toi);
   }
 }
 
@@ -860,14 +940,14 @@ __device__ __forceinline__ bool round_contact(const ContactSet& cs, int r,
 }
 
 // One Gauss-Seidel projection of contact c (contact_constraints.cu:245-297)
-// against positions P and corrections C (LDS or global).
-__device__ __forceinline__ void gs_project(const ContactDev& c, float* P,
-                                           float* C) {
+// against positions P; corr is the contact's correction, updated in place.
+__device__ __forceinline__ void gs_core(const ContactDev& c, float* P,
+                                        dev::f3& corr) {
   using namespace dev;
   f3 u = {c.affine[0], c.affine[1], c.affine[2]};
   for (int s = 0; s < 4; ++s)
     if (c.state[s] >= 0) u = u + c.coef[s] * ld3(P, c.state[s]);
-  const f3 old = ld3(C, c.id);
+  const f3 old = corr;
   const f3 us = u - old;
   const f3 n = {c.normal[0], c.normal[1], c.normal[2]};
   const float un = dot3(us, n);
@@ -881,21 +961,33 @@ __device__ __forceinline__ void gs_project(const ContactDev& c, float* P,
     unew = (utn <= rad) ? f3{0.0f, 0.0f, 0.0f} : (1.0f - rad / utn) * ut;
   }
   const f3 du = unew - u;
-  st3(C, c.id, old + du);
+  corr = old + du;
   for (int s = 0; s < 4; ++s) {
     if (c.gamma[s] == 0.0f) continue;
     st3(P, c.state[s], ld3(P, c.state[s]) + c.gamma[s] * du);
   }
 }
 
-__device__ __forceinline__ void warm_start(const ContactDev& c, float* P,
-                                           const float* C) {
+// Corrections C in LDS or global memory, indexed by contact id.
+__device__ __forceinline__ void gs_project(const ContactDev& c, float* P,
+                                           float* C) {
+  dev::f3 corr = dev::ld3(C, c.id);
+  gs_core(c, P, corr);
+  dev::st3(C, c.id, corr);
+}
+
+__device__ __forceinline__ void warm_core(const ContactDev& c, float* P,
+                                          const dev::f3& corr) {
   using namespace dev;
-  const f3 corr = ld3(C, c.id);
   for (int s = 0; s < 4; ++s) {
     if (c.gamma[s] == 0.0f) continue;
     st3(P, c.state[s], ld3(P, c.state[s]) + c.gamma[s] * corr);
   }
+}
+
+__device__ __forceinline__ void warm_start(const ContactDev& c, float* P,
+                                           const float* C) {
+  warm_core(c, P, dev::ld3(C, c.id));
 }
 
 template <bool kLds>
@@ -966,6 +1058,79 @@ __global__ void __launch_bounds__(kContactThreads)
     for (int k = threadIdx.x; k < 3 * cs.n; k += blockDim.x) cs.corr[k] = C[k];
 }
 
+// Register-resident variant: contact e (colour order) and its correction
+// live in thread e % kContactThreads for the whole update, so the rounds
+// touch LDS only and their barriers wait on LDS traffic alone (a barrier
+// also drains outstanding global loads, which made every round pay one).
+// Contacts of one colour are independent, so a thread holding two of the
+// same colour may project them in either order.
+constexpr int kContactSlots = 4;
+
+__global__ void __launch_bounds__(kContactThreads)
+    k_contact_update_reg(ContactSet cs, const float* __restrict__ x, int nv) {
+  using namespace dev;
+  extern __shared__ float sh[];
+  if (cs.skip != nullptr && *cs.skip != 0) return;
+  float* P = sh;
+  const int tid = static_cast<int>(threadIdx.x);
+  for (int i = tid; i < nv; i += blockDim.x) {
+    const float w = cs.w[i];
+    for (int d = 0; d < 3; ++d) {
+      const int k = 3 * i + d;
+      P[k] = w > 0.0f ? x[k] - cs.eta[k] / w : x[k];
+    }
+  }
+  ContactDev c[kContactSlots];
+  f3 corr[kContactSlots];
+  int color[kContactSlots];
+  for (int s = 0; s < kContactSlots; ++s) {
+    const int e = s * kContactThreads + tid;
+    color[s] = -1;
+    corr[s] = f3{0.0f, 0.0f, 0.0f};
+    if (e < cs.n) {
+      c[s] = cs.c[e];
+      corr[s] = ld3(cs.corr, c[s].id);
+      int lo = 0, hi = cs.ncolor;  // color_off[lo] <= e < color_off[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (cs.color_off[mid] <= e) lo = mid;
+        else hi = mid;
+      }
+      color[s] = lo;
+    }
+  }
+  __syncthreads();
+  // Warm start (one pass over the colours), then the sweeps.
+  const int rounds = cs.ncolor * (1 + cs.sweeps);
+  int col = 0;
+  for (int r = 0; r < rounds; ++r) {
+    const bool warm = r < cs.ncolor;
+    for (int s = 0; s < kContactSlots; ++s) {
+      if (color[s] != col) continue;
+      if (warm) warm_core(c[s], P, corr[s]);
+      else gs_core(c[s], P, corr[s]);
+    }
+    __syncthreads();
+    if (++col == cs.ncolor) col = 0;
+  }
+  // Multiplier update and dual residual terms (contact_constraints.cu:316-364).
+  for (int i = tid; i < nv; i += blockDim.x) {
+    const float w = cs.w[i];
+    for (int d = 0; d < 3; ++d) {
+      const int k = 3 * i + d;
+      const float pn = P[k];
+      const float po = cs.p[k];
+      cs.p[k] = pn;
+      if (w > 0.0f) {
+        cs.eta[k] += w * (pn - x[k]);
+        cs.dual[k] = w * (pn - po);
+      }
+    }
+  }
+  for (int s = 0; s < kContactSlots; ++s)
+    if (color[s] >= 0) st3(cs.corr, c[s].id, corr[s]);
+}
+
 }  // namespace
 
 void launch_refit(hipStream_t s, const CollVerts& v, const TreeDev* trees_dev,
@@ -1014,6 +1179,16 @@ void launch_ccd_each(hipStream_t s, int n, const int* is_vf, const double* p0,
   k_ccd<<<blocks, 256, 0, s>>>(in, EachOut{hit, toi}, work, tolerance, max_iter);
 }
 
+void launch_ccd_min(hipStream_t s, int n, const int* is_vf, const double* p0,
+                    const double* p1, const double* err, double ms,
+                    double tolerance, int max_iter, int* work,
+                    unsigned long long* toi_bits) {
+  if (n <= 0) return;
+  ExplicitQueries in{n, is_vf, p0, p1, err, ms};
+  const int blocks = std::min(kCcdBlocks, (n + 3) / 4);
+  k_ccd<<<blocks, 256, 0, s>>>(in, MinToiOut{toi_bits}, work, tolerance, max_iter);
+}
+
 bool contact_update_fits_lds(int nv, int n) {
   return 12 * (static_cast<size_t>(nv) + static_cast<size_t>(n)) <= kContactLdsBytes;
 }
@@ -1021,7 +1196,17 @@ bool contact_update_fits_lds(int nv, int n) {
 void launch_contact_update(hipStream_t s, const ContactSet& cs,
                            const float* x) {
   if (cs.n == 0 || cs.ncolor == 0) return;
-  if (contact_update_fits_lds(cs.nv, cs.n)) {
+  const size_t p_bytes = 12 * static_cast<size_t>(cs.nv);
+  if (cs.n <= kContactSlots * kContactThreads && p_bytes <= kContactLdsBytes) {
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_contact_update_reg),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(kContactLdsBytes));
+      attr = true;
+    }
+    k_contact_update_reg<<<1, kContactThreads, p_bytes, s>>>(cs, x, cs.nv);
+  } else if (contact_update_fits_lds(cs.nv, cs.n)) {
     const size_t shmem = 12 * (static_cast<size_t>(cs.nv) + cs.n);
     static bool attr = false;
     if (!attr) {

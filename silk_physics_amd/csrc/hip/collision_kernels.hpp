@@ -152,6 +152,14 @@ void launch_ccd_each(hipStream_t s, int n, const int* is_vf, const double* p0,
                      double tolerance, int max_iter, int* work, int* hit,
                      double* toi);
 
+/// Earliest TOI over explicit queries with the kernel of launch_ccd
+/// (queries that cannot lower the running minimum are cut short); *toi_bits
+/// starts at 1.0 and *work at zero.
+void launch_ccd_min(hipStream_t s, int n, const int* is_vf, const double* p0,
+                    const double* p1, const double* err, double ms,
+                    double tolerance, int max_iter, int* work,
+                    unsigned long long* toi_bits);
+
 // ---------------------------------------------------------------- contacts
 
 /// Daviet contact (contact_constraints.cuh:19-42).

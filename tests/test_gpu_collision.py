@@ -60,6 +60,41 @@ def test_ccd_kernel_bit_exact_vs_oracle(ms):
     np.testing.assert_array_equal(toi, otoi)  # bit for bit
 
 
+def _oracle_each(vf, p0, p1, err, ms):
+    toi = np.zeros(vf.size)
+    hit = np.zeros(vf.size, dtype=bool)
+    for i in range(vf.size):
+        h, t = O.ccd(bool(vf[i]), p0[i].reshape(4, 3), p1[i].reshape(4, 3), ms, TOL,
+                     err[i], stack_cap=64)
+        hit[i], toi[i] = h, (t if h else 0.0)
+    return hit, toi
+
+
+@pytest.mark.parametrize("order", ["as_drawn", "earliest_first", "earliest_last"])
+def test_ccd_min_kernel_equals_oracle_minimum(order):
+    """The step's earliest-TOI kernel cuts queries short once they cannot
+    lower the running minimum; the minimum must still be the oracle's, bit
+    for bit, whatever the order the queries come in."""
+    w = gpu_world()
+    vf, p0, p1, err = _queries(3000, seed=7)
+    hit, toi = _oracle_each(vf, p0, p1, err, MS)
+    pos = hit & (toi > 0)
+    assert pos.sum() > 100
+    key = np.where(pos, toi, 2.0)
+    if order == "earliest_first":
+        perm = np.argsort(key, kind="stable")
+    elif order == "earliest_last":
+        perm = np.argsort(-key, kind="stable")
+    else:
+        perm = np.arange(vf.size)
+    expect = float(toi[pos].min())
+    got = w.debug_ccd_min(vf[perm], p0[perm], p1[perm], err[perm], MS, TOL)
+    assert got == expect, (got, expect)
+    # Only non-colliding queries left: the minimum stays at 1.
+    miss = ~hit
+    assert w.debug_ccd_min(vf[miss], p0[miss], p1[miss], err[miss], MS, TOL) == 1.0
+
+
 def test_ccd_kernel_closed_forms():
     w = gpu_world()
     tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], dtype=np.float64)
