@@ -399,7 +399,8 @@ void Context::build_scene() {
     }
     // Persistent-PCG view of this system: slots, system-local coarse ids
     // (level 1 first, then deeper levels), coarse banks and, per coarse
-    // node, the level-1 nodes whose residual sums make up its own.
+    // node of level >= 2, the level-1 nodes whose residual sums make up its
+    // own.
     {
       const auto& po = cloths_[ci].part.offsets;
       const int nb0 = t.level_banks[0];
@@ -436,13 +437,13 @@ void Context::build_scene() {
           }
         }
       sd.ncb = static_cast<int>(ps_cb_bank.size()) - sd.cb0;
-      // Direct children of every coarse node of level >= 2 (level-1 nodes
+      // Level-1 nodes below every coarse node of level >= 2 (level-1 nodes
       // take their sums from the residual itself).
       std::vector<std::vector<int>> desc(sd.ncn);
       for (int lr = 0; lr < t.n; ++lr)
         for (int l = 2; l <= t.coarse_levels; ++l)
-          desc[lvl_off[l] + t.cmap[l - 1][lr]].push_back(lvl_off[l - 1] +
-                                                          t.cmap[l - 2][lr]);
+          desc[lvl_off[l] + t.cmap[l - 1][lr]].push_back(lvl_off[1] +
+                                                          t.cmap[0][lr]);
       for (int l = 0; l < 5; ++l)
         sd.lvl[l] = l + 1 <= t.coarse_levels + 1 ? lvl_off[l + 1] : sd.ncn;
       for (auto& dsc : desc) {
@@ -505,6 +506,70 @@ void Context::build_scene() {
           ps_sell_col[e] = s < nslot ? s : 0;
           ps_sell_val[e] = 0.0f;
         }
+      }
+    }
+  }
+  // Workgroup of every slot (the kernel's bank split), the other
+  // workgroups' slots each one reads (its halo) and every neighbour as an
+  // index into the workgroup's slots [0, kPcgThreads) then its halo.
+  std::vector<int> ps_halo_ptr{0}, ps_halo_slot, ps_sell_lcol(ps_sell_col.size(), 0);
+  int nhalo_max = 0, nwg_max = 0;
+  bool xch_fits = true;
+  {
+    const int nslot = static_cast<int>(ps_slot_row.size());
+    std::vector<int> slot_wg(nslot, -1), slot_local(nslot, 0);
+    for (const auto& sd : ps_sys) {
+      nwg_max = std::max(nwg_max, sd.nwg);
+      const int nb0 = sd.nslot / kBankSlots;
+      for (int j = 0; j < sd.nwg; ++j) {
+        const int lo = sd.slot0 + kBankSlots * ((j * nb0) / sd.nwg);
+        const int hi = sd.slot0 + kBankSlots * (((j + 1) * nb0) / sd.nwg);
+        for (int sl = lo; sl < hi; ++sl) {
+          slot_wg[sl] = sd.wg0 + j;
+          slot_local[sl] = sl - lo;
+        }
+      }
+    }
+    std::vector<std::vector<int>> halo(ps_wg);
+    auto for_entries = [&](auto&& fn) {
+      for (size_t si = 0; si < ps_sell_off.size(); ++si) {
+        const int s0 = static_cast<int>(si) * kSlice;
+        for (int lane = 0; lane < kSlice && s0 + lane < nslot; ++lane)
+          for (int k = 0; k < ps_sell_width[si]; ++k)
+            fn(s0 + lane, static_cast<size_t>(ps_sell_off[si]) +
+                              static_cast<size_t>(k) * kSlice + lane);
+      }
+    };
+    for_entries([&](int sl, size_t e) {
+      const int c = ps_sell_col[e];
+      if (slot_wg[c] != slot_wg[sl]) halo[slot_wg[sl]].push_back(c);
+    });
+    for (auto& hl : halo) {
+      std::sort(hl.begin(), hl.end());
+      hl.erase(std::unique(hl.begin(), hl.end()), hl.end());
+      nhalo_max = std::max(nhalo_max, static_cast<int>(hl.size()));
+      ps_halo_slot.insert(ps_halo_slot.end(), hl.begin(), hl.end());
+      ps_halo_ptr.push_back(static_cast<int>(ps_halo_slot.size()));
+    }
+    for_entries([&](int sl, size_t e) {
+      const int c = ps_sell_col[e];
+      const int w = slot_wg[sl];
+      if (slot_wg[c] == w) {
+        ps_sell_lcol[e] = slot_local[c];
+      } else {
+        const auto& hl = halo[w];
+        ps_sell_lcol[e] = kPcgThreads + static_cast<int>(
+            std::lower_bound(hl.begin(), hl.end(), c) - hl.begin());
+      }
+    });
+    // Tagged words of the two exchanges of an iteration, per workgroup.
+    const int cap = kPcgThreads * kPcgXchWords;
+    for (const auto& sd : ps_sys) {
+      if (3 * sd.n1 + 2 * sd.nwg > cap) xch_fits = false;
+      for (int j = 0; j < sd.nwg; ++j) {
+        const int w = sd.wg0 + j;
+        const int nh = ps_halo_ptr[w + 1] - ps_halo_ptr[w];
+        if (4 * sd.nwg + 3 * nh > cap) xch_fits = false;
       }
     }
   }
@@ -608,7 +673,9 @@ void Context::build_scene() {
     up(S.ps_slot_row, ps_slot_row);
     up(S.ps_sell_off, ps_sell_off);
     up(S.ps_sell_width, ps_sell_width);
-    up(S.ps_sell_col, ps_sell_col);
+    up(S.ps_sell_lcol, ps_sell_lcol);
+    up(S.ps_halo_ptr, ps_halo_ptr);
+    up(S.ps_halo_slot, ps_halo_slot);
     if (ps_sell_val.empty()) ps_sell_val.push_back(0.0f);
     S.ps_sell_val.upload(ps_sell_val, st);
     up(S.ps_slot_cn, ps_slot_cn);
@@ -617,10 +684,12 @@ void Context::build_scene() {
     up(S.ps_cn_ptr, ps_cn_ptr);
     up(S.ps_cn_list, ps_cn_list);
     const int ncn_all = static_cast<int>(ps_cn_ptr.size()) - 1;
-    S.ps_pbuf.resize(8 * static_cast<size_t>(std::max(1, nslot)));
-    S.ps_zbuf.resize(4 * static_cast<size_t>(std::max(1, nslot)));
-    S.ps_cr.resize(4 * static_cast<size_t>(std::max(1, ncn_all)));
-    S.ps_part.resize(4 * static_cast<size_t>(std::max(1, ps_wg)));
+    S.ps_tz.resize(3 * static_cast<size_t>(std::max(1, nslot)));
+    S.ps_tz.zero(st);
+    S.ps_tn.resize(3 * static_cast<size_t>(std::max(1, ncn_all)));
+    S.ps_tn.zero(st);
+    S.ps_tp.resize(8 * static_cast<size_t>(std::max(1, ps_wg)));
+    S.ps_tp.zero(st);
     S.ps_bar.resize(kPcgBarrierStride * static_cast<size_t>(std::max(1, S.nsys)));
     S.ps_bar.zero(st);
     S.ps_abort.resize(1);
@@ -634,7 +703,9 @@ void Context::build_scene() {
     a.slot_row = S.ps_slot_row.get();
     a.sell_off = S.ps_sell_off.get();
     a.sell_width = S.ps_sell_width.get();
-    a.sell_col = S.ps_sell_col.get();
+    a.sell_lcol = S.ps_sell_lcol.get();
+    a.halo_ptr = S.ps_halo_ptr.get();
+    a.halo_slot = S.ps_halo_slot.get();
     a.sell_val = S.ps_sell_val.get();
     a.slot_cn = S.ps_slot_cn.get();
     a.cb_bank = S.ps_cb_bank.get();
@@ -645,10 +716,9 @@ void Context::build_scene() {
     a.d = S.d.get();
     a.r0 = S.r.get();
     a.x = S.inner_x.get();
-    a.pbuf = S.ps_pbuf.get();
-    a.zbuf = S.ps_zbuf.get();
-    a.nsum = S.ps_cr.get();
-    a.part = S.ps_part.get();
+    a.tz = S.ps_tz.get();
+    a.tn = S.ps_tn.get();
+    a.tp = S.ps_tp.get();
     a.bar = S.ps_bar.get();
     a.abort_flag = S.ps_abort.get();
     a.out = S.ps_out.get();
@@ -658,10 +728,12 @@ void Context::build_scene() {
     a.ncn_max = max_ncn;
     a.ncb_max = max_ncb;
     a.cnl_max = max_cnl;
+    a.nhalo_max = nhalo_max;
+    a.nwg_max = nwg_max;
     a.grid = ps_wg;
     const char* mode = std::getenv("SILK_HIP_PCG");
     const bool want = !(mode && std::strcmp(mode, "launch") == 0);
-    S.use_persistent = want && ps_wg > 0 &&
+    S.use_persistent = want && ps_wg > 0 && xch_fits &&
                        pcg_persistent_fits(ps_wg, pcg_persistent_lds(a));
   }
   S.coll = make_collision_world(*this, S);

@@ -101,10 +101,11 @@ struct Scene {
   bool use_persistent = false;
   PersistDev pers;  // device pointers into the buffers below
   DevBuf<PcgSysDev> ps_sys;
-  DevBuf<int> ps_slot_row, ps_sell_off, ps_sell_width, ps_sell_col, ps_slot_cn,
-      ps_cb_bank, ps_cb_member, ps_cn_ptr, ps_cn_list, ps_abort;
-  DevBuf<float> ps_sell_val, ps_pbuf, ps_zbuf, ps_cr;
-  DevBuf<double> ps_part;
+  DevBuf<int> ps_slot_row, ps_sell_off, ps_sell_width, ps_sell_lcol, ps_halo_ptr,
+      ps_halo_slot, ps_slot_cn, ps_cb_bank, ps_cb_member, ps_cn_ptr, ps_cn_list,
+      ps_abort;
+  DevBuf<float> ps_sell_val;
+  DevBuf<unsigned long long> ps_tz, ps_tn, ps_tp;
   DevBuf<unsigned> ps_bar;
   DevBuf<PcgState> ps_out;
   DevBuf<AdmmCtl> admm_ctl;

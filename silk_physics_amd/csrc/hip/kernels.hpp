@@ -257,22 +257,25 @@ struct PersistDev {
   const int* slot_row = nullptr;    // global row of a slot, -1 for padding
   const int* sell_off = nullptr;    // slot-ELL: 64-slot slices
   const int* sell_width = nullptr;
-  const int* sell_col = nullptr;    // neighbour slot
+  const int* sell_lcol = nullptr;   // neighbour as an index into the owning
+                                    // workgroup's slots [0, 256) then halo
   const float* sell_val = nullptr;
+  const int* halo_ptr = nullptr;    // per workgroup: its halo slots
+  const int* halo_slot = nullptr;   //   (other workgroups' slots it reads)
   const int* slot_cn = nullptr;     // 4 per slot: coarse node (cn0-relative) per level
   const int* cb_bank = nullptr;     // per coarse bank: global bank id (bank_inv)
   const int* cb_member = nullptr;   // 32 per coarse bank: node (cn0-relative), -1
-  const int* cn_ptr = nullptr;      // per global coarse node: its children at
-  const int* cn_list = nullptr;     //   the level below (cn0-relative ids)
+  const int* cn_ptr = nullptr;      // per global coarse node of level >= 2: the
+  const int* cn_list = nullptr;     //   level-1 nodes below it (cn0-relative)
   const float* bank_inv = nullptr;  // 1024 per bank
   const float* d = nullptr;         // diag per row
   const float* r0 = nullptr;        // FP32 initial residual per row
   float* x = nullptr;               // x += dx at the end (rows)
-  float* pbuf = nullptr;            // 2 x 4 floats per slot (xyz, pad)
-  float* zbuf = nullptr;            // 4 floats per slot
-  float* nsum = nullptr;            // 4 floats per coarse node: level-1 sums
-  double* part = nullptr;           // 4 per workgroup
-  unsigned* bar = nullptr;          // kPcgBarrierStride per system
+  // Epoch-tagged exchange words (epoch << 32 | payload bits).
+  unsigned long long* tz = nullptr;  // 3 per slot: z
+  unsigned long long* tn = nullptr;  // 3 per coarse node: level-1 sums
+  unsigned long long* tp = nullptr;  // 8 per workgroup: pAp, rz, rr halves
+  unsigned* bar = nullptr;          // kPcgBarrierStride per system: epoch
   int* abort_flag = nullptr;
   PcgState* out = nullptr;          // per system
   AdmmCtl* ctl = nullptr;           // optional: skip flag, tolerance, counters
@@ -282,8 +285,14 @@ struct PersistDev {
   int ncn_max = 0;      // max coarse nodes of a system (LDS)
   int ncb_max = 0;      // max coarse banks of a system (LDS inverses)
   int cnl_max = 0;      // max coarse-node descendant list length (LDS)
+  int nhalo_max = 0;    // max halo slots of a workgroup (LDS)
+  int nwg_max = 0;      // max workgroups of a system (LDS)
   int grid = 0;         // total workgroups
 };
+
+/// Tagged words one thread of the persistent kernel polls per exchange:
+/// an exchange of n words fits when n <= kPcgThreads * kPcgXchWords.
+constexpr int kPcgXchWords = 8;
 
 /// Dynamic LDS bytes of the persistent kernel for the sizes in `a`.
 size_t pcg_persistent_lds(const PersistDev& a);

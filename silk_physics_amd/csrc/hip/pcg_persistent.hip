@@ -7,28 +7,33 @@
 //
 // Work split: a system (cloth) owns a group of workgroups; a workgroup owns
 // up to eight consecutive 32-slot level-0 banks, one thread per slot.  A
-// thread keeps its row's r, dx, z, p and q in registers for the whole solve;
-// the workgroup keeps its eight bank inverses and every coarse bank inverse
-// of the system in LDS.
+// thread keeps its row's r, dx, z, p, q, neighbour list and row of its bank
+// inverse in registers for the whole solve.  The workgroup keeps in LDS every
+// coarse bank inverse of the system, the coarse sums, and p of its own slots
+// and of its halo (the other workgroups' slots its rows reference).
 //
-// Per iteration k (p_k = z_k + beta_k p_{k-1}):
-//   q = A p_k         neighbour values p_k(j) are formed on the fly from
-//                     z_k(j) and p_{k-1}(j), both published before the last
-//                     barrier, the same way their owner forms them
-//   publish p.q partial and the per-level-1-node sums of q
-//   B1                alpha; dx += alpha p; r -= alpha q; every workgroup
-//                     updates its copy of the level-1 sums of r by
-//                     cr -= alpha * qsum; level-0 banks solved from the
-//                     workgroup's own rows; coarse banks solved from cr (all
-//                     in LDS); z = z0 + coarse z; publish z, r.z, r.r
-//   B2                beta and the stopping test (every workgroup sums the
-//                     same partials in the same order: same decision)
+// Per iteration k:
+//   p_k = z_k + beta_k p_{k-1} for the own slots and, with the same fmaf,
+//                     for the halo slots (their z came with the last exchange)
+//   q = A p_k         difference form, neighbour p from LDS
+//   exchange 1        level-1 node sums of q and the p.q partials
+//   alpha; dx += alpha p; r -= alpha q; the level-1 sums of r follow by
+//                     cr -= alpha * qsum; z = M r: level-0 banks from the
+//                     workgroup's rows, coarse levels from the level-1 sums
+//   exchange 2        z of the own slots, r.z and r.r partials; beta and the
+//                     stopping test (every workgroup sums the same partials
+//                     in the same order, so all take the same decision)
 //
-// Cross-workgroup data (z, p, node sums, partials) is written and read with
-// device-scope atomic stores and loads, which are coherent across the XCDs
-// without invalidating the caches that hold the matrix; the barrier arrival
-// is a release.  Barrier waits are bounded: a wait that gives up raises an
-// abort flag, every workgroup leaves, and the host reports an error.
+// An exchange is a set of 64-bit words (epoch << 32 | payload bits) written
+// with device-scope relaxed atomic stores; a reader polls the words it needs
+// until each carries the exchange's epoch.  A 64-bit atomic is read whole, so
+// a word with the right epoch holds the right payload: the words are their
+// own barrier, with no fence and no flag.  A word is rewritten only after an
+// exchange that each of its readers enters after reading it (node sums and
+// p.q after the next exchange 2; z, r.z and r.r after the next exchange 1).
+// Epochs continue across launches from a per-system counter.  Waits are
+// bounded: a wait that gives up raises an abort flag, every workgroup leaves,
+// and the host reports an error.
 #include <cstdlib>
 
 #include "hip/device_common.hpp"
@@ -43,89 +48,92 @@ namespace {
 constexpr unsigned kSpinLimit = 1u << 22;
 constexpr int kBanksPerWg = kPcgThreads / kBankSlots;  // 8
 
-__device__ __forceinline__ void ast2(float* p, float a, float b) {
-  const unsigned long long v =
-      (static_cast<unsigned long long>(__float_as_uint(b)) << 32) |
-      __float_as_uint(a);
-  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), v,
+__device__ __forceinline__ void put_word(unsigned long long* w, unsigned epoch,
+                                         unsigned bits) {
+  __hip_atomic_store(w, (static_cast<unsigned long long>(epoch) << 32) | bits,
                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Store x, y, z of a 4-float record.
-__device__ __forceinline__ void ast4(float* rec, f3 v) {
-  ast2(rec, v.x, v.y);
-  ast2(rec + 2, v.z, 0.0f);
+__device__ __forceinline__ void put_f3(unsigned long long* w, unsigned epoch,
+                                       f3 v) {
+  put_word(w, epoch, __float_as_uint(v.x));
+  put_word(w + 1, epoch, __float_as_uint(v.y));
+  put_word(w + 2, epoch, __float_as_uint(v.z));
 }
 
-__device__ __forceinline__ f3 ald4(const float* rec) {
-  unsigned long long* q =
-      reinterpret_cast<unsigned long long*>(const_cast<float*>(rec));
-  const unsigned long long a =
-      __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const unsigned long long b =
-      __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return {__uint_as_float(static_cast<unsigned>(a)),
-          __uint_as_float(static_cast<unsigned>(a >> 32)),
-          __uint_as_float(static_cast<unsigned>(b))};
+__device__ __forceinline__ void put_double(unsigned long long* w, unsigned epoch,
+                                           double v) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  put_word(w, epoch, static_cast<unsigned>(b));
+  put_word(w + 1, epoch, static_cast<unsigned>(b >> 32));
 }
 
-__device__ __forceinline__ void astd(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// A thread's share of an exchange: word i of the workgroup's list belongs to
+// thread i % kPcgThreads; addr is the tagged word, dest the float of the LDS
+// array that receives its payload.
+struct Share {
+  const unsigned long long* addr[kPcgXchWords];
+  int dest[kPcgXchWords];
+  unsigned mask;
+};
 
-__device__ __forceinline__ double aldd(const double* p) {
-  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Grid barrier of one workgroup group; count and gen sit in different
-// 128-byte lines.  The arrival is a device-scope release; the wait polls
-// with relaxed loads.  Returns false when the wait gave up.
-__device__ bool group_barrier(unsigned* count, unsigned* gen, unsigned nwg,
-                              unsigned& epoch, int* abort_flag, int* s_ok) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int ok = 1;
-    const unsigned arrived = __hip_atomic_fetch_add(
-        count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    if (arrived == nwg - 1) {
-      __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(gen, epoch + 1u, __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      unsigned spins = 0;
-      while (__hip_atomic_load(gen, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT) == epoch) {
-        if (++spins > kSpinLimit ||
-            __hip_atomic_load(abort_flag, __ATOMIC_RELAXED,
-                              __HIP_MEMORY_SCOPE_AGENT) != 0) {
-          ok = 0;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
+// Waits until each of the thread's words carries `epoch`; stores the
+// payloads.  Returns false when the wait gave up.
+__device__ bool collect(const Share& sh, unsigned epoch, float* lds,
+                        int* abort_flag) {
+  unsigned pending = sh.mask;
+  unsigned spins = 0;
+  while (pending != 0) {
+    unsigned long long v[kPcgXchWords];
+#pragma unroll
+    for (int m = 0; m < kPcgXchWords; ++m) {
+      v[m] = 0;
+      if (pending & (1u << m))
+        v[m] = __hip_atomic_load(const_cast<unsigned long long*>(sh.addr[m]),
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    epoch += 1u;
-    if (!ok)
-      __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    *s_ok = ok;
+#pragma unroll
+    for (int m = 0; m < kPcgXchWords; ++m)
+      if ((pending & (1u << m)) && static_cast<unsigned>(v[m] >> 32) == epoch) {
+        lds[sh.dest[m]] = __uint_as_float(static_cast<unsigned>(v[m]));
+        pending &= ~(1u << m);
+      }
+    if (pending == 0) break;
+    if (++spins > kSpinLimit ||
+        __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+      __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
   }
-  __syncthreads();
-  return *s_ok != 0;
+  return true;
 }
 
-// Sum of partials a[4 * i + c], i < n, read with device-scope loads, in a
-// fixed order; valid in every thread.
-__device__ double wg_total(const double* a, int c, int n, double* lds) {
-  double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) v += aldd(a + 4 * i + c);
-  double s = block_sum(v, lds);
-  if (threadIdx.x == 0) lds[16] = s;
-  __syncthreads();
-  s = lds[16];
-  __syncthreads();
-  return s;
+__device__ __forceinline__ double part_at(const float* pw, int w, int h) {
+  const unsigned long long lo = __float_as_uint(pw[8 * w + h]);
+  const unsigned long long hi = __float_as_uint(pw[8 * w + h + 1]);
+  return __longlong_as_double(static_cast<long long>((hi << 32) | lo));
+}
+
+// Sums over the workgroups of the doubles received as word pairs at
+// pw[8 w + h], pw[8 w + h + 1] (h = h0, h1): lane l adds workgroups l,
+// l + 64, ... in order, then a butterfly over the wavefront.  Each step adds
+// the same two partial sums in every lane (in either order), so every lane
+// of every wavefront of every workgroup gets the same bits.
+__device__ __forceinline__ void part_totals(const float* pw, int nwg, int h0,
+                                            int h1, double& t0, double& t1) {
+  const int lane = static_cast<int>(threadIdx.x) % kWave;
+  double a = 0.0, b = 0.0;
+  for (int w = lane; w < nwg; w += kWave) {
+    a += part_at(pw, w, h0);
+    b += part_at(pw, w, h1);
+  }
+  for (int m = kWave / 2; m > 0; m >>= 1) {
+    a += __shfl_xor(a, m, kWave);
+    b += __shfl_xor(b, m, kWave);
+  }
+  t0 = a;
+  t1 = b;
 }
 
 __device__ __forceinline__ f3 fma3(float a, f3 x, f3 y) {
@@ -137,8 +145,16 @@ __device__ __forceinline__ double ddot3(f3 a, f3 b) {
          static_cast<double>(a.z) * b.z;
 }
 
+// x, y, z of the 16-byte record i (one LDS read).
 __device__ __forceinline__ f3 lds4(const float* base, int i) {
-  return {base[4 * i], base[4 * i + 1], base[4 * i + 2]};
+  const float4 v = *reinterpret_cast<const float4*>(base + 4 * i);
+  return {v.x, v.y, v.z};
+}
+
+__device__ __forceinline__ void sts4(float* base, int i, f3 v) {
+  base[4 * i] = v.x;
+  base[4 * i + 1] = v.y;
+  base[4 * i + 2] = v.z;
 }
 
 // Sum over the 32 lanes of a half-wave (fixed butterfly order).
@@ -174,97 +190,80 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* lds) {
   __syncthreads();
 }
 
-// Totals of partials a[4 i + c0] and a[4 i + c1], i < n, read with
-// device-scope loads, summed in a fixed order; valid in every thread.
-__device__ void wg_total2(const double* a, int c0, int c1, int n, double* lds,
-                          double& t0, double& t1) {
-  double v0 = 0.0, v1 = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    v0 += aldd(a + 4 * i + c0);
-    v1 += aldd(a + 4 * i + c1);
-  }
-  block_sum2(v0, v1, lds);
-  if (threadIdx.x == 0) {
-    lds[16] = v0;
-    lds[15] = v1;
-  }
-  __syncthreads();
-  t0 = lds[16];
-  t1 = lds[15];
-  __syncthreads();
-}
-
 __global__ void __launch_bounds__(kPcgThreads)
     k_pcg_persistent(PersistDev a, float rel_tol) {
-  // Dynamic LDS: [8 level-0 inverses][ncb_max coarse inverses]
-  //              [n1_max level-1 sums][ncn_max coarse z][ncn_max coarse sums]
-  //              (4 floats per node), then the coarse bank members, the
-  //              descendant-list offsets and the descendant lists (ints).
-  extern __shared__ float dyn[];
-  __shared__ float rb[kPcgThreads][3];
-  __shared__ float cbuf[kPcgThreads][3];
+  // Dynamic LDS (floats): [ncb_max coarse inverses][coarse sums: level-1
+  // nodes first][coarse z][q sums of the level-1 nodes][partials, 8 per
+  // workgroup][z of own + halo slots][p of own + halo slots] (4 floats per
+  // node or slot), then the coarse bank members, the descendant-list
+  // offsets and the lists (ints).
+  extern __shared__ __align__(16) float dyn[];
+  __shared__ __align__(16) float rb[kPcgThreads][4];
   __shared__ double red[17];
-  __shared__ int s_ok;
 
   int sys = -1;
   for (int s = 0; s < a.nsys; ++s)
     if (static_cast<int>(blockIdx.x) >= a.sys[s].wg0 &&
         static_cast<int>(blockIdx.x) < a.sys[s].wg0 + a.sys[s].nwg)
       sys = s;
-  if (sys < 0) return;  // spare workgroup: takes no part in any barrier
+  if (sys < 0) return;  // spare workgroup: takes no part in any exchange
   // Stopped ADMM loop: nothing to solve.  ctl->done is written only by
   // other kernels, so every workgroup reads the same value.
   if (a.ctl != nullptr && a.ctl->done != 0) return;
   if (a.ctl != nullptr) rel_tol = a.ctl->rel_tol;
   const PcgSysDev S = a.sys[sys];
-  unsigned* bcount = a.bar + kPcgBarrierStride * sys;
-  unsigned* bgen = bcount + kPcgBarrierStride / 2;
-  unsigned epoch = 0;
-  if (threadIdx.x == 0)
-    epoch = __hip_atomic_load(bgen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const unsigned W = static_cast<unsigned>(S.nwg);
+  // The previous solve's last epoch; workgroup 0 advances it at the end,
+  // after every workgroup has read it (all take part in the first exchange).
+  unsigned* epoch_ctr = a.bar + kPcgBarrierStride * sys;
+  unsigned ep = __hip_atomic_load(epoch_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int t = static_cast<int>(threadIdx.x);
   const int j = static_cast<int>(blockIdx.x) - S.wg0;
   const int nb0 = S.nslot / kBankSlots;
   const int b_lo = (j * nb0) / S.nwg, b_hi = ((j + 1) * nb0) / S.nwg;
-  const int lb = b_lo + static_cast<int>(threadIdx.x) / kBankSlots;
-  const int hw = threadIdx.x / kBankSlots;
-  const int s_in_bank = threadIdx.x & (kBankSlots - 1);
-  const int base = threadIdx.x & ~(kBankSlots - 1);
+  const int lb = b_lo + t / kBankSlots;
+  const int hw = t / kBankSlots;
+  const int s_in_bank = t & (kBankSlots - 1);
+  const int base = t & ~(kBankSlots - 1);
   const bool in_bank = lb < b_hi;
   const int slot = S.slot0 + kBankSlots * lb + s_in_bank;
   const int row = in_bank ? a.slot_row[slot] : -1;
   const bool own = row >= 0;
   const bool coarse = S.ncb > 0;
-  const size_t ns4 = 4 * static_cast<size_t>(a.nslot_total);
+  const int h0 = a.halo_ptr[blockIdx.x];
+  const int nh = a.halo_ptr[blockIdx.x + 1] - h0;
+  const size_t hl = static_cast<size_t>(kPcgThreads) + a.nhalo_max;
 
-  float* inv0 = dyn;                                           // 8 x 1024
-  float* invc = inv0 + kBanksPerWg * 1024;                     // ncb_max x 1024
-  float* crl = invc + static_cast<size_t>(a.ncb_max) * 1024;  // 4 x n1_max
-  float* czl = crl + 4 * static_cast<size_t>(a.n1_max);       // 4 x ncn_max
-  float* csum = czl + 4 * static_cast<size_t>(a.ncn_max);     // 4 x ncn_max
-  int* cbm = reinterpret_cast<int*>(csum + 4 * static_cast<size_t>(a.ncn_max));
+  float* invc = dyn;                                           // ncb_max x 1024
+  float* csum = invc + static_cast<size_t>(a.ncb_max) * 1024;  // 4 x ncn_max
+  float* czl = csum + 4 * static_cast<size_t>(a.ncn_max);     // 4 x ncn_max
+  float* qs = czl + 4 * static_cast<size_t>(a.ncn_max);       // 4 x n1_max
+  float* pw = qs + 4 * static_cast<size_t>(a.n1_max);         // 8 x nwg_max
+  float* zl = pw + 8 * static_cast<size_t>(a.nwg_max);        // 4 x hl
+  float* pl = zl + 4 * hl;                                     // 4 x hl
+  int* cbm = reinterpret_cast<int*>(pl + 4 * hl);              // 32 x ncb_max
   int* cnp = cbm + kBankSlots * a.ncb_max;                     // ncn_max + 1
   int* cnl = cnp + a.ncn_max + 1;                              // cnl_max
 
   // Static data into LDS.
-  for (int e = threadIdx.x; e < kBanksPerWg * 1024; e += kPcgThreads) {
-    const int b = b_lo + e / 1024;
-    inv0[e] = b < b_hi
-                  ? a.bank_inv[static_cast<size_t>(S.bank0 + b) * 1024 + (e & 1023)]
-                  : 0.0f;
-  }
-  for (int e = threadIdx.x; e < S.ncb * 1024; e += kPcgThreads)
+  for (int e = t; e < S.ncb * 1024; e += kPcgThreads)
     invc[e] = a.bank_inv[static_cast<size_t>(a.cb_bank[S.cb0 + e / 1024]) * 1024 +
                          (e & 1023)];
-  for (int e = threadIdx.x; e < S.ncb * kBankSlots; e += kPcgThreads)
+  for (int e = t; e < S.ncb * kBankSlots; e += kPcgThreads)
     cbm[e] = a.cb_member[kBankSlots * S.cb0 + e];
   const int cn_first = S.ncn > 0 ? a.cn_ptr[S.cn0] : 0;
-  for (int e = threadIdx.x; e <= S.ncn; e += kPcgThreads)
+  for (int e = t; e <= S.ncn; e += kPcgThreads)
     cnp[e] = a.cn_ptr[S.cn0 + e] - cn_first;
-  for (int e = threadIdx.x; S.ncn > 0 && e < a.cn_ptr[S.cn0 + S.ncn] - cn_first;
+  for (int e = t; S.ncn > 0 && e < a.cn_ptr[S.cn0 + S.ncn] - cn_first;
        e += kPcgThreads)
     cnl[e] = a.cn_list[cn_first + e];
 
+  // This slot's row of its level-0 bank inverse.
+  float winv[kBankSlots];
+#pragma unroll
+  for (int k = 0; k < kBankSlots; ++k)
+    winv[k] = in_bank ? a.bank_inv[static_cast<size_t>(S.bank0 + lb) * 1024 +
+                                   k * kBankSlots + s_in_bank]
+                      : 0.0f;
   int e_off = 0, e_w = 0, e_lane = 0;
   if (own) {
     const int sl = slot / kSlice;
@@ -272,15 +271,16 @@ __global__ void __launch_bounds__(kPcgThreads)
     e_w = a.sell_width[sl];
     e_lane = slot % kSlice;
   }
-  // This row's neighbours (slot-ELL, padded to whole chunks with the row's
-  // own slot and weight 0) and coarse nodes, kept in registers.
+  // This row's neighbours as indices into pl (slot-ELL, padded to whole
+  // chunks with the row's own slot and weight 0) and coarse nodes, kept in
+  // registers.
   const bool reg_ell = e_w <= kPcgNbr;
   int ncol[kPcgNbr];
   float nval[kPcgNbr];
 #pragma unroll
   for (int k = 0; k < kPcgNbr; ++k) {
     const bool v = own && reg_ell && k < e_w;
-    ncol[k] = v ? a.sell_col[e_off + k * kSlice + e_lane] : slot;
+    ncol[k] = v ? a.sell_lcol[e_off + k * kSlice + e_lane] : t;
     nval[k] = v ? a.sell_val[e_off + k * kSlice + e_lane] : 0.0f;
   }
   int scn[4] = {-1, -1, -1, -1};
@@ -299,89 +299,131 @@ __global__ void __launch_bounds__(kPcgThreads)
   }
   const int n1_count = n1_last >= 0 ? n1_last - n1_first + 1 : 0;
 
-  double* my_part = a.part + 4 * blockIdx.x;
-  const double* grp_part = a.part + 4 * S.wg0;
+  // Exchange 1: level-1 sums (3 words per node) then p.q (2 per workgroup).
+  // Exchange 2: r.z, r.r (4 per workgroup) then z of the halo (3 per slot).
+  Share x1, x2;
+  {
+    const int nw1 = coarse ? 3 * S.n1 : 0;
+    const int w1 = nw1 + 2 * S.nwg;
+    const int pw2 = 4 * S.nwg;
+    const int w2 = pw2 + 3 * nh;
+    x1.mask = 0;
+    x2.mask = 0;
+#pragma unroll
+    for (int m = 0; m < kPcgXchWords; ++m) {
+      const int i = t + kPcgThreads * m;
+      x1.addr[m] = a.tp;
+      x1.dest[m] = 0;
+      if (i < w1) {
+        x1.mask |= 1u << m;
+        if (i < nw1) {
+          x1.addr[m] = a.tn + 3 * static_cast<size_t>(S.cn0 + i / 3) + i % 3;
+          x1.dest[m] = static_cast<int>(qs - dyn) + 4 * (i / 3) + i % 3;
+        } else {
+          const int w = (i - nw1) / 2, h = (i - nw1) % 2;
+          x1.addr[m] = a.tp + 8 * static_cast<size_t>(S.wg0 + w) + h;
+          x1.dest[m] = static_cast<int>(pw - dyn) + 8 * w + h;
+        }
+      }
+      x2.addr[m] = a.tp;
+      x2.dest[m] = 0;
+      if (i < w2) {
+        x2.mask |= 1u << m;
+        if (i < pw2) {
+          const int w = i / 4, h = i % 4;
+          x2.addr[m] = a.tp + 8 * static_cast<size_t>(S.wg0 + w) + 2 + h;
+          x2.dest[m] = static_cast<int>(pw - dyn) + 8 * w + 2 + h;
+        } else {
+          const int hh = (i - pw2) / 3, c = (i - pw2) % 3;
+          x2.addr[m] = a.tz + 3 * static_cast<size_t>(a.halo_slot[h0 + hh]) + c;
+          x2.dest[m] = static_cast<int>(zl - dyn) + 4 * (kPcgThreads + hh) + c;
+        }
+      }
+    }
+  }
+  unsigned long long* my_tp = a.tp + 8 * static_cast<size_t>(blockIdx.x);
   __syncthreads();
 
   // Sum of v over the slots of this thread's level-1 node, published by the
   // first slot of the node (level-1 nodes never straddle banks).
-  auto publish_node_sums = [&](f3 v) {
+  auto publish_node_sums = [&](f3 v, unsigned e) {
     for (int k = 0; k < n1_count; ++k) {
       const int target = n1_first + k;
-      const f3 t = half_sum(scn[0] == target ? v : f3{0.0f, 0.0f, 0.0f});
-      if (s_in_bank == 0) ast4(a.nsum + 4 * static_cast<size_t>(S.cn0 + target), t);
+      const f3 s = half_sum(scn[0] == target ? v : f3{0.0f, 0.0f, 0.0f});
+      if (s_in_bank == 0) put_f3(a.tn + 3 * static_cast<size_t>(S.cn0 + target), e, s);
     }
   };
 
-  // z = M r from this thread's r (registers), the workgroup's banks and the
-  // level-1 sums in crl.
+  // z = M r from this thread's r, its bank row and the level-1 sums in csum.
   auto mas = [&](f3 rv) -> f3 {
-    rb[threadIdx.x][0] = rv.x;
-    rb[threadIdx.x][1] = rv.y;
-    rb[threadIdx.x][2] = rv.z;
+    rb[t][0] = rv.x;
+    rb[t][1] = rv.y;
+    rb[t][2] = rv.z;
     __syncthreads();
     f3 z0 = {0.0f, 0.0f, 0.0f};
     if (in_bank) {
-      const float* inv = inv0 + hw * 1024;
+#pragma unroll
       for (int k = 0; k < kBankSlots; ++k) {
-        const float w = inv[k * kBankSlots + s_in_bank];
-        z0.x += w * rb[base + k][0];
-        z0.y += w * rb[base + k][1];
-        z0.z += w * rb[base + k][2];
+        const float w = winv[k];
+        const f3 rk = lds4(&rb[0][0], base + k);
+        z0.x += w * rk.x;
+        z0.y += w * rk.y;
+        z0.z += w * rk.z;
       }
     }
     if (!coarse) return z0;
-    // Residual sums of the coarse nodes, level by level from the level-1
-    // sums: one half-wave per node, children strided over its lanes.
-    for (int n = threadIdx.x; n < S.n1; n += kPcgThreads) {
-      csum[4 * n] = crl[4 * n];
-      csum[4 * n + 1] = crl[4 * n + 1];
-      csum[4 * n + 2] = crl[4 * n + 2];
+    // Sums of the deeper coarse nodes from their level-1 nodes: one
+    // half-wave per node, the level-1 nodes strided over its lanes.
+    for (int n = S.n1 + hw; n < S.ncn; n += kBanksPerWg) {
+      const int e_end = cnp[n + 1];
+      f3 v = {0.0f, 0.0f, 0.0f};
+      for (int e = cnp[n] + s_in_bank; e < e_end; e += 4 * kBankSlots) {
+        int id[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          id[u] = e + u * kBankSlots < e_end ? cnl[e + u * kBankSlots] : -1;
+        f3 c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          c[u] = id[u] >= 0 ? lds4(csum, id[u]) : f3{0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (id[u] >= 0) v = v + c[u];
+      }
+      v = half_sum(v);
+      if (s_in_bank == 0) sts4(csum, n, v);
     }
     __syncthreads();
-    for (int l = 2; l <= 4; ++l) {
-      const int lo = S.lvl[l - 1], hi = S.lvl[l];
-      if (lo >= hi) break;
-      for (int n = lo + hw; n < hi; n += kBanksPerWg) {
-        f3 v = {0.0f, 0.0f, 0.0f};
-        for (int e = cnp[n] + s_in_bank; e < cnp[n + 1]; e += kBankSlots)
-          v = v + lds4(csum, cnl[e]);
-        v = half_sum(v);
-        if (s_in_bank == 0) {
-          csum[4 * n] = v.x;
-          csum[4 * n + 1] = v.y;
-          csum[4 * n + 2] = v.z;
+    // Coarse banks: one thread per (bank, member); the bank's inputs are
+    // read by the whole half-wave at once.
+    // All loads of a dot product are issued before its first product.
+    for (int wk = t; wk < S.ncb * kBankSlots; wk += kPcgThreads) {
+      const int cb = wk / kBankSlots, i = wk % kBankSlots;
+      const float* inv = invc + static_cast<size_t>(cb) * 1024;
+      const int* mem = cbm + kBankSlots * cb;
+      f3 zc = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int k0 = 0; k0 < kBankSlots; k0 += 16) {
+        int mk[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) mk[k] = mem[k0 + k];
+        f3 v[16];
+        float w[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          v[k] = mk[k] >= 0 ? lds4(csum, mk[k]) : f3{0.0f, 0.0f, 0.0f};
+          w[k] = inv[(k0 + k) * kBankSlots + i];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          zc.x += w[k] * v[k].x;
+          zc.y += w[k] * v[k].y;
+          zc.z += w[k] * v[k].z;
         }
       }
-      __syncthreads();
+      if (mem[i] >= 0) sts4(czl, mem[i], zc);
     }
-    for (int round = 0; round < S.ncb; round += kBanksPerWg) {
-      const int cb = round + hw;
-      int m = -1;
-      f3 v = {0.0f, 0.0f, 0.0f};
-      if (cb < S.ncb) {
-        m = cbm[kBankSlots * cb + s_in_bank];
-        if (m >= 0) v = lds4(csum, m);
-      }
-      cbuf[threadIdx.x][0] = v.x;
-      cbuf[threadIdx.x][1] = v.y;
-      cbuf[threadIdx.x][2] = v.z;
-      __syncthreads();
-      if (cb < S.ncb && m >= 0) {
-        const float* inv = invc + static_cast<size_t>(cb) * 1024;
-        f3 zc = {0.0f, 0.0f, 0.0f};
-        for (int k = 0; k < kBankSlots; ++k) {
-          const float w = inv[k * kBankSlots + s_in_bank];
-          zc.x += w * cbuf[base + k][0];
-          zc.y += w * cbuf[base + k][1];
-          zc.z += w * cbuf[base + k][2];
-        }
-        czl[4 * m] = zc.x;
-        czl[4 * m + 1] = zc.y;
-        czl[4 * m + 2] = zc.z;
-      }
-      __syncthreads();
-    }
+    __syncthreads();
     if (in_bank) {
 #pragma unroll
       for (int l = 0; l < 4; ++l)
@@ -392,29 +434,26 @@ __global__ void __launch_bounds__(kPcgThreads)
 
   // ---- prologue: level-1 sums of r0, z0 = M r0, r.z, r.r
   if (coarse) {
-    publish_node_sums(r);
-    if (!group_barrier(bcount, bgen, W, epoch, a.abort_flag, &s_ok)) return;
-    for (int n = threadIdx.x; n < S.n1; n += kPcgThreads) {
-      const f3 v = ald4(a.nsum + 4 * static_cast<size_t>(S.cn0 + n));
-      crl[4 * n] = v.x;
-      crl[4 * n + 1] = v.y;
-      crl[4 * n + 2] = v.z;
-    }
-    __syncthreads();
+    ++ep;
+    publish_node_sums(r, ep);
+    if (t == 0) put_double(my_tp, ep, 0.0);
+    if (!__syncthreads_and(collect(x1, ep, dyn, a.abort_flag))) return;
+    for (int n = t; n < S.n1; n += kPcgThreads) sts4(csum, n, lds4(qs, n));
   }
   f3 z = mas(r);
-  if (own) ast4(a.zbuf + 4 * static_cast<size_t>(slot), z);
+  ++ep;
+  if (own) put_f3(a.tz + 3 * static_cast<size_t>(slot), ep, z);
   {
     double s0 = own ? ddot3(r, z) : 0.0, s1 = own ? ddot3(r, r) : 0.0;
     block_sum2(s0, s1, red);
-    if (threadIdx.x == 0) {
-      astd(my_part + 1, s0);
-      astd(my_part + 2, s1);
+    if (t == 0) {
+      put_double(my_tp + 2, ep, s0);
+      put_double(my_tp + 4, ep, s1);
     }
   }
-  if (!group_barrier(bcount, bgen, W, epoch, a.abort_flag, &s_ok)) return;
+  if (!__syncthreads_and(collect(x2, ep, dyn, a.abort_flag))) return;
   double rz, rr;
-  wg_total2(grp_part, 1, 2, S.nwg, red, rz, rr);
+  part_totals(pw, S.nwg, 2, 4, rz, rr);
   const double rr0 = rr;
   const double abs2 = static_cast<double>(a.abs_tol) * a.abs_tol;
   const double rel2 = static_cast<double>(rel_tol) * rel_tol;
@@ -423,42 +462,37 @@ __global__ void __launch_bounds__(kPcgThreads)
   float beta = 0.0f;
 
   while (status < 0) {
-    const float* p_prev = a.pbuf + static_cast<size_t>((iter + 1) & 1) * ns4;
-    float* p_cur = a.pbuf + static_cast<size_t>(iter & 1) * ns4;
     p = (iter == 0) ? z : fma3(beta, p, z);
-    if (own) ast4(p_cur + 4 * static_cast<size_t>(slot), p);
-    // q = (K + D) p in difference form; neighbour p formed as its owner does.
+    sts4(pl, t, p);
+    for (int hh = t; hh < nh; hh += kPcgThreads) {
+      const f3 zh = lds4(zl, kPcgThreads + hh);
+      sts4(pl, kPcgThreads + hh,
+           iter == 0 ? zh : fma3(beta, lds4(pl, kPcgThreads + hh), zh));
+    }
+    __syncthreads();
+    // q = (K + D) p in difference form.
     f3 q = {0.0f, 0.0f, 0.0f};
     if (own) {
       float ax = 0.0f, ay = 0.0f, az = 0.0f;
       if (reg_ell) {
-        // Chunks of 8 neighbours: all their loads are issued before any use.
 #pragma unroll
         for (int c0 = 0; c0 < kPcgNbr; c0 += kPcgNbrChunk) {
           if (c0 >= e_w) break;
-          f3 zn[kPcgNbrChunk], pp[kPcgNbrChunk];
+          f3 pn[kPcgNbrChunk];
+#pragma unroll
+          for (int k = 0; k < kPcgNbrChunk; ++k) pn[k] = lds4(pl, ncol[c0 + k]);
 #pragma unroll
           for (int k = 0; k < kPcgNbrChunk; ++k) {
-            zn[k] = ald4(a.zbuf + 4 * static_cast<size_t>(ncol[c0 + k]));
-            pp[k] = iter > 0 ? ald4(p_prev + 4 * static_cast<size_t>(ncol[c0 + k]))
-                             : f3{0.0f, 0.0f, 0.0f};
-          }
-#pragma unroll
-          for (int k = 0; k < kPcgNbrChunk; ++k) {
-            const f3 pn = iter > 0 ? fma3(beta, pp[k], zn[k]) : zn[k];
             const float v = nval[c0 + k];
-            ax += v * (pn.x - p.x);
-            ay += v * (pn.y - p.y);
-            az += v * (pn.z - p.z);
+            ax += v * (pn[k].x - p.x);
+            ay += v * (pn[k].y - p.y);
+            az += v * (pn[k].z - p.z);
           }
         }
       } else {
         for (int k = 0; k < e_w; ++k) {
-          const int c = a.sell_col[e_off + k * kSlice + e_lane];
+          const f3 pn = lds4(pl, a.sell_lcol[e_off + k * kSlice + e_lane]);
           const float v = a.sell_val[e_off + k * kSlice + e_lane];
-          f3 pn = ald4(a.zbuf + 4 * static_cast<size_t>(c));
-          if (iter > 0)
-            pn = fma3(beta, ald4(p_prev + 4 * static_cast<size_t>(c)), pn);
           ax += v * (pn.x - p.x);
           ay += v * (pn.y - p.y);
           az += v * (pn.z - p.z);
@@ -466,38 +500,38 @@ __global__ void __launch_bounds__(kPcgThreads)
       }
       q = {ax + dd * p.x, ay + dd * p.y, az + dd * p.z};
     }
-    if (coarse) publish_node_sums(q);
+    ++ep;
+    if (coarse) publish_node_sums(q, ep);
     {
       const double s = block_sum(own ? ddot3(p, q) : 0.0, red);
-      if (threadIdx.x == 0) astd(my_part + 0, s);
+      if (t == 0) put_double(my_tp, ep, s);
     }
-    if (!group_barrier(bcount, bgen, W, epoch, a.abort_flag, &s_ok)) return;
-    const double pAp = wg_total(grp_part, 0, S.nwg, red);
+    if (!__syncthreads_and(collect(x1, ep, dyn, a.abort_flag))) return;
+    double pAp, unused;
+    part_totals(pw, S.nwg, 0, 0, pAp, unused);
     const float alpha = static_cast<float>(fabs(pAp) < 1e-20 ? 0.0 : rz / pAp);
     dx = fma3(alpha, p, dx);
     r = fma3(-alpha, q, r);
-    if (coarse) {
-      for (int n = threadIdx.x; n < S.n1; n += kPcgThreads) {
-        const f3 qs = ald4(a.nsum + 4 * static_cast<size_t>(S.cn0 + n));
-        crl[4 * n] = fmaf(-alpha, qs.x, crl[4 * n]);
-        crl[4 * n + 1] = fmaf(-alpha, qs.y, crl[4 * n + 1]);
-        crl[4 * n + 2] = fmaf(-alpha, qs.z, crl[4 * n + 2]);
+    if (coarse)
+      for (int n = t; n < S.n1; n += kPcgThreads) {
+        const f3 qn = lds4(qs, n), cn = lds4(csum, n);
+        sts4(csum, n, {fmaf(-alpha, qn.x, cn.x), fmaf(-alpha, qn.y, cn.y),
+                       fmaf(-alpha, qn.z, cn.z)});
       }
-      __syncthreads();
-    }
     z = mas(r);
-    if (own) ast4(a.zbuf + 4 * static_cast<size_t>(slot), z);
+    ++ep;
+    if (own) put_f3(a.tz + 3 * static_cast<size_t>(slot), ep, z);
     {
       double s0 = own ? ddot3(r, z) : 0.0, s1 = own ? ddot3(r, r) : 0.0;
       block_sum2(s0, s1, red);
-      if (threadIdx.x == 0) {
-        astd(my_part + 1, s0);
-        astd(my_part + 2, s1);
+      if (t == 0) {
+        put_double(my_tp + 2, ep, s0);
+        put_double(my_tp + 4, ep, s1);
       }
     }
-    if (!group_barrier(bcount, bgen, W, epoch, a.abort_flag, &s_ok)) return;
+    if (!__syncthreads_and(collect(x2, ep, dyn, a.abort_flag))) return;
     double rz_new;
-    wg_total2(grp_part, 1, 2, S.nwg, red, rz_new, rr);
+    part_totals(pw, S.nwg, 2, 4, rz_new, rr);
     ++iter;
     if (iter >= a.max_iter) status = 2;
     else if (rr <= abs2) status = 0;
@@ -506,7 +540,8 @@ __global__ void __launch_bounds__(kPcgThreads)
     rz = rz_new;
   }
   if (own) st3(a.x, row, ld3(a.x, row) + dx);
-  if (j == 0 && threadIdx.x == 0) {
+  if (j == 0 && t == 0) {
+    __hip_atomic_store(epoch_ctr, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     PcgState o;
     o.rz = rz;
     o.rr = rr;
@@ -534,10 +569,11 @@ size_t g_lds_checked = static_cast<size_t>(-1);
 }  // namespace
 
 size_t pcg_persistent_lds(const PersistDev& a) {
-  return sizeof(float) * (static_cast<size_t>(kBanksPerWg) * 1024 +
-                          static_cast<size_t>(a.ncb_max) * 1024 +
+  const size_t hl = static_cast<size_t>(kPcgThreads) + a.nhalo_max;
+  return sizeof(float) * (static_cast<size_t>(a.ncb_max) * 1024 +
+                          8 * static_cast<size_t>(a.ncn_max) +
                           4 * static_cast<size_t>(a.n1_max) +
-                          8 * static_cast<size_t>(a.ncn_max)) +
+                          8 * static_cast<size_t>(a.nwg_max) + 8 * hl) +
          sizeof(int) * (static_cast<size_t>(kBankSlots) * a.ncb_max +
                         static_cast<size_t>(a.ncn_max) + 1 +
                         static_cast<size_t>(a.cnl_max));
@@ -577,11 +613,11 @@ hipError_t launch_pcg_persistent(hipStream_t s, const PersistDev& a,
   void* args[] = {&arg, &tol};
   const size_t shmem = pcg_persistent_lds(a);
   // The grid was sized to fit the device (pcg_persistent_fits) and every
-  // barrier wait is bounded, so a plain launch is safe as well: it is used
-  // when the cooperative one is refused (the runtime grants cooperative
-  // launches to one process of a device at a time) and with SILK_HIP_COOP=0
-  // (profilers that do not follow cooperative launches).  Without the
-  // runtime's residency guarantee a failure is reported, never a hang.
+  // wait is bounded, so a plain launch is safe as well: it is used when the
+  // cooperative one is refused (the runtime grants cooperative launches to
+  // one process of a device at a time) and with SILK_HIP_COOP=0 (profilers
+  // that do not follow cooperative launches).  Without the runtime's
+  // residency guarantee a failure is reported, never a hang.
   static const bool coop_env = [] {
     const char* e = std::getenv("SILK_HIP_COOP");
     return !(e != nullptr && e[0] == '0');
