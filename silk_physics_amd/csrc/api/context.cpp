@@ -437,9 +437,10 @@ void Context::build_scene() {
           }
         }
       sd.ncb = static_cast<int>(ps_cb_bank.size()) - sd.cb0;
-      // Level-1 nodes below every coarse node of level >= 2 (level-1 nodes
-      // take their sums from the residual itself).
+      // The level-1 nodes that make up every coarse node (a level-1 node is
+      // its own list).
       std::vector<std::vector<int>> desc(sd.ncn);
+      for (int m = 0; m < sd.n1; ++m) desc[m].push_back(m);
       for (int lr = 0; lr < t.n; ++lr)
         for (int l = 2; l <= t.coarse_levels; ++l)
           desc[lvl_off[l] + t.cmap[l - 1][lr]].push_back(lvl_off[1] +
@@ -513,7 +514,8 @@ void Context::build_scene() {
   // workgroups' slots each one reads (its halo) and every neighbour as an
   // index into the workgroup's slots [0, kPcgThreads) then its halo.
   std::vector<int> ps_halo_ptr{0}, ps_halo_slot, ps_sell_lcol(ps_sell_col.size(), 0);
-  int nhalo_max = 0, nwg_max = 0;
+  std::vector<int> ps_wg_n1_ptr{0}, ps_wg_n1, ps_wg_c_ptr{0}, ps_wg_c;
+  int nhalo_max = 0, nwg_max = 0, cbw_max = 0, need1_max = 0;
   bool xch_fits = true;
   {
     const int nslot = static_cast<int>(ps_slot_row.size());
@@ -562,14 +564,46 @@ void Context::build_scene() {
             std::lower_bound(hl.begin(), hl.end(), c) - hl.begin());
       }
     });
-    // Tagged words of the two exchanges of an iteration, per workgroup.
-    const int cap = kPcgThreads * kPcgXchWords;
+    // Coarse bank cb of a system is solved by its workgroup cb % nwg.  Per
+    // workgroup: the level-1 nodes whose sums its banks read, and the coarse
+    // nodes whose corrections its own and halo slots add.
     for (const auto& sd : ps_sys) {
-      if (3 * sd.n1 + 2 * sd.nwg > cap) xch_fits = false;
+      cbw_max = std::max(cbw_max, (sd.ncb + sd.nwg - 1) / sd.nwg);
       for (int j = 0; j < sd.nwg; ++j) {
         const int w = sd.wg0 + j;
+        std::vector<int> n1s, cs;
+        for (int cb = j; cb < sd.ncb; cb += sd.nwg)
+          for (int k = 0; k < kBankSlots; ++k) {
+            const int m = ps_cb_member[kBankSlots * (sd.cb0 + cb) + k];
+            if (m < 0) continue;
+            for (int e = ps_cn_ptr[sd.cn0 + m]; e < ps_cn_ptr[sd.cn0 + m + 1]; ++e)
+              n1s.push_back(ps_cn_list[e]);
+          }
+        auto add_slot = [&](int sl) {
+          for (int l = 0; l < 4; ++l)
+            if (ps_slot_cn[4 * static_cast<size_t>(sl) + l] >= 0)
+              cs.push_back(ps_slot_cn[4 * static_cast<size_t>(sl) + l]);
+        };
+        for (int sl = 0; sl < nslot; ++sl)
+          if (slot_wg[sl] == w && ps_slot_row[sl] >= 0) add_slot(sl);
+        for (int e = ps_halo_ptr[w]; e < ps_halo_ptr[w + 1]; ++e) add_slot(ps_halo_slot[e]);
+        for (auto* v : {&n1s, &cs}) {
+          std::sort(v->begin(), v->end());
+          v->erase(std::unique(v->begin(), v->end()), v->end());
+        }
+        ps_wg_n1.insert(ps_wg_n1.end(), n1s.begin(), n1s.end());
+        ps_wg_n1_ptr.push_back(static_cast<int>(ps_wg_n1.size()));
+        ps_wg_c.insert(ps_wg_c.end(), cs.begin(), cs.end());
+        ps_wg_c_ptr.push_back(static_cast<int>(ps_wg_c.size()));
+        need1_max = std::max(need1_max, static_cast<int>(n1s.size()));
+        // Tagged words of the two exchanges of an iteration.
+        const int cap = kPcgThreads * kPcgXchWords;
         const int nh = ps_halo_ptr[w + 1] - ps_halo_ptr[w];
-        if (4 * sd.nwg + 3 * nh > cap) xch_fits = false;
+        if (3 * static_cast<int>(n1s.size()) + 2 * kPcgWaves * sd.nwg > cap ||
+            4 * kPcgWaves * sd.nwg + 2 * sd.ncb + 3 * nh +
+                    3 * static_cast<int>(cs.size()) > cap ||
+            nh > kPcgThreads * kPcgHaloPerThread)
+          xch_fits = false;
       }
     }
   }
@@ -676,6 +710,10 @@ void Context::build_scene() {
     up(S.ps_sell_lcol, ps_sell_lcol);
     up(S.ps_halo_ptr, ps_halo_ptr);
     up(S.ps_halo_slot, ps_halo_slot);
+    up(S.ps_wg_n1_ptr, ps_wg_n1_ptr);
+    up(S.ps_wg_n1, ps_wg_n1);
+    up(S.ps_wg_c_ptr, ps_wg_c_ptr);
+    up(S.ps_wg_c, ps_wg_c);
     if (ps_sell_val.empty()) ps_sell_val.push_back(0.0f);
     S.ps_sell_val.upload(ps_sell_val, st);
     up(S.ps_slot_cn, ps_slot_cn);
@@ -688,8 +726,12 @@ void Context::build_scene() {
     S.ps_tz.zero(st);
     S.ps_tn.resize(3 * static_cast<size_t>(std::max(1, ncn_all)));
     S.ps_tn.zero(st);
-    S.ps_tp.resize(8 * static_cast<size_t>(std::max(1, ps_wg)));
+    S.ps_tp.resize(kPcgPartWords * static_cast<size_t>(std::max(1, ps_wg)));
     S.ps_tp.zero(st);
+    S.ps_tc.resize(3 * static_cast<size_t>(std::max(1, ncn_all)));
+    S.ps_tc.zero(st);
+    S.ps_tcp.resize(2 * std::max<size_t>(1, ps_cb_bank.size()));
+    S.ps_tcp.zero(st);
     S.ps_bar.resize(kPcgBarrierStride * static_cast<size_t>(std::max(1, S.nsys)));
     S.ps_bar.zero(st);
     S.ps_abort.resize(1);
@@ -706,6 +748,10 @@ void Context::build_scene() {
     a.sell_lcol = S.ps_sell_lcol.get();
     a.halo_ptr = S.ps_halo_ptr.get();
     a.halo_slot = S.ps_halo_slot.get();
+    a.wg_n1_ptr = S.ps_wg_n1_ptr.get();
+    a.wg_n1 = S.ps_wg_n1.get();
+    a.wg_c_ptr = S.ps_wg_c_ptr.get();
+    a.wg_c = S.ps_wg_c.get();
     a.sell_val = S.ps_sell_val.get();
     a.slot_cn = S.ps_slot_cn.get();
     a.cb_bank = S.ps_cb_bank.get();
@@ -719,6 +765,8 @@ void Context::build_scene() {
     a.tz = S.ps_tz.get();
     a.tn = S.ps_tn.get();
     a.tp = S.ps_tp.get();
+    a.tc = S.ps_tc.get();
+    a.tcp = S.ps_tcp.get();
     a.bar = S.ps_bar.get();
     a.abort_flag = S.ps_abort.get();
     a.out = S.ps_out.get();
@@ -730,6 +778,8 @@ void Context::build_scene() {
     a.cnl_max = max_cnl;
     a.nhalo_max = nhalo_max;
     a.nwg_max = nwg_max;
+    a.cbw_max = cbw_max;
+    a.need1_max = need1_max;
     a.grid = ps_wg;
     const char* mode = std::getenv("SILK_HIP_PCG");
     const bool want = !(mode && std::strcmp(mode, "launch") == 0);

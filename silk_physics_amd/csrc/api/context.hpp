@@ -102,10 +102,10 @@ struct Scene {
   PersistDev pers;  // device pointers into the buffers below
   DevBuf<PcgSysDev> ps_sys;
   DevBuf<int> ps_slot_row, ps_sell_off, ps_sell_width, ps_sell_lcol, ps_halo_ptr,
-      ps_halo_slot, ps_slot_cn, ps_cb_bank, ps_cb_member, ps_cn_ptr, ps_cn_list,
-      ps_abort;
+      ps_halo_slot, ps_wg_n1_ptr, ps_wg_n1, ps_wg_c_ptr, ps_wg_c, ps_slot_cn,
+      ps_cb_bank, ps_cb_member, ps_cn_ptr, ps_cn_list, ps_abort;
   DevBuf<float> ps_sell_val;
-  DevBuf<unsigned long long> ps_tz, ps_tn, ps_tp;
+  DevBuf<unsigned long long> ps_tz, ps_tn, ps_tp, ps_tc, ps_tcp;
   DevBuf<unsigned> ps_bar;
   DevBuf<PcgState> ps_out;
   DevBuf<AdmmCtl> admm_ctl;

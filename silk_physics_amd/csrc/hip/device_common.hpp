@@ -39,6 +39,108 @@ __device__ __forceinline__ float dot3(f3 a, f3 b) {
   return a.x * b.x + a.y * b.y + a.z * b.z;
 }
 
+// ---- cross-lane steps without LDS (DPP and the gfx950 permlane swaps).
+// All lanes taking part must be active.
+
+/// Value of lane l ^ 1 (DPP quad_perm [1, 0, 3, 2]).
+constexpr int kDppXor1 = 0xB1;
+/// Value of lane l ^ 2 (DPP quad_perm [2, 3, 0, 1]).
+constexpr int kDppXor2 = 0x4E;
+/// Value of lane 7 - l within each group of 8 (DPP row_half_mirror).
+constexpr int kDppHalfMirror = 0x141;
+/// Value of lane 15 - l within each row of 16 (DPP row_mirror).
+constexpr int kDppMirror = 0x140;
+
+template <int kCtrl>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xF, 0xF, false));
+}
+
+template <int kCtrl>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_update_dpp(
+      0, static_cast<int>(b), kCtrl, 0xF, 0xF, false));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_update_dpp(
+      0, static_cast<int>(b >> 32), kCtrl, 0xF, 0xF, false));
+  return __longlong_as_double(static_cast<long long>(
+      (static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+/// Value of lane l ^ 16 (v_permlane16_swap: even and odd rows of 16 swap).
+__device__ __forceinline__ unsigned xor16_u(unsigned v) {
+  const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+  return ((__lane_id() >> 4) & 1u) ? r[0] : r[1];
+}
+
+/// Value of lane l ^ 32 (v_permlane32_swap: the two halves swap).
+__device__ __forceinline__ unsigned xor32_u(unsigned v) {
+  const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  return (__lane_id() >> 5) ? r[0] : r[1];
+}
+
+__device__ __forceinline__ float xor16_f(float v) {
+  return __uint_as_float(xor16_u(__float_as_uint(v)));
+}
+__device__ __forceinline__ float xor32_f(float v) {
+  return __uint_as_float(xor32_u(__float_as_uint(v)));
+}
+__device__ __forceinline__ double xor16_d(double v) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  const unsigned lo = xor16_u(static_cast<unsigned>(b));
+  const unsigned hi = xor16_u(static_cast<unsigned>(b >> 32));
+  return __longlong_as_double(static_cast<long long>(
+      (static_cast<unsigned long long>(hi) << 32) | lo));
+}
+__device__ __forceinline__ double xor32_d(double v) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  const unsigned lo = xor32_u(static_cast<unsigned>(b));
+  const unsigned hi = xor32_u(static_cast<unsigned>(b >> 32));
+  return __longlong_as_double(static_cast<long long>(
+      (static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+/// Sums over groups of 8, 32 or 64 lanes.  Every step adds two partial sums
+/// of disjoint lane sets that are paired symmetrically (each lane of a pair
+/// adds the same two values, in either order), so every lane of a group ends
+/// with the same bits, and so does any group summing the same values in the
+/// same lane positions.
+template <typename T>
+__device__ __forceinline__ T sum8(T v) {
+  if constexpr (sizeof(T) == 8) {
+    v += dpp_d<kDppXor1>(v);
+    v += dpp_d<kDppXor2>(v);
+    v += dpp_d<kDppHalfMirror>(v);
+  } else {
+    v += dpp_f<kDppXor1>(v);
+    v += dpp_f<kDppXor2>(v);
+    v += dpp_f<kDppHalfMirror>(v);
+  }
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T sum32(T v) {
+  v = sum8(v);
+  if constexpr (sizeof(T) == 8) {
+    v += dpp_d<kDppMirror>(v);
+    v += xor16_d(v);
+  } else {
+    v += dpp_f<kDppMirror>(v);
+    v += xor16_f(v);
+  }
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T sum64(T v) {
+  v = sum32(v);
+  if constexpr (sizeof(T) == 8) v += xor32_d(v);
+  else v += xor32_f(v);
+  return v;
+}
+
 /// Sum of `v` over the workgroup in FP64, valid in thread 0.  blockDim.x must
 /// be a multiple of 64 and at most 1024.  `lds` needs 16 doubles.
 __device__ __forceinline__ double block_sum(double v, double* lds) {

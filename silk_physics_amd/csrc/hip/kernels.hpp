@@ -262,11 +262,15 @@ struct PersistDev {
   const float* sell_val = nullptr;
   const int* halo_ptr = nullptr;    // per workgroup: its halo slots
   const int* halo_slot = nullptr;   //   (other workgroups' slots it reads)
+  const int* wg_n1_ptr = nullptr;   // per workgroup: level-1 nodes (cn0-
+  const int* wg_n1 = nullptr;       //   relative) its coarse banks read
+  const int* wg_c_ptr = nullptr;    // per workgroup: coarse nodes whose
+  const int* wg_c = nullptr;        //   corrections its slots and halo add
   const int* slot_cn = nullptr;     // 4 per slot: coarse node (cn0-relative) per level
   const int* cb_bank = nullptr;     // per coarse bank: global bank id (bank_inv)
   const int* cb_member = nullptr;   // 32 per coarse bank: node (cn0-relative), -1
-  const int* cn_ptr = nullptr;      // per global coarse node of level >= 2: the
-  const int* cn_list = nullptr;     //   level-1 nodes below it (cn0-relative)
+  const int* cn_ptr = nullptr;      // per global coarse node: the level-1
+  const int* cn_list = nullptr;     //   nodes it is made of (cn0-relative)
   const float* bank_inv = nullptr;  // 1024 per bank
   const float* d = nullptr;         // diag per row
   const float* r0 = nullptr;        // FP32 initial residual per row
@@ -274,7 +278,9 @@ struct PersistDev {
   // Epoch-tagged exchange words (epoch << 32 | payload bits).
   unsigned long long* tz = nullptr;  // 3 per slot: z
   unsigned long long* tn = nullptr;  // 3 per coarse node: level-1 sums
-  unsigned long long* tp = nullptr;  // 8 per workgroup: pAp, rz, rr halves
+  unsigned long long* tp = nullptr;  // kPcgPartWords per workgroup: partials
+  unsigned long long* tc = nullptr;  // 3 per coarse node: coarse correction
+  unsigned long long* tcp = nullptr; // 2 per coarse bank: its r.z_coarse
   unsigned* bar = nullptr;          // kPcgBarrierStride per system: epoch
   int* abort_flag = nullptr;
   PcgState* out = nullptr;          // per system
@@ -287,12 +293,22 @@ struct PersistDev {
   int cnl_max = 0;      // max coarse-node descendant list length (LDS)
   int nhalo_max = 0;    // max halo slots of a workgroup (LDS)
   int nwg_max = 0;      // max workgroups of a system (LDS)
+  int cbw_max = 0;      // max coarse banks solved by one workgroup (LDS)
+  int need1_max = 0;    // max level-1 nodes read by one workgroup (LDS)
   int grid = 0;         // total workgroups
 };
 
 /// Tagged words one thread of the persistent kernel polls per exchange:
 /// an exchange of n words fits when n <= kPcgThreads * kPcgXchWords.
 constexpr int kPcgXchWords = 8;
+/// Wavefronts per persistent-PCG workgroup; each publishes its own partial
+/// sums (two words per double): per workgroup, p.q at [0, 8), r.z0 at
+/// [8, 16), r.r at [16, 24) of kPcgPartWords words.
+constexpr int kPcgWaves = kPcgThreads / 64;
+constexpr int kPcgPartWords = 32;
+/// Halo slots one thread handles (an exchange-2 list fits in
+/// kPcgThreads * kPcgXchWords words, so a halo has fewer than 683 slots).
+constexpr int kPcgHaloPerThread = 3;
 
 /// Dynamic LDS bytes of the persistent kernel for the sizes in `a`.
 size_t pcg_persistent_lds(const PersistDev& a);

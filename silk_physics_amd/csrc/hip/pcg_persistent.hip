@@ -8,21 +8,27 @@
 // Work split: a system (cloth) owns a group of workgroups; a workgroup owns
 // up to eight consecutive 32-slot level-0 banks, one thread per slot.  A
 // thread keeps its row's r, dx, z, p, q, neighbour list and row of its bank
-// inverse in registers for the whole solve.  The workgroup keeps in LDS every
-// coarse bank inverse of the system, the coarse sums, and p of its own slots
-// and of its halo (the other workgroups' slots its rows reference).
+// inverse in registers for the whole solve.  The workgroup keeps in LDS the
+// inverses of the coarse banks it solves, the level-1 sums they read, and p
+// of its own slots and of its halo (the other workgroups' slots its rows
+// reference).
 //
 // Per iteration k:
-//   p_k = z_k + beta_k p_{k-1} for the own slots and, with the same fmaf,
-//                     for the halo slots (their z came with the last exchange)
+//   p_k = z_k + beta_k p_{k-1} for the own slots and, with the same
+//                     operations, for the halo slots (z0 from the last
+//                     exchange plus the coarse corrections of their nodes)
 //   q = A p_k         difference form, neighbour p from LDS
 //   exchange 1        level-1 node sums of q and the p.q partials
 //   alpha; dx += alpha p; r -= alpha q; the level-1 sums of r follow by
-//                     cr -= alpha * qsum; z = M r: level-0 banks from the
-//                     workgroup's rows, coarse levels from the level-1 sums
-//   exchange 2        z of the own slots, r.z and r.r partials; beta and the
-//                     stopping test (every workgroup sums the same partials
-//                     in the same order, so all take the same decision)
+//                     cr -= alpha * qsum; z0 = level-0 part of M r
+//   exchange 2        z0 of the own slots, r.z0 and r.r partials, and the
+//                     coarse levels: coarse bank cb is solved by workgroup
+//                     cb % nwg from the level-1 sums it keeps, which
+//                     publishes its members' corrections and the bank's
+//                     r.z_coarse (the correction is constant over a node);
+//                     then z = z0 + coarse corrections, beta and the stopping
+//                     test (every workgroup sums the same partials in the same
+//                     order, so all take the same decision)
 //
 // An exchange is a set of 64-bit words (epoch << 32 | payload bits) written
 // with device-scope relaxed atomic stores; a reader polls the words it needs
@@ -30,7 +36,7 @@
 // a word with the right epoch holds the right payload: the words are their
 // own barrier, with no fence and no flag.  A word is rewritten only after an
 // exchange that each of its readers enters after reading it (node sums and
-// p.q after the next exchange 2; z, r.z and r.r after the next exchange 1).
+// p.q after the next exchange 2; exchange-2 words after the next exchange 1).
 // Epochs continue across launches from a per-system counter.  Waits are
 // bounded: a wait that gives up raises an abort flag, every workgroup leaves,
 // and the host reports an error.
@@ -109,31 +115,32 @@ __device__ bool collect(const Share& sh, unsigned epoch, float* lds,
   return true;
 }
 
-__device__ __forceinline__ double part_at(const float* pw, int w, int h) {
-  const unsigned long long lo = __float_as_uint(pw[8 * w + h]);
-  const unsigned long long hi = __float_as_uint(pw[8 * w + h + 1]);
+// Partial e (workgroup e / kPcgWaves, wavefront e % kPcgWaves) of the
+// block at word offset h of each workgroup's kPcgPartWords.
+__device__ __forceinline__ double part_at(const float* pw, int e, int h) {
+  const int o = kPcgPartWords * (e / kPcgWaves) + h + 2 * (e % kPcgWaves);
+  const unsigned long long lo = __float_as_uint(pw[o]);
+  const unsigned long long hi = __float_as_uint(pw[o + 1]);
   return __longlong_as_double(static_cast<long long>((hi << 32) | lo));
 }
 
-// Sums over the workgroups of the doubles received as word pairs at
-// pw[8 w + h], pw[8 w + h + 1] (h = h0, h1): lane l adds workgroups l,
-// l + 64, ... in order, then a butterfly over the wavefront.  Each step adds
-// the same two partial sums in every lane (in either order), so every lane
-// of every wavefront of every workgroup gets the same bits.
+// Sum over the wavefront; the same bits in every lane and in every
+// wavefront summing the same values (sum64).
+__device__ __forceinline__ double wave_sum(double v) { return sum64(v); }
+
+// Totals of the per-wavefront partials of blocks h0 and h1 over the
+// workgroups: lane l adds partials l, l + 64, ... in order, then a
+// butterfly; the same bits in every thread of every workgroup.
 __device__ __forceinline__ void part_totals(const float* pw, int nwg, int h0,
                                             int h1, double& t0, double& t1) {
   const int lane = static_cast<int>(threadIdx.x) % kWave;
   double a = 0.0, b = 0.0;
-  for (int w = lane; w < nwg; w += kWave) {
-    a += part_at(pw, w, h0);
-    b += part_at(pw, w, h1);
+  for (int e = lane; e < kPcgWaves * nwg; e += kWave) {
+    a += part_at(pw, e, h0);
+    b += part_at(pw, e, h1);
   }
-  for (int m = kWave / 2; m > 0; m >>= 1) {
-    a += __shfl_xor(a, m, kWave);
-    b += __shfl_xor(b, m, kWave);
-  }
-  t0 = a;
-  t1 = b;
+  t0 = wave_sum(a);
+  t1 = wave_sum(b);
 }
 
 __device__ __forceinline__ f3 fma3(float a, f3 x, f3 y) {
@@ -157,14 +164,9 @@ __device__ __forceinline__ void sts4(float* base, int i, f3 v) {
   base[4 * i + 2] = v.z;
 }
 
-// Sum over the 32 lanes of a half-wave (fixed butterfly order).
+// Sum over the 32 lanes of a half-wave (sum32: fixed order).
 __device__ __forceinline__ f3 half_sum(f3 v) {
-  for (int m = kBankSlots / 2; m > 0; m >>= 1) {
-    v.x += __shfl_xor(v.x, m, kBankSlots);
-    v.y += __shfl_xor(v.y, m, kBankSlots);
-    v.z += __shfl_xor(v.z, m, kBankSlots);
-  }
-  return v;
+  return {sum32(v.x), sum32(v.y), sum32(v.z)};
 }
 
 // Workgroup sums of two doubles at once, valid in thread 0.
@@ -233,29 +235,44 @@ __global__ void __launch_bounds__(kPcgThreads)
   const int nh = a.halo_ptr[blockIdx.x + 1] - h0;
   const size_t hl = static_cast<size_t>(kPcgThreads) + a.nhalo_max;
 
-  float* invc = dyn;                                           // ncb_max x 1024
-  float* csum = invc + static_cast<size_t>(a.ncb_max) * 1024;  // 4 x ncn_max
-  float* czl = csum + 4 * static_cast<size_t>(a.ncn_max);     // 4 x ncn_max
+  float* invc = dyn;                                           // cbw_max x 1024
+  float* csum = invc + static_cast<size_t>(a.cbw_max) * 1024;  // 4 x n1_max
+  float* czl = csum + 4 * static_cast<size_t>(a.n1_max);      // 4 x ncn_max
   float* qs = czl + 4 * static_cast<size_t>(a.ncn_max);       // 4 x n1_max
-  float* pw = qs + 4 * static_cast<size_t>(a.n1_max);         // 8 x nwg_max
-  float* zl = pw + 8 * static_cast<size_t>(a.nwg_max);        // 4 x hl
+  float* cin = qs + 4 * static_cast<size_t>(a.n1_max);        // 4 x 32
+  float* cpart = cin + 4 * kBankSlots;                         // 4 x 256
+  float* pw = cpart + 4 * kPcgThreads;                         // parts x nwg_max
+  float* cpw = pw + kPcgPartWords * static_cast<size_t>(a.nwg_max);  // 4 x ncb_max
+  float* zl = cpw + 4 * static_cast<size_t>(a.ncb_max);       // 4 x hl
   float* pl = zl + 4 * hl;                                     // 4 x hl
-  int* cbm = reinterpret_cast<int*>(pl + 4 * hl);              // 32 x ncb_max
-  int* cnp = cbm + kBankSlots * a.ncb_max;                     // ncn_max + 1
+  int* cbm = reinterpret_cast<int*>(pl + 4 * hl);              // 32 x cbw_max
+  int* cnp = cbm + kBankSlots * a.cbw_max;                     // ncn_max + 1
   int* cnl = cnp + a.ncn_max + 1;                              // cnl_max
+  int* nl1 = cnl + a.cnl_max;                                  // need1_max
+  const int n1_0 = a.wg_n1_ptr[blockIdx.x];
+  const int nn1 = a.wg_n1_ptr[blockIdx.x + 1] - n1_0;
+  const int c_0 = a.wg_c_ptr[blockIdx.x];
+  const int nc = a.wg_c_ptr[blockIdx.x + 1] - c_0;
 
-  // Static data into LDS.
-  for (int e = t; e < S.ncb * 1024; e += kPcgThreads)
-    invc[e] = a.bank_inv[static_cast<size_t>(a.cb_bank[S.cb0 + e / 1024]) * 1024 +
-                         (e & 1023)];
-  for (int e = t; e < S.ncb * kBankSlots; e += kPcgThreads)
-    cbm[e] = a.cb_member[kBankSlots * S.cb0 + e];
+  // Static data into LDS: the inverses and members of the coarse banks this
+  // workgroup solves (cb = j, j + nwg, ...), every coarse node's level-1
+  // list, and the level-1 nodes those banks read.
+  for (int e = t; e < a.cbw_max * 1024; e += kPcgThreads) {
+    const int cb = j + S.nwg * (e / 1024);
+    if (cb < S.ncb)
+      invc[e] = a.bank_inv[static_cast<size_t>(a.cb_bank[S.cb0 + cb]) * 1024 + (e & 1023)];
+  }
+  for (int e = t; e < a.cbw_max * kBankSlots; e += kPcgThreads) {
+    const int cb = j + S.nwg * (e / kBankSlots);
+    cbm[e] = cb < S.ncb ? a.cb_member[kBankSlots * (S.cb0 + cb) + e % kBankSlots] : -1;
+  }
   const int cn_first = S.ncn > 0 ? a.cn_ptr[S.cn0] : 0;
   for (int e = t; e <= S.ncn; e += kPcgThreads)
     cnp[e] = a.cn_ptr[S.cn0 + e] - cn_first;
   for (int e = t; S.ncn > 0 && e < a.cn_ptr[S.cn0 + S.ncn] - cn_first;
        e += kPcgThreads)
     cnl[e] = a.cn_list[cn_first + e];
+  for (int e = t; e < nn1; e += kPcgThreads) nl1[e] = a.wg_n1[n1_0 + e];
 
   // This slot's row of its level-0 bank inverse.
   float winv[kBankSlots];
@@ -286,6 +303,15 @@ __global__ void __launch_bounds__(kPcgThreads)
   int scn[4] = {-1, -1, -1, -1};
   if (in_bank && coarse)
     for (int l = 0; l < 4; ++l) scn[l] = a.slot_cn[4 * slot + l];
+  // Coarse nodes of the halo slots this thread completes.
+  int hsc[kPcgHaloPerThread][4];
+#pragma unroll
+  for (int m = 0; m < kPcgHaloPerThread; ++m) {
+    const int hh = t + kPcgThreads * m;
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+      hsc[m][l] = coarse && hh < nh ? a.slot_cn[4 * a.halo_slot[h0 + hh] + l] : -1;
+  }
   const float dd = own ? a.d[row] : 0.0f;
   f3 r = own ? ld3(a.r0, row) : f3{0.0f, 0.0f, 0.0f};
   f3 dx = {0.0f, 0.0f, 0.0f};
@@ -299,14 +325,19 @@ __global__ void __launch_bounds__(kPcgThreads)
   }
   const int n1_count = n1_last >= 0 ? n1_last - n1_first + 1 : 0;
 
-  // Exchange 1: level-1 sums (3 words per node) then p.q (2 per workgroup).
-  // Exchange 2: r.z, r.r (4 per workgroup) then z of the halo (3 per slot).
+  // Exchange 1: the level-1 sums the coarse banks of this workgroup read
+  // (3 words per node), then the p.q partials.  Exchange 2: the r.z0 and r.r
+  // partials, each coarse bank's r.z_coarse (2 words), z0 of the halo and the
+  // corrections of the coarse nodes the slots add (3 words each).
+  constexpr int kWaveWords = 2 * kPcgWaves;  // one double per wavefront
   Share x1, x2;
   {
-    const int nw1 = coarse ? 3 * S.n1 : 0;
-    const int w1 = nw1 + 2 * S.nwg;
-    const int pw2 = 4 * S.nwg;
-    const int w2 = pw2 + 3 * nh;
+    const int nw1 = 3 * nn1;
+    const int w1 = nw1 + kWaveWords * S.nwg;
+    const int pw2 = 2 * kWaveWords * S.nwg;
+    const int cw2 = pw2 + 2 * S.ncb;
+    const int hw2 = cw2 + 3 * nh;
+    const int w2 = hw2 + 3 * nc;
     x1.mask = 0;
     x2.mask = 0;
 #pragma unroll
@@ -317,12 +348,13 @@ __global__ void __launch_bounds__(kPcgThreads)
       if (i < w1) {
         x1.mask |= 1u << m;
         if (i < nw1) {
-          x1.addr[m] = a.tn + 3 * static_cast<size_t>(S.cn0 + i / 3) + i % 3;
-          x1.dest[m] = static_cast<int>(qs - dyn) + 4 * (i / 3) + i % 3;
+          const int n = a.wg_n1[n1_0 + i / 3];
+          x1.addr[m] = a.tn + 3 * static_cast<size_t>(S.cn0 + n) + i % 3;
+          x1.dest[m] = static_cast<int>(qs - dyn) + 4 * n + i % 3;
         } else {
-          const int w = (i - nw1) / 2, h = (i - nw1) % 2;
-          x1.addr[m] = a.tp + 8 * static_cast<size_t>(S.wg0 + w) + h;
-          x1.dest[m] = static_cast<int>(pw - dyn) + 8 * w + h;
+          const int w = (i - nw1) / kWaveWords, h = (i - nw1) % kWaveWords;
+          x1.addr[m] = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + h;
+          x1.dest[m] = static_cast<int>(pw - dyn) + kPcgPartWords * w + h;
         }
       }
       x2.addr[m] = a.tp;
@@ -330,18 +362,28 @@ __global__ void __launch_bounds__(kPcgThreads)
       if (i < w2) {
         x2.mask |= 1u << m;
         if (i < pw2) {
-          const int w = i / 4, h = i % 4;
-          x2.addr[m] = a.tp + 8 * static_cast<size_t>(S.wg0 + w) + 2 + h;
-          x2.dest[m] = static_cast<int>(pw - dyn) + 8 * w + 2 + h;
-        } else {
-          const int hh = (i - pw2) / 3, c = (i - pw2) % 3;
+          const int w = i / (2 * kWaveWords), h = kWaveWords + i % (2 * kWaveWords);
+          x2.addr[m] = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + h;
+          x2.dest[m] = static_cast<int>(pw - dyn) + kPcgPartWords * w + h;
+        } else if (i < cw2) {
+          x2.addr[m] = a.tcp + 2 * static_cast<size_t>(S.cb0) + (i - pw2);
+          x2.dest[m] = static_cast<int>(cpw - dyn) + (i - pw2);
+        } else if (i < hw2) {
+          const int hh = (i - cw2) / 3, c = (i - cw2) % 3;
           x2.addr[m] = a.tz + 3 * static_cast<size_t>(a.halo_slot[h0 + hh]) + c;
           x2.dest[m] = static_cast<int>(zl - dyn) + 4 * (kPcgThreads + hh) + c;
+        } else {
+          const int n = a.wg_c[c_0 + (i - hw2) / 3], c = (i - hw2) % 3;
+          x2.addr[m] = a.tc + 3 * static_cast<size_t>(S.cn0 + n) + c;
+          x2.dest[m] = static_cast<int>(czl - dyn) + 4 * n + c;
         }
       }
     }
   }
-  unsigned long long* my_tp = a.tp + 8 * static_cast<size_t>(blockIdx.x);
+  const int wave = t / kWave;
+  const bool lane0 = (t % kWave) == 0;
+  unsigned long long* my_tp =
+      a.tp + kPcgPartWords * static_cast<size_t>(blockIdx.x) + 2 * wave;
   __syncthreads();
 
   // Sum of v over the slots of this thread's level-1 node, published by the
@@ -354,11 +396,9 @@ __global__ void __launch_bounds__(kPcgThreads)
     }
   };
 
-  // z = M r from this thread's r, its bank row and the level-1 sums in csum.
-  auto mas = [&](f3 rv) -> f3 {
-    rb[t][0] = rv.x;
-    rb[t][1] = rv.y;
-    rb[t][2] = rv.z;
+  // Level-0 part of z = M r: this slot's bank row times r of its bank.
+  auto mas_l0 = [&](f3 rv) -> f3 {
+    sts4(&rb[0][0], t, rv);
     __syncthreads();
     f3 z0 = {0.0f, 0.0f, 0.0f};
     if (in_bank) {
@@ -371,89 +411,118 @@ __global__ void __launch_bounds__(kPcgThreads)
         z0.z += w * rk.z;
       }
     }
-    if (!coarse) return z0;
-    // Sums of the deeper coarse nodes from their level-1 nodes: one
-    // half-wave per node, the level-1 nodes strided over its lanes.
-    for (int n = S.n1 + hw; n < S.ncn; n += kBanksPerWg) {
-      const int e_end = cnp[n + 1];
-      f3 v = {0.0f, 0.0f, 0.0f};
-      for (int e = cnp[n] + s_in_bank; e < e_end; e += 4 * kBankSlots) {
-        int id[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          id[u] = e + u * kBankSlots < e_end ? cnl[e + u * kBankSlots] : -1;
-        f3 c[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          c[u] = id[u] >= 0 ? lds4(csum, id[u]) : f3{0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (id[u] >= 0) v = v + c[u];
-      }
-      v = half_sum(v);
-      if (s_in_bank == 0) sts4(csum, n, v);
-    }
-    __syncthreads();
-    // Coarse banks: one thread per (bank, member); the bank's inputs are
-    // read by the whole half-wave at once.
-    // All loads of a dot product are issued before its first product.
-    for (int wk = t; wk < S.ncb * kBankSlots; wk += kPcgThreads) {
-      const int cb = wk / kBankSlots, i = wk % kBankSlots;
-      const float* inv = invc + static_cast<size_t>(cb) * 1024;
-      const int* mem = cbm + kBankSlots * cb;
-      f3 zc = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int k0 = 0; k0 < kBankSlots; k0 += 16) {
-        int mk[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) mk[k] = mem[k0 + k];
-        f3 v[16];
-        float w[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-          v[k] = mk[k] >= 0 ? lds4(csum, mk[k]) : f3{0.0f, 0.0f, 0.0f};
-          w[k] = inv[(k0 + k) * kBankSlots + i];
-        }
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-          zc.x += w[k] * v[k].x;
-          zc.y += w[k] * v[k].y;
-          zc.z += w[k] * v[k].z;
-        }
-      }
-      if (mem[i] >= 0) sts4(czl, mem[i], zc);
-    }
-    __syncthreads();
-    if (in_bank) {
-#pragma unroll
-      for (int l = 0; l < 4; ++l)
-        if (scn[l] >= 0) z0 = z0 + lds4(czl, scn[l]);
-    }
     return z0;
+  };
+
+  // The coarse banks of this workgroup: each member's input is the sum of r
+  // over its level-1 nodes (eight lanes per member); the product with the
+  // bank inverse is split over the eight half-waves (four inputs each) and
+  // joined by the first one, which publishes the members' corrections and
+  // the bank's r . z_coarse (the correction is constant over a node).
+  auto solve_banks = [&](unsigned e) {
+    int ob = 0;
+    for (int cb = j; cb < S.ncb; cb += S.nwg, ++ob) {
+      const int* mem = cbm + kBankSlots * ob;
+      {
+        const int k = t / 8, sub = t & 7;
+        const int m = mem[k];
+        f3 v = {0.0f, 0.0f, 0.0f};
+        if (m >= 0) {
+          const int e_end = cnp[m + 1];
+          for (int ee = cnp[m] + sub; ee < e_end; ee += 4 * 8) {
+            int id[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) id[u] = ee + 8 * u < e_end ? cnl[ee + 8 * u] : -1;
+            f3 c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              c[u] = id[u] >= 0 ? lds4(csum, id[u]) : f3{0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (id[u] >= 0) v = v + c[u];
+          }
+        }
+        v = {sum8(v.x), sum8(v.y), sum8(v.z)};
+        if (sub == 0) sts4(cin, k, v);
+      }
+      __syncthreads();
+      {
+        const float* inv = invc + static_cast<size_t>(ob) * 1024;
+        const int k0 = 4 * hw;
+        f3 part = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int k = k0; k < k0 + 4; ++k) {
+          const float w = inv[k * kBankSlots + s_in_bank];
+          const f3 c = lds4(cin, k);
+          part.x += w * c.x;
+          part.y += w * c.y;
+          part.z += w * c.z;
+        }
+        sts4(cpart, t, part);
+      }
+      __syncthreads();
+      if (t < kBankSlots) {
+        f3 zc = lds4(cpart, t);
+#pragma unroll
+        for (int q = 1; q < kBanksPerWg; ++q) zc = zc + lds4(cpart, q * kBankSlots + t);
+        const int mi = mem[t];
+        if (mi >= 0) put_f3(a.tc + 3 * static_cast<size_t>(S.cn0 + mi), e, zc);
+        const double d = sum32(mi >= 0 ? ddot3(lds4(cin, t), zc) : 0.0);
+        if (t == 0) put_double(a.tcp + 2 * static_cast<size_t>(S.cb0 + cb), e, d);
+      }
+      if (cb + S.nwg < S.ncb) __syncthreads();
+    }
+  };
+
+  // z of an own slot: the level-0 part plus the coarse nodes' corrections,
+  // in level order; halo slots are completed with the same additions.
+  auto add_coarse = [&](f3 z0, const int* cn) -> f3 {
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+      if (cn[l] >= 0) z0 = z0 + lds4(czl, cn[l]);
+    return z0;
+  };
+
+  // Publishes z0 of the own slot and this wavefront's r.z0 and r.r, solves
+  // this workgroup's coarse banks, then collects exchange 2; sets z and the
+  // totals.  False when the exchange gave up.
+  auto finish_z = [&](f3 z0, f3& z_out, double& rz_out, double& rr_out) -> bool {
+    ++ep;
+    if (own) put_f3(a.tz + 3 * static_cast<size_t>(slot), ep, z0);
+    {
+      const double s0 = wave_sum(own ? ddot3(r, z0) : 0.0);
+      const double s1 = wave_sum(own ? ddot3(r, r) : 0.0);
+      if (lane0) {
+        put_double(my_tp + kWaveWords, ep, s0);
+        put_double(my_tp + 2 * kWaveWords, ep, s1);
+      }
+    }
+    if (coarse) solve_banks(ep);
+    if (!__syncthreads_and(collect(x2, ep, dyn, a.abort_flag))) return false;
+    z_out = coarse && in_bank ? add_coarse(z0, scn) : z0;
+    double rz0;
+    part_totals(pw, S.nwg, kWaveWords, 2 * kWaveWords, rz0, rr_out);
+    double rzc = 0.0;
+    for (int cb = 0; cb < S.ncb; ++cb) {
+      const unsigned long long lo = __float_as_uint(cpw[2 * cb]);
+      const unsigned long long hi = __float_as_uint(cpw[2 * cb + 1]);
+      rzc += __longlong_as_double(static_cast<long long>((hi << 32) | lo));
+    }
+    rz_out = rz0 + rzc;
+    return true;
   };
 
   // ---- prologue: level-1 sums of r0, z0 = M r0, r.z, r.r
   if (coarse) {
     ++ep;
     publish_node_sums(r, ep);
-    if (t == 0) put_double(my_tp, ep, 0.0);
+    if (lane0) put_double(my_tp, ep, 0.0);
     if (!__syncthreads_and(collect(x1, ep, dyn, a.abort_flag))) return;
-    for (int n = t; n < S.n1; n += kPcgThreads) sts4(csum, n, lds4(qs, n));
+    for (int e = t; e < nn1; e += kPcgThreads) sts4(csum, nl1[e], lds4(qs, nl1[e]));
   }
-  f3 z = mas(r);
-  ++ep;
-  if (own) put_f3(a.tz + 3 * static_cast<size_t>(slot), ep, z);
-  {
-    double s0 = own ? ddot3(r, z) : 0.0, s1 = own ? ddot3(r, r) : 0.0;
-    block_sum2(s0, s1, red);
-    if (t == 0) {
-      put_double(my_tp + 2, ep, s0);
-      put_double(my_tp + 4, ep, s1);
-    }
-  }
-  if (!__syncthreads_and(collect(x2, ep, dyn, a.abort_flag))) return;
   double rz, rr;
-  part_totals(pw, S.nwg, 2, 4, rz, rr);
+  f3 z;
+  if (!finish_z(mas_l0(r), z, rz, rr)) return;
   const double rr0 = rr;
   const double abs2 = static_cast<double>(a.abs_tol) * a.abs_tol;
   const double rel2 = static_cast<double>(rel_tol) * rel_tol;
@@ -464,10 +533,15 @@ __global__ void __launch_bounds__(kPcgThreads)
   while (status < 0) {
     p = (iter == 0) ? z : fma3(beta, p, z);
     sts4(pl, t, p);
-    for (int hh = t; hh < nh; hh += kPcgThreads) {
-      const f3 zh = lds4(zl, kPcgThreads + hh);
-      sts4(pl, kPcgThreads + hh,
-           iter == 0 ? zh : fma3(beta, lds4(pl, kPcgThreads + hh), zh));
+#pragma unroll
+    for (int m = 0; m < kPcgHaloPerThread; ++m) {
+      const int hh = t + kPcgThreads * m;
+      if (hh < nh) {
+        const f3 zh = coarse ? add_coarse(lds4(zl, kPcgThreads + hh), hsc[m])
+                             : lds4(zl, kPcgThreads + hh);
+        sts4(pl, kPcgThreads + hh,
+             iter == 0 ? zh : fma3(beta, lds4(pl, kPcgThreads + hh), zh));
+      }
     }
     __syncthreads();
     // q = (K + D) p in difference form.
@@ -503,8 +577,8 @@ __global__ void __launch_bounds__(kPcgThreads)
     ++ep;
     if (coarse) publish_node_sums(q, ep);
     {
-      const double s = block_sum(own ? ddot3(p, q) : 0.0, red);
-      if (t == 0) put_double(my_tp, ep, s);
+      const double s = wave_sum(own ? ddot3(p, q) : 0.0);
+      if (lane0) put_double(my_tp, ep, s);
     }
     if (!__syncthreads_and(collect(x1, ep, dyn, a.abort_flag))) return;
     double pAp, unused;
@@ -512,26 +586,14 @@ __global__ void __launch_bounds__(kPcgThreads)
     const float alpha = static_cast<float>(fabs(pAp) < 1e-20 ? 0.0 : rz / pAp);
     dx = fma3(alpha, p, dx);
     r = fma3(-alpha, q, r);
-    if (coarse)
-      for (int n = t; n < S.n1; n += kPcgThreads) {
-        const f3 qn = lds4(qs, n), cn = lds4(csum, n);
-        sts4(csum, n, {fmaf(-alpha, qn.x, cn.x), fmaf(-alpha, qn.y, cn.y),
-                       fmaf(-alpha, qn.z, cn.z)});
-      }
-    z = mas(r);
-    ++ep;
-    if (own) put_f3(a.tz + 3 * static_cast<size_t>(slot), ep, z);
-    {
-      double s0 = own ? ddot3(r, z) : 0.0, s1 = own ? ddot3(r, r) : 0.0;
-      block_sum2(s0, s1, red);
-      if (t == 0) {
-        put_double(my_tp + 2, ep, s0);
-        put_double(my_tp + 4, ep, s1);
-      }
+    for (int e = t; e < nn1; e += kPcgThreads) {
+      const int n = nl1[e];
+      const f3 qn = lds4(qs, n), cn = lds4(csum, n);
+      sts4(csum, n, {fmaf(-alpha, qn.x, cn.x), fmaf(-alpha, qn.y, cn.y),
+                     fmaf(-alpha, qn.z, cn.z)});
     }
-    if (!__syncthreads_and(collect(x2, ep, dyn, a.abort_flag))) return;
     double rz_new;
-    part_totals(pw, S.nwg, 2, 4, rz_new, rr);
+    if (!finish_z(mas_l0(r), z, rz_new, rr)) return;
     ++iter;
     if (iter >= a.max_iter) status = 2;
     else if (rr <= abs2) status = 0;
@@ -570,13 +632,16 @@ size_t g_lds_checked = static_cast<size_t>(-1);
 
 size_t pcg_persistent_lds(const PersistDev& a) {
   const size_t hl = static_cast<size_t>(kPcgThreads) + a.nhalo_max;
-  return sizeof(float) * (static_cast<size_t>(a.ncb_max) * 1024 +
-                          8 * static_cast<size_t>(a.ncn_max) +
-                          4 * static_cast<size_t>(a.n1_max) +
-                          8 * static_cast<size_t>(a.nwg_max) + 8 * hl) +
-         sizeof(int) * (static_cast<size_t>(kBankSlots) * a.ncb_max +
+  return sizeof(float) * (static_cast<size_t>(a.cbw_max) * 1024 +
+                          8 * static_cast<size_t>(a.n1_max) +
+                          4 * static_cast<size_t>(a.ncn_max) + 4 * kBankSlots +
+                          4 * kPcgThreads +
+                          kPcgPartWords * static_cast<size_t>(a.nwg_max) +
+                          4 * static_cast<size_t>(a.ncb_max) + 8 * hl) +
+         sizeof(int) * (static_cast<size_t>(kBankSlots) * a.cbw_max +
                         static_cast<size_t>(a.ncn_max) + 1 +
-                        static_cast<size_t>(a.cnl_max));
+                        static_cast<size_t>(a.cnl_max) +
+                        static_cast<size_t>(a.need1_max));
 }
 
 bool pcg_persistent_fits(int grid, size_t shmem) {
