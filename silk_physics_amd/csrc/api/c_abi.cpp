@@ -415,11 +415,8 @@ int silk_hip_debug_ccd(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
     std::vector<int> hv(is_vf, is_vf + m);
     std::vector<double> h0(p0, p0 + 12 * m), h1(p1, p1 + 12 * m),
         he(err, err + 3 * m);
-    DevBuf<int> dv, dhit, dwork;
+    DevBuf<int> dv, dhit;
     DevBuf<double> d0, d1, de, dtoi;
-    dwork.resize(1);
-    silk::amd::hip_check(hipMemsetAsync(dwork.get(), 0, sizeof(int), c.stream_),
-                         "ccd work");
     dv.upload(hv, c.stream_);
     d0.upload(h0, c.stream_);
     d1.upload(h1, c.stream_);
@@ -428,7 +425,7 @@ int silk_hip_debug_ccd(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
     dtoi.resize(m);
     silk::amd::launch_ccd_each(c.stream_, static_cast<int>(n), dv.get(),
                                d0.get(), d1.get(), de.get(), ms, tolerance,
-                               max_iter, dwork.get(), dhit.get(), dtoi.get());
+                               max_iter, dhit.get(), dtoi.get());
     silk::amd::hip_check(hipGetLastError(), "ccd launch");
     silk::amd::hip_check(hipMemcpyAsync(hit, dhit.get(), m * sizeof(int),
                                         hipMemcpyDeviceToHost, c.stream_),
@@ -457,11 +454,8 @@ int silk_hip_debug_ccd_min(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
     std::vector<int> hv(is_vf, is_vf + m);
     std::vector<double> h0(p0, p0 + 12 * m), h1(p1, p1 + 12 * m),
         he(err, err + 3 * m), one{1.0};
-    DevBuf<int> dv, dwork;
+    DevBuf<int> dv;
     DevBuf<double> d0, d1, de, dmin;
-    dwork.resize(1);
-    silk::amd::hip_check(hipMemsetAsync(dwork.get(), 0, sizeof(int), c.stream_),
-                         "ccd work");
     dv.upload(hv, c.stream_);
     d0.upload(h0, c.stream_);
     d1.upload(h1, c.stream_);
@@ -469,7 +463,7 @@ int silk_hip_debug_ccd_min(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
     dmin.upload(one, c.stream_);
     silk::amd::launch_ccd_min(
         c.stream_, static_cast<int>(n), dv.get(), d0.get(), d1.get(), de.get(),
-        ms, tolerance, max_iter, dwork.get(),
+        ms, tolerance, max_iter,
         reinterpret_cast<unsigned long long*>(dmin.get()));
     silk::amd::hip_check(hipGetLastError(), "ccd launch");
     silk::amd::hip_check(hipMemcpyAsync(min_toi, dmin.get(), sizeof(double),

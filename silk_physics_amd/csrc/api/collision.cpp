@@ -159,7 +159,7 @@ class HipCollision final : public CollisionWorld {
   int pt_cap_ = 1 << 16, ee_cap_ = 1 << 16;
   DevBuf<DcdRec> recs_;
   int rec_cap_ = 1 << 12;
-  DevBuf<int> counters_;  // pt, ee, dcd, ccd work (zeroed per broadphase)
+  DevBuf<int> counters_;  // pt, ee, dcd, pad (zeroed per broadphase)
   DevBuf<unsigned long long> toi_bits_;
   DevBuf<double> err6_;
   int* h_io_ = nullptr;  // pinned: counters[4] then toi bits (8 bytes)
@@ -390,7 +390,7 @@ void HipCollision::build_jobs(const std::vector<float>& pos0) {
     j.nitem = q.n << j.depth;
     j.gate_a = ga;
     j.gate_b = gb;
-    items += (j.nitem + 63) / 64 * 64;
+    items += (j.nitem + kTraverseWg - 1) / kTraverseWg * kTraverseWg;
     jobs_h_.push_back(j);
   };
   auto root = [&](const ObjDesc& o) { return trees_[o.tree_t].node + 1; };
@@ -677,7 +677,7 @@ float HipCollision::min_toi(const float* outer_x, const float* inner_x,
     ctx_.timed_begin(kTimeCcd);
     launch_ccd(st_, verts(), tables(), pt_pairs_.get(), counters_.get(), pt_cap_,
                ee_pairs_.get(), counters_.get() + 1, ee_cap_, cp,
-               counters_.get() + 3, toi_bits_.get());
+               toi_bits_.get());
     ctx_.timed_end();
     read_io();
     ctx_.timed_collect();

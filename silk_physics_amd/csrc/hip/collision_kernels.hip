@@ -87,7 +87,8 @@ __global__ void __launch_bounds__(kRefitThreads)
 }
 
 // ---------------------------------------------------------------- traversal
-constexpr int kTraverseThreads = 256;
+constexpr int kTraverseThreads = kTraverseWg;
+constexpr int kPairStage = 2048;  // pairs a workgroup stages in LDS
 
 // All passes of a broadphase in one launch.  A work item is (job, subtree,
 // query): lane `item` walks the subtree rooted at heap node 2^depth + sub
@@ -96,73 +97,94 @@ constexpr int kTraverseThreads = 256;
 // otherwise climb while on a right child and step to the right sibling).
 // Parent boxes contain their children, so the leaves reached over all
 // subtrees are the leaves of a walk from the root (oibvh.cuh:417-516).
+// Pairs are staged in LDS and written out with one counter reservation per
+// workgroup (one shared counter per pair serialised the whole launch on it);
+// pairs beyond the stage go out one by one.
 __global__ void __launch_bounds__(kTraverseThreads)
     k_broadphase(CollVerts v, const TraverseJob* __restrict__ jobs, int njobs,
                  PairOut pt, PairOut ee) {
-  // Job ranges are whole wavefronts: the job is uniform over the wavefront.
-  const int wave0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63);
+  __shared__ int2 stage[kPairStage];
+  __shared__ int stage_n, stage_base;
+  // Job ranges are whole workgroups: the job is uniform over the workgroup.
+  const int wg0 = blockIdx.x * blockDim.x;
   int j = 0;
-  while (j + 1 < njobs && jobs[j + 1].item0 <= wave0) ++j;
+  while (j + 1 < njobs && jobs[j + 1].item0 <= wg0) ++j;
   j = __builtin_amdgcn_readfirstlane(j);
   const TraverseJob& job = jobs[j];
-  const int local = wave0 + static_cast<int>(__lane_id()) - job.item0;
-  if (local >= job.nitem) return;
+  const int local = wg0 + static_cast<int>(threadIdx.x) - job.item0;
+  if (threadIdx.x == 0) stage_n = 0;
+  __syncthreads();
+  const PairOut out = job.kind == 0 ? pt : ee;
+  bool active = local < job.nitem;
   if (job.gate_a != nullptr && job.gate_b != nullptr &&
       !overlap(*job.gate_a, *job.gate_b))
-    return;
-  const QuerySet q = job.q;
-  const TreeDev t = job.tree;
-  const int kind = job.kind, self = job.self;
-  const PairOut out = kind == 0 ? pt : ee;
-  const int sub = local / q.n;
-  const int li = q.order[local - sub * q.n];
-  const int qi = q.gid0 + li;
-  int qv[2];
-  for (int k = 0; k < q.nvert; ++k) qv[k] = q.prim[q.nvert * li + k];
-  Box6 qb;
-  swept_box(v, qv, q.nvert, q.pad, qb);
-  float qinv = 0.0f;
-  for (int k = 0; k < q.nvert; ++k) qinv += v.invm[qv[k]];
-  const int r = (1 << job.depth) + sub;
-  int node = r;
-  for (;;) {
-    bool down = false;
-    if (overlap(qb, t.node[node])) {
-      if (node < t.P) {
-        down = true;
-      } else {
-        const int j0 = (node - t.P) * kLeafPrims;
-        const int j1 = min(j0 + kLeafPrims, t.nprim);
-        for (int jj = j0; jj < j1; ++jj) {
-          if (!overlap(qb, t.sbox[jj])) continue;
-          int tv[3];
-          float inv = qinv;
-          for (int k = 0; k < t.nvert; ++k) {
-            tv[k] = t.prim[t.nvert * jj + k];
-            inv += v.invm[tv[k]];
+    active = false;
+  if (active) {
+    const QuerySet q = job.q;
+    const TreeDev t = job.tree;
+    const int kind = job.kind, self = job.self;
+    const int sub = local / q.n;
+    const int li = q.order[local - sub * q.n];
+    const int qi = q.gid0 + li;
+    int qv[2];
+    for (int k = 0; k < q.nvert; ++k) qv[k] = q.prim[q.nvert * li + k];
+    Box6 qb;
+    swept_box(v, qv, q.nvert, q.pad, qb);
+    float qinv = 0.0f;
+    for (int k = 0; k < q.nvert; ++k) qinv += v.invm[qv[k]];
+    const int r = (1 << job.depth) + sub;
+    int node = r;
+    for (;;) {
+      bool down = false;
+      if (overlap(qb, t.node[node])) {
+        if (node < t.P) {
+          down = true;
+        } else {
+          const int j0 = (node - t.P) * kLeafPrims;
+          const int j1 = min(j0 + kLeafPrims, t.nprim);
+          for (int jj = j0; jj < j1; ++jj) {
+            if (!overlap(qb, t.sbox[jj])) continue;
+            int tv[3];
+            float inv = qinv;
+            for (int k = 0; k < t.nvert; ++k) {
+              tv[k] = t.prim[t.nvert * jj + k];
+              inv += v.invm[tv[k]];
+            }
+            if (inv == 0.0f) continue;  // all pinned / kinematic
+            const int tid_orig = t.prim_id[jj];
+            if (self) {
+              bool adjacent = false;
+              for (int a = 0; a < t.nvert; ++a)
+                for (int b = 0; b < q.nvert; ++b) adjacent |= (tv[a] == qv[b]);
+              if (adjacent) continue;
+              if (kind == 1 && !(tid_orig < qi)) continue;  // each pair once
+            }
+            const int2 pr = make_int2(tid_orig, qi);
+            const int slot = atomicAdd(&stage_n, 1);
+            if (slot < kPairStage) {
+              stage[slot] = pr;
+            } else {
+              const int g = atomicAdd(out.count, 1);
+              if (g < out.cap) out.pairs[g] = pr;
+            }
           }
-          if (inv == 0.0f) continue;  // all pinned / kinematic
-          const int tid_orig = t.prim_id[jj];
-          if (self) {
-            bool adjacent = false;
-            for (int a = 0; a < t.nvert; ++a)
-              for (int b = 0; b < q.nvert; ++b) adjacent |= (tv[a] == qv[b]);
-            if (adjacent) continue;
-            if (kind == 1 && !(tid_orig < qi)) continue;  // each pair once
-          }
-          const int slot = atomicAdd(out.count, 1);
-          if (slot < out.cap) out.pairs[slot] = make_int2(tid_orig, qi);
         }
       }
+      if (down) {
+        node <<= 1;
+        continue;
+      }
+      while (node != r && (node & 1)) node >>= 1;
+      if (node == r) break;
+      ++node;
     }
-    if (down) {
-      node <<= 1;
-      continue;
-    }
-    while (node != r && (node & 1)) node >>= 1;
-    if (node == r) break;
-    ++node;
   }
+  __syncthreads();
+  const int staged = min(stage_n, kPairStage);
+  if (threadIdx.x == 0) stage_base = staged > 0 ? atomicAdd(out.count, staged) : 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < staged; i += blockDim.x)
+    if (stage_base + i < out.cap) out.pairs[stage_base + i] = stage[i];
 }
 
 // ---------------------------------------------------------------- DCD
@@ -842,21 +864,20 @@ struct EachOut {
   }
 };
 
-// CCD, one query per wavefront at a time; wavefronts take queries from the
-// zero-initialised counter *next.  Per query (ccd.cpp:235-256,
-// narrowphase.cu:70-76): the root search at ms; a root at t = 0 is searched
-// again at ms / 2, where no root means a collision at t = 1 and a root at
-// t = 0 means none.
+// CCD, one query per wavefront at a time: wavefront w of W takes queries
+// w, w + W, ... (the grid holds as many wavefronts as can be resident; a
+// shared query counter serialised every wavefront on one atomic).  Per query
+// (ccd.cpp:235-256, narrowphase.cu:70-76): the root search at ms; a root at
+// t = 0 is searched again at ms / 2, where no root means a collision at t = 1
+// and a root at t = 0 means none.
 template <class In, class Out>
 __global__ void __launch_bounds__(256)
-    k_ccd(In in, Out out, int* next, double tolerance, int max_iter) {
+    k_ccd(In in, Out out, double tolerance, int max_iter) {
   const int n = in.count();
   const int lane = static_cast<int>(__lane_id());
-  for (;;) {
-    // Every lane takes part (lane 0 adds one): the query index must not
-    // come from a branch on the lane.
-    const int q = __builtin_amdgcn_readfirstlane(atomicAdd(next, lane == 0 ? 1 : 0));
-    if (q >= n) break;
+  const int nwave = static_cast<int>(gridDim.x * blockDim.x) / dev::kWave;
+  const int wave0 = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) / dev::kWave);
+  for (int q = __builtin_amdgcn_readfirstlane(wave0); q < n; q += nwave) {
     bool vf = false;
     double p0[4][3], p1[4][3], err[3];
     in.load(q, &vf, p0, p1, err);
@@ -968,6 +989,49 @@ __device__ __forceinline__ void gs_core(const ContactDev& c, float* P,
   }
 }
 
+// gs_core for a contact whose state vertices are distinct: the positions
+// read for u are reused for the update (nothing else writes them during the
+// round: contacts of one colour share no vertex), which saves a second read.
+__device__ __forceinline__ void gs_core_once(const ContactDev& c, float* P,
+                                             dev::f3& corr) {
+  using namespace dev;
+  f3 pv[4];
+  f3 u = {c.affine[0], c.affine[1], c.affine[2]};
+  for (int s = 0; s < 4; ++s) {
+    pv[s] = f3{0.0f, 0.0f, 0.0f};
+    if (c.state[s] >= 0) {
+      pv[s] = ld3(P, c.state[s]);
+      u = u + c.coef[s] * pv[s];
+    }
+  }
+  const f3 old = corr;
+  const f3 us = u - old;
+  const f3 n = {c.normal[0], c.normal[1], c.normal[2]};
+  const float un = dot3(us, n);
+  f3 unew;
+  if (un >= 0.0f) {
+    unew = us;
+  } else {
+    const f3 ut = us - un * n;
+    const float utn = sqrtf(dot3(ut, ut));
+    const float rad = -c.mu * un;
+    unew = (utn <= rad) ? f3{0.0f, 0.0f, 0.0f} : (1.0f - rad / utn) * ut;
+  }
+  const f3 du = unew - u;
+  corr = old + du;
+  for (int s = 0; s < 4; ++s) {
+    if (c.gamma[s] == 0.0f) continue;
+    st3(P, c.state[s], pv[s] + c.gamma[s] * du);
+  }
+}
+
+__device__ __forceinline__ bool distinct_states(const ContactDev& c) {
+  for (int a = 0; a < 4; ++a)
+    for (int b = a + 1; b < 4; ++b)
+      if (c.state[a] >= 0 && c.state[a] == c.state[b]) return false;
+  return true;
+}
+
 // Corrections C in LDS or global memory, indexed by contact id.
 __device__ __forceinline__ void gs_project(const ContactDev& c, float* P,
                                            float* C) {
@@ -1059,14 +1123,15 @@ __global__ void __launch_bounds__(kContactThreads)
 }
 
 // Register-resident variant: contact e (colour order) and its correction
-// live in thread e % kContactThreads for the whole update, so the rounds
+// live in thread e % kContactRegThreads for the whole update, so the rounds
 // touch LDS only and their barriers wait on LDS traffic alone (a barrier
 // also drains outstanding global loads, which made every round pay one).
 // Contacts of one colour are independent, so a thread holding two of the
 // same colour may project them in either order.
+constexpr int kContactRegThreads = kContactThreads;
 constexpr int kContactSlots = 4;
 
-__global__ void __launch_bounds__(kContactThreads)
+__global__ void __launch_bounds__(kContactRegThreads)
     k_contact_update_reg(ContactSet cs, const float* __restrict__ x, int nv) {
   using namespace dev;
   extern __shared__ float sh[];
@@ -1083,12 +1148,15 @@ __global__ void __launch_bounds__(kContactThreads)
   ContactDev c[kContactSlots];
   f3 corr[kContactSlots];
   int color[kContactSlots];
+  unsigned once = 0;  // bit s: contact s has distinct state vertices
+#pragma unroll
   for (int s = 0; s < kContactSlots; ++s) {
-    const int e = s * kContactThreads + tid;
+    const int e = s * kContactRegThreads + tid;
     color[s] = -1;
     corr[s] = f3{0.0f, 0.0f, 0.0f};
     if (e < cs.n) {
       c[s] = cs.c[e];
+      if (distinct_states(c[s])) once |= 1u << s;
       corr[s] = ld3(cs.corr, c[s].id);
       int lo = 0, hi = cs.ncolor;  // color_off[lo] <= e < color_off[hi]
       while (hi - lo > 1) {
@@ -1105,9 +1173,11 @@ __global__ void __launch_bounds__(kContactThreads)
   int col = 0;
   for (int r = 0; r < rounds; ++r) {
     const bool warm = r < cs.ncolor;
+#pragma unroll
     for (int s = 0; s < kContactSlots; ++s) {
       if (color[s] != col) continue;
       if (warm) warm_core(c[s], P, corr[s]);
+      else if (once & (1u << s)) gs_core_once(c[s], P, corr[s]);
       else gs_core(c[s], P, corr[s]);
     }
     __syncthreads();
@@ -1127,6 +1197,7 @@ __global__ void __launch_bounds__(kContactThreads)
       }
     }
   }
+#pragma unroll
   for (int s = 0; s < kContactSlots; ++s)
     if (color[s] >= 0) st3(cs.corr, c[s].id, corr[s]);
 }
@@ -1160,33 +1231,51 @@ void launch_ccd_filter(hipStream_t s, const CollVerts& v, double* err6) {
   k_ccd_filter<<<1, 1024, 0, s>>>(v, err6);
 }
 
+// Workgroups of a CCD launch: as many as can be resident at once.
+template <class In, class Out>
+int ccd_blocks() {
+  static const int blocks = [] {
+    int nb = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, k_ccd<In, Out>, 256, 0) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+            hipSuccess ||
+        nb <= 0 || cus <= 0) {
+      (void)hipGetLastError();
+      return kCcdBlocks;
+    }
+    return nb * cus;
+  }();
+  return blocks;
+}
+
 void launch_ccd(hipStream_t s, const CollVerts& v, const PrimTables& pt,
                 const int2* pt_pairs, const int* pt_count, int pt_cap,
                 const int2* ee_pairs, const int* ee_count, int ee_cap,
-                const CcdParams& cp, int* work, unsigned long long* toi_bits) {
+                const CcdParams& cp, unsigned long long* toi_bits) {
   PairQueries in{v, pt, pt_pairs, pt_count, pt_cap, ee_pairs, ee_count, ee_cap, cp};
-  k_ccd<<<kCcdBlocks, 256, 0, s>>>(in, MinToiOut{toi_bits}, work, cp.tolerance,
-                                   cp.max_iter);
+  k_ccd<<<ccd_blocks<PairQueries, MinToiOut>(), 256, 0, s>>>(
+      in, MinToiOut{toi_bits}, cp.tolerance, cp.max_iter);
 }
 
 void launch_ccd_each(hipStream_t s, int n, const int* is_vf, const double* p0,
                      const double* p1, const double* err, double ms,
-                     double tolerance, int max_iter, int* work, int* hit,
-                     double* toi) {
+                     double tolerance, int max_iter, int* hit, double* toi) {
   if (n <= 0) return;
   ExplicitQueries in{n, is_vf, p0, p1, err, ms};
-  const int blocks = std::min(kCcdBlocks, (n + 3) / 4);
-  k_ccd<<<blocks, 256, 0, s>>>(in, EachOut{hit, toi}, work, tolerance, max_iter);
+  const int blocks = std::min(ccd_blocks<ExplicitQueries, EachOut>(), (n + 3) / 4);
+  k_ccd<<<blocks, 256, 0, s>>>(in, EachOut{hit, toi}, tolerance, max_iter);
 }
 
 void launch_ccd_min(hipStream_t s, int n, const int* is_vf, const double* p0,
                     const double* p1, const double* err, double ms,
-                    double tolerance, int max_iter, int* work,
+                    double tolerance, int max_iter,
                     unsigned long long* toi_bits) {
   if (n <= 0) return;
   ExplicitQueries in{n, is_vf, p0, p1, err, ms};
-  const int blocks = std::min(kCcdBlocks, (n + 3) / 4);
-  k_ccd<<<blocks, 256, 0, s>>>(in, MinToiOut{toi_bits}, work, tolerance, max_iter);
+  const int blocks = std::min(ccd_blocks<ExplicitQueries, MinToiOut>(), (n + 3) / 4);
+  k_ccd<<<blocks, 256, 0, s>>>(in, MinToiOut{toi_bits}, tolerance, max_iter);
 }
 
 bool contact_update_fits_lds(int nv, int n) {
@@ -1197,7 +1286,7 @@ void launch_contact_update(hipStream_t s, const ContactSet& cs,
                            const float* x) {
   if (cs.n == 0 || cs.ncolor == 0) return;
   const size_t p_bytes = 12 * static_cast<size_t>(cs.nv);
-  if (cs.n <= kContactSlots * kContactThreads && p_bytes <= kContactLdsBytes) {
+  if (cs.n <= kContactSlots * kContactRegThreads && p_bytes <= kContactLdsBytes) {
     static bool attr = false;
     if (!attr) {
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_contact_update_reg),
@@ -1205,7 +1294,7 @@ void launch_contact_update(hipStream_t s, const ContactSet& cs,
                                 static_cast<int>(kContactLdsBytes));
       attr = true;
     }
-    k_contact_update_reg<<<1, kContactThreads, p_bytes, s>>>(cs, x, cs.nv);
+    k_contact_update_reg<<<1, kContactRegThreads, p_bytes, s>>>(cs, x, cs.nv);
   } else if (contact_update_fits_lds(cs.nv, cs.n)) {
     const size_t shmem = 12 * (static_cast<size_t>(cs.nv) + cs.n);
     static bool attr = false;

@@ -63,7 +63,7 @@ struct PairOut {
 /// gate_a / gate_b: object boxes (triangle tree roots); when both are given
 /// and do not overlap the pass reports nothing (find_collision.cu:45-92).
 /// Each query is walked by 2^depth lanes, one per subtree at that depth;
-/// the job's work items start at item0 (a multiple of the wavefront size).
+/// the job's work items start at item0 (a multiple of kTraverseWg).
 struct TraverseJob {
   TreeDev tree;
   QuerySet q;
@@ -78,6 +78,8 @@ struct TraverseJob {
 
 /// Subtree split depth of a traversal job (clamped to the tree height).
 constexpr int kTraverseSplitDepth = 3;
+/// Threads per traversal workgroup; job item ranges are whole workgroups.
+constexpr int kTraverseWg = 256;
 
 /// All passes of a broadphase in one launch (nitems: end of the last job's
 /// padded item range).  Pairs go to pt (kind 0) or ee (kind 1).
@@ -131,34 +133,31 @@ void launch_ccd_filter(hipStream_t s, const CollVerts& v, double* err6);
 /// t.lower: an earlier, conservative TOI).  One box per lane of a wavefront.
 constexpr int kCcdStackCap = 64;
 
-/// Workgroups (of four wavefronts) of a CCD launch; wavefronts take queries
-/// from a shared counter until none are left.
+/// Workgroups (of four wavefronts) of a CCD launch when the occupancy query
+/// fails; otherwise the launch holds as many as can be resident.
 constexpr int kCcdBlocks = 1024;
 
 /// Earliest TOI over the candidates (local units, written as the bit pattern
 /// of a positive double into *toi_bits via atomicMin; initialise to 1.0).
-/// *work is the query counter and must be zero at launch.
 void launch_ccd(hipStream_t s, const CollVerts& v, const PrimTables& pt,
                 const int2* pt_pairs, const int* pt_count, int pt_cap,
                 const int2* ee_pairs, const int* ee_count, int ee_cap,
-                const CcdParams& cp, int* work, unsigned long long* toi_bits);
+                const CcdParams& cp, unsigned long long* toi_bits);
 
 /// Per-query CCD on explicit FP64 end positions (12 doubles each for p0 and
 /// p1, err 3 doubles per query) with the kernel of launch_ccd: hit[i] = 1 and
 /// toi[i] as the step's CCD computes them (a root at t = 0 at both
-/// separations is no hit).  *work must be zero at launch.
+/// separations is no hit).
 void launch_ccd_each(hipStream_t s, int n, const int* is_vf, const double* p0,
                      const double* p1, const double* err, double ms,
-                     double tolerance, int max_iter, int* work, int* hit,
-                     double* toi);
+                     double tolerance, int max_iter, int* hit, double* toi);
 
 /// Earliest TOI over explicit queries with the kernel of launch_ccd
 /// (queries that cannot lower the running minimum are cut short); *toi_bits
-/// starts at 1.0 and *work at zero.
+/// starts at 1.0.
 void launch_ccd_min(hipStream_t s, int n, const int* is_vf, const double* p0,
                     const double* p1, const double* err, double ms,
-                    double tolerance, int max_iter, int* work,
-                    unsigned long long* toi_bits);
+                    double tolerance, int max_iter, unsigned long long* toi_bits);
 
 // ---------------------------------------------------------------- contacts
 

@@ -415,15 +415,23 @@ __global__ void __launch_bounds__(kPcgThreads)
   };
 
   // The coarse banks of this workgroup: each member's input is the sum of r
-  // over its level-1 nodes (eight lanes per member); the product with the
-  // bank inverse is split over the eight half-waves (four inputs each) and
-  // joined by the first one, which publishes the members' corrections and
-  // the bank's r . z_coarse (the correction is constant over a node).
+  // over its level-1 nodes (in a level-1 bank, recognised by its first
+  // member, which is always present, the level-1 sum itself; above, eight
+  // lanes per member).  The product with the bank inverse is split over the
+  // eight half-waves (four inputs each) and joined by the first one, which
+  // publishes the members' corrections and the bank's r . z_coarse (the
+  // correction is constant over a node).
   auto solve_banks = [&](unsigned e) {
     int ob = 0;
     for (int cb = j; cb < S.ncb; cb += S.nwg, ++ob) {
       const int* mem = cbm + kBankSlots * ob;
-      {
+      if (mem[0] < S.n1) {
+        // A level-1 bank: its inputs are level-1 sums themselves.
+        if (t < kBankSlots) {
+          const int m = mem[t];
+          sts4(cin, t, m >= 0 ? lds4(csum, m) : f3{0.0f, 0.0f, 0.0f});
+        }
+      } else {
         const int k = t / 8, sub = t & 7;
         const int m = mem[k];
         f3 v = {0.0f, 0.0f, 0.0f};
