@@ -21,8 +21,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
-#include <map>
-#include <set>
+#include <unordered_set>
 
 #include "api/context.hpp"
 #include "hip/collision_kernels.hpp"
@@ -93,6 +92,38 @@ struct HostContact {
 
 // Order-independent identity of a DCD record: type and global vertex ids,
 // triangle corners sorted, each edge (min, max) and the smaller edge first.
+struct KeyHash {
+  size_t operator()(const std::array<int, 5>& k) const {
+    uint64_t h = 1469598103934665603ull;
+    for (int v : k) h = (h ^ static_cast<uint32_t>(v)) * 1099511628211ull;
+    return static_cast<size_t>(h);
+  }
+};
+
+// Pinned host staging buffer that grows; the caller makes sure no copy from
+// it is in flight when it grows or is rewritten.
+template <class T>
+class PinnedVec {
+ public:
+  ~PinnedVec() {
+    if (p_ != nullptr) (void)hipHostFree(p_);
+  }
+  T* reserve(size_t n) {
+    if (n > cap_) {
+      if (p_ != nullptr) (void)hipHostFree(p_);
+      cap_ = std::max<size_t>(n, 2 * cap_);
+      hip_check(hipHostMalloc(reinterpret_cast<void**>(&p_), cap_ * sizeof(T)),
+                "hipHostMalloc");
+    }
+    return p_;
+  }
+  T* get() { return p_; }
+
+ private:
+  T* p_ = nullptr;
+  size_t cap_ = 0;
+};
+
 std::array<int, 5> contact_key(const DcdRec& r) {
   std::array<int, 5> k{r.type, r.gv[0], r.gv[1], r.gv[2], r.gv[3]};
   if (r.type == 0) {
@@ -166,7 +197,14 @@ class HipCollision final : public CollisionWorld {
 
   // contacts of the step
   std::vector<HostContact> contacts_;
-  std::set<std::array<int, 5>> keys_;
+  std::unordered_set<std::array<int, 5>, KeyHash> keys_;
+  std::vector<int> color_;               // per contact, from 1
+  std::vector<std::vector<int>> vinc_;   // per cloth vertex: its contacts
+  std::vector<int> touched_;             // vertices with contacts
+  int max_color_ = 0;
+  PinnedVec<ContactDev> hc_pin_;
+  PinnedVec<int> off_pin_;
+  PinnedVec<float> w_pin_;
   std::vector<int> color_off_h_, color_list_h_;
   int n_active_ = 0;
   DevBuf<ContactDev> c_dev_;
@@ -183,7 +221,7 @@ class HipCollision final : public CollisionWorld {
   void launch_broadphase();
   std::vector<DcdRec> dcd(float time, float dhat, silk_hip_step_stats* stats);
   void append(std::vector<DcdRec> recs);
-  void recolor();
+  void recolor(int n_old);
   void upload_contacts(int n_old);
   void read_io();
 };
@@ -520,57 +558,63 @@ void HipCollision::append(std::vector<DcdRec> recs) {
     contacts_.push_back(c);
   }
   if (static_cast<int>(contacts_.size()) == n_old) return;
-  recolor();
+  recolor(n_old);
   upload_contacts(n_old);
 }
 
 // Greedy first-fit colouring in contact order over shared movable vertices
-// (contact_coloring.cu:241-392 with a deterministic schedule).
-void HipCollision::recolor() {
+// (contact_coloring.cu:241-392 with a deterministic schedule).  A contact's
+// colour depends only on the contacts before it, so appended contacts are
+// coloured alone and earlier colours stay.
+void HipCollision::recolor(int n_old) {
   const int n = static_cast<int>(contacts_.size());
-  std::map<int, std::vector<int>> incid;
-  for (int c = 0; c < n; ++c)
-    for (int k = 0; k < 4; ++k)
-      if (contacts_[c].gamma[k] != 0.0) incid[contacts_[c].state[k]].push_back(c);
-  std::vector<int> color(n, 0);
-  int maxc = 0;
+  if (vinc_.size() < static_cast<size_t>(nvc_)) vinc_.resize(nvc_);
+  float* hw = w_pin_.reserve(std::max(1, nvc_));
+  if (n_old == 0 && touched_.empty())
+    std::fill(hw, hw + std::max(1, nvc_), 0.0f);
+  color_.resize(n, 0);
   std::vector<char> used;
-  for (int c = 0; c < n; ++c) {
-    used.assign(maxc + 2, 0);
+  for (int c = n_old; c < n; ++c) {
+    used.assign(max_color_ + 2, 0);
+    const HostContact& hc = contacts_[c];
     for (int k = 0; k < 4; ++k) {
-      if (contacts_[c].gamma[k] == 0.0) continue;
-      for (int o : incid[contacts_[c].state[k]])
-        if (o < c) used[color[o]] = 1;
+      if (hc.gamma[k] == 0.0) continue;
+      for (int o : vinc_[hc.state[k]]) used[color_[o]] = 1;
     }
     int col = 1;
     while (used[col]) ++col;
-    color[c] = col;
-    maxc = std::max(maxc, col);
+    color_[c] = col;
+    max_color_ = std::max(max_color_, col);
+    for (int k = 0; k < 4; ++k) {
+      if (hc.gamma[k] == 0.0) continue;
+      auto& lst = vinc_[hc.state[k]];
+      if (lst.empty()) {
+        touched_.push_back(hc.state[k]);
+        hw[hc.state[k]] = static_cast<float>(S_.h_contact_w[hc.state[k]]);
+      }
+      if (lst.empty() || lst.back() != c) lst.push_back(c);
+    }
   }
+  const int maxc = max_color_;
   color_off_h_.assign(maxc + 1, 0);
-  for (int c = 0; c < n; ++c) color_off_h_[color[c]] += 1;
+  for (int c = 0; c < n; ++c) color_off_h_[color_[c]] += 1;
   for (int k = 1; k <= maxc; ++k) color_off_h_[k] += color_off_h_[k - 1];
   color_list_h_.assign(n, 0);
   std::vector<int> fill(color_off_h_.begin(), color_off_h_.end() - 1);
-  for (int c = 0; c < n; ++c) color_list_h_[fill[color[c] - 1]++] = c;
-  std::vector<float> hw(std::max(1, nvc_), 0.0f);
-  n_active_ = 0;
-  for (const auto& [v, list] : incid) {
-    (void)list;
-    ++n_active_;
-    hw[v] = static_cast<float>(S_.h_contact_w[v]);
-  }
-  hip_check(hipMemcpyAsync(w_.get(), hw.data(), hw.size() * sizeof(float),
+  for (int c = 0; c < n; ++c) color_list_h_[fill[color_[c] - 1]++] = c;
+  n_active_ = static_cast<int>(touched_.size());
+  // Staging buffers are not in flight here: the records just read back
+  // drained the stream.
+  hip_check(hipMemcpyAsync(w_.get(), hw, std::max(1, nvc_) * sizeof(float),
                            hipMemcpyHostToDevice, st_),
             "contact weights");
-  hip_check(hipStreamSynchronize(st_), "contact weights sync");
 }
 
 void HipCollision::upload_contacts(int n_old) {
   const int n = static_cast<int>(contacts_.size());
   // Device copy in colour order (ids ascending inside a colour), so that a
   // colour is one contiguous range.
-  std::vector<ContactDev> hc(n);
+  ContactDev* hc = hc_pin_.reserve(std::max(1, n));
   for (int e = 0; e < n; ++e) {
     const int i = color_list_h_[e];
     const HostContact& c = contacts_[i];
@@ -587,8 +631,16 @@ void HipCollision::upload_contacts(int n_old) {
     d.mu = static_cast<float>(c.mu);
     d.id = i;
   }
-  c_dev_.upload(hc, st_);
-  color_off_.upload(color_off_h_, st_);
+  c_dev_.reserve(std::max(1, n));
+  hip_check(hipMemcpyAsync(c_dev_.get(), hc, n * sizeof(ContactDev),
+                           hipMemcpyHostToDevice, st_),
+            "contacts");
+  int* off = off_pin_.reserve(color_off_h_.size());
+  std::copy(color_off_h_.begin(), color_off_h_.end(), off);
+  color_off_.reserve(color_off_h_.size());
+  hip_check(hipMemcpyAsync(color_off_.get(), off, color_off_h_.size() * sizeof(int),
+                           hipMemcpyHostToDevice, st_),
+            "colour offsets");
   // Corrections persist across appends; new contacts start at zero.
   if (n > corr_cap_) {
     const int cap = std::max(1024, 2 * n);
@@ -603,7 +655,6 @@ void HipCollision::upload_contacts(int n_old) {
   hip_check(hipMemsetAsync(corr_.get() + 3 * static_cast<size_t>(n_old), 0,
                            3 * sizeof(float) * (n - n_old), st_),
             "corr tail");
-  hip_check(hipStreamSynchronize(st_), "contacts upload");
 }
 
 void HipCollision::begin_step(const float* outer_x, float* ms, float* dhat,
@@ -638,6 +689,13 @@ void HipCollision::begin_step(const float* outer_x, float* ms, float* dhat,
   keys_.clear();
   color_off_h_.clear();
   color_list_h_.clear();
+  color_.clear();
+  for (int v : touched_) {
+    vinc_[v].clear();
+    w_pin_.get()[v] = 0.0f;
+  }
+  touched_.clear();
+  max_color_ = 0;
   n_active_ = 0;
   const size_t nc3 = 3 * static_cast<size_t>(nvc_);
   hip_check(hipMemsetAsync(w_.get(), 0, nvc_ * sizeof(float), st_), "w reset");
