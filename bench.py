@@ -36,7 +36,7 @@ from silk_physics_amd import scenes  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 METRIC = "sim steps/sec, 80x80 sheet + collision; PCG iters to 1e-6 rel residual"
 CLASSES = {1: "k_pcg_persistent", 2: "k_local_step", 3: "k_contact_update",
-           4: "k_traverse", 5: "k_ccd", 6: "k_residual"}
+           4: "k_broadphase", 5: "k_ccd", 6: "k_residual"}
 
 
 def global_kw(true_residual: bool = False) -> dict:
