@@ -77,7 +77,7 @@ struct TraverseJob {
 };
 
 /// Subtree split depth of a traversal job (clamped to the tree height).
-constexpr int kTraverseSplitDepth = 3;
+constexpr int kTraverseSplitDepth = 5;
 /// Threads per traversal workgroup; job item ranges are whole workgroups.
 constexpr int kTraverseWg = 256;
 
