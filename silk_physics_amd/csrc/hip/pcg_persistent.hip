@@ -249,6 +249,7 @@ __global__ void __launch_bounds__(kPcgThreads)
   int* cnp = cbm + kBankSlots * a.cbw_max;                     // ncn_max + 1
   int* cnl = cnp + a.ncn_max + 1;                              // cnl_max
   int* nl1 = cnl + a.cnl_max;                                  // need1_max
+  int* cbn = nl1 + a.need1_max;                                // cbw_max
   const int n1_0 = a.wg_n1_ptr[blockIdx.x];
   const int nn1 = a.wg_n1_ptr[blockIdx.x + 1] - n1_0;
   const int c_0 = a.wg_c_ptr[blockIdx.x];
@@ -273,6 +274,14 @@ __global__ void __launch_bounds__(kPcgThreads)
        e += kPcgThreads)
     cnl[e] = a.cn_list[cn_first + e];
   for (int e = t; e < nn1; e += kPcgThreads) nl1[e] = a.wg_n1[n1_0 + e];
+  // Members present in each of those banks (they fill the first slots).
+  for (int ob = t; ob < a.cbw_max; ob += kPcgThreads) {
+    const int cb = j + S.nwg * ob;
+    int nm = 0;
+    if (cb < S.ncb)
+      while (nm < kBankSlots && a.cb_member[kBankSlots * (S.cb0 + cb) + nm] >= 0) ++nm;
+    cbn[ob] = nm;
+  }
 
   // This slot's row of its level-0 bank inverse.
   float winv[kBankSlots];
@@ -416,8 +425,8 @@ __global__ void __launch_bounds__(kPcgThreads)
 
   // The coarse banks of this workgroup: each member's input is the sum of r
   // over its level-1 nodes (in a level-1 bank, recognised by its first
-  // member, which is always present, the level-1 sum itself; above, eight
-  // lanes per member).  The product with the bank inverse is split over the
+  // member, which is always present, the level-1 sum itself; above, a group
+  // of lanes per member).  The product with the bank inverse is split over the
   // eight half-waves (four inputs each) and joined by the first one, which
   // publishes the members' corrections and the bank's r . z_coarse (the
   // correction is constant over a node).
@@ -432,15 +441,19 @@ __global__ void __launch_bounds__(kPcgThreads)
           sts4(cin, t, m >= 0 ? lds4(csum, m) : f3{0.0f, 0.0f, 0.0f});
         }
       } else {
-        const int k = t / 8, sub = t & 7;
-        const int m = mem[k];
+        // Members are the first nm slots; each gets g lanes (8 to 64, as
+        // many as the bank's members leave), the level-1 nodes strided.
+        const int nm = cbn[ob];
+        const int g = nm <= 4 ? 64 : (nm <= 8 ? 32 : (nm <= 16 ? 16 : 8));
+        const int k = t / g, sub = t % g;
+        const int m = k < nm ? mem[k] : -1;
         f3 v = {0.0f, 0.0f, 0.0f};
         if (m >= 0) {
           const int e_end = cnp[m + 1];
-          for (int ee = cnp[m] + sub; ee < e_end; ee += 4 * 8) {
+          for (int ee = cnp[m] + sub; ee < e_end; ee += 4 * g) {
             int id[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) id[u] = ee + 8 * u < e_end ? cnl[ee + 8 * u] : -1;
+            for (int u = 0; u < 4; ++u) id[u] = ee + g * u < e_end ? cnl[ee + g * u] : -1;
             f3 c[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
@@ -450,8 +463,17 @@ __global__ void __launch_bounds__(kPcgThreads)
               if (id[u] >= 0) v = v + c[u];
           }
         }
+        // Sum over the lane group (the same g in the whole workgroup).
         v = {sum8(v.x), sum8(v.y), sum8(v.z)};
-        if (sub == 0) sts4(cin, k, v);
+        if (g >= 16) {
+          v.x += dpp_f<kDppMirror>(v.x);
+          v.y += dpp_f<kDppMirror>(v.y);
+          v.z += dpp_f<kDppMirror>(v.z);
+        }
+        if (g >= 32) v = {v.x + xor16_f(v.x), v.y + xor16_f(v.y), v.z + xor16_f(v.z)};
+        if (g >= 64) v = {v.x + xor32_f(v.x), v.y + xor32_f(v.y), v.z + xor32_f(v.z)};
+        if (sub == 0 && k < nm) sts4(cin, k, v);
+        if (t >= nm && t < kBankSlots) sts4(cin, t, f3{0.0f, 0.0f, 0.0f});
       }
       __syncthreads();
       {
@@ -649,7 +671,8 @@ size_t pcg_persistent_lds(const PersistDev& a) {
          sizeof(int) * (static_cast<size_t>(kBankSlots) * a.cbw_max +
                         static_cast<size_t>(a.ncn_max) + 1 +
                         static_cast<size_t>(a.cnl_max) +
-                        static_cast<size_t>(a.need1_max));
+                        static_cast<size_t>(a.need1_max) +
+                        static_cast<size_t>(a.cbw_max));
 }
 
 bool pcg_persistent_fits(int grid, size_t shmem) {
