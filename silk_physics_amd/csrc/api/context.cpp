@@ -455,8 +455,12 @@ void Context::build_scene() {
       }
       const int per_wg = kPcgThreads / kBankSlots;
       sd.nwg = std::max(1, (nb0 + per_wg - 1) / per_wg);
+      // Coarse banks are solved by workgroups of their own (one per bank, up
+      // to a third of the slot workgroups), in parallel with the slot
+      // workgroups' level-0 work.
+      sd.ncw = std::min(sd.ncb, std::max(1, (sd.nwg + 2) / 3));
       sd.wg0 = ps_wg;
-      ps_wg += sd.nwg;
+      ps_wg += sd.nwg + sd.ncw;
       ps_sys.push_back(sd);
     }
   }
@@ -521,7 +525,7 @@ void Context::build_scene() {
     const int nslot = static_cast<int>(ps_slot_row.size());
     std::vector<int> slot_wg(nslot, -1), slot_local(nslot, 0);
     for (const auto& sd : ps_sys) {
-      nwg_max = std::max(nwg_max, sd.nwg);
+      nwg_max = std::max(nwg_max, sd.nwg + sd.ncw);
       const int nb0 = sd.nslot / kBankSlots;
       for (int j = 0; j < sd.nwg; ++j) {
         const int lo = sd.slot0 + kBankSlots * ((j * nb0) / sd.nwg);
@@ -564,15 +568,17 @@ void Context::build_scene() {
             std::lower_bound(hl.begin(), hl.end(), c) - hl.begin());
       }
     });
-    // Coarse bank cb of a system is solved by its workgroup cb % nwg.  Per
-    // workgroup: the level-1 nodes whose sums its banks read, and the coarse
-    // nodes whose corrections its own and halo slots add.
+    // Coarse bank cb of a system is solved by its coarse workgroup cb % ncw.
+    // Per workgroup: the level-1 nodes whose sums its banks read (coarse
+    // workgroups), and the coarse nodes whose corrections its own and halo
+    // slots add (slot workgroups).
     for (const auto& sd : ps_sys) {
-      cbw_max = std::max(cbw_max, (sd.ncb + sd.nwg - 1) / sd.nwg);
-      for (int j = 0; j < sd.nwg; ++j) {
+      if (sd.ncw > 0) cbw_max = std::max(cbw_max, (sd.ncb + sd.ncw - 1) / sd.ncw);
+      for (int j = 0; j < sd.nwg + sd.ncw; ++j) {
         const int w = sd.wg0 + j;
+        const bool cwg = j >= sd.nwg;
         std::vector<int> n1s, cs;
-        for (int cb = j; cb < sd.ncb; cb += sd.nwg)
+        for (int cb = j - sd.nwg; cwg && cb < sd.ncb; cb += sd.ncw)
           for (int k = 0; k < kBankSlots; ++k) {
             const int m = ps_cb_member[kBankSlots * (sd.cb0 + cb) + k];
             if (m < 0) continue;
@@ -599,7 +605,7 @@ void Context::build_scene() {
         // Tagged words of the two exchanges of an iteration.
         const int cap = kPcgThreads * kPcgXchWords;
         const int nh = ps_halo_ptr[w + 1] - ps_halo_ptr[w];
-        if (3 * static_cast<int>(n1s.size()) + 2 * kPcgWaves * sd.nwg > cap ||
+        if (3 * static_cast<int>(n1s.size()) + 2 * kPcgWaves * sd.nwg + sd.ncw > cap ||
             4 * kPcgWaves * sd.nwg + 2 * sd.ncb + 3 * nh +
                     3 * static_cast<int>(cs.size()) > cap ||
             nh > kPcgThreads * kPcgHaloPerThread)

@@ -51,6 +51,10 @@ constexpr int kDppHalfMirror = 0x141;
 /// Value of lane 15 - l within each row of 16 (DPP row_mirror).
 constexpr int kDppMirror = 0x140;
 
+/// Value of lane 16 * (l / 16) + n, for n = kDppRowBcast - 0x150 (DPP
+/// row_newbcast:n; n in [0, 16)).
+constexpr int kDppRowBcast = 0x150;
+
 template <int kCtrl>
 __device__ __forceinline__ float dpp_f(float v) {
   return __int_as_float(

@@ -247,7 +247,9 @@ struct PcgSysDev {
   int cn0 = 0, ncn = 0;      // coarse nodes (system-local ids cn0 + local)
   int n1 = 0;                // level-1 nodes (local ids [0, n1))
   int lvl[5] = {0, 0, 0, 0, 0};  // level l nodes: local ids [lvl[l-1], lvl[l])
-  int wg0 = 0, nwg = 0;      // workgroups of the group
+  int wg0 = 0, nwg = 0;      // slot workgroups of the group
+  int ncw = 0;               // coarse workgroups, after the slot ones: coarse
+                             // bank cb is solved by coarse workgroup cb % ncw
 };
 
 struct PersistDev {

@@ -5,42 +5,49 @@
 // rr <= rel^2 rr0 or max_iter) and the same additive MAS preconditioner
 // (mas_preconditioner.cu:601-728), without a launch per phase.
 //
-// Work split: a system (cloth) owns a group of workgroups; a workgroup owns
-// up to eight consecutive 32-slot level-0 banks, one thread per slot.  A
+// Work split: a system (cloth) owns a group of workgroups.  A slot workgroup
+// owns up to eight consecutive 32-slot level-0 banks, one thread per slot; a
 // thread keeps its row's r, dx, z, p, q, neighbour list and row of its bank
-// inverse in registers for the whole solve.  The workgroup keeps in LDS the
-// inverses of the coarse banks it solves, the level-1 sums they read, and p
+// inverse in registers for the whole solve, and the workgroup keeps in LDS p
 // of its own slots and of its halo (the other workgroups' slots its rows
-// reference).
+// reference).  A coarse workgroup owns no slots: it keeps in LDS the
+// inverses of the coarse banks it solves and the level-1 sums they read, and
+// solves them while the slot workgroups compute their level-0 part.
 //
 // Per iteration k:
 //   p_k = z_k + beta_k p_{k-1} for the own slots and, with the same
 //                     operations, for the halo slots (z0 from the last
 //                     exchange plus the coarse corrections of their nodes)
 //   q = A p_k         difference form, neighbour p from LDS
-//   exchange 1        level-1 node sums of q and the p.q partials
-//   alpha; dx += alpha p; r -= alpha q; the level-1 sums of r follow by
-//                     cr -= alpha * qsum; z0 = level-0 part of M r
-//   exchange 2        z0 of the own slots, r.z0 and r.r partials, and the
-//                     coarse levels: coarse bank cb is solved by workgroup
-//                     cb % nwg from the level-1 sums it keeps, which
-//                     publishes its members' corrections and the bank's
-//                     r.z_coarse (the correction is constant over a node);
-//                     then z = z0 + coarse corrections, beta and the stopping
-//                     test (every workgroup sums the same partials in the same
-//                     order, so all take the same decision)
+//   exchange 1        level-1 node sums of q and the p.q partials (and a
+//                     word of each coarse workgroup)
+//   alpha; slot workgroups: dx += alpha p; r -= alpha q; z0 = level-0 part
+//                     of M r.  Coarse workgroups: the level-1 sums of r
+//                     follow by cr -= alpha * qsum; coarse bank cb is solved
+//                     by coarse workgroup cb % ncw, which publishes its
+//                     members' corrections and the bank's r.z_coarse (the
+//                     correction is constant over a node)
+//   exchange 2        z0 of the own slots, r.z0 and r.r partials and the
+//                     coarse words; then z = z0 + coarse corrections, beta
+//                     and the stopping test (every workgroup sums the same
+//                     partials in the same order, so all take the same
+//                     decision)
 //
 // An exchange is a set of 64-bit words (epoch << 32 | payload bits) written
 // with device-scope relaxed atomic stores; a reader polls the words it needs
 // until each carries the exchange's epoch.  A 64-bit atomic is read whole, so
 // a word with the right epoch holds the right payload: the words are their
 // own barrier, with no fence and no flag.  A word is rewritten only after an
-// exchange that each of its readers enters after reading it (node sums and
-// p.q after the next exchange 2; exchange-2 words after the next exchange 1).
+// exchange that each of its readers enters after reading it: exchange-1
+// words after the next exchange 2, whose coarse words each coarse workgroup
+// publishes after reading exchange 1; exchange-2 words after the next
+// exchange 1, whose coarse-workgroup words are published after reading
+// exchange 2.
 // Epochs continue across launches from a per-system counter.  Waits are
 // bounded: a wait that gives up raises an abort flag, every workgroup leaves,
 // and the host reports an error.
 #include <cstdlib>
+#include <utility>
 
 #include "hip/device_common.hpp"
 #include "hip/kernels.hpp"
@@ -105,8 +112,11 @@ __device__ bool collect(const Share& sh, unsigned epoch, float* lds,
         pending &= ~(1u << m);
       }
     if (pending == 0) break;
-    if (++spins > kSpinLimit ||
-        __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+    // The flag is read every 256 polls: a read per poll would double the
+    // time between two looks at the words.
+    if ((++spins & 255) == 0 &&
+        (spins > kSpinLimit ||
+         __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
       __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
     }
@@ -116,22 +126,36 @@ __device__ bool collect(const Share& sh, unsigned epoch, float* lds,
 }
 
 // Partial e (workgroup e / kPcgWaves, wavefront e % kPcgWaves) of the
-// block at word offset h of each workgroup's kPcgPartWords.
+// block at word offset h of each workgroup's kPcgPartWords (a double whose
+// low word comes first; offsets are even and the array 16-byte aligned).
 __device__ __forceinline__ double part_at(const float* pw, int e, int h) {
   const int o = kPcgPartWords * (e / kPcgWaves) + h + 2 * (e % kPcgWaves);
-  const unsigned long long lo = __float_as_uint(pw[o]);
-  const unsigned long long hi = __float_as_uint(pw[o + 1]);
-  return __longlong_as_double(static_cast<long long>((hi << 32) | lo));
+  return *reinterpret_cast<const double*>(pw + o);
 }
 
 // Sum over the wavefront; the same bits in every lane and in every
 // wavefront summing the same values (sum64).
 __device__ __forceinline__ double wave_sum(double v) { return sum64(v); }
 
-// Totals of the per-wavefront partials of blocks h0 and h1 over the
-// workgroups: lane l adds partials l, l + 64, ... in order, then a
-// butterfly; the same bits in every thread of every workgroup.
-__device__ __forceinline__ void part_totals(const float* pw, int nwg, int h0,
+// Total of the per-wavefront partials of block h over the workgroups plus
+// the n_extra doubles at `extra`: lane l adds partials l, l + 64, ... and
+// then extras l, l + 64, ... in order, then a butterfly; the same bits in
+// every thread of every workgroup.
+__device__ __forceinline__ double part_total(const float* pw, int nwg, int h,
+                                             const float* extra = nullptr,
+                                             int n_extra = 0) {
+  const int lane = static_cast<int>(threadIdx.x) % kWave;
+  double a = 0.0;
+  for (int e = lane; e < kPcgWaves * nwg; e += kWave) a += part_at(pw, e, h);
+  for (int e = lane; e < n_extra; e += kWave)
+    a += *reinterpret_cast<const double*>(extra + 2 * e);
+  return wave_sum(a);
+}
+
+// part_total of blocks h0 (with the extras) and h1 at once: the same bits
+// as two calls, with the two sums interleaved.
+__device__ __forceinline__ void part_total2(const float* pw, int nwg, int h0,
+                                            const float* extra, int n_extra,
                                             int h1, double& t0, double& t1) {
   const int lane = static_cast<int>(threadIdx.x) % kWave;
   double a = 0.0, b = 0.0;
@@ -139,6 +163,8 @@ __device__ __forceinline__ void part_totals(const float* pw, int nwg, int h0,
     a += part_at(pw, e, h0);
     b += part_at(pw, e, h1);
   }
+  for (int e = lane; e < n_extra; e += kWave)
+    a += *reinterpret_cast<const double*>(extra + 2 * e);
   t0 = wave_sum(a);
   t1 = wave_sum(b);
 }
@@ -200,13 +226,12 @@ __global__ void __launch_bounds__(kPcgThreads)
   // node or slot), then the coarse bank members, the descendant-list
   // offsets and the lists (ints).
   extern __shared__ __align__(16) float dyn[];
-  __shared__ __align__(16) float rb[kPcgThreads][4];
   __shared__ double red[17];
 
   int sys = -1;
   for (int s = 0; s < a.nsys; ++s)
     if (static_cast<int>(blockIdx.x) >= a.sys[s].wg0 &&
-        static_cast<int>(blockIdx.x) < a.sys[s].wg0 + a.sys[s].nwg)
+        static_cast<int>(blockIdx.x) < a.sys[s].wg0 + a.sys[s].nwg + a.sys[s].ncw)
       sys = s;
   if (sys < 0) return;  // spare workgroup: takes no part in any exchange
   // Stopped ADMM loop: nothing to solve.  ctl->done is written only by
@@ -220,12 +245,16 @@ __global__ void __launch_bounds__(kPcgThreads)
   unsigned ep = __hip_atomic_load(epoch_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int t = static_cast<int>(threadIdx.x);
   const int j = static_cast<int>(blockIdx.x) - S.wg0;
+  // A slot workgroup (j < nwg) owns level-0 banks; a coarse workgroup
+  // (c = j - nwg) owns no slots and solves coarse banks c, c + ncw, ...
+  const bool cwg = j >= S.nwg;
+  const int c = j - S.nwg;
   const int nb0 = S.nslot / kBankSlots;
-  const int b_lo = (j * nb0) / S.nwg, b_hi = ((j + 1) * nb0) / S.nwg;
+  const int b_lo = cwg ? 0 : (j * nb0) / S.nwg;
+  const int b_hi = cwg ? 0 : ((j + 1) * nb0) / S.nwg;
   const int lb = b_lo + t / kBankSlots;
   const int hw = t / kBankSlots;
   const int s_in_bank = t & (kBankSlots - 1);
-  const int base = t & ~(kBankSlots - 1);
   const bool in_bank = lb < b_hi;
   const int slot = S.slot0 + kBankSlots * lb + s_in_bank;
   const int row = in_bank ? a.slot_row[slot] : -1;
@@ -256,16 +285,16 @@ __global__ void __launch_bounds__(kPcgThreads)
   const int nc = a.wg_c_ptr[blockIdx.x + 1] - c_0;
 
   // Static data into LDS: the inverses and members of the coarse banks this
-  // workgroup solves (cb = j, j + nwg, ...), every coarse node's level-1
-  // list, and the level-1 nodes those banks read.
-  for (int e = t; e < a.cbw_max * 1024; e += kPcgThreads) {
-    const int cb = j + S.nwg * (e / 1024);
+  // workgroup solves (cb = c, c + ncw, ...; none for a slot workgroup),
+  // every coarse node's level-1 list, and the level-1 nodes those banks read.
+  for (int e = t; cwg && e < a.cbw_max * 1024; e += kPcgThreads) {
+    const int cb = c + S.ncw * (e / 1024);
     if (cb < S.ncb)
       invc[e] = a.bank_inv[static_cast<size_t>(a.cb_bank[S.cb0 + cb]) * 1024 + (e & 1023)];
   }
   for (int e = t; e < a.cbw_max * kBankSlots; e += kPcgThreads) {
-    const int cb = j + S.nwg * (e / kBankSlots);
-    cbm[e] = cb < S.ncb ? a.cb_member[kBankSlots * (S.cb0 + cb) + e % kBankSlots] : -1;
+    const int cb = c + S.ncw * (e / kBankSlots);
+    cbm[e] = cwg && cb < S.ncb ? a.cb_member[kBankSlots * (S.cb0 + cb) + e % kBankSlots] : -1;
   }
   const int cn_first = S.ncn > 0 ? a.cn_ptr[S.cn0] : 0;
   for (int e = t; e <= S.ncn; e += kPcgThreads)
@@ -276,19 +305,21 @@ __global__ void __launch_bounds__(kPcgThreads)
   for (int e = t; e < nn1; e += kPcgThreads) nl1[e] = a.wg_n1[n1_0 + e];
   // Members present in each of those banks (they fill the first slots).
   for (int ob = t; ob < a.cbw_max; ob += kPcgThreads) {
-    const int cb = j + S.nwg * ob;
+    const int cb = c + S.ncw * ob;
     int nm = 0;
-    if (cb < S.ncb)
+    if (cwg && cb < S.ncb)
       while (nm < kBankSlots && a.cb_member[kBankSlots * (S.cb0 + cb) + nm] >= 0) ++nm;
     cbn[ob] = nm;
   }
 
-  // This slot's row of its level-0 bank inverse.
+  // This slot's row of its level-0 bank inverse, starting with the entries of
+  // the slots in its own row of 16 lanes: winv[k] is entry k ^ (16 * row).
   float winv[kBankSlots];
+  const int row16 = s_in_bank & 16;
 #pragma unroll
   for (int k = 0; k < kBankSlots; ++k)
     winv[k] = in_bank ? a.bank_inv[static_cast<size_t>(S.bank0 + lb) * 1024 +
-                                   k * kBankSlots + s_in_bank]
+                                   (k ^ row16) * kBankSlots + s_in_bank]
                       : 0.0f;
   int e_off = 0, e_w = 0, e_lane = 0;
   if (own) {
@@ -335,14 +366,20 @@ __global__ void __launch_bounds__(kPcgThreads)
   const int n1_count = n1_last >= 0 ? n1_last - n1_first + 1 : 0;
 
   // Exchange 1: the level-1 sums the coarse banks of this workgroup read
-  // (3 words per node), then the p.q partials.  Exchange 2: the r.z0 and r.r
-  // partials, each coarse bank's r.z_coarse (2 words), z0 of the halo and the
-  // corrections of the coarse nodes the slots add (3 words each).
+  // (3 words per node), the slot workgroups' p.q partials and one word of
+  // each coarse workgroup (published once it has read the last exchange 2,
+  // so that no word is rewritten before all its readers are done).
+  // Exchange 2: the slot workgroups' r.z0 and r.r partials, each coarse
+  // bank's r.z_coarse (2 words), and for a slot workgroup z0 of its halo and
+  // the corrections of the coarse nodes its slots add (3 words each).
   constexpr int kWaveWords = 2 * kPcgWaves;  // one double per wavefront
+  // A coarse workgroup's word: its wavefront 0's (zero) p.q partial.
+  constexpr int kAckWord = 0;
   Share x1, x2;
   {
     const int nw1 = 3 * nn1;
-    const int w1 = nw1 + kWaveWords * S.nwg;
+    const int aw1 = nw1 + kWaveWords * S.nwg;
+    const int w1 = aw1 + S.ncw;
     const int pw2 = 2 * kWaveWords * S.nwg;
     const int cw2 = pw2 + 2 * S.ncb;
     const int hw2 = cw2 + 3 * nh;
@@ -360,10 +397,14 @@ __global__ void __launch_bounds__(kPcgThreads)
           const int n = a.wg_n1[n1_0 + i / 3];
           x1.addr[m] = a.tn + 3 * static_cast<size_t>(S.cn0 + n) + i % 3;
           x1.dest[m] = static_cast<int>(qs - dyn) + 4 * n + i % 3;
-        } else {
+        } else if (i < aw1) {
           const int w = (i - nw1) / kWaveWords, h = (i - nw1) % kWaveWords;
           x1.addr[m] = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + h;
           x1.dest[m] = static_cast<int>(pw - dyn) + kPcgPartWords * w + h;
+        } else {
+          const int w = S.nwg + (i - aw1);
+          x1.addr[m] = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + kAckWord;
+          x1.dest[m] = static_cast<int>(pw - dyn) + kPcgPartWords * w + kAckWord;
         }
       }
       x2.addr[m] = a.tp;
@@ -405,21 +446,22 @@ __global__ void __launch_bounds__(kPcgThreads)
     }
   };
 
-  // Level-0 part of z = M r: this slot's bank row times r of its bank.
+  // Level-0 part of z = M r: this slot's bank row times r of its bank (the
+  // 32 lanes of its half-wave), r of each slot broadcast from its lane: step
+  // m adds slot m of the own row of 16 lanes and slot m of the other row.
+  // Called by every thread (all lanes active); lanes outside a bank give 0.
   auto mas_l0 = [&](f3 rv) -> f3 {
-    sts4(&rb[0][0], t, rv);
-    __syncthreads();
+    const f3 rs = {xor16_f(rv.x), xor16_f(rv.y), xor16_f(rv.z)};
     f3 z0 = {0.0f, 0.0f, 0.0f};
-    if (in_bank) {
-#pragma unroll
-      for (int k = 0; k < kBankSlots; ++k) {
-        const float w = winv[k];
-        const f3 rk = lds4(&rb[0][0], base + k);
-        z0.x += w * rk.x;
-        z0.y += w * rk.y;
-        z0.z += w * rk.z;
-      }
-    }
+    [&]<int... M>(std::integer_sequence<int, M...>) {
+      ((z0 = {fmaf(winv[M], dpp_f<kDppRowBcast + M>(rv.x), z0.x),
+              fmaf(winv[M], dpp_f<kDppRowBcast + M>(rv.y), z0.y),
+              fmaf(winv[M], dpp_f<kDppRowBcast + M>(rv.z), z0.z)},
+        z0 = {fmaf(winv[16 + M], dpp_f<kDppRowBcast + M>(rs.x), z0.x),
+              fmaf(winv[16 + M], dpp_f<kDppRowBcast + M>(rs.y), z0.y),
+              fmaf(winv[16 + M], dpp_f<kDppRowBcast + M>(rs.z), z0.z)}),
+       ...);
+    }(std::make_integer_sequence<int, 16>{});
     return z0;
   };
 
@@ -432,7 +474,7 @@ __global__ void __launch_bounds__(kPcgThreads)
   // correction is constant over a node).
   auto solve_banks = [&](unsigned e) {
     int ob = 0;
-    for (int cb = j; cb < S.ncb; cb += S.nwg, ++ob) {
+    for (int cb = c; cwg && cb < S.ncb; cb += S.ncw, ++ob) {
       const int* mem = cbm + kBankSlots * ob;
       if (mem[0] < S.n1) {
         // A level-1 bank: its inputs are level-1 sums themselves.
@@ -500,7 +542,7 @@ __global__ void __launch_bounds__(kPcgThreads)
         const double d = sum32(mi >= 0 ? ddot3(lds4(cin, t), zc) : 0.0);
         if (t == 0) put_double(a.tcp + 2 * static_cast<size_t>(S.cb0 + cb), e, d);
       }
-      if (cb + S.nwg < S.ncb) __syncthreads();
+      if (cb + S.ncw < S.ncb) __syncthreads();
     }
   };
 
@@ -513,13 +555,19 @@ __global__ void __launch_bounds__(kPcgThreads)
     return z0;
   };
 
-  // Publishes z0 of the own slot and this wavefront's r.z0 and r.r, solves
-  // this workgroup's coarse banks, then collects exchange 2; sets z and the
-  // totals.  False when the exchange gave up.
-  auto finish_z = [&](f3 z0, f3& z_out, double& rz_out, double& rr_out) -> bool {
+  // Exchange 2: a slot workgroup computes z0 = level-0 part of M r and
+  // publishes z0 of its slots and its wavefronts' r.z0 and r.r; a coarse
+  // workgroup solves its banks.  Then every workgroup collects the exchange;
+  // sets z (slot workgroups) and the totals.  False when the exchange gave
+  // up.
+  auto finish_z = [&](f3& z_out, double& rz_out, double& rr_out) -> bool {
     ++ep;
-    if (own) put_f3(a.tz + 3 * static_cast<size_t>(slot), ep, z0);
-    {
+    f3 z0 = {0.0f, 0.0f, 0.0f};
+    if (cwg) {
+      solve_banks(ep);
+    } else {
+      z0 = mas_l0(r);
+      if (own) put_f3(a.tz + 3 * static_cast<size_t>(slot), ep, z0);
       const double s0 = wave_sum(own ? ddot3(r, z0) : 0.0);
       const double s1 = wave_sum(own ? ddot3(r, r) : 0.0);
       if (lane0) {
@@ -527,18 +575,10 @@ __global__ void __launch_bounds__(kPcgThreads)
         put_double(my_tp + 2 * kWaveWords, ep, s1);
       }
     }
-    if (coarse) solve_banks(ep);
     if (!__syncthreads_and(collect(x2, ep, dyn, a.abort_flag))) return false;
     z_out = coarse && in_bank ? add_coarse(z0, scn) : z0;
-    double rz0;
-    part_totals(pw, S.nwg, kWaveWords, 2 * kWaveWords, rz0, rr_out);
-    double rzc = 0.0;
-    for (int cb = 0; cb < S.ncb; ++cb) {
-      const unsigned long long lo = __float_as_uint(cpw[2 * cb]);
-      const unsigned long long hi = __float_as_uint(cpw[2 * cb + 1]);
-      rzc += __longlong_as_double(static_cast<long long>((hi << 32) | lo));
-    }
-    rz_out = rz0 + rzc;
+    // r.z = r.z0 over the slots plus r.z_coarse over the coarse banks.
+    part_total2(pw, S.nwg, kWaveWords, cpw, S.ncb, 2 * kWaveWords, rz_out, rr_out);
     return true;
   };
 
@@ -552,7 +592,7 @@ __global__ void __launch_bounds__(kPcgThreads)
   }
   double rz, rr;
   f3 z;
-  if (!finish_z(mas_l0(r), z, rz, rr)) return;
+  if (!finish_z(z, rz, rr)) return;
   const double rr0 = rr;
   const double abs2 = static_cast<double>(a.abs_tol) * a.abs_tol;
   const double rel2 = static_cast<double>(rel_tol) * rel_tol;
@@ -611,8 +651,7 @@ __global__ void __launch_bounds__(kPcgThreads)
       if (lane0) put_double(my_tp, ep, s);
     }
     if (!__syncthreads_and(collect(x1, ep, dyn, a.abort_flag))) return;
-    double pAp, unused;
-    part_totals(pw, S.nwg, 0, 0, pAp, unused);
+    const double pAp = part_total(pw, S.nwg, 0);
     const float alpha = static_cast<float>(fabs(pAp) < 1e-20 ? 0.0 : rz / pAp);
     dx = fma3(alpha, p, dx);
     r = fma3(-alpha, q, r);
@@ -623,7 +662,7 @@ __global__ void __launch_bounds__(kPcgThreads)
                      fmaf(-alpha, qn.z, cn.z)});
     }
     double rz_new;
-    if (!finish_z(mas_l0(r), z, rz_new, rr)) return;
+    if (!finish_z(z, rz_new, rr)) return;
     ++iter;
     if (iter >= a.max_iter) status = 2;
     else if (rr <= abs2) status = 0;
