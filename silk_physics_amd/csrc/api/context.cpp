@@ -455,10 +455,12 @@ void Context::build_scene() {
       }
       const int per_wg = kPcgThreads / kBankSlots;
       sd.nwg = std::max(1, (nb0 + per_wg - 1) / per_wg);
-      // Coarse banks are solved by workgroups of their own (one per bank, up
-      // to a third of the slot workgroups), in parallel with the slot
-      // workgroups' level-0 work.
-      sd.ncw = std::min(sd.ncb, std::max(1, (sd.nwg + 2) / 3));
+      // Coarse banks are solved by workgroups of their own, one wavefront
+      // per bank (one bank per workgroup while they number at most a third
+      // of the slot workgroups), in parallel with the slot workgroups'
+      // level-0 work.
+      sd.ncw = std::max(std::min(sd.ncb, std::max(1, (sd.nwg + 2) / 3)),
+                        (sd.ncb + kPcgWaves - 1) / kPcgWaves);
       sd.wg0 = ps_wg;
       ps_wg += sd.nwg + sd.ncw;
       ps_sys.push_back(sd);
@@ -519,7 +521,7 @@ void Context::build_scene() {
   // index into the workgroup's slots [0, kPcgThreads) then its halo.
   std::vector<int> ps_halo_ptr{0}, ps_halo_slot, ps_sell_lcol(ps_sell_col.size(), 0);
   std::vector<int> ps_wg_n1_ptr{0}, ps_wg_n1, ps_wg_c_ptr{0}, ps_wg_c;
-  int nhalo_max = 0, nwg_max = 0, cbw_max = 0, need1_max = 0;
+  int nhalo_max = 0, nwg_max = 0;
   bool xch_fits = true;
   {
     const int nslot = static_cast<int>(ps_slot_row.size());
@@ -573,7 +575,6 @@ void Context::build_scene() {
     // workgroups), and the coarse nodes whose corrections its own and halo
     // slots add (slot workgroups).
     for (const auto& sd : ps_sys) {
-      if (sd.ncw > 0) cbw_max = std::max(cbw_max, (sd.ncb + sd.ncw - 1) / sd.ncw);
       for (int j = 0; j < sd.nwg + sd.ncw; ++j) {
         const int w = sd.wg0 + j;
         const bool cwg = j >= sd.nwg;
@@ -601,7 +602,6 @@ void Context::build_scene() {
         ps_wg_n1_ptr.push_back(static_cast<int>(ps_wg_n1.size()));
         ps_wg_c.insert(ps_wg_c.end(), cs.begin(), cs.end());
         ps_wg_c_ptr.push_back(static_cast<int>(ps_wg_c.size()));
-        need1_max = std::max(need1_max, static_cast<int>(n1s.size()));
         // Tagged words of the two exchanges of an iteration.
         const int cap = kPcgThreads * kPcgXchWords;
         const int nh = ps_halo_ptr[w + 1] - ps_halo_ptr[w];
@@ -784,8 +784,6 @@ void Context::build_scene() {
     a.cnl_max = max_cnl;
     a.nhalo_max = nhalo_max;
     a.nwg_max = nwg_max;
-    a.cbw_max = cbw_max;
-    a.need1_max = need1_max;
     a.grid = ps_wg;
     const char* mode = std::getenv("SILK_HIP_PCG");
     const bool want = !(mode && std::strcmp(mode, "launch") == 0);

@@ -145,6 +145,23 @@ __device__ __forceinline__ T sum64(T v) {
   return v;
 }
 
+/// sum64 of two doubles with the steps of the two sums interleaved (the
+/// same bits as two calls).
+__device__ __forceinline__ void sum64_pair(double& a, double& b) {
+  a += dpp_d<kDppXor1>(a);
+  b += dpp_d<kDppXor1>(b);
+  a += dpp_d<kDppXor2>(a);
+  b += dpp_d<kDppXor2>(b);
+  a += dpp_d<kDppHalfMirror>(a);
+  b += dpp_d<kDppHalfMirror>(b);
+  a += dpp_d<kDppMirror>(a);
+  b += dpp_d<kDppMirror>(b);
+  a += xor16_d(a);
+  b += xor16_d(b);
+  a += xor32_d(a);
+  b += xor32_d(b);
+}
+
 /// Sum of `v` over the workgroup in FP64, valid in thread 0.  blockDim.x must
 /// be a multiple of 64 and at most 1024.  `lds` needs 16 doubles.
 __device__ __forceinline__ double block_sum(double v, double* lds) {

@@ -295,8 +295,6 @@ struct PersistDev {
   int cnl_max = 0;      // max coarse-node descendant list length (LDS)
   int nhalo_max = 0;    // max halo slots of a workgroup (LDS)
   int nwg_max = 0;      // max workgroups of a system (LDS)
-  int cbw_max = 0;      // max coarse banks solved by one workgroup (LDS)
-  int need1_max = 0;    // max level-1 nodes read by one workgroup (LDS)
   int grid = 0;         // total workgroups
 };
 

@@ -10,9 +10,11 @@
 // thread keeps its row's r, dx, z, p, q, neighbour list and row of its bank
 // inverse in registers for the whole solve, and the workgroup keeps in LDS p
 // of its own slots and of its halo (the other workgroups' slots its rows
-// reference).  A coarse workgroup owns no slots: it keeps in LDS the
-// inverses of the coarse banks it solves and the level-1 sums they read, and
-// solves them while the slot workgroups compute their level-0 part.
+// reference).  A coarse workgroup owns no slots: each of its wavefronts
+// solves one coarse bank, keeping its members' residual sums (in double) and
+// its rows of the bank inverse in registers, while the slot workgroups
+// compute their level-0 part.  Bank products broadcast the inputs across
+// lanes with DPP (row_newbcast) instead of staging them in LDS.
 //
 // Per iteration k:
 //   p_k = z_k + beta_k p_{k-1} for the own slots and, with the same
@@ -22,9 +24,10 @@
 //   exchange 1        level-1 node sums of q and the p.q partials (and a
 //                     word of each coarse workgroup)
 //   alpha; slot workgroups: dx += alpha p; r -= alpha q; z0 = level-0 part
-//                     of M r.  Coarse workgroups: the level-1 sums of r
-//                     follow by cr -= alpha * qsum; coarse bank cb is solved
-//                     by coarse workgroup cb % ncw, which publishes its
+//                     of M r.  Coarse workgroups: a member's residual sum
+//                     follows by cr -= alpha * (its sum of the level-1 q
+//                     sums); coarse bank cb is solved by wavefront cb / ncw
+//                     of coarse workgroup cb % ncw, which publishes its
 //                     members' corrections and the bank's r.z_coarse (the
 //                     correction is constant over a node)
 //   exchange 2        z0 of the own slots, r.z0 and r.r partials and the
@@ -47,9 +50,9 @@
 // bounded: a wait that gives up raises an abort flag, every workgroup leaves,
 // and the host reports an error.
 #include <cstdlib>
-#include <utility>
 
 #include "hip/device_common.hpp"
+#include "hip/dpp_dot.hpp"
 #include "hip/kernels.hpp"
 
 namespace silk::amd {
@@ -125,53 +128,63 @@ __device__ bool collect(const Share& sh, unsigned epoch, float* lds,
   return true;
 }
 
-// Partial e (workgroup e / kPcgWaves, wavefront e % kPcgWaves) of the
-// block at word offset h of each workgroup's kPcgPartWords (a double whose
-// low word comes first; offsets are even and the array 16-byte aligned).
-__device__ __forceinline__ double part_at(const float* pw, int e, int h) {
-  const int o = kPcgPartWords * (e / kPcgWaves) + h + 2 * (e % kPcgWaves);
-  return *reinterpret_cast<const double*>(pw + o);
-}
-
 // Sum over the wavefront; the same bits in every lane and in every
 // wavefront summing the same values (sum64).
 __device__ __forceinline__ double wave_sum(double v) { return sum64(v); }
 
-// Total of the per-wavefront partials of block h over the workgroups plus
-// the n_extra doubles at `extra`: lane l adds partials l, l + 64, ... and
-// then extras l, l + 64, ... in order, then a butterfly; the same bits in
-// every thread of every workgroup.
-__device__ __forceinline__ double part_total(const float* pw, int nwg, int h,
+// Adds to `a` the doubles src[e] (two floats each in LDS, low word first,
+// 8-byte aligned) for e = lane, lane + 64, ... < n in that order; the loads
+// of four steps are issued before their additions.
+__device__ __forceinline__ void add_strided(double& a, const float* src, int n) {
+  const double* d = reinterpret_cast<const double*>(src);
+  const int lane = static_cast<int>(threadIdx.x % kWave);
+  for (int e0 = lane; e0 < n; e0 += 4 * kWave) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = d[min(e0 + u * kWave, n - 1)];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (e0 + u * kWave < n) a += v[u];
+  }
+}
+
+// Total of n per-wavefront partials (partial e of wavefront e % kPcgWaves
+// of workgroup e / kPcgWaves) plus the n_extra doubles at `extra`: lane l
+// adds partials l, l + 64, ... and then extras l, l + 64, ... in order,
+// then a butterfly; the same bits in every thread of every workgroup.
+__device__ __forceinline__ double part_total(const float* parts, int n,
                                              const float* extra = nullptr,
                                              int n_extra = 0) {
-  const int lane = static_cast<int>(threadIdx.x) % kWave;
   double a = 0.0;
-  for (int e = lane; e < kPcgWaves * nwg; e += kWave) a += part_at(pw, e, h);
-  for (int e = lane; e < n_extra; e += kWave)
-    a += *reinterpret_cast<const double*>(extra + 2 * e);
+  add_strided(a, parts, n);
+  add_strided(a, extra, n_extra);
   return wave_sum(a);
 }
 
-// part_total of blocks h0 (with the extras) and h1 at once: the same bits
-// as two calls, with the two sums interleaved.
-__device__ __forceinline__ void part_total2(const float* pw, int nwg, int h0,
+// part_total of two sets of partials (the first with the extras): the same
+// bits as two calls, with the two sums interleaved.
+__device__ __forceinline__ void part_total2(const float* pa, const float* pb, int n,
                                             const float* extra, int n_extra,
-                                            int h1, double& t0, double& t1) {
-  const int lane = static_cast<int>(threadIdx.x) % kWave;
+                                            double& ta, double& tb) {
   double a = 0.0, b = 0.0;
-  for (int e = lane; e < kPcgWaves * nwg; e += kWave) {
-    a += part_at(pw, e, h0);
-    b += part_at(pw, e, h1);
-  }
-  for (int e = lane; e < n_extra; e += kWave)
-    a += *reinterpret_cast<const double*>(extra + 2 * e);
-  t0 = wave_sum(a);
-  t1 = wave_sum(b);
+  add_strided(a, pa, n);
+  add_strided(b, pb, n);
+  add_strided(a, extra, n_extra);
+  sum64_pair(a, b);
+  ta = a;
+  tb = b;
 }
 
 __device__ __forceinline__ f3 fma3(float a, f3 x, f3 y) {
   return {fmaf(a, x.x, y.x), fmaf(a, x.y, y.y), fmaf(a, x.z, y.z)};
 }
+
+// Three doubles: a coarse member's residual sum.  A sum over thousands of
+// slots kept as a float would carry a rounding of its initial size, which
+// near convergence exceeds the sum itself.
+struct d3 {
+  double x, y, z;
+};
 
 __device__ __forceinline__ double ddot3(f3 a, f3 b) {
   return static_cast<double>(a.x) * b.x + static_cast<double>(a.y) * b.y +
@@ -253,7 +266,6 @@ __global__ void __launch_bounds__(kPcgThreads)
   const int b_lo = cwg ? 0 : (j * nb0) / S.nwg;
   const int b_hi = cwg ? 0 : ((j + 1) * nb0) / S.nwg;
   const int lb = b_lo + t / kBankSlots;
-  const int hw = t / kBankSlots;
   const int s_in_bank = t & (kBankSlots - 1);
   const bool in_bank = lb < b_hi;
   const int slot = S.slot0 + kBankSlots * lb + s_in_bank;
@@ -264,63 +276,68 @@ __global__ void __launch_bounds__(kPcgThreads)
   const int nh = a.halo_ptr[blockIdx.x + 1] - h0;
   const size_t hl = static_cast<size_t>(kPcgThreads) + a.nhalo_max;
 
-  float* invc = dyn;                                           // cbw_max x 1024
-  float* csum = invc + static_cast<size_t>(a.cbw_max) * 1024;  // 4 x n1_max
-  float* czl = csum + 4 * static_cast<size_t>(a.n1_max);      // 4 x ncn_max
-  float* qs = czl + 4 * static_cast<size_t>(a.ncn_max);       // 4 x n1_max
-  float* cin = qs + 4 * static_cast<size_t>(a.n1_max);        // 4 x 32
-  float* cpart = cin + 4 * kBankSlots;                         // 4 x 256
-  float* pw = cpart + 4 * kPcgThreads;                         // parts x nwg_max
-  float* cpw = pw + kPcgPartWords * static_cast<size_t>(a.nwg_max);  // 4 x ncb_max
+  float* czl = dyn;                                            // 4 x (ncn_max + 1)
+  float* qs = czl + 4 * static_cast<size_t>(a.ncn_max + 1);   // 4 x n1_max
+  // Per-wavefront partials as doubles (two floats, partial e = wavefront
+  // e % kPcgWaves of workgroup e / kPcgWaves): p.q, r.z0 and r.r.
+  float* ppq = qs + 4 * static_cast<size_t>(a.n1_max);        // 8 x nwg_max
+  float* prz = ppq + 8 * static_cast<size_t>(a.nwg_max);      // 8 x nwg_max
+  float* prr = prz + 8 * static_cast<size_t>(a.nwg_max);      // 8 x nwg_max
+  float* cpw = prr + 8 * static_cast<size_t>(a.nwg_max);      // 4 x ncb_max
   float* zl = cpw + 4 * static_cast<size_t>(a.ncb_max);       // 4 x hl
   float* pl = zl + 4 * hl;                                     // 4 x hl
-  int* cbm = reinterpret_cast<int*>(pl + 4 * hl);              // 32 x cbw_max
-  int* cnp = cbm + kBankSlots * a.cbw_max;                     // ncn_max + 1
+  int* cbm = reinterpret_cast<int*>(pl + 4 * hl);              // 32 x kPcgWaves
+  int* cnp = cbm + kBankSlots * kPcgWaves;                     // ncn_max + 1
   int* cnl = cnp + a.ncn_max + 1;                              // cnl_max
-  int* nl1 = cnl + a.cnl_max;                                  // need1_max
-  int* cbn = nl1 + a.need1_max;                                // cbw_max
   const int n1_0 = a.wg_n1_ptr[blockIdx.x];
   const int nn1 = a.wg_n1_ptr[blockIdx.x + 1] - n1_0;
   const int c_0 = a.wg_c_ptr[blockIdx.x];
   const int nc = a.wg_c_ptr[blockIdx.x + 1] - c_0;
+  const int wave = t / kWave;
+  const int lane = t % kWave;
+  const bool lane0 = lane == 0;
 
-  // Static data into LDS: the inverses and members of the coarse banks this
-  // workgroup solves (cb = c, c + ncw, ...; none for a slot workgroup),
-  // every coarse node's level-1 list, and the level-1 nodes those banks read.
-  for (int e = t; cwg && e < a.cbw_max * 1024; e += kPcgThreads) {
-    const int cb = c + S.ncw * (e / 1024);
-    if (cb < S.ncb)
-      invc[e] = a.bank_inv[static_cast<size_t>(a.cb_bank[S.cb0 + cb]) * 1024 + (e & 1023)];
-  }
-  for (int e = t; e < a.cbw_max * kBankSlots; e += kPcgThreads) {
-    const int cb = c + S.ncw * (e / kBankSlots);
-    cbm[e] = cwg && cb < S.ncb ? a.cb_member[kBankSlots * (S.cb0 + cb) + e % kBankSlots] : -1;
-  }
+  // Coarse workgroup: wavefront w solves coarse bank cbk = c + ncw * w (at
+  // most one per wavefront).  Lane l keeps the residual sum of member l % 32
+  // (cs) and computes output member i_out = l % 16 + 16 * (l / 32) over the
+  // inputs kh + [0, 16), kh = l & 16; lanes l and l ^ 16 join their halves.
+  const int cbk = c + S.ncw * wave;
+  const bool has_bank = cwg && cbk < S.ncb;
+  const int i_out = (lane & 15) + 16 * (lane >> 5);
+  const int kh = lane & 16;
+  const int cbase = kBankSlots * (S.cb0 + (has_bank ? cbk : 0));
+  const int mem_s = has_bank ? a.cb_member[cbase + (lane & 31)] : -1;
+  const int mem_o = has_bank ? a.cb_member[cbase + i_out] : -1;
+  // Members present (they fill the first slots) and whether the bank is of
+  // level 1 (its inputs are level-1 sums themselves; its first member is
+  // always present).
+  const int nm = __popcll(__ballot(lane < kBankSlots && mem_s >= 0));
+  const bool deep = __shfl(mem_s, 0) >= S.n1;
+
+  // Static data into LDS: the members of the wavefronts' coarse banks and
+  // every coarse node's level-1 list.
+  if (lane < kBankSlots) cbm[kBankSlots * wave + lane] = mem_s;
+  if (t < 4) czl[4 * a.ncn_max + t] = 0.0f;
   const int cn_first = S.ncn > 0 ? a.cn_ptr[S.cn0] : 0;
   for (int e = t; e <= S.ncn; e += kPcgThreads)
     cnp[e] = a.cn_ptr[S.cn0 + e] - cn_first;
   for (int e = t; S.ncn > 0 && e < a.cn_ptr[S.cn0 + S.ncn] - cn_first;
        e += kPcgThreads)
     cnl[e] = a.cn_list[cn_first + e];
-  for (int e = t; e < nn1; e += kPcgThreads) nl1[e] = a.wg_n1[n1_0 + e];
-  // Members present in each of those banks (they fill the first slots).
-  for (int ob = t; ob < a.cbw_max; ob += kPcgThreads) {
-    const int cb = c + S.ncw * ob;
-    int nm = 0;
-    if (cwg && cb < S.ncb)
-      while (nm < kBankSlots && a.cb_member[kBankSlots * (S.cb0 + cb) + nm] >= 0) ++nm;
-    cbn[ob] = nm;
-  }
 
   // This slot's row of its level-0 bank inverse, starting with the entries of
   // the slots in its own row of 16 lanes: winv[k] is entry k ^ (16 * row).
+  // In a coarse workgroup, winv[m] (m < 16) is entry kh + m of row i_out of
+  // the wavefront's coarse bank inverse.
   float winv[kBankSlots];
   const int row16 = s_in_bank & 16;
+  const size_t cinv = has_bank ? static_cast<size_t>(a.cb_bank[S.cb0 + cbk]) * 1024 : 0;
 #pragma unroll
   for (int k = 0; k < kBankSlots; ++k)
     winv[k] = in_bank ? a.bank_inv[static_cast<size_t>(S.bank0 + lb) * 1024 +
                                    (k ^ row16) * kBankSlots + s_in_bank]
-                      : 0.0f;
+              : has_bank && k < 16 ? a.bank_inv[cinv + (kh + k) * kBankSlots + i_out]
+                                   : 0.0f;
   int e_off = 0, e_w = 0, e_lane = 0;
   if (own) {
     const int sl = slot / kSlice;
@@ -340,17 +357,25 @@ __global__ void __launch_bounds__(kPcgThreads)
     ncol[k] = v ? a.sell_lcol[e_off + k * kSlice + e_lane] : t;
     nval[k] = v ? a.sell_val[e_off + k * kSlice + e_lane] : 0.0f;
   }
+  // Coarse nodes of the slot (and of the halo slots this thread completes)
+  // per level; an absent level points at record ncn_max of czl, which stays
+  // zero, so that the additions need no branch.
+  const int zrec = a.ncn_max;
   int scn[4] = {-1, -1, -1, -1};
   if (in_bank && coarse)
     for (int l = 0; l < 4; ++l) scn[l] = a.slot_cn[4 * slot + l];
-  // Coarse nodes of the halo slots this thread completes.
+  int scz[4];
+#pragma unroll
+  for (int l = 0; l < 4; ++l) scz[l] = scn[l] >= 0 ? scn[l] : zrec;
   int hsc[kPcgHaloPerThread][4];
 #pragma unroll
   for (int m = 0; m < kPcgHaloPerThread; ++m) {
     const int hh = t + kPcgThreads * m;
 #pragma unroll
-    for (int l = 0; l < 4; ++l)
-      hsc[m][l] = coarse && hh < nh ? a.slot_cn[4 * a.halo_slot[h0 + hh] + l] : -1;
+    for (int l = 0; l < 4; ++l) {
+      const int cn = coarse && hh < nh ? a.slot_cn[4 * a.halo_slot[h0 + hh] + l] : -1;
+      hsc[m][l] = cn >= 0 ? cn : zrec;
+    }
   }
   const float dd = own ? a.d[row] : 0.0f;
   f3 r = own ? ld3(a.r0, row) : f3{0.0f, 0.0f, 0.0f};
@@ -400,11 +425,11 @@ __global__ void __launch_bounds__(kPcgThreads)
         } else if (i < aw1) {
           const int w = (i - nw1) / kWaveWords, h = (i - nw1) % kWaveWords;
           x1.addr[m] = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + h;
-          x1.dest[m] = static_cast<int>(pw - dyn) + kPcgPartWords * w + h;
+          x1.dest[m] = static_cast<int>(ppq - dyn) + kWaveWords * w + h;
         } else {
           const int w = S.nwg + (i - aw1);
           x1.addr[m] = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + kAckWord;
-          x1.dest[m] = static_cast<int>(pw - dyn) + kPcgPartWords * w + kAckWord;
+          x1.dest[m] = static_cast<int>(ppq - dyn) + kWaveWords * w + kAckWord;
         }
       }
       x2.addr[m] = a.tp;
@@ -412,9 +437,10 @@ __global__ void __launch_bounds__(kPcgThreads)
       if (i < w2) {
         x2.mask |= 1u << m;
         if (i < pw2) {
-          const int w = i / (2 * kWaveWords), h = kWaveWords + i % (2 * kWaveWords);
-          x2.addr[m] = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + h;
-          x2.dest[m] = static_cast<int>(pw - dyn) + kPcgPartWords * w + h;
+          const int w = i / (2 * kWaveWords), h = i % (2 * kWaveWords);
+          x2.addr[m] = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + kWaveWords + h;
+          x2.dest[m] = static_cast<int>((h < kWaveWords ? prz : prr) - dyn) +
+                       kWaveWords * w + h % kWaveWords;
         } else if (i < cw2) {
           x2.addr[m] = a.tcp + 2 * static_cast<size_t>(S.cb0) + (i - pw2);
           x2.dest[m] = static_cast<int>(cpw - dyn) + (i - pw2);
@@ -430,8 +456,6 @@ __global__ void __launch_bounds__(kPcgThreads)
       }
     }
   }
-  const int wave = t / kWave;
-  const bool lane0 = (t % kWave) == 0;
   unsigned long long* my_tp =
       a.tp + kPcgPartWords * static_cast<size_t>(blockIdx.x) + 2 * wave;
   __syncthreads();
@@ -447,113 +471,88 @@ __global__ void __launch_bounds__(kPcgThreads)
   };
 
   // Level-0 part of z = M r: this slot's bank row times r of its bank (the
-  // 32 lanes of its half-wave), r of each slot broadcast from its lane: step
-  // m adds slot m of the own row of 16 lanes and slot m of the other row.
-  // Called by every thread (all lanes active); lanes outside a bank give 0.
+  // 32 lanes of its half-wave), r of each slot broadcast from its lane: the
+  // slots of the own row of 16 lanes, then those of the other row.  Called
+  // by every thread (all lanes active); lanes outside a bank give 0.
   auto mas_l0 = [&](f3 rv) -> f3 {
     const f3 rs = {xor16_f(rv.x), xor16_f(rv.y), xor16_f(rv.z)};
-    f3 z0 = {0.0f, 0.0f, 0.0f};
-    [&]<int... M>(std::integer_sequence<int, M...>) {
-      ((z0 = {fmaf(winv[M], dpp_f<kDppRowBcast + M>(rv.x), z0.x),
-              fmaf(winv[M], dpp_f<kDppRowBcast + M>(rv.y), z0.y),
-              fmaf(winv[M], dpp_f<kDppRowBcast + M>(rv.z), z0.z)},
-        z0 = {fmaf(winv[16 + M], dpp_f<kDppRowBcast + M>(rs.x), z0.x),
-              fmaf(winv[16 + M], dpp_f<kDppRowBcast + M>(rs.y), z0.y),
-              fmaf(winv[16 + M], dpp_f<kDppRowBcast + M>(rs.z), z0.z)}),
-       ...);
-    }(std::make_integer_sequence<int, 16>{});
-    return z0;
+    return {row_bcast_dot16(row_bcast_dot16(0.0f, rv.x, winv), rs.x, winv + 16),
+            row_bcast_dot16(row_bcast_dot16(0.0f, rv.y, winv), rs.y, winv + 16),
+            row_bcast_dot16(row_bcast_dot16(0.0f, rv.z, winv), rs.z, winv + 16)};
   };
 
-  // The coarse banks of this workgroup: each member's input is the sum of r
-  // over its level-1 nodes (in a level-1 bank, recognised by its first
-  // member, which is always present, the level-1 sum itself; above, a group
-  // of lanes per member).  The product with the bank inverse is split over the
-  // eight half-waves (four inputs each) and joined by the first one, which
-  // publishes the members' corrections and the bank's r . z_coarse (the
-  // correction is constant over a node).
-  auto solve_banks = [&](unsigned e) {
-    int ob = 0;
-    for (int cb = c; cwg && cb < S.ncb; cb += S.ncw, ++ob) {
-      const int* mem = cbm + kBankSlots * ob;
-      if (mem[0] < S.n1) {
-        // A level-1 bank: its inputs are level-1 sums themselves.
-        if (t < kBankSlots) {
-          const int m = mem[t];
-          sts4(cin, t, m >= 0 ? lds4(csum, m) : f3{0.0f, 0.0f, 0.0f});
-        }
-      } else {
-        // Members are the first nm slots; each gets g lanes (8 to 64, as
-        // many as the bank's members leave), the level-1 nodes strided.
-        const int nm = cbn[ob];
-        const int g = nm <= 4 ? 64 : (nm <= 8 ? 32 : (nm <= 16 ? 16 : 8));
-        const int k = t / g, sub = t % g;
-        const int m = k < nm ? mem[k] : -1;
-        f3 v = {0.0f, 0.0f, 0.0f};
-        if (m >= 0) {
-          const int e_end = cnp[m + 1];
-          for (int ee = cnp[m] + sub; ee < e_end; ee += 4 * g) {
-            int id[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) id[u] = ee + g * u < e_end ? cnl[ee + g * u] : -1;
-            f3 c[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-              c[u] = id[u] >= 0 ? lds4(csum, id[u]) : f3{0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-              if (id[u] >= 0) v = v + c[u];
-          }
-        }
-        // Sum over the lane group (the same g in the whole workgroup).
-        v = {sum8(v.x), sum8(v.y), sum8(v.z)};
-        if (g >= 16) {
-          v.x += dpp_f<kDppMirror>(v.x);
-          v.y += dpp_f<kDppMirror>(v.y);
-          v.z += dpp_f<kDppMirror>(v.z);
-        }
-        if (g >= 32) v = {v.x + xor16_f(v.x), v.y + xor16_f(v.y), v.z + xor16_f(v.z)};
-        if (g >= 64) v = {v.x + xor32_f(v.x), v.y + xor32_f(v.y), v.z + xor32_f(v.z)};
-        if (sub == 0 && k < nm) sts4(cin, k, v);
-        if (t >= nm && t < kBankSlots) sts4(cin, t, f3{0.0f, 0.0f, 0.0f});
-      }
-      __syncthreads();
-      {
-        const float* inv = invc + static_cast<size_t>(ob) * 1024;
-        const int k0 = 4 * hw;
-        f3 part = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int k = k0; k < k0 + 4; ++k) {
-          const float w = inv[k * kBankSlots + s_in_bank];
-          const f3 c = lds4(cin, k);
-          part.x += w * c.x;
-          part.y += w * c.y;
-          part.z += w * c.z;
-        }
-        sts4(cpart, t, part);
-      }
-      __syncthreads();
-      if (t < kBankSlots) {
-        f3 zc = lds4(cpart, t);
-#pragma unroll
-        for (int q = 1; q < kBanksPerWg; ++q) zc = zc + lds4(cpart, q * kBankSlots + t);
-        const int mi = mem[t];
-        if (mi >= 0) put_f3(a.tc + 3 * static_cast<size_t>(S.cn0 + mi), e, zc);
-        const double d = sum32(mi >= 0 ? ddot3(lds4(cin, t), zc) : 0.0);
-        if (t == 0) put_double(a.tcp + 2 * static_cast<size_t>(S.cb0 + cb), e, d);
-      }
-      if (cb + S.ncw < S.ncb) __syncthreads();
+  // Coarse bank of this wavefront (called by all its lanes, wave-uniformly).
+  // sum_nodes(src): for lane l, the sum (in double) of src over the level-1
+  // nodes of member l % 32 (zero for an absent member): in a level-1 bank the
+  // member's own entry; above, g lanes per member (as many as the members
+  // leave) sum its level-1 nodes strided, and lane l takes its member's
+  // group sum.
+  const int grp = nm <= 1 ? 64 : nm <= 2 ? 32 : nm <= 4 ? 16 : nm <= 8 ? 8 : nm <= 16 ? 4 : 2;
+  auto sum_nodes = [&](const float* src) -> d3 {
+    if (!deep) {
+      const f3 v = mem_s >= 0 ? lds4(src, mem_s) : f3{0.0f, 0.0f, 0.0f};
+      return {v.x, v.y, v.z};
     }
+    const int k = lane / grp, sub = lane % grp;
+    const int m = k < nm ? cbm[kBankSlots * wave + k] : -1;
+    d3 v = {0.0, 0.0, 0.0};
+    if (m >= 0) {
+      const int e_end = cnp[m + 1];
+      for (int ee = cnp[m] + sub; ee < e_end; ee += 4 * grp) {
+        int id[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) id[u] = ee + grp * u < e_end ? cnl[ee + grp * u] : -1;
+        f3 cv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          cv[u] = id[u] >= 0 ? lds4(src, id[u]) : f3{0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (id[u] >= 0) v = {v.x + cv[u].x, v.y + cv[u].y, v.z + cv[u].z};
+      }
+    }
+    // Sum over the lane group (grp is the same in the whole wavefront).
+    auto step = [&](auto&& other) { v = {v.x + other(v.x), v.y + other(v.y), v.z + other(v.z)}; };
+    if (grp >= 2) step([](double x) { return dpp_d<kDppXor1>(x); });
+    if (grp >= 4) step([](double x) { return dpp_d<kDppXor2>(x); });
+    if (grp >= 8) step([](double x) { return dpp_d<kDppHalfMirror>(x); });
+    if (grp >= 16) step([](double x) { return dpp_d<kDppMirror>(x); });
+    if (grp >= 32) step([](double x) { return xor16_d(x); });
+    if (grp >= 64) step([](double x) { return xor32_d(x); });
+    const int from = ((lane & 31) * grp) & (kWave - 1);
+    const d3 own_sum = {__shfl(v.x, from), __shfl(v.y, from), __shfl(v.z, from)};
+    return mem_s >= 0 ? own_sum : d3{0.0, 0.0, 0.0};
+  };
+
+  // zc = bank inverse times the members' residual sums cs (cs of member
+  // kh + m broadcast from lane 16 * (l / 16) + m), halves joined over l ^ 16
+  // (both lanes then hold the same bits); publishes the members'
+  // corrections (the correction is constant over a node) and the bank's
+  // r . z_coarse.
+  auto solve_bank = [&](d3 csd, unsigned e) {
+    const f3 cs = {static_cast<float>(csd.x), static_cast<float>(csd.y),
+                   static_cast<float>(csd.z)};
+    f3 zc = {row_bcast_dot16(0.0f, cs.x, winv), row_bcast_dot16(0.0f, cs.y, winv),
+             row_bcast_dot16(0.0f, cs.z, winv)};
+    zc = {zc.x + xor16_f(zc.x), zc.y + xor16_f(zc.y), zc.z + xor16_f(zc.z)};
+    if (kh == 0 && mem_o >= 0) put_f3(a.tc + 3 * static_cast<size_t>(S.cn0 + mem_o), e, zc);
+    const double d = sum64((lane & 31) == i_out && mem_o >= 0 ? ddot3(cs, zc) : 0.0);
+    if (lane0) put_double(a.tcp + 2 * static_cast<size_t>(S.cb0 + cbk), e, d);
   };
 
   // z of an own slot: the level-0 part plus the coarse nodes' corrections,
   // in level order; halo slots are completed with the same additions.
   auto add_coarse = [&](f3 z0, const int* cn) -> f3 {
+    f3 cv[4];
 #pragma unroll
-    for (int l = 0; l < 4; ++l)
-      if (cn[l] >= 0) z0 = z0 + lds4(czl, cn[l]);
+    for (int l = 0; l < 4; ++l) cv[l] = lds4(czl, cn[l]);
+#pragma unroll
+    for (int l = 0; l < 4; ++l) z0 = z0 + cv[l];
     return z0;
   };
+
+  // Residual sum of this lane's coarse member (coarse workgroups).
+  d3 cs = {0.0, 0.0, 0.0};
 
   // Exchange 2: a slot workgroup computes z0 = level-0 part of M r and
   // publishes z0 of its slots and its wavefronts' r.z0 and r.r; a coarse
@@ -564,21 +563,22 @@ __global__ void __launch_bounds__(kPcgThreads)
     ++ep;
     f3 z0 = {0.0f, 0.0f, 0.0f};
     if (cwg) {
-      solve_banks(ep);
+      if (has_bank) solve_bank(cs, ep);
     } else {
       z0 = mas_l0(r);
       if (own) put_f3(a.tz + 3 * static_cast<size_t>(slot), ep, z0);
-      const double s0 = wave_sum(own ? ddot3(r, z0) : 0.0);
-      const double s1 = wave_sum(own ? ddot3(r, r) : 0.0);
+      double s0 = own ? ddot3(r, z0) : 0.0;
+      double s1 = own ? ddot3(r, r) : 0.0;
+      sum64_pair(s0, s1);
       if (lane0) {
         put_double(my_tp + kWaveWords, ep, s0);
         put_double(my_tp + 2 * kWaveWords, ep, s1);
       }
     }
     if (!__syncthreads_and(collect(x2, ep, dyn, a.abort_flag))) return false;
-    z_out = coarse && in_bank ? add_coarse(z0, scn) : z0;
+    z_out = coarse && in_bank ? add_coarse(z0, scz) : z0;
     // r.z = r.z0 over the slots plus r.z_coarse over the coarse banks.
-    part_total2(pw, S.nwg, kWaveWords, cpw, S.ncb, 2 * kWaveWords, rz_out, rr_out);
+    part_total2(prz, prr, kPcgWaves * S.nwg, cpw, S.ncb, rz_out, rr_out);
     return true;
   };
 
@@ -588,7 +588,7 @@ __global__ void __launch_bounds__(kPcgThreads)
     publish_node_sums(r, ep);
     if (lane0) put_double(my_tp, ep, 0.0);
     if (!__syncthreads_and(collect(x1, ep, dyn, a.abort_flag))) return;
-    for (int e = t; e < nn1; e += kPcgThreads) sts4(csum, nl1[e], lds4(qs, nl1[e]));
+    if (has_bank) cs = sum_nodes(qs);
   }
   double rz, rr;
   f3 z;
@@ -651,15 +651,17 @@ __global__ void __launch_bounds__(kPcgThreads)
       if (lane0) put_double(my_tp, ep, s);
     }
     if (!__syncthreads_and(collect(x1, ep, dyn, a.abort_flag))) return;
-    const double pAp = part_total(pw, S.nwg, 0);
+    // The members' sums of q (independent of alpha); their residual sums
+    // follow by cs -= alpha * qsum.
+    d3 cq = {0.0, 0.0, 0.0};
+    if (has_bank) cq = sum_nodes(qs);
+    const double pAp = part_total(ppq, kPcgWaves * S.nwg);
     const float alpha = static_cast<float>(fabs(pAp) < 1e-20 ? 0.0 : rz / pAp);
     dx = fma3(alpha, p, dx);
     r = fma3(-alpha, q, r);
-    for (int e = t; e < nn1; e += kPcgThreads) {
-      const int n = nl1[e];
-      const f3 qn = lds4(qs, n), cn = lds4(csum, n);
-      sts4(csum, n, {fmaf(-alpha, qn.x, cn.x), fmaf(-alpha, qn.y, cn.y),
-                     fmaf(-alpha, qn.z, cn.z)});
+    {
+      const double am = -static_cast<double>(alpha);
+      cs = {fma(am, cq.x, cs.x), fma(am, cq.y, cs.y), fma(am, cq.z, cs.z)};
     }
     double rz_new;
     if (!finish_z(z, rz_new, rr)) return;
@@ -701,17 +703,13 @@ size_t g_lds_checked = static_cast<size_t>(-1);
 
 size_t pcg_persistent_lds(const PersistDev& a) {
   const size_t hl = static_cast<size_t>(kPcgThreads) + a.nhalo_max;
-  return sizeof(float) * (static_cast<size_t>(a.cbw_max) * 1024 +
-                          8 * static_cast<size_t>(a.n1_max) +
-                          4 * static_cast<size_t>(a.ncn_max) + 4 * kBankSlots +
-                          4 * kPcgThreads +
-                          kPcgPartWords * static_cast<size_t>(a.nwg_max) +
+  return sizeof(float) * (4 * static_cast<size_t>(a.ncn_max + 1) +
+                          4 * static_cast<size_t>(a.n1_max) +
+                          24 * static_cast<size_t>(a.nwg_max) +
                           4 * static_cast<size_t>(a.ncb_max) + 8 * hl) +
-         sizeof(int) * (static_cast<size_t>(kBankSlots) * a.cbw_max +
+         sizeof(int) * (static_cast<size_t>(kBankSlots) * kPcgWaves +
                         static_cast<size_t>(a.ncn_max) + 1 +
-                        static_cast<size_t>(a.cnl_max) +
-                        static_cast<size_t>(a.need1_max) +
-                        static_cast<size_t>(a.cbw_max));
+                        static_cast<size_t>(a.cnl_max));
 }
 
 bool pcg_persistent_fits(int grid, size_t shmem) {
