@@ -62,7 +62,6 @@ using namespace dev;
 namespace {
 
 constexpr unsigned kSpinLimit = 1u << 22;
-constexpr int kBanksPerWg = kPcgThreads / kBankSlots;  // 8
 
 __device__ __forceinline__ void put_word(unsigned long long* w, unsigned epoch,
                                          unsigned bits) {
@@ -156,8 +155,20 @@ __device__ __forceinline__ double part_total(const float* parts, int n,
                                              const float* extra = nullptr,
                                              int n_extra = 0) {
   double a = 0.0;
-  add_strided(a, parts, n);
-  add_strided(a, extra, n_extra);
+  if (n <= 2 * kWave && n_extra <= kWave) {
+    // Up to two partials and one extra per lane: all loads at once.
+    const double* d = reinterpret_cast<const double*>(parts);
+    const double* x = reinterpret_cast<const double*>(extra);
+    const int lane = static_cast<int>(threadIdx.x % kWave);
+    const double v0 = d[min(lane, n - 1)], v1 = d[min(lane + kWave, n - 1)];
+    const double v2 = n_extra > 0 ? x[min(lane, n_extra - 1)] : 0.0;
+    if (lane < n) a += v0;
+    if (lane + kWave < n) a += v1;
+    if (lane < n_extra) a += v2;
+  } else {
+    add_strided(a, parts, n);
+    add_strided(a, extra, n_extra);
+  }
   return wave_sum(a);
 }
 
@@ -167,9 +178,30 @@ __device__ __forceinline__ void part_total2(const float* pa, const float* pb, in
                                             const float* extra, int n_extra,
                                             double& ta, double& tb) {
   double a = 0.0, b = 0.0;
-  add_strided(a, pa, n);
-  add_strided(b, pb, n);
-  add_strided(a, extra, n_extra);
+  if (n <= 2 * kWave && n_extra <= kWave) {
+    // Up to two partials of each set and one extra per lane: all loads at
+    // once (the additions in the order of the general case).
+    const double* da = reinterpret_cast<const double*>(pa);
+    const double* db = reinterpret_cast<const double*>(pb);
+    const double* x = reinterpret_cast<const double*>(extra);
+    const int lane = static_cast<int>(threadIdx.x % kWave);
+    const int i0 = min(lane, n - 1), i1 = min(lane + kWave, n - 1);
+    const double a0 = da[i0], a1 = da[i1], b0 = db[i0], b1 = db[i1];
+    const double x0 = n_extra > 0 ? x[min(lane, n_extra - 1)] : 0.0;
+    if (lane < n) {
+      a += a0;
+      b += b0;
+    }
+    if (lane + kWave < n) {
+      a += a1;
+      b += b1;
+    }
+    if (lane < n_extra) a += x0;
+  } else {
+    add_strided(a, pa, n);
+    add_strided(b, pb, n);
+    add_strided(a, extra, n_extra);
+  }
   sum64_pair(a, b);
   ta = a;
   tb = b;
