@@ -962,12 +962,23 @@ __device__ __forceinline__ bool round_contact(const ContactSet& cs, int r,
 
 // One Gauss-Seidel projection of contact c (contact_constraints.cu:245-297)
 // against positions P; corr is the contact's correction, updated in place.
+// The four positions of a contact, read unconditionally (an absent vertex
+// reads vertex 0 and is not used), so that the reads go out together.
+__device__ __forceinline__ void load_states(const ContactDev& c, const float* P,
+                                            dev::f3* pv) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) pv[s] = dev::ld3(P, max(c.state[s], 0));
+}
+
 __device__ __forceinline__ void gs_core(const ContactDev& c, float* P,
                                         dev::f3& corr) {
   using namespace dev;
+  f3 pv[4];
+  load_states(c, P, pv);
   f3 u = {c.affine[0], c.affine[1], c.affine[2]};
+#pragma unroll
   for (int s = 0; s < 4; ++s)
-    if (c.state[s] >= 0) u = u + c.coef[s] * ld3(P, c.state[s]);
+    if (c.state[s] >= 0) u = u + c.coef[s] * pv[s];
   const f3 old = corr;
   const f3 us = u - old;
   const f3 n = {c.normal[0], c.normal[1], c.normal[2]};
@@ -996,14 +1007,11 @@ __device__ __forceinline__ void gs_core_once(const ContactDev& c, float* P,
                                              dev::f3& corr) {
   using namespace dev;
   f3 pv[4];
+  load_states(c, P, pv);
   f3 u = {c.affine[0], c.affine[1], c.affine[2]};
-  for (int s = 0; s < 4; ++s) {
-    pv[s] = f3{0.0f, 0.0f, 0.0f};
-    if (c.state[s] >= 0) {
-      pv[s] = ld3(P, c.state[s]);
-      u = u + c.coef[s] * pv[s];
-    }
-  }
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+    if (c.state[s] >= 0) u = u + c.coef[s] * pv[s];
   const f3 old = corr;
   const f3 us = u - old;
   const f3 n = {c.normal[0], c.normal[1], c.normal[2]};
@@ -1046,6 +1054,19 @@ __device__ __forceinline__ void warm_core(const ContactDev& c, float* P,
   for (int s = 0; s < 4; ++s) {
     if (c.gamma[s] == 0.0f) continue;
     st3(P, c.state[s], ld3(P, c.state[s]) + c.gamma[s] * corr);
+  }
+}
+
+// warm_core for a contact whose state vertices are distinct (reads first).
+__device__ __forceinline__ void warm_core_once(const ContactDev& c, float* P,
+                                               const dev::f3& corr) {
+  using namespace dev;
+  f3 pv[4];
+  load_states(c, P, pv);
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    if (c.gamma[s] == 0.0f) continue;
+    st3(P, c.state[s], pv[s] + c.gamma[s] * corr);
   }
 }
 
@@ -1137,14 +1158,27 @@ __global__ void __launch_bounds__(kContactRegThreads)
   extern __shared__ float sh[];
   if (cs.skip != nullptr && *cs.skip != 0) return;
   float* P = sh;
+  int* off = reinterpret_cast<int*>(sh + 3 * static_cast<size_t>(nv));  // ncolor + 1
   const int tid = static_cast<int>(threadIdx.x);
-  for (int i = tid; i < nv; i += blockDim.x) {
-    const float w = cs.w[i];
-    for (int d = 0; d < 3; ++d) {
-      const int k = 3 * i + d;
-      P[k] = w > 0.0f ? x[k] - cs.eta[k] / w : x[k];
+  const int n3 = 3 * nv;
+  // Per coordinate, with the loads of four steps in flight at once.
+  for (int k0 = tid; k0 < n3; k0 += 4 * kContactRegThreads) {
+    float xv[4], ev[4], wv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = min(k0 + u * kContactRegThreads, n3 - 1);
+      xv[u] = x[k];
+      ev[u] = cs.eta[k];
+      wv[u] = cs.w[k / 3];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + u * kContactRegThreads;
+      if (k < n3) P[k] = wv[u] > 0.0f ? xv[u] - ev[u] / wv[u] : xv[u];
     }
   }
+  for (int i = tid; i <= cs.ncolor; i += kContactRegThreads) off[i] = cs.color_off[i];
+  __syncthreads();
   ContactDev c[kContactSlots];
   f3 corr[kContactSlots];
   int color[kContactSlots];
@@ -1158,10 +1192,10 @@ __global__ void __launch_bounds__(kContactRegThreads)
       c[s] = cs.c[e];
       if (distinct_states(c[s])) once |= 1u << s;
       corr[s] = ld3(cs.corr, c[s].id);
-      int lo = 0, hi = cs.ncolor;  // color_off[lo] <= e < color_off[hi]
+      int lo = 0, hi = cs.ncolor;  // off[lo] <= e < off[hi]
       while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (cs.color_off[mid] <= e) lo = mid;
+        if (off[mid] <= e) lo = mid;
         else hi = mid;
       }
       color[s] = lo;
@@ -1176,24 +1210,39 @@ __global__ void __launch_bounds__(kContactRegThreads)
 #pragma unroll
     for (int s = 0; s < kContactSlots; ++s) {
       if (color[s] != col) continue;
-      if (warm) warm_core(c[s], P, corr[s]);
-      else if (once & (1u << s)) gs_core_once(c[s], P, corr[s]);
-      else gs_core(c[s], P, corr[s]);
+      if (warm) {
+        if (once & (1u << s)) warm_core_once(c[s], P, corr[s]);
+        else warm_core(c[s], P, corr[s]);
+      } else if (once & (1u << s)) {
+        gs_core_once(c[s], P, corr[s]);
+      } else {
+        gs_core(c[s], P, corr[s]);
+      }
     }
     __syncthreads();
     if (++col == cs.ncolor) col = 0;
   }
-  // Multiplier update and dual residual terms (contact_constraints.cu:316-364).
-  for (int i = tid; i < nv; i += blockDim.x) {
-    const float w = cs.w[i];
-    for (int d = 0; d < 3; ++d) {
-      const int k = 3 * i + d;
+  // Multiplier update and dual residual terms (contact_constraints.cu:316-364),
+  // per coordinate with four steps' loads in flight.
+  for (int k0 = tid; k0 < n3; k0 += 4 * kContactRegThreads) {
+    float wv[4], po[4], xv[4], ev[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = min(k0 + u * kContactRegThreads, n3 - 1);
+      wv[u] = cs.w[k / 3];
+      po[u] = cs.p[k];
+      xv[u] = x[k];
+      ev[u] = cs.eta[k];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = k0 + u * kContactRegThreads;
+      if (k >= n3) continue;
       const float pn = P[k];
-      const float po = cs.p[k];
       cs.p[k] = pn;
-      if (w > 0.0f) {
-        cs.eta[k] += w * (pn - x[k]);
-        cs.dual[k] = w * (pn - po);
+      if (wv[u] > 0.0f) {
+        cs.eta[k] = ev[u] + wv[u] * (pn - xv[u]);
+        cs.dual[k] = wv[u] * (pn - po[u]);
       }
     }
   }
@@ -1285,7 +1334,7 @@ bool contact_update_fits_lds(int nv, int n) {
 void launch_contact_update(hipStream_t s, const ContactSet& cs,
                            const float* x) {
   if (cs.n == 0 || cs.ncolor == 0) return;
-  const size_t p_bytes = 12 * static_cast<size_t>(cs.nv);
+  const size_t p_bytes = 12 * static_cast<size_t>(cs.nv) + 4 * (static_cast<size_t>(cs.ncolor) + 1);
   if (cs.n <= kContactSlots * kContactRegThreads && p_bytes <= kContactLdsBytes) {
     static bool attr = false;
     if (!attr) {
