@@ -456,11 +456,10 @@ void Context::build_scene() {
       const int per_wg = kPcgThreads / kBankSlots;
       sd.nwg = std::max(1, (nb0 + per_wg - 1) / per_wg);
       // Coarse banks are solved by workgroups of their own, one wavefront
-      // per bank (one bank per workgroup while they number at most a third
-      // of the slot workgroups), in parallel with the slot workgroups'
-      // level-0 work.
-      sd.ncw = std::max(std::min(sd.ncb, std::max(1, (sd.nwg + 2) / 3)),
-                        (sd.ncb + kPcgWaves - 1) / kPcgWaves);
+      // per bank, in parallel with the slot workgroups' level-0 work; they
+      // fill whole workgroups, as every workgroup of a system takes part in
+      // its exchanges and an exchange slows with their number.
+      sd.ncw = (sd.ncb + kPcgWaves - 1) / kPcgWaves;
       sd.wg0 = ps_wg;
       ps_wg += sd.nwg + sd.ncw;
       ps_sys.push_back(sd);
