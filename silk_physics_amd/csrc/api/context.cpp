@@ -520,6 +520,10 @@ void Context::build_scene() {
   // index into the workgroup's slots [0, kPcgThreads) then its halo.
   std::vector<int> ps_halo_ptr{0}, ps_halo_slot, ps_sell_lcol(ps_sell_col.size(), 0);
   std::vector<int> ps_wg_n1_ptr{0}, ps_wg_n1, ps_wg_c_ptr{0}, ps_wg_c;
+  // Per coarse bank member: the positions [begin, end) of its level-1 nodes
+  // in its coarse workgroup's list.
+  std::vector<int> ps_cb_rng(2 * ps_cb_member.size() + 2, 0);
+  int need_max = 0;
   int nhalo_max = 0, nwg_max = 0;
   bool xch_fits = true;
   {
@@ -569,10 +573,11 @@ void Context::build_scene() {
             std::lower_bound(hl.begin(), hl.end(), c) - hl.begin());
       }
     });
-    // Coarse bank cb of a system is solved by its coarse workgroup cb % ncw.
-    // Per workgroup: the level-1 nodes whose sums its banks read (coarse
-    // workgroups), and the coarse nodes whose corrections its own and halo
-    // slots add (slot workgroups).
+    // Coarse bank cb of a system is solved by wavefront cb / ncw of its
+    // coarse workgroup cb % ncw.  Per workgroup: the level-1 nodes whose sums
+    // its banks read, member by member (a node read by several members is
+    // listed for each), for coarse workgroups, and the coarse nodes whose
+    // corrections its own and halo slots add, for slot workgroups.
     for (const auto& sd : ps_sys) {
       for (int j = 0; j < sd.nwg + sd.ncw; ++j) {
         const int w = sd.wg0 + j;
@@ -580,10 +585,13 @@ void Context::build_scene() {
         std::vector<int> n1s, cs;
         for (int cb = j - sd.nwg; cwg && cb < sd.ncb; cb += sd.ncw)
           for (int k = 0; k < kBankSlots; ++k) {
-            const int m = ps_cb_member[kBankSlots * (sd.cb0 + cb) + k];
-            if (m < 0) continue;
-            for (int e = ps_cn_ptr[sd.cn0 + m]; e < ps_cn_ptr[sd.cn0 + m + 1]; ++e)
-              n1s.push_back(ps_cn_list[e]);
+            const size_t mi = kBankSlots * static_cast<size_t>(sd.cb0 + cb) + k;
+            const int m = ps_cb_member[mi];
+            ps_cb_rng[2 * mi] = static_cast<int>(n1s.size());
+            for (int e = m < 0 ? 0 : ps_cn_ptr[sd.cn0 + m];
+                 m >= 0 && e < ps_cn_ptr[sd.cn0 + m + 1]; ++e)
+              n1s.push_back(sd.cn0 + ps_cn_list[e]);
+            ps_cb_rng[2 * mi + 1] = static_cast<int>(n1s.size());
           }
         auto add_slot = [&](int sl) {
           for (int l = 0; l < 4; ++l)
@@ -593,10 +601,9 @@ void Context::build_scene() {
         for (int sl = 0; sl < nslot; ++sl)
           if (slot_wg[sl] == w && ps_slot_row[sl] >= 0) add_slot(sl);
         for (int e = ps_halo_ptr[w]; e < ps_halo_ptr[w + 1]; ++e) add_slot(ps_halo_slot[e]);
-        for (auto* v : {&n1s, &cs}) {
-          std::sort(v->begin(), v->end());
-          v->erase(std::unique(v->begin(), v->end()), v->end());
-        }
+        std::sort(cs.begin(), cs.end());
+        cs.erase(std::unique(cs.begin(), cs.end()), cs.end());
+        need_max = std::max(need_max, static_cast<int>(n1s.size()));
         ps_wg_n1.insert(ps_wg_n1.end(), n1s.begin(), n1s.end());
         ps_wg_n1_ptr.push_back(static_cast<int>(ps_wg_n1.size()));
         ps_wg_c.insert(ps_wg_c.end(), cs.begin(), cs.end());
@@ -701,12 +708,10 @@ void Context::build_scene() {
       b.upload(v, st);
     };
     const int nslot = static_cast<int>(ps_slot_row.size());
-    int max_ncb = 0, max_ncn = 0, max_n1 = 0, max_cnl = 0;
+    int max_ncb = 0, max_ncn = 0;
     for (const auto& sd : ps_sys) {
       max_ncb = std::max(max_ncb, sd.ncb);
       max_ncn = std::max(max_ncn, sd.ncn);
-      max_n1 = std::max(max_n1, sd.n1);
-      max_cnl = std::max(max_cnl, ps_cn_ptr[sd.cn0 + sd.ncn] - ps_cn_ptr[sd.cn0]);
     }
     S.ps_sys.upload(ps_sys, st);
     up(S.ps_slot_row, ps_slot_row);
@@ -724,8 +729,7 @@ void Context::build_scene() {
     up(S.ps_slot_cn, ps_slot_cn);
     up(S.ps_cb_bank, ps_cb_bank);
     up(S.ps_cb_member, ps_cb_member);
-    up(S.ps_cn_ptr, ps_cn_ptr);
-    up(S.ps_cn_list, ps_cn_list);
+    up(S.ps_cb_rng, ps_cb_rng);
     const int ncn_all = static_cast<int>(ps_cn_ptr.size()) - 1;
     S.ps_tz.resize(3 * static_cast<size_t>(std::max(1, nslot)));
     S.ps_tz.zero(st);
@@ -761,8 +765,7 @@ void Context::build_scene() {
     a.slot_cn = S.ps_slot_cn.get();
     a.cb_bank = S.ps_cb_bank.get();
     a.cb_member = S.ps_cb_member.get();
-    a.cn_ptr = S.ps_cn_ptr.get();
-    a.cn_list = S.ps_cn_list.get();
+    a.cb_rng = S.ps_cb_rng.get();
     a.bank_inv = S.bank_inv.get();
     a.d = S.d.get();
     a.r0 = S.r.get();
@@ -777,10 +780,9 @@ void Context::build_scene() {
     a.out = S.ps_out.get();
     a.abs_tol = global_.linear_solver_abs_tol;
     a.max_iter = pcg_max_iter();
-    a.n1_max = max_n1;
+    a.need_max = need_max;
     a.ncn_max = max_ncn;
     a.ncb_max = max_ncb;
-    a.cnl_max = max_cnl;
     a.nhalo_max = nhalo_max;
     a.nwg_max = nwg_max;
     a.grid = ps_wg;

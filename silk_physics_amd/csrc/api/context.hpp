@@ -103,7 +103,7 @@ struct Scene {
   DevBuf<PcgSysDev> ps_sys;
   DevBuf<int> ps_slot_row, ps_sell_off, ps_sell_width, ps_sell_lcol, ps_halo_ptr,
       ps_halo_slot, ps_wg_n1_ptr, ps_wg_n1, ps_wg_c_ptr, ps_wg_c, ps_slot_cn,
-      ps_cb_bank, ps_cb_member, ps_cn_ptr, ps_cn_list, ps_abort;
+      ps_cb_bank, ps_cb_member, ps_cb_rng, ps_abort;
   DevBuf<float> ps_sell_val;
   DevBuf<unsigned long long> ps_tz, ps_tn, ps_tp, ps_tc, ps_tcp;
   DevBuf<unsigned> ps_bar;

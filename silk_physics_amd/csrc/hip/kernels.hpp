@@ -264,15 +264,16 @@ struct PersistDev {
   const float* sell_val = nullptr;
   const int* halo_ptr = nullptr;    // per workgroup: its halo slots
   const int* halo_slot = nullptr;   //   (other workgroups' slots it reads)
-  const int* wg_n1_ptr = nullptr;   // per workgroup: level-1 nodes (cn0-
-  const int* wg_n1 = nullptr;       //   relative) its coarse banks read
+  const int* wg_n1_ptr = nullptr;   // per coarse workgroup: the level-1 nodes
+  const int* wg_n1 = nullptr;       //   (global ids) its banks read, member by
+                                    //   member
   const int* wg_c_ptr = nullptr;    // per workgroup: coarse nodes whose
   const int* wg_c = nullptr;        //   corrections its slots and halo add
   const int* slot_cn = nullptr;     // 4 per slot: coarse node (cn0-relative) per level
   const int* cb_bank = nullptr;     // per coarse bank: global bank id (bank_inv)
   const int* cb_member = nullptr;   // 32 per coarse bank: node (cn0-relative), -1
-  const int* cn_ptr = nullptr;      // per global coarse node: the level-1
-  const int* cn_list = nullptr;     //   nodes it is made of (cn0-relative)
+  const int* cb_rng = nullptr;      // 2 per coarse bank member: its nodes'
+                                    //   positions [begin, end) in the list
   const float* bank_inv = nullptr;  // 1024 per bank
   const float* d = nullptr;         // diag per row
   const float* r0 = nullptr;        // FP32 initial residual per row
@@ -289,10 +290,9 @@ struct PersistDev {
   AdmmCtl* ctl = nullptr;           // optional: skip flag, tolerance, counters
   float abs_tol = 0.0f;
   int max_iter = 1000;
-  int n1_max = 0;       // max level-1 nodes of a system (LDS)
+  int need_max = 0;     // max level-1 nodes listed for one workgroup (LDS)
   int ncn_max = 0;      // max coarse nodes of a system (LDS)
-  int ncb_max = 0;      // max coarse banks of a system (LDS inverses)
-  int cnl_max = 0;      // max coarse-node descendant list length (LDS)
+  int ncb_max = 0;      // max coarse banks of a system (LDS)
   int nhalo_max = 0;    // max halo slots of a workgroup (LDS)
   int nwg_max = 0;      // max workgroups of a system (LDS)
   int grid = 0;         // total workgroups

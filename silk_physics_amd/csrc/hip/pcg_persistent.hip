@@ -309,18 +309,15 @@ __global__ void __launch_bounds__(kPcgThreads)
   const size_t hl = static_cast<size_t>(kPcgThreads) + a.nhalo_max;
 
   float* czl = dyn;                                            // 4 x (ncn_max + 1)
-  float* qs = czl + 4 * static_cast<size_t>(a.ncn_max + 1);   // 4 x n1_max
+  float* qs = czl + 4 * static_cast<size_t>(a.ncn_max + 1);   // 4 x need_max
   // Per-wavefront partials as doubles (two floats, partial e = wavefront
   // e % kPcgWaves of workgroup e / kPcgWaves): p.q, r.z0 and r.r.
-  float* ppq = qs + 4 * static_cast<size_t>(a.n1_max);        // 8 x nwg_max
+  float* ppq = qs + 4 * static_cast<size_t>(a.need_max);      // 8 x nwg_max
   float* prz = ppq + 8 * static_cast<size_t>(a.nwg_max);      // 8 x nwg_max
   float* prr = prz + 8 * static_cast<size_t>(a.nwg_max);      // 8 x nwg_max
   float* cpw = prr + 8 * static_cast<size_t>(a.nwg_max);      // 4 x ncb_max
   float* zl = cpw + 4 * static_cast<size_t>(a.ncb_max);       // 4 x hl
   float* pl = zl + 4 * hl;                                     // 4 x hl
-  int* cbm = reinterpret_cast<int*>(pl + 4 * hl);              // 32 x kPcgWaves
-  int* cnp = cbm + kBankSlots * kPcgWaves;                     // ncn_max + 1
-  int* cnl = cnp + a.ncn_max + 1;                              // cnl_max
   const int n1_0 = a.wg_n1_ptr[blockIdx.x];
   const int nn1 = a.wg_n1_ptr[blockIdx.x + 1] - n1_0;
   const int c_0 = a.wg_c_ptr[blockIdx.x];
@@ -340,22 +337,14 @@ __global__ void __launch_bounds__(kPcgThreads)
   const int cbase = kBankSlots * (S.cb0 + (has_bank ? cbk : 0));
   const int mem_s = has_bank ? a.cb_member[cbase + (lane & 31)] : -1;
   const int mem_o = has_bank ? a.cb_member[cbase + i_out] : -1;
-  // Members present (they fill the first slots) and whether the bank is of
-  // level 1 (its inputs are level-1 sums themselves; its first member is
-  // always present).
+  // Positions [rb, re) of the level-1 sums of member l % 32 in this
+  // workgroup's list (qs), the members present (they fill the first slots)
+  // and whether some member reads more than one sum (a bank above level 1).
+  const int rb = has_bank ? a.cb_rng[2 * (cbase + (lane & 31))] : 0;
+  const int re = has_bank ? a.cb_rng[2 * (cbase + (lane & 31)) + 1] : 0;
   const int nm = __popcll(__ballot(lane < kBankSlots && mem_s >= 0));
-  const bool deep = __shfl(mem_s, 0) >= S.n1;
-
-  // Static data into LDS: the members of the wavefronts' coarse banks and
-  // every coarse node's level-1 list.
-  if (lane < kBankSlots) cbm[kBankSlots * wave + lane] = mem_s;
+  const bool deep = __ballot(re - rb > 1) != 0;
   if (t < 4) czl[4 * a.ncn_max + t] = 0.0f;
-  const int cn_first = S.ncn > 0 ? a.cn_ptr[S.cn0] : 0;
-  for (int e = t; e <= S.ncn; e += kPcgThreads)
-    cnp[e] = a.cn_ptr[S.cn0 + e] - cn_first;
-  for (int e = t; S.ncn > 0 && e < a.cn_ptr[S.cn0 + S.ncn] - cn_first;
-       e += kPcgThreads)
-    cnl[e] = a.cn_list[cn_first + e];
 
   // This slot's row of its level-0 bank inverse, starting with the entries of
   // the slots in its own row of 16 lanes: winv[k] is entry k ^ (16 * row).
@@ -452,8 +441,8 @@ __global__ void __launch_bounds__(kPcgThreads)
         x1.mask |= 1u << m;
         if (i < nw1) {
           const int n = a.wg_n1[n1_0 + i / 3];
-          x1.addr[m] = a.tn + 3 * static_cast<size_t>(S.cn0 + n) + i % 3;
-          x1.dest[m] = static_cast<int>(qs - dyn) + 4 * n + i % 3;
+          x1.addr[m] = a.tn + 3 * static_cast<size_t>(n) + i % 3;
+          x1.dest[m] = static_cast<int>(qs - dyn) + 4 * (i / 3) + i % 3;
         } else if (i < aw1) {
           const int w = (i - nw1) / kWaveWords, h = (i - nw1) % kWaveWords;
           x1.addr[m] = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + h;
@@ -515,33 +504,28 @@ __global__ void __launch_bounds__(kPcgThreads)
 
   // Coarse bank of this wavefront (called by all its lanes, wave-uniformly).
   // sum_nodes(src): for lane l, the sum (in double) of src over the level-1
-  // nodes of member l % 32 (zero for an absent member): in a level-1 bank the
-  // member's own entry; above, g lanes per member (as many as the members
-  // leave) sum its level-1 nodes strided, and lane l takes its member's
+  // nodes of member l % 32, positions [rb, re) of the list (zero for an
+  // absent member): in a level-1 bank the member's own entry; above, g lanes
+  // per member (as many as the members leave) sum its range strided, with
+  // the loads of four steps issued together, and lane l takes its member's
   // group sum.
   const int grp = nm <= 1 ? 64 : nm <= 2 ? 32 : nm <= 4 ? 16 : nm <= 8 ? 8 : nm <= 16 ? 4 : 2;
   auto sum_nodes = [&](const float* src) -> d3 {
     if (!deep) {
-      const f3 v = mem_s >= 0 ? lds4(src, mem_s) : f3{0.0f, 0.0f, 0.0f};
+      const f3 v = rb < re ? lds4(src, rb) : f3{0.0f, 0.0f, 0.0f};
       return {v.x, v.y, v.z};
     }
     const int k = lane / grp, sub = lane % grp;
-    const int m = k < nm ? cbm[kBankSlots * wave + k] : -1;
+    const int kb = __shfl(rb, k & 31), ke0 = __shfl(re, k & 31);
+    const int ke = k < nm ? ke0 : kb;
     d3 v = {0.0, 0.0, 0.0};
-    if (m >= 0) {
-      const int e_end = cnp[m + 1];
-      for (int ee = cnp[m] + sub; ee < e_end; ee += 4 * grp) {
-        int id[4];
+    for (int e0 = kb + sub; e0 < ke; e0 += 4 * grp) {
+      f3 cv[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) id[u] = ee + grp * u < e_end ? cnl[ee + grp * u] : -1;
-        f3 cv[4];
+      for (int u = 0; u < 4; ++u) cv[u] = lds4(src, min(e0 + grp * u, ke - 1));
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-          cv[u] = id[u] >= 0 ? lds4(src, id[u]) : f3{0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (id[u] >= 0) v = {v.x + cv[u].x, v.y + cv[u].y, v.z + cv[u].z};
-      }
+      for (int u = 0; u < 4; ++u)
+        if (e0 + grp * u < ke) v = {v.x + cv[u].x, v.y + cv[u].y, v.z + cv[u].z};
     }
     // Sum over the lane group (grp is the same in the whole wavefront).
     auto step = [&](auto&& other) { v = {v.x + other(v.x), v.y + other(v.y), v.z + other(v.z)}; };
@@ -553,7 +537,7 @@ __global__ void __launch_bounds__(kPcgThreads)
     if (grp >= 64) step([](double x) { return xor32_d(x); });
     const int from = ((lane & 31) * grp) & (kWave - 1);
     const d3 own_sum = {__shfl(v.x, from), __shfl(v.y, from), __shfl(v.z, from)};
-    return mem_s >= 0 ? own_sum : d3{0.0, 0.0, 0.0};
+    return rb < re ? own_sum : d3{0.0, 0.0, 0.0};
   };
 
   // zc = bank inverse times the members' residual sums cs (cs of member
@@ -736,12 +720,9 @@ size_t g_lds_checked = static_cast<size_t>(-1);
 size_t pcg_persistent_lds(const PersistDev& a) {
   const size_t hl = static_cast<size_t>(kPcgThreads) + a.nhalo_max;
   return sizeof(float) * (4 * static_cast<size_t>(a.ncn_max + 1) +
-                          4 * static_cast<size_t>(a.n1_max) +
+                          4 * static_cast<size_t>(a.need_max) +
                           24 * static_cast<size_t>(a.nwg_max) +
-                          4 * static_cast<size_t>(a.ncb_max) + 8 * hl) +
-         sizeof(int) * (static_cast<size_t>(kBankSlots) * kPcgWaves +
-                        static_cast<size_t>(a.ncn_max) + 1 +
-                        static_cast<size_t>(a.cnl_max));
+                          4 * static_cast<size_t>(a.ncb_max) + 8 * hl);
 }
 
 bool pcg_persistent_fits(int grid, size_t shmem) {
