@@ -552,7 +552,16 @@ __global__ void __launch_bounds__(kPcgThreads)
              row_bcast_dot16(0.0f, cs.z, winv)};
     zc = {zc.x + xor16_f(zc.x), zc.y + xor16_f(zc.y), zc.z + xor16_f(zc.z)};
     if (kh == 0 && mem_o >= 0) put_f3(a.tc + 3 * static_cast<size_t>(S.cn0 + mem_o), e, zc);
-    const double d = sum64((lane & 31) == i_out && mem_o >= 0 ? ddot3(cs, zc) : 0.0);
+    // Lane l contributes member i_out when it also holds that member's sum
+    // (lanes 0-15 and 48-63); with at most 16 members only lanes 0-15 do,
+    // so the sum spans as many lanes as the members need.
+    double d = (lane & 31) == i_out && mem_o >= 0 ? ddot3(cs, zc) : 0.0;
+    if (nm > 16) {
+      d = sum64(d);
+    } else if (nm > 1) {
+      d = sum8(d);
+      if (nm > 8) d += dpp_d<kDppMirror>(d);
+    }
     if (lane0) put_double(a.tcp + 2 * static_cast<size_t>(S.cb0 + cbk), e, d);
   };
 
