@@ -89,6 +89,21 @@ __global__ void __launch_bounds__(kRefitThreads)
 // ---------------------------------------------------------------- traversal
 constexpr int kTraverseThreads = kTraverseWg;
 constexpr int kPairStage = 2048;  // pairs a workgroup stages in LDS
+// A subtree a workgroup stages in LDS: at most this many nodes and leaf
+// primitive boxes (about 40 KB of LDS with the pair stage, four workgroups
+// per CU; twice these sizes halved the residency and lost more than the
+// larger trees gained).
+constexpr int kSubNodes = 511;
+constexpr int kSubPrims = 512;
+
+__device__ __forceinline__ Box6 lds_box(const float* b, int i) {
+  Box6 r;
+  for (int d = 0; d < 3; ++d) {
+    r.mn[d] = b[6 * i + d];
+    r.mx[d] = b[6 * i + 3 + d];
+  }
+  return r;
+}
 
 // All passes of a broadphase in one launch.  A work item is (job, subtree,
 // query): lane `item` walks the subtree rooted at heap node 2^depth + sub
@@ -99,12 +114,17 @@ constexpr int kPairStage = 2048;  // pairs a workgroup stages in LDS
 // subtrees are the leaves of a walk from the root (oibvh.cuh:417-516).
 // Pairs are staged in LDS and written out with one counter reservation per
 // workgroup (one shared counter per pair serialised the whole launch on it);
-// pairs beyond the stage go out one by one.
+// pairs beyond the stage go out one by one.  When all items of a workgroup
+// walk the same subtree (consecutive items are consecutive queries of one
+// subtree) and it fits, its node boxes and leaf primitive boxes are first
+// copied to LDS with coalesced loads, and the walks read them there.
 __global__ void __launch_bounds__(kTraverseThreads)
     k_broadphase(CollVerts v, const TraverseJob* __restrict__ jobs, int njobs,
                  PairOut pt, PairOut ee) {
   __shared__ int2 stage[kPairStage];
   __shared__ int stage_n, stage_base;
+  __shared__ float snode[6 * kSubNodes];
+  __shared__ float sprim[6 * kSubPrims];
   // Job ranges are whole workgroups: the job is uniform over the workgroup.
   const int wg0 = blockIdx.x * blockDim.x;
   int j = 0;
@@ -119,6 +139,31 @@ __global__ void __launch_bounds__(kTraverseThreads)
   if (job.gate_a != nullptr && job.gate_b != nullptr &&
       !overlap(*job.gate_a, *job.gate_b))
     active = false;
+  // The subtree of this workgroup's items, when it is one and fits: heap
+  // node r at depth `depth`, levels 0..D below it, leaves' primitives
+  // [jb, je).
+  const TreeDev& tj = job.tree;
+  const int first = wg0 - job.item0;
+  const int last = min(first + kTraverseThreads, job.nitem) - 1;
+  const int levels = 31 - __clz(tj.P) - job.depth;  // D
+  const int sub_nodes = (2 << levels) - 1;
+  const int r0 = (1 << job.depth) + (first >= 0 && job.q.n > 0 ? first / job.q.n : 0);
+  const int jb = ((r0 << levels) - tj.P) * kLeafPrims;
+  const int je = min(jb + (kLeafPrims << levels), tj.nprim);
+  const bool cached = first >= 0 && last >= first && job.q.n > 0 &&
+                      first / job.q.n == last / job.q.n && levels >= 0 &&
+                      sub_nodes <= kSubNodes && (kLeafPrims << levels) <= kSubPrims;
+  if (cached) {
+    for (int i = threadIdx.x; i < 6 * sub_nodes; i += kTraverseThreads) {
+      const int li = i / 6, c = i - 6 * li;
+      const int k = 31 - __clz(li + 1);
+      const int heap = (r0 << k) + (li + 1 - (1 << k));
+      snode[i] = reinterpret_cast<const float*>(tj.node)[6 * heap + c];
+    }
+    for (int i = threadIdx.x; i < 6 * max(0, je - jb); i += kTraverseThreads)
+      sprim[i] = reinterpret_cast<const float*>(tj.sbox)[6 * jb + i];
+    __syncthreads();
+  }
   if (active) {
     const QuerySet q = job.q;
     const TreeDev t = job.tree;
@@ -136,14 +181,21 @@ __global__ void __launch_bounds__(kTraverseThreads)
     int node = r;
     for (;;) {
       bool down = false;
-      if (overlap(qb, t.node[node])) {
+      Box6 nb;
+      if (cached) {
+        const int k = (31 - __clz(node)) - job.depth;
+        nb = lds_box(snode, (1 << k) - 1 + (node - (r << k)));
+      } else {
+        nb = t.node[node];
+      }
+      if (overlap(qb, nb)) {
         if (node < t.P) {
           down = true;
         } else {
           const int j0 = (node - t.P) * kLeafPrims;
           const int j1 = min(j0 + kLeafPrims, t.nprim);
           for (int jj = j0; jj < j1; ++jj) {
-            if (!overlap(qb, t.sbox[jj])) continue;
+            if (!overlap(qb, cached ? lds_box(sprim, jj - jb) : t.sbox[jj])) continue;
             int tv[3];
             float inv = qinv;
             for (int k = 0; k < t.nvert; ++k) {
