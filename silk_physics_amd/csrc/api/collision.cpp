@@ -177,6 +177,9 @@ class HipCollision final : public CollisionWorld {
   DevBuf<float> obj_friction_;
   std::vector<TreeDev> trees_;
   DevBuf<TreeDev> trees_dev_;
+  // First refit workgroup of every tree (256 leaves each), then the total.
+  DevBuf<int> refit_off_;
+  int refit_blocks_ = 0;
   DevBuf<int> tree_prim_, tree_prim_id_;
   DevBuf<Box6> sbox_, node_;
   // Broadphase passes (object pairs, then self) in one launch.
@@ -318,6 +321,11 @@ void HipCollision::finish_setup(const std::vector<float>& pos0) {
     trees_.push_back(t);
   }
   trees_dev_.upload(trees_, st_);
+  std::vector<int> roff(trees_.size() + 1, 0);
+  for (size_t i = 0; i < trees_.size(); ++i)
+    roff[i + 1] = roff[i] + (trees_[i].P + 255) / 256;
+  refit_off_.upload(roff, st_);
+  refit_blocks_ = roff.back();
   build_jobs(pos0);
 
   pt_pairs_.resize(pt_cap_);
@@ -364,7 +372,8 @@ void HipCollision::set_cloth(const float* a, const float* b) {
 }
 
 void HipCollision::refit() {
-  launch_refit(st_, verts(), trees_dev_.get(), static_cast<int>(trees_.size()));
+  launch_refit(st_, verts(), trees_dev_.get(), static_cast<int>(trees_.size()),
+               refit_off_.get(), refit_blocks_);
 }
 
 // Object pairs and self pairs of find_collision.cu:17-26,45-166: for each

@@ -60,26 +60,55 @@ __device__ __forceinline__ Box6 empty_box() {
 }
 
 // ---------------------------------------------------------------- refit
-__global__ void __launch_bounds__(kRefitThreads)
-    k_refit(CollVerts v, const TreeDev* __restrict__ trees) {
-  const TreeDev t = trees[blockIdx.x];
-  for (int i = threadIdx.x; i < t.nprim; i += blockDim.x) {
-    int gv[3];
-    for (int k = 0; k < t.nvert; ++k) gv[k] = t.prim[t.nvert * i + k];
-    Box6 b;
-    swept_box(v, gv, t.nvert, t.pad, b);
-    t.sbox[i] = b;
-  }
-  __syncthreads();
-  for (int l = threadIdx.x; l < t.P; l += blockDim.x) {
-    Box6 b = empty_box();
-    const int j0 = l * kLeafPrims;
-    const int j1 = min(j0 + kLeafPrims, t.nprim);
-    for (int j = j0; j < j1; ++j) b = merge(b, t.sbox[j]);
+// Refit in two launches.  Leaves: workgroup b of tree t (blocks
+// off[t] <= b < off[t + 1]) computes the primitive boxes and leaf boxes of
+// 256 consecutive leaves and the levels of their 256-leaf subtree in LDS
+// (boxes merge with min/max, so the order of merging does not matter).
+constexpr int kRefitLeaves = 256;
+
+__global__ void __launch_bounds__(kRefitLeaves)
+    k_refit_leaves(CollVerts v, const TreeDev* __restrict__ trees,
+                   const int* __restrict__ off, int ntrees) {
+  __shared__ Box6 lb[kRefitLeaves];
+  int ti = 0;
+  while (ti + 1 < ntrees && off[ti + 1] <= static_cast<int>(blockIdx.x)) ++ti;
+  const TreeDev t = trees[ti];
+  const int tid = static_cast<int>(threadIdx.x);
+  const int l0 = (static_cast<int>(blockIdx.x) - off[ti]) * kRefitLeaves;
+  const int l = l0 + tid;
+  Box6 b = empty_box();
+  if (l < t.P) {
+    const int j1 = min(l * kLeafPrims + kLeafPrims, t.nprim);
+    for (int j = l * kLeafPrims; j < j1; ++j) {
+      int gv[3];
+      for (int k = 0; k < t.nvert; ++k) gv[k] = t.prim[t.nvert * j + k];
+      Box6 pb;
+      swept_box(v, gv, t.nvert, t.pad, pb);
+      t.sbox[j] = pb;
+      b = merge(b, pb);
+    }
     t.node[t.P + l] = b;
   }
+  lb[tid] = b;
   __syncthreads();
-  for (int half = t.P >> 1; half >= 1; half >>= 1) {
+  for (int k = 1; (kRefitLeaves >> k) >= 1 && (t.P >> k) >= 1; ++k) {
+    const int w = min(kRefitLeaves >> k, t.P >> k);
+    Box6 m;
+    if (tid < w) m = merge(lb[2 * tid], lb[2 * tid + 1]);
+    __syncthreads();
+    if (tid < w) {
+      lb[tid] = m;
+      t.node[((t.P + l0) >> k) + tid] = m;
+    }
+    __syncthreads();
+  }
+}
+
+// Refit, top: the levels above the 256-leaf subtrees, one workgroup per tree.
+__global__ void __launch_bounds__(kRefitThreads)
+    k_refit_top(const TreeDev* __restrict__ trees) {
+  const TreeDev t = trees[blockIdx.x];
+  for (int half = (t.P / kRefitLeaves) >> 1; half >= 1; half >>= 1) {
     for (int j = half + threadIdx.x; j < 2 * half; j += blockDim.x)
       t.node[j] = merge(t.node[2 * j], t.node[2 * j + 1]);
     __syncthreads();
@@ -1306,9 +1335,10 @@ __global__ void __launch_bounds__(kContactRegThreads)
 }  // namespace
 
 void launch_refit(hipStream_t s, const CollVerts& v, const TreeDev* trees_dev,
-                  int ntrees) {
-  if (ntrees == 0) return;
-  k_refit<<<ntrees, kRefitThreads, 0, s>>>(v, trees_dev);
+                  int ntrees, const int* leaf_block_off, int nblocks) {
+  if (ntrees == 0 || nblocks == 0) return;
+  k_refit_leaves<<<nblocks, kRefitLeaves, 0, s>>>(v, trees_dev, leaf_block_off, ntrees);
+  k_refit_top<<<ntrees, kRefitThreads, 0, s>>>(trees_dev);
 }
 
 void launch_traverse_jobs(hipStream_t s, const CollVerts& v, const TraverseJob* jobs,

@@ -36,9 +36,11 @@ struct TreeDev {
   Box6* node = nullptr;        // 2P
 };
 
-/// Refit all trees: one workgroup per tree.
+/// Refit all trees: 256 leaves per workgroup (leaf_block_off: ntrees + 1
+/// prefix offsets of ceil(P / 256) workgroups per tree, nblocks in all),
+/// then the levels above them, one workgroup per tree.
 void launch_refit(hipStream_t s, const CollVerts& v, const TreeDev* trees_dev,
-                  int ntrees);
+                  int ntrees, const int* leaf_block_off, int nblocks);
 
 struct QuerySet {
   int n = 0;
