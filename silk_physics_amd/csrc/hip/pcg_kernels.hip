@@ -238,28 +238,37 @@ __global__ void __launch_bounds__(64) k_mas_factor(PcgDev p,
     A[s][s] = v;
   }
   __syncthreads();
-  // In-place Gauss-Jordan inverse (SPD, no pivoting needed).
-  float nv[16];
+  // In-place Gauss-Jordan inverse (SPD, no pivoting needed), in registers:
+  // lane i of the first wavefront holds row i; the pivot row is read with
+  // v_readlane (scalar registers) at each of the 32 fully unrolled steps.
+  float row[kBankSlots];
+  const int i = threadIdx.x & 31;
+#pragma unroll
+  for (int j = 0; j < kBankSlots; ++j) row[j] = A[i][j];
+#pragma unroll
   for (int piv = 0; piv < kBankSlots; ++piv) {
-    const float pv = A[piv][piv];
-    const float ipv = 1.0f / pv;
-    for (int m = 0; m < 16; ++m) {
-      const int e = threadIdx.x + 64 * m;
-      const int i = e >> 5, j = e & 31;
+    float prow[kBankSlots];
+#pragma unroll
+    for (int j = 0; j < kBankSlots; ++j)
+      prow[j] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(row[j]), piv));
+    const float ipv = 1.0f / prow[piv];
+    const float aip = row[piv];
+#pragma unroll
+    for (int j = 0; j < kBankSlots; ++j) {
       float val;
       if (i == piv && j == piv) val = ipv;
-      else if (i == piv) val = A[piv][j] * ipv;
-      else if (j == piv) val = -A[i][piv] * ipv;
-      else val = A[i][j] - A[i][piv] * (A[piv][j] * ipv);
-      nv[m] = val;
+      else if (i == piv) val = prow[j] * ipv;
+      else if (j == piv) val = -aip * ipv;
+      else val = row[j] - aip * (prow[j] * ipv);
+      row[j] = val;
     }
-    __syncthreads();
-    for (int m = 0; m < 16; ++m) {
-      const int e = threadIdx.x + 64 * m;
-      A[e >> 5][e & 31] = nv[m];
-    }
-    __syncthreads();
   }
+  __syncthreads();
+  if (threadIdx.x < kBankSlots) {
+#pragma unroll
+    for (int j = 0; j < kBankSlots; ++j) A[i][j] = row[j];
+  }
+  __syncthreads();
   float* out = p.bank_inv + static_cast<size_t>(bank) * 1024;
   for (int e = threadIdx.x; e < 1024; e += 64) {
     // Symmetrize: the exact inverse is symmetric.
