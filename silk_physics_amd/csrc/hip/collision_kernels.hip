@@ -1051,8 +1051,10 @@ __device__ __forceinline__ void load_states(const ContactDev& c, const float* P,
   for (int s = 0; s < 4; ++s) pv[s] = dev::ld3(P, max(c.state[s], 0));
 }
 
+// Normal and friction coefficient given apart from the record (the register
+// kernel keeps them in LDS).
 __device__ __forceinline__ void gs_core(const ContactDev& c, float* P,
-                                        dev::f3& corr) {
+                                        dev::f3& corr, dev::f3 n, float mu) {
   using namespace dev;
   f3 pv[4];
   load_states(c, P, pv);
@@ -1062,7 +1064,6 @@ __device__ __forceinline__ void gs_core(const ContactDev& c, float* P,
     if (c.state[s] >= 0) u = u + c.coef[s] * pv[s];
   const f3 old = corr;
   const f3 us = u - old;
-  const f3 n = {c.normal[0], c.normal[1], c.normal[2]};
   const float un = dot3(us, n);
   f3 unew;
   if (un >= 0.0f) {
@@ -1070,7 +1071,7 @@ __device__ __forceinline__ void gs_core(const ContactDev& c, float* P,
   } else {
     const f3 ut = us - un * n;
     const float utn = sqrtf(dot3(ut, ut));
-    const float rad = -c.mu * un;
+    const float rad = -mu * un;
     unew = (utn <= rad) ? f3{0.0f, 0.0f, 0.0f} : (1.0f - rad / utn) * ut;
   }
   const f3 du = unew - u;
@@ -1085,7 +1086,7 @@ __device__ __forceinline__ void gs_core(const ContactDev& c, float* P,
 // read for u are reused for the update (nothing else writes them during the
 // round: contacts of one colour share no vertex), which saves a second read.
 __device__ __forceinline__ void gs_core_once(const ContactDev& c, float* P,
-                                             dev::f3& corr) {
+                                             dev::f3& corr, dev::f3 n, float mu) {
   using namespace dev;
   f3 pv[4];
   load_states(c, P, pv);
@@ -1095,7 +1096,6 @@ __device__ __forceinline__ void gs_core_once(const ContactDev& c, float* P,
     if (c.state[s] >= 0) u = u + c.coef[s] * pv[s];
   const f3 old = corr;
   const f3 us = u - old;
-  const f3 n = {c.normal[0], c.normal[1], c.normal[2]};
   const float un = dot3(us, n);
   f3 unew;
   if (un >= 0.0f) {
@@ -1103,7 +1103,7 @@ __device__ __forceinline__ void gs_core_once(const ContactDev& c, float* P,
   } else {
     const f3 ut = us - un * n;
     const float utn = sqrtf(dot3(ut, ut));
-    const float rad = -c.mu * un;
+    const float rad = -mu * un;
     unew = (utn <= rad) ? f3{0.0f, 0.0f, 0.0f} : (1.0f - rad / utn) * ut;
   }
   const f3 du = unew - u;
@@ -1112,6 +1112,11 @@ __device__ __forceinline__ void gs_core_once(const ContactDev& c, float* P,
     if (c.gamma[s] == 0.0f) continue;
     st3(P, c.state[s], pv[s] + c.gamma[s] * du);
   }
+}
+
+__device__ __forceinline__ void gs_core(const ContactDev& c, float* P,
+                                        dev::f3& corr) {
+  gs_core(c, P, corr, dev::f3{c.normal[0], c.normal[1], c.normal[2]}, c.mu);
 }
 
 __device__ __forceinline__ bool distinct_states(const ContactDev& c) {
@@ -1230,9 +1235,18 @@ __global__ void __launch_bounds__(kContactThreads)
 // also drains outstanding global loads, which made every round pay one).
 // Contacts of one colour are independent, so a thread holding two of the
 // same colour may project them in either order.
+// With kNmLds each contact's normal and friction coefficient sit in LDS
+// (4 floats per contact after P and the offsets, 16-byte aligned) rather than
+// in registers: that frees 16 VGPRs per thread, which the 1024-thread launch
+// would otherwise spill, and a spill reload sat on every round's chain.
 constexpr int kContactRegThreads = kContactThreads;
 constexpr int kContactSlots = 4;
 
+__host__ __device__ constexpr size_t contact_nm_word(int nv, int ncolor) {
+  return (3 * static_cast<size_t>(nv) + static_cast<size_t>(ncolor) + 1 + 3) & ~size_t{3};
+}
+
+template <bool kNmLds>
 __global__ void __launch_bounds__(kContactRegThreads)
     k_contact_update_reg(ContactSet cs, const float* __restrict__ x, int nv) {
   using namespace dev;
@@ -1240,6 +1254,7 @@ __global__ void __launch_bounds__(kContactRegThreads)
   if (cs.skip != nullptr && *cs.skip != 0) return;
   float* P = sh;
   int* off = reinterpret_cast<int*>(sh + 3 * static_cast<size_t>(nv));  // ncolor + 1
+  float* NM = sh + contact_nm_word(nv, cs.ncolor);  // kNmLds: 4 per contact
   const int tid = static_cast<int>(threadIdx.x);
   const int n3 = 3 * nv;
   // Per coordinate, with the loads of four steps in flight at once.
@@ -1271,6 +1286,9 @@ __global__ void __launch_bounds__(kContactRegThreads)
     corr[s] = f3{0.0f, 0.0f, 0.0f};
     if (e < cs.n) {
       c[s] = cs.c[e];
+      if constexpr (kNmLds)
+        *reinterpret_cast<float4*>(NM + 4 * e) =
+            float4{c[s].normal[0], c[s].normal[1], c[s].normal[2], c[s].mu};
       if (distinct_states(c[s])) once |= 1u << s;
       corr[s] = ld3(cs.corr, c[s].id);
       int lo = 0, hi = cs.ncolor;  // off[lo] <= e < off[hi]
@@ -1294,11 +1312,21 @@ __global__ void __launch_bounds__(kContactRegThreads)
       if (warm) {
         if (once & (1u << s)) warm_core_once(c[s], P, corr[s]);
         else warm_core(c[s], P, corr[s]);
-      } else if (once & (1u << s)) {
-        gs_core_once(c[s], P, corr[s]);
-      } else {
-        gs_core(c[s], P, corr[s]);
+        continue;
       }
+      f3 n;
+      float mu;
+      if constexpr (kNmLds) {
+        const float4 q = *reinterpret_cast<const float4*>(
+            NM + 4 * (s * kContactRegThreads + tid));
+        n = f3{q.x, q.y, q.z};
+        mu = q.w;
+      } else {
+        n = f3{c[s].normal[0], c[s].normal[1], c[s].normal[2]};
+        mu = c[s].mu;
+      }
+      if (once & (1u << s)) gs_core_once(c[s], P, corr[s], n, mu);
+      else gs_core(c[s], P, corr[s], n, mu);
     }
     __syncthreads();
     if (++col == cs.ncolor) col = 0;
@@ -1417,15 +1445,26 @@ void launch_contact_update(hipStream_t s, const ContactSet& cs,
                            const float* x) {
   if (cs.n == 0 || cs.ncolor == 0) return;
   const size_t p_bytes = 12 * static_cast<size_t>(cs.nv) + 4 * (static_cast<size_t>(cs.ncolor) + 1);
-  if (cs.n <= kContactSlots * kContactRegThreads && p_bytes <= kContactLdsBytes) {
+  const size_t nm_bytes =
+      4 * contact_nm_word(cs.nv, cs.ncolor) + 16 * static_cast<size_t>(cs.n);
+  if (cs.n <= kContactSlots * kContactRegThreads && nm_bytes <= kContactLdsBytes) {
     static bool attr = false;
     if (!attr) {
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_contact_update_reg),
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_contact_update_reg<true>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 static_cast<int>(kContactLdsBytes));
       attr = true;
     }
-    k_contact_update_reg<<<1, kContactRegThreads, p_bytes, s>>>(cs, x, cs.nv);
+    k_contact_update_reg<true><<<1, kContactRegThreads, nm_bytes, s>>>(cs, x, cs.nv);
+  } else if (cs.n <= kContactSlots * kContactRegThreads && p_bytes <= kContactLdsBytes) {
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_contact_update_reg<false>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(kContactLdsBytes));
+      attr = true;
+    }
+    k_contact_update_reg<false><<<1, kContactRegThreads, p_bytes, s>>>(cs, x, cs.nv);
   } else if (contact_update_fits_lds(cs.nv, cs.n)) {
     const size_t shmem = 12 * (static_cast<size_t>(cs.nv) + cs.n);
     static bool attr = false;
