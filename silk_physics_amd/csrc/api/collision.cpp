@@ -630,7 +630,9 @@ void HipCollision::upload_contacts(int n_old) {
     ContactDev& d = hc[e];
     for (int k = 0; k < 4; ++k) {
       d.state[k] = c.state[k];
-      d.coef[k] = static_cast<float>(c.coef[k]);
+      // A static vertex's term is in affine; its coefficient is 0 here so
+      // that the kernels add every term without a test.
+      d.coef[k] = c.state[k] >= 0 ? static_cast<float>(c.coef[k]) : 0.0f;
       d.gamma[k] = static_cast<float>(c.gamma[k]);
     }
     for (int j = 0; j < 3; ++j) {

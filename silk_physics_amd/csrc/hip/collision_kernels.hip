@@ -1051,6 +1051,17 @@ __device__ __forceinline__ void load_states(const ContactDev& c, const float* P,
   for (int s = 0; s < 4; ++s) pv[s] = dev::ld3(P, max(c.state[s], 0));
 }
 
+// Projection of the tangential part ut onto the friction disc of radius
+// rad >= 0 (contact_constraints.cu:391-411): zero when |ut| <= rad, else
+// ut scaled by 1 - rad/|ut|.  The test is made on |ut|^2 and the scale uses
+// one reciprocal square root, which keeps the IEEE square-root and divide
+// sequences off the colour round's dependent chain.
+__device__ __forceinline__ dev::f3 friction_cone(const dev::f3& ut, float rad) {
+  const float t2 = dev::dot3(ut, ut);
+  if (t2 <= rad * rad) return dev::f3{0.0f, 0.0f, 0.0f};
+  return (1.0f - rad * rsqrtf(t2)) * ut;
+}
+
 // Normal and friction coefficient given apart from the record (the register
 // kernel keeps them in LDS).
 __device__ __forceinline__ void gs_core(const ContactDev& c, float* P,
@@ -1060,8 +1071,7 @@ __device__ __forceinline__ void gs_core(const ContactDev& c, float* P,
   load_states(c, P, pv);
   f3 u = {c.affine[0], c.affine[1], c.affine[2]};
 #pragma unroll
-  for (int s = 0; s < 4; ++s)
-    if (c.state[s] >= 0) u = u + c.coef[s] * pv[s];
+  for (int s = 0; s < 4; ++s) u = u + c.coef[s] * pv[s];  // coef 0 if static
   const f3 old = corr;
   const f3 us = u - old;
   const float un = dot3(us, n);
@@ -1069,10 +1079,7 @@ __device__ __forceinline__ void gs_core(const ContactDev& c, float* P,
   if (un >= 0.0f) {
     unew = us;
   } else {
-    const f3 ut = us - un * n;
-    const float utn = sqrtf(dot3(ut, ut));
-    const float rad = -mu * un;
-    unew = (utn <= rad) ? f3{0.0f, 0.0f, 0.0f} : (1.0f - rad / utn) * ut;
+    unew = friction_cone(us - un * n, -mu * un);
   }
   const f3 du = unew - u;
   corr = old + du;
@@ -1092,8 +1099,7 @@ __device__ __forceinline__ void gs_core_once(const ContactDev& c, float* P,
   load_states(c, P, pv);
   f3 u = {c.affine[0], c.affine[1], c.affine[2]};
 #pragma unroll
-  for (int s = 0; s < 4; ++s)
-    if (c.state[s] >= 0) u = u + c.coef[s] * pv[s];
+  for (int s = 0; s < 4; ++s) u = u + c.coef[s] * pv[s];  // coef 0 if static
   const f3 old = corr;
   const f3 us = u - old;
   const float un = dot3(us, n);
@@ -1101,10 +1107,7 @@ __device__ __forceinline__ void gs_core_once(const ContactDev& c, float* P,
   if (un >= 0.0f) {
     unew = us;
   } else {
-    const f3 ut = us - un * n;
-    const float utn = sqrtf(dot3(ut, ut));
-    const float rad = -mu * un;
-    unew = (utn <= rad) ? f3{0.0f, 0.0f, 0.0f} : (1.0f - rad / utn) * ut;
+    unew = friction_cone(us - un * n, -mu * un);
   }
   const f3 du = unew - u;
   corr = old + du;
