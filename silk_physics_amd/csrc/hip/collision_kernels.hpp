@@ -166,7 +166,7 @@ void launch_ccd_min(hipStream_t s, int n, const int* is_vf, const double* p0,
 /// Daviet contact (contact_constraints.cuh:19-42).
 struct ContactDev {
   int state[4];     // cloth vertex (solver order, global) or -1
-  float coef[4];
+  float coef[4];    // 0 where state is -1 (that term is in affine)
   float gamma[4];
   float normal[3];
   float affine[3];
