@@ -192,7 +192,8 @@ int silk_hip_get_partition(const silk_hip_ctx* ctx, uint32_t handle,
  * recorded; the others only while silk_hip_set_timing(ctx, 1) is on, from
  * HIP events around each launch on the library stream: 1 = PCG solve kernel,
  * 2 = local projection, 3 = contact solve, 4 = broadphase, 5 = CCD,
- * 6 = FP64 residual. */
+ * 6 = FP64 residual; 7 = host wall time of contact assembly and colouring
+ * (per call: sort, assembly, colouring, upload enqueue; no device time). */
 int silk_hip_get_timing(silk_hip_ctx* ctx, int32_t cls, double* ms,
                         int64_t* launches, int32_t reset);
 

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <unordered_set>
@@ -527,6 +528,18 @@ std::vector<DcdRec> HipCollision::dcd(float time, float dhat,
 
 // Contact assembly of contact_constraints.cu:23-150, in double.
 void HipCollision::append(std::vector<DcdRec> recs) {
+  const auto t0 = std::chrono::steady_clock::now();
+  struct HostTime {  // records on every return path
+    Context& c;
+    std::chrono::steady_clock::time_point t0;
+    ~HostTime() {
+      if (!c.timing_on_) return;
+      c.timing_ms_[kTimeHostContacts] +=
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+              .count();
+      c.timing_launch_[kTimeHostContacts] += 1;
+    }
+  } host_time{ctx_, t0};
   std::sort(recs.begin(), recs.end(), [](const DcdRec& a, const DcdRec& b) {
     return contact_key(a) < contact_key(b);
   });

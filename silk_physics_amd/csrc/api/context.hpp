@@ -28,6 +28,7 @@ enum : int {
   kTimeBroad = 4,     // refit + traversal
   kTimeCcd = 5,       // CCD kernel
   kTimeResidual = 6,  // FP64 residual / RHS kernel
+  kTimeHostContacts = 7,  // host wall time of contact assembly and colouring
 };
 
 struct ClothEntity {
