@@ -244,6 +244,12 @@ int silk_hip_debug_ccd_min(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
                            const double* err, double ms, double tolerance,
                            int32_t max_iter, double* min_toi);
 
+/* The contact solve's Coulomb projection (replaces solve_coulomb_contact,
+ * contact_constraints.cu:391-411, as the contact kernels run it) on n
+ * explicit cases: u and normal hold 3 floats per case, mu 1, out 3. */
+int silk_hip_debug_coulomb(silk_hip_ctx* ctx, int64_t n, const float* u,
+                           const float* normal, const float* mu, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,8 @@ def load_library():
                                                 C.POINTER(i64)]),
         "silk_hip_debug_ccd": (C.c_int, [vp, i64, ip, dp, dp, dp, C.c_double,
                                          C.c_double, i32, ip, dp]),
+        "silk_hip_debug_coulomb": (C.c_int, [vp, i64, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_void_p]),
         "silk_hip_debug_ccd_min": (C.c_int, [vp, i64, ip, dp, dp, dp, C.c_double,
                                              C.c_double, i32, dp]),
     }
@@ -501,6 +503,21 @@ class World:
         if st != 0:
             raise RuntimeError(self._wrap(st).to_string())
         return out.value
+
+    def debug_coulomb(self, u, normal, mu):
+        """The contact solve's Coulomb projection on explicit cases (n x 3
+        velocities and normals, n friction coefficients); returns n x 3."""
+        a = np.ascontiguousarray(u, dtype=np.float32).reshape(-1)
+        b = np.ascontiguousarray(normal, dtype=np.float32).reshape(-1)
+        m = np.ascontiguousarray(mu, dtype=np.float32).reshape(-1)
+        n = m.size
+        assert a.size == 3 * n and b.size == 3 * n
+        out = np.zeros(3 * n, dtype=np.float32)
+        st = load_library().silk_hip_debug_coulomb(
+            self._ctx, n, a.ctypes.data, b.ctypes.data, m.ctypes.data, out.ctypes.data)
+        if st != 0:
+            raise RuntimeError(self._wrap(st).to_string())
+        return out.reshape(n, 3)
 
     def timing(self, cls: int, reset: bool = False) -> tuple[float, int]:
         ms = C.c_double(0)

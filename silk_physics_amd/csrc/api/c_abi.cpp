@@ -474,4 +474,31 @@ int silk_hip_debug_ccd_min(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
   });
 }
 
+int silk_hip_debug_coulomb(silk_hip_ctx* ctx, int64_t n, const float* u,
+                           const float* normal, const float* mu, float* out) {
+  return guarded(ctx, [&](Context& c) {
+    if (n < 0 || n > (1 << 24)) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    if (n == 0) return int{SILK_HIP_OK};
+    if (!u || !normal || !mu || !out) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    c.ensure_device();
+    using silk::amd::DevBuf;
+    const size_t m = static_cast<size_t>(n);
+    std::vector<float> hu(u, u + 3 * m), hn(normal, normal + 3 * m), hm(mu, mu + m),
+        ho(3 * m, 0.0f);
+    DevBuf<float> du, dn, dm, dout;
+    du.upload(hu, c.stream_);
+    dn.upload(hn, c.stream_);
+    dm.upload(hm, c.stream_);
+    dout.upload(ho, c.stream_);
+    silk::amd::launch_coulomb_each(c.stream_, static_cast<int>(n), du.get(), dn.get(),
+                                   dm.get(), dout.get());
+    silk::amd::hip_check(hipGetLastError(), "coulomb launch");
+    silk::amd::hip_check(hipMemcpyAsync(out, dout.get(), 3 * m * sizeof(float),
+                                        hipMemcpyDeviceToHost, c.stream_),
+                         "coulomb out");
+    silk::amd::hip_check(hipStreamSynchronize(c.stream_), "coulomb sync");
+    return int{SILK_HIP_OK};
+  });
+}
+
 }  // extern "C"

@@ -1062,6 +1062,16 @@ __device__ __forceinline__ dev::f3 friction_cone(const dev::f3& ut, float rad) {
   return (1.0f - rad * rsqrtf(t2)) * ut;
 }
 
+// Coulomb projection of a contact velocity u with normal n and friction mu
+// (solve_coulomb_contact, contact_constraints.cu:391-411): a separating u
+// (u.n >= 0) is kept; otherwise the normal part is removed and the
+// tangential part projected onto the friction disc of radius -mu u.n.
+__device__ __forceinline__ dev::f3 coulomb(const dev::f3& u, const dev::f3& n, float mu) {
+  const float un = dev::dot3(u, n);
+  if (un >= 0.0f) return u;
+  return friction_cone(u - un * n, -mu * un);
+}
+
 // Normal and friction coefficient given apart from the record (the register
 // kernel keeps them in LDS).
 __device__ __forceinline__ void gs_core(const ContactDev& c, float* P,
@@ -1073,14 +1083,7 @@ __device__ __forceinline__ void gs_core(const ContactDev& c, float* P,
 #pragma unroll
   for (int s = 0; s < 4; ++s) u = u + c.coef[s] * pv[s];  // coef 0 if static
   const f3 old = corr;
-  const f3 us = u - old;
-  const float un = dot3(us, n);
-  f3 unew;
-  if (un >= 0.0f) {
-    unew = us;
-  } else {
-    unew = friction_cone(us - un * n, -mu * un);
-  }
+  const f3 unew = coulomb(u - old, n, mu);
   const f3 du = unew - u;
   corr = old + du;
   for (int s = 0; s < 4; ++s) {
@@ -1101,14 +1104,7 @@ __device__ __forceinline__ void gs_core_once(const ContactDev& c, float* P,
 #pragma unroll
   for (int s = 0; s < 4; ++s) u = u + c.coef[s] * pv[s];  // coef 0 if static
   const f3 old = corr;
-  const f3 us = u - old;
-  const float un = dot3(us, n);
-  f3 unew;
-  if (un >= 0.0f) {
-    unew = us;
-  } else {
-    unew = friction_cone(us - un * n, -mu * un);
-  }
+  const f3 unew = coulomb(u - old, n, mu);
   const f3 du = unew - u;
   corr = old + du;
   for (int s = 0; s < 4; ++s) {
@@ -1363,7 +1359,23 @@ __global__ void __launch_bounds__(kContactRegThreads)
     if (color[s] >= 0) st3(cs.corr, c[s].id, corr[s]);
 }
 
+// The contact kernels' Coulomb projection on explicit cases (3 floats per
+// case for u, n and out).
+__global__ void k_coulomb_each(int n, const float* __restrict__ u,
+                               const float* __restrict__ nrm,
+                               const float* __restrict__ mu, float* __restrict__ out) {
+  const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= n) return;
+  dev::st3(out, i, coulomb(dev::ld3(u, i), dev::ld3(nrm, i), mu[i]));
+}
+
 }  // namespace
+
+void launch_coulomb_each(hipStream_t s, int n, const float* u, const float* nrm,
+                         const float* mu, float* out) {
+  if (n <= 0) return;
+  k_coulomb_each<<<(n + 255) / 256, 256, 0, s>>>(n, u, nrm, mu, out);
+}
 
 void launch_refit(hipStream_t s, const CollVerts& v, const TreeDev* trees_dev,
                   int ntrees, const int* leaf_block_off, int nblocks) {

@@ -163,6 +163,11 @@ void launch_ccd_min(hipStream_t s, int n, const int* is_vf, const double* p0,
 
 // ---------------------------------------------------------------- contacts
 
+/// The Coulomb projection of the contact kernels (contact_constraints.cu:
+/// 391-411) on n explicit cases: u, nrm and out hold 3 floats per case.
+void launch_coulomb_each(hipStream_t s, int n, const float* u, const float* nrm,
+                         const float* mu, float* out);
+
 /// Daviet contact (contact_constraints.cuh:19-42).
 struct ContactDev {
   int state[4];     // cloth vertex (solver order, global) or -1

@@ -202,3 +202,43 @@ def test_free_running_sheet_does_not_penetrate_sphere():
         d = np.linalg.norm(x.reshape(-1, 3) - c, axis=1)
         assert d.min() >= r_in - 1e-3, (i, d.min(), r_in)
     assert touched
+
+
+def test_coulomb_kernel_against_reference_kats_and_oracle():
+    """The contact solve's Coulomb projection (solve_coulomb_contact,
+    contact_constraints.cu:391-411) as the contact kernels run it, against
+    the reference's known answers (friction_contact_test.cu:10-93, in
+    tests/golden/reference_kats.json) and against the FP64 oracle on the
+    reference's sweep grid and on random cases. FP32 tolerance: 1e-6
+    absolute on the exact cases, 1e-5 x max(1, |u|) elsewhere (the
+    projection is continuous across the cone boundary, so a boundary case
+    decided the other way in FP32 still lands within it)."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden",
+                           "reference_kats.json")) as f:
+        kat = json.load(f)
+    w = gpu_world()
+    ex = kat["coulomb_exact"]
+    got = w.debug_coulomb([c["u"] for c in ex], [c["n"] for c in ex],
+                          [c["mu"] for c in ex])
+    for c, g in zip(ex, got):
+        np.testing.assert_allclose(g, c["expected"], rtol=0, atol=1e-6, err_msg=c["name"])
+
+    sw = kat["coulomb_sweep"]
+    us = [u for _ in sw["mu"] for u in sw["u"]]
+    mus = [m for m in sw["mu"] for _ in sw["u"]]
+    rng = np.random.default_rng(7)
+    ru = rng.uniform(-2, 2, (2000, 3)).astype(np.float32)
+    rn = rng.normal(size=(2000, 3))
+    rn = (rn / np.linalg.norm(rn, axis=1, keepdims=True)).astype(np.float32)
+    rmu = rng.uniform(0, 2, 2000).astype(np.float32)
+    u_all = np.vstack([np.asarray(us, dtype=np.float32), ru])
+    n_all = np.vstack([np.tile(np.asarray(sw["n"], dtype=np.float32), (len(us), 1)), rn])
+    mu_all = np.concatenate([np.asarray(mus, dtype=np.float32), rmu])
+    got = w.debug_coulomb(u_all, n_all, mu_all)
+    for i in range(len(mu_all)):
+        u = u_all[i].astype(np.float64)
+        want = O.coulomb(u, n_all[i].astype(np.float64), float(mu_all[i]))
+        tol = 1e-5 * max(1.0, float(np.linalg.norm(u)))
+        assert np.max(np.abs(got[i] - want)) <= tol, (i, u, n_all[i], mu_all[i], got[i], want)
