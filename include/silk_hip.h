@@ -87,11 +87,41 @@ typedef struct {
   float ccd_tolerance;            /* 1e-6 */
   float ccd_min_separation_scale; /* 0.2 */
   float dcd_activation_scale;     /* 1.0 */
-  int32_t mas_precision;          /* 0 = fp32 blocks, 1 = fp16 blocks */
-  int32_t true_residual_stats;    /* 1 = evaluate the FP64 true residual of
-                                     every PCG solve (reported in stats) */
-  int32_t reserved[6];
+  int32_t mas_precision;          /* SILK_HIP_MAS_FP32 / _FP16 / _BF16: storage
+                                     of the MAS bank inverses (products are
+                                     accumulated in FP32 either way) */
+  int32_t true_residual_stats;    /* 1 = evaluate, on the device after every
+                                     PCG solve, the FP64 residual of the
+                                     iterate x0 + dx and of the committed
+                                     FP32 x (reported in stats) */
+  int32_t pcg_criterion;          /* SILK_HIP_PCG_RR0 (reference) or
+                                     SILK_HIP_PCG_RHS */
+  int32_t warm_start;             /* SILK_HIP_WARM_PREVIOUS (reference) or
+                                     SILK_HIP_WARM_LEARNED */
+  int32_t reserved[4];
 } silk_hip_global;
+
+enum {
+  SILK_HIP_MAS_FP32 = 0,
+  SILK_HIP_MAS_FP16 = 1,
+  SILK_HIP_MAS_BF16 = 2
+};
+
+enum {
+  /* Stop when rr <= rel^2 * rr0 (mas_cg_solver.cu:87-90). */
+  SILK_HIP_PCG_RR0 = 0,
+  /* Stop when ||b - A x|| <= rel * ||b|| (b the whole right-hand side of
+   * the x-update, FP64 norms): a better warm start takes fewer iterations. */
+  SILK_HIP_PCG_RHS = 1
+};
+
+enum {
+  /* CG starts from the previous ADMM iterate (admm_solver.cu:272-292). */
+  SILK_HIP_WARM_PREVIOUS = 0,
+  /* CG starts from the previous iterate plus the correction predicted by the
+   * per-vertex network loaded with silk_hip_set_warm_start_net. */
+  SILK_HIP_WARM_LEARNED = 1
+};
 
 /* Per-step counters returned by silk_hip_step. */
 typedef struct {
@@ -105,9 +135,13 @@ typedef struct {
   int32_t dcd_candidates;
   float min_toi;             /* earliest accepted TOI fraction (1 = none) */
   float last_pcg_rel_residual; /* sqrt(rr/rr0) of the last PCG solve */
-  float max_true_rel_residual; /* max over solves of |b - A x| / |b| (FP64),
+  float max_true_rel_residual; /* max over solves of |b - A (x0 + dx)| / |b|
+                                  (FP64, the PCG iterate before it is stored),
                                   when true_residual_stats is set */
-  int32_t reserved[5];
+  float max_committed_rel_residual; /* the same for the stored FP32 x */
+  int32_t pcg_max_solve_iterations; /* largest iteration count of one solve */
+  int32_t pcg_stalled_solves;       /* solves that ended at pcg_max_iter */
+  int32_t reserved[2];
 } silk_hip_step_stats;
 
 /* Number of visible HIP devices (0 when none).  Never fails. */

@@ -65,6 +65,10 @@ def lib():
             "oracle_ee_uv": (C.c_int, [d, C.c_double, d]),
             "oracle_candidates": (C.c_int, [vp, d, d, i, C.c_int, i, i, C.c_int,
                                             i]),
+            "oracle_dcd_contact": (C.c_int, [C.c_int, d, d, i, d, C.c_double, d]),
+            "oracle_sym3_min_eig": (C.c_double, [d]),
+            "oracle_contact_sweeps": (None, [C.c_int, i, d, d, d, d, d, C.c_int, d,
+                                             d, d, d, C.c_int, d, i]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -103,7 +107,8 @@ class OracleWorld:
                       g.get("admm_rel", 1e-3), g.get("pcg_max_iter", 1000),
                       g.get("sweeps", 10), g.get("ccd_max_iter", 4096),
                       g.get("ccd_tol", 1e-6), g.get("ms_scale", 0.2),
-                      g.get("dcd_scale", 1.0)], dtype=np.float64)
+                      g.get("dcd_scale", 1.0), g.get("pcg_criterion", 0)],
+                     dtype=np.float64)
         self._keep = [p]
         lib().oracle_set_global(self._w, _d(p)[1])
         self.sizes = []
@@ -322,3 +327,48 @@ def ee_uv(x, eps=1e-6):
     out = np.zeros(2)
     ok = lib().oracle_ee_uv(ap, eps, out.ctypes.data_as(C.POINTER(C.c_double)))
     return bool(ok), out
+
+
+def dcd_contact(pt: bool, x, inv_mass, is_static, cw, dhat):
+    """DCD record and Daviet contact of one pair on explicit positions (4x3):
+    None when the pair makes no contact, else a dict with dist, u, v, normal,
+    coef, gamma, state (bool), affine and the contact's u(x)."""
+    out = np.zeros(37)
+    xs, xp = _d(np.asarray(x).reshape(-1))
+    im, imp = _d(inv_mass)
+    st, stp = _i(np.asarray(is_static, dtype=np.int32))
+    c, cp = _d(cw)
+    rc = lib().oracle_dcd_contact(int(pt), xp, imp, stp, cp, float(dhat),
+                                  out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc == 0:
+        return None
+    return dict(dist=out[0], u=out[1], v=out[2], normal=out[3:6], coef=out[6:10],
+                gamma=out[10:14], state=out[14:18] > 0.5, affine=out[18:21],
+                u_at_x=out[21:24])
+
+
+def sym3_min_eig(a) -> float:
+    a, ap = _d(np.asarray(a).reshape(9))
+    return float(lib().oracle_sym3_min_eig(ap))
+
+
+def contact_sweeps(state, coef, gamma, normal, affine, mu, x, cw, eta=None,
+                   corr=None, sweeps=10):
+    """One ADMM contact update on explicit contacts; returns (p, eta, corr,
+    colours)."""
+    state = np.ascontiguousarray(state, dtype=np.int32).reshape(-1, 4)
+    nc = state.shape[0]
+    nv = np.asarray(x).size // 3
+    eta = np.zeros(3 * nv) if eta is None else np.array(eta, dtype=np.float64).reshape(-1)
+    corr = np.zeros(3 * nc) if corr is None else np.array(corr, dtype=np.float64).reshape(-1)
+    p = np.zeros(3 * nv)
+    col = np.zeros(nc, dtype=np.int32)
+    keep = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            for a in (coef, gamma, normal, affine, mu, x, cw)]
+    ptr = [a.ctypes.data_as(C.POINTER(C.c_double)) for a in keep]
+    lib().oracle_contact_sweeps(
+        nc, state.ctypes.data_as(C.POINTER(C.c_int)), ptr[0], ptr[1], ptr[2], ptr[3],
+        ptr[4], nv, ptr[5], ptr[6], eta.ctypes.data_as(C.POINTER(C.c_double)),
+        corr.ctypes.data_as(C.POINTER(C.c_double)), int(sweeps),
+        p.ctypes.data_as(C.POINTER(C.c_double)), col.ctypes.data_as(C.POINTER(C.c_int)))
+    return p, eta, corr.reshape(nc, 3), col

@@ -77,10 +77,12 @@ struct PcgResult {
   double rr0 = 0, rr = 0;
 };
 
-/// mas_cg_solver.cu:264-382 in double.
+/// mas_cg_solver.cu:264-382 in double.  criterion 0 stops on
+/// rr <= rel^2 rr0 (the reference), 1 on rr <= rel^2 |b|^2.
 PcgResult pcg_solve(const Bsr3& A, const std::vector<double>& diag,
                     const MasOracle& M, const double* b, double* x,
-                    double rel_tol, double abs_tol, int max_iter);
+                    double rel_tol, double abs_tol, int max_iter,
+                    int criterion = 0);
 
 /// Clamp the singular values of a 3x2 matrix (column-major y[6]) to
 /// [lo, hi] via a one-sided Jacobi SVD (cloth_admm_helper.cu:19-51).

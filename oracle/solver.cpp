@@ -251,7 +251,8 @@ void MasOracle::apply(const double* r, double* z) const {
 
 PcgResult pcg_solve(const Bsr3& A, const std::vector<double>& diag,
                     const MasOracle& M, const double* b, double* x,
-                    double rel_tol, double abs_tol, int max_iter) {
+                    double rel_tol, double abs_tol, int max_iter,
+                    int criterion) {
   const size_t N = 3 * static_cast<size_t>(A.n);
   std::vector<double> r(N), z(N), p(N), Ap(N);
   auto spmv = [&](const double* src, double* dst) {
@@ -275,6 +276,19 @@ PcgResult pcg_solve(const Bsr3& A, const std::vector<double>& diag,
     return res;
   }
   res.rr0 = rr;
+  // Reference of the relative test: rr0 (mas_cg_solver.cu:87-90), or |b|^2
+  // for the right-hand-side criterion, under which a warm start that already
+  // meets the tolerance takes no iteration.
+  double ref2 = rr;
+  if (criterion == 1) {
+    ref2 = 0.0;
+    for (size_t i = 0; i < N; ++i) ref2 += b[i] * b[i];
+    if (rr <= abs_tol * abs_tol || rr <= rel_tol * rel_tol * ref2) {
+      res.status = rr <= abs_tol * abs_tol ? 0 : 1;
+      res.rr = rr;
+      return res;
+    }
+  }
   int iter = 0;
   for (;;) {
     spmv(p.data(), Ap.data());
@@ -292,7 +306,7 @@ PcgResult pcg_solve(const Bsr3& A, const std::vector<double>& diag,
     int status = -1;
     if (iter >= max_iter) status = 2;
     else if (rr <= abs_tol * abs_tol) status = 0;
-    else if (rr <= rel_tol * rel_tol * res.rr0) status = 1;
+    else if (rr <= rel_tol * rel_tol * ref2) status = 1;
     if (status >= 0) {
       res.status = status;
       break;

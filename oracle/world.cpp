@@ -27,6 +27,7 @@ struct Params {
   double admm_abs = 1e-5, admm_rel = 1e-3;
   int pcg_max_iter = 1000, sweeps = 10, ccd_max_iter = 4096;
   double ccd_tol = 1e-6, ms_scale = 0.2, dcd_scale = 1.0;
+  int pcg_criterion = 0;
 };
 
 struct Coll {
@@ -358,28 +359,13 @@ void coulomb_solve(const double us[3], const double n[3], double mu,
   for (int d = 0; d < 3; ++d) out[d] = (1.0 - rad / utn) * ut[d];
 }
 
-namespace {
-
-// DCD at fixed time (dcd.cu:102-261).  Cloth colliders are refit with
-// t0 = t1, obstacles interpolate along the frame.
-bool make_collision(const World& w, bool pt, const Prim* A, const Prim* B,
-                    double time, double dhat, const std::vector<ObjInfo>& objs,
-                    CollisionRec* out) {
-  int gv[4];
-  if (pt) {
-    gv[0] = B->v[0];
-    gv[1] = A->v[0];
-    gv[2] = A->v[1];
-    gv[3] = A->v[2];
-  } else {
-    gv[0] = A->v[0];
-    gv[1] = A->v[1];
-    gv[2] = B->v[0];
-    gv[3] = B->v[1];
-  }
-  V3 x[4];
-  for (int k = 0; k < 4; ++k)
-    x[k] = lerp(&w.pos_t0[3 * gv[k]], &w.pos_t1[3 * gv[k]], time);
+// Closest-point record of one PT (gv = point, tri0..2) or EE (gv = a0, a1,
+// b0, b1) pair on explicit positions (dcd.cu:102-261): false when the
+// closest point lies outside the triangle, the edges are degenerate, the
+// distance is 0 or at least dhat, or no vertex can move.  friction and presc
+// are left for the caller.
+bool dcd_pair(bool pt, const int gv[4], const V3 x[4], const double inv[4],
+              double dhat, CollisionRec* out) {
   const double eps = 1e-6;
   double u, v;
   V3 disp;
@@ -403,11 +389,8 @@ bool make_collision(const World& w, bool pt, const Prim* A, const Prim* B,
   const double dist = std::sqrt(d2);
   if (dist >= dhat) return false;
   V3 n = (-1.0 / dist) * disp;
-  double inv[4], den = 0.0;
-  for (int k = 0; k < 4; ++k) {
-    inv[k] = inv_mass_of(w, gv[k]);
-    den += param[k] * param[k] * inv[k];
-  }
+  double den = 0.0;
+  for (int k = 0; k < 4; ++k) den += param[k] * param[k] * inv[k];
   if (den == 0.0) return false;
   CollisionRec r;
   r.type = pt ? 0 : 1;
@@ -417,15 +400,48 @@ bool make_collision(const World& w, bool pt, const Prim* A, const Prim* B,
     r.inv_mass[k] = inv[k];
     for (int d = 0; d < 3; ++d) r.x[k][d] = x[k][d];
   }
-  for (int d = 0; d < 3; ++d) r.normal[d] = n[d];
+  for (int d = 0; d < 3; ++d) {
+    r.normal[d] = n[d];
+    r.presc[d] = 0.0;
+  }
+  r.dhat = dhat;
+  *out = r;
+  return true;
+}
+
+namespace {
+
+// DCD at fixed time (dcd.cu:102-261).  Cloth colliders are refit with
+// t0 = t1, obstacles interpolate along the frame.
+bool make_collision(const World& w, bool pt, const Prim* A, const Prim* B,
+                    double time, double dhat, const std::vector<ObjInfo>& objs,
+                    CollisionRec* out) {
+  int gv[4];
+  if (pt) {
+    gv[0] = B->v[0];
+    gv[1] = A->v[0];
+    gv[2] = A->v[1];
+    gv[3] = A->v[2];
+  } else {
+    gv[0] = A->v[0];
+    gv[1] = A->v[1];
+    gv[2] = B->v[0];
+    gv[3] = B->v[1];
+  }
+  V3 x[4];
+  double inv[4];
+  for (int k = 0; k < 4; ++k) {
+    x[k] = lerp(&w.pos_t0[3 * gv[k]], &w.pos_t1[3 * gv[k]], time);
+    inv[k] = inv_mass_of(w, gv[k]);
+  }
+  CollisionRec r;
+  if (!dcd_pair(pt, gv, x, inv, dhat, &r)) return false;
   const float fa = objs[A->obj].friction, fb = objs[B->obj].friction;
   r.friction = std::sqrt(static_cast<double>(fa) * static_cast<double>(fb));
-  r.dhat = dhat;
-  for (int d = 0; d < 3; ++d) r.presc[d] = 0.0;
   // Prescribed obstacle displacement: coef_k (x_k(t1) - x_k(time)).
   for (int k = 0; k < 4; ++k) {
     if (gv[k] < w.nv_cloth) continue;
-    for (int d = 0; d < 3; ++d) r.presc[d] += coef[k] * (w.pos_t1[3 * gv[k] + d] - x[k][d]);
+    for (int d = 0; d < 3; ++d) r.presc[d] += r.coef[k] * (w.pos_t1[3 * gv[k] + d] - x[k][d]);
   }
   *out = r;
   return true;
@@ -450,7 +466,11 @@ std::array<int, 5> contact_key(const CollisionRec& r) {
 
 // ----------------------------------------------------------------- contacts
 
-static Contact build_contact(const World& w, const CollisionRec& r) {
+// Daviet contact of a DCD record (contact_constraints.cu:23-150): vertices
+// with id >= nv_cloth are static (their terms go to affine); cbase_w holds
+// the base contact weight per cloth dof.
+Contact contact_of(const CollisionRec& r, int nv_cloth,
+                   const std::vector<double>& cbase_w) {
   Contact c;
   c.mu = r.friction;
   double ref[3] = {0, 0, 0};
@@ -462,20 +482,20 @@ static Contact build_contact(const World& w, const CollisionRec& r) {
   for (int k = 0; k < 4; ++k) {
     c.coef[k] = r.coef[k];
     for (int d = 0; d < 3; ++d) ref[d] += r.coef[k] * r.x[k][d];
-    if (r.gv[k] >= w.nv_cloth) {
+    if (r.gv[k] >= nv_cloth) {
       c.state[k] = -1;
       for (int d = 0; d < 3; ++d) c.affine[d] += r.coef[k] * r.x[k][d];
       continue;
     }
     c.state[k] = r.gv[k];
-    if (r.inv_mass[k] > 0.0) scatter += r.coef[k] * r.coef[k] / w.cbase_w[3 * r.gv[k]];
+    if (r.inv_mass[k] > 0.0) scatter += r.coef[k] * r.coef[k] / cbase_w[3 * r.gv[k]];
   }
   for (int d = 0; d < 3; ++d) c.affine[d] += r.presc[d];
   const double rn = ref[0] * c.normal[0] + ref[1] * c.normal[1] + ref[2] * c.normal[2];
   for (int d = 0; d < 3; ++d) c.affine[d] -= ref[d] - rn * c.normal[d];
   for (int k = 0; k < 4; ++k) {
     if (c.state[k] >= 0 && r.inv_mass[k] > 0.0)
-      c.gamma[k] = c.coef[k] / (scatter * w.cbase_w[3 * c.state[k]]);
+      c.gamma[k] = c.coef[k] / (scatter * cbase_w[3 * c.state[k]]);
     else
       c.gamma[k] = 0.0;
   }
@@ -525,7 +545,7 @@ static int append_collisions(World& w, std::vector<CollisionRec> recs) {
   int added = 0;
   for (const auto& r : recs) {
     if (!w.keys.insert(contact_key(r)).second) continue;
-    w.contacts.push_back(build_contact(w, r));
+    w.contacts.push_back(contact_of(r, w.nv_cloth, w.cbase_w));
     ++added;
   }
   if (added > 0) recolor(w);
@@ -695,7 +715,7 @@ static bool admm_solve(World& w, std::vector<double>& X,
         }
         if (it == 1) c.mas.factorize(c.A, diag);
         auto res = pcg_solve(c.A, diag, c.mas, b.data(), xs.data(), rel,
-                             w.P.lin_abs, w.P.pcg_max_iter);
+                             w.P.lin_abs, w.P.pcg_max_iter, w.P.pcg_criterion);
         w.stats.pcg += res.iters;
         for (int i = 0; i < 3 * c.nv; ++i) X[3 * o + i] = xs[i];
       }
@@ -900,9 +920,9 @@ extern "C" {
 void* oracle_create() { return new World(); }
 void oracle_destroy(void* w) { delete static_cast<World*>(w); }
 
-/// p: 18 doubles: acc[3], max_outer, max_inner, dt, lin_abs, lin_rel_min,
+/// p: 19 doubles: acc[3], max_outer, max_inner, dt, lin_abs, lin_rel_min,
 /// lin_rel_max, lin_rel_init, admm_abs, admm_rel, pcg_max_iter, sweeps,
-/// ccd_max_iter, ccd_tol, ms_scale, dcd_scale.
+/// ccd_max_iter, ccd_tol, ms_scale, dcd_scale, pcg_criterion.
 void oracle_set_global(void* wp, const double* p) {
   auto& P = static_cast<World*>(wp)->P;
   P.acc[0] = p[0]; P.acc[1] = p[1]; P.acc[2] = p[2];
@@ -915,6 +935,7 @@ void oracle_set_global(void* wp, const double* p) {
   P.sweeps = static_cast<int>(p[13]);
   P.ccd_max_iter = static_cast<int>(p[14]);
   P.ccd_tol = p[15]; P.ms_scale = p[16]; P.dcd_scale = p[17];
+  P.pcg_criterion = static_cast<int>(p[18]);
 }
 
 /// cloth: k, kb, density, damping; coll: on, self, group, restitution,
@@ -1090,6 +1111,109 @@ void oracle_get_positions(void* wp, int idx, double* out) {
   const auto& c = w.cloths[idx];
   for (int i = 0; i < c.nv; ++i)
     for (int d = 0; d < 3; ++d) out[3 * i + d] = c.x[3 * c.perm[i] + d];
+}
+
+/// The DCD record and Daviet contact of one pair on explicit positions
+/// (x12: 4 vertices; PT: point, tri0..2; EE: a0, a1, b0, b1).  Vertices
+/// with is_static[k] have no state (their terms go to affine); inv_mass[k] = 0
+/// marks a pinned cloth vertex; cw4: base contact weight of each vertex.
+/// out (37): dist, u-param, v-param, normal[3], coef[4], gamma[4],
+/// state_valid[4], affine[3], and u = sum coef x + affine at x12 [3], then
+/// zeros.  Returns 0 when the pair makes no contact.
+int oracle_dcd_contact(int pt, const double* x12, const double* inv_mass4,
+                       const int* is_static4, const double* cw4, double dhat,
+                       double* out) {
+  using namespace oracle;
+  int gv[4];
+  V3 x[4];
+  double inv[4];
+  std::vector<double> cbase(3 * 8, 0.0);
+  for (int k = 0; k < 4; ++k) {
+    gv[k] = is_static4[k] ? 4 + k : k;
+    for (int d = 0; d < 3; ++d) x[k][d] = x12[3 * k + d];
+    inv[k] = is_static4[k] ? 0.0 : inv_mass4[k];
+    for (int d = 0; d < 3; ++d) cbase[3 * gv[k] + d] = cw4[k];
+  }
+  CollisionRec r;
+  if (!dcd_pair(pt != 0, gv, x, inv, dhat, &r)) return 0;
+  r.friction = 0.3;
+  Contact c = contact_of(r, 4, cbase);
+  std::fill(out, out + 37, 0.0);
+  V3 disp{0, 0, 0};
+  for (int k = 0; k < 4; ++k)
+    for (int d = 0; d < 3; ++d) disp[d] += r.coef[k] * x[k][d];
+  out[0] = std::sqrt(dot(disp, disp));
+  out[1] = pt ? -r.coef[2] : r.coef[1];
+  out[2] = pt ? -r.coef[3] : -r.coef[3];
+  for (int d = 0; d < 3; ++d) out[3 + d] = r.normal[d];
+  for (int k = 0; k < 4; ++k) {
+    out[6 + k] = c.coef[k];
+    out[10 + k] = c.gamma[k];
+    out[14 + k] = c.state[k] >= 0 ? 1.0 : 0.0;
+  }
+  for (int d = 0; d < 3; ++d) out[18 + d] = c.affine[d];
+  for (int d = 0; d < 3; ++d) {
+    double u = c.affine[d];
+    for (int k = 0; k < 4; ++k)
+      if (c.state[k] >= 0) u += c.coef[k] * x[k][d];
+    out[21 + d] = u;
+  }
+  return 1;
+}
+
+/// Smallest eigenvalue of a symmetric 3x3 matrix (the contact weight's eta,
+/// cloth_assembly_l1_cache.cu:60-78).
+double oracle_sym3_min_eig(const double* a9) { return oracle::sym3_min_eig(a9); }
+
+/// The contact solve of one ADMM iteration on explicit contacts
+/// (contact_constraints.cu:203-411 as the oracle step runs it): nc contacts
+/// over nv cloth vertices, state (4 per contact, -1 = static), coef, gamma
+/// (4), normal, affine (3), mu (1); x (3nv), base weight cw (nv; a vertex is
+/// active when some contact moves it), eta (3nv, in/out), corr (3nc,
+/// in/out).  Writes p (3nv) and the greedy colour of each contact (from 1).
+void oracle_contact_sweeps(int nc, const int* state, const double* coef,
+                           const double* gamma, const double* normal,
+                           const double* affine, const double* mu, int nv,
+                           const double* x, const double* cw, double* eta,
+                           double* corr, int sweeps, double* p_out,
+                           int* color_out) {
+  using namespace oracle;
+  World w;
+  w.P.sweeps = sweeps;
+  w.nv_cloth = nv;
+  const int N = 3 * nv;
+  w.cbase_w.assign(N, 0.0);
+  for (int i = 0; i < nv; ++i)
+    for (int d = 0; d < 3; ++d) w.cbase_w[3 * i + d] = cw[i];
+  w.cw.assign(N, 0.0);
+  w.p.assign(N, 0.0);
+  w.p_prev.assign(N, 0.0);
+  w.eta.assign(eta, eta + N);
+  for (int c = 0; c < nc; ++c) {
+    Contact k;
+    for (int j = 0; j < 4; ++j) {
+      k.state[j] = state[4 * c + j];
+      k.coef[j] = coef[4 * c + j];
+      k.gamma[j] = gamma[4 * c + j];
+    }
+    for (int d = 0; d < 3; ++d) {
+      k.normal[d] = normal[3 * c + d];
+      k.affine[d] = affine[3 * c + d];
+      k.corr[d] = corr[3 * c + d];
+    }
+    k.mu = mu[c];
+    w.contacts.push_back(k);
+  }
+  recolor(w);
+  std::vector<double> X(x, x + N), dual(N, 0.0), dcur(N, 0.0), dprev(N, 0.0);
+  double primal = 0.0;
+  contact_update(w, X, dual, dcur, dprev, &primal);
+  std::copy(w.p.begin(), w.p.end(), p_out);
+  std::copy(w.eta.begin(), w.eta.end(), eta);
+  for (int c = 0; c < nc; ++c)
+    for (int d = 0; d < 3; ++d) corr[3 * c + d] = w.contacts[c].corr[d];
+  for (size_t g = 0; g < w.color_groups.size(); ++g)
+    for (int c : w.color_groups[g]) color_out[c] = static_cast<int>(g) + 1;
 }
 
 void oracle_set_state(void* wp, int idx, const double* x, const double* v) {

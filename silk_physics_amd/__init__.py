@@ -89,6 +89,12 @@ def load_library():
 
 # ---------------------------------------------------------------- C structs
 
+# Values of GlobalConfig.mas_precision, .pcg_criterion and .warm_start
+# (include/silk_hip.h).
+MAS_FP32, MAS_FP16, MAS_BF16 = 0, 1, 2
+PCG_RR0, PCG_RHS = 0, 1
+WARM_PREVIOUS, WARM_LEARNED = 0, 1
+
 class SilkHipClothParams(C.Structure):
     _fields_ = [("elastic_stiffness", C.c_float), ("bending_stiffness", C.c_float),
                 ("density", C.c_float), ("damping", C.c_float)]
@@ -114,7 +120,8 @@ class SilkHipGlobal(C.Structure):
                 ("ccd_min_separation_scale", C.c_float),
                 ("dcd_activation_scale", C.c_float),
                 ("mas_precision", C.c_int32), ("true_residual_stats", C.c_int32),
-                ("reserved", C.c_int32 * 6)]
+                ("pcg_criterion", C.c_int32), ("warm_start", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
 
 
 class SilkHipStepStats(C.Structure):
@@ -123,7 +130,10 @@ class SilkHipStepStats(C.Structure):
                 ("contacts", C.c_int32), ("contact_colors", C.c_int32),
                 ("ccd_candidates", C.c_int32), ("dcd_candidates", C.c_int32),
                 ("min_toi", C.c_float), ("last_pcg_rel_residual", C.c_float),
-                ("max_true_rel_residual", C.c_float), ("reserved", C.c_int32 * 5)]
+                ("max_true_rel_residual", C.c_float),
+                ("max_committed_rel_residual", C.c_float),
+                ("pcg_max_solve_iterations", C.c_int32),
+                ("pcg_stalled_solves", C.c_int32), ("reserved", C.c_int32 * 2)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -215,8 +225,10 @@ class GlobalConfig:
     ccd_tolerance: float = 0.0
     ccd_min_separation_scale: float = 0.0
     dcd_activation_scale: float = 0.0
-    mas_precision: int = 0
+    mas_precision: int = 0        # MAS_FP32 / MAS_FP16 / MAS_BF16
     true_residual_stats: int = 0
+    pcg_criterion: int = 0        # PCG_RR0 (reference) / PCG_RHS
+    warm_start: int = 0           # WARM_PREVIOUS (reference) / WARM_LEARNED
 
     def _c(self) -> SilkHipGlobal:
         g = SilkHipGlobal()
@@ -229,7 +241,8 @@ class GlobalConfig:
                      "admm_abs_tol", "admm_rel_tol", "pcg_max_iter",
                      "contact_sweeps", "ccd_max_iter", "ccd_tolerance",
                      "ccd_min_separation_scale", "dcd_activation_scale",
-                     "mas_precision", "true_residual_stats"):
+                     "mas_precision", "true_residual_stats", "pcg_criterion",
+                     "warm_start"):
             setattr(g, name, getattr(self, name))
         return g
 

@@ -89,6 +89,20 @@ const char* silk_hip_last_error(const silk_hip_ctx* ctx) {
 int silk_hip_set_global(silk_hip_ctx* ctx, const silk_hip_global* g) {
   return guarded(ctx, [&](Context& c) {
     if (g == nullptr) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    const char* bad = nullptr;
+    if (g->mas_precision < SILK_HIP_MAS_FP32 || g->mas_precision > SILK_HIP_MAS_BF16)
+      bad = "mas_precision must be SILK_HIP_MAS_FP32, _FP16 or _BF16";
+    else if (g->pcg_criterion != SILK_HIP_PCG_RR0 && g->pcg_criterion != SILK_HIP_PCG_RHS)
+      bad = "pcg_criterion must be SILK_HIP_PCG_RR0 or SILK_HIP_PCG_RHS";
+    else if (g->warm_start != SILK_HIP_WARM_PREVIOUS && g->warm_start != SILK_HIP_WARM_LEARNED)
+      bad = "warm_start must be SILK_HIP_WARM_PREVIOUS or SILK_HIP_WARM_LEARNED";
+    else if (g->warm_start == SILK_HIP_WARM_LEARNED && !c.has_warm_net())
+      bad = "warm_start = SILK_HIP_WARM_LEARNED needs a network "
+            "(silk_hip_set_warm_start_net)";
+    if (bad != nullptr) {
+      c.last_error_ = bad;
+      return int{SILK_HIP_ERR_INVALID_CONFIG};
+    }
     c.global_ = *g;
     return int{SILK_HIP_OK};
   });

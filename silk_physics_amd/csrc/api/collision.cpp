@@ -21,6 +21,7 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <unordered_set>
 
@@ -205,9 +206,12 @@ class HipCollision final : public CollisionWorld {
   std::vector<int> color_;               // per contact, from 1
   std::vector<std::vector<int>> vinc_;   // per cloth vertex: its contacts
   std::vector<int> touched_;             // vertices with contacts
+  std::vector<int> cvert_;               // contact vertices, first use order
+  std::vector<int> vloc_;                // per cloth vertex: index in cvert_
   int max_color_ = 0;
   PinnedVec<ContactDev> hc_pin_;
   PinnedVec<int> off_pin_;
+  PinnedVec<int> act_pin_;
   PinnedVec<float> w_pin_;
   std::vector<int> color_off_h_, color_list_h_;
   int n_active_ = 0;
@@ -215,7 +219,10 @@ class HipCollision final : public CollisionWorld {
   DevBuf<int> color_off_;
   DevBuf<float> corr_;
   int corr_cap_ = 0;
-  DevBuf<float> w_, p_, p_prev_, eta_, dual_;
+  DevBuf<float> w_, p_, eta_, dual_;
+  DevBuf<int> act_;     // contact vertices (cvert_ on the device)
+  DevBuf<float> pc_;    // 3 per contact vertex: scratch of the global variant
+  int force_variant_ = -1;
 
   CollVerts verts() const;
   PrimTables tables() const;
@@ -337,7 +344,12 @@ void HipCollision::finish_setup(const std::vector<float>& pos0) {
   err6_.resize(6);
   const size_t nc3 = 3 * static_cast<size_t>(std::max(1, nvc_));
   w_.resize(std::max(1, nvc_));
-  for (DevBuf<float>* b : {&p_, &p_prev_, &eta_, &dual_}) b->resize(nc3);
+  for (DevBuf<float>* b : {&p_, &eta_, &dual_}) b->resize(nc3);
+  vloc_.assign(std::max(1, nvc_), -1);
+  // SILK_HIP_CONTACT_VARIANT forces a contact-update variant (0..3) where it
+  // fits, so that tests can run every one of them.
+  if (const char* e = std::getenv("SILK_HIP_CONTACT_VARIANT"))
+    force_variant_ = std::clamp(std::atoi(e), -1, 3);
   // Host vectors above are temporaries: drain the uploads before they go.
   hip_check(hipStreamSynchronize(st_), "collision setup");
 }
@@ -634,6 +646,23 @@ void HipCollision::recolor(int n_old) {
 
 void HipCollision::upload_contacts(int n_old) {
   const int n = static_cast<int>(contacts_.size());
+  // Contact vertices of the new contacts, numbered in order of first use.
+  for (int i = n_old; i < n; ++i)
+    for (int k = 0; k < 4; ++k) {
+      const int v = contacts_[i].state[k];
+      if (v >= 0 && vloc_[v] < 0) {
+        vloc_[v] = static_cast<int>(cvert_.size());
+        cvert_.push_back(v);
+      }
+    }
+  const int nact = static_cast<int>(cvert_.size());
+  int* act = act_pin_.reserve(std::max(1, nact));
+  std::copy(cvert_.begin(), cvert_.end(), act);
+  act_.reserve(std::max(1, nact));
+  hip_check(hipMemcpyAsync(act_.get(), act, nact * sizeof(int), hipMemcpyHostToDevice,
+                           st_),
+            "contact vertices");
+  pc_.reserve(3 * static_cast<size_t>(std::max(1, nact)));
   // Device copy in colour order (ids ascending inside a colour), so that a
   // colour is one contiguous range.
   ContactDev* hc = hc_pin_.reserve(std::max(1, n));
@@ -642,7 +671,7 @@ void HipCollision::upload_contacts(int n_old) {
     const HostContact& c = contacts_[i];
     ContactDev& d = hc[e];
     for (int k = 0; k < 4; ++k) {
-      d.state[k] = c.state[k];
+      d.state[k] = c.state[k] >= 0 ? vloc_[c.state[k]] : -1;
       // A static vertex's term is in affine; its coefficient is 0 here so
       // that the kernels add every term without a test.
       d.coef[k] = c.state[k] >= 0 ? static_cast<float>(c.coef[k]) : 0.0f;
@@ -719,11 +748,13 @@ void HipCollision::begin_step(const float* outer_x, float* ms, float* dhat,
     w_pin_.get()[v] = 0.0f;
   }
   touched_.clear();
+  for (int v : cvert_) vloc_[v] = -1;
+  cvert_.clear();
   max_color_ = 0;
   n_active_ = 0;
   const size_t nc3 = 3 * static_cast<size_t>(nvc_);
   hip_check(hipMemsetAsync(w_.get(), 0, nvc_ * sizeof(float), st_), "w reset");
-  for (DevBuf<float>* b : {&p_, &p_prev_, &eta_, &dual_})
+  for (DevBuf<float>* b : {&p_, &eta_, &dual_})
     hip_check(hipMemsetAsync(b->get(), 0, nc3 * sizeof(float), st_), "reset");
   set_cloth(outer_x, outer_x);
   refit();
@@ -795,15 +826,17 @@ void HipCollision::contact_update(const float* x, const int* skip) {
   ContactSet cs;
   cs.n = static_cast<int>(contacts_.size());
   cs.ncolor = static_cast<int>(color_off_h_.size()) - 1;
-  cs.nv = nvc_;
+  cs.nact = static_cast<int>(cvert_.size());
+  cs.act = act_.get();
   cs.c = c_dev_.get();
   cs.color_off = color_off_.get();
   cs.corr = corr_.get();
   cs.w = w_.get();
   cs.p = p_.get();
-  cs.p_prev = p_prev_.get();
   cs.eta = eta_.get();
   cs.dual = dual_.get();
+  cs.pc = pc_.get();
+  cs.force_variant = force_variant_;
   cs.sweeps = ctx_.contact_sweeps();
   cs.skip = skip;
   launch_contact_update(st_, cs, x);

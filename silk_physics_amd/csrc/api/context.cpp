@@ -684,6 +684,9 @@ void Context::build_scene() {
   S.bank_member.upload(bank_member, st);
   S.bank_static.upload(bank_static, st);
   S.bank_inv.resize(std::max<size_t>(1, bank_static.size()));
+  S.bank_inv_h.resize(std::max<size_t>(1, bank_static.size()));
+  S.bank_scale.resize(std::max<size_t>(1, bank_sys.size()));
+  S.x0_copy.resize(3 * static_cast<size_t>(nv));
   if (cnode_list.empty()) cnode_list.push_back(0);
   S.cnode_ptr.upload(cnode_ptr, st);
   S.cnode_list.upload(cnode_list, st);
@@ -700,6 +703,7 @@ void Context::build_scene() {
   S.cd.resize(std::max(1, n1));
   S.part.resize(2 * static_cast<size_t>(std::max(1, S.nblk)));
   S.part2.resize(2 * static_cast<size_t>(std::max(1, S.nblk)));
+  S.part3.resize(3 * static_cast<size_t>(std::max(1, S.nblk)));
   S.state.resize(2 * static_cast<size_t>(std::max(1, S.nsys)));
   // ---- persistent PCG
   {
@@ -767,6 +771,10 @@ void Context::build_scene() {
     a.cb_member = S.ps_cb_member.get();
     a.cb_rng = S.ps_cb_rng.get();
     a.bank_inv = S.bank_inv.get();
+    a.bank_inv_h = S.bank_inv_h.get();
+    a.bank_scale = S.bank_scale.get();
+    a.sys_blk0 = S.sys_blk0.get();
+    a.part2 = S.part2.get();
     a.d = S.d.get();
     a.r0 = S.r.get();
     a.x = S.inner_x.get();
@@ -858,7 +866,23 @@ PcgDev Context::pcg_dev() {
   p.state = S.state.get();
   p.abs_tol = global_.linear_solver_abs_tol;
   p.max_iter = pcg_max_iter();
+  p.criterion = global_.pcg_criterion;
+  p.mas_half = global_.mas_precision;
+  p.bank_inv_h = S.bank_inv_h.get();
+  p.bank_scale = S.bank_scale.get();
   return p;
+}
+
+// The persistent solve's arguments for the current global configuration.
+static PersistDev persist_args(Scene& S, const silk_hip_global& g,
+                               int max_iter) {
+  PersistDev a = S.pers;
+  a.abs_tol = g.linear_solver_abs_tol;
+  a.max_iter = max_iter;
+  a.criterion = g.pcg_criterion;
+  a.mas_half = g.mas_precision;
+  a.dx_out = g.true_residual_stats ? S.dx.get() : nullptr;
+  return a;
 }
 
 int Context::pcg_solve(float rel_tol, bool factor, const RhsArgs& ra,
@@ -867,10 +891,13 @@ int Context::pcg_solve(float rel_tol, bool factor, const RhsArgs& ra,
   PcgDev p = pcg_dev();
   launch_pcg_residual(stream_, p, ra);
   if (factor) launch_mas_factor(stream_, p, S.cd.get());
+  const size_t n3 = 3 * static_cast<size_t>(S.nv);
+  if (global_.true_residual_stats)
+    hip_check(hipMemcpyAsync(S.x0_copy.get(), S.inner_x.get(), n3 * sizeof(float),
+                             hipMemcpyDeviceToDevice, stream_),
+              "warm start copy");
   if (S.use_persistent) {
-    PersistDev a = S.pers;
-    a.abs_tol = global_.linear_solver_abs_tol;
-    a.max_iter = pcg_max_iter();
+    PersistDev a = persist_args(S, global_, pcg_max_iter());
     hip_check(launch_pcg_persistent(stream_, a, rel_tol), "persistent PCG launch");
     S.h_state.resize(S.nsys);
     int abort_flag = 0;
@@ -890,21 +917,26 @@ int Context::pcg_solve(float rel_tol, bool factor, const RhsArgs& ra,
       throw HipError("persistent PCG: grid barrier timed out");
     }
   } else {
-    pcg_iterate(rel_tol);
+    pcg_iterate(rel_tol, ra);
   }
   int iters = 0;
   double rel = 0.0;
   for (const auto& s : S.h_state) {
     iters += s.iter;
     if (s.rr0 > 0) rel = std::max(rel, std::sqrt(s.rr / s.rr0));
+    if (stats) {
+      stats->pcg_max_solve_iterations = std::max(stats->pcg_max_solve_iterations, s.iter);
+      stats->pcg_stalled_solves += s.status == 2 ? 1 : 0;
+    }
   }
   return finish_pcg(iters, rel, ra, stats);
 }
 
-void Context::pcg_iterate(float rel_tol) {
+void Context::pcg_iterate(float rel_tol, const RhsArgs& ra) {
+  (void)ra;
   Scene& S = *scene_;
   PcgDev p = pcg_dev();
-  launch_pcg_start(stream_, p);
+  launch_pcg_start(stream_, p, rel_tol);
   int k = 0;
   int chunk = 8;
   const int max_iter = pcg_max_iter();
@@ -934,19 +966,24 @@ int Context::finish_pcg(int iters, double rel, const RhsArgs& ra,
     stats->pcg_solves += 1;
     stats->last_pcg_rel_residual = static_cast<float>(rel);
     if (global_.true_residual_stats) {
-      launch_true_residual(stream_, p, ra);
-      download(S.h_part2, S.part2.get(), 2 * static_cast<size_t>(S.nblk), stream_);
-      hip_check(hipStreamSynchronize(stream_), "true residual");
-      for (int sys = 0; sys < S.nsys; ++sys) {
-        double rr = 0.0, bb = 0.0;
-        for (int b = S.h_blk0[sys]; b < S.h_blk0[sys + 1]; ++b) {
-          rr += S.h_part2[2 * b];
-          bb += S.h_part2[2 * b + 1];
-        }
-        if (bb > 0.0)
-          stats->max_true_rel_residual = std::max(
-              stats->max_true_rel_residual, static_cast<float>(std::sqrt(rr / bb)));
-      }
+      // x0 was copied before the solve; dx is the solve's correction.
+      AdmmCtl* ctl = S.admm_ctl.get();
+      hip_check(hipMemsetAsync(ctl, 0, sizeof(AdmmCtl), stream_), "ctl reset");
+      RhsArgs rv = ra;
+      rv.skip = nullptr;
+      launch_solve_residuals(stream_, p, rv, S.x0_copy.get(), S.dx.get(),
+                             S.inner_x.get(), S.part3.get(), ctl);
+      hip_check(hipMemcpyAsync(S.h_admm_ctl, ctl, sizeof(AdmmCtl),
+                               hipMemcpyDeviceToHost, stream_),
+                "solve residuals");
+      hip_check(hipStreamSynchronize(stream_), "solve residuals");
+      double it2, cm2;
+      std::memcpy(&it2, &S.h_admm_ctl->max_true_bits, sizeof(double));
+      std::memcpy(&cm2, &S.h_admm_ctl->max_commit_bits, sizeof(double));
+      stats->max_true_rel_residual =
+          std::max(stats->max_true_rel_residual, static_cast<float>(std::sqrt(it2)));
+      stats->max_committed_rel_residual = std::max(
+          stats->max_committed_rel_residual, static_cast<float>(std::sqrt(cm2)));
     }
   }
   return iters;
@@ -979,9 +1016,7 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
   ra.ceta = cv.eta;
   ra.skip = skip;
   PcgDev p = pcg_dev();
-  PersistDev pa = S.pers;
-  pa.abs_tol = g.linear_solver_abs_tol;
-  pa.max_iter = pcg_max_iter();
+  PersistDev pa = persist_args(S, g, pcg_max_iter());
   pa.ctl = ctl;
   AdmmCheckArgs ca;
   ca.grid = local_step_grid(S.nv);
@@ -998,10 +1033,17 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
       launch_pcg_residual(stream_, p, ra);
       timed_end();
       if (it == 1) launch_mas_factor(stream_, p, S.cd.get(), skip);
+      if (g.true_residual_stats)
+        hip_check(hipMemcpyAsync(S.x0_copy.get(), S.inner_x.get(), n3 * sizeof(float),
+                                 hipMemcpyDeviceToDevice, stream_),
+                  "warm start copy");
       timed_begin(kTimePcg);
       hip_check(launch_pcg_persistent(stream_, pa, 0.0f),
                 "persistent PCG launch");
       timed_end();
+      if (g.true_residual_stats)
+        launch_solve_residuals(stream_, p, ra, S.x0_copy.get(), S.dx.get(),
+                               S.inner_x.get(), S.part3.get(), ctl);
     }
     timed_begin(kTimeContact);
     if (S.coll) S.coll->contact_update(S.inner_x.get(), skip);
@@ -1049,14 +1091,25 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
     stats->pcg_iterations += hc.pcg_iters;
     stats->pcg_solves += hc.pcg_solves;
     if (hc.pcg_solves > 0) stats->last_pcg_rel_residual = hc.last_rel;
+    stats->pcg_max_solve_iterations =
+        std::max(stats->pcg_max_solve_iterations, hc.pcg_max_solve);
+    stats->pcg_stalled_solves += hc.pcg_stalled;
+    if (g.true_residual_stats) {
+      double it2, cm2;
+      std::memcpy(&it2, &hc.max_true_bits, sizeof(double));
+      std::memcpy(&cm2, &hc.max_commit_bits, sizeof(double));
+      stats->max_true_rel_residual =
+          std::max(stats->max_true_rel_residual, static_cast<float>(std::sqrt(it2)));
+      stats->max_committed_rel_residual = std::max(
+          stats->max_committed_rel_residual, static_cast<float>(std::sqrt(cm2)));
+    }
   }
   return hc.diverged == 0;
 }
 
 bool Context::admm_solve(silk_hip_step_stats* stats) {
   Scene& S = *scene_;
-  if (S.use_persistent && !global_.true_residual_stats)
-    return admm_solve_device(stats);
+  if (S.use_persistent) return admm_solve_device(stats);
   ClothDev c = cloth_dev();
   const size_t n3 = 3 * static_cast<size_t>(S.nv);
   hip_check(hipMemsetAsync(S.pin_lambda.get(), 0, n3 * sizeof(float), stream_),

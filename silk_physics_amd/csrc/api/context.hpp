@@ -88,8 +88,11 @@ struct Scene {
   DevBuf<int> blk_sys, blk_row0, blk_rows, sys_blk0, slice_off, slice_width,
       ell_col, bank_sys, bank_level, bank_member, cnode_ptr, cnode_list,
       cnode_level, row_cnode;
-  DevBuf<float> ell_val, bank_static, bank_inv, r, p, q, z_pcg, zl0, cr, cz,
-      cd;
+  DevBuf<float> ell_val, bank_static, bank_inv, bank_scale, r, p, q, z_pcg, zl0,
+      cr, cz, cd;
+  DevBuf<unsigned short> bank_inv_h;  // 16-bit bank inverses (mas_precision)
+  DevBuf<float> x0_copy;    // 3*nv: warm start of the last solve (stats)
+  DevBuf<double> part3;     // 3 per row block: solve residual partials
   DevBuf<double> part, part2;
   std::vector<double> h_part2;
   DevBuf<PcgState> state;
@@ -130,6 +133,10 @@ class Context {
   std::vector<ObstacleEntity> obstacles_;
   uint64_t rng_state_ = 0x5117C0DE2024ULL;
 
+  /// A learned warm-start network is loaded (silk_hip_set_warm_start_net).
+  bool has_warm_net() const { return warm_net_loaded_; }
+  bool warm_net_loaded_ = false;
+
   bool dev_ready_ = false;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev_[2] = {nullptr, nullptr};
@@ -163,7 +170,7 @@ class Context {
                 silk_hip_step_stats* st);
   /// Multi-launch PCG iterations (used when the persistent kernel cannot
   /// keep its grid resident); leaves per-system states in scene h_state.
-  void pcg_iterate(float rel_tol);
+  void pcg_iterate(float rel_tol, const RhsArgs& ra);
   int finish_pcg(int iters, double rel, const RhsArgs& ra,
                  silk_hip_step_stats* st);
   bool admm_solve(silk_hip_step_stats* st);

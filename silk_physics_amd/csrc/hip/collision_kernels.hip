@@ -1162,22 +1162,21 @@ __device__ __forceinline__ void warm_start(const ContactDev& c, float* P,
 
 template <bool kLds>
 __global__ void __launch_bounds__(kContactThreads)
-    k_contact_update(ContactSet cs, const float* __restrict__ x, int nv) {
+    k_contact_update(ContactSet cs, const float* __restrict__ x, int nact) {
   using namespace dev;
   extern __shared__ float sh[];
   if (cs.skip != nullptr && *cs.skip != 0) return;
-  float* P = kLds ? sh : cs.p;
-  float* C = kLds ? sh + 3 * static_cast<size_t>(nv) : cs.corr;
-  // q = x - eta / w for contact vertices, p = x elsewhere
-  // (contact_constraints.cu:203-214); the previous p stays in cs.p (LDS
-  // case) or is copied to p_prev for the dual residual.
-  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+  float* P = kLds ? sh : cs.pc;
+  float* C = kLds ? sh + 3 * static_cast<size_t>(nact) : cs.corr;
+  // q = x - eta / w for active contact vertices, x for the others
+  // (contact_constraints.cu:203-214); the previous p stays in cs.p for the
+  // dual residual.
+  for (int l = threadIdx.x; l < nact; l += blockDim.x) {
+    const int i = cs.act[l];
     const float w = cs.w[i];
     for (int d = 0; d < 3; ++d) {
       const int k = 3 * i + d;
-      const float q = w > 0.0f ? x[k] - cs.eta[k] / w : x[k];
-      if (!kLds) cs.p_prev[k] = cs.p[k];
-      P[k] = q;
+      P[3 * l + d] = w > 0.0f ? x[k] - cs.eta[k] / w : x[k];
     }
   }
   if (kLds)
@@ -1211,13 +1210,14 @@ __global__ void __launch_bounds__(kContactThreads)
     have = have_next;
   }
   // Multiplier update and dual residual terms (contact_constraints.cu:316-364).
-  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+  for (int l = threadIdx.x; l < nact; l += blockDim.x) {
+    const int i = cs.act[l];
     const float w = cs.w[i];
     for (int d = 0; d < 3; ++d) {
       const int k = 3 * i + d;
-      const float pn = P[k];
-      const float po = kLds ? cs.p[k] : cs.p_prev[k];
-      if (kLds) cs.p[k] = pn;
+      const float pn = P[3 * l + d];
+      const float po = cs.p[k];
+      cs.p[k] = pn;
       if (w > 0.0f) {
         cs.eta[k] += w * (pn - x[k]);
         cs.dual[k] = w * (pn - po);
@@ -1247,29 +1247,32 @@ __host__ __device__ constexpr size_t contact_nm_word(int nv, int ncolor) {
 
 template <bool kNmLds>
 __global__ void __launch_bounds__(kContactRegThreads)
-    k_contact_update_reg(ContactSet cs, const float* __restrict__ x, int nv) {
+    k_contact_update_reg(ContactSet cs, const float* __restrict__ x, int nact) {
   using namespace dev;
   extern __shared__ float sh[];
   if (cs.skip != nullptr && *cs.skip != 0) return;
   float* P = sh;
-  int* off = reinterpret_cast<int*>(sh + 3 * static_cast<size_t>(nv));  // ncolor + 1
-  float* NM = sh + contact_nm_word(nv, cs.ncolor);  // kNmLds: 4 per contact
+  int* off = reinterpret_cast<int*>(sh + 3 * static_cast<size_t>(nact));  // ncolor + 1
+  float* NM = sh + contact_nm_word(nact, cs.ncolor);  // kNmLds: 4 per contact
   const int tid = static_cast<int>(threadIdx.x);
-  const int n3 = 3 * nv;
-  // Per coordinate, with the loads of four steps in flight at once.
+  const int n3 = 3 * nact;
+  // Per coordinate of the contact vertices, with the loads of four steps in
+  // flight at once.
   for (int k0 = tid; k0 < n3; k0 += 4 * kContactRegThreads) {
     float xv[4], ev[4], wv[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int k = min(k0 + u * kContactRegThreads, n3 - 1);
+      const int kl = min(k0 + u * kContactRegThreads, n3 - 1);
+      const int i = cs.act[kl / 3];
+      const int k = 3 * i + kl % 3;
       xv[u] = x[k];
       ev[u] = cs.eta[k];
-      wv[u] = cs.w[k / 3];
+      wv[u] = cs.w[i];
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int k = k0 + u * kContactRegThreads;
-      if (k < n3) P[k] = wv[u] > 0.0f ? xv[u] - ev[u] / wv[u] : xv[u];
+      const int kl = k0 + u * kContactRegThreads;
+      if (kl < n3) P[kl] = wv[u] > 0.0f ? xv[u] - ev[u] / wv[u] : xv[u];
     }
   }
   for (int i = tid; i <= cs.ncolor; i += kContactRegThreads) off[i] = cs.color_off[i];
@@ -1334,19 +1337,23 @@ __global__ void __launch_bounds__(kContactRegThreads)
   // per coordinate with four steps' loads in flight.
   for (int k0 = tid; k0 < n3; k0 += 4 * kContactRegThreads) {
     float wv[4], po[4], xv[4], ev[4];
+    int kg[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int k = min(k0 + u * kContactRegThreads, n3 - 1);
-      wv[u] = cs.w[k / 3];
-      po[u] = cs.p[k];
-      xv[u] = x[k];
-      ev[u] = cs.eta[k];
+      const int kl = min(k0 + u * kContactRegThreads, n3 - 1);
+      const int i = cs.act[kl / 3];
+      kg[u] = 3 * i + kl % 3;
+      wv[u] = cs.w[i];
+      po[u] = cs.p[kg[u]];
+      xv[u] = x[kg[u]];
+      ev[u] = cs.eta[kg[u]];
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int k = k0 + u * kContactRegThreads;
-      if (k >= n3) continue;
-      const float pn = P[k];
+      const int kl = k0 + u * kContactRegThreads;
+      if (kl >= n3) continue;
+      const int k = kg[u];
+      const float pn = P[kl];
       cs.p[k] = pn;
       if (wv[u] > 0.0f) {
         cs.eta[k] = ev[u] + wv[u] * (pn - xv[u]);
@@ -1452,17 +1459,51 @@ void launch_ccd_min(hipStream_t s, int n, const int* is_vf, const double* p0,
   k_ccd<<<blocks, 256, 0, s>>>(in, MinToiOut{toi_bits}, tolerance, max_iter);
 }
 
-bool contact_update_fits_lds(int nv, int n) {
-  return 12 * (static_cast<size_t>(nv) + static_cast<size_t>(n)) <= kContactLdsBytes;
+namespace {
+
+size_t contact_p_bytes(int nact, int ncolor) {
+  return 12 * static_cast<size_t>(nact) + 4 * (static_cast<size_t>(ncolor) + 1);
+}
+size_t contact_nm_bytes(int nact, int n, int ncolor) {
+  return 4 * contact_nm_word(nact, ncolor) + 16 * static_cast<size_t>(n);
+}
+size_t contact_lds_bytes(int nact, int n) {
+  return 12 * (static_cast<size_t>(nact) + static_cast<size_t>(n));
+}
+bool contact_variant_fits(int v, int nact, int n, int ncolor) {
+  switch (v) {
+    case kContactRegNm:
+      return n <= kContactSlots * kContactRegThreads &&
+             contact_nm_bytes(nact, n, ncolor) <= kContactLdsBytes;
+    case kContactReg:
+      return n <= kContactSlots * kContactRegThreads &&
+             contact_p_bytes(nact, ncolor) <= kContactLdsBytes;
+    case kContactLds:
+      return contact_lds_bytes(nact, n) <= kContactLdsBytes;
+    default:
+      return true;
+  }
+}
+
+}  // namespace
+
+int contact_update_variant(int nact, int n, int ncolor) {
+  for (int v = kContactRegNm; v < kContactGlobal; ++v)
+    if (contact_variant_fits(v, nact, n, ncolor)) return v;
+  return kContactGlobal;
 }
 
 void launch_contact_update(hipStream_t s, const ContactSet& cs,
                            const float* x) {
   if (cs.n == 0 || cs.ncolor == 0) return;
-  const size_t p_bytes = 12 * static_cast<size_t>(cs.nv) + 4 * (static_cast<size_t>(cs.ncolor) + 1);
-  const size_t nm_bytes =
-      4 * contact_nm_word(cs.nv, cs.ncolor) + 16 * static_cast<size_t>(cs.n);
-  if (cs.n <= kContactSlots * kContactRegThreads && nm_bytes <= kContactLdsBytes) {
+  int variant = contact_update_variant(cs.nact, cs.n, cs.ncolor);
+  // A forced variant is taken only where it fits (the global one always does).
+  if (cs.force_variant >= 0 &&
+      contact_variant_fits(cs.force_variant, cs.nact, cs.n, cs.ncolor))
+    variant = cs.force_variant;
+  const size_t p_bytes = contact_p_bytes(cs.nact, cs.ncolor);
+  const size_t nm_bytes = contact_nm_bytes(cs.nact, cs.n, cs.ncolor);
+  if (variant == kContactRegNm) {
     static bool attr = false;
     if (!attr) {
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_contact_update_reg<true>),
@@ -1470,8 +1511,8 @@ void launch_contact_update(hipStream_t s, const ContactSet& cs,
                                 static_cast<int>(kContactLdsBytes));
       attr = true;
     }
-    k_contact_update_reg<true><<<1, kContactRegThreads, nm_bytes, s>>>(cs, x, cs.nv);
-  } else if (cs.n <= kContactSlots * kContactRegThreads && p_bytes <= kContactLdsBytes) {
+    k_contact_update_reg<true><<<1, kContactRegThreads, nm_bytes, s>>>(cs, x, cs.nact);
+  } else if (variant == kContactReg) {
     static bool attr = false;
     if (!attr) {
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_contact_update_reg<false>),
@@ -1479,9 +1520,9 @@ void launch_contact_update(hipStream_t s, const ContactSet& cs,
                                 static_cast<int>(kContactLdsBytes));
       attr = true;
     }
-    k_contact_update_reg<false><<<1, kContactRegThreads, p_bytes, s>>>(cs, x, cs.nv);
-  } else if (contact_update_fits_lds(cs.nv, cs.n)) {
-    const size_t shmem = 12 * (static_cast<size_t>(cs.nv) + cs.n);
+    k_contact_update_reg<false><<<1, kContactRegThreads, p_bytes, s>>>(cs, x, cs.nact);
+  } else if (variant == kContactLds) {
+    const size_t shmem = contact_lds_bytes(cs.nact, cs.n);
     static bool attr = false;
     if (!attr) {
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_contact_update<true>),
@@ -1489,9 +1530,9 @@ void launch_contact_update(hipStream_t s, const ContactSet& cs,
                                 static_cast<int>(kContactLdsBytes));
       attr = true;
     }
-    k_contact_update<true><<<1, kContactThreads, shmem, s>>>(cs, x, cs.nv);
+    k_contact_update<true><<<1, kContactThreads, shmem, s>>>(cs, x, cs.nact);
   } else {
-    k_contact_update<false><<<1, kContactThreads, 0, s>>>(cs, x, cs.nv);
+    k_contact_update<false><<<1, kContactThreads, 0, s>>>(cs, x, cs.nact);
   }
 }
 

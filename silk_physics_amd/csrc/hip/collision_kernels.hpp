@@ -170,7 +170,7 @@ void launch_coulomb_each(hipStream_t s, int n, const float* u, const float* nrm,
 
 /// Daviet contact (contact_constraints.cuh:19-42).
 struct ContactDev {
-  int state[4];     // cloth vertex (solver order, global) or -1
+  int state[4];     // contact vertex (index into ContactSet::act) or -1
   float coef[4];    // 0 where state is -1 (that term is in affine)
   float gamma[4];
   float normal[3];
@@ -182,17 +182,22 @@ struct ContactDev {
 struct ContactSet {
   int n = 0;
   int ncolor = 0;
-  int nv = 0;                       // cloth vertices
+  // Contact vertices: the cloth vertices some contact references (movable
+  // or not), in order of first use; the update works on these only.
+  int nact = 0;
+  const int* act = nullptr;         // nact cloth vertex ids (solver order)
   const ContactDev* c = nullptr;    // stored colour by colour
   const int* color_off = nullptr;   // ncolor + 1 offsets into c
   float* corr = nullptr;            // 3 per contact
-  const float* w = nullptr;         // nv, active weight (0 = inactive)
-  float* p = nullptr;               // 3 nv
-  float* p_prev = nullptr;
+  const float* w = nullptr;         // per cloth vertex: active weight (0 = inactive)
+  float* p = nullptr;               // 3 per cloth vertex (read where w > 0)
   float* eta = nullptr;
-  float* dual = nullptr;            // 3 nv, w (p - p_prev)
+  float* dual = nullptr;            // w (p - p_previous)
+  float* pc = nullptr;              // 3 per contact vertex: scratch of the
+                                    //   variant that keeps p out of LDS
   int sweeps = 10;
   const int* skip = nullptr;        // no-op when *skip != 0
+  int force_variant = -1;           // >= 0: use that variant (tests)
 };
 
 /// One ADMM contact update (contact_constraints.cu:246-264): single
@@ -201,7 +206,16 @@ struct ContactSet {
 void launch_contact_update(hipStream_t s, const ContactSet& cs,
                            const float* x);
 
-/// Largest (vertices, contacts) pair whose p and corrections fit in LDS.
-bool contact_update_fits_lds(int nv, int n);
+/// Variants of launch_contact_update, in the order it prefers them.
+enum : int {
+  kContactRegNm = 0,   // contacts in registers, normal and mu in LDS
+  kContactReg = 1,     // contacts in registers
+  kContactLds = 2,     // p and corrections in LDS
+  kContactGlobal = 3,  // p and corrections in global memory
+};
+
+/// The variant launch_contact_update takes for nact contact vertices, n
+/// contacts in ncolor colours.
+int contact_update_variant(int nact, int n, int ncolor);
 
 }  // namespace silk::amd

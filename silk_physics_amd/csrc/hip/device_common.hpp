@@ -162,6 +162,25 @@ __device__ __forceinline__ void sum64_pair(double& a, double& b) {
   b += xor32_d(b);
 }
 
+// ---- 16-bit storage of MAS bank inverses.  kind 1: IEEE binary16 of the
+// entry divided by the bank's scale (its largest magnitude), so that the
+// entries sit in binary16's normal range whatever the stiffness; kind 2:
+// bfloat16 (FP32 exponent range, scale 1).  Both round to nearest even.
+
+__device__ __forceinline__ unsigned short float_to_half_bits(float v, int kind) {
+  if (kind == 2) {
+    unsigned u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<unsigned short>(u >> 16);
+  }
+  return __builtin_bit_cast(unsigned short, static_cast<_Float16>(v));
+}
+
+__device__ __forceinline__ float half_bits_to_float(unsigned short h, int kind) {
+  if (kind == 2) return __uint_as_float(static_cast<unsigned>(h) << 16);
+  return static_cast<float>(__builtin_bit_cast(_Float16, h));
+}
+
 /// Sum of `v` over the workgroup in FP64, valid in thread 0.  blockDim.x must
 /// be a multiple of 64 and at most 1024.  `lds` needs 16 doubles.
 __device__ __forceinline__ double block_sum(double v, double* lds) {

@@ -85,10 +85,16 @@ struct AdmmCtl {
   int admm_iters = 0;
   int pcg_iters = 0;
   int pcg_solves = 0;
+  int pcg_max_solve = 0;  // largest iteration count of one solve (atomicMax)
+  int pcg_stalled = 0;    // solves that ended at max_iter
   int pad = 0;
   float rel_tol = 0;  // linear tolerance of the next x-update
   float last_rel = 0; // sqrt(rr / rr0) of the last PCG solve
   double init_primal = 0, init_dual = 0, ratio = 1;
+  // Largest |b - A x|^2 / |b|^2 over the solves (bits of a non-negative
+  // double, so that an integer atomicMax orders them): the PCG iterate
+  // x0 + dx and the stored FP32 x.
+  unsigned long long max_true_bits = 0, max_commit_bits = 0;
 };
 
 struct AdmmCheckArgs {
@@ -152,7 +158,7 @@ struct PcgDev {
   const int* bank_level = nullptr;     // 0 = fine, >0 coarse
   const int* bank_member = nullptr;    // 32 per bank: row (L0) or node (coarse), -1 pad
   const float* bank_static = nullptr;  // 1024 per bank
-  float* bank_inv = nullptr;           // 1024 per bank (fp32) or fp16 view
+  float* bank_inv = nullptr;           // 1024 per bank (fp32)
   int ncnode = 0;                      // coarse nodes, all levels and systems
   const int* cnode_ptr = nullptr;      // ncnode+1 : members (rows for level 1,
   const int* cnode_list = nullptr;     //   level-1 nodes for deeper levels)
@@ -175,6 +181,11 @@ struct PcgDev {
   PcgState* state = nullptr; // 2*nsys
   float abs_tol = 0.0f;
   int max_iter = 1000;
+  int criterion = 0;         // 0: rr <= rel^2 rr0; 1: rr <= rel^2 |b|^2
+  int mas_half = 0;          // bank inverses read from bank_inv_h (0 = fp32,
+                             // 1 = fp16, 2 = bf16), written by the factor
+  unsigned short* bank_inv_h = nullptr;  // 1024 per bank
+  float* bank_scale = nullptr;           // per bank (fp16: entries / scale)
 };
 
 /// Factorize: bank matrices = static + aggregated diag, inverted in place
@@ -204,12 +215,21 @@ struct RhsArgs {
 void launch_pcg_residual(hipStream_t s, const PcgDev& p, const RhsArgs& ra);
 
 /// PCG init, part 2 (mas_cg_solver.cu:279-296): z = M r0, p = z, rz0/rr0,
-/// dx = 0.
-void launch_pcg_start(hipStream_t s, const PcgDev& p);
+/// dx = 0; with criterion 1 a system whose r0 already meets rel_tol * |b|
+/// is done at iteration 0.
+void launch_pcg_start(hipStream_t s, const PcgDev& p, float rel_tol);
 
 /// FP64 true residual b - (K + D) x of the committed solution: per-block
 /// partials (|r|^2, |b|^2) in p.part2; r written to p.q.
 void launch_true_residual(hipStream_t s, const PcgDev& p, const RhsArgs& ra);
+
+/// FP64 residuals of a finished solve (b from ra): the iterate x0 + dx and
+/// the stored xc, relative to |b|, per system; their squares are folded into
+/// ctl->max_true_bits / max_commit_bits.  part3: 3 doubles per row block.
+/// No-op when *ra.skip != 0.
+void launch_solve_residuals(hipStream_t s, const PcgDev& p, const RhsArgs& ra,
+                            const float* x0, const float* dx, const float* xc,
+                            double* part3, AdmmCtl* ctl);
 
 /// x0 += dx.
 void launch_pcg_commit(hipStream_t s, const PcgDev& p);
@@ -275,6 +295,9 @@ struct PersistDev {
   const int* cb_rng = nullptr;      // 2 per coarse bank member: its nodes'
                                     //   positions [begin, end) in the list
   const float* bank_inv = nullptr;  // 1024 per bank
+  const unsigned short* bank_inv_h = nullptr;  // 1024 per bank, half/bf16
+  const float* bank_scale = nullptr;  // per bank
+  int mas_half = 0;                 // 0 fp32 (bank_inv), 1 fp16, 2 bf16
   const float* d = nullptr;         // diag per row
   const float* r0 = nullptr;        // FP32 initial residual per row
   float* x = nullptr;               // x += dx at the end (rows)
@@ -288,8 +311,13 @@ struct PersistDev {
   int* abort_flag = nullptr;
   PcgState* out = nullptr;          // per system
   AdmmCtl* ctl = nullptr;           // optional: skip flag, tolerance, counters
+  const int* sys_blk0 = nullptr;    // per system: first row block (nsys+1)
+  const double* part2 = nullptr;    // per row block: (|r0|^2, |b|^2) from the
+                                    //   residual kernel
+  float* dx_out = nullptr;          // optional: the solve's dx (rows)
   float abs_tol = 0.0f;
   int max_iter = 1000;
+  int criterion = 0;                // 0: rr <= rel^2 rr0; 1: rr <= rel^2 |b|^2
   int need_max = 0;     // max level-1 nodes listed for one workgroup (LDS)
   int ncn_max = 0;      // max coarse nodes of a system (LDS)
   int ncb_max = 0;      // max coarse banks of a system (LDS)

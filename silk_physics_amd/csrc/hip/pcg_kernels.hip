@@ -76,6 +76,53 @@ __device__ __forceinline__ double spmv_row(const PcgDev& p, int b,
 // (|r|^2, |b|^2) in part2.  r and b are formed in double from float inputs,
 // so the cancellation between b and A x (both O(1), their difference small)
 // does not lose the FP32 mantissa.
+// Right-hand side and diagonal of one row of the x-update system in double.
+__device__ __forceinline__ void rhs_row(const RhsArgs& ra, int row, double rhs[3],
+                                        double& dd) {
+  const ClothDev& c = ra.c;
+  const double h = c.h;
+  const double m = c.mass[row];
+  dd = m / (h * h);
+  for (int k = 0; k < 3; ++k) {
+    // -m * inertia_mod, inertia_mod = -x/h^2 - v/h - a (admm_solver.cu:141-151)
+    const double px = ra.prev_x[3 * row + k], pv = ra.prev_v[3 * row + k];
+    rhs[k] = ra.elastic_rhs[3 * row + k] + c.bend_rhs[3 * row + k] +
+             m * (px / (h * h) + pv / h + static_cast<double>(c.acc[k]));
+  }
+  if (c.pinned[row]) {
+    dd += ra.pin_rho;
+    for (int k = 0; k < 3; ++k)
+      rhs[k] += static_cast<double>(ra.pin_rho) * c.pin_target[3 * row + k] -
+                static_cast<double>(ra.pin_lambda[3 * row + k]);
+  }
+  if (ra.cw != nullptr) {
+    const double w = ra.cw[row];
+    if (w > 0.0) {
+      dd += w;
+      for (int k = 0; k < 3; ++k)
+        rhs[k] += w * ra.cp[3 * row + k] + static_cast<double>(ra.ceta[3 * row + k]);
+    }
+  }
+}
+
+// K x of one row in double, difference form (see spmv_row).
+__device__ __forceinline__ void kx_row(const PcgDev& p, int b, int row,
+                                       const float* __restrict__ x, double ax[3]) {
+  const int g = b * (kBlock / kSlice) + threadIdx.x / kSlice;
+  const int lane = threadIdx.x % kSlice;
+  const int off = p.slice_off[g];
+  const int width = p.slice_width[g];
+  const double xi[3] = {x[3 * row], x[3 * row + 1], x[3 * row + 2]};
+  ax[0] = ax[1] = ax[2] = 0.0;
+  for (int k = 0; k < width; ++k) {
+    const int col = p.ell_col[off + k * kSlice + lane];
+    const double a = p.ell_val[off + k * kSlice + lane];
+    ax[0] += a * (static_cast<double>(x[3 * col]) - xi[0]);
+    ax[1] += a * (static_cast<double>(x[3 * col + 1]) - xi[1]);
+    ax[2] += a * (static_cast<double>(x[3 * col + 2]) - xi[2]);
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_residual(PcgDev p, RhsArgs ra,
                                                      const float* __restrict__ x,
                                                      float* __restrict__ r_out,
@@ -87,46 +134,12 @@ __global__ void __launch_bounds__(kBlock) k_residual(PcgDev p, RhsArgs ra,
   const bool valid = threadIdx.x < p.blk_rows[b];
   double rr = 0.0, bb = 0.0;
   if (valid) {
-    const ClothDev& c = ra.c;
-    const double h = c.h;
-    const double m = c.mass[row];
-    double dd = m / (h * h);
-    double rhs[3];
-    for (int k = 0; k < 3; ++k) {
-      // -m * inertia_mod, inertia_mod = -x/h^2 - v/h - a (admm_solver.cu:141-151)
-      const double px = ra.prev_x[3 * row + k], pv = ra.prev_v[3 * row + k];
-      rhs[k] = ra.elastic_rhs[3 * row + k] + c.bend_rhs[3 * row + k] +
-               m * (px / (h * h) + pv / h + static_cast<double>(c.acc[k]));
-    }
-    if (c.pinned[row]) {
-      dd += ra.pin_rho;
-      for (int k = 0; k < 3; ++k)
-        rhs[k] += static_cast<double>(ra.pin_rho) * c.pin_target[3 * row + k] -
-                  static_cast<double>(ra.pin_lambda[3 * row + k]);
-    }
-    if (ra.cw != nullptr) {
-      const double w = ra.cw[row];
-      if (w > 0.0) {
-        dd += w;
-        for (int k = 0; k < 3; ++k)
-          rhs[k] += w * ra.cp[3 * row + k] + static_cast<double>(ra.ceta[3 * row + k]);
-      }
-    }
+    double rhs[3], dd;
+    rhs_row(ra, row, rhs, dd);
     if (write_d) p.d[row] = static_cast<float>(dd);
-    const int g = b * (kBlock / kSlice) + threadIdx.x / kSlice;
-    const int lane = threadIdx.x % kSlice;
-    const int off = p.slice_off[g];
-    const int width = p.slice_width[g];
     // Difference form (see spmv_row): the zero row sum of K is exact.
-    const double xi[3] = {x[3 * row], x[3 * row + 1], x[3 * row + 2]};
-    double ax[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < width; ++k) {
-      const int col = p.ell_col[off + k * kSlice + lane];
-      const double a = p.ell_val[off + k * kSlice + lane];
-      ax[0] += a * (static_cast<double>(x[3 * col]) - xi[0]);
-      ax[1] += a * (static_cast<double>(x[3 * col + 1]) - xi[1]);
-      ax[2] += a * (static_cast<double>(x[3 * col + 2]) - xi[2]);
-    }
+    double ax[3];
+    kx_row(p, b, row, x, ax);
     for (int k = 0; k < 3; ++k) {
       const double res = rhs[k] - (ax[k] + dd * x[3 * row + k]);
       r_out[3 * row + k] = static_cast<float>(res);
@@ -139,6 +152,65 @@ __global__ void __launch_bounds__(kBlock) k_residual(PcgDev p, RhsArgs ra,
   if (threadIdx.x == 0) {
     p.part2[2 * b] = s0;
     p.part2[2 * b + 1] = s1;
+  }
+}
+
+// FP64 residuals of a finished solve, per row block: |b - A (x0 + dx)|^2
+// (the iterate, x0 and dx applied separately so that their FP32 sum is never
+// rounded), |b - A xc|^2 (the stored x) and |b|^2.
+__global__ void __launch_bounds__(kBlock) k_solve_residuals(
+    PcgDev p, RhsArgs ra, const float* __restrict__ x0,
+    const float* __restrict__ dx, const float* __restrict__ xc,
+    double* __restrict__ part3) {
+  __shared__ double lds[17];
+  if (ra.skip != nullptr && *ra.skip != 0) return;
+  const int b = blockIdx.x;
+  const int row = p.blk_row0[b] + threadIdx.x;
+  const bool valid = threadIdx.x < p.blk_rows[b];
+  double ri = 0.0, rc = 0.0, bb = 0.0;
+  if (valid) {
+    double rhs[3], dd;
+    rhs_row(ra, row, rhs, dd);
+    double a0[3], a1[3], a2[3];
+    kx_row(p, b, row, x0, a0);
+    kx_row(p, b, row, dx, a1);
+    kx_row(p, b, row, xc, a2);
+    for (int k = 0; k < 3; ++k) {
+      const double xi = x0[3 * row + k], di = dx[3 * row + k], ci = xc[3 * row + k];
+      const double e_it = rhs[k] - (a0[k] + dd * xi) - (a1[k] + dd * di);
+      const double e_c = rhs[k] - (a2[k] + dd * ci);
+      ri += e_it * e_it;
+      rc += e_c * e_c;
+      bb += rhs[k] * rhs[k];
+    }
+  }
+  const double s0 = block_sum(ri, lds);
+  const double s1 = block_sum(rc, lds);
+  const double s2 = block_sum(bb, lds);
+  if (threadIdx.x == 0) {
+    part3[3 * b] = s0;
+    part3[3 * b + 1] = s1;
+    part3[3 * b + 2] = s2;
+  }
+}
+
+// Per system: the ratios of the block sums, folded into ctl's maxima.
+__global__ void __launch_bounds__(kBlock) k_solve_residual_max(
+    PcgDev p, const double* __restrict__ part3, const int* skip, AdmmCtl* ctl) {
+  __shared__ double lds[17];
+  if (skip != nullptr && *skip != 0) return;
+  const int sys = blockIdx.x;
+  const int b0 = p.sys_blk0[sys], b1 = p.sys_blk0[sys + 1];
+  double s[3];
+  for (int c = 0; c < 3; ++c) {
+    double v = 0.0;
+    for (int q = b0 + threadIdx.x; q < b1; q += blockDim.x) v += part3[3 * q + c];
+    s[c] = block_sum(v, lds);
+  }
+  if (threadIdx.x == 0 && s[2] > 0.0) {
+    const double it = s[0] / s[2], cm = s[1] / s[2];
+    atomicMax(&ctl->max_true_bits, static_cast<unsigned long long>(__double_as_longlong(it)));
+    atomicMax(&ctl->max_commit_bits, static_cast<unsigned long long>(__double_as_longlong(cm)));
   }
 }
 
@@ -270,11 +342,25 @@ __global__ void __launch_bounds__(64) k_mas_factor(PcgDev p,
   }
   __syncthreads();
   float* out = p.bank_inv + static_cast<size_t>(bank) * 1024;
+  float amax = 0.0f;
   for (int e = threadIdx.x; e < 1024; e += 64) {
     // Symmetrize: the exact inverse is symmetric.
     const int i = e >> 5, j = e & 31;
-    out[e] = 0.5f * (A[i][j] + A[j][i]);
+    const float v = 0.5f * (A[i][j] + A[j][i]);
+    out[e] = v;
+    amax = fmaxf(amax, fabsf(v));
   }
+  if (p.mas_half == 0) return;
+  // 16-bit copy of the inverse (the PCG kernels read it instead): binary16
+  // scaled by the bank's largest entry, bfloat16 unscaled.
+  for (int off = kWave / 2; off > 0; off >>= 1)
+    amax = fmaxf(amax, __shfl_xor(amax, off, kWave));
+  const float scale = p.mas_half == 1 && amax > 0.0f ? amax : 1.0f;
+  const float inv_scale = 1.0f / scale;
+  if (threadIdx.x == 0) p.bank_scale[bank] = scale;
+  unsigned short* oh = p.bank_inv_h + static_cast<size_t>(bank) * 1024;
+  for (int e = threadIdx.x; e < 1024; e += 64)
+    oh[e] = float_to_half_bits(out[e] * inv_scale, p.mas_half);
 }
 
 // ---- MAS bank solve: z_bank = inv(A_bank) r_bank (32 x 3).
@@ -290,9 +376,13 @@ __global__ void __launch_bounds__(64) k_bank_solve(PcgDev p,
   if (threadIdx.x >= 32) return;
   const int row = threadIdx.x;
   const float* inv = p.bank_inv + static_cast<size_t>(bank) * 1024;
+  const unsigned short* invh = p.bank_inv_h + static_cast<size_t>(bank) * 1024;
+  const float scale = p.mas_half != 0 ? p.bank_scale[bank] : 1.0f;
   float zx = 0.0f, zy = 0.0f, zz = 0.0f;
   for (int k = 0; k < kBankSlots; ++k) {
-    const float a = inv[k * kBankSlots + row];
+    const float a = p.mas_half != 0
+                        ? scale * half_bits_to_float(invh[k * kBankSlots + row], p.mas_half)
+                        : inv[k * kBankSlots + row];
     zx += a * rb[k][0];
     zy += a * rb[k][1];
     zz += a * rb[k][2];
@@ -337,7 +427,7 @@ __global__ void __launch_bounds__(kBlock) k_gather_z(PcgDev p,
 }
 
 // ---- init finalize: rz0, rr0, p = z
-__global__ void __launch_bounds__(kBlock) k_init_final(PcgDev p) {
+__global__ void __launch_bounds__(kBlock) k_init_final(PcgDev p, float rel_tol) {
   __shared__ double lds[17];
   const int b = blockIdx.x;
   const int sys = p.blk_sys[b];
@@ -365,6 +455,14 @@ __global__ void __launch_bounds__(kBlock) k_init_final(PcgDev p) {
     const bool ok = isfinite(rz);
     s.done = ok ? 0 : 1;
     s.status = ok ? -1 : 3;
+    // Right-hand-side criterion: a warm start that already meets it takes no
+    // iteration.
+    const double rel2 = static_cast<double>(rel_tol) * rel_tol;
+    const double abs2 = static_cast<double>(p.abs_tol) * p.abs_tol;
+    if (ok && p.criterion == 1 && (rr <= abs2 || rr <= rel2 * bb)) {
+      s.done = 1;
+      s.status = rr <= abs2 ? 0 : 1;
+    }
     s.pad = 0;
     p.state[sys] = s;  // parity 0
   }
@@ -425,7 +523,7 @@ __global__ void __launch_bounds__(kBlock) k_it_final(PcgDev p, int k,
   int status = -1;
   if (iter >= p.max_iter) status = 2;
   else if (rr <= abs2) status = 0;
-  else if (rr <= rel2 * s.rr0) status = 1;
+  else if (rr <= rel2 * (p.criterion == 1 ? s.bb : s.rr0)) status = 1;
   const double beta_d = (fabs(s.rz) < 1e-20) ? 0.0 : rz / s.rz;
   const float beta = static_cast<float>(beta_d);
   const int row = p.blk_row0[b] + threadIdx.x;
@@ -468,15 +566,23 @@ void launch_pcg_residual(hipStream_t s, const PcgDev& p, const RhsArgs& ra) {
   k_residual<<<p.nblk, kBlock, 0, s>>>(p, ra, p.x, p.r, 1);
 }
 
-void launch_pcg_start(hipStream_t s, const PcgDev& p) {
+void launch_pcg_start(hipStream_t s, const PcgDev& p, float rel_tol) {
   if (p.nblk == 0) return;
   mas_apply(s, p, p.r, nullptr);
-  k_init_final<<<p.nblk, kBlock, 0, s>>>(p);
+  k_init_final<<<p.nblk, kBlock, 0, s>>>(p, rel_tol);
 }
 
 void launch_true_residual(hipStream_t s, const PcgDev& p, const RhsArgs& ra) {
   if (p.nblk == 0) return;
   k_residual<<<p.nblk, kBlock, 0, s>>>(p, ra, p.x, p.q, 0);
+}
+
+void launch_solve_residuals(hipStream_t s, const PcgDev& p, const RhsArgs& ra,
+                            const float* x0, const float* dx, const float* xc,
+                            double* part3, AdmmCtl* ctl) {
+  if (p.nblk == 0) return;
+  k_solve_residuals<<<p.nblk, kBlock, 0, s>>>(p, ra, x0, dx, xc, part3);
+  k_solve_residual_max<<<p.nsys, kBlock, 0, s>>>(p, part3, ra.skip, ctl);
 }
 
 void launch_pcg_commit(hipStream_t s, const PcgDev& p) {

@@ -352,12 +352,18 @@ __global__ void __launch_bounds__(kPcgThreads)
   // the wavefront's coarse bank inverse.
   float winv[kBankSlots];
   const int row16 = s_in_bank & 16;
-  const size_t cinv = has_bank ? static_cast<size_t>(a.cb_bank[S.cb0 + cbk]) * 1024 : 0;
+  // Entry e of bank bk's inverse (16-bit storage when mas_half is set).
+  auto inv_at = [&](int bk, size_t e) -> float {
+    const size_t i = static_cast<size_t>(bk) * 1024 + e;
+    return a.mas_half != 0
+               ? a.bank_scale[bk] * half_bits_to_float(a.bank_inv_h[i], a.mas_half)
+               : a.bank_inv[i];
+  };
+  const int cbank = has_bank ? a.cb_bank[S.cb0 + cbk] : 0;
 #pragma unroll
   for (int k = 0; k < kBankSlots; ++k)
-    winv[k] = in_bank ? a.bank_inv[static_cast<size_t>(S.bank0 + lb) * 1024 +
-                                   (k ^ row16) * kBankSlots + s_in_bank]
-              : has_bank && k < 16 ? a.bank_inv[cinv + (kh + k) * kBankSlots + i_out]
+    winv[k] = in_bank ? inv_at(S.bank0 + lb, (k ^ row16) * kBankSlots + s_in_bank)
+              : has_bank && k < 16 ? inv_at(cbank, (kh + k) * kBankSlots + i_out)
                                    : 0.0f;
   int e_off = 0, e_w = 0, e_lane = 0;
   if (own) {
@@ -621,7 +627,23 @@ __global__ void __launch_bounds__(kPcgThreads)
   const double rr0 = rr;
   const double abs2 = static_cast<double>(a.abs_tol) * a.abs_tol;
   const double rel2 = static_cast<double>(rel_tol) * rel_tol;
+  // Reference of the relative test: rr0, or |b|^2 summed from the residual
+  // kernel's block partials (every wavefront adds the same values in the
+  // same order, so all take the same decisions).
+  double ref2 = rr0;
+  if (a.criterion == 1) {
+    const int q0 = a.sys_blk0[sys], q1 = a.sys_blk0[sys + 1];
+    double v = 0.0;
+    for (int q = q0 + lane; q < q1; q += kWave) v += a.part2[2 * q + 1];
+    ref2 = wave_sum(v);
+  }
   int status = isfinite(rz) ? -1 : 3;
+  // Right-hand-side criterion: a warm start that already meets it takes no
+  // iteration.
+  if (status < 0 && a.criterion == 1) {
+    if (rr <= abs2) status = 0;
+    else if (rr <= rel2 * ref2) status = 1;
+  }
   int iter = 0;
   float beta = 0.0f;
 
@@ -693,11 +715,14 @@ __global__ void __launch_bounds__(kPcgThreads)
     ++iter;
     if (iter >= a.max_iter) status = 2;
     else if (rr <= abs2) status = 0;
-    else if (rr <= rel2 * rr0) status = 1;
+    else if (rr <= rel2 * ref2) status = 1;
     beta = static_cast<float>(fabs(rz) < 1e-20 ? 0.0 : rz_new / rz);
     rz = rz_new;
   }
-  if (own) st3(a.x, row, ld3(a.x, row) + dx);
+  if (own) {
+    st3(a.x, row, ld3(a.x, row) + dx);
+    if (a.dx_out != nullptr) st3(a.dx_out, row, dx);
+  }
   if (j == 0 && t == 0) {
     __hip_atomic_store(epoch_ctr, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     PcgState o;
@@ -712,6 +737,8 @@ __global__ void __launch_bounds__(kPcgThreads)
     a.out[sys] = o;
     if (a.ctl != nullptr) {
       atomicAdd(&a.ctl->pcg_iters, iter);
+      atomicMax(&a.ctl->pcg_max_solve, iter);
+      if (status == 2) atomicAdd(&a.ctl->pcg_stalled, 1);
       if (sys == 0) {
         atomicAdd(&a.ctl->pcg_solves, 1);
         a.ctl->last_rel = static_cast<float>(rr0 > 0.0 ? sqrt(rr / rr0) : 0.0);

@@ -34,7 +34,8 @@ def oracle_for(global_kw: dict) -> dict:
         lin_rel_min=f32(g.linear_solver_rel_tol_min),
         lin_rel_max=f32(g.linear_solver_rel_tol_max),
         lin_rel_init=f32(g.initial_linear_rel_tol), admm_abs=f32(g.admm_abs_tol),
-        admm_rel=f32(g.admm_rel_tol), ms_scale=f32(0.2), ccd_tol=f32(1e-6))
+        admm_rel=f32(g.admm_rel_tol), ms_scale=f32(0.2), ccd_tol=f32(1e-6),
+        pcg_criterion=g.pcg_criterion)
 
 
 def coll_tuple(c: "sp.CollisionConfig") -> tuple:

@@ -135,3 +135,23 @@ def test_world_api_host_contract():
     out = os.popen(f"{exe} host").read()
     assert "FAIL" not in out, out
     assert out.count("PASS") >= 5, out
+
+
+def test_set_global_rejects_unknown_solver_modes(ctx):
+    """mas_precision, pcg_criterion and warm_start accept only their declared
+    values; a learned warm start needs a loaded network."""
+    lib, h = ctx
+    ok = sp.GlobalConfig()._c()
+    assert lib.silk_hip_set_global(h, C.byref(ok)) == 0
+    for field, value in [("mas_precision", 3), ("mas_precision", -1),
+                         ("pcg_criterion", 2), ("warm_start", 5),
+                         ("warm_start", sp.WARM_LEARNED)]:
+        g = sp.GlobalConfig()._c()
+        setattr(g, field, value)
+        assert lib.silk_hip_set_global(h, C.byref(g)) == sp.ErrorCode.InvalidConfig + 1, field
+        assert lib.silk_hip_last_error(h)
+    for field, value in [("mas_precision", sp.MAS_FP16), ("mas_precision", sp.MAS_BF16),
+                         ("pcg_criterion", sp.PCG_RHS)]:
+        g = sp.GlobalConfig()._c()
+        setattr(g, field, value)
+        assert lib.silk_hip_set_global(h, C.byref(g)) == 0, field
