@@ -141,7 +141,9 @@ typedef struct {
   float max_committed_rel_residual; /* the same for the stored FP32 x */
   int32_t pcg_max_solve_iterations; /* largest iteration count of one solve */
   int32_t pcg_stalled_solves;       /* solves that ended at pcg_max_iter */
-  int32_t reserved[2];
+  int32_t warm_start_kept;   /* solves that started from the network's
+                                prediction (SILK_HIP_WARM_LEARNED) */
+  int32_t warm_start_solves; /* solves the network was evaluated for */
 } silk_hip_step_stats;
 
 /* Number of visible HIP devices (0 when none).  Never fails. */
@@ -227,7 +229,8 @@ int silk_hip_get_partition(const silk_hip_ctx* ctx, uint32_t handle,
  * HIP events around each launch on the library stream: 1 = PCG solve kernel,
  * 2 = local projection, 3 = contact solve, 4 = broadphase, 5 = CCD,
  * 6 = FP64 residual; 7 = host wall time of contact assembly and colouring
- * (per call: sort, assembly, colouring, upload enqueue; no device time). */
+ * (per call: sort, assembly, colouring, upload enqueue; no device time);
+ * 8 = learned warm start (features and network). */
 int silk_hip_get_timing(silk_hip_ctx* ctx, int32_t cls, double* ms,
                         int64_t* launches, int32_t reset);
 
@@ -278,11 +281,67 @@ int silk_hip_debug_ccd_min(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
                            const double* err, double ms, double tolerance,
                            int32_t max_iter, double* min_toi);
 
+/* One PCG solve of the scene's x-update operator K + diag(d) on an explicit
+ * right-hand side, from x = 0, with the current global configuration
+ * (criterion, tolerances, mas_precision): d (nv floats) and b (3 nv) in solver
+ * order (cloths concatenated, each in the order of silk_hip_get_partition);
+ * rel_tol replaces the adaptive tolerance.  x (3 nv) receives the solution,
+ * *iterations the iterations summed over the cloths' systems. */
+int silk_hip_debug_pcg(silk_hip_ctx* ctx, const float* d, int64_t nv,
+                       const float* b, int64_t n3, float rel_tol, float* x,
+                       int32_t* iterations);
+
 /* The contact solve's Coulomb projection (replaces solve_coulomb_contact,
  * contact_constraints.cu:391-411, as the contact kernels run it) on n
  * explicit cases: u and normal hold 3 floats per case, mu 1, out 3. */
 int silk_hip_debug_coulomb(silk_hip_ctx* ctx, int64_t n, const float* u,
                            const float* normal, const float* mu, float* out);
+
+/* ---- Learned CG warm start ----
+ * No counterpart in the reference (it starts every CG solve from the previous
+ * ADMM iterate, mas_cg_solver.cu:193-311).  For callers that solve with
+ * SILK_HIP_PCG_RHS, where a better start takes fewer iterations.  A per-vertex
+ * network (16 features of the vertex and its operator 1-ring -> 64 -> 64 -> 1,
+ * SiLU, BF16 MFMA with FP32 accumulation) predicts each x-update's
+ * correction; the prediction is used only when its FP64 residual is below the
+ * previous iterate's, and PCG then runs to the same stopping test. */
+
+/* Parameters of the network (FP32, in this order): W1[64][16], b1[64],
+ * W2[64][64], b2[64], W3[64], b3. */
+#define SILK_HIP_WARM_NET_PARAMS 5313
+
+/* Load the network; n must equal SILK_HIP_WARM_NET_PARAMS. */
+int silk_hip_set_warm_start_net(silk_hip_ctx* ctx, const float* params,
+                                int64_t n);
+
+/* Read the network back; SILK_HIP_ERR_NEED_INIT_SOLVER when none is loaded. */
+int silk_hip_get_warm_start_net(silk_hip_ctx* ctx, float* params, int64_t n);
+
+/* Record training samples during the following silk_hip_step calls: each
+ * (vertex, coordinate) of each x-update solve (started from the previous
+ * iterate) is kept with probability `rate`, up to `capacity` samples held in
+ * device memory; earlier samples are dropped.  capacity 0 stops recording
+ * and frees them. */
+int silk_hip_warm_start_record(silk_hip_ctx* ctx, int64_t capacity, float rate);
+
+/* Number of samples recorded so far. */
+int silk_hip_warm_start_samples(silk_hip_ctx* ctx, int64_t* n);
+
+/* Train on the recorded samples on the device: `steps` Adam steps with
+ * learning rate lr on batches of `batch` samples drawn by a hash of (step,
+ * index); starts from the loaded network, or from a seeded initialisation
+ * when none is loaded.  loss (optional, `steps` floats) receives the batch
+ * losses (mean squared error of the correction / sigma). */
+int silk_hip_warm_start_train(silk_hip_ctx* ctx, int32_t steps, int32_t batch,
+                              float lr, uint64_t seed, float* loss);
+
+/* Forward and backward of the loaded network on n explicit samples (16
+ * features each, rounded to BF16 on the device) with targets: y (n), the
+ * gradient of the mean squared error (SILK_HIP_WARM_NET_PARAMS) and the
+ * loss.  For gradient checks. */
+int silk_hip_debug_warm_net(silk_hip_ctx* ctx, int64_t n, const float* feat,
+                            const float* target, float* y, float* grad,
+                            double* loss);
 
 #ifdef __cplusplus
 }

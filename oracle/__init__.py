@@ -53,7 +53,7 @@ def lib():
                                         C.c_double, i, C.c_int, d, d, d, i]),
             "oracle_pcg": (None, [d, C.c_int, i, C.c_int, C.c_double, C.c_double, i,
                                   C.c_int, d, d, d, C.c_double, C.c_double,
-                                  C.c_int, d]),
+                                  C.c_int, d, C.c_int]),
             "oracle_overlap_pairs": (C.c_int, [f, C.c_int, f, C.c_int, C.c_int, i,
                                                C.c_int]),
             "oracle_ccd": (C.c_int, [C.c_int, d, d, C.c_double, C.c_double,
@@ -259,7 +259,8 @@ def mas_apply(V, F, k, kb, offsets, diag, r):
     return z, lv.value
 
 
-def pcg(V, F, k, kb, offsets, diag, b, x0, rel_tol, abs_tol, max_iter=1000):
+def pcg(V, F, k, kb, offsets, diag, b, x0, rel_tol, abs_tol, max_iter=1000,
+        criterion=0):
     v, vp = _d(V)
     f, fp = _i(F)
     o, op = _i(offsets)
@@ -269,7 +270,8 @@ def pcg(V, F, k, kb, offsets, diag, b, x0, rel_tol, abs_tol, max_iter=1000):
     out = np.zeros(4)
     lib().oracle_pcg(vp, v.size // 3, fp, f.size // 3, k, kb, op, o.size - 1, dgp,
                      bp, x.ctypes.data_as(C.POINTER(C.c_double)), rel_tol, abs_tol,
-                     max_iter, out.ctypes.data_as(C.POINTER(C.c_double)))
+                     max_iter, out.ctypes.data_as(C.POINTER(C.c_double)),
+                     int(criterion))
     return x, {"iters": int(out[0]), "status": int(out[1]), "rr0": out[2],
                "rr": out[3]}
 

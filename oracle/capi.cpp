@@ -125,7 +125,7 @@ void oracle_mas_apply(const double* V, int nv, const int* F, int nf, double k,
 void oracle_pcg(const double* V, int nv, const int* F, int nf, double k,
                 double kb, const int* offsets, int nparts, const double* diag,
                 const double* b, double* x, double rel_tol, double abs_tol,
-                int max_iter, double* out) {
+                int max_iter, double* out, int criterion) {
   std::vector<double> Vv(V, V + 3 * nv);
   std::vector<int> Fv(F, F + 3 * nf);
   L2Cache l2 = build_l2(Vv, Fv);
@@ -134,7 +134,7 @@ void oracle_pcg(const double* V, int nv, const int* F, int nf, double k,
   M.build_topology(A, std::vector<int>(offsets, offsets + nparts + 1));
   std::vector<double> dg(diag, diag + 3 * nv);
   M.factorize(A, dg);
-  PcgResult r = pcg_solve(A, dg, M, b, x, rel_tol, abs_tol, max_iter);
+  PcgResult r = pcg_solve(A, dg, M, b, x, rel_tol, abs_tol, max_iter, criterion);
   out[0] = r.iters;
   out[1] = r.status;
   out[2] = r.rr0;

@@ -78,6 +78,13 @@ def load_library():
                                              C.c_void_p, C.c_void_p]),
         "silk_hip_debug_ccd_min": (C.c_int, [vp, i64, ip, dp, dp, dp, C.c_double,
                                              C.c_double, i32, dp]),
+        "silk_hip_debug_pcg": (C.c_int, [vp, fp, i64, fp, i64, C.c_float, fp, ip]),
+        "silk_hip_set_warm_start_net": (C.c_int, [vp, fp, i64]),
+        "silk_hip_get_warm_start_net": (C.c_int, [vp, fp, i64]),
+        "silk_hip_warm_start_record": (C.c_int, [vp, i64, C.c_float]),
+        "silk_hip_warm_start_samples": (C.c_int, [vp, C.POINTER(i64)]),
+        "silk_hip_warm_start_train": (C.c_int, [vp, i32, i32, C.c_float, C.c_uint64, fp]),
+        "silk_hip_debug_warm_net": (C.c_int, [vp, i64, fp, fp, fp, fp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -94,6 +101,7 @@ def load_library():
 MAS_FP32, MAS_FP16, MAS_BF16 = 0, 1, 2
 PCG_RR0, PCG_RHS = 0, 1
 WARM_PREVIOUS, WARM_LEARNED = 0, 1
+WARM_NET_PARAMS = 5313  # SILK_HIP_WARM_NET_PARAMS
 
 class SilkHipClothParams(C.Structure):
     _fields_ = [("elastic_stiffness", C.c_float), ("bending_stiffness", C.c_float),
@@ -133,7 +141,8 @@ class SilkHipStepStats(C.Structure):
                 ("max_true_rel_residual", C.c_float),
                 ("max_committed_rel_residual", C.c_float),
                 ("pcg_max_solve_iterations", C.c_int32),
-                ("pcg_stalled_solves", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("pcg_stalled_solves", C.c_int32), ("warm_start_kept", C.c_int32),
+                ("warm_start_solves", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -531,6 +540,59 @@ class World:
         if st != 0:
             raise RuntimeError(self._wrap(st).to_string())
         return out.reshape(n, 3)
+
+    # -- learned warm start (library extension, see include/silk_hip.h)
+    def set_warm_start_net(self, params) -> Result:
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        return self._wrap(load_library().silk_hip_set_warm_start_net(
+            self._ctx, _fptr(p), p.size))
+
+    def get_warm_start_net(self) -> tuple[Result, np.ndarray]:
+        p = np.zeros(WARM_NET_PARAMS, dtype=np.float32)
+        r = self._wrap(load_library().silk_hip_get_warm_start_net(
+            self._ctx, _fptr(p), p.size))
+        return r, p
+
+    def warm_start_record(self, capacity: int, rate: float) -> Result:
+        return self._wrap(load_library().silk_hip_warm_start_record(
+            self._ctx, int(capacity), float(rate)))
+
+    def warm_start_samples(self) -> int:
+        n = C.c_int64(0)
+        load_library().silk_hip_warm_start_samples(self._ctx, C.byref(n))
+        return n.value
+
+    def warm_start_train(self, steps: int, batch: int, lr: float,
+                         seed: int = 0) -> tuple[Result, np.ndarray]:
+        loss = np.zeros(max(1, steps), dtype=np.float32)
+        r = self._wrap(load_library().silk_hip_warm_start_train(
+            self._ctx, int(steps), int(batch), float(lr), int(seed), _fptr(loss)))
+        return r, loss[:steps]
+
+    def debug_pcg(self, d, b, rel_tol: float):
+        """One PCG solve of K + diag(d) on b from zero (solver order):
+        returns (x, iterations)."""
+        dd = np.ascontiguousarray(d, dtype=np.float32).reshape(-1)
+        bb = np.ascontiguousarray(b, dtype=np.float32).reshape(-1)
+        x = np.zeros_like(bb)
+        it = C.c_int32(0)
+        r = self._wrap(load_library().silk_hip_debug_pcg(
+            self._ctx, _fptr(dd), dd.size, _fptr(bb), bb.size, float(rel_tol),
+            _fptr(x), C.byref(it)))
+        assert r, r.to_string()
+        return x, it.value
+
+    def debug_warm_net(self, feat, target):
+        f = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1, 16)
+        t = np.ascontiguousarray(target, dtype=np.float32).reshape(-1)
+        y = np.zeros(t.size, dtype=np.float32)
+        g = np.zeros(WARM_NET_PARAMS, dtype=np.float32)
+        loss = C.c_double(0.0)
+        r = self._wrap(load_library().silk_hip_debug_warm_net(
+            self._ctx, t.size, _fptr(f), _fptr(t), _fptr(y), _fptr(g),
+            C.byref(loss)))
+        assert r, r.to_string()
+        return y, g, loss.value
 
     def timing(self, cls: int, reset: bool = False) -> tuple[float, int]:
         ms = C.c_double(0)

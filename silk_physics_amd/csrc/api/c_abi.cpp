@@ -1,6 +1,7 @@
 // extern "C" entry points of include/silk_hip.h.  Every function converts
 // exceptions into status codes; nothing unwinds across the ABI.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
@@ -337,7 +338,7 @@ int silk_hip_get_partition(const silk_hip_ctx* cctx, uint32_t handle,
 int silk_hip_get_timing(silk_hip_ctx* ctx, int32_t cls, double* ms,
                         int64_t* launches, int32_t reset) {
   return guarded(ctx, [&](Context& c) {
-    if (cls < 0 || cls >= 8) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    if (cls < 0 || cls >= silk::amd::kTimeClasses) return int{SILK_HIP_ERR_INVALID_CONFIG};
     if (ms) *ms = c.timing_ms_[cls];
     if (launches) *launches = c.timing_launch_[cls];
     if (reset) {
@@ -511,6 +512,88 @@ int silk_hip_debug_coulomb(silk_hip_ctx* ctx, int64_t n, const float* u,
                                         hipMemcpyDeviceToHost, c.stream_),
                          "coulomb out");
     silk::amd::hip_check(hipStreamSynchronize(c.stream_), "coulomb sync");
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_debug_pcg(silk_hip_ctx* ctx, const float* d, int64_t nv,
+                       const float* b, int64_t n3, float rel_tol, float* x,
+                       int32_t* iterations) {
+  return guarded(ctx, [&](Context& c) {
+    if (!d || !b || !x || !iterations || !(rel_tol >= 0.0f))
+      return int{SILK_HIP_ERR_INVALID_CONFIG};
+    if (c.cloths_.empty()) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    int64_t total = 0;
+    for (const auto& e : c.cloths_) total += e.mesh.nv();
+    if (nv != total || n3 != 3 * total) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    *iterations = c.debug_pcg(d, b, rel_tol, x);
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_set_warm_start_net(silk_hip_ctx* ctx, const float* params, int64_t n) {
+  return guarded(ctx, [&](Context& c) {
+    if (params == nullptr || n != SILK_HIP_WARM_NET_PARAMS)
+      return int{SILK_HIP_ERR_INVALID_CONFIG};
+    for (int64_t i = 0; i < n; ++i)
+      if (!std::isfinite(params[i])) {
+        c.last_error_ = "non-finite network parameter";
+        return int{SILK_HIP_ERR_INVALID_CONFIG};
+      }
+    c.warm_set(params);
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_get_warm_start_net(silk_hip_ctx* ctx, float* params, int64_t n) {
+  return guarded(ctx, [&](Context& c) {
+    if (params == nullptr || n != SILK_HIP_WARM_NET_PARAMS)
+      return int{SILK_HIP_ERR_INVALID_CONFIG};
+    if (!c.has_warm_net()) return int{SILK_HIP_ERR_NEED_INIT_SOLVER};
+    c.warm_get(params);
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_warm_start_record(silk_hip_ctx* ctx, int64_t capacity, float rate) {
+  return guarded(ctx, [&](Context& c) {
+    if (capacity < 0 || capacity > (int64_t{1} << 31) || !(rate >= 0.0f && rate <= 1.0f))
+      return int{SILK_HIP_ERR_INVALID_CONFIG};
+    c.warm_record(capacity, rate);
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_warm_start_samples(silk_hip_ctx* ctx, int64_t* n) {
+  return guarded(ctx, [&](Context& c) {
+    if (n == nullptr) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    *n = c.recording() ? c.warm_samples() : 0;
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_warm_start_train(silk_hip_ctx* ctx, int32_t steps, int32_t batch,
+                              float lr, uint64_t seed, float* loss) {
+  return guarded(ctx, [&](Context& c) {
+    if (steps < 0 || batch <= 0 || batch > (1 << 24) || !(lr > 0.0f))
+      return int{SILK_HIP_ERR_INVALID_CONFIG};
+    if (!c.recording() || c.warm_samples() == 0) {
+      c.last_error_ = "no recorded samples (silk_hip_warm_start_record)";
+      return int{SILK_HIP_ERR_NEED_INIT_SOLVER};
+    }
+    c.warm_train(steps, batch, lr, seed, loss);
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_debug_warm_net(silk_hip_ctx* ctx, int64_t n, const float* feat,
+                            const float* target, float* y, float* grad,
+                            double* loss) {
+  return guarded(ctx, [&](Context& c) {
+    if (n <= 0 || n > (1 << 22) || !feat || !target || !y || !grad || !loss)
+      return int{SILK_HIP_ERR_INVALID_CONFIG};
+    if (!c.has_warm_net()) return int{SILK_HIP_ERR_NEED_INIT_SOLVER};
+    c.warm_debug(n, feat, target, y, grad, loss);
     return int{SILK_HIP_OK};
   });
 }

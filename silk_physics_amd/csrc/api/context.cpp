@@ -1018,6 +1018,33 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
   PcgDev p = pcg_dev();
   PersistDev pa = persist_args(S, g, pcg_max_iter());
   pa.ctl = ctl;
+  // Learned warm start, or recording of training samples (from solves that
+  // start at the previous iterate); both need each solve's correction.
+  const bool learned = g.warm_start == SILK_HIP_WARM_LEARNED && has_warm_net();
+  const bool record = recording() && !learned;
+  if (learned || record) {
+    pa.dx_out = S.dx.get();
+    const size_t ns = n3;
+    if (S.wfeat.size() < ns * kMlpIn) {
+      S.wfeat.resize(ns * kMlpIn);
+      S.wsig.resize(ns);
+      S.wdelta.resize(ns);
+      S.w_xprev.resize(ns);
+      S.w_rprev.resize(ns);
+      S.w_part2.resize(2 * static_cast<size_t>(std::max(1, S.nblk)));
+    }
+    // The first solve of a segment has no previous correction.
+    hip_check(hipMemsetAsync(S.dx.get(), 0, n3 * sizeof(float), stream_), "dx reset");
+  }
+  WarmFeatArgs wf;
+  wf.r = S.r.get();
+  wf.d = S.d.get();
+  wf.dxp = S.dx.get();
+  wf.v = S.outer_v.get();
+  wf.cw = cv.w;
+  wf.pinned = S.pinned.get();
+  wf.h = S.dt;
+  wf.skip = skip;
   AdmmCheckArgs ca;
   ca.grid = local_step_grid(S.nv);
   ca.n3 = static_cast<int>(n3);
@@ -1032,6 +1059,33 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
       timed_begin(kTimeResidual);
       launch_pcg_residual(stream_, p, ra);
       timed_end();
+      if (learned) {
+        // Keep the previous iterate and its residual, predict the start, and
+        // let the judge keep whichever of the two has the smaller residual.
+        hip_check(hipMemcpyAsync(S.w_xprev.get(), S.inner_x.get(), n3 * sizeof(float),
+                                 hipMemcpyDeviceToDevice, stream_),
+                  "warm x copy");
+        hip_check(hipMemcpyAsync(S.w_rprev.get(), S.r.get(), n3 * sizeof(float),
+                                 hipMemcpyDeviceToDevice, stream_),
+                  "warm r copy");
+        hip_check(hipMemcpyAsync(S.w_part2.get(), S.part2.get(),
+                                 2 * static_cast<size_t>(S.nblk) * sizeof(double),
+                                 hipMemcpyDeviceToDevice, stream_),
+                  "warm partials copy");
+        timed_begin(kTimeWarm);
+        launch_warm_features(stream_, p, wf, S.wfeat.get(), S.wsig.get());
+        launch_mlp_predict(stream_, warm_.dev(), S.wfeat.get(), S.wsig.get(),
+                           static_cast<int64_t>(n3), S.inner_x.get(), S.wdelta.get(),
+                           skip);
+        timed_end();
+        timed_begin(kTimeResidual);
+        launch_pcg_residual(stream_, p, ra);
+        timed_end();
+        launch_warm_judge(stream_, p, S.w_part2.get(), S.w_xprev.get(), S.w_rprev.get(),
+                          S.inner_x.get(), S.r.get(), &ctl->warm_kept, skip);
+      } else if (record) {
+        launch_warm_features(stream_, p, wf, S.wfeat.get(), S.wsig.get());
+      }
       if (it == 1) launch_mas_factor(stream_, p, S.cd.get(), skip);
       if (g.true_residual_stats)
         hip_check(hipMemcpyAsync(S.x0_copy.get(), S.inner_x.get(), n3 * sizeof(float),
@@ -1044,6 +1098,11 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
       if (g.true_residual_stats)
         launch_solve_residuals(stream_, p, ra, S.x0_copy.get(), S.dx.get(),
                                S.inner_x.get(), S.part3.get(), ctl);
+      if (record)
+        launch_warm_record(stream_, S.wfeat.get(), S.wsig.get(), S.dx.get(),
+                           static_cast<int64_t>(n3), warm_.ds_rate, warm_.ds_salt++,
+                           warm_.ds_feat.get(), warm_.ds_target.get(),
+                           warm_.ds_count.get(), warm_.ds_cap, skip);
     }
     timed_begin(kTimeContact);
     if (S.coll) S.coll->contact_update(S.inner_x.get(), skip);
@@ -1094,6 +1153,8 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
     stats->pcg_max_solve_iterations =
         std::max(stats->pcg_max_solve_iterations, hc.pcg_max_solve);
     stats->pcg_stalled_solves += hc.pcg_stalled;
+    stats->warm_start_kept += hc.warm_kept;
+    stats->warm_start_solves += hc.warm_solves;
     if (g.true_residual_stats) {
       double it2, cm2;
       std::memcpy(&it2, &hc.max_true_bits, sizeof(double));
@@ -1262,6 +1323,63 @@ int Context::step(silk_hip_step_stats* stats) {
     stats->min_toi = accepted_toi;
   }
   return SILK_HIP_OK;
+}
+
+int Context::debug_pcg(const float* d, const float* b, float rel_tol, float* x) {
+  if (scene_ && scene_->dt != global_.dt) invalidate_scene();
+  if (scene_dirty_ || !scene_) build_scene();
+  Scene& S = *scene_;
+  const size_t nv = static_cast<size_t>(S.nv), n3 = 3 * nv;
+  PcgDev p = pcg_dev();
+  // r0 = b (x0 = 0), and the per-block |r0|^2, |b|^2 partials the
+  // right-hand-side criterion reads.
+  std::vector<double> part2(2 * static_cast<size_t>(std::max(1, S.nblk)), 0.0);
+  {
+    std::vector<int> row0(S.nblk), rows(S.nblk);
+    hip_check(hipMemcpyAsync(row0.data(), S.blk_row0.get(), S.nblk * sizeof(int),
+                             hipMemcpyDeviceToHost, stream_),
+              "blocks");
+    hip_check(hipMemcpyAsync(rows.data(), S.blk_rows.get(), S.nblk * sizeof(int),
+                             hipMemcpyDeviceToHost, stream_),
+              "blocks");
+    hip_check(hipStreamSynchronize(stream_), "blocks");
+    for (int blk = 0; blk < S.nblk; ++blk) {
+      double s = 0.0;
+      for (int r = row0[blk]; r < row0[blk] + rows[blk]; ++r)
+        for (int k = 0; k < 3; ++k) s += static_cast<double>(b[3 * r + k]) * b[3 * r + k];
+      part2[2 * blk] = s;
+      part2[2 * blk + 1] = s;
+    }
+  }
+  hip_check(hipMemcpyAsync(S.d.get(), d, nv * sizeof(float), hipMemcpyHostToDevice,
+                           stream_),
+            "debug d");
+  hip_check(hipMemcpyAsync(S.r.get(), b, n3 * sizeof(float), hipMemcpyHostToDevice,
+                           stream_),
+            "debug b");
+  hip_check(hipMemcpyAsync(S.part2.get(), part2.data(), part2.size() * sizeof(double),
+                           hipMemcpyHostToDevice, stream_),
+            "debug partials");
+  hip_check(hipMemsetAsync(S.inner_x.get(), 0, n3 * sizeof(float), stream_), "x0");
+  launch_mas_factor(stream_, p, S.cd.get());
+  int iters = 0;
+  if (S.use_persistent) {
+    PersistDev a = persist_args(S, global_, pcg_max_iter());
+    a.dx_out = nullptr;
+    hip_check(launch_pcg_persistent(stream_, a, rel_tol), "persistent PCG launch");
+    S.h_state.resize(S.nsys);
+    hip_check(hipMemcpyAsync(S.h_state.data(), S.ps_out.get(), S.nsys * sizeof(PcgState),
+                             hipMemcpyDeviceToHost, stream_),
+              "pcg state");
+  } else {
+    pcg_iterate(rel_tol, RhsArgs{});  // commits x += dx
+  }
+  hip_check(hipMemcpyAsync(x, S.inner_x.get(), n3 * sizeof(float), hipMemcpyDeviceToHost,
+                           stream_),
+            "debug x");
+  hip_check(hipStreamSynchronize(stream_), "debug pcg");
+  for (const auto& st : S.h_state) iters += st.iter;
+  return iters;
 }
 
 void Context::update_pin_targets(const ClothEntity& c) {

@@ -15,6 +15,7 @@
 #include "api/devbuf.hpp"
 #include "host/geometry.hpp"
 #include "hip/kernels.hpp"
+#include "hip/warm_start.hpp"
 #include "silk_hip.h"
 
 namespace silk::amd {
@@ -29,7 +30,9 @@ enum : int {
   kTimeCcd = 5,       // CCD kernel
   kTimeResidual = 6,  // FP64 residual / RHS kernel
   kTimeHostContacts = 7,  // host wall time of contact assembly and colouring
+  kTimeWarm = 8,      // learned warm start: features and network
 };
+constexpr int kTimeClasses = 9;
 
 struct ClothEntity {
   uint32_t handle = 0;
@@ -93,6 +96,10 @@ struct Scene {
   DevBuf<unsigned short> bank_inv_h;  // 16-bit bank inverses (mas_precision)
   DevBuf<float> x0_copy;    // 3*nv: warm start of the last solve (stats)
   DevBuf<double> part3;     // 3 per row block: solve residual partials
+  // ---- learned warm start (allocated when used)
+  DevBuf<unsigned short> wfeat;  // 3*nv samples x 16 features
+  DevBuf<float> wsig, wdelta, w_xprev, w_rprev;  // 3*nv
+  DevBuf<double> w_part2;        // residual partials of the previous iterate
   DevBuf<double> part, part2;
   std::vector<double> h_part2;
   DevBuf<PcgState> state;
@@ -133,17 +140,43 @@ class Context {
   std::vector<ObstacleEntity> obstacles_;
   uint64_t rng_state_ = 0x5117C0DE2024ULL;
 
-  /// A learned warm-start network is loaded (silk_hip_set_warm_start_net).
-  bool has_warm_net() const { return warm_net_loaded_; }
-  bool warm_net_loaded_ = false;
+  /// Learned warm start: network (FP32 master, Adam moments, BF16 copies),
+  /// training scratch and the recorded samples.  Device buffers of the
+  /// context's device; they outlive scene rebuilds.
+  struct WarmNet {
+    DevBuf<float> params, m, v, grad, grad_part;
+    DevBuf<unsigned short> w1, w2, w3;
+    DevBuf<double> loss_part, loss;
+    bool loaded = false;
+    int adam_t = 0;
+    DevBuf<unsigned short> ds_feat;
+    DevBuf<float> ds_target;
+    DevBuf<unsigned long long> ds_count;
+    int64_t ds_cap = 0;
+    float ds_rate = 0.0f;
+    uint32_t ds_salt = 0;
+    MlpDev dev() const;
+  };
+  WarmNet warm_;
+  bool has_warm_net() const { return warm_.loaded; }
+  bool recording() const { return warm_.ds_cap > 0; }
+  /// Allocates the network buffers (device must be ready).
+  void warm_alloc();
+  void warm_set(const float* params);
+  void warm_get(float* params);
+  void warm_record(int64_t capacity, float rate);
+  int64_t warm_samples();
+  void warm_train(int steps, int batch, float lr, uint64_t seed, float* loss);
+  void warm_debug(int64_t n, const float* feat, const float* target, float* y,
+                  float* grad, double* loss);
 
   bool dev_ready_ = false;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev_[2] = {nullptr, nullptr};
   bool scene_dirty_ = true;
   std::unique_ptr<Scene> scene_;
-  double timing_ms_[8] = {};
-  int64_t timing_launch_[8] = {};
+  double timing_ms_[kTimeClasses] = {};
+  int64_t timing_launch_[kTimeClasses] = {};
   // Per-kernel event timing (off unless enabled): pairs of events recorded
   // around launches on stream_, resolved after the next synchronisation.
   bool timing_on_ = false;
@@ -176,6 +209,8 @@ class Context {
   bool admm_solve(silk_hip_step_stats* st);
   bool admm_solve_device(silk_hip_step_stats* st);
   void get_positions(const ClothEntity& c, float* out) const;
+  /// silk_hip_debug_pcg: one solve of K + diag(d) x = b from x = 0.
+  int debug_pcg(const float* d, const float* b, float rel_tol, float* x);
   /// Push one cloth's pin targets to the device scene (if built).
   void update_pin_targets(const ClothEntity& c);
 

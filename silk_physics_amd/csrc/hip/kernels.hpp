@@ -87,6 +87,8 @@ struct AdmmCtl {
   int pcg_solves = 0;
   int pcg_max_solve = 0;  // largest iteration count of one solve (atomicMax)
   int pcg_stalled = 0;    // solves that ended at max_iter
+  int warm_kept = 0;      // learned warm starts kept by the judge
+  int warm_solves = 0;    // learned warm starts evaluated
   int pad = 0;
   float rel_tol = 0;  // linear tolerance of the next x-update
   float last_rel = 0; // sqrt(rr / rr0) of the last PCG solve
