@@ -1,21 +1,30 @@
 #!/usr/bin/env python3
 """Benchmark: simulated steps per second of the 80x80 sheet dropped 0.5 m onto
 a 2562-vertex icosphere with collision (broadphase, FP64 CCD, coloured
-contacts), CG to 1e-6 relative residual, on one GPU.
+contacts), every PCG solve run to ||b - A x|| <= 1e-6 ||b|| (FP64 residual of
+the solve's iterate, restarts included), on one GPU.
 
 Prints one JSON line.  Every solver_step call drains the library's stream
 before returning, so host wall time brackets the device work of the timed
 steps.  Step 0 (lazy scene assembly) is the warmup and is reported apart.
 
 roofline: the kernel with the largest device time over the timed steps,
-timed with HIP events on the library stream around every launch; its
-algorithmic bytes per launch follow the byte model below (the sizes come
-from the library's scene info):
-  PCG solve   iterations x (8 nnz_off + 48 V + 4096 banks)
-  local step  136 F + 60 V
-  residual    8 nnz_off + 64 V
-cpu_baseline: the FP64 CPU oracle, one thread, stepped from the GPU's state
-after the timed steps (the draped, contact-rich phase) for a bounded sample.
+timed with HIP events on the library stream around every launch.  Its
+algorithmic bytes per launch follow the reference formulation's byte model
+(3x3 BSR operator and packed 96x96 MAS blocks):
+  PCG iteration  40 nnzb + 4 (V + 1) + S n_blk + 120 V,  nnzb = offdiag + V,
+                 S = 18624 (FP32 blocks) or 9312 (16-bit), n_blk = MAS banks
+  local step     136 F + 60 V
+  residual       8 offdiag + 64 V
+and, beside it, this library's own bytes per PCG iteration (vertex-scalar
+ELL and 32x32 banks): 8 offdiag + 48 V + 4096 banks (2048 at 16 bits).
+traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE
+counter runs (profiles/pcg_traffic.json, made with --pmc-probe), scaled by
+the same counters on the library's copy kernel of known size.
+cpu_baseline: the reference CPU algorithm restated (PD with a sparse
+Cholesky factor, SVD stretch projection, barrier reflection targets, CCD
+rollback; oracle/cpu_pipeline.cpp) on the same scene, timed on the host's
+cores for a bounded sample.
 """
 from __future__ import annotations
 
@@ -35,24 +44,27 @@ from silk_physics_amd import scenes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "sim steps/sec, 80x80 sheet + collision; PCG iters to 1e-6 rel residual"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pcg_traffic.json")
 CLASSES = {1: "k_pcg_persistent", 2: "k_local_step", 3: "k_contact_update",
            4: "k_broadphase", 5: "k_ccd", 6: "k_residual"}
 
 
-def global_kw(true_residual: bool = False) -> dict:
+def global_kw(true_residual: bool = False, criterion: int = sp.PCG_TRUE,
+              mas: int = sp.MAS_FP32) -> dict:
     return dict(acceleration_z=-9.81, dt=1 / 60, max_inner_iteration=10,
                 max_outer_iteration=100, linear_solver_rel_tol_min=1e-6,
                 linear_solver_rel_tol_max=1e-6, initial_linear_rel_tol=1e-6,
-                true_residual_stats=int(true_residual))
+                true_residual_stats=int(true_residual), pcg_criterion=criterion,
+                mas_precision=mas)
 
 
-def build_world(n: int):
+def build_world(n: int, mas: int = sp.MAS_FP32):
     s = scenes.c2_scene(n)
     w = sp.World()
     r = w.init(sp.Backend.GPU)
     if not r:
         raise SystemExit(f"GPU init failed: {r.to_string()}")
-    assert w.set_global_config(sp.GlobalConfig(**global_kw()))
+    assert w.set_global_config(sp.GlobalConfig(**global_kw(mas=mas)))
     cc = sp.CollisionConfig()  # on, self on, group 0, mu 0.3
     r, h = w.add_cloth(sp.ClothConfig(), cc, s["V"], s["F"], s["pins"])
     assert r, r.to_string()
@@ -61,16 +73,26 @@ def build_world(n: int):
     return s, w, h
 
 
-def byte_model(cls: int, info: dict, iters: int, launches: int) -> float:
+def byte_model(cls: int, info: dict, iters: int, launches: int,
+               half: bool = False) -> float:
+    """Algorithmic bytes of `launches` launches of a kernel class in the
+    reference formulation (3x3 BSR, packed 96x96 MAS blocks)."""
     V, F = info["vertices"], info["triangles"]
     if cls == 1:
-        return iters * (8.0 * info["offdiag_entries"] + 48.0 * V +
-                        4096.0 * info["mas_banks"])
+        nnzb = info["offdiag_entries"] + V
+        S = 9312.0 if half else 18624.0
+        return iters * (40.0 * nnzb + 4.0 * (V + 1) + S * info["mas_banks"] + 120.0 * V)
     if cls == 2:
         return launches * (136.0 * F + 60.0 * V)
     if cls == 6:
         return launches * (8.0 * info["offdiag_entries"] + 64.0 * V)
     return 0.0
+
+
+def own_pcg_bytes(info: dict, half: bool = False) -> float:
+    """This library's bytes per PCG iteration (vertex-scalar ELL, 32x32 banks)."""
+    return (8.0 * info["offdiag_entries"] + 48.0 * info["vertices"] +
+            (2048.0 if half else 4096.0) * info["mas_banks"])
 
 
 def cpu_baseline(s, w, h, x_last, x_prev, dt, budget_s: float, max_steps: int):

@@ -94,11 +94,14 @@ typedef struct {
                                      PCG solve, the FP64 residual of the
                                      iterate x0 + dx and of the committed
                                      FP32 x (reported in stats) */
-  int32_t pcg_criterion;          /* SILK_HIP_PCG_RR0 (reference) or
-                                     SILK_HIP_PCG_RHS */
-  int32_t warm_start;             /* SILK_HIP_WARM_PREVIOUS (reference) or
-                                     SILK_HIP_WARM_LEARNED */
-  int32_t reserved[4];
+  int32_t pcg_criterion;          /* SILK_HIP_PCG_RR0 (reference),
+                                     SILK_HIP_PCG_RHS or SILK_HIP_PCG_TRUE */
+  int32_t warm_start;             /* SILK_HIP_WARM_PREVIOUS (reference),
+                                     SILK_HIP_WARM_LEARNED or
+                                     SILK_HIP_WARM_EXTRAPOLATE */
+  int32_t warm_projection;        /* SILK_HIP_PROJECT_RESIDUAL (default) or
+                                     SILK_HIP_PROJECT_GALERKIN */
+  int32_t reserved[3];
 } silk_hip_global;
 
 enum {
@@ -112,15 +115,33 @@ enum {
   SILK_HIP_PCG_RR0 = 0,
   /* Stop when ||b - A x|| <= rel * ||b|| (b the whole right-hand side of
    * the x-update, FP64 norms): a better warm start takes fewer iterations. */
-  SILK_HIP_PCG_RHS = 1
+  SILK_HIP_PCG_RHS = 1,
+  /* As SILK_HIP_PCG_RHS, then the FP64 residual ||b - A (x0 + dx)|| of the
+   * iterate (x0 + dx summed in FP64, never rounded to FP32) is tested on the
+   * device; a system above rel * ||b|| restarts from its committed x, at most
+   * 3 times (counted in pcg_restarts / pcg_unverified_solves). */
+  SILK_HIP_PCG_TRUE = 2
 };
 
 enum {
   /* CG starts from the previous ADMM iterate (admm_solver.cu:272-292). */
   SILK_HIP_WARM_PREVIOUS = 0,
-  /* CG starts from the previous iterate plus the correction predicted by the
-   * per-vertex network loaded with silk_hip_set_warm_start_net. */
-  SILK_HIP_WARM_LEARNED = 1
+  /* CG starts from x0 + c1 * (the correction predicted by the per-vertex
+   * network loaded with silk_hip_set_warm_start_net) + c2 * (the previous
+   * solve's correction), (c1, c2) per cloth and coordinate from a 2x2
+   * projection of the x-update system; the start is kept only when its FP64
+   * residual is below that of x0. */
+  SILK_HIP_WARM_LEARNED = 1,
+  /* As SILK_HIP_WARM_LEARNED without the network: x0 + c2 * (the previous
+   * solve's correction). */
+  SILK_HIP_WARM_EXTRAPOLATE = 2
+};
+
+enum {
+  /* (c1, c2) minimise the FP64 residual norm ||b - A x||. */
+  SILK_HIP_PROJECT_RESIDUAL = 0,
+  /* (c1, c2) minimise the A-norm of the error (Galerkin). */
+  SILK_HIP_PROJECT_GALERKIN = 1
 };
 
 /* Per-step counters returned by silk_hip_step. */
@@ -144,6 +165,9 @@ typedef struct {
   int32_t warm_start_kept;   /* solves that started from the network's
                                 prediction (SILK_HIP_WARM_LEARNED) */
   int32_t warm_start_solves; /* solves the network was evaluated for */
+  int32_t pcg_restarts;      /* SILK_HIP_PCG_TRUE: restarts of a system's solve */
+  int32_t pcg_unverified_solves; /* SILK_HIP_PCG_TRUE: solves still above the
+                                    tolerance after the last restart */
 } silk_hip_step_stats;
 
 /* Number of visible HIP devices (0 when none).  Never fails. */
@@ -262,6 +286,24 @@ int silk_hip_debug_candidates(silk_hip_ctx* ctx, const float* pos0,
                               int64_t pt_cap, int64_t* npt, int32_t* ee,
                               int64_t ee_cap, int64_t* nee);
 
+/* DCD on explicit positions (n = 3 * total collider vertices per end,
+ * collider numbering as silk_hip_debug_candidates) at `time` with activation
+ * distance dhat: 6 ints per record (type 0 PT / 1 EE, the four global vertex
+ * ids in the record's order, 0) and its distance in dist.  Up to cap records
+ * are copied; *nrec receives the count (dcd.cu:102-261). */
+int silk_hip_debug_dcd(silk_hip_ctx* ctx, const float* pos0, const float* pos1,
+                       int64_t n, float time, float dhat, int32_t* recs,
+                       float* dist, int64_t cap, int64_t* nrec);
+
+/* Discrete decisions of the last silk_hip_step: the TOI returned by the CCD
+ * of every outer iteration (frame-time units, one per iteration), and every
+ * contact the step appended, 6 ints each: the DCD call that found it (0 at
+ * the frame start, k after the k-th rollback), the type and the four global
+ * vertex ids in the record's order.  NULL buffers query the counts. */
+int silk_hip_debug_decisions(silk_hip_ctx* ctx, float* tois, int32_t cap_toi,
+                             int32_t* n_toi, int32_t* contacts, int64_t cap_contacts,
+                             int64_t* n_contacts);
+
 /* The step's FP64 CCD kernel on n explicit queries: is_vf[i] selects
  * vertex-face (point, tri0, tri1, tri2) or edge-edge (a0, a1, b0, b1);
  * p0 / p1 hold 12 doubles per query, err 3 per query.  hit[i] = 1 and toi[i]
@@ -280,6 +322,12 @@ int silk_hip_debug_ccd_min(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
                            const double* p0, const double* p1,
                            const double* err, double ms, double tolerance,
                            int32_t max_iter, double* min_toi);
+
+/* Device copy of `bytes` (a multiple of 4) from one buffer to another, reps
+ * times, with the library's copy kernel (k_copy_unless); *ms receives the
+ * elapsed device time.  A reference of known traffic for hardware counters:
+ * each launch reads and writes `bytes`. */
+int silk_hip_debug_copy(silk_hip_ctx* ctx, int64_t bytes, int32_t reps, double* ms);
 
 /* One PCG solve of the scene's x-update operator K + diag(d) on an explicit
  * right-hand side, from x = 0, with the current global configuration

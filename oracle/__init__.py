@@ -66,6 +66,11 @@ def lib():
             "oracle_candidates": (C.c_int, [vp, d, d, i, C.c_int, i, i, C.c_int,
                                             i]),
             "oracle_dcd_contact": (C.c_int, [C.c_int, d, d, i, d, C.c_double, d]),
+            "oracle_set_replay": (None, [vp, d, C.c_int, i, C.c_int]),
+            "oracle_set_obstacle_frame": (None, [vp, C.c_int, f, f]),
+            "oracle_decisions": (C.c_int, [vp, d, C.c_int, i, i, C.c_int, i]),
+            "oracle_dcd": (C.c_int, [vp, d, d, C.c_double, C.c_double, i, d, C.c_int,
+                                     i]),
             "oracle_sym3_min_eig": (C.c_double, [d]),
             "oracle_contact_sweeps": (None, [C.c_int, i, d, d, d, d, d, C.c_int, d,
                                              d, d, d, C.c_int, d, i]),
@@ -149,6 +154,53 @@ class OracleWorld:
     def set_obstacle_position(self, idx: int, V):
         v, vp = _f(V)
         lib().oracle_set_obstacle_position(self._w, idx, vp)
+
+    def set_obstacle_frame(self, idx: int, prev, curr):
+        """Obstacle idx moves from prev to curr over the next step."""
+        a, ap = _f(prev)
+        b, bp = _f(curr)
+        lib().oracle_set_obstacle_frame(self._w, idx, ap, bp)
+
+    def set_replay(self, tois, contacts):
+        """Replay another solver's decisions in the next step: tois[k] is the
+        CCD result of outer iteration k (frame-time units); contacts (n, 6)
+        holds (DCD call, type, four global vertex ids in record order) of
+        every contact to append."""
+        t, tp = _d(np.asarray(tois, dtype=np.float64).reshape(-1))
+        c, cp = _i(np.asarray(contacts, dtype=np.int32).reshape(-1, 6))
+        lib().oracle_set_replay(self._w, tp, t.size, cp, c.shape[0])
+
+    def decisions(self):
+        """Decisions of the last step: (TOIs, contacts (n, 6), replayed
+        contacts whose record could not be formed)."""
+        nt, nc = C.c_int(0), C.c_int(0)
+        lib().oracle_decisions(self._w, None, 0, C.byref(nt), None, 0, C.byref(nc))
+        t = np.zeros(max(1, nt.value))
+        c = np.zeros((max(1, nc.value), 6), dtype=np.int32)
+        miss = lib().oracle_decisions(self._w, _d(t)[1] if False else
+                                      t.ctypes.data_as(C.POINTER(C.c_double)),
+                                      nt.value, C.byref(nt),
+                                      c.ctypes.data_as(C.POINTER(C.c_int)), nc.value,
+                                      C.byref(nc))
+        return t[:nt.value], c[:nc.value], miss
+
+    def dcd(self, pos0, pos1, time, dhat):
+        """DCD records on explicit collider positions (global numbering):
+        (records (n, 5): type, four ids in record order; distances (n,))."""
+        a, ap = _d(pos0)
+        b, bp = _d(pos1)
+        n = C.c_int(0)
+        cap = 1 << 14
+        while True:
+            recs = np.zeros((cap, 6), dtype=np.int32)
+            dist = np.zeros(cap)
+            rc = lib().oracle_dcd(self._w, ap, bp, time, dhat,
+                                  recs.ctypes.data_as(C.POINTER(C.c_int)),
+                                  dist.ctypes.data_as(C.POINTER(C.c_double)), cap,
+                                  C.byref(n))
+            if rc == 0:
+                return recs[:n.value, :5], dist[:n.value]
+            cap = n.value
 
     def candidates(self, pos0, pos1):
         """Broadphase candidates for explicit collider positions (global
@@ -374,3 +426,91 @@ def contact_sweeps(state, coef, gamma, normal, affine, mu, x, cw, eta=None,
         corr.ctypes.data_as(C.POINTER(C.c_double)), int(sweeps),
         p.ctypes.data_as(C.POINTER(C.c_double)), col.ctypes.data_as(C.POINTER(C.c_int)))
     return p, eta, corr.reshape(nc, 3), col
+
+
+# ----------------------------------------------- reference CPU pipeline
+
+PIPE_LIB = HERE / "build" / "libcpu_pipeline.so"
+_pipe = None
+
+
+def pipe_lib():
+    """The reference CPU algorithm restated (oracle/cpu_pipeline.cpp), built
+    -O3 -march=native: the timing baseline of bench.py."""
+    global _pipe
+    if _pipe is None:
+        if not PIPE_LIB.exists():
+            build()
+        L = C.CDLL(str(PIPE_LIB))
+        d, i, f = C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_float)
+        vp = C.c_void_p
+        sig = {
+            "cpu_pipeline_create": (vp, [C.c_int, d]),
+            "cpu_pipeline_destroy": (None, [vp]),
+            "cpu_pipeline_add_cloth": (C.c_int, [vp, f, C.c_int, i, C.c_int, i, C.c_int]),
+            "cpu_pipeline_add_obstacle": (C.c_int, [vp, f, C.c_int, i, C.c_int]),
+            "cpu_pipeline_step": (C.c_int, [vp, d]),
+            "cpu_pipeline_state": (None, [vp, C.c_int, f, f, C.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _pipe = L
+    return _pipe
+
+
+class CpuPipeline:
+    """The reference CPU solver step (pipeline.cpp:45-168) restated: PD with
+    a project-written sparse Cholesky factor, per-face SVD projection,
+    barrier reflection targets and FP32 CCD rollback, on `threads` host
+    threads.  Default parameters are the reference's ClothConfig and
+    GlobalConfig defaults with dt 1/60 and g = -9.81 z."""
+
+    def __init__(self, threads: int = 1, dt: float = 1 / 60, g=(0.0, 0.0, -9.81),
+                 k: float = 100.0, kb: float = 1e-4, density: float = 0.1,
+                 damping: float = 0.01, max_outer: int = 100, max_inner: int = 10,
+                 ms_scale: float = 0.2, dcd_scale: float = 1.0, ccd_tol: float = 1e-6,
+                 ccd_max_iter: int = 4096):
+        p = np.array([dt, *g, k, kb, density, damping, max_outer, max_inner, ms_scale,
+                      dcd_scale, ccd_tol, ccd_max_iter], dtype=np.float64)
+        self._w = pipe_lib().cpu_pipeline_create(threads, p.ctypes.data_as(C.POINTER(C.c_double)))
+        self.sizes = []
+
+    def __del__(self):
+        if getattr(self, "_w", None):
+            pipe_lib().cpu_pipeline_destroy(self._w)
+            self._w = None
+
+    def add_cloth(self, V, F, pins=()):
+        v, vp = _f(V)
+        f, fp = _i(F)
+        p, pp = _i(np.asarray(pins, dtype=np.int32).reshape(-1))
+        rc = pipe_lib().cpu_pipeline_add_cloth(self._w, vp, v.size // 3, fp, f.size // 3,
+                                               pp, p.size)
+        self.sizes.append(v.size)
+        return rc
+
+    def add_obstacle(self, V, F):
+        v, vp = _f(V)
+        f, fp = _i(F)
+        return pipe_lib().cpu_pipeline_add_obstacle(self._w, vp, v.size // 3, fp, f.size // 3)
+
+    def step(self) -> tuple[int, dict]:
+        st = np.zeros(5)
+        rc = pipe_lib().cpu_pipeline_step(self._w, st.ctypes.data_as(C.POINTER(C.c_double)))
+        return rc, dict(zip(["outer", "inner", "collisions", "ccd_candidates", "min_toi"],
+                            st.tolist()))
+
+    def positions(self, idx: int = 0) -> np.ndarray:
+        x = np.zeros(self.sizes[idx], dtype=np.float32)
+        v = np.zeros_like(x)
+        pipe_lib().cpu_pipeline_state(self._w, idx, _f(x)[1] if False else
+                                      x.ctypes.data_as(C.POINTER(C.c_float)),
+                                      v.ctypes.data_as(C.POINTER(C.c_float)), 0)
+        return x
+
+    def set_state(self, idx: int, x, v):
+        a, ap = _f(np.asarray(x).reshape(-1))
+        b, bp = _f(np.asarray(v).reshape(-1))
+        pipe_lib().cpu_pipeline_state(self._w, idx, ap, bp, 1)

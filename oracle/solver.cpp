@@ -280,7 +280,7 @@ PcgResult pcg_solve(const Bsr3& A, const std::vector<double>& diag,
   // for the right-hand-side criterion, under which a warm start that already
   // meets the tolerance takes no iteration.
   double ref2 = rr;
-  if (criterion == 1) {
+  if (criterion >= 1) {  // 2 (true residual) is 1 in FP64: no drift
     ref2 = 0.0;
     for (size_t i = 0; i < N; ++i) ref2 += b[i] * b[i];
     if (rr <= abs_tol * abs_tol || rr <= rel_tol * rel_tol * ref2) {

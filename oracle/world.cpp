@@ -100,6 +100,17 @@ struct World {
   std::vector<double> cbase_w, cw, p, p_prev, eta;  // per dof of cloth state
   int active_dof = 0;
   StepStats stats;
+  // Decisions of the last step: CCD TOI of every outer iteration, appended
+  // contacts (DCD call, type, four global ids in record order).
+  std::vector<double> log_toi;
+  std::vector<int> log_contacts;
+  // Replay of another solver's decisions for the next step (set_replay):
+  // the outer loop takes these TOIs and every DCD call appends exactly these
+  // contacts, their records formed on the oracle's own positions.
+  bool replay = false;
+  std::vector<double> rp_toi;
+  std::vector<int> rp_contacts;
+  int replay_missing = 0;  // replayed contacts whose record could not be formed
 };
 
 namespace {
@@ -303,8 +314,10 @@ V3 lerp(const double* a, const double* b, double t) {
 }  // namespace
 
 // exact_pt_uv / exact_ee_uv (dcd.cu:321-394).
-bool pt_uv(const V3& x0, const V3& x1, const V3& x2, const V3& x3, double eps,
-           double* u, double* v) {
+// clip = false keeps a projection that falls outside the triangle (used
+// when replaying a contact that another solver activated).
+static bool pt_uv_impl(const V3& x0, const V3& x1, const V3& x2, const V3& x3,
+                       double eps, bool clip, double* u, double* v) {
   V3 x21 = x2 - x1, x31 = x3 - x1, x01 = x0 - x1;
   double a = dot(x21, x21), b = dot(x21, x31), c = dot(x31, x31);
   double d = dot(x21, x01), e = dot(x31, x01);
@@ -312,10 +325,15 @@ bool pt_uv(const V3& x0, const V3& x1, const V3& x2, const V3& x3, double eps,
   double area_eps = eps * std::max(a, c);
   if (det < area_eps * area_eps) return false;
   double b1 = (c * d - b * e) / det, b2 = (a * e - b * d) / det;
-  if (b1 < 0 || b2 < 0 || b1 + b2 > 1) return false;
+  if (clip && (b1 < 0 || b2 < 0 || b1 + b2 > 1)) return false;
   *u = b1;
   *v = b2;
   return true;
+}
+
+bool pt_uv(const V3& x0, const V3& x1, const V3& x2, const V3& x3, double eps,
+           double* u, double* v) {
+  return pt_uv_impl(x0, x1, x2, x3, eps, true, u, v);
 }
 
 bool ee_uv(const V3& x0, const V3& x1, const V3& x2, const V3& x3, double eps,
@@ -364,14 +382,22 @@ void coulomb_solve(const double us[3], const double n[3], double mu,
 // closest point lies outside the triangle, the edges are degenerate, the
 // distance is 0 or at least dhat, or no vertex can move.  friction and presc
 // are left for the caller.
+bool dcd_pair_forced(bool pt, const int gv[4], const V3 x[4], const double inv[4],
+                     double dhat, bool force, CollisionRec* out);
+
 bool dcd_pair(bool pt, const int gv[4], const V3 x[4], const double inv[4],
               double dhat, CollisionRec* out) {
+  return dcd_pair_forced(pt, gv, x, inv, dhat, false, out);
+}
+
+bool dcd_pair_forced(bool pt, const int gv[4], const V3 x[4], const double inv[4],
+                     double dhat, bool force, CollisionRec* out) {
   const double eps = 1e-6;
   double u, v;
   V3 disp;
   double param[4], coef[4];
   if (pt) {
-    if (!pt_uv(x[0], x[1], x[2], x[3], eps, &u, &v)) return false;
+    if (!pt_uv_impl(x[0], x[1], x[2], x[3], eps, !force, &u, &v)) return false;
     V3 tp = (1.0 - u - v) * x[1] + u * x[2] + v * x[3];
     disp = tp - x[0];
     param[0] = 1.0; param[1] = 1.0 - u - v; param[2] = u; param[3] = v;
@@ -387,7 +413,7 @@ bool dcd_pair(bool pt, const int gv[4], const V3 x[4], const double inv[4],
   const double d2 = dot(disp, disp);
   if (d2 == 0.0) return false;
   const double dist = std::sqrt(d2);
-  if (dist >= dhat) return false;
+  if (dist >= dhat && !force) return false;
   V3 n = (-1.0 / dist) * disp;
   double den = 0.0;
   for (int k = 0; k < 4; ++k) den += param[k] * param[k] * inv[k];
@@ -413,6 +439,29 @@ namespace {
 
 // DCD at fixed time (dcd.cu:102-261).  Cloth colliders are refit with
 // t0 = t1, obstacles interpolate along the frame.
+bool make_collision_gv(const World& w, bool pt, const int gv[4], int obj_a,
+                       int obj_b, double time, double dhat,
+                       const std::vector<ObjInfo>& objs, bool force,
+                       CollisionRec* out) {
+  V3 x[4];
+  double inv[4];
+  for (int k = 0; k < 4; ++k) {
+    x[k] = lerp(&w.pos_t0[3 * gv[k]], &w.pos_t1[3 * gv[k]], time);
+    inv[k] = inv_mass_of(w, gv[k]);
+  }
+  CollisionRec r;
+  if (!dcd_pair_forced(pt, gv, x, inv, dhat, force, &r)) return false;
+  const float fa = objs[obj_a].friction, fb = objs[obj_b].friction;
+  r.friction = std::sqrt(static_cast<double>(fa) * static_cast<double>(fb));
+  // Prescribed obstacle displacement: coef_k (x_k(t1) - x_k(time)).
+  for (int k = 0; k < 4; ++k) {
+    if (gv[k] < w.nv_cloth) continue;
+    for (int d = 0; d < 3; ++d) r.presc[d] += r.coef[k] * (w.pos_t1[3 * gv[k] + d] - x[k][d]);
+  }
+  *out = r;
+  return true;
+}
+
 bool make_collision(const World& w, bool pt, const Prim* A, const Prim* B,
                     double time, double dhat, const std::vector<ObjInfo>& objs,
                     CollisionRec* out) {
@@ -428,23 +477,17 @@ bool make_collision(const World& w, bool pt, const Prim* A, const Prim* B,
     gv[2] = B->v[0];
     gv[3] = B->v[1];
   }
-  V3 x[4];
-  double inv[4];
-  for (int k = 0; k < 4; ++k) {
-    x[k] = lerp(&w.pos_t0[3 * gv[k]], &w.pos_t1[3 * gv[k]], time);
-    inv[k] = inv_mass_of(w, gv[k]);
-  }
-  CollisionRec r;
-  if (!dcd_pair(pt, gv, x, inv, dhat, &r)) return false;
-  const float fa = objs[A->obj].friction, fb = objs[B->obj].friction;
-  r.friction = std::sqrt(static_cast<double>(fa) * static_cast<double>(fb));
-  // Prescribed obstacle displacement: coef_k (x_k(t1) - x_k(time)).
-  for (int k = 0; k < 4; ++k) {
-    if (gv[k] < w.nv_cloth) continue;
-    for (int d = 0; d < 3; ++d) r.presc[d] += r.coef[k] * (w.pos_t1[3 * gv[k] + d] - x[k][d]);
-  }
-  *out = r;
-  return true;
+  return make_collision_gv(w, pt, gv, A->obj, B->obj, time, dhat, objs, false, out);
+}
+
+// Object index (cloth c -> c, obstacle o -> ncloth + o) of a global vertex.
+int object_of(const World& w, int gv) {
+  const int c = cloth_of(w, gv);
+  if (c >= 0) return c;
+  for (size_t o = 0; o < w.obstacles.size(); ++o)
+    if (gv >= w.obstacles[o].off && gv < w.obstacles[o].off + w.obstacles[o].nv)
+      return static_cast<int>(w.cloths.size() + o);
+  return -1;
 }
 
 std::array<int, 5> contact_key(const CollisionRec& r) {
@@ -538,13 +581,15 @@ static void recolor(World& w) {
   w.active_dof = 3 * active_v;
 }
 
-static int append_collisions(World& w, std::vector<CollisionRec> recs) {
+static int append_collisions(World& w, std::vector<CollisionRec> recs, int call) {
   std::sort(recs.begin(), recs.end(), [](const CollisionRec& a, const CollisionRec& b) {
     return contact_key(a) < contact_key(b);
   });
   int added = 0;
   for (const auto& r : recs) {
     if (!w.keys.insert(contact_key(r)).second) continue;
+    w.log_contacts.insert(w.log_contacts.end(),
+                          {call, r.type, r.gv[0], r.gv[1], r.gv[2], r.gv[3]});
     w.contacts.push_back(contact_of(r, w.nv_cloth, w.cbase_w));
     ++added;
   }
@@ -561,6 +606,34 @@ static std::vector<CollisionRec> dcd(World& w, double time, double dhat) {
     if (make_collision(w, true, t, p, time, dhat, objs, &r)) out.push_back(r);
   for (auto& [a, b] : cand.ee)
     if (make_collision(w, false, a, b, time, dhat, objs, &r)) out.push_back(r);
+  return out;
+}
+
+// Replay: the records of the contacts another solver appended at DCD call
+// `call`, formed on this world's positions whatever its own activation test
+// says (a projection just outside the triangle or a distance just above
+// dhat is kept).
+static std::vector<CollisionRec> dcd_replay(World& w, int call, double time,
+                                            double dhat) {
+  auto objs = build_objects(w);
+  std::vector<CollisionRec> out;
+  CollisionRec r;
+  for (size_t i = 0; i + 6 <= w.rp_contacts.size(); i += 6) {
+    const int* e = &w.rp_contacts[i];
+    if (e[0] != call) continue;
+    const bool pt = e[1] == 0;
+    const int gv[4] = {e[2], e[3], e[4], e[5]};
+    bool valid = true;
+    for (int k = 0; k < 4; ++k) valid = valid && gv[k] >= 0 && gv[k] < w.nv_total;
+    const int oa = valid ? object_of(w, pt ? gv[1] : gv[0]) : -1;
+    const int ob = valid ? object_of(w, pt ? gv[0] : gv[2]) : -1;
+    if (oa < 0 || ob < 0 ||
+        !make_collision_gv(w, pt, gv, oa, ob, time, dhat, objs, true, &r)) {
+      ++w.replay_missing;
+      continue;
+    }
+    out.push_back(r);
+  }
   return out;
 }
 
@@ -868,8 +941,17 @@ static bool step(World& w) {
   w.p_prev.assign(N, 0.0);
   w.eta.assign(N, 0.0);
   w.active_dof = 0;
+  w.log_toi.clear();
+  w.log_contacts.clear();
+  w.replay_missing = 0;
+  int call = 0;
+  auto find_contacts = [&](double time) {
+    append_collisions(w, w.replay ? dcd_replay(w, call, time, dhat) : dcd(w, time, dhat),
+                      call);
+    ++call;
+  };
   set_cloth(outer_x, outer_x);
-  append_collisions(w, dcd(w, 0.0, dhat));
+  find_contacts(0.0);
   double remaining = 1.0;
   int outer = 0;
   for (; outer < w.P.max_outer; ++outer) {
@@ -879,7 +961,10 @@ static bool step(World& w) {
     }
     if (!admm_solve(w, inner, minert)) return false;
     set_cloth(outer_x, inner);
-    const double toi = min_toi(w, 1.0 - remaining, remaining, ms);
+    const double toi = w.replay && outer < static_cast<int>(w.rp_toi.size())
+                           ? w.rp_toi[outer]
+                           : min_toi(w, 1.0 - remaining, remaining, ms);
+    w.log_toi.push_back(toi);
     for (int i = 0; i < N; ++i) outer_v[i] = (inner[i] - outer_x[i]) / h;
     if (toi >= remaining) {
       outer_x = inner;
@@ -890,8 +975,9 @@ static bool step(World& w) {
     for (int i = 0; i < N; ++i) outer_x[i] = frac * inner[i] + (1.0 - frac) * outer_x[i];
     remaining -= rb;
     set_cloth(outer_x, outer_x);
-    append_collisions(w, dcd(w, 1.0 - remaining, dhat));
+    find_contacts(1.0 - remaining);
   }
+  w.replay = false;
   w.stats.outer = std::min(outer + 1, w.P.max_outer);
   w.stats.contacts = static_cast<int>(w.contacts.size());
   w.stats.colors = static_cast<int>(w.color_groups.size());
@@ -1044,6 +1130,82 @@ void oracle_set_obstacle_position(void* wp, int idx, const float* V) {
   o.is_static_twice = false;
   o.prev = o.curr;
   o.curr.assign(V, V + 3 * o.nv);
+}
+
+/// Obstacle idx moves from prev to curr over the next step (as after two
+/// position records; both 3 * nv floats).
+void oracle_set_obstacle_frame(void* wp, int idx, const float* prev, const float* curr) {
+  World& w = *static_cast<World*>(wp);
+  auto& o = w.obstacles[idx];
+  o.prev.assign(prev, prev + 3 * o.nv);
+  o.curr.assign(curr, curr + 3 * o.nv);
+  o.has_pos = true;
+  o.is_static = false;
+  o.is_static_twice = false;
+}
+
+/// Replay for the next step: the outer loop takes tois[k] as the CCD result
+/// of iteration k (frame-time units) and the DCD call c appends exactly the
+/// contacts listed with call index c (6 ints each: call, type, four global
+/// vertex ids in record order).
+void oracle_set_replay(void* wp, const double* tois, int n_toi, const int* contacts,
+                       int n_contacts) {
+  World& w = *static_cast<World*>(wp);
+  w.replay = true;
+  w.rp_toi.assign(tois, tois + n_toi);
+  w.rp_contacts.assign(contacts, contacts + 6 * static_cast<size_t>(n_contacts));
+}
+
+/// Decisions of the last step in the replay format; counts in n_toi and
+/// n_contacts (buffers may be null).  Returns the number of replayed
+/// contacts whose record could not be formed.
+int oracle_decisions(void* wp, double* tois, int cap_toi, int* n_toi, int* contacts,
+                     int cap_contacts, int* n_contacts) {
+  World& w = *static_cast<World*>(wp);
+  *n_toi = static_cast<int>(w.log_toi.size());
+  *n_contacts = static_cast<int>(w.log_contacts.size() / 6);
+  if (tois)
+    for (int i = 0; i < std::min(cap_toi, *n_toi); ++i) tois[i] = w.log_toi[i];
+  if (contacts)
+    for (size_t i = 0; i < 6 * static_cast<size_t>(std::min(cap_contacts, *n_contacts)); ++i)
+      contacts[i] = w.log_contacts[i];
+  return w.replay_missing;
+}
+
+/// DCD records of the current objects on explicit positions (global
+/// numbering, 3 per vertex per end) at `time`: 6 ints per record (type, four
+/// global vertex ids in record order, 0) and its distance.  Returns 0, or 1
+/// if cap was exceeded (the count is still exact).
+int oracle_dcd(void* wp, const double* pos0, const double* pos1, double time,
+               double dhat, int* recs, double* dist, int cap, int* n) {
+  World& w = *static_cast<World*>(wp);
+  int off = 0;
+  for (auto& c : w.cloths) {
+    c.off = off;
+    off += c.nv;
+  }
+  w.nv_cloth = off;
+  for (auto& o : w.obstacles) {
+    o.off = off;
+    off += o.nv;
+  }
+  w.nv_total = off;
+  w.pos_t0.assign(pos0, pos0 + 3 * static_cast<size_t>(off));
+  w.pos_t1.assign(pos1, pos1 + 3 * static_cast<size_t>(off));
+  const std::vector<oracle::CollisionRec> out = oracle::dcd(w, time, dhat);
+  *n = static_cast<int>(out.size());
+  for (int i = 0; i < std::min(cap, *n); ++i) {
+    const auto& r = out[i];
+    int* e = recs + 6 * static_cast<size_t>(i);
+    e[0] = r.type;
+    for (int k = 0; k < 4; ++k) e[1 + k] = r.gv[k];
+    e[5] = 0;
+    double d = 0.0;
+    for (int k = 0; k < 4; ++k)
+      for (int c = 0; c < 3; ++c) d += r.coef[k] * r.x[k][c] * r.normal[c];
+    dist[i] = d;
+  }
+  return *n > cap ? 1 : 0;
 }
 
 /// stats: outer, admm, pcg, contacts, colors, ccd candidates, min_toi.

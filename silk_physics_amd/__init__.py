@@ -72,6 +72,10 @@ def load_library():
         "silk_hip_debug_candidates": (C.c_int, [vp, fp, fp, i64, ip, i64,
                                                 C.POINTER(i64), ip, i64,
                                                 C.POINTER(i64)]),
+        "silk_hip_debug_dcd": (C.c_int, [vp, fp, fp, i64, C.c_float, C.c_float, ip,
+                                         fp, i64, C.POINTER(i64)]),
+        "silk_hip_debug_decisions": (C.c_int, [vp, fp, i32, ip, ip, i64,
+                                               C.POINTER(i64)]),
         "silk_hip_debug_ccd": (C.c_int, [vp, i64, ip, dp, dp, dp, C.c_double,
                                          C.c_double, i32, ip, dp]),
         "silk_hip_debug_coulomb": (C.c_int, [vp, i64, C.c_void_p, C.c_void_p,
@@ -79,6 +83,7 @@ def load_library():
         "silk_hip_debug_ccd_min": (C.c_int, [vp, i64, ip, dp, dp, dp, C.c_double,
                                              C.c_double, i32, dp]),
         "silk_hip_debug_pcg": (C.c_int, [vp, fp, i64, fp, i64, C.c_float, fp, ip]),
+        "silk_hip_debug_copy": (C.c_int, [vp, i64, i32, dp]),
         "silk_hip_set_warm_start_net": (C.c_int, [vp, fp, i64]),
         "silk_hip_get_warm_start_net": (C.c_int, [vp, fp, i64]),
         "silk_hip_warm_start_record": (C.c_int, [vp, i64, C.c_float]),
@@ -99,8 +104,9 @@ def load_library():
 # Values of GlobalConfig.mas_precision, .pcg_criterion and .warm_start
 # (include/silk_hip.h).
 MAS_FP32, MAS_FP16, MAS_BF16 = 0, 1, 2
-PCG_RR0, PCG_RHS = 0, 1
-WARM_PREVIOUS, WARM_LEARNED = 0, 1
+PCG_RR0, PCG_RHS, PCG_TRUE = 0, 1, 2
+WARM_PREVIOUS, WARM_LEARNED, WARM_EXTRAPOLATE = 0, 1, 2
+PROJECT_RESIDUAL, PROJECT_GALERKIN = 0, 1
 WARM_NET_PARAMS = 5313  # SILK_HIP_WARM_NET_PARAMS
 
 class SilkHipClothParams(C.Structure):
@@ -129,7 +135,7 @@ class SilkHipGlobal(C.Structure):
                 ("dcd_activation_scale", C.c_float),
                 ("mas_precision", C.c_int32), ("true_residual_stats", C.c_int32),
                 ("pcg_criterion", C.c_int32), ("warm_start", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("warm_projection", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class SilkHipStepStats(C.Structure):
@@ -142,7 +148,8 @@ class SilkHipStepStats(C.Structure):
                 ("max_committed_rel_residual", C.c_float),
                 ("pcg_max_solve_iterations", C.c_int32),
                 ("pcg_stalled_solves", C.c_int32), ("warm_start_kept", C.c_int32),
-                ("warm_start_solves", C.c_int32)]
+                ("warm_start_solves", C.c_int32), ("pcg_restarts", C.c_int32),
+                ("pcg_unverified_solves", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -236,8 +243,10 @@ class GlobalConfig:
     dcd_activation_scale: float = 0.0
     mas_precision: int = 0        # MAS_FP32 / MAS_FP16 / MAS_BF16
     true_residual_stats: int = 0
-    pcg_criterion: int = 0        # PCG_RR0 (reference) / PCG_RHS
-    warm_start: int = 0           # WARM_PREVIOUS (reference) / WARM_LEARNED
+    pcg_criterion: int = 0        # PCG_RR0 (reference) / PCG_RHS / PCG_TRUE
+    warm_start: int = 0           # WARM_PREVIOUS (reference) / WARM_LEARNED /
+                                  # WARM_EXTRAPOLATE
+    warm_projection: int = 0      # PROJECT_RESIDUAL / PROJECT_GALERKIN
 
     def _c(self) -> SilkHipGlobal:
         g = SilkHipGlobal()
@@ -251,7 +260,7 @@ class GlobalConfig:
                      "contact_sweeps", "ccd_max_iter", "ccd_tolerance",
                      "ccd_min_separation_scale", "dcd_activation_scale",
                      "mas_precision", "true_residual_stats", "pcg_criterion",
-                     "warm_start"):
+                     "warm_start", "warm_projection"):
             setattr(g, name, getattr(self, name))
         return g
 
@@ -489,6 +498,53 @@ class World:
         if st != 0:
             raise RuntimeError(self._wrap(st).to_string())
         return pt[:npt.value], ee[:nee.value]
+
+    def debug_copy(self, nbytes: int, reps: int = 1) -> float:
+        """Device copy of nbytes, reps times, with the library's copy kernel;
+        returns the elapsed device time (ms)."""
+        ms = C.c_double(0.0)
+        st = load_library().silk_hip_debug_copy(self._ctx, nbytes, reps, C.byref(ms))
+        if st != 0:
+            raise RuntimeError(self._wrap(st).to_string())
+        return ms.value
+
+    def debug_dcd(self, pos0, pos1, time, dhat):
+        """DCD of the current scene on explicit collider positions:
+        (records (n, 5): type, four global vertex ids in record order;
+        distances (n,))."""
+        a = np.ascontiguousarray(pos0, dtype=np.float32).reshape(-1)
+        b = np.ascontiguousarray(pos1, dtype=np.float32).reshape(-1)
+        lib = load_library()
+        n = C.c_int64(0)
+        st = lib.silk_hip_debug_dcd(self._ctx, _fptr(a), _fptr(b), a.size, time, dhat,
+                                    None, None, 0, C.byref(n))
+        if st != 0:
+            raise RuntimeError(self._wrap(st).to_string())
+        recs = np.zeros((max(1, n.value), 6), dtype=np.int32)
+        dist = np.zeros(max(1, n.value), dtype=np.float32)
+        st = lib.silk_hip_debug_dcd(self._ctx, _fptr(a), _fptr(b), a.size, time, dhat,
+                                    _iptr(recs), _fptr(dist), n.value, C.byref(n))
+        if st != 0:
+            raise RuntimeError(self._wrap(st).to_string())
+        return recs[:n.value, :5], dist[:n.value]
+
+    def decisions(self):
+        """Discrete decisions of the last step: (TOI of every outer
+        iteration, float32; appended contacts (n, 6): DCD call, type, four
+        global vertex ids in record order)."""
+        lib = load_library()
+        nt, nc = C.c_int32(0), C.c_int64(0)
+        st = lib.silk_hip_debug_decisions(self._ctx, None, 0, C.byref(nt), None, 0,
+                                          C.byref(nc))
+        if st != 0:
+            raise RuntimeError(self._wrap(st).to_string())
+        tois = np.zeros(max(1, nt.value), dtype=np.float32)
+        recs = np.zeros((max(1, nc.value), 6), dtype=np.int32)
+        st = lib.silk_hip_debug_decisions(self._ctx, _fptr(tois), nt.value, C.byref(nt),
+                                          _iptr(recs), nc.value, C.byref(nc))
+        if st != 0:
+            raise RuntimeError(self._wrap(st).to_string())
+        return tois[:nt.value], recs[:nc.value]
 
     def debug_ccd(self, is_vf, p0, p1, err, ms, tolerance, max_iter=4096):
         """The step's FP64 CCD kernel on explicit queries: (hit, toi)."""

@@ -11,6 +11,7 @@
 #include "silk_hip.h"
 
 using silk::amd::ClothEntity;
+using silk::amd::CollisionWorld;
 using silk::amd::Context;
 using silk::amd::ObstacleEntity;
 using silk::amd::TriMesh;
@@ -93,10 +94,16 @@ int silk_hip_set_global(silk_hip_ctx* ctx, const silk_hip_global* g) {
     const char* bad = nullptr;
     if (g->mas_precision < SILK_HIP_MAS_FP32 || g->mas_precision > SILK_HIP_MAS_BF16)
       bad = "mas_precision must be SILK_HIP_MAS_FP32, _FP16 or _BF16";
-    else if (g->pcg_criterion != SILK_HIP_PCG_RR0 && g->pcg_criterion != SILK_HIP_PCG_RHS)
-      bad = "pcg_criterion must be SILK_HIP_PCG_RR0 or SILK_HIP_PCG_RHS";
-    else if (g->warm_start != SILK_HIP_WARM_PREVIOUS && g->warm_start != SILK_HIP_WARM_LEARNED)
-      bad = "warm_start must be SILK_HIP_WARM_PREVIOUS or SILK_HIP_WARM_LEARNED";
+    else if (g->pcg_criterion != SILK_HIP_PCG_RR0 && g->pcg_criterion != SILK_HIP_PCG_RHS &&
+             g->pcg_criterion != SILK_HIP_PCG_TRUE)
+      bad = "pcg_criterion must be SILK_HIP_PCG_RR0, SILK_HIP_PCG_RHS or SILK_HIP_PCG_TRUE";
+    else if (g->warm_start != SILK_HIP_WARM_PREVIOUS && g->warm_start != SILK_HIP_WARM_LEARNED &&
+             g->warm_start != SILK_HIP_WARM_EXTRAPOLATE)
+      bad = "warm_start must be SILK_HIP_WARM_PREVIOUS, SILK_HIP_WARM_LEARNED or "
+            "SILK_HIP_WARM_EXTRAPOLATE";
+    else if (g->warm_projection != SILK_HIP_PROJECT_RESIDUAL &&
+             g->warm_projection != SILK_HIP_PROJECT_GALERKIN)
+      bad = "warm_projection must be SILK_HIP_PROJECT_RESIDUAL or SILK_HIP_PROJECT_GALERKIN";
     else if (g->warm_start == SILK_HIP_WARM_LEARNED && !c.has_warm_net())
       bad = "warm_start = SILK_HIP_WARM_LEARNED needs a network "
             "(silk_hip_set_warm_start_net)";
@@ -415,6 +422,52 @@ int silk_hip_debug_candidates(silk_hip_ctx* ctx, const float* pos0,
   });
 }
 
+int silk_hip_debug_dcd(silk_hip_ctx* ctx, const float* pos0, const float* pos1,
+                       int64_t n, float time, float dhat, int32_t* recs,
+                       float* dist, int64_t cap, int64_t* nrec) {
+  return guarded(ctx, [&](Context& c) {
+    if (nrec == nullptr || !(time >= 0.0f && time <= 1.0f) || !(dhat > 0.0f))
+      return int{SILK_HIP_ERR_INVALID_CONFIG};
+    if (c.cloths_.empty()) return int{SILK_HIP_ERR_NEED_INIT_SOLVER};
+    if (c.scene_dirty_ || !c.scene_) c.build_scene();
+    auto* cw = c.scene_->coll.get();
+    if (cw == nullptr) {
+      *nrec = 0;
+      return int{SILK_HIP_OK};
+    }
+    if (pos0 == nullptr || pos1 == nullptr ||
+        n != 3 * static_cast<int64_t>(cw->total_vertices()))
+      return int{SILK_HIP_ERR_INVALID_POSITION};
+    std::vector<int> vr;
+    std::vector<float> vd;
+    cw->debug_dcd(pos0, pos1, time, dhat, vr, vd);
+    *nrec = static_cast<int64_t>(vd.size());
+    const int64_t m = std::min<int64_t>(cap, *nrec);
+    if (recs != nullptr) std::copy_n(vr.begin(), 6 * m, recs);
+    if (dist != nullptr) std::copy_n(vd.begin(), m, dist);
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_debug_decisions(silk_hip_ctx* ctx, float* tois, int32_t cap_toi,
+                             int32_t* n_toi, int32_t* contacts, int64_t cap_contacts,
+                             int64_t* n_contacts) {
+  return guarded(ctx, [&](Context& c) {
+    if (n_toi == nullptr || n_contacts == nullptr) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    const std::vector<float>& tl = c.toi_log_;
+    *n_toi = static_cast<int32_t>(tl.size());
+    if (tois != nullptr)
+      std::copy_n(tl.begin(), std::min<int64_t>(cap_toi, *n_toi), tois);
+    const CollisionWorld* cw = c.scene_ ? c.scene_->coll.get() : nullptr;
+    static const std::vector<int> none;
+    const std::vector<int>& log = cw != nullptr ? cw->decision_log() : none;
+    *n_contacts = static_cast<int64_t>(log.size() / 6);
+    if (contacts != nullptr)
+      std::copy_n(log.begin(), 6 * std::min<int64_t>(cap_contacts, *n_contacts), contacts);
+    return int{SILK_HIP_OK};
+  });
+}
+
 int silk_hip_debug_ccd(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
                        const double* p0, const double* p1, const double* err,
                        double ms, double tolerance, int32_t max_iter,
@@ -512,6 +565,35 @@ int silk_hip_debug_coulomb(silk_hip_ctx* ctx, int64_t n, const float* u,
                                         hipMemcpyDeviceToHost, c.stream_),
                          "coulomb out");
     silk::amd::hip_check(hipStreamSynchronize(c.stream_), "coulomb sync");
+    return int{SILK_HIP_OK};
+  });
+}
+
+int silk_hip_debug_copy(silk_hip_ctx* ctx, int64_t bytes, int32_t reps, double* ms) {
+  return guarded(ctx, [&](Context& c) {
+    if (bytes < 4 || bytes > (int64_t{1} << 32) || reps < 1 || reps > 100)
+      return int{SILK_HIP_ERR_INVALID_CONFIG};
+    c.ensure_device();
+    using silk::amd::DevBuf;
+    const int64_t n = bytes / 4;
+    DevBuf<float> src, dst;
+    src.resize(static_cast<size_t>(n));
+    dst.resize(static_cast<size_t>(n));
+    silk::amd::hip_check(hipMemsetAsync(src.get(), 0, static_cast<size_t>(n) * 4, c.stream_),
+                         "copy init");
+    hipEvent_t e0, e1;
+    silk::amd::hip_check(hipEventCreate(&e0), "event");
+    silk::amd::hip_check(hipEventCreate(&e1), "event");
+    silk::amd::hip_check(hipEventRecord(e0, c.stream_), "event");
+    for (int r = 0; r < reps; ++r)
+      silk::amd::launch_copy_unless(c.stream_, dst.get(), src.get(), n, nullptr);
+    silk::amd::hip_check(hipEventRecord(e1, c.stream_), "event");
+    silk::amd::hip_check(hipStreamSynchronize(c.stream_), "copy sync");
+    float el = 0.0f;
+    silk::amd::hip_check(hipEventElapsedTime(&el, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms != nullptr) *ms = el;
     return int{SILK_HIP_OK};
   });
 }

@@ -160,6 +160,9 @@ class HipCollision final : public CollisionWorld {
   void debug_candidates(const float* pos0, const float* pos1,
                         std::vector<int>& pt, std::vector<int>& ee) override;
   int total_vertices() const override { return nvt_; }
+  void debug_dcd(const float* pos0, const float* pos1, float time, float dhat,
+                 std::vector<int>& recs, std::vector<float>& dist) override;
+  const std::vector<int>& decision_log() const override { return log_; }
 
   // Static tables (filled by make_collision_world).
   std::vector<int> h_tri, h_edge, h_tri_obj, h_edge_obj, h_point_obj;
@@ -201,6 +204,8 @@ class HipCollision final : public CollisionWorld {
   int* h_io_ = nullptr;  // pinned: counters[4] then toi bits (8 bytes)
 
   // contacts of the step
+  int dcd_call_ = 0;                     // DCD calls so far in this step
+  std::vector<int> log_;                 // appended contacts (decision_log)
   std::vector<HostContact> contacts_;
   std::unordered_set<std::array<int, 5>, KeyHash> keys_;
   std::vector<int> color_;               // per contact, from 1
@@ -556,8 +561,10 @@ void HipCollision::append(std::vector<DcdRec> recs) {
     return contact_key(a) < contact_key(b);
   });
   const int n_old = static_cast<int>(contacts_.size());
+  const int call = dcd_call_++;
   for (const DcdRec& r : recs) {
     if (!keys_.insert(contact_key(r)).second) continue;
+    log_.insert(log_.end(), {call, r.type, r.gv[0], r.gv[1], r.gv[2], r.gv[3]});
     HostContact c;
     c.mu = r.friction;
     double ref[3] = {0.0, 0.0, 0.0};
@@ -740,6 +747,8 @@ void HipCollision::begin_step(const float* outer_x, float* ms, float* dhat,
   // The contact set restarts every step.
   contacts_.clear();
   keys_.clear();
+  log_.clear();
+  dcd_call_ = 0;
   color_off_h_.clear();
   color_list_h_.clear();
   color_.clear();
@@ -857,6 +866,30 @@ void HipCollision::end_step(silk_hip_step_stats* stats) {
   stats->contacts = static_cast<int>(contacts_.size());
   stats->contact_colors =
       color_off_h_.empty() ? 0 : static_cast<int>(color_off_h_.size()) - 1;
+}
+
+void HipCollision::debug_dcd(const float* pos0, const float* pos1, float time,
+                             float dhat, std::vector<int>& recs,
+                             std::vector<float>& dist) {
+  const size_t bytes = 3 * static_cast<size_t>(nvt_) * sizeof(float);
+  hip_check(hipMemcpyAsync(p0_.get(), pos0, bytes, hipMemcpyHostToDevice, st_),
+            "debug t0");
+  hip_check(hipMemcpyAsync(p1_.get(), pos1, bytes, hipMemcpyHostToDevice, st_),
+            "debug t1");
+  refit();
+  const std::vector<DcdRec> out = dcd(time, dhat, nullptr);
+  recs.clear();
+  dist.clear();
+  for (const DcdRec& r : out) {
+    recs.insert(recs.end(), {r.type, r.gv[0], r.gv[1], r.gv[2], r.gv[3], 0});
+    // Distance along the normal of the closest points: sum_k coef_k x_k
+    // projected on the normal (the coefficients carry the signs).
+    double d = 0.0;
+    for (int k = 0; k < 4; ++k)
+      for (int c = 0; c < 3; ++c)
+        d += static_cast<double>(r.coef[k]) * r.x[k][c] * r.normal[c];
+    dist.push_back(static_cast<float>(d));
+  }
 }
 
 void HipCollision::debug_candidates(const float* pos0, const float* pos1,

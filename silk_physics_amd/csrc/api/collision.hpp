@@ -50,6 +50,16 @@ class CollisionWorld {
   virtual void debug_candidates(const float* pos0, const float* pos1,
                                 std::vector<int>& pt, std::vector<int>& ee) = 0;
   virtual int total_vertices() const = 0;
+  /// DCD on explicit host positions (3 * total collider vertices per end) at
+  /// `time` with activation distance dhat: 6 ints per record (type, then the
+  /// four global vertex ids in the record's order) and its distance.
+  virtual void debug_dcd(const float* pos0, const float* pos1, float time,
+                         float dhat, std::vector<int>& recs,
+                         std::vector<float>& dist) = 0;
+  /// Contacts appended during the last step, 6 ints each: the DCD call that
+  /// found it (0 at the frame start, k after the k-th rollback), the type and
+  /// the four global vertex ids in the record's order.
+  virtual const std::vector<int>& decision_log() const = 0;
 };
 
 /// nullptr when no object has collision enabled.

@@ -687,6 +687,9 @@ void Context::build_scene() {
   S.bank_inv_h.resize(std::max<size_t>(1, bank_static.size()));
   S.bank_scale.resize(std::max<size_t>(1, bank_sys.size()));
   S.x0_copy.resize(3 * static_cast<size_t>(nv));
+  S.x0r.resize(3 * static_cast<size_t>(nv));
+  S.dxr.resize(3 * static_cast<size_t>(nv));
+  S.sys_done.resize(static_cast<size_t>(std::max(1, S.nsys)));
   if (cnode_list.empty()) cnode_list.push_back(0);
   S.cnode_ptr.upload(cnode_ptr, st);
   S.cnode_list.upload(cnode_list, st);
@@ -881,18 +884,19 @@ static PersistDev persist_args(Scene& S, const silk_hip_global& g,
   a.max_iter = max_iter;
   a.criterion = g.pcg_criterion;
   a.mas_half = g.mas_precision;
-  a.dx_out = g.true_residual_stats ? S.dx.get() : nullptr;
+  a.dx_out = g.true_residual_stats || g.pcg_criterion == SILK_HIP_PCG_TRUE
+                 ? S.dx.get() : nullptr;
   return a;
 }
 
 int Context::pcg_solve(float rel_tol, bool factor, const RhsArgs& ra,
-                       silk_hip_step_stats* stats) {
+                       silk_hip_step_stats* stats, bool restart) {
   Scene& S = *scene_;
   PcgDev p = pcg_dev();
   launch_pcg_residual(stream_, p, ra);
   if (factor) launch_mas_factor(stream_, p, S.cd.get());
   const size_t n3 = 3 * static_cast<size_t>(S.nv);
-  if (global_.true_residual_stats)
+  if (global_.true_residual_stats || global_.pcg_criterion == SILK_HIP_PCG_TRUE)
     hip_check(hipMemcpyAsync(S.x0_copy.get(), S.inner_x.get(), n3 * sizeof(float),
                              hipMemcpyDeviceToDevice, stream_),
               "warm start copy");
@@ -929,7 +933,37 @@ int Context::pcg_solve(float rel_tol, bool factor, const RhsArgs& ra,
       stats->pcg_stalled_solves += s.status == 2 ? 1 : 0;
     }
   }
-  return finish_pcg(iters, rel, ra, stats);
+  return finish_pcg(iters, rel, ra, stats, restart);
+}
+
+void Context::verify_host(float rel_tol, const RhsArgs& ra, silk_hip_step_stats* stats) {
+  Scene& S = *scene_;
+  PcgDev p = pcg_dev();
+  AdmmCtl* ctl = S.admm_ctl.get();
+  for (int k = 0;; ++k) {
+    const bool final = k == kPcgRestarts;
+    launch_admm_init(stream_, ctl, rel_tol, rel_tol, rel_tol);
+    launch_iterate_verify(stream_, p, ra, S.x0_copy.get(), S.dx.get(), S.part3.get(),
+                          S.sys_done.get(), ctl, rel_tol, 1, final ? 1 : 0);
+    hip_check(hipMemcpyAsync(S.h_admm_ctl, ctl, sizeof(AdmmCtl),
+                             hipMemcpyDeviceToHost, stream_),
+              "verify counters");
+    hip_check(hipStreamSynchronize(stream_), "verify sync");
+    const AdmmCtl hc = S.h_admm_ctl[0];
+    if (hc.verify_done != 0 || final) {
+      if (stats) {
+        double it2;
+        std::memcpy(&it2, &hc.max_true_bits, sizeof(double));
+        stats->max_true_rel_residual =
+            std::max(stats->max_true_rel_residual, static_cast<float>(std::sqrt(it2)));
+        stats->pcg_unverified_solves += hc.pcg_unverified;
+      }
+      return;
+    }
+    // Every system runs again from its committed x (the multi-launch PCG
+    // has no per-system skip); the next round tests the restart's iterate.
+    pcg_solve(rel_tol, false, ra, stats, true);
+  }
 }
 
 void Context::pcg_iterate(float rel_tol, const RhsArgs& ra) {
@@ -958,12 +992,14 @@ void Context::pcg_iterate(float rel_tol, const RhsArgs& ra) {
 }
 
 int Context::finish_pcg(int iters, double rel, const RhsArgs& ra,
-                        silk_hip_step_stats* stats) {
+                        silk_hip_step_stats* stats, bool restart) {
   Scene& S = *scene_;
   PcgDev p = pcg_dev();
+  const bool verify = global_.pcg_criterion == SILK_HIP_PCG_TRUE;
   if (stats) {
     stats->pcg_iterations += iters;
-    stats->pcg_solves += 1;
+    if (restart) stats->pcg_restarts += S.nsys;
+    else stats->pcg_solves += 1;
     stats->last_pcg_rel_residual = static_cast<float>(rel);
     if (global_.true_residual_stats) {
       // x0 was copied before the solve; dx is the solve's correction.
@@ -972,7 +1008,7 @@ int Context::finish_pcg(int iters, double rel, const RhsArgs& ra,
       RhsArgs rv = ra;
       rv.skip = nullptr;
       launch_solve_residuals(stream_, p, rv, S.x0_copy.get(), S.dx.get(),
-                             S.inner_x.get(), S.part3.get(), ctl);
+                             S.inner_x.get(), S.part3.get(), ctl, verify ? 0 : 1);
       hip_check(hipMemcpyAsync(S.h_admm_ctl, ctl, sizeof(AdmmCtl),
                                hipMemcpyDeviceToHost, stream_),
                 "solve residuals");
@@ -1021,8 +1057,11 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
   // Learned warm start, or recording of training samples (from solves that
   // start at the previous iterate); both need each solve's correction.
   const bool learned = g.warm_start == SILK_HIP_WARM_LEARNED && has_warm_net();
-  const bool record = recording() && !learned;
-  if (learned || record) {
+  const bool extrap = g.warm_start == SILK_HIP_WARM_EXTRAPOLATE;
+  const bool projected = learned || extrap;
+  const bool record = recording() && g.warm_start == SILK_HIP_WARM_PREVIOUS;
+  const bool verify = g.pcg_criterion == SILK_HIP_PCG_TRUE;
+  if (projected || record) {
     pa.dx_out = S.dx.get();
     const size_t ns = n3;
     if (S.wfeat.size() < ns * kMlpIn) {
@@ -1032,6 +1071,8 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
       S.w_xprev.resize(ns);
       S.w_rprev.resize(ns);
       S.w_part2.resize(2 * static_cast<size_t>(std::max(1, S.nblk)));
+      S.w_gram.resize(30 * static_cast<size_t>(std::max(1, S.nblk)));
+      S.w_coef.resize(6 * static_cast<size_t>(std::max(1, S.nsys)));
     }
     // The first solve of a segment has no previous correction.
     hip_check(hipMemsetAsync(S.dx.get(), 0, n3 * sizeof(float), stream_), "dx reset");
@@ -1059,9 +1100,10 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
       timed_begin(kTimeResidual);
       launch_pcg_residual(stream_, p, ra);
       timed_end();
-      if (learned) {
-        // Keep the previous iterate and its residual, predict the start, and
-        // let the judge keep whichever of the two has the smaller residual.
+      if (projected) {
+        // Keep the previous iterate and its residual, project the start onto
+        // span{network prediction, previous correction}, and let the judge
+        // keep whichever of the two starts has the smaller FP64 residual.
         hip_check(hipMemcpyAsync(S.w_xprev.get(), S.inner_x.get(), n3 * sizeof(float),
                                  hipMemcpyDeviceToDevice, stream_),
                   "warm x copy");
@@ -1073,10 +1115,16 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
                                  hipMemcpyDeviceToDevice, stream_),
                   "warm partials copy");
         timed_begin(kTimeWarm);
-        launch_warm_features(stream_, p, wf, S.wfeat.get(), S.wsig.get());
-        launch_mlp_predict(stream_, warm_.dev(), S.wfeat.get(), S.wsig.get(),
-                           static_cast<int64_t>(n3), S.inner_x.get(), S.wdelta.get(),
-                           skip);
+        const float* v1 = nullptr;
+        if (learned) {
+          launch_warm_features(stream_, p, wf, S.wfeat.get(), S.wsig.get());
+          launch_mlp_predict(stream_, warm_.dev(), S.wfeat.get(), S.wsig.get(),
+                             static_cast<int64_t>(n3), nullptr, S.wdelta.get(), skip);
+          v1 = S.wdelta.get();
+        }
+        launch_warm_project(stream_, p, v1, S.dx.get(), S.r.get(),
+                            g.warm_projection == SILK_HIP_PROJECT_GALERKIN ? 0 : 1,
+                            S.w_gram.get(), S.w_coef.get(), S.inner_x.get(), skip);
         timed_end();
         timed_begin(kTimeResidual);
         launch_pcg_residual(stream_, p, ra);
@@ -1087,7 +1135,7 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
         launch_warm_features(stream_, p, wf, S.wfeat.get(), S.wsig.get());
       }
       if (it == 1) launch_mas_factor(stream_, p, S.cd.get(), skip);
-      if (g.true_residual_stats)
+      if (g.true_residual_stats || verify)
         hip_check(hipMemcpyAsync(S.x0_copy.get(), S.inner_x.get(), n3 * sizeof(float),
                                  hipMemcpyDeviceToDevice, stream_),
                   "warm start copy");
@@ -1095,14 +1143,46 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
       hip_check(launch_pcg_persistent(stream_, pa, 0.0f),
                 "persistent PCG launch");
       timed_end();
-      if (g.true_residual_stats)
+      if (g.true_residual_stats && !verify)
         launch_solve_residuals(stream_, p, ra, S.x0_copy.get(), S.dx.get(),
-                               S.inner_x.get(), S.part3.get(), ctl);
+                               S.inner_x.get(), S.part3.get(), ctl, 1);
       if (record)
         launch_warm_record(stream_, S.wfeat.get(), S.wsig.get(), S.dx.get(),
                            static_cast<int64_t>(n3), warm_.ds_rate, warm_.ds_salt++,
                            warm_.ds_feat.get(), warm_.ds_target.get(),
                            warm_.ds_count.get(), warm_.ds_cap, skip);
+      if (verify) {
+        // True-residual test of the iterate; systems above rel |b| restart
+        // from the committed x (the restart's own iterate x + dx' is tested
+        // again).  Every kernel of a round is a no-op once all pass.
+        launch_iterate_verify(stream_, p, ra, S.x0_copy.get(), S.dx.get(),
+                              S.part3.get(), S.sys_done.get(), ctl, 0.0f, 1,
+                              kPcgRestarts == 0);
+        for (int k = 0; k < kPcgRestarts; ++k) {
+          const int* vskip = &ctl->verify_done;
+          launch_copy_unless(stream_, S.x0r.get(), S.inner_x.get(),
+                             static_cast<int64_t>(n3), vskip);
+          RhsArgs rr = ra;
+          rr.skip = vskip;
+          timed_begin(kTimeResidual);
+          launch_pcg_residual(stream_, p, rr);
+          timed_end();
+          PersistDev pr = pa;
+          pr.sys_done = S.sys_done.get();
+          pr.restart = 1;
+          pr.dx_out = S.dxr.get();
+          timed_begin(kTimePcg);
+          hip_check(launch_pcg_persistent(stream_, pr, 0.0f), "persistent PCG restart");
+          timed_end();
+          launch_iterate_verify(stream_, p, ra, S.x0r.get(), S.dxr.get(), S.part3.get(),
+                                S.sys_done.get(), ctl, 0.0f, 0, k == kPcgRestarts - 1);
+        }
+        // The stored FP32 x after the restarts (the iterate's residual was
+        // folded by the verification).
+        if (g.true_residual_stats)
+          launch_solve_residuals(stream_, p, ra, S.x0_copy.get(), S.dx.get(),
+                                 S.inner_x.get(), S.part3.get(), ctl, 0);
+      }
     }
     timed_begin(kTimeContact);
     if (S.coll) S.coll->contact_update(S.inner_x.get(), skip);
@@ -1155,6 +1235,14 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
     stats->pcg_stalled_solves += hc.pcg_stalled;
     stats->warm_start_kept += hc.warm_kept;
     stats->warm_start_solves += hc.warm_solves;
+    stats->pcg_restarts += hc.pcg_restarts;
+    stats->pcg_unverified_solves += hc.pcg_unverified;
+    if (verify && !g.true_residual_stats) {
+      double it2;
+      std::memcpy(&it2, &hc.max_true_bits, sizeof(double));
+      stats->max_true_rel_residual =
+          std::max(stats->max_true_rel_residual, static_cast<float>(std::sqrt(it2)));
+    }
     if (g.true_residual_stats) {
       double it2, cm2;
       std::memcpy(&it2, &hc.max_true_bits, sizeof(double));
@@ -1195,6 +1283,7 @@ bool Context::admm_solve(silk_hip_step_stats* stats) {
           static_cast<float>(g.initial_linear_rel_tol * ratio),
           g.linear_solver_rel_tol_min, g.linear_solver_rel_tol_max);
       pcg_solve(rel, it == 1, ra, stats);
+      if (g.pcg_criterion == SILK_HIP_PCG_TRUE) verify_host(rel, ra, stats);
     }
     if (S.coll) S.coll->contact_update(S.inner_x.get(), nullptr);
     LocalStepArgs a;
@@ -1268,6 +1357,7 @@ int Context::step(silk_hip_step_stats* stats) {
 
   float remaining = 1.0f;
   float accepted_toi = 1.0f;
+  toi_log_.clear();
   int outer = 0;
   for (; outer < global_.max_outer_iteration; ++outer) {
     launch_predict(stream_, c, S.outer_x.get(), S.outer_v.get(),
@@ -1280,6 +1370,7 @@ int Context::step(silk_hip_step_stats* stats) {
     if (S.coll) {
       min_toi = S.coll->min_toi(S.outer_x.get(), S.inner_x.get(),
                                 1.0f - remaining, remaining, ms, stats);
+      toi_log_.push_back(min_toi);
     }
     launch_velocity(stream_, static_cast<int>(n3), S.dt, S.outer_x.get(),
                     S.inner_x.get(), S.outer_v.get());

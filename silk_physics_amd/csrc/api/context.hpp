@@ -34,6 +34,11 @@ enum : int {
 };
 constexpr int kTimeClasses = 9;
 
+/// Restarts of a PCG solve under the true-residual criterion (the FP32
+/// recurrence drifts from the FP64 residual by a small factor, so one or two
+/// short restarts close the gap).
+constexpr int kPcgRestarts = 3;
+
 struct ClothEntity {
   uint32_t handle = 0;
   silk_hip_cloth_params cloth{};
@@ -94,12 +99,16 @@ struct Scene {
   DevBuf<float> ell_val, bank_static, bank_inv, bank_scale, r, p, q, z_pcg, zl0,
       cr, cz, cd;
   DevBuf<unsigned short> bank_inv_h;  // 16-bit bank inverses (mas_precision)
-  DevBuf<float> x0_copy;    // 3*nv: warm start of the last solve (stats)
+  DevBuf<float> x0_copy;    // 3*nv: start of the last solve (x0 of x0 + dx)
+  DevBuf<float> x0r, dxr;   // 3*nv: start and correction of a restart
+  DevBuf<int> sys_done;     // per system: the true-residual test passed
   DevBuf<double> part3;     // 3 per row block: solve residual partials
   // ---- learned warm start (allocated when used)
   DevBuf<unsigned short> wfeat;  // 3*nv samples x 16 features
   DevBuf<float> wsig, wdelta, w_xprev, w_rprev;  // 3*nv
   DevBuf<double> w_part2;        // residual partials of the previous iterate
+  DevBuf<double> w_gram;         // 30 per row block: projection sums
+  DevBuf<float> w_coef;          // 6 per system: projection coefficients
   DevBuf<double> part, part2;
   std::vector<double> h_part2;
   DevBuf<PcgState> state;
@@ -139,6 +148,9 @@ class Context {
   std::vector<ClothEntity> cloths_;
   std::vector<ObstacleEntity> obstacles_;
   uint64_t rng_state_ = 0x5117C0DE2024ULL;
+  /// TOI (frame-time units) returned by the CCD of every outer iteration of
+  /// the last step, in order.
+  std::vector<float> toi_log_;
 
   /// Learned warm start: network (FP32 master, Adam moments, BF16 copies),
   /// training scratch and the recorded samples.  Device buffers of the
@@ -200,12 +212,16 @@ class Context {
   void build_scene();
   int step(silk_hip_step_stats* stats);
   int pcg_solve(float rel_tol, bool factor, const RhsArgs& ra,
-                silk_hip_step_stats* st);
+                silk_hip_step_stats* st, bool restart = false);
+  /// True-residual criterion on the host-driven ADMM path: tests the FP64
+  /// residual of the last solve's iterate and restarts the solve from the
+  /// committed x while it is above rel_tol |b| (at most kPcgRestarts times).
+  void verify_host(float rel_tol, const RhsArgs& ra, silk_hip_step_stats* stats);
   /// Multi-launch PCG iterations (used when the persistent kernel cannot
   /// keep its grid resident); leaves per-system states in scene h_state.
   void pcg_iterate(float rel_tol, const RhsArgs& ra);
   int finish_pcg(int iters, double rel, const RhsArgs& ra,
-                 silk_hip_step_stats* st);
+                 silk_hip_step_stats* st, bool restart = false);
   bool admm_solve(silk_hip_step_stats* st);
   bool admm_solve_device(silk_hip_step_stats* st);
   void get_positions(const ClothEntity& c, float* out) const;

@@ -89,7 +89,11 @@ struct AdmmCtl {
   int pcg_stalled = 0;    // solves that ended at max_iter
   int warm_kept = 0;      // learned warm starts kept by the judge
   int warm_solves = 0;    // learned warm starts evaluated
-  int pad = 0;
+  int pcg_restarts = 0;   // true-residual criterion: restarts of a system
+  int verify_done = 0;    // true-residual criterion: every system's iterate
+                          // meets the tolerance (or the ADMM loop stopped)
+  int pcg_unverified = 0; // solves still above the tolerance after the last
+                          // restart
   float rel_tol = 0;  // linear tolerance of the next x-update
   float last_rel = 0; // sqrt(rr / rr0) of the last PCG solve
   double init_primal = 0, init_dual = 0, ratio = 1;
@@ -183,7 +187,10 @@ struct PcgDev {
   PcgState* state = nullptr; // 2*nsys
   float abs_tol = 0.0f;
   int max_iter = 1000;
-  int criterion = 0;         // 0: rr <= rel^2 rr0; 1: rr <= rel^2 |b|^2
+  int criterion = 0;         // 0: rr <= rel^2 rr0; 1: rr <= rel^2 |b|^2;
+                             // 2: as 1, then the FP64 residual of the iterate
+                             // is checked and the solve restarted while it
+                             // is above rel |b|
   int mas_half = 0;          // bank inverses read from bank_inv_h (0 = fp32,
                              // 1 = fp16, 2 = bf16), written by the factor
   unsigned short* bank_inv_h = nullptr;  // 1024 per bank
@@ -227,11 +234,32 @@ void launch_true_residual(hipStream_t s, const PcgDev& p, const RhsArgs& ra);
 
 /// FP64 residuals of a finished solve (b from ra): the iterate x0 + dx and
 /// the stored xc, relative to |b|, per system; their squares are folded into
-/// ctl->max_true_bits / max_commit_bits.  part3: 3 doubles per row block.
+/// ctl->max_true_bits (when fold_iterate) / max_commit_bits.  part3: 3
+/// doubles per row block.
 /// No-op when *ra.skip != 0.
 void launch_solve_residuals(hipStream_t s, const PcgDev& p, const RhsArgs& ra,
                             const float* x0, const float* dx, const float* xc,
-                            double* part3, AdmmCtl* ctl);
+                            double* part3, AdmmCtl* ctl, int fold_iterate);
+
+/// True-residual stopping test of a finished solve: per system, the FP64
+/// residual of the iterate x0 + dx (never rounded to FP32) against
+/// max(abs_tol, rel * |b|), rel = ctl->rel_tol (or `rel_tol` when
+/// ctl->rel_tol is 0).  Writes sys_done[sys] and ctl->verify_done (1 when
+/// every system passes or ctl->done is set).  Systems that pass, and with
+/// `final` set every system, fold their ratio into ctl->max_true_bits;
+/// with `final` set, failing systems are counted in ctl->pcg_unverified.
+/// part3: 2 doubles per row block.  No-op when ctl->done is set (then only
+/// verify_done is written).
+/// With `first` unset, systems whose sys_done is already set keep it and are
+/// not evaluated again (their dx was not written by the restart).
+void launch_iterate_verify(hipStream_t s, const PcgDev& p, const RhsArgs& ra,
+                           const float* x0, const float* dx, double* part3,
+                           int* sys_done, AdmmCtl* ctl, float rel_tol, int first,
+                           int final);
+
+/// dst = src (n floats); no-op when *skip != 0.
+void launch_copy_unless(hipStream_t s, float* dst, const float* src, int64_t n,
+                        const int* skip);
 
 /// x0 += dx.
 void launch_pcg_commit(hipStream_t s, const PcgDev& p);
@@ -317,9 +345,13 @@ struct PersistDev {
   const double* part2 = nullptr;    // per row block: (|r0|^2, |b|^2) from the
                                     //   residual kernel
   float* dx_out = nullptr;          // optional: the solve's dx (rows)
+  const int* sys_done = nullptr;    // optional, per system: nonzero = already
+                                    //   converged, the launch leaves it alone
   float abs_tol = 0.0f;
   int max_iter = 1000;
-  int criterion = 0;                // 0: rr <= rel^2 rr0; 1: rr <= rel^2 |b|^2
+  int criterion = 0;                // 0: rr <= rel^2 rr0; 1 and 2: rr <= rel^2 |b|^2
+  int restart = 0;                  // 1: a restart of unconverged systems
+                                    //   (counted in ctl->pcg_restarts)
   int need_max = 0;     // max level-1 nodes listed for one workgroup (LDS)
   int ncn_max = 0;      // max coarse nodes of a system (LDS)
   int ncb_max = 0;      // max coarse banks of a system (LDS)

@@ -196,7 +196,8 @@ __global__ void __launch_bounds__(kBlock) k_solve_residuals(
 
 // Per system: the ratios of the block sums, folded into ctl's maxima.
 __global__ void __launch_bounds__(kBlock) k_solve_residual_max(
-    PcgDev p, const double* __restrict__ part3, const int* skip, AdmmCtl* ctl) {
+    PcgDev p, const double* __restrict__ part3, const int* skip, AdmmCtl* ctl,
+    int fold_iterate) {
   __shared__ double lds[17];
   if (skip != nullptr && *skip != 0) return;
   const int sys = blockIdx.x;
@@ -209,9 +210,91 @@ __global__ void __launch_bounds__(kBlock) k_solve_residual_max(
   }
   if (threadIdx.x == 0 && s[2] > 0.0) {
     const double it = s[0] / s[2], cm = s[1] / s[2];
-    atomicMax(&ctl->max_true_bits, static_cast<unsigned long long>(__double_as_longlong(it)));
+    if (fold_iterate)
+      atomicMax(&ctl->max_true_bits, static_cast<unsigned long long>(__double_as_longlong(it)));
     atomicMax(&ctl->max_commit_bits, static_cast<unsigned long long>(__double_as_longlong(cm)));
   }
+}
+
+// FP64 residual of the iterate x0 + dx per row block: (|b - A (x0 + dx)|^2,
+// |b|^2), x0 and dx applied separately as in k_solve_residuals.
+__global__ void __launch_bounds__(kBlock) k_iterate_residual(
+    PcgDev p, RhsArgs ra, const float* __restrict__ x0,
+    const float* __restrict__ dx, double* __restrict__ part2, const AdmmCtl* ctl,
+    const int* sys_done) {
+  __shared__ double lds[17];
+  if (ctl != nullptr && ctl->done != 0) return;
+  const int b = blockIdx.x;
+  if (sys_done != nullptr && sys_done[p.blk_sys[b]] != 0) return;
+  const int row = p.blk_row0[b] + threadIdx.x;
+  const bool valid = threadIdx.x < p.blk_rows[b];
+  double ri = 0.0, bb = 0.0;
+  if (valid) {
+    double rhs[3], dd;
+    rhs_row(ra, row, rhs, dd);
+    double a0[3], a1[3];
+    kx_row(p, b, row, x0, a0);
+    kx_row(p, b, row, dx, a1);
+    for (int k = 0; k < 3; ++k) {
+      const double e = rhs[k] - (a0[k] + dd * x0[3 * row + k]) -
+                       (a1[k] + dd * dx[3 * row + k]);
+      ri += e * e;
+      bb += rhs[k] * rhs[k];
+    }
+  }
+  const double s0 = block_sum(ri, lds);
+  const double s1 = block_sum(bb, lds);
+  if (threadIdx.x == 0) {
+    part2[2 * b] = s0;
+    part2[2 * b + 1] = s1;
+  }
+}
+
+// One workgroup: every system's test, then the all-systems flag.
+__global__ void __launch_bounds__(kBlock) k_iterate_verify(
+    PcgDev p, const double* __restrict__ part2, int* sys_done, AdmmCtl* ctl,
+    float rel_tol, int first, int final) {
+  __shared__ double lds[17];
+  __shared__ int all_ok;
+  if (ctl->done != 0) {
+    if (threadIdx.x == 0) ctl->verify_done = 1;
+    return;
+  }
+  const float rel = ctl->rel_tol > 0.0f ? ctl->rel_tol : rel_tol;
+  const double rel2 = static_cast<double>(rel) * rel;
+  const double abs2 = static_cast<double>(p.abs_tol) * p.abs_tol;
+  if (threadIdx.x == 0) all_ok = 1;
+  __syncthreads();
+  for (int sys = 0; sys < p.nsys; ++sys) {
+    if (!first && sys_done[sys] != 0) continue;  // same value in every thread
+    const int b0 = p.sys_blk0[sys], b1 = p.sys_blk0[sys + 1];
+    double v0 = 0.0, v1 = 0.0;
+    for (int q = b0 + threadIdx.x; q < b1; q += blockDim.x) {
+      v0 += part2[2 * q];
+      v1 += part2[2 * q + 1];
+    }
+    const double rr = block_sum(v0, lds);
+    const double bb = block_sum(v1, lds);
+    if (threadIdx.x == 0) {
+      const bool ok = rr <= abs2 || rr <= rel2 * bb;
+      sys_done[sys] = ok ? 1 : 0;
+      if (!ok) all_ok = 0;
+      if ((ok || final) && bb > 0.0)
+        atomicMax(&ctl->max_true_bits,
+                  static_cast<unsigned long long>(__double_as_longlong(rr / bb)));
+      if (!ok && final) ctl->pcg_unverified += 1;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ctl->verify_done = all_ok;
+}
+
+__global__ void k_copy_unless(float* __restrict__ dst, const float* __restrict__ src,
+                              int64_t n, const int* skip) {
+  if (skip != nullptr && *skip != 0) return;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
 }
 
 __global__ void k_commit(int n3, float* __restrict__ x,
@@ -459,7 +542,7 @@ __global__ void __launch_bounds__(kBlock) k_init_final(PcgDev p, float rel_tol) 
     // iteration.
     const double rel2 = static_cast<double>(rel_tol) * rel_tol;
     const double abs2 = static_cast<double>(p.abs_tol) * p.abs_tol;
-    if (ok && p.criterion == 1 && (rr <= abs2 || rr <= rel2 * bb)) {
+    if (ok && p.criterion >= 1 && (rr <= abs2 || rr <= rel2 * bb)) {
       s.done = 1;
       s.status = rr <= abs2 ? 0 : 1;
     }
@@ -523,7 +606,7 @@ __global__ void __launch_bounds__(kBlock) k_it_final(PcgDev p, int k,
   int status = -1;
   if (iter >= p.max_iter) status = 2;
   else if (rr <= abs2) status = 0;
-  else if (rr <= rel2 * (p.criterion == 1 ? s.bb : s.rr0)) status = 1;
+  else if (rr <= rel2 * (p.criterion >= 1 ? s.bb : s.rr0)) status = 1;
   const double beta_d = (fabs(s.rz) < 1e-20) ? 0.0 : rz / s.rz;
   const float beta = static_cast<float>(beta_d);
   const int row = p.blk_row0[b] + threadIdx.x;
@@ -579,10 +662,27 @@ void launch_true_residual(hipStream_t s, const PcgDev& p, const RhsArgs& ra) {
 
 void launch_solve_residuals(hipStream_t s, const PcgDev& p, const RhsArgs& ra,
                             const float* x0, const float* dx, const float* xc,
-                            double* part3, AdmmCtl* ctl) {
+                            double* part3, AdmmCtl* ctl, int fold_iterate) {
   if (p.nblk == 0) return;
   k_solve_residuals<<<p.nblk, kBlock, 0, s>>>(p, ra, x0, dx, xc, part3);
-  k_solve_residual_max<<<p.nsys, kBlock, 0, s>>>(p, part3, ra.skip, ctl);
+  k_solve_residual_max<<<p.nsys, kBlock, 0, s>>>(p, part3, ra.skip, ctl, fold_iterate);
+}
+
+void launch_iterate_verify(hipStream_t s, const PcgDev& p, const RhsArgs& ra,
+                           const float* x0, const float* dx, double* part3,
+                           int* sys_done, AdmmCtl* ctl, float rel_tol, int first,
+                           int final) {
+  if (p.nblk == 0) return;
+  k_iterate_residual<<<p.nblk, kBlock, 0, s>>>(p, ra, x0, dx, part3, ctl,
+                                               first ? nullptr : sys_done);
+  k_iterate_verify<<<1, kBlock, 0, s>>>(p, part3, sys_done, ctl, rel_tol, first, final);
+}
+
+void launch_copy_unless(hipStream_t s, float* dst, const float* src, int64_t n,
+                        const int* skip) {
+  if (n <= 0) return;
+  const int blocks = static_cast<int>(std::min<int64_t>(1024, (n + kBlock - 1) / kBlock));
+  k_copy_unless<<<blocks, kBlock, 0, s>>>(dst, src, n, skip);
 }
 
 void launch_pcg_commit(hipStream_t s, const PcgDev& p) {

@@ -282,6 +282,9 @@ __global__ void __launch_bounds__(kPcgThreads)
   // Stopped ADMM loop: nothing to solve.  ctl->done is written only by
   // other kernels, so every workgroup reads the same value.
   if (a.ctl != nullptr && a.ctl->done != 0) return;
+  // A restart leaves converged systems alone (sys_done is written by the
+  // verification kernel before this launch, so all workgroups agree).
+  if (a.sys_done != nullptr && a.sys_done[sys] != 0) return;
   if (a.ctl != nullptr) rel_tol = a.ctl->rel_tol;
   const PcgSysDev S = a.sys[sys];
   // The previous solve's last epoch; workgroup 0 advances it at the end,
@@ -631,7 +634,7 @@ __global__ void __launch_bounds__(kPcgThreads)
   // kernel's block partials (every wavefront adds the same values in the
   // same order, so all take the same decisions).
   double ref2 = rr0;
-  if (a.criterion == 1) {
+  if (a.criterion >= 1) {
     const int q0 = a.sys_blk0[sys], q1 = a.sys_blk0[sys + 1];
     double v = 0.0;
     for (int q = q0 + lane; q < q1; q += kWave) v += a.part2[2 * q + 1];
@@ -640,7 +643,7 @@ __global__ void __launch_bounds__(kPcgThreads)
   int status = isfinite(rz) ? -1 : 3;
   // Right-hand-side criterion: a warm start that already meets it takes no
   // iteration.
-  if (status < 0 && a.criterion == 1) {
+  if (status < 0 && a.criterion >= 1) {
     if (rr <= abs2) status = 0;
     else if (rr <= rel2 * ref2) status = 1;
   }
@@ -739,7 +742,9 @@ __global__ void __launch_bounds__(kPcgThreads)
       atomicAdd(&a.ctl->pcg_iters, iter);
       atomicMax(&a.ctl->pcg_max_solve, iter);
       if (status == 2) atomicAdd(&a.ctl->pcg_stalled, 1);
-      if (sys == 0) {
+      if (a.restart != 0) {
+        atomicAdd(&a.ctl->pcg_restarts, 1);
+      } else if (sys == 0) {
         atomicAdd(&a.ctl->pcg_solves, 1);
         a.ctl->last_rel = static_cast<float>(rr0 > 0.0 ? sqrt(rr / rr0) : 0.0);
       }

@@ -207,8 +207,121 @@ __global__ void __launch_bounds__(kMlpThreads) k_mlp_predict(MlpDev net,
     if (g == 0 && s < n) {
       const float dl = sigma[s] * fminf(fmaxf(o.y, -kClip), kClip);
       delta[s] = dl;
-      x[s] += dl;
+      if (x != nullptr) x[s] += dl;
     }
+  }
+}
+
+// ----------------------------------------------------- projected start
+
+// (A v)_i of one row for the three coordinates, difference form as the PCG's
+// operator (the zero row sum of K is exact).
+__device__ __forceinline__ void apply_row(const PcgDev& p, int b, int row,
+                                          const float* __restrict__ v, float out[3]) {
+  const int sl = b * (kBlock / kSlice) + threadIdx.x / kSlice;
+  const int lane = threadIdx.x % kSlice;
+  const int off = p.slice_off[sl], width = p.slice_width[sl];
+  const float vi[3] = {v[3 * row], v[3 * row + 1], v[3 * row + 2]};
+  float acc[3] = {0.0f, 0.0f, 0.0f};
+  for (int k = 0; k < width; ++k) {
+    const int col = p.ell_col[off + k * kSlice + lane];
+    const float a = p.ell_val[off + k * kSlice + lane];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[c] += a * (v[3 * col + c] - vi[c]);
+  }
+  const float di = p.d[row];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[c] = acc[c] + di * vi[c];
+}
+
+// Per row block and coordinate c the sums the 2x2 projections need, with
+// u1 = A v1, u2 = A v2 (v1 the prediction, v2 the previous correction; a
+// null v is zero): [10 c + 0..2] v1.u1, v1.u2, v2.u2; [+3, 4] v1.r, v2.r;
+// [+5..7] u1.u1, u1.u2, u2.u2; [+8, 9] u1.r, u2.r.
+__global__ void __launch_bounds__(kBlock) k_warm_gram(PcgDev p, const float* v1,
+                                                      const float* v2, const float* r,
+                                                      double* part, const int* skip) {
+  __shared__ double lds[17];
+  if (skip != nullptr && *skip != 0) return;
+  const int b = blockIdx.x;
+  const int row = p.blk_row0[b] + threadIdx.x;
+  const bool valid = static_cast<int>(threadIdx.x) < p.blk_rows[b];
+  float u1[3] = {0.0f, 0.0f, 0.0f}, u2[3] = {0.0f, 0.0f, 0.0f};
+  if (valid && v1 != nullptr) apply_row(p, b, row, v1, u1);
+  if (valid && v2 != nullptr) apply_row(p, b, row, v2, u2);
+  for (int c = 0; c < 3; ++c) {
+    double t[10];
+    for (int k = 0; k < 10; ++k) t[k] = 0.0;
+    if (valid) {
+      const double a1 = v1 != nullptr ? v1[3 * row + c] : 0.0;
+      const double a2 = v2 != nullptr ? v2[3 * row + c] : 0.0;
+      const double rr = r[3 * row + c];
+      const double w1 = u1[c], w2 = u2[c];
+      t[0] = a1 * w1; t[1] = a1 * w2; t[2] = a2 * w2;
+      t[3] = a1 * rr; t[4] = a2 * rr;
+      t[5] = w1 * w1; t[6] = w1 * w2; t[7] = w2 * w2;
+      t[8] = w1 * rr; t[9] = w2 * rr;
+    }
+    for (int k = 0; k < 10; ++k) {
+      const double sum = block_sum(t[k], lds);
+      if (threadIdx.x == 0) part[30 * static_cast<size_t>(b) + 10 * c + k] = sum;
+    }
+  }
+}
+
+// One thread per (system, coordinate): the coefficients (c1, c2) of
+// x0 + c1 v1 + c2 v2.  mode 0: the Galerkin projection (minimal A-norm
+// error over the span); mode 1: the minimal residual ||r0 - c1 A v1 -
+// c2 A v2||.  A direction whose Gram entry is not positive is dropped; a
+// nearly dependent pair falls back to the better single direction.
+__global__ void k_warm_coef(PcgDev p, const double* part, int mode, float* coef,
+                            const int* skip) {
+  if (skip != nullptr && *skip != 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * p.nsys) return;
+  const int sys = i / 3, c = i % 3;
+  double t[10];
+  for (int k = 0; k < 10; ++k) t[k] = 0.0;
+  for (int q = p.sys_blk0[sys]; q < p.sys_blk0[sys + 1]; ++q)
+    for (int k = 0; k < 10; ++k) t[k] += part[30 * static_cast<size_t>(q) + 10 * c + k];
+  // G = [[g11, g12], [g12, g22]], rhs (h1, h2).
+  double g11, g12, g22, h1, h2;
+  if (mode == 0) {
+    g11 = t[0]; g12 = t[1]; g22 = t[2]; h1 = t[3]; h2 = t[4];
+  } else {
+    g11 = t[5]; g12 = t[6]; g22 = t[7]; h1 = t[8]; h2 = t[9];
+  }
+  double c1 = 0.0, c2 = 0.0;
+  const bool ok1 = g11 > 0.0 && isfinite(g11), ok2 = g22 > 0.0 && isfinite(g22);
+  const double det = g11 * g22 - g12 * g12;
+  if (ok1 && ok2 && det > 1e-10 * g11 * g22) {
+    c1 = (h1 * g22 - h2 * g12) / det;
+    c2 = (h2 * g11 - h1 * g12) / det;
+  } else {
+    // Single directions: the one that lowers the objective most.
+    const double gain1 = ok1 ? h1 * h1 / g11 : 0.0, gain2 = ok2 ? h2 * h2 / g22 : 0.0;
+    if (gain1 >= gain2 && ok1) c1 = h1 / g11;
+    else if (ok2) c2 = h2 / g22;
+  }
+  if (!isfinite(c1) || !isfinite(c2)) c1 = c2 = 0.0;
+  coef[6 * sys + c] = static_cast<float>(c1);
+  coef[6 * sys + 3 + c] = static_cast<float>(c2);
+}
+
+__global__ void __launch_bounds__(kBlock) k_warm_apply(PcgDev p, const float* v1,
+                                                       const float* v2, const float* coef,
+                                                       float* x, const int* skip) {
+  if (skip != nullptr && *skip != 0) return;
+  const int b = blockIdx.x;
+  if (static_cast<int>(threadIdx.x) >= p.blk_rows[b]) return;
+  const int row = p.blk_row0[b] + threadIdx.x;
+  const float* cf = coef + 6 * p.blk_sys[b];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float v = x[3 * row + c];
+    if (v1 != nullptr) v += cf[c] * v1[3 * row + c];
+    if (v2 != nullptr) v += cf[3 + c] * v2[3 * row + c];
+    x[3 * row + c] = v;
   }
 }
 
@@ -494,6 +607,15 @@ void launch_mlp_predict(hipStream_t s, const MlpDev& net, const unsigned short* 
   const int64_t tiles = (n + 15) / 16;
   const int blocks = static_cast<int>(std::min<int64_t>(1024, (tiles + 3) / 4));
   k_mlp_predict<<<blocks, kMlpThreads, 0, s>>>(net, feat, sigma, n, x, delta, skip);
+}
+
+void launch_warm_project(hipStream_t s, const PcgDev& p, const float* v1, const float* v2,
+                         const float* r, int mode, double* part, float* coef, float* x,
+                         const int* skip) {
+  if (p.nblk == 0) return;
+  k_warm_gram<<<p.nblk, kBlock, 0, s>>>(p, v1, v2, r, part, skip);
+  k_warm_coef<<<(3 * p.nsys + 63) / 64, 64, 0, s>>>(p, part, mode, coef, skip);
+  k_warm_apply<<<p.nblk, kBlock, 0, s>>>(p, v1, v2, coef, x, skip);
 }
 
 void launch_warm_judge(hipStream_t s, const PcgDev& p, const double* part2_prev,

@@ -3,9 +3,15 @@
 //
 // No counterpart in the reference, which starts every CG solve from the
 // previous ADMM iterate (mas_cg_solver.cu:193-311, admm_solver.cu:272-292).
-// Plain PCG stays in charge: the predicted start is kept only when its FP64
-// residual is below that of the previous iterate, and the solve then runs to
-// the same stopping test either way.
+// The prediction is not added to the start as it is: a per-vertex prediction
+// carries high-frequency error that the stiff operator amplifies, so its
+// residual is nearly always worse than the previous iterate's.  Instead the
+// start is x0 + c1 * prediction + c2 * (previous solve's correction), with
+// (c1, c2) per cloth and coordinate from a 2x2 projection of the x-update
+// system (minimal FP64 residual norm, or Galerkin); a noisy direction gets
+// a small coefficient.  Plain PCG stays in charge: the projected start is
+// kept only when its FP64 residual is below that of the previous iterate,
+// and the solve then runs to the same stopping test either way.
 //
 // One sample per (vertex, coordinate): the operator acts on the three
 // coordinates alike (vertex-scalar matrix times I3), so the network is shared
@@ -84,11 +90,21 @@ struct WarmFeatArgs {
 void launch_warm_features(hipStream_t s, const PcgDev& p, const WarmFeatArgs& a,
                           unsigned short* feat, float* sigma);
 
-/// x += sigma * net(feat) for every sample; the prediction is also written to
-/// delta.  No-op when *skip != 0.
+/// delta = sigma * net(feat) for every sample, added to x when x is not
+/// null.  No-op when *skip != 0.
 void launch_mlp_predict(hipStream_t s, const MlpDev& net, const unsigned short* feat,
                         const float* sigma, int64_t n, float* x, float* delta,
                         const int* skip);
+
+/// Projected start: per system and coordinate, x += c1 v1 + c2 v2 with
+/// (c1, c2) from the 2x2 projection of the x-update system onto span{v1,
+/// v2} (v1: the network's prediction, v2: the previous solve's correction;
+/// either may be null).  mode 0: Galerkin (minimal A-norm error); mode 1:
+/// minimal FP64 residual norm.  part: 30 doubles per row block; coef: 6
+/// floats per system.  No-op when *skip != 0.
+void launch_warm_project(hipStream_t s, const PcgDev& p, const float* v1, const float* v2,
+                         const float* r, int mode, double* part, float* coef, float* x,
+                         const int* skip);
 
 /// Per system: keep the predicted start when the FP64 residual of x0 + delta
 /// (part2 now) is below that of x0 (part2_prev, saved before the

@@ -66,8 +66,9 @@ def c1_sheet() -> dict:
     return {"V": V, "F": F, "pins": top_corner_pins(32)}
 
 
-def c2_scene(n: int = 80) -> dict:
-    """n x n sheet at z = 0.8 dropped onto an r = 0.3 icosphere at (0.5, 0.5, 0)."""
-    V, F = grid_sheet(n, z=0.8)
+def c2_scene(n: int = 80, z: float = 0.8) -> dict:
+    """n x n sheet at height z (0.5 m above the top by default) dropped onto an
+    r = 0.3 icosphere (2562 vertices) at (0.5, 0.5, 0)."""
+    V, F = grid_sheet(n, z=z)
     SV, SF = icosphere(4, 0.3, (0.5, 0.5, 0.0))
     return {"V": V, "F": F, "pins": np.zeros(0, np.int32), "SV": SV, "SF": SF}

@@ -144,14 +144,16 @@ def test_set_global_rejects_unknown_solver_modes(ctx):
     ok = sp.GlobalConfig()._c()
     assert lib.silk_hip_set_global(h, C.byref(ok)) == 0
     for field, value in [("mas_precision", 3), ("mas_precision", -1),
-                         ("pcg_criterion", 2), ("warm_start", 5),
+                         ("pcg_criterion", 3), ("warm_start", 5), ("warm_projection", 2),
                          ("warm_start", sp.WARM_LEARNED)]:
         g = sp.GlobalConfig()._c()
         setattr(g, field, value)
         assert lib.silk_hip_set_global(h, C.byref(g)) == sp.ErrorCode.InvalidConfig + 1, field
         assert lib.silk_hip_last_error(h)
     for field, value in [("mas_precision", sp.MAS_FP16), ("mas_precision", sp.MAS_BF16),
-                         ("pcg_criterion", sp.PCG_RHS)]:
+                         ("pcg_criterion", sp.PCG_RHS), ("pcg_criterion", sp.PCG_TRUE),
+                         ("warm_start", sp.WARM_EXTRAPOLATE),
+                         ("warm_projection", sp.PROJECT_GALERKIN)]:
         g = sp.GlobalConfig()._c()
         setattr(g, field, value)
         assert lib.silk_hip_set_global(h, C.byref(g)) == 0, field

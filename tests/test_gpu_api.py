@@ -10,7 +10,7 @@ import oracle as O
 import silk_physics_amd as sp
 from silk_physics_amd import scenes
 
-from helpers import coll_tuple, gpu_world, inscribed_radius, oracle_for
+from helpers import coll_tuple, gpu_world, inside_convex, oracle_for, same_decisions
 
 pytestmark = pytest.mark.gpu
 
@@ -38,28 +38,44 @@ def test_rising_obstacle_lockstep_and_no_penetration():
     ow.add_obstacle(SV, SF, coll=coll_tuple(cc))
     ow.finalize()
     dt = float(np.float32(1 / 60))
-    r_in = inscribed_radius(SV, SF)
     x_prev = V.astype(np.float64).reshape(-1)
     v_prev = np.zeros_like(x_prev)
-    errs = []
+    rep, free_match = [], []
+    prev_pos = SV.copy()
     for i in range(30):
         lift = np.float32(0.004 * (i + 1))
         pos = SV + np.array([0, 0, lift], np.float32)
         assert w.set_obstacle_position(ho, pos)
-        ow.set_obstacle_position(0, pos)
+        # The oracle steps twice from the same frame (own decisions, then the
+        # GPU's): the obstacle's segment is set before each.
+        ow.set_obstacle_frame(0, prev_pos, pos)
         ow.set_state(0, x_prev, v_prev)
         assert w.solver_step()
         rc, _ = ow.step()
         assert rc == 0
         _, xg = w.get_cloth_position(h, n=V.size)
         xg = xg.astype(np.float64)
-        errs.append(np.abs(xg - ow.positions(0)).max())
-        d = np.linalg.norm(xg.reshape(-1, 3) - np.array([0.5, 0.5, float(lift)]), axis=1)
-        assert d.min() >= r_in - 1e-3, (i, d.min())
+        e_free = np.abs(xg - ow.positions(0)).max()
+        same = same_decisions(w.decisions(), ow.decisions())
+        ow.set_obstacle_frame(0, prev_pos, pos)
+        ow.set_state(0, x_prev, v_prev)
+        ow.set_replay(*w.decisions())
+        rc, _ = ow.step()
+        assert rc == 0
+        assert ow.decisions()[2] == 0
+        rep.append(np.abs(xg - ow.positions(0)).max())
+        if same:
+            free_match.append(e_free)
+        s_out = inside_convex(xg, pos, SF)
+        assert s_out.min() > 0.0, (i, s_out.min())
         v_prev = (xg - x_prev) / dt
         x_prev = xg
-    print(f"rising obstacle lockstep: median {np.median(errs):.3e} max {max(errs):.3e}")
-    assert np.median(errs) <= 1e-4
+        prev_pos = pos
+    print(f"rising obstacle lockstep, GPU decisions replayed: median "
+          f"{np.median(rep):.3e} max {max(rep):.3e}; own decisions matched on "
+          f"{len(free_match)} of 30 steps, max {max(free_match):.3e}")
+    assert max(rep) <= 1e-4
+    assert len(free_match) >= 15 and max(free_match) <= 1e-4
 
 
 def test_pin_targets_move_pinned_vertices():

@@ -13,7 +13,7 @@ import oracle as O
 import silk_physics_amd as sp
 from silk_physics_amd import scenes
 
-from helpers import f32, gpu_world, inscribed_radius
+from helpers import f32, gpu_world, inside_convex
 
 pytestmark = pytest.mark.gpu
 
@@ -75,39 +75,41 @@ def test_c3_pcg_each_precision_matches_fp64_oracle():
 
 def test_c3_steps_each_precision():
     """Three steps of the C3 drape (sheet 1 cm above the sphere's top) per
-    bank precision: every solve meets ||b - A x|| <= 1e-6 ||b|| (FP64, the
-    iterate), no vertex enters the sphere, and the positions agree with the
-    FP32-bank form."""
+    bank precision under the true-residual criterion: every solve's iterate
+    meets ||b - A (x0 + dx)|| <= 1e-6 ||b|| (FP64; the FP32 recurrence alone
+    stops at about 1.6e-6 here), no vertex enters the sphere, and the
+    positions agree with the FP32-bank form."""
     V, F = c3_mesh()
     SV, SF = scenes.icosphere(4, 0.3, (0.5, 0.5, 0.0))
-    r_in = inscribed_radius(SV, SF)
     out = {}
     for name, prec in PRECISIONS:
         w = gpu_world(acceleration_z=-9.81, dt=1 / 60, max_inner_iteration=10,
                       linear_solver_rel_tol_min=1e-6, linear_solver_rel_tol_max=1e-6,
-                      initial_linear_rel_tol=1e-6, pcg_criterion=sp.PCG_RHS,
+                      initial_linear_rel_tol=1e-6, pcg_criterion=sp.PCG_TRUE,
                       mas_precision=prec, true_residual_stats=1)
         cc = sp.CollisionConfig()
         r, h = w.add_cloth(sp.ClothConfig(), cc, V, F, [])
         assert r
         r, _ = w.add_obstacle(cc, SV, SF)
         assert r
-        its = sols = 0
+        its = sols = rst = 0
         res = 0.0
         for _ in range(3):
             assert w.solver_step()
             st = w.last_stats
             its += st.pcg_iterations
             sols += st.pcg_solves
+            rst += st.pcg_restarts
             res = max(res, st.max_true_rel_residual)
             assert st.pcg_stalled_solves == 0
+            assert st.pcg_unverified_solves == 0
         _, x = w.get_cloth_position(h, n=V.size)
-        dmin = np.linalg.norm(x.reshape(-1, 3) - [0.5, 0.5, 0.0], axis=1).min()
+        dmin = inside_convex(x, SV, SF).min()
         print(f"{name} banks: {its / max(1, sols):.1f} PCG iterations per solve over "
-              f"{sols} solves, max true residual {res:.2e}, closest vertex "
-              f"{dmin - r_in:+.2e} m from the sphere")
+              f"{sols} solves ({rst} restarts), max true residual {res:.2e}, closest "
+              f"vertex {dmin:+.2e} m outside the sphere")
         assert res <= 1e-6
-        assert dmin >= r_in - 1e-4
+        assert dmin > 0.0
         out[name] = x.astype(np.float64)
     for name in ("fp16", "bf16"):
         diff = np.abs(out[name] - out["fp32"]).max()

@@ -56,7 +56,7 @@ def _sheet_world(n, kw=None):
     SV, SF = scenes.icosphere(2, 0.3, (0.5, 0.5, 0.0))
     g = dict(acceleration_z=-9.81, dt=1 / 60, max_inner_iteration=10,
              linear_solver_rel_tol_min=1e-6, linear_solver_rel_tol_max=1e-6,
-             initial_linear_rel_tol=1e-6, pcg_criterion=sp.PCG_RHS)
+             initial_linear_rel_tol=1e-6, pcg_criterion=sp.PCG_TRUE)
     g.update(kw or {})
     w = gpu_world(**g)
     cc = sp.CollisionConfig()
@@ -85,11 +85,14 @@ def test_record_and_train_reduce_the_loss():
     assert r and np.isfinite(p).all()
 
 
-def test_learned_start_keeps_the_answer_and_the_residual():
-    """With the network in charge of the start, every solve still meets the
-    stopping test (FP64 residual of the iterate <= 1e-6 |b|), the judge keeps
-    the network's start only where it lowers the residual, and the positions
-    stay those of the plain start to within the solver tolerance."""
+def test_projected_starts_cut_iterations():
+    """The same 20-step drape from the plain start (the previous iterate, as
+    the reference), the extrapolated start (x0 + c2 * previous correction)
+    and the learned start (x0 + c1 * network + c2 * previous correction),
+    both projections: every solve still meets the true-residual test (FP64
+    residual of the iterate <= 1e-6 |b|, restarts included), the judge keeps
+    a projected start only where it lowers the FP64 residual, and the
+    iterations per solve are reported for each."""
     w, h, V, g = _sheet_world(24)
     assert w.warm_start_record(200_000, 0.25)
     for _ in range(12):
@@ -99,11 +102,12 @@ def test_learned_start_keeps_the_answer_and_the_residual():
     r, net = w.get_warm_start_net()
     assert w.warm_start_record(0, 0.0)
 
-    def run(learned):
-        ww, hh, VV, gg = _sheet_world(24, dict(true_residual_stats=1))
-        if learned:
+    def run(mode, proj=sp.PROJECT_RESIDUAL):
+        ww, hh, VV, gg = _sheet_world(24)
+        if mode == sp.WARM_LEARNED:
             assert ww.set_warm_start_net(net)
-            assert ww.set_global_config(sp.GlobalConfig(**gg, warm_start=sp.WARM_LEARNED))
+        assert ww.set_global_config(sp.GlobalConfig(**gg, warm_start=mode,
+                                                    warm_projection=proj))
         it = sol = kept = ev = 0
         res = 0.0
         for _ in range(20):
@@ -115,15 +119,20 @@ def test_learned_start_keeps_the_answer_and_the_residual():
             ev += st.warm_start_solves
             res = max(res, st.max_true_rel_residual)
             assert st.pcg_stalled_solves == 0
-        _, x = ww.get_cloth_position(hh, n=VV.size)
-        return x, it, sol, kept, ev, res
+            assert st.pcg_unverified_solves == 0
+        return it / sol, sol, kept, ev, res
 
-    x0, it0, s0, _, _, res0 = run(False)
-    x1, it1, s1, kept, ev, res1 = run(True)
-    print(f"plain start: {it0} PCG iterations in {s0} solves ({it0 / s0:.2f} per solve), "
-          f"max true residual {res0:.2e}")
-    print(f"learned start: {it1} PCG iterations in {s1} solves ({it1 / s1:.2f} per solve), "
-          f"network start kept in {kept} of {ev} solves, max true residual {res1:.2e}")
-    assert res0 <= 1e-6 and res1 <= 1e-6
-    assert ev > 0 and 0 <= kept <= ev
-    assert np.abs(x1 - x0).max() <= 1e-3
+    out = {}
+    for name, mode, proj in [("previous iterate", sp.WARM_PREVIOUS, 0),
+                             ("extrapolated, min residual", sp.WARM_EXTRAPOLATE, 0),
+                             ("extrapolated, Galerkin", sp.WARM_EXTRAPOLATE, 1),
+                             ("learned, min residual", sp.WARM_LEARNED, 0),
+                             ("learned, Galerkin", sp.WARM_LEARNED, 1)]:
+        ips, sol, kept, ev, res = run(mode, proj)
+        out[name] = ips
+        print(f"{name}: {ips:.2f} PCG iterations per solve over {sol} solves, "
+              f"projected start kept in {kept} of {ev}, max true residual {res:.2e}")
+        assert res <= 1e-6
+        assert 0 <= kept <= ev
+        if mode != sp.WARM_PREVIOUS:
+            assert ev > 0
