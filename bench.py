@@ -95,32 +95,82 @@ def own_pcg_bytes(info: dict, half: bool = False) -> float:
             (2048.0 if half else 4096.0) * info["mas_banks"])
 
 
-def cpu_baseline(s, w, h, x_last, x_prev, dt, budget_s: float, max_steps: int):
-    import oracle as O  # the CPU oracle: reported baseline only
+def host_cpu() -> dict:
+    """Host core count and model name (from os and /proc/cpuinfo)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return {"nproc": cores, "model": model}
 
-    f32 = lambda v: float(np.float32(v))  # noqa: E731
-    r, perm, offs = w.get_partition(h, s["V"].shape[0])
-    ow = O.OracleWorld(dict(az=f32(-9.81), dt=f32(dt), max_inner=10,
-                            lin_rel_min=f32(1e-6), lin_rel_max=f32(1e-6),
-                            lin_rel_init=f32(1e-6), ms_scale=f32(0.2),
-                            ccd_tol=f32(1e-6)))
-    ow.add_cloth(s["V"], s["F"], [], perm, offs)
-    ow.add_obstacle(s["SV"], s["SF"])
-    ow.finalize()
-    v = (x_last.astype(np.float64) - x_prev.astype(np.float64)) / f32(dt)
-    ow.set_state(0, x_last.astype(np.float64), v)
-    n = 0
-    t0 = time.perf_counter()
-    while n < max_steps and (n == 0 or time.perf_counter() - t0 < budget_s):
-        rc, _ = ow.step()
-        assert rc == 0
-        n += 1
-    el = time.perf_counter() - t0
-    return {"value": round(n / el, 4), "unit": "steps/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{n} steps of the same workload with the FP64 oracle "
-                      f"(single thread), started from the GPU state after the "
-                      f"timed steps (draped, in contact), {el:.1f} s"}
+
+def cpu_baseline(n: int, budget_s: float, gpu_step_s: list) -> dict:
+    """The reference CPU algorithm restated (oracle/cpu_pipeline.cpp) on the
+    same scene from its rest state: steps/s over a bounded sample on all host
+    cores, and on one thread; the GPU's rate over the same first steps is
+    given beside it."""
+    import oracle as O  # the restated reference CPU path: baseline only
+
+    hc = host_cpu()
+    out = {}
+    for label, threads in (("all", hc["nproc"]), ("one", 1)):
+        s = scenes.c2_scene(n)
+        p = O.CpuPipeline(threads=threads)
+        p.add_cloth(s["V"], s["F"], s["pins"])
+        p.add_obstacle(s["SV"], s["SF"])
+        k = 0
+        t0 = time.perf_counter()
+        while k < len(gpu_step_s) and (k == 0 or time.perf_counter() - t0 < budget_s):
+            rc, _ = p.step()
+            if rc != 0:
+                raise RuntimeError("CPU pipeline step failed")
+            k += 1
+        out[label] = (k, time.perf_counter() - t0)
+    k, el = out["all"]
+    k1, el1 = out["one"]
+    gpu_same = k / max(1e-9, sum(gpu_step_s[:k]))
+    return {"value": round(k / el, 4), "unit": "steps/s", "cores": hc["nproc"],
+            "kind": "port", "one_thread": round(k1 / el1, 4),
+            "host": hc["model"],
+            "sample": f"steps 1..{k} of the same 80x80 drop from rest with the "
+                      f"reference CPU algorithm restated (PD, envelope Cholesky, "
+                      f"SVD projection, barrier targets, FP32 CCD) on "
+                      f"{hc['nproc']} threads, {el:.1f} s; one thread: {k1} steps "
+                      f"in {el1:.1f} s; the GPU ran the same first {k} steps at "
+                      f"{gpu_same:.2f} steps/s"}
+
+
+def traffic_summary() -> dict | None:
+    """HBM bytes per launch of the persistent PCG from the committed counter
+    summary (profiles/pcg_traffic.json), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def pmc_probe(args) -> int:
+    """Short run for the hardware-counter passes: a few C2 steps (PCG solves
+    from rest, then the first contacts), then the library's copy kernel on
+    256 MiB three times as the reference of known traffic."""
+    s, w, h = build_world(args.n)
+    for _ in range(args.steps):
+        assert w.solver_step()
+    info = w.scene_info()
+    ms = w.debug_copy(256 << 20, 3)
+    print(json.dumps({"probe": "pmc", "steps": args.steps, "copy_bytes": 256 << 20,
+                      "copy_ms": round(ms, 3), **info}), flush=True)
+    return 0
 
 
 def main() -> int:
@@ -129,19 +179,28 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=80, help="sheet quads per side")
+    ap.add_argument("--mas", choices=["fp32", "fp16", "bf16"], default="fp32",
+                    help="storage of the MAS bank inverses")
     ap.add_argument("--cpu-budget", type=float, default=15.0,
-                    help="seconds of CPU oracle work for cpu_baseline")
+                    help="seconds of CPU work per thread count for cpu_baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verify-steps", type=int, default=5,
-                    help="steps after the timed region with the FP64 true "
-                         "residual evaluated for every PCG solve")
+                    help="steps after the timed region with the FP64 residual of "
+                         "the stored x reported, then with the reference's "
+                         "stopping rule for its iteration count")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="short run for rocprofv3 --pmc passes (no JSON metric)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the step "
-              "is not sharded across GPUs.", file=sys.stderr)
+              "is not sharded across GPUs.",
+              file=sys.stderr)
         return 2
+    if args.pmc_probe:
+        return pmc_probe(args)
+    mas = {"fp32": sp.MAS_FP32, "fp16": sp.MAS_FP16, "bf16": sp.MAS_BF16}[args.mas]
 
-    s, w, h = build_world(args.n)
+    s, w, h = build_world(args.n, mas)
     nvx = s["V"].size
     t0 = time.perf_counter()
     for _ in range(args.warmup):
@@ -151,49 +210,73 @@ def main() -> int:
     info = w.scene_info()
 
     w.set_timing(True)
-    for c in range(8):
+    for c in range(9):
         w.timing(c, reset=True)
-    tot = dict(pcg=0, solves=0, admm=0, outer=0, contacts=0, ccd=0)
+    tot = dict(pcg=0, solves=0, restarts=0, admm=0, outer=0, contacts=0, ccd=0,
+               unverified=0, stalled=0)
+    step_s = []
     t0 = time.perf_counter()
     for k in range(args.steps):
+        ts = time.perf_counter()
         r = w.solver_step()
+        step_s.append(time.perf_counter() - ts)
         assert r, r.to_string()
         st = w.last_stats
         tot["pcg"] += st.pcg_iterations
         tot["solves"] += st.pcg_solves
+        tot["restarts"] += st.pcg_restarts
         tot["admm"] += st.admm_iterations
         tot["outer"] += st.outer_iterations
         tot["contacts"] += st.contacts
         tot["ccd"] += st.ccd_candidates
+        tot["unverified"] += st.pcg_unverified_solves
+        tot["stalled"] += st.pcg_stalled_solves
     dt_wall = time.perf_counter() - t0
     w.set_timing(False)
     kern = {}
     for c, name in CLASSES.items():
         ms, n = w.timing(c)
         kern[c] = (ms, n)
-    # Two more (untimed) states for the oracle's starting velocity.
-    _, x_prev = w.get_cloth_position(h, n=nvx)
-    assert w.solver_step()
-    _, x_last = w.get_cloth_position(h, n=nvx)
 
     # Roofline of the kernel with the most device time.
+    half = mas != sp.MAS_FP32
     dom = max((c for c in kern if c in (1, 2, 6)), key=lambda c: kern[c][0])
     ms, n = kern[dom]
-    bytes_total = byte_model(dom, info, tot["pcg"], n)
+    bytes_total = byte_model(dom, info, tot["pcg"], n, half)
     achieved = bytes_total / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     roof = {"bound": "hbm", "kernel": CLASSES[dom],
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
             "avg_launch_us": round(1e3 * ms / max(1, n), 3), "launches": n,
-            "bytes_per_launch": round(bytes_total / max(1, n), 1)}
+            "bytes_per_launch": round(bytes_total / max(1, n), 1),
+            "bytes_model": "reference formulation: 40 nnzb + 4 (V+1) + S n_blk + "
+                           "120 V per PCG iteration"}
+    if dom == 1 and ms > 0:
+        own = own_pcg_bytes(info, half) * tot["pcg"]
+        roof["own_bytes_per_launch"] = round(own / max(1, n), 1)
+        roof["own_achieved"] = round(own / (ms * 1e-3) / 1e9, 2)
+        roof["own_frac"] = round(own / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
+        roof["us_per_iteration"] = round(1e3 * ms / max(1, tot["pcg"]), 3)
+        tr = traffic_summary()
+        if tr is not None and tr.get("vertices") == info["vertices"]:
+            roof["traffic"] = tr.get("bytes_per_launch")
+            roof["traffic_source"] = tr.get("source")
 
-    # Verification pass: FP64 true residual of every PCG solve.
-    gk = global_kw(true_residual=True)
-    assert w.set_global_config(sp.GlobalConfig(**gk))
-    true_res = 0.0
+    # Verification: the FP64 residual of the stored FP32 x beside the
+    # iterate's, then the reference's stopping rule (rr <= rel^2 rr0) for its
+    # iteration count.
+    assert w.set_global_config(sp.GlobalConfig(**global_kw(True, sp.PCG_TRUE, mas)))
+    true_res = commit_res = 0.0
     for _ in range(args.verify_steps):
         assert w.solver_step()
         true_res = max(true_res, w.last_stats.max_true_rel_residual)
+        commit_res = max(commit_res, w.last_stats.max_committed_rel_residual)
+    assert w.set_global_config(sp.GlobalConfig(**global_kw(False, sp.PCG_RR0, mas)))
+    ref_it = ref_sol = 0
+    for _ in range(args.verify_steps):
+        assert w.solver_step()
+        ref_it += w.last_stats.pcg_iterations
+        ref_sol += w.last_stats.pcg_solves
 
     sps = args.steps / dt_wall
     line = {
@@ -204,13 +287,18 @@ def main() -> int:
         "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"{args.n}x{args.n}-quad sheet dropped 0.5 m on a "
                                "2562-vertex icosphere, collision and self-"
-                               "collision on, dt 1/60, 10 ADMM iters, CG rel "
-                               "tol 1e-6",
-                   **info},
+                               "collision on, dt 1/60, 10 ADMM iters, CG to "
+                               "||b - A x|| <= 1e-6 ||b|| (FP64)",
+                   "mas_precision": args.mas, **info},
         "pcg_iters_total": tot["pcg"], "pcg_solves": tot["solves"],
+        "pcg_restarts": tot["restarts"],
         "pcg_iters_per_solve": round(tot["pcg"] / max(1, tot["solves"]), 2),
         "pcg_iters_per_step": round(tot["pcg"] / args.steps, 2),
+        "pcg_unverified_solves": tot["unverified"],
+        "pcg_stalled_solves": tot["stalled"],
         "max_true_rel_residual": float(f"{true_res:.3e}"),
+        "max_stored_x_rel_residual": float(f"{commit_res:.3e}"),
+        "reference_rule_iters_per_solve": round(ref_it / max(1, ref_sol), 2),
         "admm_iters_per_step": round(tot["admm"] / args.steps, 2),
         "outer_iters_per_step": round(tot["outer"] / args.steps, 2),
         "mean_contacts": round(tot["contacts"] / args.steps, 1),
@@ -221,8 +309,7 @@ def main() -> int:
         "roofline": roof,
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(s, w, h, x_last, x_prev, 1 / 60,
-                                            args.cpu_budget, 20)
+        line["cpu_baseline"] = cpu_baseline(args.n, args.cpu_budget, step_s)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -264,8 +264,10 @@ int silk_hip_set_timing(silk_hip_ctx* ctx, int32_t on);
 /* Sizes of the device scene (builds it if needed), in this order:
  * 0 cloth vertices, 1 cloth triangles, 2 off-diagonal scalar entries of the
  * PD operator, 3 MAS banks (all levels), 4 padded slots, 5 persistent PCG
- * workgroups (0 when the multi-launch PCG is used), 6 coarse MAS banks,
- * 7 collider vertices.  n: capacity of info (entries beyond it are not
+ * workgroups over all systems (0 when the multi-launch PCG is used), 6
+ * coarse MAS banks, 7 collider vertices, 8 persistent PCG launches per solve
+ * (batches of systems whose workgroups fit the device together), 9 PCG
+ * systems (cloths).  n: capacity of info (entries beyond it are not
  * written). */
 int silk_hip_get_scene_info(silk_hip_ctx* ctx, int64_t* info, int32_t n);
 

@@ -462,11 +462,11 @@ class World:
 
     SCENE_INFO_KEYS = ("vertices", "triangles", "offdiag_entries", "mas_banks",
                        "slots", "pcg_workgroups", "coarse_banks",
-                       "collider_vertices")
+                       "collider_vertices", "pcg_launches", "systems")
 
     def scene_info(self) -> dict:
-        info = (C.c_int64 * 8)()
-        st = load_library().silk_hip_get_scene_info(self._ctx, info, 8)
+        info = (C.c_int64 * 10)()
+        st = load_library().silk_hip_get_scene_info(self._ctx, info, 10)
         if st != 0:
             raise RuntimeError(self._wrap(st).to_string())
         return dict(zip(self.SCENE_INFO_KEYS, [int(v) for v in info]))

@@ -4,6 +4,8 @@ Products (all next to this file, under ``lib/``):
   libsilk_hip.so  HIP device library + C ABI (include/silk_hip.h)
   libsilk.so      C++20 silk::World front end (include/silk/silk.hpp)
   world_api_test  C++ test driver for the World API (tests call it)
+  silk_headless   scene-file driver: runs a JSON scene through silk::World and
+                  writes the trajectory (csrc/headless)
 
 Device code is compiled for gfx950 only, with packed-FP32 instruction
 selection disabled.
@@ -79,13 +81,18 @@ def build(verbose: bool = False) -> dict[str, Path]:
     _run([CXX, *HOST_FLAGS, "-shared", "-o", str(world_so),
           str(CSRC / "frontend" / "world.cpp"), f"-L{LIB}", "-lsilk_hip",
           "-Wl,-rpath,$ORIGIN"])
+    headless = LIB / "silk_headless"
+    _run([CXX, *HOST_FLAGS, "-o", str(headless),
+          str(CSRC / "headless" / "headless.cpp"), str(CSRC / "headless" / "mesh_io.cpp"),
+          f"-L{LIB}", "-lsilk", "-lsilk_hip", f"-Wl,-rpath,$ORIGIN"])
     test_bin = LIB / "world_api_test"
     _run([CXX, *HOST_FLAGS, "-o", str(test_bin),
           str(ROOT / "tests" / "cpp" / "world_api_test.cpp"), f"-L{LIB}",
           "-lsilk", "-lsilk_hip", f"-Wl,-rpath,{LIB}"])
     if verbose:
         print(f"built {hip_so}, {world_so}, {test_bin}")
-    return {"silk_hip": hip_so, "silk": world_so, "world_api_test": test_bin}
+    return {"silk_hip": hip_so, "silk": world_so, "world_api_test": test_bin,
+            "silk_headless": headless}
 
 
 if __name__ == "__main__":

@@ -193,6 +193,20 @@ class HipCollision final : public CollisionWorld {
   DevBuf<TraverseJob> jobs_;
   int job_items_ = 0;
 
+  // Per-step Morton re-ordering of tree primitives and query orders
+  // (SILK_HIP_BVH_REBUILD=0 keeps the set-up order).
+  bool rebuild_order_ = true;
+  int order_total_ = 0, order_nseg_ = 0;
+  std::vector<size_t> point_off_, edge_off_;  // per object, into query_order_
+  DevBuf<OrderSeg> order_segs_;
+  DevBuf<int> order_off_, order_of_, ovals_in_, ovals_out_;
+  DevBuf<uint32_t> okeys_in_, okeys_out_;
+  DevBuf<float> order_box_;
+  DevBuf<char> order_temp_;
+  size_t order_temp_bytes_ = 0;
+  void setup_order_rebuild();
+  void rebuild_order();
+
   // candidates and outputs
   DevBuf<int2> pt_pairs_, ee_pairs_;
   int pt_cap_ = 1 << 16, ee_cap_ = 1 << 16;
@@ -355,8 +369,87 @@ void HipCollision::finish_setup(const std::vector<float>& pos0) {
   // fits, so that tests can run every one of them.
   if (const char* e = std::getenv("SILK_HIP_CONTACT_VARIANT"))
     force_variant_ = std::clamp(std::atoi(e), -1, 3);
+  if (const char* e = std::getenv("SILK_HIP_BVH_REBUILD")) rebuild_order_ = std::atoi(e) != 0;
+  setup_order_rebuild();
   // Host vectors above are temporaries: drain the uploads before they go.
   hip_check(hipStreamSynchronize(st_), "collision setup");
+}
+
+// Sort segments: every tree's primitives, then every object's point and
+// edge query orders.
+void HipCollision::setup_order_rebuild() {
+  std::vector<OrderSeg> segs;
+  std::vector<int> off{0};
+  auto add = [&](OrderSeg g) {
+    if (g.n <= 0) return;
+    g.off = off.back();
+    segs.push_back(g);
+    off.push_back(off.back() + g.n);
+  };
+  for (const auto& o : objs_) {
+    const TreeDev& tt = trees_[o.tree_t];
+    OrderSeg g;
+    g.n = tt.nprim;
+    g.nvert = 3;
+    g.src = tri_.get() + 3 * static_cast<size_t>(o.t0);
+    g.out_prim = const_cast<int*>(tt.prim);
+    g.out_id = const_cast<int*>(tt.prim_id);
+    g.gid0 = o.t0;
+    add(g);
+    const TreeDev& te = trees_[o.tree_e];
+    g.n = te.nprim;
+    g.nvert = 2;
+    g.src = edge_.get() + 2 * static_cast<size_t>(o.e0);
+    g.out_prim = const_cast<int*>(te.prim);
+    g.out_id = const_cast<int*>(te.prim_id);
+    g.gid0 = o.e0;
+    add(g);
+  }
+  for (size_t i = 0; i < objs_.size(); ++i) {
+    const ObjDesc& o = objs_[i];
+    OrderSeg g;
+    g.n = o.nv;
+    g.nvert = 1;
+    g.src = point_.get() + o.v0;
+    g.out_order = query_order_.get() + point_off_[i];
+    add(g);
+    g.n = o.ne;
+    g.nvert = 2;
+    g.src = edge_.get() + 2 * static_cast<size_t>(o.e0);
+    g.out_order = query_order_.get() + edge_off_[i];
+    add(g);
+  }
+  order_nseg_ = static_cast<int>(segs.size());
+  order_total_ = off.back();
+  if (order_total_ == 0) return;
+  std::vector<int> of(order_total_);
+  for (int k = 0; k < order_nseg_; ++k)
+    for (int i = off[k]; i < off[k + 1]; ++i) of[i] = k;
+  order_segs_.upload(segs, st_);
+  order_off_.upload(off, st_);
+  order_of_.upload(of, st_);
+  order_box_.resize(6 * static_cast<size_t>(order_nseg_));
+  okeys_in_.resize(order_total_);
+  okeys_out_.resize(order_total_);
+  ovals_in_.resize(order_total_);
+  ovals_out_.resize(order_total_);
+  order_temp_bytes_ = order_sort_temp_bytes(order_total_, order_nseg_);
+  order_temp_.resize(std::max<size_t>(1, order_temp_bytes_));
+}
+
+void HipCollision::rebuild_order() {
+  if (order_total_ == 0) return;
+  OrderScratch sc;
+  sc.box = order_box_.get();
+  sc.keys_in = okeys_in_.get();
+  sc.keys_out = okeys_out_.get();
+  sc.vals_in = ovals_in_.get();
+  sc.vals_out = ovals_out_.get();
+  sc.temp = order_temp_.get();
+  sc.temp_bytes = order_temp_bytes_;
+  hip_check(launch_order_rebuild(st_, p1_.get(), order_segs_.get(), order_nseg_,
+                                 order_off_.get(), order_of_.get(), order_total_, sc),
+            "BVH re-ordering");
 }
 
 CollVerts HipCollision::verts() const {
@@ -401,7 +494,10 @@ void HipCollision::refit() {
 // object with self-collision against itself.
 void HipCollision::build_jobs(const std::vector<float>& pos0) {
   std::vector<int> order;
-  std::vector<size_t> point_off(objs_.size()), edge_off(objs_.size());
+  std::vector<size_t>& point_off = point_off_;
+  std::vector<size_t>& edge_off = edge_off_;
+  point_off.assign(objs_.size(), 0);
+  edge_off.assign(objs_.size(), 0);
   std::vector<int> ids;
   for (size_t i = 0; i < objs_.size(); ++i) {
     const ObjDesc& o = objs_[i];
@@ -766,6 +862,7 @@ void HipCollision::begin_step(const float* outer_x, float* ms, float* dhat,
   for (DevBuf<float>* b : {&p_, &eta_, &dual_})
     hip_check(hipMemsetAsync(b->get(), 0, nc3 * sizeof(float), st_), "reset");
   set_cloth(outer_x, outer_x);
+  if (rebuild_order_) rebuild_order();
   refit();
   append(dcd(0.0f, *dhat, stats));
 }
@@ -899,6 +996,7 @@ void HipCollision::debug_candidates(const float* pos0, const float* pos1,
             "debug t0");
   hip_check(hipMemcpyAsync(p1_.get(), pos1, bytes, hipMemcpyHostToDevice, st_),
             "debug t1");
+  if (rebuild_order_) rebuild_order();
   refit();
   int npt = 0, nee = 0;
   for (;;) {

@@ -799,8 +799,42 @@ void Context::build_scene() {
     a.grid = ps_wg;
     const char* mode = std::getenv("SILK_HIP_PCG");
     const bool want = !(mode && std::strcmp(mode, "launch") == 0);
-    S.use_persistent = want && ps_wg > 0 && xch_fits &&
-                       pcg_persistent_fits(ps_wg, pcg_persistent_lds(a));
+    // Systems are launched in batches whose workgroups the device keeps
+    // resident together: one launch when they all fit, else consecutive
+    // launches of consecutive systems.  SILK_HIP_PCG_BATCH_WG caps a batch's
+    // workgroups (forces several batches on a small scene).
+    int cap_wg = 1 << 30;
+    if (const char* e = std::getenv("SILK_HIP_PCG_BATCH_WG")) cap_wg = std::max(1, std::atoi(e));
+    // The one-workgroup-per-CU variant first; when a system alone does not
+    // fit the device that way, the variant that puts two on a CU.
+    const size_t lds = pcg_persistent_lds(a);
+    auto make_batches = [&](int dense) {
+      S.ps_batches.clear();
+      if (ps_wg <= 0) return false;
+      for (int s0 = 0; s0 < S.nsys;) {
+        const int w0 = ps_sys[s0].wg0;
+        int s1 = s0, w1 = w0;
+        while (s1 < S.nsys && ps_sys[s1].wg0 == w1) {
+          const int we = w1 + ps_sys[s1].nwg + ps_sys[s1].ncw;
+          if (we - w0 > cap_wg && s1 > s0) break;
+          if (!pcg_persistent_fits(we - w0, lds, dense)) break;
+          w1 = we;
+          ++s1;
+        }
+        if (s1 == s0) return false;  // a system alone does not fit
+        S.ps_batches.emplace_back(w0, w1 - w0);
+        s0 = s1;
+      }
+      return true;
+    };
+    a.dense = 0;
+    bool batches_ok = make_batches(0);
+    if (!batches_ok && !(std::getenv("SILK_HIP_PCG_DENSE") &&
+                         std::strcmp(std::getenv("SILK_HIP_PCG_DENSE"), "0") == 0)) {
+      batches_ok = make_batches(1);
+      a.dense = batches_ok ? 1 : 0;
+    }
+    S.use_persistent = want && xch_fits && batches_ok;
   }
   S.coll = make_collision_world(*this, S);
   hip_check(hipStreamSynchronize(st), "scene upload");
@@ -889,6 +923,14 @@ static PersistDev persist_args(Scene& S, const silk_hip_global& g,
   return a;
 }
 
+void Context::launch_persistent(PersistDev a, float rel_tol) {
+  for (const auto& [w0, n] : scene_->ps_batches) {
+    a.wg_base = w0;
+    a.grid = n;
+    hip_check(launch_pcg_persistent(stream_, a, rel_tol), "persistent PCG launch");
+  }
+}
+
 int Context::pcg_solve(float rel_tol, bool factor, const RhsArgs& ra,
                        silk_hip_step_stats* stats, bool restart) {
   Scene& S = *scene_;
@@ -902,7 +944,7 @@ int Context::pcg_solve(float rel_tol, bool factor, const RhsArgs& ra,
               "warm start copy");
   if (S.use_persistent) {
     PersistDev a = persist_args(S, global_, pcg_max_iter());
-    hip_check(launch_pcg_persistent(stream_, a, rel_tol), "persistent PCG launch");
+    launch_persistent(a, rel_tol);
     S.h_state.resize(S.nsys);
     int abort_flag = 0;
     hip_check(hipMemcpyAsync(S.h_state.data(), S.ps_out.get(),
@@ -1140,8 +1182,7 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
                                  hipMemcpyDeviceToDevice, stream_),
                   "warm start copy");
       timed_begin(kTimePcg);
-      hip_check(launch_pcg_persistent(stream_, pa, 0.0f),
-                "persistent PCG launch");
+      launch_persistent(pa, 0.0f);
       timed_end();
       if (g.true_residual_stats && !verify)
         launch_solve_residuals(stream_, p, ra, S.x0_copy.get(), S.dx.get(),
@@ -1172,7 +1213,7 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
           pr.restart = 1;
           pr.dx_out = S.dxr.get();
           timed_begin(kTimePcg);
-          hip_check(launch_pcg_persistent(stream_, pr, 0.0f), "persistent PCG restart");
+          launch_persistent(pr, 0.0f);
           timed_end();
           launch_iterate_verify(stream_, p, ra, S.x0r.get(), S.dxr.get(), S.part3.get(),
                                 S.sys_done.get(), ctl, 0.0f, 0, k == kPcgRestarts - 1);
@@ -1457,7 +1498,7 @@ int Context::debug_pcg(const float* d, const float* b, float rel_tol, float* x) 
   if (S.use_persistent) {
     PersistDev a = persist_args(S, global_, pcg_max_iter());
     a.dx_out = nullptr;
-    hip_check(launch_pcg_persistent(stream_, a, rel_tol), "persistent PCG launch");
+    launch_persistent(a, rel_tol);
     S.h_state.resize(S.nsys);
     hip_check(hipMemcpyAsync(S.h_state.data(), S.ps_out.get(), S.nsys * sizeof(PcgState),
                              hipMemcpyDeviceToHost, stream_),

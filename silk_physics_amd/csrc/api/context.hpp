@@ -120,6 +120,7 @@ struct Scene {
   // ---- persistent PCG (one cooperative launch per solve)
   bool use_persistent = false;
   PersistDev pers;  // device pointers into the buffers below
+  std::vector<std::pair<int, int>> ps_batches;  // (first workgroup, count) per launch
   DevBuf<PcgSysDev> ps_sys;
   DevBuf<int> ps_slot_row, ps_sell_off, ps_sell_width, ps_sell_lcol, ps_halo_ptr,
       ps_halo_slot, ps_wg_n1_ptr, ps_wg_n1, ps_wg_c_ptr, ps_wg_c, ps_slot_cn,
@@ -213,6 +214,8 @@ class Context {
   int step(silk_hip_step_stats* stats);
   int pcg_solve(float rel_tol, bool factor, const RhsArgs& ra,
                 silk_hip_step_stats* st, bool restart = false);
+  /// One persistent PCG solve: a launch per batch of systems.
+  void launch_persistent(PersistDev a, float rel_tol);
   /// True-residual criterion on the host-driven ADMM path: tests the FP64
   /// residual of the last solve's iterate and restarts the solve from the
   /// committed x while it is above rel_tol |b| (at most kPcgRestarts times).

@@ -1,0 +1,140 @@
+// Per-step Morton re-ordering of the broadphase trees' primitives and of the
+// query visiting orders on the device.  The reference recomputes the Morton
+// codes of its OIBVH primitives and radix-sorts them on every update
+// (oibvh.cuh:521-560,603-633); the trees here keep their implicit layout and
+// only their primitive order changes, so a refit afterwards rebuilds them.
+//
+// Codes are those of the host order made at set-up (10 bits per axis of the
+// item centroid over the segment's centroid bounds, ties in index order: the
+// radix sort is stable), so the first re-ordering at the set-up positions
+// reproduces the host order exactly.
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include "hip/collision_kernels.hpp"
+#include "hip/device_common.hpp"
+
+namespace silk::amd {
+namespace {
+
+constexpr int kOrderBlock = 256;
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__device__ __forceinline__ void centroid(const OrderSeg& s, const float* p, int local,
+                                         float c[3]) {
+  for (int d = 0; d < 3; ++d) {
+    float acc = 0.0f;
+    for (int k = 0; k < s.nvert; ++k) acc += p[3 * s.src[s.nvert * local + k] + d];
+    c[d] = acc / static_cast<float>(s.nvert);
+  }
+}
+
+// One workgroup per segment: bounds of its item centroids.
+__global__ void __launch_bounds__(kOrderBlock) k_seg_bounds(const OrderSeg* segs,
+                                                            const float* p, float* box) {
+  __shared__ float red[6][kOrderBlock];
+  const OrderSeg s = segs[blockIdx.x];
+  float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+  for (int i = threadIdx.x; i < s.n; i += kOrderBlock) {
+    float c[3];
+    centroid(s, p, i, c);
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fminf(lo[d], c[d]);
+      hi[d] = fmaxf(hi[d], c[d]);
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    red[d][threadIdx.x] = lo[d];
+    red[3 + d][threadIdx.x] = hi[d];
+  }
+  __syncthreads();
+  for (int w = kOrderBlock / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w)
+      for (int d = 0; d < 3; ++d) {
+        red[d][threadIdx.x] = fminf(red[d][threadIdx.x], red[d][threadIdx.x + w]);
+        red[3 + d][threadIdx.x] = fmaxf(red[3 + d][threadIdx.x], red[3 + d][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) box[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(kOrderBlock) k_morton(const OrderSeg* segs,
+                                                        const int* seg_of, int total,
+                                                        const float* p, const float* box,
+                                                        uint32_t* keys, int* vals) {
+  const int i = blockIdx.x * kOrderBlock + threadIdx.x;
+  if (i >= total) return;
+  const int si = seg_of[i];
+  const OrderSeg s = segs[si];
+  const int local = i - s.off;
+  float c[3];
+  centroid(s, p, local, c);
+  uint32_t code = 0;
+  for (int d = 0; d < 3; ++d) {
+    const float lo = box[6 * si + d], ext = box[6 * si + 3 + d] - lo;
+    const float t = ext > 0.0f ? (c[d] - lo) / ext : 0.0f;
+    const uint32_t q = static_cast<uint32_t>(fminf(fmaxf(t * 1023.0f, 0.0f), 1023.0f));
+    code |= spread10(q) << (2 - d);
+  }
+  keys[i] = code;
+  vals[i] = local;
+}
+
+__global__ void __launch_bounds__(kOrderBlock) k_apply_order(const OrderSeg* segs,
+                                                             const int* seg_of, int total,
+                                                             const int* vals) {
+  const int i = blockIdx.x * kOrderBlock + threadIdx.x;
+  if (i >= total) return;
+  const OrderSeg s = segs[seg_of[i]];
+  const int k = i - s.off;
+  const int local = vals[i];
+  if (s.out_order != nullptr) {
+    s.out_order[k] = local;
+    return;
+  }
+  for (int v = 0; v < s.nvert; ++v) s.out_prim[s.nvert * k + v] = s.src[s.nvert * local + v];
+  s.out_id[k] = s.gid0 + local;
+}
+
+}  // namespace
+
+size_t order_sort_temp_bytes(int total, int nseg) {
+  size_t bytes = 0;
+  if (total <= 0 || nseg <= 0) return 0;
+  if (hipcub::DeviceSegmentedRadixSort::SortPairs(
+          nullptr, bytes, static_cast<const uint32_t*>(nullptr),
+          static_cast<uint32_t*>(nullptr), static_cast<const int*>(nullptr),
+          static_cast<int*>(nullptr), total, nseg, static_cast<const int*>(nullptr),
+          static_cast<const int*>(nullptr), 0, 30, nullptr) != hipSuccess)
+    return 0;
+  return bytes;
+}
+
+hipError_t launch_order_rebuild(hipStream_t s, const float* p, const OrderSeg* segs, int nseg,
+                                const int* seg_off, const int* seg_of, int total,
+                                OrderScratch sc) {
+  if (total <= 0 || nseg <= 0) return hipSuccess;
+  k_seg_bounds<<<nseg, kOrderBlock, 0, s>>>(segs, p, sc.box);
+  const int grid = (total + kOrderBlock - 1) / kOrderBlock;
+  k_morton<<<grid, kOrderBlock, 0, s>>>(segs, seg_of, total, p, sc.box, sc.keys_in,
+                                        sc.vals_in);
+  size_t bytes = sc.temp_bytes;
+  const hipError_t e = hipcub::DeviceSegmentedRadixSort::SortPairs(
+      sc.temp, bytes, sc.keys_in, sc.keys_out, sc.vals_in, sc.vals_out, total, nseg, seg_off,
+      seg_off + 1, 0, 30, s);
+  if (e != hipSuccess) return e;
+  k_apply_order<<<grid, kOrderBlock, 0, s>>>(segs, seg_of, total, sc.vals_out);
+  return hipGetLastError();
+}
+
+}  // namespace silk::amd

@@ -42,6 +42,40 @@ struct TreeDev {
 void launch_refit(hipStream_t s, const CollVerts& v, const TreeDev* trees_dev,
                   int ntrees, const int* leaf_block_off, int nblocks);
 
+/// One sort segment of the per-step re-ordering: a tree's primitives (writes
+/// out_prim and out_id) or an object's broadphase queries (writes out_order).
+struct OrderSeg {
+  int n = 0;                  // items
+  int nvert = 1;              // vertices per item (1 point, 2 edge, 3 triangle)
+  const int* src = nullptr;   // nvert * n global vertex ids, original order
+  int* out_prim = nullptr;    // tree: nvert * n vertex ids in the new order
+  int* out_id = nullptr;      // tree: n global primitive ids in the new order
+  int* out_order = nullptr;   // queries: n local indices in the new order
+  int gid0 = 0;               // global primitive id of item 0
+  int off = 0;                // first key of the segment
+};
+
+struct OrderScratch {
+  float* box = nullptr;        // 6 per segment
+  uint32_t* keys_in = nullptr; // total each
+  uint32_t* keys_out = nullptr;
+  int* vals_in = nullptr;
+  int* vals_out = nullptr;
+  void* temp = nullptr;        // radix sort workspace
+  size_t temp_bytes = 0;
+};
+
+/// Radix-sort workspace bytes for `total` keys in `nseg` segments.
+size_t order_sort_temp_bytes(int total, int nseg);
+
+/// Morton codes of every segment's items at positions p (3 per collider
+/// vertex), a stable segmented radix sort, then the new orders written
+/// (oibvh.cuh:521-560,603-633).  seg_off: nseg + 1 key offsets; seg_of: the
+/// segment of every key.
+hipError_t launch_order_rebuild(hipStream_t s, const float* p, const OrderSeg* segs, int nseg,
+                                const int* seg_off, const int* seg_of, int total,
+                                OrderScratch sc);
+
 struct QuerySet {
   int n = 0;
   int nvert = 1;                 // 1 point, 2 edge

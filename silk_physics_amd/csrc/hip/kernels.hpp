@@ -357,7 +357,9 @@ struct PersistDev {
   int ncb_max = 0;      // max coarse banks of a system (LDS)
   int nhalo_max = 0;    // max halo slots of a workgroup (LDS)
   int nwg_max = 0;      // max workgroups of a system (LDS)
-  int grid = 0;         // total workgroups
+  int grid = 0;         // workgroups of one launch
+  int wg_base = 0;      // first workgroup (over all systems) of the launch
+  int dense = 0;        // 1: the two-workgroups-per-CU variant
 };
 
 /// Tagged words one thread of the persistent kernel polls per exchange:
@@ -376,8 +378,9 @@ constexpr int kPcgHaloPerThread = 3;
 size_t pcg_persistent_lds(const PersistDev& a);
 
 /// True when the device can keep `grid` workgroups of the persistent kernel
-/// resident at once with the given dynamic LDS.
-bool pcg_persistent_fits(int grid, size_t lds_bytes);
+/// resident at once with the given dynamic LDS (dense: the variant that
+/// shares a CU between two workgroups).
+bool pcg_persistent_fits(int grid, size_t lds_bytes, int dense);
 
 /// One whole PCG solve (mas_cg_solver.cu:193-311) per system, x += solution.
 hipError_t launch_pcg_persistent(hipStream_t s, const PersistDev& a,

@@ -263,7 +263,12 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* lds) {
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(kPcgThreads)
+// kWavesPerSimd 1: one workgroup per CU (no register cap below the
+// kernel's ~270 VGPRs); 2: the compiler keeps the kernel within 256 VGPRs
+// (a few spills) so that two workgroups share a CU, for systems whose grid
+// exceeds the CU count.
+template <int kWavesPerSimd>
+__global__ void __launch_bounds__(kPcgThreads, kWavesPerSimd)
     k_pcg_persistent(PersistDev a, float rel_tol) {
   // Dynamic LDS (floats): [ncb_max coarse inverses][coarse sums: level-1
   // nodes first][coarse z][q sums of the level-1 nodes][partials, 8 per
@@ -273,11 +278,12 @@ __global__ void __launch_bounds__(kPcgThreads)
   extern __shared__ __align__(16) float dyn[];
   __shared__ double red[17];
 
+  // Workgroup id over all systems: a launch covers the systems of one batch
+  // (workgroups [wg_base, wg_base + grid)).
+  const int bx = static_cast<int>(blockIdx.x) + a.wg_base;
   int sys = -1;
   for (int s = 0; s < a.nsys; ++s)
-    if (static_cast<int>(blockIdx.x) >= a.sys[s].wg0 &&
-        static_cast<int>(blockIdx.x) < a.sys[s].wg0 + a.sys[s].nwg + a.sys[s].ncw)
-      sys = s;
+    if (bx >= a.sys[s].wg0 && bx < a.sys[s].wg0 + a.sys[s].nwg + a.sys[s].ncw) sys = s;
   if (sys < 0) return;  // spare workgroup: takes no part in any exchange
   // Stopped ADMM loop: nothing to solve.  ctl->done is written only by
   // other kernels, so every workgroup reads the same value.
@@ -292,7 +298,7 @@ __global__ void __launch_bounds__(kPcgThreads)
   unsigned* epoch_ctr = a.bar + kPcgBarrierStride * sys;
   unsigned ep = __hip_atomic_load(epoch_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int t = static_cast<int>(threadIdx.x);
-  const int j = static_cast<int>(blockIdx.x) - S.wg0;
+  const int j = bx - S.wg0;
   // A slot workgroup (j < nwg) owns level-0 banks; a coarse workgroup
   // (c = j - nwg) owns no slots and solves coarse banks c, c + ncw, ...
   const bool cwg = j >= S.nwg;
@@ -307,8 +313,8 @@ __global__ void __launch_bounds__(kPcgThreads)
   const int row = in_bank ? a.slot_row[slot] : -1;
   const bool own = row >= 0;
   const bool coarse = S.ncb > 0;
-  const int h0 = a.halo_ptr[blockIdx.x];
-  const int nh = a.halo_ptr[blockIdx.x + 1] - h0;
+  const int h0 = a.halo_ptr[bx];
+  const int nh = a.halo_ptr[bx + 1] - h0;
   const size_t hl = static_cast<size_t>(kPcgThreads) + a.nhalo_max;
 
   float* czl = dyn;                                            // 4 x (ncn_max + 1)
@@ -321,10 +327,10 @@ __global__ void __launch_bounds__(kPcgThreads)
   float* cpw = prr + 8 * static_cast<size_t>(a.nwg_max);      // 4 x ncb_max
   float* zl = cpw + 4 * static_cast<size_t>(a.ncb_max);       // 4 x hl
   float* pl = zl + 4 * hl;                                     // 4 x hl
-  const int n1_0 = a.wg_n1_ptr[blockIdx.x];
-  const int nn1 = a.wg_n1_ptr[blockIdx.x + 1] - n1_0;
-  const int c_0 = a.wg_c_ptr[blockIdx.x];
-  const int nc = a.wg_c_ptr[blockIdx.x + 1] - c_0;
+  const int n1_0 = a.wg_n1_ptr[bx];
+  const int nn1 = a.wg_n1_ptr[bx + 1] - n1_0;
+  const int c_0 = a.wg_c_ptr[bx];
+  const int nc = a.wg_c_ptr[bx + 1] - c_0;
   const int wave = t / kWave;
   const int lane = t % kWave;
   const bool lane0 = lane == 0;
@@ -487,7 +493,7 @@ __global__ void __launch_bounds__(kPcgThreads)
     }
   }
   unsigned long long* my_tp =
-      a.tp + kPcgPartWords * static_cast<size_t>(blockIdx.x) + 2 * wave;
+      a.tp + kPcgPartWords * static_cast<size_t>(bx) + 2 * wave;
   __syncthreads();
 
   // Sum of v over the slots of this thread's level-1 node, published by the
@@ -752,9 +758,17 @@ __global__ void __launch_bounds__(kPcgThreads)
   }
 }
 
-int g_max_blocks_per_cu = -1;
+struct Occupancy {
+  int blocks_per_cu = -1;
+  size_t lds = static_cast<size_t>(-1);
+};
+Occupancy g_occ[2];
 int g_cus = 0;
-size_t g_lds_checked = static_cast<size_t>(-1);
+
+const void* persistent_fn(int dense) {
+  return dense ? reinterpret_cast<const void*>(k_pcg_persistent<2>)
+               : reinterpret_cast<const void*>(k_pcg_persistent<1>);
+}
 
 }  // namespace
 
@@ -766,31 +780,36 @@ size_t pcg_persistent_lds(const PersistDev& a) {
                           4 * static_cast<size_t>(a.ncb_max) + 8 * hl);
 }
 
-bool pcg_persistent_fits(int grid, size_t shmem) {
+bool pcg_persistent_fits(int grid, size_t shmem, int dense) {
   if (shmem > static_cast<size_t>(kPcgLdsBytes)) return false;
-  if (g_max_blocks_per_cu < 0 || g_lds_checked != shmem) {
+  Occupancy& oc = g_occ[dense ? 1 : 0];
+  if (oc.blocks_per_cu < 0 || oc.lds != shmem) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return false;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
     if (!prop.cooperativeLaunch) return false;
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(k_pcg_persistent),
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
+    const void* fn = persistent_fn(dense);
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                             static_cast<int>(shmem)) != hipSuccess) {
       (void)hipGetLastError();
       return false;
     }
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, k_pcg_persistent, kPcgThreads, shmem) != hipSuccess) {
+    const hipError_t e =
+        dense ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_pcg_persistent<2>,
+                                                             kPcgThreads, shmem)
+              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_pcg_persistent<1>,
+                                                             kPcgThreads, shmem);
+    if (e != hipSuccess) {
       (void)hipGetLastError();
       return false;
     }
-    g_max_blocks_per_cu = nb;
+    oc.blocks_per_cu = nb;
+    oc.lds = shmem;
     g_cus = prop.multiProcessorCount;
-    g_lds_checked = shmem;
   }
-  return grid > 0 && grid <= g_max_blocks_per_cu * g_cus;
+  return grid > 0 && grid <= oc.blocks_per_cu * g_cus;
 }
 
 hipError_t launch_pcg_persistent(hipStream_t s, const PersistDev& a,
@@ -810,14 +829,18 @@ hipError_t launch_pcg_persistent(hipStream_t s, const PersistDev& a,
     return !(e != nullptr && e[0] == '0');
   }();
   if (coop_env) {
-    const hipError_t e = hipLaunchCooperativeKernel(
-        reinterpret_cast<const void*>(k_pcg_persistent), dim3(a.grid),
-        dim3(kPcgThreads), args, static_cast<unsigned>(shmem), s);
+    const hipError_t e = hipLaunchCooperativeKernel(persistent_fn(a.dense), dim3(a.grid),
+                                                    dim3(kPcgThreads), args,
+                                                    static_cast<unsigned>(shmem), s);
     if (e == hipSuccess) return e;
     (void)hipGetLastError();
   }
-  k_pcg_persistent<<<dim3(a.grid), dim3(kPcgThreads),
-                     static_cast<unsigned>(shmem), s>>>(arg, tol);
+  if (a.dense)
+    k_pcg_persistent<2><<<dim3(a.grid), dim3(kPcgThreads), static_cast<unsigned>(shmem), s>>>(
+        arg, tol);
+  else
+    k_pcg_persistent<1><<<dim3(a.grid), dim3(kPcgThreads), static_cast<unsigned>(shmem), s>>>(
+        arg, tol);
   return hipGetLastError();
 }
 

@@ -72,3 +72,27 @@ def c2_scene(n: int = 80, z: float = 0.8) -> dict:
     V, F = grid_sheet(n, z=z)
     SV, SF = icosphere(4, 0.3, (0.5, 0.5, 0.0))
     return {"V": V, "F": F, "pins": np.zeros(0, np.int32), "SV": SV, "SF": SF}
+
+
+def c4_scene(count: int = 64, seed: int = 0x5117B47C, nmin: int = 16, nmax: int = 128,
+             sphere_subdiv: int = 3) -> list[dict]:
+    """`count` independent sheets on an 8-wide lattice 3 m apart: sheet i has
+    n_i ~ U{nmin..nmax} quads per side (1 m span), 0-4 random boundary
+    vertices pinned, its own r = 0.3 sphere 2 cm below it and collision
+    group i + 1, so no two instances can interact."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(nmin, nmax + 1))
+        ox, oy = 3.0 * (i % 8), 3.0 * (i // 8)
+        V, F = grid_sheet(n, z=0.32)
+        V[:, 0] += ox
+        V[:, 1] += oy
+        side = np.arange(n + 1)
+        boundary = np.unique(np.concatenate([side, n * (n + 1) + side,
+                                             side * (n + 1), side * (n + 1) + n]))
+        pins = np.sort(rng.choice(boundary, size=int(rng.integers(0, 5)), replace=False))
+        SV, SF = icosphere(sphere_subdiv, 0.3, (0.5 + ox, 0.5 + oy, 0.0))
+        out.append({"V": V.astype(np.float32), "F": F, "pins": pins.astype(np.int32),
+                    "SV": SV, "SF": SF, "group": i + 1, "n": n})
+    return out

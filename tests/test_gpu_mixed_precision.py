@@ -4,8 +4,8 @@ accumulated in FP32, dot products in FP64 (the reference's FP32 banks:
 mas_preconditioner.cu:657-702,750-831).  The preconditioner's precision may
 change the iteration count, never the answer: each form is held to the FP64
 oracle's PCG solution of the same system, and the whole step to the FP32
-form.  At this size the persistent PCG does not fit the device, so these
-tests also run the multi-launch PCG."""
+form.  At this size the persistent PCG needs more workgroups than the
+device has CUs: it runs the variant that puts two workgroups on a CU."""
 import numpy as np
 import pytest
 
@@ -37,7 +37,8 @@ def test_c3_pcg_each_precision_matches_fp64_oracle():
                        V, F, [])
     assert r
     info = w.scene_info()
-    assert info["pcg_workgroups"] == 0, "C3 is expected on the multi-launch PCG"
+    print(f"C3 scene: {info['pcg_workgroups']} persistent workgroups in "
+          f"{info['pcg_launches']} launch(es) (0: multi-launch PCG)")
     r, perm, offs = w.get_partition(h, V.shape[0])
     nv = V.shape[0]
     Vp = np.zeros_like(V)

@@ -136,3 +136,50 @@ def test_projected_starts_cut_iterations():
         assert 0 <= kept <= ev
         if mode != sp.WARM_PREVIOUS:
             assert ev > 0
+
+
+def test_c5_learned_start_saves_iterations():
+    """C5: the network trained on samples recorded from a 96x96 and a 112x112
+    drape (the solver's own solves), then the 128x128 colliding sheet run
+    20 steps from the plain start and from the learned start: both meet the
+    true-residual test on every solve; the iterations per solve and the PCG
+    time per solve of each are printed (what the network gains on C5)."""
+    w, h, V, g = _sheet_world(96)
+    assert w.warm_start_record(400_000, 0.1)
+    for _ in range(10):
+        assert w.solver_step()
+    n1 = w.warm_start_samples()
+    r, loss = w.warm_start_train(600, 8192, 2e-3, seed=5)
+    assert r
+    r, net = w.get_warm_start_net()
+    print(f"trained on {n1} samples: loss {loss[:20].mean():.3f} -> {loss[-20:].mean():.3f}")
+
+    def run(mode):
+        ww, hh, VV, gg = _sheet_world(128, dict(max_inner_iteration=10))
+        if mode == sp.WARM_LEARNED:
+            assert ww.set_warm_start_net(net)
+        assert ww.set_global_config(sp.GlobalConfig(**gg, warm_start=mode))
+        ww.set_timing(True)
+        ww.timing(1, reset=True)
+        it = sol = kept = ev = 0
+        for _ in range(20):
+            assert ww.solver_step()
+            st = ww.last_stats
+            it += st.pcg_iterations
+            sol += st.pcg_solves
+            kept += st.warm_start_kept
+            ev += st.warm_start_solves
+            assert st.pcg_unverified_solves == 0 and st.pcg_stalled_solves == 0
+            assert st.max_true_rel_residual <= 1e-6
+        ms, n = ww.timing(1)
+        return it / sol, ms / max(1, sol), kept, ev
+
+    res = {}
+    for name, mode in [("plain", sp.WARM_PREVIOUS), ("extrapolated", sp.WARM_EXTRAPOLATE),
+                       ("learned", sp.WARM_LEARNED)]:
+        ips, mps, kept, ev = run(mode)
+        res[name] = ips
+        print(f"C5 {name}: {ips:.2f} PCG iterations per solve, {1e3 * mps:.1f} us of PCG "
+              f"per solve, projected start kept in {kept} of {ev}")
+    print(f"learned / plain iterations: {res['learned'] / res['plain']:.3f}; "
+          f"learned / extrapolated: {res['learned'] / res['extrapolated']:.3f}")
