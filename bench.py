@@ -58,19 +58,25 @@ def global_kw(true_residual: bool = False, criterion: int = sp.PCG_TRUE,
                 mas_precision=mas)
 
 
-def build_world(n: int, mas: int = sp.MAS_FP32):
-    s = scenes.c2_scene(n)
+def build_world(n: int, mas: int = sp.MAS_FP32, batched: int = 0):
+    """The C2-style drop of an n x n sheet, or with batched > 0 that many
+    independent sheets of 16..128 quads (each over its own sphere, in its own
+    collision group).  Returns (scene of the first sheet, world, handle)."""
     w = sp.World()
     r = w.init(sp.Backend.GPU)
     if not r:
         raise SystemExit(f"GPU init failed: {r.to_string()}")
     assert w.set_global_config(sp.GlobalConfig(**global_kw(mas=mas)))
-    cc = sp.CollisionConfig()  # on, self on, group 0, mu 0.3
-    r, h = w.add_cloth(sp.ClothConfig(), cc, s["V"], s["F"], s["pins"])
-    assert r, r.to_string()
-    r, ho = w.add_obstacle(cc, s["SV"], s["SF"])
-    assert r, r.to_string()
-    return s, w, h
+    sheets = scenes.c4_scene(batched) if batched else [scenes.c2_scene(n)]
+    handles = []
+    for s in sheets:
+        cc = sp.CollisionConfig(group=s.get("group", 0))  # on, self on, mu 0.3
+        r, h = w.add_cloth(sp.ClothConfig(), cc, s["V"], s["F"], s["pins"])
+        assert r, r.to_string()
+        r, ho = w.add_obstacle(cc, s["SV"], s["SF"])
+        assert r, r.to_string()
+        handles.append(h)
+    return sheets[0], w, handles[0]
 
 
 def byte_model(cls: int, info: dict, iters: int, launches: int,
@@ -188,6 +194,9 @@ def main() -> int:
                     help="steps after the timed region with the FP64 residual of "
                          "the stored x reported, then with the reference's "
                          "stopping rule for its iteration count")
+    ap.add_argument("--batched", type=int, default=0,
+                    help="instead of one sheet, this many independent sheets of "
+                         "16..128 quads (the batched-instance scene)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short run for rocprofv3 --pmc passes (no JSON metric)")
     args = ap.parse_args()
@@ -200,8 +209,9 @@ def main() -> int:
         return pmc_probe(args)
     mas = {"fp32": sp.MAS_FP32, "fp16": sp.MAS_FP16, "bf16": sp.MAS_BF16}[args.mas]
 
-    s, w, h = build_world(args.n, mas)
-    nvx = s["V"].size
+    s, w, h = build_world(args.n, mas, args.batched)
+    if args.batched:
+        args.no_cpu_baseline = True
     t0 = time.perf_counter()
     for _ in range(args.warmup):
         r = w.solver_step()
@@ -285,10 +295,13 @@ def main() -> int:
         "ms_per_step": round(1e3 * dt_wall / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": f"{args.n}x{args.n}-quad sheet dropped 0.5 m on a "
-                               "2562-vertex icosphere, collision and self-"
-                               "collision on, dt 1/60, 10 ADMM iters, CG to "
-                               "||b - A x|| <= 1e-6 ||b|| (FP64)",
+        "config": {"workload": (f"{args.batched} independent sheets of 16..128 quads "
+                                "(seeded) 2 cm over their own 642-vertex spheres"
+                                if args.batched else
+                                f"{args.n}x{args.n}-quad sheet dropped 0.5 m on a "
+                                "2562-vertex icosphere") +
+                               ", collision and self-collision on, dt 1/60, 10 "
+                               "ADMM iters, CG to ||b - A x|| <= 1e-6 ||b|| (FP64)",
                    "mas_precision": args.mas, **info},
         "pcg_iters_total": tot["pcg"], "pcg_solves": tot["solves"],
         "pcg_restarts": tot["restarts"],

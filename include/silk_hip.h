@@ -101,7 +101,9 @@ typedef struct {
                                      SILK_HIP_WARM_EXTRAPOLATE */
   int32_t warm_projection;        /* SILK_HIP_PROJECT_RESIDUAL (default) or
                                      SILK_HIP_PROJECT_GALERKIN */
-  int32_t reserved[3];
+  int32_t ccd_zero_toi;           /* SILK_HIP_CCD_ZERO_SKIP (reference) or
+                                     SILK_HIP_CCD_ZERO_CONTACT */
+  int32_t reserved[2];
 } silk_hip_global;
 
 enum {
@@ -138,6 +140,18 @@ enum {
 };
 
 enum {
+  /* A CCD pair already within the minimum separation ms at the segment start
+   * is searched again at ms / 2; a root at t = 0 there too makes the pair no
+   * collision for the segment (ccd.cpp:150-180, narrowphase.cu:68), so it
+   * may cross the other primitive during the segment. */
+  SILK_HIP_CCD_ZERO_SKIP = 0,
+  /* As SILK_HIP_CCD_ZERO_SKIP, then a third search at zero separation: such
+   * a pair collides when it would touch during the segment, so that no pair
+   * separated at a segment start crosses during it. */
+  SILK_HIP_CCD_ZERO_CONTACT = 1
+};
+
+enum {
   /* (c1, c2) minimise the FP64 residual norm ||b - A x||. */
   SILK_HIP_PROJECT_RESIDUAL = 0,
   /* (c1, c2) minimise the A-norm of the error (Galerkin). */
@@ -168,6 +182,9 @@ typedef struct {
   int32_t pcg_restarts;      /* SILK_HIP_PCG_TRUE: restarts of a system's solve */
   int32_t pcg_unverified_solves; /* SILK_HIP_PCG_TRUE: solves still above the
                                     tolerance after the last restart */
+  int32_t ccd_zero_toi_pairs; /* CCD candidates (summed over segments) whose
+                                 last search still put a root at t = 0: no
+                                 collision for their segment */
 } silk_hip_step_stats;
 
 /* Number of visible HIP devices (0 when none).  Never fails. */
@@ -298,18 +315,21 @@ int silk_hip_debug_dcd(silk_hip_ctx* ctx, const float* pos0, const float* pos1,
                        float* dist, int64_t cap, int64_t* nrec);
 
 /* Discrete decisions of the last silk_hip_step: the TOI returned by the CCD
- * of every outer iteration (frame-time units, one per iteration), and every
- * contact the step appended, 6 ints each: the DCD call that found it (0 at
- * the frame start, k after the k-th rollback), the type and the four global
- * vertex ids in the record's order.  NULL buffers query the counts. */
-int silk_hip_debug_decisions(silk_hip_ctx* ctx, float* tois, int32_t cap_toi,
-                             int32_t* n_toi, int32_t* contacts, int64_t cap_contacts,
-                             int64_t* n_contacts);
+ * of every outer iteration (frame-time units, one per iteration) and the
+ * number of ADMM iterations that outer iteration ran (the convergence test
+ * stopped it or max_inner_iteration did), and every contact the step
+ * appended, 6 ints each: the DCD call that found it (0 at the frame start, k
+ * after the k-th rollback), the type and the four global vertex ids in the
+ * record's order.  NULL buffers query the counts. */
+int silk_hip_debug_decisions(silk_hip_ctx* ctx, float* tois, int32_t* admm_iters,
+                             int32_t cap_toi, int32_t* n_toi, int32_t* contacts,
+                             int64_t cap_contacts, int64_t* n_contacts);
 
 /* The step's FP64 CCD kernel on n explicit queries: is_vf[i] selects
  * vertex-face (point, tri0, tri1, tri2) or edge-edge (a0, a1, b0, b1);
  * p0 / p1 hold 12 doubles per query, err 3 per query.  hit[i] = 1 and toi[i]
- * (local time in [0, 1]) for a hit, hit[i] = 0 otherwise. */
+ * (local time in [0, 1]) for a hit, hit[i] = 0 otherwise.  A root at t = 0
+ * is refined as silk_hip_global.ccd_zero_toi selects. */
 int silk_hip_debug_ccd(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
                        const double* p0, const double* p1, const double* err,
                        double ms, double tolerance, int32_t max_iter,

@@ -57,7 +57,7 @@ def lib():
             "oracle_overlap_pairs": (C.c_int, [f, C.c_int, f, C.c_int, C.c_int, i,
                                                C.c_int]),
             "oracle_ccd": (C.c_int, [C.c_int, d, d, C.c_double, C.c_double,
-                                     C.c_int, d, C.c_double, C.c_int, d]),
+                                     C.c_int, d, C.c_double, C.c_int, C.c_int, d]),
             "oracle_ccd_f32": (C.c_int, [C.c_int, f, f, C.c_float, C.c_float,
                                          C.c_int, f, C.c_float, f]),
             "oracle_ccd_filter": (None, [d, C.c_int, d]),
@@ -66,9 +66,9 @@ def lib():
             "oracle_candidates": (C.c_int, [vp, d, d, i, C.c_int, i, i, C.c_int,
                                             i]),
             "oracle_dcd_contact": (C.c_int, [C.c_int, d, d, i, d, C.c_double, d]),
-            "oracle_set_replay": (None, [vp, d, C.c_int, i, C.c_int]),
+            "oracle_set_replay": (None, [vp, d, i, C.c_int, i, C.c_int]),
             "oracle_set_obstacle_frame": (None, [vp, C.c_int, f, f]),
-            "oracle_decisions": (C.c_int, [vp, d, C.c_int, i, i, C.c_int, i]),
+            "oracle_decisions": (C.c_int, [vp, d, i, C.c_int, i, i, C.c_int, i]),
             "oracle_dcd": (C.c_int, [vp, d, d, C.c_double, C.c_double, i, d, C.c_int,
                                      i]),
             "oracle_sym3_min_eig": (C.c_double, [d]),
@@ -112,7 +112,8 @@ class OracleWorld:
                       g.get("admm_rel", 1e-3), g.get("pcg_max_iter", 1000),
                       g.get("sweeps", 10), g.get("ccd_max_iter", 4096),
                       g.get("ccd_tol", 1e-6), g.get("ms_scale", 0.2),
-                      g.get("dcd_scale", 1.0), g.get("pcg_criterion", 0)],
+                      g.get("dcd_scale", 1.0), g.get("pcg_criterion", 0),
+                      g.get("ccd_zero", 0)],
                      dtype=np.float64)
         self._keep = [p]
         lib().oracle_set_global(self._w, _d(p)[1])
@@ -161,28 +162,36 @@ class OracleWorld:
         b, bp = _f(curr)
         lib().oracle_set_obstacle_frame(self._w, idx, ap, bp)
 
-    def set_replay(self, tois, contacts):
+    def set_replay(self, tois, contacts, admm=None):
         """Replay another solver's decisions in the next step: tois[k] is the
-        CCD result of outer iteration k (frame-time units); contacts (n, 6)
-        holds (DCD call, type, four global vertex ids in record order) of
-        every contact to append."""
+        CCD result of outer iteration k (frame-time units) and admm[k] (when
+        given) the number of ADMM iterations it ran; contacts (n, 6) holds
+        (DCD call, type, four global vertex ids in record order) of every
+        contact to append."""
         t, tp = _d(np.asarray(tois, dtype=np.float64).reshape(-1))
         c, cp = _i(np.asarray(contacts, dtype=np.int32).reshape(-1, 6))
-        lib().oracle_set_replay(self._w, tp, t.size, cp, c.shape[0])
+        if admm is None:
+            ap = None
+        else:
+            a, ap = _i(np.asarray(admm, dtype=np.int32).reshape(-1))
+            assert a.size == t.size
+        lib().oracle_set_replay(self._w, tp, ap, t.size, cp, c.shape[0])
 
     def decisions(self):
         """Decisions of the last step: (TOIs, contacts (n, 6), replayed
-        contacts whose record could not be formed)."""
+        contacts whose record could not be formed, ADMM iterations of every
+        outer iteration)."""
         nt, nc = C.c_int(0), C.c_int(0)
-        lib().oracle_decisions(self._w, None, 0, C.byref(nt), None, 0, C.byref(nc))
+        lib().oracle_decisions(self._w, None, None, 0, C.byref(nt), None, 0, C.byref(nc))
         t = np.zeros(max(1, nt.value))
+        a = np.zeros(max(1, nt.value), dtype=np.int32)
         c = np.zeros((max(1, nc.value), 6), dtype=np.int32)
-        miss = lib().oracle_decisions(self._w, _d(t)[1] if False else
-                                      t.ctypes.data_as(C.POINTER(C.c_double)),
+        miss = lib().oracle_decisions(self._w, t.ctypes.data_as(C.POINTER(C.c_double)),
+                                      a.ctypes.data_as(C.POINTER(C.c_int)),
                                       nt.value, C.byref(nt),
                                       c.ctypes.data_as(C.POINTER(C.c_int)), nc.value,
                                       C.byref(nc))
-        return t[:nt.value], c[:nc.value], miss
+        return t[:nt.value], c[:nc.value], miss, a[:nt.value]
 
     def dcd(self, pos0, pos1, time, dhat):
         """DCD records on explicit collider positions (global numbering):
@@ -226,11 +235,10 @@ class OracleWorld:
 
     def step(self) -> tuple[int, dict]:
         st = np.zeros(8)
-        rc = lib().oracle_step(self._w, _d(st)[1] if False else
-                               st.ctypes.data_as(C.POINTER(C.c_double)))
+        rc = lib().oracle_step(self._w, st.ctypes.data_as(C.POINTER(C.c_double)))
         keys = ["outer", "admm", "pcg", "contacts", "colors", "ccd_candidates",
-                "min_toi"]
-        return rc, dict(zip(keys, st[:7].tolist()))
+                "min_toi", "ccd_zero"]
+        return rc, dict(zip(keys, st.tolist()))
 
     def positions(self, idx: int) -> np.ndarray:
         out = np.zeros(self.sizes[idx], dtype=np.float64)
@@ -341,14 +349,18 @@ def overlap_pairs(A, B=None):
 
 
 def ccd(vf, p0, p1, ms, tolerance, err, max_iter=4096, max_time=1.0,
-        stack_cap=1024):
+        stack_cap=1024, zero_mode=0, with_zero=False):
+    """(hit, toi) of one FP64 CCD query; with_zero adds whether the last
+    search still had a root at t = 0 (reported as no hit)."""
     a, ap = _d(p0)
     b, bp = _d(p1)
     e, ep = _d(err)
     t = C.c_double(0)
-    hit = lib().oracle_ccd(int(vf), ap, bp, ms, tolerance, max_iter, ep, max_time,
-                           stack_cap, C.byref(t))
-    return bool(hit), t.value
+    rc = lib().oracle_ccd(int(vf), ap, bp, ms, tolerance, max_iter, ep, max_time,
+                          stack_cap, int(zero_mode), C.byref(t))
+    if with_zero:
+        return rc == 1, t.value, rc == 2
+    return rc == 1, t.value
 
 
 def ccd_f32(vf, p0, p1, ms, tolerance, err, max_iter=4096, max_time=1.0):

@@ -165,10 +165,11 @@ int oracle_overlap_pairs(const float* a, int na, const float* b, int nb,
   return n;
 }
 
-/// FP64 CCD of one query.  p0, p1: 12 doubles (4 points).  Returns 1 on hit.
+/// FP64 CCD of one query.  p0, p1: 12 doubles (4 points).  Returns 1 on hit,
+/// 2 for no hit with a root at t = 0 at the last separation searched, else 0.
 int oracle_ccd(int vf, const double* p0, const double* p1, double ms,
                double tolerance, int max_iter, const double* err,
-               double max_time, int stack_cap, double* toi) {
+               double max_time, int stack_cap, int zero_mode, double* toi) {
   double a[4][3], b[4][3];
   for (int i = 0; i < 12; ++i) {
     a[i / 3][i % 3] = p0[i];
@@ -181,7 +182,10 @@ int oracle_ccd(int vf, const double* p0, const double* p1, double ms,
   for (int d = 0; d < 3; ++d) c.err[d] = err[d];
   c.max_time = max_time;
   c.stack_cap = stack_cap;
-  return ccd_query(vf != 0, a, b, c, toi) ? 1 : 0;
+  c.zero_mode = zero_mode;
+  bool zero = false;
+  if (ccd_query(vf != 0, a, b, c, toi, &zero)) return 1;
+  return zero ? 2 : 0;
 }
 
 /// The same query in single precision (the reference CPU CCD as it runs).

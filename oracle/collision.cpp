@@ -328,10 +328,11 @@ bool root_find(bool vf, const T p0[4][3], const T p1[4][3], const T tol[3],
 template <typename T>
 bool ccd_generic(bool vf, const T p0[4][3], const T p1[4][3], T ms, T tolerance,
                  long max_iter, const T err[3], T max_time, bool no_zero_toi,
-                 int cap, T* toi) {
+                 int zero_mode, int cap, T* toi, bool* zero) {
   T tol[3];
   param_tolerance<T>(vf, p0, p1, tolerance, tol);
   T t;
+  if (zero) *zero = false;
   if (!root_find<T>(vf, p0, p1, tol, tolerance, err, ms, max_iter, max_time, cap,
                     &t))
     return false;
@@ -339,34 +340,41 @@ bool ccd_generic(bool vf, const T p0[4][3], const T p1[4][3], T ms, T tolerance,
     *toi = t;
     return true;
   }
-  // ccd.cpp:235-256 and narrowphase.cpp:70-76: one refinement at ms/2.
-  T t2;
-  if (!root_find<T>(vf, p0, p1, tol, tolerance, err, ms * T(0.5), max_iter,
-                    max_time, cap, &t2)) {
-    *toi = max_time;
-    return true;
+  // ccd.cpp:150-180: one refinement at ms/2 (zero_mode 1: then one at zero
+  // separation).  No root there: no contact before max_time; a root at t = 0
+  // again: no collision for the segment (narrowphase.cu:68).
+  const int passes = zero_mode == 1 ? 2 : 1;
+  for (int pass = 0; pass < passes; ++pass) {
+    T t2;
+    const T sep = pass == 0 ? ms * T(0.5) : T(0);
+    if (!root_find<T>(vf, p0, p1, tol, tolerance, err, sep, max_iter, max_time, cap,
+                      &t2)) {
+      *toi = max_time;
+      return true;
+    }
+    if (t2 > T(0)) {
+      *toi = t2;
+      return true;
+    }
   }
-  if (t2 > T(0)) {
-    *toi = t2;
-    return true;
-  }
+  if (zero) *zero = true;
   return false;
 }
 
 }  // namespace
 
 bool ccd_query(bool vf, const double p0[4][3], const double p1[4][3],
-               const CcdConfig& cfg, double* toi) {
+               const CcdConfig& cfg, double* toi, bool* zero) {
   return ccd_generic<double>(vf, p0, p1, cfg.ms, cfg.tolerance, cfg.max_iter,
-                             cfg.err, cfg.max_time, cfg.no_zero_toi,
-                             cfg.stack_cap, toi);
+                             cfg.err, cfg.max_time, cfg.no_zero_toi, cfg.zero_mode,
+                             cfg.stack_cap, toi, zero);
 }
 
 bool ccd_query_f32(bool vf, const float p0[4][3], const float p1[4][3],
                    float ms, float tolerance, long max_iter, const float err[3],
                    float max_time, float* toi) {
   return ccd_generic<float>(vf, p0, p1, ms, tolerance, max_iter, err, max_time,
-                            true, 1 << 30, toi);
+                            true, 0, 1 << 30, toi, nullptr);
 }
 
 void ccd_filter(const double abs_max[3], bool vf, double err[3]) {

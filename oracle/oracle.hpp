@@ -117,6 +117,9 @@ struct CcdConfig {
   double max_time = 1.0;
   bool no_zero_toi = true;
   int stack_cap = 1024;   // pending-box capacity (overflow returns early)
+  // 0: a root at t = 0 is searched again at ms / 2 only (ccd.cpp:150-180);
+  // 1: then once more at zero separation.
+  int zero_mode = 0;
 };
 
 /// Pending-box capacity of the world-level CCD; equals the HIP kernel's.
@@ -124,9 +127,10 @@ constexpr int kCcdStackCap = 64;
 
 /// Earliest conservative TOI of a vertex-face (vf=true) or edge-edge query,
 /// restating interval_root_finder.cpp:391-510 and ccd.cpp:195-257 in double.
-/// Returns false for no hit; otherwise *toi in [0, max_time].
+/// Returns false for no hit; otherwise *toi in [0, max_time].  *zero (when
+/// given) is set when the last search still had a root at t = 0 (no hit).
 bool ccd_query(bool vf, const double p0[4][3], const double p1[4][3],
-               const CcdConfig& cfg, double* toi);
+               const CcdConfig& cfg, double* toi, bool* zero = nullptr);
 
 /// The same algorithm in single precision, as the reference CPU CCD runs it.
 bool ccd_query_f32(bool vf, const float p0[4][3], const float p1[4][3],

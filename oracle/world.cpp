@@ -28,6 +28,7 @@ struct Params {
   int pcg_max_iter = 1000, sweeps = 10, ccd_max_iter = 4096;
   double ccd_tol = 1e-6, ms_scale = 0.2, dcd_scale = 1.0;
   int pcg_criterion = 0;
+  int ccd_zero = 0;  // CCD zero-TOI mode (0: ms / 2 only, 1: then zero separation)
 };
 
 struct Coll {
@@ -84,6 +85,7 @@ struct Contact {
 
 struct StepStats {
   int outer = 0, admm = 0, pcg = 0, contacts = 0, colors = 0, ccd_cand = 0;
+  int ccd_zero = 0;  // candidates left with a root at t = 0 (no collision)
   double min_toi = 1.0;
 };
 
@@ -103,12 +105,14 @@ struct World {
   // Decisions of the last step: CCD TOI of every outer iteration, appended
   // contacts (DCD call, type, four global ids in record order).
   std::vector<double> log_toi;
+  std::vector<int> log_admm;  // ADMM iterations of each outer iteration
   std::vector<int> log_contacts;
   // Replay of another solver's decisions for the next step (set_replay):
   // the outer loop takes these TOIs and every DCD call appends exactly these
   // contacts, their records formed on the oracle's own positions.
   bool replay = false;
   std::vector<double> rp_toi;
+  std::vector<int> rp_admm;  // empty: run the convergence test
   std::vector<int> rp_contacts;
   int replay_missing = 0;  // replayed contacts whose record could not be formed
 };
@@ -653,6 +657,7 @@ static double min_toi(World& w, double time_start, double interval, double ms) {
   cfg.max_iter = w.P.ccd_max_iter;
   cfg.max_time = 1.0;
   cfg.stack_cap = kCcdStackCap;
+  cfg.zero_mode = w.P.ccd_zero;
   double best = 1.0;
   auto traj = [&](int gv, double p0[3], double p1[3]) {
     if (gv < w.nv_cloth) {
@@ -675,7 +680,11 @@ static double min_toi(World& w, double time_start, double interval, double ms) {
     CcdConfig c = cfg;
     ccd_filter(abs_max, vf, c.err);
     double toi;
-    if (!ccd_query(vf, p0, p1, c, &toi)) return;
+    bool zero = false;
+    if (!ccd_query(vf, p0, p1, c, &toi, &zero)) {
+      w.stats.ccd_zero += zero ? 1 : 0;
+      return;
+    }
     if (toi == 0.0) return;
     best = std::min(best, toi);
   };
@@ -739,8 +748,10 @@ static void contact_update(World& w, const std::vector<double>& x,
 
 // ----------------------------------------------------------------- ADMM
 
+// `forced` >= 0 runs exactly that many iterations (a replayed decision of
+// the convergence test below); -1 stops on the test.
 static bool admm_solve(World& w, std::vector<double>& X,
-                       const std::vector<double>& minert) {
+                       const std::vector<double>& minert, int forced = -1) {
   const int N = 3 * w.nv_cloth;
   std::vector<double> lam(N, 0.0);
   double init_p = 0, init_d = 0, ratio = 1.0;
@@ -750,7 +761,8 @@ static bool admm_solve(World& w, std::vector<double>& X,
     for (char p : c.pinned) pin_dof += p ? 3 : 0;
   }
   const double rho_p = 1e4, max_mul = 1e8;
-  for (int it = 0; it < w.P.max_inner; ++it) {
+  const int n_it = forced >= 0 ? forced : w.P.max_inner;
+  for (int it = 0; it < n_it; ++it) {
     if (it != 0) {
       const double rel = std::clamp(w.P.lin_rel_init * ratio, w.P.lin_rel_min,
                                     w.P.lin_rel_max);
@@ -871,7 +883,7 @@ static bool admm_solve(World& w, std::vector<double>& X,
                          std::max(std::sqrt(dc), std::sqrt(dp)) * w.P.admm_rel;
     ratio = std::max(primal / std::max(init_p, w.P.admm_abs),
                      dualn / std::max(init_d, w.P.admm_abs));
-    if (primal <= eps_p && dualn <= eps_d) break;
+    if (forced < 0 && primal <= eps_p && dualn <= eps_d) break;
   }
   for (auto& c : w.cloths)
     for (int i = 0; i < c.nv; ++i)
@@ -942,6 +954,7 @@ static bool step(World& w) {
   w.eta.assign(N, 0.0);
   w.active_dof = 0;
   w.log_toi.clear();
+  w.log_admm.clear();
   w.log_contacts.clear();
   w.replay_missing = 0;
   int call = 0;
@@ -953,23 +966,37 @@ static bool step(World& w) {
   set_cloth(outer_x, outer_x);
   find_contacts(0.0);
   double remaining = 1.0;
+  // A replayed step also replays the other solver's acceptance test, which
+  // that solver makes in FP32 (toi >= remaining, remaining -= 0.8 toi): in
+  // FP64 the final TOI, equal to that solver's FP32 remaining interval, can
+  // fall just below this side's remaining interval and start an outer
+  // iteration the other solver never ran.
+  float remaining_f = 1.0f;
   int outer = 0;
   for (; outer < w.P.max_outer; ++outer) {
     for (int i = 0; i < N; ++i) {
       const int d = i % 3;
       inner[i] = (outer_x[i] + h * outer_v[i]) + h * h * w.P.acc[d];
     }
-    if (!admm_solve(w, inner, minert)) return false;
+    const int forced = w.replay && outer < static_cast<int>(w.rp_admm.size())
+                           ? w.rp_admm[outer]
+                           : -1;
+    const int admm0 = w.stats.admm;
+    if (!admm_solve(w, inner, minert, forced)) return false;
+    w.log_admm.push_back(static_cast<int>(w.stats.admm - admm0));
     set_cloth(outer_x, inner);
-    const double toi = w.replay && outer < static_cast<int>(w.rp_toi.size())
-                           ? w.rp_toi[outer]
-                           : min_toi(w, 1.0 - remaining, remaining, ms);
+    const bool replayed = w.replay && outer < static_cast<int>(w.rp_toi.size());
+    const double toi =
+        replayed ? w.rp_toi[outer] : min_toi(w, 1.0 - remaining, remaining, ms);
     w.log_toi.push_back(toi);
     for (int i = 0; i < N; ++i) outer_v[i] = (inner[i] - outer_x[i]) / h;
-    if (toi >= remaining) {
+    const bool accept =
+        replayed ? static_cast<float>(toi) >= remaining_f : toi >= remaining;
+    if (accept) {
       outer_x = inner;
       break;
     }
+    remaining_f -= 0.8f * static_cast<float>(toi);
     w.stats.min_toi = std::min(w.stats.min_toi, 1.0 - remaining + toi);
     const double rb = 0.8 * toi, frac = rb / remaining;
     for (int i = 0; i < N; ++i) outer_x[i] = frac * inner[i] + (1.0 - frac) * outer_x[i];
@@ -1006,9 +1033,9 @@ extern "C" {
 void* oracle_create() { return new World(); }
 void oracle_destroy(void* w) { delete static_cast<World*>(w); }
 
-/// p: 19 doubles: acc[3], max_outer, max_inner, dt, lin_abs, lin_rel_min,
+/// p: 20 doubles: acc[3], max_outer, max_inner, dt, lin_abs, lin_rel_min,
 /// lin_rel_max, lin_rel_init, admm_abs, admm_rel, pcg_max_iter, sweeps,
-/// ccd_max_iter, ccd_tol, ms_scale, dcd_scale, pcg_criterion.
+/// ccd_max_iter, ccd_tol, ms_scale, dcd_scale, pcg_criterion, ccd_zero.
 void oracle_set_global(void* wp, const double* p) {
   auto& P = static_cast<World*>(wp)->P;
   P.acc[0] = p[0]; P.acc[1] = p[1]; P.acc[2] = p[2];
@@ -1022,6 +1049,7 @@ void oracle_set_global(void* wp, const double* p) {
   P.ccd_max_iter = static_cast<int>(p[14]);
   P.ccd_tol = p[15]; P.ms_scale = p[16]; P.dcd_scale = p[17];
   P.pcg_criterion = static_cast<int>(p[18]);
+  P.ccd_zero = static_cast<int>(p[19]);
 }
 
 /// cloth: k, kb, density, damping; coll: on, self, group, restitution,
@@ -1148,24 +1176,28 @@ void oracle_set_obstacle_frame(void* wp, int idx, const float* prev, const float
 /// of iteration k (frame-time units) and the DCD call c appends exactly the
 /// contacts listed with call index c (6 ints each: call, type, four global
 /// vertex ids in record order).
-void oracle_set_replay(void* wp, const double* tois, int n_toi, const int* contacts,
-                       int n_contacts) {
+void oracle_set_replay(void* wp, const double* tois, const int* admm, int n_toi,
+                       const int* contacts, int n_contacts) {
   World& w = *static_cast<World*>(wp);
   w.replay = true;
   w.rp_toi.assign(tois, tois + n_toi);
+  if (admm != nullptr) w.rp_admm.assign(admm, admm + n_toi);
+  else w.rp_admm.clear();
   w.rp_contacts.assign(contacts, contacts + 6 * static_cast<size_t>(n_contacts));
 }
 
 /// Decisions of the last step in the replay format; counts in n_toi and
 /// n_contacts (buffers may be null).  Returns the number of replayed
 /// contacts whose record could not be formed.
-int oracle_decisions(void* wp, double* tois, int cap_toi, int* n_toi, int* contacts,
-                     int cap_contacts, int* n_contacts) {
+int oracle_decisions(void* wp, double* tois, int* admm, int cap_toi, int* n_toi,
+                     int* contacts, int cap_contacts, int* n_contacts) {
   World& w = *static_cast<World*>(wp);
   *n_toi = static_cast<int>(w.log_toi.size());
   *n_contacts = static_cast<int>(w.log_contacts.size() / 6);
   if (tois)
     for (int i = 0; i < std::min(cap_toi, *n_toi); ++i) tois[i] = w.log_toi[i];
+  if (admm)
+    for (int i = 0; i < std::min(cap_toi, *n_toi); ++i) admm[i] = w.log_admm[i];
   if (contacts)
     for (size_t i = 0; i < 6 * static_cast<size_t>(std::min(cap_contacts, *n_contacts)); ++i)
       contacts[i] = w.log_contacts[i];
@@ -1208,7 +1240,8 @@ int oracle_dcd(void* wp, const double* pos0, const double* pos1, double time,
   return *n > cap ? 1 : 0;
 }
 
-/// stats: outer, admm, pcg, contacts, colors, ccd candidates, min_toi.
+/// stats: outer, admm, pcg, contacts, colors, ccd candidates, min_toi,
+/// candidates left with a root at t = 0.
 int oracle_step(void* wp, double* stats) {
   World& w = *static_cast<World*>(wp);
   const bool ok = oracle::step(w);
@@ -1216,6 +1249,7 @@ int oracle_step(void* wp, double* stats) {
     stats[0] = w.stats.outer; stats[1] = w.stats.admm; stats[2] = w.stats.pcg;
     stats[3] = w.stats.contacts; stats[4] = w.stats.colors;
     stats[5] = w.stats.ccd_cand; stats[6] = w.stats.min_toi;
+    stats[7] = w.stats.ccd_zero;
   }
   return ok ? 0 : 1;
 }

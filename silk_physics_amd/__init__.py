@@ -74,7 +74,7 @@ def load_library():
                                                 C.POINTER(i64)]),
         "silk_hip_debug_dcd": (C.c_int, [vp, fp, fp, i64, C.c_float, C.c_float, ip,
                                          fp, i64, C.POINTER(i64)]),
-        "silk_hip_debug_decisions": (C.c_int, [vp, fp, i32, ip, ip, i64,
+        "silk_hip_debug_decisions": (C.c_int, [vp, fp, ip, i32, ip, ip, i64,
                                                C.POINTER(i64)]),
         "silk_hip_debug_ccd": (C.c_int, [vp, i64, ip, dp, dp, dp, C.c_double,
                                          C.c_double, i32, ip, dp]),
@@ -107,6 +107,7 @@ MAS_FP32, MAS_FP16, MAS_BF16 = 0, 1, 2
 PCG_RR0, PCG_RHS, PCG_TRUE = 0, 1, 2
 WARM_PREVIOUS, WARM_LEARNED, WARM_EXTRAPOLATE = 0, 1, 2
 PROJECT_RESIDUAL, PROJECT_GALERKIN = 0, 1
+CCD_ZERO_SKIP, CCD_ZERO_CONTACT = 0, 1
 WARM_NET_PARAMS = 5313  # SILK_HIP_WARM_NET_PARAMS
 
 class SilkHipClothParams(C.Structure):
@@ -135,7 +136,8 @@ class SilkHipGlobal(C.Structure):
                 ("dcd_activation_scale", C.c_float),
                 ("mas_precision", C.c_int32), ("true_residual_stats", C.c_int32),
                 ("pcg_criterion", C.c_int32), ("warm_start", C.c_int32),
-                ("warm_projection", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("warm_projection", C.c_int32), ("ccd_zero_toi", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class SilkHipStepStats(C.Structure):
@@ -149,7 +151,7 @@ class SilkHipStepStats(C.Structure):
                 ("pcg_max_solve_iterations", C.c_int32),
                 ("pcg_stalled_solves", C.c_int32), ("warm_start_kept", C.c_int32),
                 ("warm_start_solves", C.c_int32), ("pcg_restarts", C.c_int32),
-                ("pcg_unverified_solves", C.c_int32)]
+                ("pcg_unverified_solves", C.c_int32), ("ccd_zero_toi_pairs", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -247,6 +249,7 @@ class GlobalConfig:
     warm_start: int = 0           # WARM_PREVIOUS (reference) / WARM_LEARNED /
                                   # WARM_EXTRAPOLATE
     warm_projection: int = 0      # PROJECT_RESIDUAL / PROJECT_GALERKIN
+    ccd_zero_toi: int = 0         # CCD_ZERO_SKIP (reference) / CCD_ZERO_CONTACT
 
     def _c(self) -> SilkHipGlobal:
         g = SilkHipGlobal()
@@ -260,7 +263,7 @@ class GlobalConfig:
                      "contact_sweeps", "ccd_max_iter", "ccd_tolerance",
                      "ccd_min_separation_scale", "dcd_activation_scale",
                      "mas_precision", "true_residual_stats", "pcg_criterion",
-                     "warm_start", "warm_projection"):
+                     "warm_start", "warm_projection", "ccd_zero_toi"):
             setattr(g, name, getattr(self, name))
         return g
 
@@ -531,20 +534,22 @@ class World:
     def decisions(self):
         """Discrete decisions of the last step: (TOI of every outer
         iteration, float32; appended contacts (n, 6): DCD call, type, four
-        global vertex ids in record order)."""
+        global vertex ids in record order; ADMM iterations of every outer
+        iteration)."""
         lib = load_library()
         nt, nc = C.c_int32(0), C.c_int64(0)
-        st = lib.silk_hip_debug_decisions(self._ctx, None, 0, C.byref(nt), None, 0,
+        st = lib.silk_hip_debug_decisions(self._ctx, None, None, 0, C.byref(nt), None, 0,
                                           C.byref(nc))
         if st != 0:
             raise RuntimeError(self._wrap(st).to_string())
         tois = np.zeros(max(1, nt.value), dtype=np.float32)
+        admm = np.zeros(max(1, nt.value), dtype=np.int32)
         recs = np.zeros((max(1, nc.value), 6), dtype=np.int32)
-        st = lib.silk_hip_debug_decisions(self._ctx, _fptr(tois), nt.value, C.byref(nt),
-                                          _iptr(recs), nc.value, C.byref(nc))
+        st = lib.silk_hip_debug_decisions(self._ctx, _fptr(tois), _iptr(admm), nt.value,
+                                          C.byref(nt), _iptr(recs), nc.value, C.byref(nc))
         if st != 0:
             raise RuntimeError(self._wrap(st).to_string())
-        return tois[:nt.value], recs[:nc.value]
+        return tois[:nt.value], recs[:nc.value], admm[:nt.value]
 
     def debug_ccd(self, is_vf, p0, p1, err, ms, tolerance, max_iter=4096):
         """The step's FP64 CCD kernel on explicit queries: (hit, toi)."""

@@ -104,6 +104,9 @@ int silk_hip_set_global(silk_hip_ctx* ctx, const silk_hip_global* g) {
     else if (g->warm_projection != SILK_HIP_PROJECT_RESIDUAL &&
              g->warm_projection != SILK_HIP_PROJECT_GALERKIN)
       bad = "warm_projection must be SILK_HIP_PROJECT_RESIDUAL or SILK_HIP_PROJECT_GALERKIN";
+    else if (g->ccd_zero_toi != SILK_HIP_CCD_ZERO_SKIP &&
+             g->ccd_zero_toi != SILK_HIP_CCD_ZERO_CONTACT)
+      bad = "ccd_zero_toi must be SILK_HIP_CCD_ZERO_SKIP or SILK_HIP_CCD_ZERO_CONTACT";
     else if (g->warm_start == SILK_HIP_WARM_LEARNED && !c.has_warm_net())
       bad = "warm_start = SILK_HIP_WARM_LEARNED needs a network "
             "(silk_hip_set_warm_start_net)";
@@ -451,15 +454,16 @@ int silk_hip_debug_dcd(silk_hip_ctx* ctx, const float* pos0, const float* pos1,
   });
 }
 
-int silk_hip_debug_decisions(silk_hip_ctx* ctx, float* tois, int32_t cap_toi,
-                             int32_t* n_toi, int32_t* contacts, int64_t cap_contacts,
-                             int64_t* n_contacts) {
+int silk_hip_debug_decisions(silk_hip_ctx* ctx, float* tois, int32_t* admm_iters,
+                             int32_t cap_toi, int32_t* n_toi, int32_t* contacts,
+                             int64_t cap_contacts, int64_t* n_contacts) {
   return guarded(ctx, [&](Context& c) {
     if (n_toi == nullptr || n_contacts == nullptr) return int{SILK_HIP_ERR_INVALID_CONFIG};
     const std::vector<float>& tl = c.toi_log_;
     *n_toi = static_cast<int32_t>(tl.size());
-    if (tois != nullptr)
-      std::copy_n(tl.begin(), std::min<int64_t>(cap_toi, *n_toi), tois);
+    const int64_t nt = std::min<int64_t>(cap_toi, *n_toi);
+    if (tois != nullptr) std::copy_n(tl.begin(), nt, tois);
+    if (admm_iters != nullptr) std::copy_n(c.admm_log_.begin(), nt, admm_iters);
     const CollisionWorld* cw = c.scene_ ? c.scene_->coll.get() : nullptr;
     static const std::vector<int> none;
     const std::vector<int>& log = cw != nullptr ? cw->decision_log() : none;
@@ -495,7 +499,7 @@ int silk_hip_debug_ccd(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
     dtoi.resize(m);
     silk::amd::launch_ccd_each(c.stream_, static_cast<int>(n), dv.get(),
                                d0.get(), d1.get(), de.get(), ms, tolerance,
-                               max_iter, dhit.get(), dtoi.get());
+                               max_iter, dhit.get(), dtoi.get(), c.ccd_zero_mode());
     silk::amd::hip_check(hipGetLastError(), "ccd launch");
     silk::amd::hip_check(hipMemcpyAsync(hit, dhit.get(), m * sizeof(int),
                                         hipMemcpyDeviceToHost, c.stream_),
@@ -534,7 +538,7 @@ int silk_hip_debug_ccd_min(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
     silk::amd::launch_ccd_min(
         c.stream_, static_cast<int>(n), dv.get(), d0.get(), d1.get(), de.get(),
         ms, tolerance, max_iter,
-        reinterpret_cast<unsigned long long*>(dmin.get()));
+        reinterpret_cast<unsigned long long*>(dmin.get()), c.ccd_zero_mode());
     silk::amd::hip_check(hipGetLastError(), "ccd launch");
     silk::amd::hip_check(hipMemcpyAsync(min_toi, dmin.get(), sizeof(double),
                                         hipMemcpyDeviceToHost, c.stream_),

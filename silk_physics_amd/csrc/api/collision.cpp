@@ -881,6 +881,9 @@ float HipCollision::min_toi(const float* outer_x, const float* inner_x,
   cp.interval = interval;
   cp.err_vf = err6_.get();
   cp.err_ee = err6_.get() + 3;
+  cp.zero_mode = ctx_.ccd_zero_mode();
+  // Counter 3 (cleared with the pair counters by every broadphase launch).
+  cp.zero_count = counters_.get() + 3;
   for (;;) {
     const double one = 1.0;
     unsigned long long bits;
@@ -913,7 +916,10 @@ float HipCollision::min_toi(const float* outer_x, const float* inner_x,
       again = true;
     }
     if (again) continue;
-    if (stats) stats->ccd_candidates += npt + nee;
+    if (stats) {
+      stats->ccd_candidates += npt + nee;
+      stats->ccd_zero_toi_pairs += h_io_[3];
+    }
     double toi;
     std::memcpy(&toi, h_io_ + 4, sizeof(toi));
     return static_cast<float>(static_cast<double>(interval) * toi);

@@ -608,14 +608,11 @@ void Context::build_scene() {
         ps_wg_n1_ptr.push_back(static_cast<int>(ps_wg_n1.size()));
         ps_wg_c.insert(ps_wg_c.end(), cs.begin(), cs.end());
         ps_wg_c_ptr.push_back(static_cast<int>(ps_wg_c.size()));
-        // Tagged words of the two exchanges of an iteration.
-        const int cap = kPcgThreads * kPcgXchWords;
+        // The halo slots a workgroup completes (the exchanges themselves
+        // have no size limit: words beyond the ones kept in registers are
+        // mapped when collected).
         const int nh = ps_halo_ptr[w + 1] - ps_halo_ptr[w];
-        if (3 * static_cast<int>(n1s.size()) + 2 * kPcgWaves * sd.nwg + sd.ncw > cap ||
-            4 * kPcgWaves * sd.nwg + 2 * sd.ncb + 3 * nh +
-                    3 * static_cast<int>(cs.size()) > cap ||
-            nh > kPcgThreads * kPcgHaloPerThread)
-          xch_fits = false;
+        if (nh > kPcgThreads * kPcgHaloPerThread) xch_fits = false;
       }
     }
   }
@@ -1137,10 +1134,16 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
   ca.lin_init = g.initial_linear_rel_tol;
   ca.lin_min = g.linear_solver_rel_tol_min;
   ca.lin_max = g.linear_solver_rel_tol_max;
+  const bool keep_x0 = g.true_residual_stats || verify;
   for (int it = 0; it < g.max_inner_iteration; ++it) {
+    // Without a projected start the residual kernel also keeps the solve's
+    // start x for the true-residual test.
+    const bool fold_copy = keep_x0 && it != 0 && !projected;
     if (it != 0) {
+      RhsArgs r0 = ra;
+      if (fold_copy) r0.x_copy = S.x0_copy.get();
       timed_begin(kTimeResidual);
-      launch_pcg_residual(stream_, p, ra);
+      launch_pcg_residual(stream_, p, r0);
       timed_end();
       if (projected) {
         // Keep the previous iterate and its residual, project the start onto
@@ -1177,7 +1180,7 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
         launch_warm_features(stream_, p, wf, S.wfeat.get(), S.wsig.get());
       }
       if (it == 1) launch_mas_factor(stream_, p, S.cd.get(), skip);
-      if (g.true_residual_stats || verify)
+      if (keep_x0 && !fold_copy)
         hip_check(hipMemcpyAsync(S.x0_copy.get(), S.inner_x.get(), n3 * sizeof(float),
                                  hipMemcpyDeviceToDevice, stream_),
                   "warm start copy");
@@ -1259,6 +1262,7 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
   hip_check(hipStreamSynchronize(stream_), "admm sync");
   timed_collect();
   const AdmmCtl& hc = S.h_admm_ctl[0];
+  last_admm_iters_ = hc.admm_iters;
   const int abort_flag = *reinterpret_cast<const int*>(S.h_admm_ctl + 1);
   if (abort_flag != 0) {
     S.ps_bar.zero(stream_);
@@ -1318,6 +1322,7 @@ bool Context::admm_solve(silk_hip_step_stats* stats) {
   ra.cw = cv.w;
   ra.cp = cv.p;
   ra.ceta = cv.eta;
+  last_admm_iters_ = 0;
   for (int it = 0; it < g.max_inner_iteration; ++it) {
     if (it != 0) {
       const float rel = std::clamp(
@@ -1343,6 +1348,7 @@ bool Context::admm_solve(silk_hip_step_stats* stats) {
     launch_local_step(stream_, c, a);
     S.aux_cur ^= 1;
     if (stats) stats->admm_iterations += 1;
+    ++last_admm_iters_;
     if (it == 0) continue;
 
     download(S.h_admm_part, S.admm_part.get(),
@@ -1399,6 +1405,7 @@ int Context::step(silk_hip_step_stats* stats) {
   float remaining = 1.0f;
   float accepted_toi = 1.0f;
   toi_log_.clear();
+  admm_log_.clear();
   int outer = 0;
   for (; outer < global_.max_outer_iteration; ++outer) {
     launch_predict(stream_, c, S.outer_x.get(), S.outer_v.get(),
@@ -1412,6 +1419,7 @@ int Context::step(silk_hip_step_stats* stats) {
       min_toi = S.coll->min_toi(S.outer_x.get(), S.inner_x.get(),
                                 1.0f - remaining, remaining, ms, stats);
       toi_log_.push_back(min_toi);
+      admm_log_.push_back(last_admm_iters_);
     }
     launch_velocity(stream_, static_cast<int>(n3), S.dt, S.outer_x.get(),
                     S.inner_x.get(), S.outer_v.get());

@@ -152,6 +152,8 @@ class Context {
   /// TOI (frame-time units) returned by the CCD of every outer iteration of
   /// the last step, in order.
   std::vector<float> toi_log_;
+  std::vector<int> admm_log_;  // ADMM iterations of each outer iteration
+  int last_admm_iters_ = 0;     // of the last admm_solve
 
   /// Learned warm start: network (FP32 master, Adam moments, BF16 copies),
   /// training scratch and the recorded samples.  Device buffers of the
@@ -238,6 +240,7 @@ class Context {
   int contact_sweeps() const { return global_.contact_sweeps > 0 ? global_.contact_sweeps : 10; }
   int ccd_max_iter() const { return global_.ccd_max_iter > 0 ? global_.ccd_max_iter : 4096; }
   float ccd_tolerance() const { return global_.ccd_tolerance > 0 ? global_.ccd_tolerance : 1e-6f; }
+  int ccd_zero_mode() const { return global_.ccd_zero_toi; }
   float ccd_ms_scale() const { return global_.ccd_min_separation_scale > 0 ? global_.ccd_min_separation_scale : 0.2f; }
   float dcd_scale() const { return global_.dcd_activation_scale > 0 ? global_.dcd_activation_scale : 1.0f; }
 

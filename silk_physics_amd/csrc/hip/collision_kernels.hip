@@ -923,15 +923,20 @@ struct ExplicitQueries {
   }
 };
 
-// Outputs are written by every lane of the wavefront with the same values.
+// Outputs are written by every lane of the wavefront with the same values
+// (zero() is called by lane 0 only).
 // Earliest TOI over the candidates (toi == 0 is not a collision).  Queries
 // that cannot lower it are cut short.
 struct MinToiOut {
   unsigned long long* toi_bits;
+  int* zero_count;  // may be null
   __device__ const unsigned long long* cutoff() const { return toi_bits; }
   __device__ void put(int, bool hit, double toi) const {
     if (!hit || toi == 0.0) return;
     atomicMin(toi_bits, static_cast<unsigned long long>(__double_as_longlong(toi)));
+  }
+  __device__ void zero() const {
+    if (zero_count != nullptr) atomicAdd(zero_count, 1);
   }
 };
 
@@ -943,17 +948,20 @@ struct EachOut {
     hit[q] = h ? 1 : 0;
     toi[q] = h ? t : 0.0;
   }
+  __device__ void zero() const {}
 };
 
 // CCD, one query per wavefront at a time: wavefront w of W takes queries
 // w, w + W, ... (the grid holds as many wavefronts as can be resident; a
 // shared query counter serialised every wavefront on one atomic).  Per query
-// (ccd.cpp:235-256, narrowphase.cu:70-76): the root search at ms; a root at
+// (ccd.cpp:150-180, narrowphase.cu:68): the root search at ms; a root at
 // t = 0 is searched again at ms / 2, where no root means a collision at t = 1
-// and a root at t = 0 means none.
+// and a root at t = 0 means none.  With SILK_HIP_CCD_ZERO_CONTACT a root at
+// t = 0 at ms / 2 is searched a third time at zero separation, with the
+// same meaning of its outcomes.
 template <class In, class Out>
 __global__ void __launch_bounds__(256)
-    k_ccd(In in, Out out, double tolerance, int max_iter) {
+    k_ccd(In in, Out out, double tolerance, int max_iter, int zero_mode) {
   const int n = in.count();
   const int lane = static_cast<int>(__lane_id());
   const int nwave = static_cast<int>(gridDim.x * blockDim.x) / dev::kWave;
@@ -969,7 +977,7 @@ __global__ void __launch_bounds__(256)
     for (int d = 0; d < 3; ++d) unit_range(vf, p0, p1, d, &umn[d], &umx[d]);
     const double ms = in.ms();
     const unsigned long long* cut = out.cutoff();
-    bool hit = false, pruned = false;
+    bool hit = false, pruned = false, zero = false;
     double toi = 0.0, t = 0.0;
     const int r1 = wave_root_find(vf, umn, umx, k, tol, tolerance, err, ms,
                                   max_iter, cut, &t);
@@ -980,21 +988,32 @@ __global__ void __launch_bounds__(256)
         hit = true;
         toi = t;
       } else {
-        double t2 = 0.0;
-        const int r2 = wave_root_find(vf, umn, umx, k, tol, tolerance, err,
-                                      ms * 0.5, max_iter, cut, &t2);
-        if (r2 == kRootPruned) {
-          pruned = true;
-        } else if (r2 == kRootNone) {
-          hit = true;
-          toi = 1.0;
-        } else if (t2 > 0.0) {
-          hit = true;
-          toi = t2;
+        // Searches again at the smaller separations while the root stays at
+        // t = 0 (the loop bound and separation are wavefront-uniform).
+        const int passes = zero_mode == 1 ? 2 : 1;
+        zero = true;
+        for (int pass = 0; pass < passes && zero; ++pass) {
+          double t2 = 0.0;
+          const double sep = pass == 0 ? ms * 0.5 : 0.0;
+          const int r2 = wave_root_find(vf, umn, umx, k, tol, tolerance, err, sep,
+                                        max_iter, cut, &t2);
+          if (r2 == kRootPruned) {
+            pruned = true;
+            zero = false;
+          } else if (r2 == kRootNone) {
+            hit = true;
+            toi = 1.0;
+            zero = false;
+          } else if (t2 > 0.0) {
+            hit = true;
+            toi = t2;
+            zero = false;
+          }
         }
       }
     }
     if (!pruned) out.put(q, hit, toi);
+    if (zero && lane == 0) out.zero();
   }
 }
 
@@ -1437,26 +1456,28 @@ void launch_ccd(hipStream_t s, const CollVerts& v, const PrimTables& pt,
                 const CcdParams& cp, unsigned long long* toi_bits) {
   PairQueries in{v, pt, pt_pairs, pt_count, pt_cap, ee_pairs, ee_count, ee_cap, cp};
   k_ccd<<<ccd_blocks<PairQueries, MinToiOut>(), 256, 0, s>>>(
-      in, MinToiOut{toi_bits}, cp.tolerance, cp.max_iter);
+      in, MinToiOut{toi_bits, cp.zero_count}, cp.tolerance, cp.max_iter, cp.zero_mode);
 }
 
 void launch_ccd_each(hipStream_t s, int n, const int* is_vf, const double* p0,
                      const double* p1, const double* err, double ms,
-                     double tolerance, int max_iter, int* hit, double* toi) {
+                     double tolerance, int max_iter, int* hit, double* toi,
+                     int zero_mode) {
   if (n <= 0) return;
   ExplicitQueries in{n, is_vf, p0, p1, err, ms};
   const int blocks = std::min(ccd_blocks<ExplicitQueries, EachOut>(), (n + 3) / 4);
-  k_ccd<<<blocks, 256, 0, s>>>(in, EachOut{hit, toi}, tolerance, max_iter);
+  k_ccd<<<blocks, 256, 0, s>>>(in, EachOut{hit, toi}, tolerance, max_iter, zero_mode);
 }
 
 void launch_ccd_min(hipStream_t s, int n, const int* is_vf, const double* p0,
                     const double* p1, const double* err, double ms,
                     double tolerance, int max_iter,
-                    unsigned long long* toi_bits) {
+                    unsigned long long* toi_bits, int zero_mode) {
   if (n <= 0) return;
   ExplicitQueries in{n, is_vf, p0, p1, err, ms};
   const int blocks = std::min(ccd_blocks<ExplicitQueries, MinToiOut>(), (n + 3) / 4);
-  k_ccd<<<blocks, 256, 0, s>>>(in, MinToiOut{toi_bits}, tolerance, max_iter);
+  k_ccd<<<blocks, 256, 0, s>>>(in, MinToiOut{toi_bits, nullptr}, tolerance, max_iter,
+                               zero_mode);
 }
 
 namespace {

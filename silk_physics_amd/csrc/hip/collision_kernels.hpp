@@ -159,6 +159,8 @@ struct CcdParams {
   double interval = 1.0;
   const double* err_vf = nullptr;  // device, 3 doubles
   const double* err_ee = nullptr;
+  int zero_mode = 0;           // SILK_HIP_CCD_ZERO_SKIP / _CONTACT
+  int* zero_count = nullptr;   // device: candidates left with a root at t = 0
 };
 
 /// Per-axis max |coordinate| over all collider vertices at t0 and t1, then the
@@ -182,18 +184,20 @@ void launch_ccd(hipStream_t s, const CollVerts& v, const PrimTables& pt,
 
 /// Per-query CCD on explicit FP64 end positions (12 doubles each for p0 and
 /// p1, err 3 doubles per query) with the kernel of launch_ccd: hit[i] = 1 and
-/// toi[i] as the step's CCD computes them (a root at t = 0 at both
-/// separations is no hit).
+/// toi[i] as the step's CCD computes them under zero_mode (a root at t = 0
+/// at the last separation searched is no hit).
 void launch_ccd_each(hipStream_t s, int n, const int* is_vf, const double* p0,
                      const double* p1, const double* err, double ms,
-                     double tolerance, int max_iter, int* hit, double* toi);
+                     double tolerance, int max_iter, int* hit, double* toi,
+                     int zero_mode = 0);
 
 /// Earliest TOI over explicit queries with the kernel of launch_ccd
 /// (queries that cannot lower the running minimum are cut short); *toi_bits
 /// starts at 1.0.
 void launch_ccd_min(hipStream_t s, int n, const int* is_vf, const double* p0,
                     const double* p1, const double* err, double ms,
-                    double tolerance, int max_iter, unsigned long long* toi_bits);
+                    double tolerance, int max_iter, unsigned long long* toi_bits,
+                    int zero_mode = 0);
 
 // ---------------------------------------------------------------- contacts
 

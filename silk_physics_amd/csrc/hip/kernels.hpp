@@ -94,6 +94,7 @@ struct AdmmCtl {
                           // meets the tolerance (or the ADMM loop stopped)
   int pcg_unverified = 0; // solves still above the tolerance after the last
                           // restart
+  int verify_arrive = 0;  // row blocks done in the verification kernel
   float rel_tol = 0;  // linear tolerance of the next x-update
   float last_rel = 0; // sqrt(rr / rr0) of the last PCG solve
   double init_primal = 0, init_dual = 0, ratio = 1;
@@ -216,6 +217,8 @@ struct RhsArgs {
   const float* cp = nullptr;
   const float* ceta = nullptr;
   const int* skip = nullptr;           // no-op when *skip != 0
+  float* x_copy = nullptr;             // optional: the residual kernel copies
+                                       //   x here (the solve's start x0)
 };
 
 /// PCG init, part 1 (mas_cg_solver.cu:273-277): d and r0 = b - (K + D) x0
@@ -362,16 +365,18 @@ struct PersistDev {
   int dense = 0;        // 1: the two-workgroups-per-CU variant
 };
 
-/// Tagged words one thread of the persistent kernel polls per exchange:
-/// an exchange of n words fits when n <= kPcgThreads * kPcgXchWords.
+/// Tagged words of an exchange one thread of the persistent kernel keeps the
+/// addresses of in registers; the words beyond kPcgThreads * kPcgXchWords are
+/// mapped when collected, kPcgXchBatch per thread at a time.
 constexpr int kPcgXchWords = 8;
+constexpr int kPcgXchBatch = 4;
 /// Wavefronts per persistent-PCG workgroup; each publishes its own partial
 /// sums (two words per double): per workgroup, p.q at [0, 8), r.z0 at
 /// [8, 16), r.r at [16, 24) of kPcgPartWords words.
 constexpr int kPcgWaves = kPcgThreads / 64;
 constexpr int kPcgPartWords = 32;
-/// Halo slots one thread handles (an exchange-2 list fits in
-/// kPcgThreads * kPcgXchWords words, so a halo has fewer than 683 slots).
+/// Halo slots one thread completes (a workgroup's halo has at most
+/// kPcgThreads * kPcgHaloPerThread slots).
 constexpr int kPcgHaloPerThread = 3;
 
 /// Dynamic LDS bytes of the persistent kernel for the sizes in `a`.

@@ -145,6 +145,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(PcgDev p, RhsArgs ra,
       r_out[3 * row + k] = static_cast<float>(res);
       rr += res * res;
       bb += rhs[k] * rhs[k];
+      if (ra.x_copy != nullptr) ra.x_copy[3 * row + k] = x[3 * row + k];
     }
   }
   const double s0 = block_sum(rr, lds);
@@ -216,20 +217,26 @@ __global__ void __launch_bounds__(kBlock) k_solve_residual_max(
   }
 }
 
-// FP64 residual of the iterate x0 + dx per row block: (|b - A (x0 + dx)|^2,
-// |b|^2), x0 and dx applied separately as in k_solve_residuals.
-__global__ void __launch_bounds__(kBlock) k_iterate_residual(
-    PcgDev p, RhsArgs ra, const float* __restrict__ x0,
-    const float* __restrict__ dx, double* __restrict__ part2, const AdmmCtl* ctl,
-    const int* sys_done) {
+// True-residual test of a finished solve in one launch.  Every row block
+// adds (|b - A (x0 + dx)|^2, |b|^2) of its rows (x0 and dx applied
+// separately, never summed in FP32) to part2; the last block to finish then
+// tests every system against max(abs^2, rel^2 |b|^2), writes sys_done and
+// ctl->verify_done, and folds the ratios into ctl->max_true_bits.
+__global__ void __launch_bounds__(kBlock) k_iterate_verify(
+    PcgDev p, RhsArgs ra, const float* __restrict__ x0, const float* __restrict__ dx,
+    double* __restrict__ part2, int* sys_done, AdmmCtl* ctl, float rel_tol, int first,
+    int final) {
   __shared__ double lds[17];
-  if (ctl != nullptr && ctl->done != 0) return;
+  __shared__ int last, all_ok;
+  if (ctl->done != 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->verify_done = 1;
+    return;
+  }
   const int b = blockIdx.x;
-  if (sys_done != nullptr && sys_done[p.blk_sys[b]] != 0) return;
-  const int row = p.blk_row0[b] + threadIdx.x;
-  const bool valid = threadIdx.x < p.blk_rows[b];
+  const bool skip_blk = !first && sys_done[p.blk_sys[b]] != 0;
   double ri = 0.0, bb = 0.0;
-  if (valid) {
+  if (!skip_blk && threadIdx.x < p.blk_rows[b]) {
+    const int row = p.blk_row0[b] + threadIdx.x;
     double rhs[3], dd;
     rhs_row(ra, row, rhs, dd);
     double a0[3], a1[3];
@@ -245,48 +252,45 @@ __global__ void __launch_bounds__(kBlock) k_iterate_residual(
   const double s0 = block_sum(ri, lds);
   const double s1 = block_sum(bb, lds);
   if (threadIdx.x == 0) {
-    part2[2 * b] = s0;
-    part2[2 * b + 1] = s1;
+    if (!skip_blk) {
+      part2[2 * b] = s0;
+      part2[2 * b + 1] = s1;
+    }
+    __threadfence();
+    last = atomicAdd(&ctl->verify_arrive, 1) == static_cast<int>(gridDim.x) - 1;
+    all_ok = 1;
   }
-}
-
-// One workgroup: every system's test, then the all-systems flag.
-__global__ void __launch_bounds__(kBlock) k_iterate_verify(
-    PcgDev p, const double* __restrict__ part2, int* sys_done, AdmmCtl* ctl,
-    float rel_tol, int first, int final) {
-  __shared__ double lds[17];
-  __shared__ int all_ok;
-  if (ctl->done != 0) {
-    if (threadIdx.x == 0) ctl->verify_done = 1;
-    return;
-  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
   const float rel = ctl->rel_tol > 0.0f ? ctl->rel_tol : rel_tol;
   const double rel2 = static_cast<double>(rel) * rel;
   const double abs2 = static_cast<double>(p.abs_tol) * p.abs_tol;
-  if (threadIdx.x == 0) all_ok = 1;
-  __syncthreads();
   for (int sys = 0; sys < p.nsys; ++sys) {
     if (!first && sys_done[sys] != 0) continue;  // same value in every thread
     const int b0 = p.sys_blk0[sys], b1 = p.sys_blk0[sys + 1];
     double v0 = 0.0, v1 = 0.0;
     for (int q = b0 + threadIdx.x; q < b1; q += blockDim.x) {
-      v0 += part2[2 * q];
-      v1 += part2[2 * q + 1];
+      v0 += __hip_atomic_load(&part2[2 * q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      v1 += __hip_atomic_load(&part2[2 * q + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     const double rr = block_sum(v0, lds);
-    const double bb = block_sum(v1, lds);
+    const double bsq = block_sum(v1, lds);
     if (threadIdx.x == 0) {
-      const bool ok = rr <= abs2 || rr <= rel2 * bb;
+      const bool ok = rr <= abs2 || rr <= rel2 * bsq;
       sys_done[sys] = ok ? 1 : 0;
       if (!ok) all_ok = 0;
-      if ((ok || final) && bb > 0.0)
+      if ((ok || final) && bsq > 0.0)
         atomicMax(&ctl->max_true_bits,
-                  static_cast<unsigned long long>(__double_as_longlong(rr / bb)));
+                  static_cast<unsigned long long>(__double_as_longlong(rr / bsq)));
       if (!ok && final) ctl->pcg_unverified += 1;
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) ctl->verify_done = all_ok;
+  if (threadIdx.x == 0) {
+    ctl->verify_done = all_ok;
+    ctl->verify_arrive = 0;
+  }
 }
 
 __global__ void k_copy_unless(float* __restrict__ dst, const float* __restrict__ src,
@@ -673,9 +677,8 @@ void launch_iterate_verify(hipStream_t s, const PcgDev& p, const RhsArgs& ra,
                            int* sys_done, AdmmCtl* ctl, float rel_tol, int first,
                            int final) {
   if (p.nblk == 0) return;
-  k_iterate_residual<<<p.nblk, kBlock, 0, s>>>(p, ra, x0, dx, part3, ctl,
-                                               first ? nullptr : sys_done);
-  k_iterate_verify<<<1, kBlock, 0, s>>>(p, part3, sys_done, ctl, rel_tol, first, final);
+  k_iterate_verify<<<p.nblk, kBlock, 0, s>>>(p, ra, x0, dx, part3, sys_done, ctl, rel_tol,
+                                             first, final);
 }
 
 void launch_copy_unless(hipStream_t s, float* dst, const float* src, int64_t n,

@@ -127,6 +127,59 @@ __device__ bool collect(const Share& sh, unsigned epoch, float* lds,
   return true;
 }
 
+// Words i >= kPcgThreads * kPcgXchWords of an exchange (a coarse bank that
+// reads the level-1 sums of a whole system, a large halo): thread t polls
+// words t + kPcgThreads * m, m >= kPcgXchWords, kPcgXchBatch at a time,
+// mapping each with word(i, &addr, &dest) when its batch starts.  Returns
+// false when the wait gave up.
+template <class WordFn>
+__device__ bool collect_rest(int total, WordFn&& word, unsigned epoch, float* lds,
+                             int* abort_flag) {
+  constexpr int kStride = kPcgThreads * kPcgXchBatch;
+  for (int i0 = static_cast<int>(threadIdx.x) + kPcgThreads * kPcgXchWords; i0 < total;
+       i0 += kStride) {
+    const unsigned long long* addr[kPcgXchBatch];
+    int dest[kPcgXchBatch];
+    unsigned pending = 0;
+#pragma unroll
+    for (int m = 0; m < kPcgXchBatch; ++m) {
+      const int i = i0 + kPcgThreads * m;
+      addr[m] = nullptr;
+      dest[m] = 0;
+      if (i < total) {
+        word(i, addr[m], dest[m]);
+        pending |= 1u << m;
+      }
+    }
+    unsigned spins = 0;
+    while (pending != 0) {
+      unsigned long long v[kPcgXchBatch];
+#pragma unroll
+      for (int m = 0; m < kPcgXchBatch; ++m) {
+        v[m] = 0;
+        if (pending & (1u << m))
+          v[m] = __hip_atomic_load(const_cast<unsigned long long*>(addr[m]),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+#pragma unroll
+      for (int m = 0; m < kPcgXchBatch; ++m)
+        if ((pending & (1u << m)) && static_cast<unsigned>(v[m] >> 32) == epoch) {
+          lds[dest[m]] = __uint_as_float(static_cast<unsigned>(v[m]));
+          pending &= ~(1u << m);
+        }
+      if (pending == 0) break;
+      if ((++spins & 255) == 0 &&
+          (spins > kSpinLimit ||
+           __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+        __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return false;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  return true;
+}
+
 // Sum over the wavefront; the same bits in every lane and in every
 // wavefront summing the same values (sum64).
 __device__ __forceinline__ double wave_sum(double v) { return sum64(v); }
@@ -436,62 +489,77 @@ __global__ void __launch_bounds__(kPcgThreads, kWavesPerSimd)
   constexpr int kWaveWords = 2 * kPcgWaves;  // one double per wavefront
   // A coarse workgroup's word: its wavefront 0's (zero) p.q partial.
   constexpr int kAckWord = 0;
+  const int nw1 = 3 * nn1;
+  const int aw1 = nw1 + kWaveWords * S.nwg;
+  const int w1 = aw1 + S.ncw;
+  const int pw2 = 2 * kWaveWords * S.nwg;
+  const int cw2 = pw2 + 2 * S.ncb;
+  const int hw2 = cw2 + 3 * nh;
+  const int w2 = hw2 + 3 * nc;
+  // Word i of each exchange: its tagged word and the float of dyn that
+  // receives its payload.
+  auto x1_word = [&](int i, const unsigned long long*& addr, int& dest) {
+    if (i < nw1) {
+      const int n = a.wg_n1[n1_0 + i / 3];
+      addr = a.tn + 3 * static_cast<size_t>(n) + i % 3;
+      dest = static_cast<int>(qs - dyn) + 4 * (i / 3) + i % 3;
+    } else if (i < aw1) {
+      const int w = (i - nw1) / kWaveWords, h = (i - nw1) % kWaveWords;
+      addr = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + h;
+      dest = static_cast<int>(ppq - dyn) + kWaveWords * w + h;
+    } else {
+      const int w = S.nwg + (i - aw1);
+      addr = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + kAckWord;
+      dest = static_cast<int>(ppq - dyn) + kWaveWords * w + kAckWord;
+    }
+  };
+  auto x2_word = [&](int i, const unsigned long long*& addr, int& dest) {
+    if (i < pw2) {
+      const int w = i / (2 * kWaveWords), h = i % (2 * kWaveWords);
+      addr = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + kWaveWords + h;
+      dest = static_cast<int>((h < kWaveWords ? prz : prr) - dyn) + kWaveWords * w +
+             h % kWaveWords;
+    } else if (i < cw2) {
+      addr = a.tcp + 2 * static_cast<size_t>(S.cb0) + (i - pw2);
+      dest = static_cast<int>(cpw - dyn) + (i - pw2);
+    } else if (i < hw2) {
+      const int hh = (i - cw2) / 3, c = (i - cw2) % 3;
+      addr = a.tz + 3 * static_cast<size_t>(a.halo_slot[h0 + hh]) + c;
+      dest = static_cast<int>(zl - dyn) + 4 * (kPcgThreads + hh) + c;
+    } else {
+      const int n = a.wg_c[c_0 + (i - hw2) / 3], c = (i - hw2) % 3;
+      addr = a.tc + 3 * static_cast<size_t>(S.cn0 + n) + c;
+      dest = static_cast<int>(czl - dyn) + 4 * n + c;
+    }
+  };
   Share x1, x2;
-  {
-    const int nw1 = 3 * nn1;
-    const int aw1 = nw1 + kWaveWords * S.nwg;
-    const int w1 = aw1 + S.ncw;
-    const int pw2 = 2 * kWaveWords * S.nwg;
-    const int cw2 = pw2 + 2 * S.ncb;
-    const int hw2 = cw2 + 3 * nh;
-    const int w2 = hw2 + 3 * nc;
-    x1.mask = 0;
-    x2.mask = 0;
+  x1.mask = 0;
+  x2.mask = 0;
 #pragma unroll
-    for (int m = 0; m < kPcgXchWords; ++m) {
-      const int i = t + kPcgThreads * m;
-      x1.addr[m] = a.tp;
-      x1.dest[m] = 0;
-      if (i < w1) {
-        x1.mask |= 1u << m;
-        if (i < nw1) {
-          const int n = a.wg_n1[n1_0 + i / 3];
-          x1.addr[m] = a.tn + 3 * static_cast<size_t>(n) + i % 3;
-          x1.dest[m] = static_cast<int>(qs - dyn) + 4 * (i / 3) + i % 3;
-        } else if (i < aw1) {
-          const int w = (i - nw1) / kWaveWords, h = (i - nw1) % kWaveWords;
-          x1.addr[m] = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + h;
-          x1.dest[m] = static_cast<int>(ppq - dyn) + kWaveWords * w + h;
-        } else {
-          const int w = S.nwg + (i - aw1);
-          x1.addr[m] = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + kAckWord;
-          x1.dest[m] = static_cast<int>(ppq - dyn) + kWaveWords * w + kAckWord;
-        }
-      }
-      x2.addr[m] = a.tp;
-      x2.dest[m] = 0;
-      if (i < w2) {
-        x2.mask |= 1u << m;
-        if (i < pw2) {
-          const int w = i / (2 * kWaveWords), h = i % (2 * kWaveWords);
-          x2.addr[m] = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + kWaveWords + h;
-          x2.dest[m] = static_cast<int>((h < kWaveWords ? prz : prr) - dyn) +
-                       kWaveWords * w + h % kWaveWords;
-        } else if (i < cw2) {
-          x2.addr[m] = a.tcp + 2 * static_cast<size_t>(S.cb0) + (i - pw2);
-          x2.dest[m] = static_cast<int>(cpw - dyn) + (i - pw2);
-        } else if (i < hw2) {
-          const int hh = (i - cw2) / 3, c = (i - cw2) % 3;
-          x2.addr[m] = a.tz + 3 * static_cast<size_t>(a.halo_slot[h0 + hh]) + c;
-          x2.dest[m] = static_cast<int>(zl - dyn) + 4 * (kPcgThreads + hh) + c;
-        } else {
-          const int n = a.wg_c[c_0 + (i - hw2) / 3], c = (i - hw2) % 3;
-          x2.addr[m] = a.tc + 3 * static_cast<size_t>(S.cn0 + n) + c;
-          x2.dest[m] = static_cast<int>(czl - dyn) + 4 * n + c;
-        }
-      }
+  for (int m = 0; m < kPcgXchWords; ++m) {
+    const int i = t + kPcgThreads * m;
+    x1.addr[m] = a.tp;
+    x1.dest[m] = 0;
+    if (i < w1) {
+      x1.mask |= 1u << m;
+      x1_word(i, x1.addr[m], x1.dest[m]);
+    }
+    x2.addr[m] = a.tp;
+    x2.dest[m] = 0;
+    if (i < w2) {
+      x2.mask |= 1u << m;
+      x2_word(i, x2.addr[m], x2.dest[m]);
     }
   }
+  // Both exchanges, the words held in registers and the rest.
+  auto collect1 = [&](unsigned e) {
+    return collect(x1, e, dyn, a.abort_flag) &&
+           collect_rest(w1, x1_word, e, dyn, a.abort_flag);
+  };
+  auto collect2 = [&](unsigned e) {
+    return collect(x2, e, dyn, a.abort_flag) &&
+           collect_rest(w2, x2_word, e, dyn, a.abort_flag);
+  };
   unsigned long long* my_tp =
       a.tp + kPcgPartWords * static_cast<size_t>(bx) + 2 * wave;
   __syncthreads();
@@ -615,7 +683,7 @@ __global__ void __launch_bounds__(kPcgThreads, kWavesPerSimd)
         put_double(my_tp + 2 * kWaveWords, ep, s1);
       }
     }
-    if (!__syncthreads_and(collect(x2, ep, dyn, a.abort_flag))) return false;
+    if (!__syncthreads_and(collect2(ep))) return false;
     z_out = coarse && in_bank ? add_coarse(z0, scz) : z0;
     // r.z = r.z0 over the slots plus r.z_coarse over the coarse banks.
     part_total2(prz, prr, kPcgWaves * S.nwg, cpw, S.ncb, rz_out, rr_out);
@@ -627,7 +695,7 @@ __global__ void __launch_bounds__(kPcgThreads, kWavesPerSimd)
     ++ep;
     publish_node_sums(r, ep);
     if (lane0) put_double(my_tp, ep, 0.0);
-    if (!__syncthreads_and(collect(x1, ep, dyn, a.abort_flag))) return;
+    if (!__syncthreads_and(collect1(ep))) return;
     if (has_bank) cs = sum_nodes(qs);
   }
   double rz, rr;
@@ -706,7 +774,7 @@ __global__ void __launch_bounds__(kPcgThreads, kWavesPerSimd)
       const double s = wave_sum(own ? ddot3(p, q) : 0.0);
       if (lane0) put_double(my_tp, ep, s);
     }
-    if (!__syncthreads_and(collect(x1, ep, dyn, a.abort_flag))) return;
+    if (!__syncthreads_and(collect1(ep))) return;
     // The members' sums of q (independent of alpha); their residual sums
     // follow by cs -= alpha * qsum.
     d3 cq = {0.0, 0.0, 0.0};

@@ -35,7 +35,7 @@ def oracle_for(global_kw: dict) -> dict:
         lin_rel_max=f32(g.linear_solver_rel_tol_max),
         lin_rel_init=f32(g.initial_linear_rel_tol), admm_abs=f32(g.admm_abs_tol),
         admm_rel=f32(g.admm_rel_tol), ms_scale=f32(0.2), ccd_tol=f32(1e-6),
-        pcg_criterion=g.pcg_criterion)
+        pcg_criterion=g.pcg_criterion, ccd_zero=g.ccd_zero_toi)
 
 
 def coll_tuple(c: "sp.CollisionConfig") -> tuple:
@@ -76,14 +76,17 @@ def contact_keys(contacts: np.ndarray) -> set:
 
 
 def same_decisions(gpu: tuple, ora: tuple, toi_rtol: float = 1e-5) -> bool:
-    """GPU (TOIs, contacts) against the oracle's (TOIs, contacts, ...): the
-    same number of outer iterations, TOIs equal to FP32 rounding and the
-    same contacts found at the same DCD calls."""
-    gt, gc = gpu[0], gpu[1]
-    ot, oc = ora[0], ora[1]
+    """GPU (TOIs, contacts, ADMM iterations) against the oracle's (TOIs,
+    contacts, missing, ADMM iterations): the same number of outer iterations,
+    TOIs equal to FP32 rounding, the same ADMM iteration counts and the same
+    contacts found at the same DCD calls."""
+    gt, gc, ga = gpu[0], gpu[1], gpu[2]
+    ot, oc, oa = ora[0], ora[1], ora[3]
     if len(gt) != len(ot):
         return False
     if len(gt) and np.abs(np.asarray(gt, np.float64) - ot).max() > toi_rtol:
+        return False
+    if not np.array_equal(ga, oa):
         return False
     return contact_keys(gc) == contact_keys(oc)
 
@@ -103,7 +106,7 @@ def lockstep_step(w, h, ow, nvx: int, x_prev, v_prev):
     dec_o = ow.decisions()
     dec_g = w.decisions()
     ow.set_state(0, x_prev, v_prev)
-    ow.set_replay(dec_g[0], dec_g[1])
+    ow.set_replay(dec_g[0], dec_g[1], dec_g[2])
     rc, _ = ow.step()
     assert rc == 0
     e_rep = float(np.abs(xg - ow.positions(0)).max())

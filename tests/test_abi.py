@@ -145,6 +145,7 @@ def test_set_global_rejects_unknown_solver_modes(ctx):
     assert lib.silk_hip_set_global(h, C.byref(ok)) == 0
     for field, value in [("mas_precision", 3), ("mas_precision", -1),
                          ("pcg_criterion", 3), ("warm_start", 5), ("warm_projection", 2),
+                         ("ccd_zero_toi", 2), ("ccd_zero_toi", -1),
                          ("warm_start", sp.WARM_LEARNED)]:
         g = sp.GlobalConfig()._c()
         setattr(g, field, value)
@@ -153,7 +154,8 @@ def test_set_global_rejects_unknown_solver_modes(ctx):
     for field, value in [("mas_precision", sp.MAS_FP16), ("mas_precision", sp.MAS_BF16),
                          ("pcg_criterion", sp.PCG_RHS), ("pcg_criterion", sp.PCG_TRUE),
                          ("warm_start", sp.WARM_EXTRAPOLATE),
-                         ("warm_projection", sp.PROJECT_GALERKIN)]:
+                         ("warm_projection", sp.PROJECT_GALERKIN),
+                         ("ccd_zero_toi", sp.CCD_ZERO_CONTACT)]:
         g = sp.GlobalConfig()._c()
         setattr(g, field, value)
         assert lib.silk_hip_set_global(h, C.byref(g)) == 0, field

@@ -44,19 +44,61 @@ def _queries(n, seed):
     return vf, p0, p1, err
 
 
+def _near_queries(n, ms, seed):
+    """Queries that start within ms of contact: a point at height h in
+    (0.05, 0.95) ms over a triangle's interior, or an edge that far above
+    another it crosses, then moving through, away or along (so that the
+    refinements at ms / 2 and at zero separation decide them)."""
+    rng = np.random.default_rng(seed)
+    vf = (np.arange(n) % 2 == 0).astype(np.int32)
+    p0 = np.zeros((n, 12))
+    p1 = np.zeros((n, 12))
+    for i in range(n):
+        h = rng.uniform(0.05, 0.95) * ms
+        if vf[i]:
+            tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float64)
+            b = rng.dirichlet([2, 2, 2])
+            x = np.vstack([b @ tri + [0, 0, h], tri])
+        else:
+            x = np.array([[-1, 0, h], [1, 0, h], [0, -1, 0], [0, 1, 0]], np.float64)
+        x[:, :2] += rng.uniform(-1, 1, 2)
+        dz = rng.choice([-1.0, 1.0, 0.0]) * rng.choice([0.2 * ms, 2 * ms, 0.5])
+        v = np.zeros((4, 3))
+        v[:2 if not vf[i] else 1, 2] = dz
+        v[:, :2] += rng.uniform(-0.3, 0.3, 2) * rng.choice([0.0, 1.0])
+        p0[i] = _f32(x.reshape(-1))
+        p1[i] = _f32((x + v).reshape(-1))
+    err = np.array([err_for(p0[i].reshape(4, 3), p1[i].reshape(4, 3), bool(vf[i]))
+                    for i in range(n)])
+    return vf, p0, p1, err
+
+
+@pytest.mark.parametrize("zero_mode", [sp.CCD_ZERO_SKIP, sp.CCD_ZERO_CONTACT])
 @pytest.mark.parametrize("ms", [MS, 1e-5])
-def test_ccd_kernel_bit_exact_vs_oracle(ms):
-    w = gpu_world()
-    vf, p0, p1, err = _queries(4000, seed=int(ms * 1e6) + 1)
+def test_ccd_kernel_bit_exact_vs_oracle(ms, zero_mode):
+    """The FP64 CCD kernel and the oracle, bit for bit, on random queries
+    (half of them touching at some t in (0, 1)) and on queries that start
+    within ms of contact, under both treatments of a root at t = 0."""
+    w = gpu_world(ccd_zero_toi=zero_mode)
+    a = _queries(3000, seed=int(ms * 1e6) + 1)
+    b = _near_queries(1000, ms, seed=int(ms * 1e6) + 2)
+    vf, p0, p1, err = (np.concatenate([x, y]) for x, y in zip(a, b))
     hit, toi = w.debug_ccd(vf, p0, p1, err, ms, TOL)
     ohit = np.zeros_like(hit)
     otoi = np.zeros_like(toi)
+    zeros = 0
     for i in range(vf.size):
-        h, t = O.ccd(bool(vf[i]), p0[i].reshape(4, 3), p1[i].reshape(4, 3), ms, TOL,
-                     err[i], stack_cap=64)
+        h, t, z = O.ccd(bool(vf[i]), p0[i].reshape(4, 3), p1[i].reshape(4, 3), ms, TOL,
+                        err[i], stack_cap=64, zero_mode=zero_mode, with_zero=True)
         ohit[i], otoi[i] = h, (t if h else 0.0)
-    print(f"CCD ms={ms}: {int(hit.sum())} hits of {vf.size}, oracle {int(ohit.sum())}")
+        zeros += z
+    near_hits = int(hit[3000:].sum())
+    print(f"CCD ms={ms} zero mode {zero_mode}: {int(hit.sum())} hits of {vf.size} "
+          f"({near_hits} of the near queries), oracle {int(ohit.sum())}; {zeros} "
+          f"roots left at t = 0")
     assert ohit.sum() > vf.size // 4
+    if zero_mode == sp.CCD_ZERO_SKIP:
+        assert zeros > 100
     np.testing.assert_array_equal(hit, ohit)
     np.testing.assert_array_equal(toi, otoi)  # bit for bit
 
@@ -154,7 +196,8 @@ def test_broadphase_sets_equal_oracle(trial):
     assert sorted(map(tuple, gee.tolist())) == sorted(map(tuple, oee.tolist()))
 
 
-def test_lockstep_steps_match_oracle():
+@pytest.mark.parametrize("zero_mode", [sp.CCD_ZERO_SKIP, sp.CCD_ZERO_CONTACT])
+def test_lockstep_steps_match_oracle(zero_mode):
     """Each step starts the oracle from the GPU's own state, so the error
     measured is that of one step of the collision path, not accumulated
     divergence of two chaotic trajectories.  The oracle steps twice: with
@@ -168,7 +211,8 @@ def test_lockstep_steps_match_oracle():
     (broadphase, CCD) or to FP32 bands (DCD) by the other tests here."""
     kw = dict(acceleration_z=-9.81, dt=1 / 60, max_inner_iteration=50,
               linear_solver_rel_tol_min=1e-6, linear_solver_rel_tol_max=1e-6,
-              initial_linear_rel_tol=1e-6, admm_rel_tol=1e-5, admm_abs_tol=1e-7)
+              initial_linear_rel_tol=1e-6, admm_rel_tol=1e-5, admm_abs_tol=1e-7,
+              ccd_zero_toi=zero_mode)
     V, F, SV, SF, w, h, ho, ow, perm = _pair_worlds(16, 2, 0.4, kw)
     dt = float(np.float32(1 / 60))
     x_prev = V.astype(np.float64).reshape(-1)
@@ -211,8 +255,8 @@ def test_replayed_200_steps_stay_within_1e4():
     for i in range(200):
         assert w.solver_step()
         contacts += w.last_stats.contacts
-        tois, cts = w.decisions()
-        ow.set_replay(tois, cts)
+        tois, cts, admm = w.decisions()
+        ow.set_replay(tois, cts, admm)
         rc, _ = ow.step()
         assert rc == 0
         assert ow.decisions()[2] == 0
@@ -303,10 +347,14 @@ def test_dcd_records_equal_oracle_on_identical_inputs(trial):
     base = np.zeros((nvt, 3))
     base[perm] = V
     base[V.shape[0]:] = SV
-    # A bumpy sheet just above the sphere's top, so that many PT/EE pairs
-    # (sheet-sphere and sheet-sheet) fall inside the activation distance.
+    # The sheet laid over the sphere's cap, each vertex 0.2-3.5 mm above the
+    # sphere (r = 0.3), so that many PT and EE pairs fall on either side of
+    # the activation distance.
     pos = base.copy()
-    pos[:V.shape[0], 2] += rng.uniform(-1.5e-3, 1.5e-3, V.shape[0])
+    nv = V.shape[0]
+    r = np.hypot(pos[:nv, 0] - 0.5, pos[:nv, 1] - 0.5)
+    pos[:nv, 2] = np.where(r < 0.27, np.sqrt(np.maximum(0.09 - r * r, 0.0)), 0.3)
+    pos[:nv, 2] += rng.uniform(2e-4, 3.5e-3, nv)
     pos = _f32(pos)
     dhat = float(np.float32(0.05 / 16 * 1.02))
     grec, gdist = w.debug_dcd(pos, pos, 0.0, dhat)
@@ -323,17 +371,21 @@ def test_dcd_records_equal_oracle_on_identical_inputs(trial):
     g = {key(r): float(d) for r, d in zip(grec, gdist)}
     o = {key(r): float(d) for r, d in zip(orec, odist)}
     both = set(g) & set(o)
-    print(f"DCD trial {trial}: GPU {len(g)} records, oracle {len(o)}, common {len(both)}")
-    assert len(both) > 50
+    n_pt = sum(1 for k in both if k[0] == 0)
+    print(f"DCD trial {trial}: GPU {len(g)} records, oracle {len(o)}, common {len(both)} "
+          f"({n_pt} PT)")
+    assert len(both) > 200 and n_pt > 20
     dd = max(abs(g[k] - o[k]) for k in both)
     assert dd <= 2e-6, dd
     band = 1e-4 * dhat
     for k in set(g) ^ set(o):
         d = g.get(k, o.get(k))
         # Only near-threshold pairs may differ: the distance within the band
-        # of dhat, or a PT pair whose projection sits on the triangle edge
-        # (checked on the oracle's FP64 geometry).
-        near_dhat = abs(d - dhat) <= band
+        # of dhat or within FP32 rounding of zero (a pair at distance 0 has
+        # no normal and is dropped, dcd.cu:39,120), or a PT pair whose
+        # projection sits on the triangle edge (checked on the oracle's FP64
+        # geometry).
+        near_dhat = abs(d - dhat) <= band or d <= 1e-7
         near_edge = False
         if k[0] == 0:
             x = pos.reshape(-1, 3).astype(np.float64)

@@ -135,3 +135,41 @@ def test_f32_and_f64_agree_on_hits():
         h32, t32 = O.ccd_f32(True, p0, p1, MS, TOL, e)
         assert h64 and h32
         assert abs(t64 - t32) < 1e-4
+
+
+def _vertical_vf(z0, z1):
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], dtype=np.float64)
+    return np.vstack([[0.2, 0.2, z0], tri]), np.vstack([[0.2, 0.2, z1], tri])
+
+
+@pytest.mark.parametrize("zero_mode", [0, 1])
+def test_zero_toi_refinement_closed_forms(zero_mode):
+    """A point over a triangle's interior at height z0 moving straight down or
+    up (its distance to the face is |z(t)|, z(t) = z0 + t (z1 - z0)).
+    Starting within ms but beyond ms / 2, the ms / 2 search finds a time no
+    later than |z| = ms / 2 in both modes.  Starting within ms / 2, the
+    reference mode reports a root at t = 0 (no collision); the zero-separation
+    search finds a time no later than the crossing z = 0, or no contact at
+    all (toi = 1) when the point moves away.  TOIs are conservative (never
+    after the closed form) and within the search's resolution of it."""
+    def q(z0, z1):
+        p0, p1 = _vertical_vf(z0, z1)
+        return O.ccd(True, p0, p1, MS, TOL, err_for(p0, p1, True), stack_cap=64,
+                     zero_mode=zero_mode, with_zero=True)
+
+    hit, toi, zero = q(0.7 * MS, -0.7 * MS)
+    t_half = (0.7 - 0.5) / 1.4
+    assert hit and not zero and 0.7 * t_half <= toi <= t_half
+
+    for z1, t_cross in ((-0.1 * MS, 0.5), (-1.0, 0.1 * MS / (1.0 + 0.1 * MS))):
+        hit, toi, zero = q(0.1 * MS, z1)
+        if zero_mode == 0:
+            assert not hit and zero
+        else:
+            assert hit and not zero and 0.9 * t_cross <= toi <= t_cross
+
+    hit, toi, zero = q(0.1 * MS, 1.0)
+    if zero_mode == 0:
+        assert not hit and zero
+    else:
+        assert hit and not zero and toi == 1.0

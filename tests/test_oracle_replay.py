@@ -35,17 +35,18 @@ def test_replaying_own_decisions_reproduces_the_step():
         rc, st = ow.step()
         assert rc == 0
         x1 = ow.positions(0)
-        tois, contacts, _ = ow.decisions()
+        tois, contacts, _, admm = ow.decisions()
         seen += len(contacts)
         ow.set_state(0, x, v)
-        ow.set_replay(tois, contacts)
+        ow.set_replay(tois, contacts, admm)
         rc, st2 = ow.step()
         assert rc == 0
         x2 = ow.positions(0)
-        t2, c2, miss = ow.decisions()
+        t2, c2, miss, a2 = ow.decisions()
         assert miss == 0
         np.testing.assert_array_equal(t2, tois)
         np.testing.assert_array_equal(c2, contacts)
+        np.testing.assert_array_equal(a2, admm)
         np.testing.assert_array_equal(x2, x1)
         v = (x1 - x) / np.float32(1 / 60)
         x = x1
@@ -60,10 +61,83 @@ def test_replay_without_the_contacts_differs():
     ow.set_state(0, x, v)
     rc, _ = ow.step()
     x1 = ow.positions(0)
-    tois, contacts, _ = ow.decisions()
+    tois, contacts, _, admm = ow.decisions()
     assert len(contacts) > 1
     ow.set_state(0, x, v)
     ow.set_replay(tois, contacts[:0])
     rc, _ = ow.step()
     assert rc == 0
+    assert np.abs(ow.positions(0) - x1).max() > 0.0
+
+
+def _fp32_log(tois):
+    """The TOI log an FP32 solver writes for the same decisions: each TOI
+    rounded to FP32, the final (accepted) one equal to that solver's FP32
+    remaining interval (remaining -= 0.8 toi after each rejected TOI)."""
+    t = [np.float32(v) for v in tois]
+    rem = np.float32(1.0)
+    for k in range(len(t) - 1):
+        rem = np.float32(rem - np.float32(np.float32(0.8) * t[k]))
+    t[-1] = rem
+    return np.array(t, np.float64), rem
+
+
+def test_replaying_fp32_tois_keeps_the_outer_iterations():
+    """Replayed FP32 TOIs: the oracle takes the accept/reject decisions as the
+    FP32 solver made them, so it runs exactly as many outer iterations and
+    lands within FP32 rounding of its own step, including steps where the
+    FP32 remaining interval is below the FP64 one (the final TOI would fail an
+    FP64 acceptance test)."""
+    ow, V = _world()
+    x = V.astype(np.float64).reshape(-1)
+    v = np.zeros_like(x)
+    v[2::3] = -0.8
+    below, multi = 0, 0
+    for _ in range(12):
+        ow.set_state(0, x, v)
+        rc, st = ow.step()
+        assert rc == 0
+        x1 = ow.positions(0)
+        tois, contacts, _, admm = ow.decisions()
+        if len(tois) >= 2:
+            multi += 1
+            log, rem_f = _fp32_log(tois)
+            rem_d = 1.0
+            for t in tois[:-1]:
+                rem_d -= 0.8 * t
+            below += float(rem_f) < rem_d
+            ow.set_state(0, x, v)
+            ow.set_replay(log, contacts, admm)
+            rc, st2 = ow.step()
+            assert rc == 0
+            assert st2["outer"] == st["outer"]
+            assert len(ow.decisions()[0]) == len(tois)
+            assert np.abs(ow.positions(0) - x1).max() < 1e-6
+        v = (x1 - x) / np.float32(1 / 60)
+        x = x1
+    assert multi > 0 and below > 0, (multi, below)
+
+
+def test_replayed_admm_counts_are_run_exactly():
+    """The ADMM iteration count of each outer iteration is a decision of the
+    convergence test: a replayed count is run as given (one more iteration
+    than the test chose changes the step, the test's own count reproduces
+    it)."""
+    ow, V = _world()
+    x = V.astype(np.float64).reshape(-1)
+    v = np.zeros_like(x)
+    v[2::3] = -0.8
+    ow.set_state(0, x, v)
+    rc, st = ow.step()
+    x1 = ow.positions(0)
+    tois, contacts, _, admm = ow.decisions()
+    assert len(admm) == len(tois) and int(admm.sum()) == int(st["admm"])
+    assert (admm < 20).any()  # the test stopped some outer iteration early
+    more = admm.copy()
+    more[np.argmin(admm)] += 1
+    ow.set_state(0, x, v)
+    ow.set_replay(tois, contacts, more)
+    rc, st2 = ow.step()
+    assert rc == 0
+    np.testing.assert_array_equal(ow.decisions()[3], more)
     assert np.abs(ow.positions(0) - x1).max() > 0.0
