@@ -345,6 +345,23 @@ int silk_hip_debug_ccd_min(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
                            const double* err, double ms, double tolerance,
                            int32_t max_iter, double* min_toi);
 
+/* Capacity of silk_hip_ccd_min_noalloc (queries per call). */
+#define SILK_HIP_CCD_NOALLOC_MAX 16384
+
+/* No counterpart in the reference: silk_hip_debug_ccd_min for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight query
+ * loops, harnesses that forbid hipMalloc).  Same arguments and result; the
+ * queries are staged and the running minimum kept in a fixed device array
+ * (about 3.5 MB per device, held until the process ends) instead of buffers
+ * allocated per call.  n above SILK_HIP_CCD_NOALLOC_MAX is rejected with
+ * SILK_HIP_ERR_INVALID_CONFIG.  Every call on a device shares that device's
+ * array: calls that overlap on one device (from several threads or
+ * contexts) race. */
+int silk_hip_ccd_min_noalloc(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
+                             const double* p0, const double* p1, const double* err,
+                             double ms, double tolerance, int32_t max_iter,
+                             double* min_toi);
+
 /* Device copy of `bytes` (a multiple of 4) from one buffer to another, reps
  * times, with the library's copy kernel (k_copy_unless); *ms receives the
  * elapsed device time.  A reference of known traffic for hardware counters:

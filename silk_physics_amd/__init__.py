@@ -84,6 +84,8 @@ def load_library():
                                              C.c_double, i32, dp]),
         "silk_hip_debug_pcg": (C.c_int, [vp, fp, i64, fp, i64, C.c_float, fp, ip]),
         "silk_hip_debug_copy": (C.c_int, [vp, i64, i32, dp]),
+        "silk_hip_ccd_min_noalloc": (C.c_int, [vp, i64, ip, dp, dp, dp, C.c_double,
+                                               C.c_double, i32, dp]),
         "silk_hip_set_warm_start_net": (C.c_int, [vp, fp, i64]),
         "silk_hip_get_warm_start_net": (C.c_int, [vp, fp, i64]),
         "silk_hip_warm_start_record": (C.c_int, [vp, i64, C.c_float]),

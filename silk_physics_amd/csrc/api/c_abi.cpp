@@ -548,6 +548,16 @@ int silk_hip_debug_ccd_min(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
   });
 }
 
+int silk_hip_ccd_min_noalloc(silk_hip_ctx* ctx, int64_t n, const int32_t* is_vf,
+                             const double* p0, const double* p1, const double* err,
+                             double ms, double tolerance, int32_t max_iter,
+                             double* min_toi) {
+  return guarded(ctx, [&](Context& c) {
+    return silk::amd::ccd_min_noalloc(c, n, is_vf, p0, p1, err, ms, tolerance, max_iter,
+                                      min_toi);
+  });
+}
+
 int silk_hip_debug_coulomb(silk_hip_ctx* ctx, int64_t n, const float* u,
                            const float* normal, const float* mu, float* out) {
   return guarded(ctx, [&](Context& c) {

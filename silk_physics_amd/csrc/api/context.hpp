@@ -248,4 +248,9 @@ class Context {
   PcgDev pcg_dev();
 };
 
+/// silk_hip_ccd_min_noalloc (hip/ccd_noalloc.hip).
+int ccd_min_noalloc(Context& c, int64_t n, const int32_t* is_vf, const double* p0,
+                    const double* p1, const double* err, double ms, double tolerance,
+                    int32_t max_iter, double* min_toi);
+
 }  // namespace silk::amd
