@@ -166,16 +166,84 @@ def traffic_summary() -> dict | None:
 
 
 def pmc_probe(args) -> int:
-    """Short run for the hardware-counter passes: a few C2 steps (PCG solves
-    from rest, then the first contacts), then the library's copy kernel on
-    256 MiB three times as the reference of known traffic."""
+    """Short run for the hardware-counter passes: C2 steps (PCG solves from
+    rest, then the first contacts), reporting the PCG iterations and launches
+    they took, then the library's copy kernel on 256 MiB three times as the
+    reference of known traffic."""
     s, w, h = build_world(args.n)
+    iters = 0
     for _ in range(args.steps):
         assert w.solver_step()
+        iters += w.last_stats.pcg_iterations
     info = w.scene_info()
     ms = w.debug_copy(256 << 20, 3)
-    print(json.dumps({"probe": "pmc", "steps": args.steps, "copy_bytes": 256 << 20,
-                      "copy_ms": round(ms, 3), **info}), flush=True)
+    print(json.dumps({"probe": "pmc", "steps": args.steps, "pcg_iterations": iters,
+                      "copy_bytes": 256 << 20, "copy_reps": 3, "copy_ms": round(ms, 3),
+                      **info}), flush=True)
+    return 0
+
+
+def pmc_summary(fetch_dir: str, write_dir: str, probe_log: str) -> int:
+    """profiles/pcg_traffic.json from two rocprofv3 --pmc runs of the probe
+    (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch): the persistent PCG's
+    bytes per iteration, each counter scaled by what it reads for the copy
+    kernel's known 256 MiB per launch."""
+    import csv
+    import glob
+
+    def per_kernel(d: str, counter: str) -> dict:
+        out: dict = {}
+        for fn in glob.glob(os.path.join(d, "**", "*counter_collection*.csv"), recursive=True):
+            with open(fn) as f:
+                for row in csv.DictReader(f):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    name = row.get("Kernel_Name", "")
+                    key = ("pcg" if "k_pcg_persistent" in name else
+                           "copy" if "k_copy_unless" in name else None)
+                    if key is None:
+                        continue
+                    disp = (fn, row.get("Dispatch_Id"))
+                    out.setdefault(key, {}).setdefault(disp, 0.0)
+                    out[key][disp] += float(row["Counter_Value"])
+        return {k: list(v.values()) for k, v in out.items()}
+
+    probe = None
+    with open(probe_log) as f:
+        for line in f:
+            if line.startswith("{") and '"probe"' in line:
+                probe = json.loads(line)
+    fetch = per_kernel(fetch_dir, "FETCH_SIZE")
+    write = per_kernel(write_dir, "WRITE_SIZE")
+    if probe is None or not fetch.get("pcg") or not fetch.get("copy") or \
+            not write.get("pcg") or not write.get("copy"):
+        print("pmc summary: missing probe line or counter rows", file=sys.stderr)
+        return 1
+    copy_kib = probe["copy_bytes"] / 1024.0
+    f_scale = copy_kib / float(np.mean(fetch["copy"]))
+    w_scale = copy_kib / float(np.mean(write["copy"]))
+    pcg_fetch = float(np.sum(fetch["pcg"])) * f_scale * 1024.0
+    pcg_write = float(np.sum(write["pcg"])) * w_scale * 1024.0
+    iters = probe["pcg_iterations"]
+    summary = {
+        "kernel": "k_pcg_persistent", "vertices": probe["vertices"],
+        "probe_steps": probe["steps"], "pcg_iterations": iters,
+        "pcg_launches": len(fetch["pcg"]),
+        "fetch_kib_per_launch_raw": round(float(np.mean(fetch["pcg"])), 1),
+        "write_kib_per_launch_raw": round(float(np.mean(write["pcg"])), 1),
+        "copy_fetch_kib_raw": round(float(np.mean(fetch["copy"])), 1),
+        "copy_write_kib_raw": round(float(np.mean(write["copy"])), 1),
+        "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
+        "bytes_per_iteration": round((pcg_fetch + pcg_write) / max(1, iters), 1),
+        "fetch_bytes_per_iteration": round(pcg_fetch / max(1, iters), 1),
+        "write_bytes_per_iteration": round(pcg_write / max(1, iters), 1),
+        "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --pmc-probe, "
+                  "scaled by the copy kernel's counters for 256 MiB",
+    }
+    os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
+    with open(TRAFFIC_FILE, "w") as f:
+        json.dump(summary, f, indent=1)
+    print(json.dumps(summary))
     return 0
 
 
@@ -199,6 +267,8 @@ def main() -> int:
                          "16..128 quads (the batched-instance scene)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short run for rocprofv3 --pmc passes (no JSON metric)")
+    ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "PROBE_LOG"),
+                    help="write profiles/pcg_traffic.json from the two counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the step "
@@ -207,6 +277,8 @@ def main() -> int:
         return 2
     if args.pmc_probe:
         return pmc_probe(args)
+    if args.pmc_summary:
+        return pmc_summary(*args.pmc_summary)
     mas = {"fp32": sp.MAS_FP32, "fp16": sp.MAS_FP16, "bf16": sp.MAS_BF16}[args.mas]
 
     s, w, h = build_world(args.n, mas, args.batched)
@@ -269,7 +341,9 @@ def main() -> int:
         roof["us_per_iteration"] = round(1e3 * ms / max(1, tot["pcg"]), 3)
         tr = traffic_summary()
         if tr is not None and tr.get("vertices") == info["vertices"]:
-            roof["traffic"] = tr.get("bytes_per_launch")
+            # Counter bytes per PCG iteration (probe run) times this run's
+            # iterations per launch.
+            roof["traffic"] = round(tr["bytes_per_iteration"] * tot["pcg"] / max(1, n), 1)
             roof["traffic_source"] = tr.get("source")
 
     # Verification: the FP64 residual of the stored FP32 x beside the

@@ -320,10 +320,13 @@ int silk_hip_debug_dcd(silk_hip_ctx* ctx, const float* pos0, const float* pos1,
  * stopped it or max_inner_iteration did), and every contact the step
  * appended, 6 ints each: the DCD call that found it (0 at the frame start, k
  * after the k-th rollback), the type and the four global vertex ids in the
- * record's order.  NULL buffers query the counts. */
+ * record's order; contact_info[i] = colour << 4 | mask of that contact's
+ * vertices in the contact solve (bit k: vertex k is movable and has a
+ * nonzero coefficient).  NULL buffers query the counts. */
 int silk_hip_debug_decisions(silk_hip_ctx* ctx, float* tois, int32_t* admm_iters,
                              int32_t cap_toi, int32_t* n_toi, int32_t* contacts,
-                             int64_t cap_contacts, int64_t* n_contacts);
+                             int32_t* contact_info, int64_t cap_contacts,
+                             int64_t* n_contacts);
 
 /* The step's FP64 CCD kernel on n explicit queries: is_vf[i] selects
  * vertex-face (point, tri0, tri1, tri2) or edge-edge (a0, a1, b0, b1);

@@ -66,9 +66,9 @@ def lib():
             "oracle_candidates": (C.c_int, [vp, d, d, i, C.c_int, i, i, C.c_int,
                                             i]),
             "oracle_dcd_contact": (C.c_int, [C.c_int, d, d, i, d, C.c_double, d]),
-            "oracle_set_replay": (None, [vp, d, i, C.c_int, i, C.c_int]),
+            "oracle_set_replay": (None, [vp, d, i, C.c_int, i, i, C.c_int]),
             "oracle_set_obstacle_frame": (None, [vp, C.c_int, f, f]),
-            "oracle_decisions": (C.c_int, [vp, d, i, C.c_int, i, i, C.c_int, i]),
+            "oracle_decisions": (C.c_int, [vp, d, i, C.c_int, i, i, i, C.c_int, i]),
             "oracle_dcd": (C.c_int, [vp, d, d, C.c_double, C.c_double, i, d, C.c_int,
                                      i]),
             "oracle_sym3_min_eig": (C.c_double, [d]),
@@ -162,36 +162,41 @@ class OracleWorld:
         b, bp = _f(curr)
         lib().oracle_set_obstacle_frame(self._w, idx, ap, bp)
 
-    def set_replay(self, tois, contacts, admm=None):
+    def set_replay(self, tois, contacts, admm=None, info=None):
         """Replay another solver's decisions in the next step: tois[k] is the
         CCD result of outer iteration k (frame-time units) and admm[k] (when
         given) the number of ADMM iterations it ran; contacts (n, 6) holds
         (DCD call, type, four global vertex ids in record order) of every
-        contact to append."""
+        contact to append and info[i] (when given) contact i's colour << 4 |
+        mask of its vertices in the contact solve."""
         t, tp = _d(np.asarray(tois, dtype=np.float64).reshape(-1))
         c, cp = _i(np.asarray(contacts, dtype=np.int32).reshape(-1, 6))
-        if admm is None:
-            ap = None
-        else:
+        ap = fp = None
+        if admm is not None:
             a, ap = _i(np.asarray(admm, dtype=np.int32).reshape(-1))
             assert a.size == t.size
-        lib().oracle_set_replay(self._w, tp, ap, t.size, cp, c.shape[0])
+        if info is not None:
+            f, fp = _i(np.asarray(info, dtype=np.int32).reshape(-1))
+            assert f.size == c.shape[0]
+        lib().oracle_set_replay(self._w, tp, ap, t.size, cp, fp, c.shape[0])
 
     def decisions(self):
         """Decisions of the last step: (TOIs, contacts (n, 6), replayed
         contacts whose record could not be formed, ADMM iterations of every
-        outer iteration)."""
+        outer iteration, per contact colour << 4 | vertex mask)."""
         nt, nc = C.c_int(0), C.c_int(0)
-        lib().oracle_decisions(self._w, None, None, 0, C.byref(nt), None, 0, C.byref(nc))
+        lib().oracle_decisions(self._w, None, None, 0, C.byref(nt), None, None, 0,
+                               C.byref(nc))
         t = np.zeros(max(1, nt.value))
         a = np.zeros(max(1, nt.value), dtype=np.int32)
         c = np.zeros((max(1, nc.value), 6), dtype=np.int32)
+        f = np.zeros(max(1, nc.value), dtype=np.int32)
+        ip = C.POINTER(C.c_int)
         miss = lib().oracle_decisions(self._w, t.ctypes.data_as(C.POINTER(C.c_double)),
-                                      a.ctypes.data_as(C.POINTER(C.c_int)),
-                                      nt.value, C.byref(nt),
-                                      c.ctypes.data_as(C.POINTER(C.c_int)), nc.value,
+                                      a.ctypes.data_as(ip), nt.value, C.byref(nt),
+                                      c.ctypes.data_as(ip), f.ctypes.data_as(ip), nc.value,
                                       C.byref(nc))
-        return t[:nt.value], c[:nc.value], miss, a[:nt.value]
+        return t[:nt.value], c[:nc.value], miss, a[:nt.value], f[:nc.value]
 
     def dcd(self, pos0, pos1, time, dhat):
         """DCD records on explicit collider positions (global numbering):

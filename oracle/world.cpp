@@ -106,6 +106,7 @@ struct World {
   // contacts (DCD call, type, four global ids in record order).
   std::vector<double> log_toi;
   std::vector<int> log_admm;  // ADMM iterations of each outer iteration
+  std::vector<int> log_info;  // per contact: colour << 4 | mask of its solved vertices
   std::vector<int> log_contacts;
   // Replay of another solver's decisions for the next step (set_replay):
   // the outer loop takes these TOIs and every DCD call appends exactly these
@@ -113,6 +114,7 @@ struct World {
   bool replay = false;
   std::vector<double> rp_toi;
   std::vector<int> rp_admm;  // empty: run the convergence test
+  std::vector<int> rp_info;  // empty: colour and mask computed here
   std::vector<int> rp_contacts;
   int replay_missing = 0;  // replayed contacts whose record could not be formed
 };
@@ -551,27 +553,50 @@ Contact contact_of(const CollisionRec& r, int nv_cloth,
 
 // Greedy first-fit colouring in contact order over shared movable vertices;
 // groups hold contact ids ascending per colour (contact_coloring.cu:241-392
-// with a deterministic schedule).
+// with a deterministic schedule).  A vertex takes part in a contact's solve
+// when its coefficient is nonzero; in a replayed step the other solver's
+// colours and vertex masks are taken instead (a barycentric coordinate
+// clamped to exactly 0 on one side and not on the other changes both).
 static void recolor(World& w) {
   const int n = static_cast<int>(w.contacts.size());
+  const int nrp = w.replay ? static_cast<int>(w.rp_info.size()) : 0;
+  std::vector<int> mask(n, 0);
+  for (int c = 0; c < n; ++c) {
+    Contact& ct = w.contacts[c];
+    for (int k = 0; k < 4; ++k) {
+      if (c < nrp) {
+        const bool on = ((w.rp_info[c] >> k) & 1) != 0 && ct.state[k] >= 0;
+        if (!on) ct.gamma[k] = 0.0;
+        mask[c] |= on ? 1 << k : 0;
+      } else {
+        mask[c] |= ct.gamma[k] != 0.0 ? 1 << k : 0;
+      }
+    }
+  }
   std::map<int, std::vector<int>> incid;
   for (int c = 0; c < n; ++c)
     for (int k = 0; k < 4; ++k)
-      if (w.contacts[c].gamma[k] != 0.0) incid[w.contacts[c].state[k]].push_back(c);
+      if ((mask[c] >> k) & 1) incid[w.contacts[c].state[k]].push_back(c);
   std::vector<int> color(n, 0);
   int maxc = 0;
   for (int c = 0; c < n; ++c) {
-    std::set<int> used;
-    for (int k = 0; k < 4; ++k) {
-      if (w.contacts[c].gamma[k] == 0.0) continue;
-      for (int o : incid[w.contacts[c].state[k]])
-        if (o < c) used.insert(color[o]);
-    }
     int col = 1;
-    while (used.count(col)) ++col;
+    if (c < nrp) {
+      col = std::max(1, w.rp_info[c] >> 4);
+    } else {
+      std::set<int> used;
+      for (int k = 0; k < 4; ++k) {
+        if (((mask[c] >> k) & 1) == 0) continue;
+        for (int o : incid[w.contacts[c].state[k]])
+          if (o < c) used.insert(color[o]);
+      }
+      while (used.count(col)) ++col;
+    }
     color[c] = col;
     maxc = std::max(maxc, col);
   }
+  w.log_info.resize(n);
+  for (int c = 0; c < n; ++c) w.log_info[c] = color[c] << 4 | mask[c];
   w.color_groups.assign(maxc, {});
   for (int c = 0; c < n; ++c) w.color_groups[color[c] - 1].push_back(c);
   // active weights
@@ -955,6 +980,7 @@ static bool step(World& w) {
   w.active_dof = 0;
   w.log_toi.clear();
   w.log_admm.clear();
+  w.log_info.clear();
   w.log_contacts.clear();
   w.replay_missing = 0;
   int call = 0;
@@ -1177,12 +1203,14 @@ void oracle_set_obstacle_frame(void* wp, int idx, const float* prev, const float
 /// contacts listed with call index c (6 ints each: call, type, four global
 /// vertex ids in record order).
 void oracle_set_replay(void* wp, const double* tois, const int* admm, int n_toi,
-                       const int* contacts, int n_contacts) {
+                       const int* contacts, const int* info, int n_contacts) {
   World& w = *static_cast<World*>(wp);
   w.replay = true;
   w.rp_toi.assign(tois, tois + n_toi);
   if (admm != nullptr) w.rp_admm.assign(admm, admm + n_toi);
   else w.rp_admm.clear();
+  if (info != nullptr) w.rp_info.assign(info, info + n_contacts);
+  else w.rp_info.clear();
   w.rp_contacts.assign(contacts, contacts + 6 * static_cast<size_t>(n_contacts));
 }
 
@@ -1190,7 +1218,7 @@ void oracle_set_replay(void* wp, const double* tois, const int* admm, int n_toi,
 /// n_contacts (buffers may be null).  Returns the number of replayed
 /// contacts whose record could not be formed.
 int oracle_decisions(void* wp, double* tois, int* admm, int cap_toi, int* n_toi,
-                     int* contacts, int cap_contacts, int* n_contacts) {
+                     int* contacts, int* info, int cap_contacts, int* n_contacts) {
   World& w = *static_cast<World*>(wp);
   *n_toi = static_cast<int>(w.log_toi.size());
   *n_contacts = static_cast<int>(w.log_contacts.size() / 6);
@@ -1201,6 +1229,8 @@ int oracle_decisions(void* wp, double* tois, int* admm, int cap_toi, int* n_toi,
   if (contacts)
     for (size_t i = 0; i < 6 * static_cast<size_t>(std::min(cap_contacts, *n_contacts)); ++i)
       contacts[i] = w.log_contacts[i];
+  if (info)
+    for (int i = 0; i < std::min(cap_contacts, *n_contacts); ++i) info[i] = w.log_info[i];
   return w.replay_missing;
 }
 

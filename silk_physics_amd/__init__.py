@@ -74,7 +74,7 @@ def load_library():
                                                 C.POINTER(i64)]),
         "silk_hip_debug_dcd": (C.c_int, [vp, fp, fp, i64, C.c_float, C.c_float, ip,
                                          fp, i64, C.POINTER(i64)]),
-        "silk_hip_debug_decisions": (C.c_int, [vp, fp, ip, i32, ip, ip, i64,
+        "silk_hip_debug_decisions": (C.c_int, [vp, fp, ip, i32, ip, ip, ip, i64,
                                                C.POINTER(i64)]),
         "silk_hip_debug_ccd": (C.c_int, [vp, i64, ip, dp, dp, dp, C.c_double,
                                          C.c_double, i32, ip, dp]),
@@ -537,21 +537,24 @@ class World:
         """Discrete decisions of the last step: (TOI of every outer
         iteration, float32; appended contacts (n, 6): DCD call, type, four
         global vertex ids in record order; ADMM iterations of every outer
-        iteration)."""
+        iteration; per contact colour << 4 | mask of its vertices in the
+        contact solve)."""
         lib = load_library()
         nt, nc = C.c_int32(0), C.c_int64(0)
-        st = lib.silk_hip_debug_decisions(self._ctx, None, None, 0, C.byref(nt), None, 0,
-                                          C.byref(nc))
+        st = lib.silk_hip_debug_decisions(self._ctx, None, None, 0, C.byref(nt), None, None,
+                                          0, C.byref(nc))
         if st != 0:
             raise RuntimeError(self._wrap(st).to_string())
         tois = np.zeros(max(1, nt.value), dtype=np.float32)
         admm = np.zeros(max(1, nt.value), dtype=np.int32)
         recs = np.zeros((max(1, nc.value), 6), dtype=np.int32)
+        info = np.zeros(max(1, nc.value), dtype=np.int32)
         st = lib.silk_hip_debug_decisions(self._ctx, _fptr(tois), _iptr(admm), nt.value,
-                                          C.byref(nt), _iptr(recs), nc.value, C.byref(nc))
+                                          C.byref(nt), _iptr(recs), _iptr(info), nc.value,
+                                          C.byref(nc))
         if st != 0:
             raise RuntimeError(self._wrap(st).to_string())
-        return tois[:nt.value], recs[:nc.value], admm[:nt.value]
+        return tois[:nt.value], recs[:nc.value], admm[:nt.value], info[:nc.value]
 
     def debug_ccd(self, is_vf, p0, p1, err, ms, tolerance, max_iter=4096):
         """The step's FP64 CCD kernel on explicit queries: (hit, toi)."""

@@ -456,7 +456,8 @@ int silk_hip_debug_dcd(silk_hip_ctx* ctx, const float* pos0, const float* pos1,
 
 int silk_hip_debug_decisions(silk_hip_ctx* ctx, float* tois, int32_t* admm_iters,
                              int32_t cap_toi, int32_t* n_toi, int32_t* contacts,
-                             int64_t cap_contacts, int64_t* n_contacts) {
+                             int32_t* contact_info, int64_t cap_contacts,
+                             int64_t* n_contacts) {
   return guarded(ctx, [&](Context& c) {
     if (n_toi == nullptr || n_contacts == nullptr) return int{SILK_HIP_ERR_INVALID_CONFIG};
     const std::vector<float>& tl = c.toi_log_;
@@ -467,9 +468,11 @@ int silk_hip_debug_decisions(silk_hip_ctx* ctx, float* tois, int32_t* admm_iters
     const CollisionWorld* cw = c.scene_ ? c.scene_->coll.get() : nullptr;
     static const std::vector<int> none;
     const std::vector<int>& log = cw != nullptr ? cw->decision_log() : none;
+    const std::vector<int>& info = cw != nullptr ? cw->decision_info() : none;
     *n_contacts = static_cast<int64_t>(log.size() / 6);
-    if (contacts != nullptr)
-      std::copy_n(log.begin(), 6 * std::min<int64_t>(cap_contacts, *n_contacts), contacts);
+    const int64_t nc = std::min<int64_t>(cap_contacts, *n_contacts);
+    if (contacts != nullptr) std::copy_n(log.begin(), 6 * nc, contacts);
+    if (contact_info != nullptr) std::copy_n(info.begin(), nc, contact_info);
     return int{SILK_HIP_OK};
   });
 }

@@ -163,6 +163,7 @@ class HipCollision final : public CollisionWorld {
   void debug_dcd(const float* pos0, const float* pos1, float time, float dhat,
                  std::vector<int>& recs, std::vector<float>& dist) override;
   const std::vector<int>& decision_log() const override { return log_; }
+  const std::vector<int>& decision_info() const override { return info_; }
 
   // Static tables (filled by make_collision_world).
   std::vector<int> h_tri, h_edge, h_tri_obj, h_edge_obj, h_point_obj;
@@ -220,6 +221,7 @@ class HipCollision final : public CollisionWorld {
   // contacts of the step
   int dcd_call_ = 0;                     // DCD calls so far in this step
   std::vector<int> log_;                 // appended contacts (decision_log)
+  std::vector<int> info_;                // their colours and masks (decision_info)
   std::vector<HostContact> contacts_;
   std::unordered_set<std::array<int, 5>, KeyHash> keys_;
   std::vector<int> color_;               // per contact, from 1
@@ -722,6 +724,9 @@ void HipCollision::recolor(int n_old) {
     while (used[col]) ++col;
     color_[c] = col;
     max_color_ = std::max(max_color_, col);
+    int mask = 0;
+    for (int k = 0; k < 4; ++k) mask |= hc.gamma[k] != 0.0 ? 1 << k : 0;
+    info_.push_back(col << 4 | mask);
     for (int k = 0; k < 4; ++k) {
       if (hc.gamma[k] == 0.0) continue;
       auto& lst = vinc_[hc.state[k]];
@@ -844,6 +849,7 @@ void HipCollision::begin_step(const float* outer_x, float* ms, float* dhat,
   contacts_.clear();
   keys_.clear();
   log_.clear();
+  info_.clear();
   dcd_call_ = 0;
   color_off_h_.clear();
   color_list_h_.clear();

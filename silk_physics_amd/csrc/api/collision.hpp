@@ -60,6 +60,10 @@ class CollisionWorld {
   /// found it (0 at the frame start, k after the k-th rollback), the type and
   /// the four global vertex ids in the record's order.
   virtual const std::vector<int>& decision_log() const = 0;
+  /// Per appended contact: its colour << 4 | the mask of its vertices that
+  /// take part in the contact solve (nonzero coefficient on a movable
+  /// vertex; bit k for vertex k).
+  virtual const std::vector<int>& decision_info() const = 0;
 };
 
 /// nullptr when no object has collision enabled.

@@ -684,8 +684,8 @@ void Context::build_scene() {
   S.bank_inv_h.resize(std::max<size_t>(1, bank_static.size()));
   S.bank_scale.resize(std::max<size_t>(1, bank_sys.size()));
   S.x0_copy.resize(3 * static_cast<size_t>(nv));
-  S.x0r.resize(3 * static_cast<size_t>(nv));
-  S.dxr.resize(3 * static_cast<size_t>(nv));
+  S.r0d.resize(3 * static_cast<size_t>(nv));
+  S.dxc.resize(3 * static_cast<size_t>(nv));
   S.sys_done.resize(static_cast<size_t>(std::max(1, S.nsys)));
   if (cnode_list.empty()) cnode_list.push_back(0);
   S.cnode_ptr.upload(cnode_ptr, st);
@@ -737,6 +737,8 @@ void Context::build_scene() {
     const int ncn_all = static_cast<int>(ps_cn_ptr.size()) - 1;
     S.ps_tz.resize(3 * static_cast<size_t>(std::max(1, nslot)));
     S.ps_tz.zero(st);
+    S.ps_tdx.resize(3 * static_cast<size_t>(std::max(1, nslot)));
+    S.ps_tdx.zero(st);
     S.ps_tn.resize(3 * static_cast<size_t>(std::max(1, ncn_all)));
     S.ps_tn.zero(st);
     S.ps_tp.resize(kPcgPartWords * static_cast<size_t>(std::max(1, ps_wg)));
@@ -779,6 +781,7 @@ void Context::build_scene() {
     a.r0 = S.r.get();
     a.x = S.inner_x.get();
     a.tz = S.ps_tz.get();
+    a.tdx = S.ps_tdx.get();
     a.tn = S.ps_tn.get();
     a.tp = S.ps_tp.get();
     a.tc = S.ps_tc.get();
@@ -1093,13 +1096,22 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
   PcgDev p = pcg_dev();
   PersistDev pa = persist_args(S, g, pcg_max_iter());
   pa.ctl = ctl;
+  const bool verify = g.pcg_criterion == SILK_HIP_PCG_TRUE;
+  if (verify) {
+    // The true-residual test runs inside each solve's launch: the residual
+    // kernel keeps b - A x0 in FP64, the solve tests b - A (x0 + dx) when its
+    // recurrence converges and replaces r by it when above the tolerance.
+    ra.r0d = S.r0d.get();
+    pa.r0d = S.r0d.get();
+    pa.dxc = S.dxc.get();
+    pa.max_replace = kPcgRestarts;
+  }
   // Learned warm start, or recording of training samples (from solves that
   // start at the previous iterate); both need each solve's correction.
   const bool learned = g.warm_start == SILK_HIP_WARM_LEARNED && has_warm_net();
   const bool extrap = g.warm_start == SILK_HIP_WARM_EXTRAPOLATE;
   const bool projected = learned || extrap;
   const bool record = recording() && g.warm_start == SILK_HIP_WARM_PREVIOUS;
-  const bool verify = g.pcg_criterion == SILK_HIP_PCG_TRUE;
   if (projected || record) {
     pa.dx_out = S.dx.get();
     const size_t ns = n3;
@@ -1109,6 +1121,7 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
       S.wdelta.resize(ns);
       S.w_xprev.resize(ns);
       S.w_rprev.resize(ns);
+      S.w_r0dprev.resize(ns);
       S.w_part2.resize(2 * static_cast<size_t>(std::max(1, S.nblk)));
       S.w_gram.resize(30 * static_cast<size_t>(std::max(1, S.nblk)));
       S.w_coef.resize(6 * static_cast<size_t>(std::max(1, S.nsys)));
@@ -1134,7 +1147,7 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
   ca.lin_init = g.initial_linear_rel_tol;
   ca.lin_min = g.linear_solver_rel_tol_min;
   ca.lin_max = g.linear_solver_rel_tol_max;
-  const bool keep_x0 = g.true_residual_stats || verify;
+  const bool keep_x0 = g.true_residual_stats != 0;
   for (int it = 0; it < g.max_inner_iteration; ++it) {
     // Without a projected start the residual kernel also keeps the solve's
     // start x for the true-residual test.
@@ -1155,6 +1168,10 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
         hip_check(hipMemcpyAsync(S.w_rprev.get(), S.r.get(), n3 * sizeof(float),
                                  hipMemcpyDeviceToDevice, stream_),
                   "warm r copy");
+        if (verify)
+          hip_check(hipMemcpyAsync(S.w_r0dprev.get(), S.r0d.get(), n3 * sizeof(double),
+                                   hipMemcpyDeviceToDevice, stream_),
+                    "warm FP64 r copy");
         hip_check(hipMemcpyAsync(S.w_part2.get(), S.part2.get(),
                                  2 * static_cast<size_t>(S.nblk) * sizeof(double),
                                  hipMemcpyDeviceToDevice, stream_),
@@ -1175,7 +1192,9 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
         launch_pcg_residual(stream_, p, ra);
         timed_end();
         launch_warm_judge(stream_, p, S.w_part2.get(), S.w_xprev.get(), S.w_rprev.get(),
-                          S.inner_x.get(), S.r.get(), &ctl->warm_kept, skip);
+                          S.inner_x.get(), S.r.get(), &ctl->warm_kept, skip,
+                          verify ? S.r0d.get() : nullptr,
+                          verify ? S.w_r0dprev.get() : nullptr);
       } else if (record) {
         launch_warm_features(stream_, p, wf, S.wfeat.get(), S.wsig.get());
       }
@@ -1187,46 +1206,16 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
       timed_begin(kTimePcg);
       launch_persistent(pa, 0.0f);
       timed_end();
-      if (g.true_residual_stats && !verify)
+      // FP64 residuals of the iterate (folded here unless the solve's own
+      // test folded it) and of the stored FP32 x.
+      if (g.true_residual_stats)
         launch_solve_residuals(stream_, p, ra, S.x0_copy.get(), S.dx.get(),
-                               S.inner_x.get(), S.part3.get(), ctl, 1);
+                               S.inner_x.get(), S.part3.get(), ctl, verify ? 0 : 1);
       if (record)
         launch_warm_record(stream_, S.wfeat.get(), S.wsig.get(), S.dx.get(),
                            static_cast<int64_t>(n3), warm_.ds_rate, warm_.ds_salt++,
                            warm_.ds_feat.get(), warm_.ds_target.get(),
                            warm_.ds_count.get(), warm_.ds_cap, skip);
-      if (verify) {
-        // True-residual test of the iterate; systems above rel |b| restart
-        // from the committed x (the restart's own iterate x + dx' is tested
-        // again).  Every kernel of a round is a no-op once all pass.
-        launch_iterate_verify(stream_, p, ra, S.x0_copy.get(), S.dx.get(),
-                              S.part3.get(), S.sys_done.get(), ctl, 0.0f, 1,
-                              kPcgRestarts == 0);
-        for (int k = 0; k < kPcgRestarts; ++k) {
-          const int* vskip = &ctl->verify_done;
-          launch_copy_unless(stream_, S.x0r.get(), S.inner_x.get(),
-                             static_cast<int64_t>(n3), vskip);
-          RhsArgs rr = ra;
-          rr.skip = vskip;
-          timed_begin(kTimeResidual);
-          launch_pcg_residual(stream_, p, rr);
-          timed_end();
-          PersistDev pr = pa;
-          pr.sys_done = S.sys_done.get();
-          pr.restart = 1;
-          pr.dx_out = S.dxr.get();
-          timed_begin(kTimePcg);
-          launch_persistent(pr, 0.0f);
-          timed_end();
-          launch_iterate_verify(stream_, p, ra, S.x0r.get(), S.dxr.get(), S.part3.get(),
-                                S.sys_done.get(), ctl, 0.0f, 0, k == kPcgRestarts - 1);
-        }
-        // The stored FP32 x after the restarts (the iterate's residual was
-        // folded by the verification).
-        if (g.true_residual_stats)
-          launch_solve_residuals(stream_, p, ra, S.x0_copy.get(), S.dx.get(),
-                                 S.inner_x.get(), S.part3.get(), ctl, 0);
-      }
     }
     timed_begin(kTimeContact);
     if (S.coll) S.coll->contact_update(S.inner_x.get(), skip);

@@ -100,13 +100,15 @@ struct Scene {
       cr, cz, cd;
   DevBuf<unsigned short> bank_inv_h;  // 16-bit bank inverses (mas_precision)
   DevBuf<float> x0_copy;    // 3*nv: start of the last solve (x0 of x0 + dx)
-  DevBuf<float> x0r, dxr;   // 3*nv: start and correction of a restart
+  DevBuf<double> r0d;       // 3*nv: its FP64 residual b - A x0
+  DevBuf<double> dxc;       // 3*nv: FP64 committed correction (residual replacement)
   DevBuf<int> sys_done;     // per system: the true-residual test passed
   DevBuf<double> part3;     // 3 per row block: solve residual partials
   // ---- learned warm start (allocated when used)
   DevBuf<unsigned short> wfeat;  // 3*nv samples x 16 features
   DevBuf<float> wsig, wdelta, w_xprev, w_rprev;  // 3*nv
   DevBuf<double> w_part2;        // residual partials of the previous iterate
+  DevBuf<double> w_r0dprev;      // 3*nv: its FP64 residual (true-residual test)
   DevBuf<double> w_gram;         // 30 per row block: projection sums
   DevBuf<float> w_coef;          // 6 per system: projection coefficients
   DevBuf<double> part, part2;
@@ -126,7 +128,7 @@ struct Scene {
       ps_halo_slot, ps_wg_n1_ptr, ps_wg_n1, ps_wg_c_ptr, ps_wg_c, ps_slot_cn,
       ps_cb_bank, ps_cb_member, ps_cb_rng, ps_abort;
   DevBuf<float> ps_sell_val;
-  DevBuf<unsigned long long> ps_tz, ps_tn, ps_tp, ps_tc, ps_tcp;
+  DevBuf<unsigned long long> ps_tz, ps_tn, ps_tp, ps_tc, ps_tcp, ps_tdx;
   DevBuf<unsigned> ps_bar;
   DevBuf<PcgState> ps_out;
   DevBuf<AdmmCtl> admm_ctl;

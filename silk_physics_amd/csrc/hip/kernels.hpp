@@ -219,6 +219,7 @@ struct RhsArgs {
   const int* skip = nullptr;           // no-op when *skip != 0
   float* x_copy = nullptr;             // optional: the residual kernel copies
                                        //   x here (the solve's start x0)
+  double* r0d = nullptr;               // optional: the FP64 residual per row
 };
 
 /// PCG init, part 1 (mas_cg_solver.cu:273-277): d and r0 = b - (K + D) x0
@@ -350,6 +351,17 @@ struct PersistDev {
   float* dx_out = nullptr;          // optional: the solve's dx (rows)
   const int* sys_done = nullptr;    // optional, per system: nonzero = already
                                     //   converged, the launch leaves it alone
+  // True-residual test inside the launch (criterion 2): when r0d is set, a
+  // solve that meets the recurrence test computes the FP64 residual
+  // r0d - A dx of its iterate (dx of the halo exchanged through tdx) and,
+  // above max(abs^2, rel^2 |b|^2), replaces r by it and continues, at most
+  // max_replace times.  A replacement moves dx into the FP64 sum dxc and
+  // the FP64 residual into r0d, so the search continues from dx = 0 and the
+  // iterate x0 + dxc + dx keeps FP64 precision in its committed part.
+  double* r0d = nullptr;            // FP64 b - A x0 per row (residual kernel)
+  double* dxc = nullptr;            // FP64 committed correction per row
+  unsigned long long* tdx = nullptr;  // 3 per slot: dx, epoch-tagged
+  int max_replace = 0;
   float abs_tol = 0.0f;
   int max_iter = 1000;
   int criterion = 0;                // 0: rr <= rel^2 rr0; 1 and 2: rr <= rel^2 |b|^2

@@ -143,6 +143,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(PcgDev p, RhsArgs ra,
     for (int k = 0; k < 3; ++k) {
       const double res = rhs[k] - (ax[k] + dd * x[3 * row + k]);
       r_out[3 * row + k] = static_cast<float>(res);
+      if (ra.r0d != nullptr) ra.r0d[3 * row + k] = res;
       rr += res * res;
       bb += rhs[k] * rhs[k];
       if (ra.x_copy != nullptr) ra.x_copy[3 * row + k] = x[3 * row + k];

@@ -127,17 +127,18 @@ __device__ bool collect(const Share& sh, unsigned epoch, float* lds,
   return true;
 }
 
-// Words i >= kPcgThreads * kPcgXchWords of an exchange (a coarse bank that
-// reads the level-1 sums of a whole system, a large halo): thread t polls
-// words t + kPcgThreads * m, m >= kPcgXchWords, kPcgXchBatch at a time,
-// mapping each with word(i, &addr, &dest) when its batch starts.  Returns
-// false when the wait gave up.
+// Words of an exchange mapped when collected: thread t polls words
+// first, first + kPcgThreads, ... < total, kPcgXchBatch at a time, mapping
+// each with word(i, &addr, &dest) when its batch starts.  Used for the words
+// beyond the kPcgThreads * kPcgXchWords kept in registers (a coarse bank
+// that reads the level-1 sums of a whole system, a large halo; first =
+// t + kPcgThreads * kPcgXchWords) and for the exchanges of the true-residual
+// test (first = t).  Returns false when the wait gave up.
 template <class WordFn>
-__device__ bool collect_rest(int total, WordFn&& word, unsigned epoch, float* lds,
-                             int* abort_flag) {
+__device__ bool collect_rest(int first, int total, WordFn&& word, unsigned epoch,
+                             float* lds, int* abort_flag) {
   constexpr int kStride = kPcgThreads * kPcgXchBatch;
-  for (int i0 = static_cast<int>(threadIdx.x) + kPcgThreads * kPcgXchWords; i0 < total;
-       i0 += kStride) {
+  for (int i0 = first; i0 < total; i0 += kStride) {
     const unsigned long long* addr[kPcgXchBatch];
     int dest[kPcgXchBatch];
     unsigned pending = 0;
@@ -552,13 +553,14 @@ __global__ void __launch_bounds__(kPcgThreads, kWavesPerSimd)
     }
   }
   // Both exchanges, the words held in registers and the rest.
+  const int rest0 = t + kPcgThreads * kPcgXchWords;
   auto collect1 = [&](unsigned e) {
     return collect(x1, e, dyn, a.abort_flag) &&
-           collect_rest(w1, x1_word, e, dyn, a.abort_flag);
+           collect_rest(rest0, w1, x1_word, e, dyn, a.abort_flag);
   };
   auto collect2 = [&](unsigned e) {
     return collect(x2, e, dyn, a.abort_flag) &&
-           collect_rest(w2, x2_word, e, dyn, a.abort_flag);
+           collect_rest(rest0, w2, x2_word, e, dyn, a.abort_flag);
   };
   unsigned long long* my_tp =
       a.tp + kPcgPartWords * static_cast<size_t>(bx) + 2 * wave;
@@ -690,115 +692,217 @@ __global__ void __launch_bounds__(kPcgThreads, kWavesPerSimd)
     return true;
   };
 
-  // ---- prologue: level-1 sums of r0, z0 = M r0, r.z, r.r
-  if (coarse) {
-    ++ep;
-    publish_node_sums(r, ep);
-    if (lane0) put_double(my_tp, ep, 0.0);
-    if (!__syncthreads_and(collect1(ep))) return;
-    if (has_bank) cs = sum_nodes(qs);
-  }
-  double rz, rr;
-  f3 z;
-  if (!finish_z(z, rz, rr)) return;
-  const double rr0 = rr;
+  // ---- start, and restart after a residual replacement: level-1 sums of
+  // r, z = M r, r.z and r.r
   const double abs2 = static_cast<double>(a.abs_tol) * a.abs_tol;
   const double rel2 = static_cast<double>(rel_tol) * rel_tol;
-  // Reference of the relative test: rr0, or |b|^2 summed from the residual
-  // kernel's block partials (every wavefront adds the same values in the
-  // same order, so all take the same decisions).
-  double ref2 = rr0;
-  if (a.criterion >= 1) {
-    const int q0 = a.sys_blk0[sys], q1 = a.sys_blk0[sys + 1];
-    double v = 0.0;
-    for (int q = q0 + lane; q < q1; q += kWave) v += a.part2[2 * q + 1];
-    ref2 = wave_sum(v);
-  }
-  int status = isfinite(rz) ? -1 : 3;
-  // Right-hand-side criterion: a warm start that already meets it takes no
-  // iteration.
-  if (status < 0 && a.criterion >= 1) {
-    if (rr <= abs2) status = 0;
-    else if (rr <= rel2 * ref2) status = 1;
-  }
-  int iter = 0;
+  double rz = 0.0, rr = 0.0, rr0 = 0.0, ref2 = 0.0;
+  f3 z = {0.0f, 0.0f, 0.0f};
+  int status = -1, iter = 0;
   float beta = 0.0f;
+  // start: the search (re)starts from r; fresh: p = z (the first iteration
+  // after a start).
+  bool start = true, fresh = true;
+  int replaced = 0, unverified = 0;
+  double true_ratio = -1.0;
 
-  while (status < 0) {
-    p = (iter == 0) ? z : fma3(beta, p, z);
-    sts4(pl, t, p);
-#pragma unroll
-    for (int m = 0; m < kPcgHaloPerThread; ++m) {
-      const int hh = t + kPcgThreads * m;
-      if (hh < nh) {
-        const f3 zh = coarse ? add_coarse(lds4(zl, kPcgThreads + hh), hsc[m])
-                             : lds4(zl, kPcgThreads + hh);
-        sts4(pl, kPcgThreads + hh,
-             iter == 0 ? zh : fma3(beta, lds4(pl, kPcgThreads + hh), zh));
+  for (;;) {
+    if (start) {
+      if (coarse) {
+        ++ep;
+        publish_node_sums(r, ep);
+        if (lane0) put_double(my_tp, ep, 0.0);
+        if (!__syncthreads_and(collect1(ep))) return;
+        if (has_bank) cs = sum_nodes(qs);
       }
+      if (!finish_z(z, rz, rr)) return;
+      if (iter == 0 && replaced == 0) {
+        rr0 = rr;
+        // Reference of the relative test: rr0, or |b|^2 summed from the
+        // residual kernel's block partials (every wavefront adds the same
+        // values in the same order, so all take the same decisions).
+        ref2 = rr0;
+        if (a.criterion >= 1) {
+          const int q0 = a.sys_blk0[sys], q1 = a.sys_blk0[sys + 1];
+          double v = 0.0;
+          for (int q = q0 + lane; q < q1; q += kWave) v += a.part2[2 * q + 1];
+          ref2 = wave_sum(v);
+        }
+      }
+      status = isfinite(rz) ? -1 : 3;
+      // Right-hand-side criterion: a start that already meets it takes no
+      // iteration.
+      if (status < 0 && a.criterion >= 1) {
+        if (rr <= abs2) status = 0;
+        else if (rr <= rel2 * ref2) status = 1;
+      }
+      start = false;
+      fresh = true;
     }
-    __syncthreads();
-    // q = (K + D) p in difference form.
-    f3 q = {0.0f, 0.0f, 0.0f};
-    if (own) {
-      float ax = 0.0f, ay = 0.0f, az = 0.0f;
-      if (reg_ell) {
+    if (status < 0) {
+      p = fresh ? z : fma3(beta, p, z);
+      sts4(pl, t, p);
 #pragma unroll
-        for (int c0 = 0; c0 < kPcgNbr; c0 += kPcgNbrChunk) {
-          if (c0 >= e_w) break;
-          f3 pn[kPcgNbrChunk];
+      for (int m = 0; m < kPcgHaloPerThread; ++m) {
+        const int hh = t + kPcgThreads * m;
+        if (hh < nh) {
+          const f3 zh = coarse ? add_coarse(lds4(zl, kPcgThreads + hh), hsc[m])
+                               : lds4(zl, kPcgThreads + hh);
+          sts4(pl, kPcgThreads + hh,
+               fresh ? zh : fma3(beta, lds4(pl, kPcgThreads + hh), zh));
+        }
+      }
+      __syncthreads();
+      // q = (K + D) p in difference form.
+      f3 q = {0.0f, 0.0f, 0.0f};
+      if (own) {
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        if (reg_ell) {
 #pragma unroll
-          for (int k = 0; k < kPcgNbrChunk; ++k) pn[k] = lds4(pl, ncol[c0 + k]);
+          for (int c0 = 0; c0 < kPcgNbr; c0 += kPcgNbrChunk) {
+            if (c0 >= e_w) break;
+            f3 pn[kPcgNbrChunk];
 #pragma unroll
-          for (int k = 0; k < kPcgNbrChunk; ++k) {
-            const float v = nval[c0 + k];
-            ax += v * (pn[k].x - p.x);
-            ay += v * (pn[k].y - p.y);
-            az += v * (pn[k].z - p.z);
+            for (int k = 0; k < kPcgNbrChunk; ++k) pn[k] = lds4(pl, ncol[c0 + k]);
+#pragma unroll
+            for (int k = 0; k < kPcgNbrChunk; ++k) {
+              const float v = nval[c0 + k];
+              ax += v * (pn[k].x - p.x);
+              ay += v * (pn[k].y - p.y);
+              az += v * (pn[k].z - p.z);
+            }
+          }
+        } else {
+          for (int k = 0; k < e_w; ++k) {
+            const f3 pn = lds4(pl, a.sell_lcol[e_off + k * kSlice + e_lane]);
+            const float v = a.sell_val[e_off + k * kSlice + e_lane];
+            ax += v * (pn.x - p.x);
+            ay += v * (pn.y - p.y);
+            az += v * (pn.z - p.z);
           }
         }
-      } else {
-        for (int k = 0; k < e_w; ++k) {
-          const f3 pn = lds4(pl, a.sell_lcol[e_off + k * kSlice + e_lane]);
-          const float v = a.sell_val[e_off + k * kSlice + e_lane];
-          ax += v * (pn.x - p.x);
-          ay += v * (pn.y - p.y);
-          az += v * (pn.z - p.z);
-        }
+        q = {ax + dd * p.x, ay + dd * p.y, az + dd * p.z};
       }
-      q = {ax + dd * p.x, ay + dd * p.y, az + dd * p.z};
+      ++ep;
+      if (coarse) publish_node_sums(q, ep);
+      {
+        const double s = wave_sum(own ? ddot3(p, q) : 0.0);
+        if (lane0) put_double(my_tp, ep, s);
+      }
+      if (!__syncthreads_and(collect1(ep))) return;
+      // The members' sums of q (independent of alpha); their residual sums
+      // follow by cs -= alpha * qsum.
+      d3 cq = {0.0, 0.0, 0.0};
+      if (has_bank) cq = sum_nodes(qs);
+      const double pAp = part_total(ppq, kPcgWaves * S.nwg);
+      const float alpha = static_cast<float>(fabs(pAp) < 1e-20 ? 0.0 : rz / pAp);
+      dx = fma3(alpha, p, dx);
+      r = fma3(-alpha, q, r);
+      {
+        const double am = -static_cast<double>(alpha);
+        cs = {fma(am, cq.x, cs.x), fma(am, cq.y, cs.y), fma(am, cq.z, cs.z)};
+      }
+      double rz_new;
+      if (!finish_z(z, rz_new, rr)) return;
+      ++iter;
+      fresh = false;
+      if (iter >= a.max_iter) status = 2;
+      else if (rr <= abs2) status = 0;
+      else if (rr <= rel2 * ref2) status = 1;
+      beta = static_cast<float>(fabs(rz) < 1e-20 ? 0.0 : rz_new / rz);
+      rz = rz_new;
+      continue;
     }
+    // True-residual test of the iterate x0 + dx (x0 is not read: r0d holds
+    // b - A x0 in FP64).  Exchange 3: dx of the own slots, the halo's into pl
+    // (p is not needed again: a replacement starts a fresh direction).
+    if (a.r0d == nullptr || status > 1) break;
     ++ep;
-    if (coarse) publish_node_sums(q, ep);
-    {
-      const double s = wave_sum(own ? ddot3(p, q) : 0.0);
-      if (lane0) put_double(my_tp, ep, s);
+    if (own) put_f3(a.tdx + 3 * static_cast<size_t>(slot), ep, dx);
+    sts4(pl, t, dx);
+    auto dx_word = [&](int i, const unsigned long long*& addr, int& dest) {
+      const int hh = i / 3, c = i % 3;
+      addr = a.tdx + 3 * static_cast<size_t>(a.halo_slot[h0 + hh]) + c;
+      dest = static_cast<int>(pl - dyn) + 4 * (kPcgThreads + hh) + c;
+    };
+    if (!__syncthreads_and(collect_rest(t, 3 * nh, dx_word, ep, dyn, a.abort_flag))) return;
+    // rt = r0d - (K + D) dx of the own row, in FP64 (difference form), read
+    // from the slot-ELL in memory: this runs once or twice per solve, and the
+    // register copy of the row would raise the kernel's register peak.
+    f3 rt = {0.0f, 0.0f, 0.0f};
+    double e2 = 0.0;
+    if (own) {
+      double ax = 0.0, ay = 0.0, az = 0.0;
+      for (int k = 0; k < e_w; ++k) {
+        const f3 dn = lds4(pl, a.sell_lcol[e_off + k * kSlice + e_lane]);
+        const double v = a.sell_val[e_off + k * kSlice + e_lane];
+        ax += v * (static_cast<double>(dn.x) - static_cast<double>(dx.x));
+        ay += v * (static_cast<double>(dn.y) - static_cast<double>(dx.y));
+        az += v * (static_cast<double>(dn.z) - static_cast<double>(dx.z));
+      }
+      const double ddd = dd;
+      const size_t i3 = 3 * static_cast<size_t>(row);
+      const double ex = a.r0d[i3] - (ax + ddd * dx.x);
+      const double ey = a.r0d[i3 + 1] - (ay + ddd * dx.y);
+      const double ez = a.r0d[i3 + 2] - (az + ddd * dx.z);
+      e2 = ex * ex + ey * ey + ez * ez;
+      rt = {static_cast<float>(ex), static_cast<float>(ey), static_cast<float>(ez)};
+      // The residual of x0 + dxc + dx: kept for a replacement (r0d is not
+      // read again once the test passes).
+      a.r0d[i3] = ex;
+      a.r0d[i3 + 1] = ey;
+      a.r0d[i3 + 2] = ez;
     }
-    if (!__syncthreads_and(collect1(ep))) return;
-    // The members' sums of q (independent of alpha); their residual sums
-    // follow by cs -= alpha * qsum.
-    d3 cq = {0.0, 0.0, 0.0};
-    if (has_bank) cq = sum_nodes(qs);
-    const double pAp = part_total(ppq, kPcgWaves * S.nwg);
-    const float alpha = static_cast<float>(fabs(pAp) < 1e-20 ? 0.0 : rz / pAp);
-    dx = fma3(alpha, p, dx);
-    r = fma3(-alpha, q, r);
+    // Exchange 4: the slot wavefronts' sums of e2 (words [24, 32) of each
+    // workgroup's partials), totalled by every workgroup in the same order.
+    ++ep;
     {
-      const double am = -static_cast<double>(alpha);
-      cs = {fma(am, cq.x, cs.x), fma(am, cq.y, cs.y), fma(am, cq.z, cs.z)};
+      const double s = wave_sum(e2);
+      if (lane0 && !cwg) put_double(my_tp + 3 * kWaveWords, ep, s);
     }
-    double rz_new;
-    if (!finish_z(z, rz_new, rr)) return;
-    ++iter;
-    if (iter >= a.max_iter) status = 2;
-    else if (rr <= abs2) status = 0;
-    else if (rr <= rel2 * ref2) status = 1;
-    beta = static_cast<float>(fabs(rz) < 1e-20 ? 0.0 : rz_new / rz);
-    rz = rz_new;
+    auto true_word = [&](int i, const unsigned long long*& addr, int& dest) {
+      const int w = i / kWaveWords, h = i % kWaveWords;
+      addr = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + 3 * kWaveWords + h;
+      dest = static_cast<int>(prz - dyn) + kWaveWords * w + h;
+    };
+    if (!__syncthreads_and(
+            collect_rest(t, kWaveWords * S.nwg, true_word, ep, dyn, a.abort_flag)))
+      return;
+    const double true2 = part_total(prz, kPcgWaves * S.nwg);
+    true_ratio = ref2 > 0.0 ? true2 / ref2 : 0.0;
+    if (true2 <= fmax(abs2, rel2 * ref2)) break;
+    if (replaced >= a.max_replace) {
+      unverified = 1;
+      break;
+    }
+    ++replaced;
+    // Residual replacement: r from the FP64 residual; dx moves into the FP64
+    // committed correction and restarts at zero; the search restarts (z = M r
+    // and the coarse members' sums follow at the start).
+    if (own) {
+      r = rt;
+      double* c = a.dxc + 3 * static_cast<size_t>(row);
+      const double d3v[3] = {dx.x, dx.y, dx.z};
+      for (int k = 0; k < 3; ++k) c[k] = (replaced == 1 ? 0.0 : c[k]) + d3v[k];
+    }
+    dx = {0.0f, 0.0f, 0.0f};
+    start = true;
   }
   if (own) {
-    st3(a.x, row, ld3(a.x, row) + dx);
-    if (a.dx_out != nullptr) st3(a.dx_out, row, dx);
+    if (replaced == 0) {
+      st3(a.x, row, ld3(a.x, row) + dx);
+      if (a.dx_out != nullptr) st3(a.dx_out, row, dx);
+    } else {
+      // x0 + dxc + dx rounded once.
+      const double* c = a.dxc + 3 * static_cast<size_t>(row);
+      const f3 x0 = ld3(a.x, row);
+      const double t0 = c[0] + dx.x, t1 = c[1] + dx.y, t2 = c[2] + dx.z;
+      st3(a.x, row, f3{static_cast<float>(x0.x + t0), static_cast<float>(x0.y + t1),
+                       static_cast<float>(x0.z + t2)});
+      if (a.dx_out != nullptr)
+        st3(a.dx_out, row,
+            f3{static_cast<float>(t0), static_cast<float>(t1), static_cast<float>(t2)});
+    }
   }
   if (j == 0 && t == 0) {
     __hip_atomic_store(epoch_ctr, ep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -816,6 +920,11 @@ __global__ void __launch_bounds__(kPcgThreads, kWavesPerSimd)
       atomicAdd(&a.ctl->pcg_iters, iter);
       atomicMax(&a.ctl->pcg_max_solve, iter);
       if (status == 2) atomicAdd(&a.ctl->pcg_stalled, 1);
+      if (replaced != 0) atomicAdd(&a.ctl->pcg_restarts, replaced);
+      if (unverified != 0) atomicAdd(&a.ctl->pcg_unverified, 1);
+      if (true_ratio >= 0.0)
+        atomicMax(&a.ctl->max_true_bits,
+                  static_cast<unsigned long long>(__double_as_longlong(true_ratio)));
       if (a.restart != 0) {
         atomicAdd(&a.ctl->pcg_restarts, 1);
       } else if (sys == 0) {

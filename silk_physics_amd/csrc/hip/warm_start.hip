@@ -331,7 +331,8 @@ __global__ void __launch_bounds__(kBlock) k_warm_judge(PcgDev p, const double* p
                                                        const float* x_prev,
                                                        const float* r_prev, float* x,
                                                        float* r, int* counters,
-                                                       const int* skip) {
+                                                       const int* skip, double* r0d,
+                                                       const double* r0d_prev) {
   __shared__ double lds[17];
   if (skip != nullptr && *skip != 0) return;
   const int b = blockIdx.x;
@@ -353,6 +354,7 @@ __global__ void __launch_bounds__(kBlock) k_warm_judge(PcgDev p, const double* p
     for (int c = 0; c < 3; ++c) {
       x[3 * row + c] = x_prev[3 * row + c];
       r[3 * row + c] = r_prev[3 * row + c];
+      if (r0d != nullptr) r0d[3 * row + c] = r0d_prev[3 * row + c];
     }
   }
   if (threadIdx.x == 0 && b == b0) {
@@ -620,10 +622,12 @@ void launch_warm_project(hipStream_t s, const PcgDev& p, const float* v1, const 
 
 void launch_warm_judge(hipStream_t s, const PcgDev& p, const double* part2_prev,
                        const float* x_prev, const float* r_prev, float* x,
-                       float* r, int* counters, const int* skip) {
+                       float* r, int* counters, const int* skip, double* r0d,
+                       const double* r0d_prev) {
   if (p.nblk == 0) return;
+  if (r0d_prev == nullptr) r0d = nullptr;
   k_warm_judge<<<p.nblk, kBlock, 0, s>>>(p, part2_prev, x_prev, r_prev, x, r,
-                                         counters, skip);
+                                         counters, skip, r0d, r0d_prev);
 }
 
 void launch_warm_record(hipStream_t s, const unsigned short* feat,

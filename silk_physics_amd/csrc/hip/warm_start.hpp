@@ -108,11 +108,13 @@ void launch_warm_project(hipStream_t s, const PcgDev& p, const float* v1, const 
 
 /// Per system: keep the predicted start when the FP64 residual of x0 + delta
 /// (part2 now) is below that of x0 (part2_prev, saved before the
-/// prediction); otherwise restore x and r.  Counts the kept predictions in
+/// prediction); otherwise restore x and r (and the FP64 residual r0d from
+/// r0d_prev when both are given).  Counts the kept predictions in
 /// counters[0] and the solves in counters[1].
 void launch_warm_judge(hipStream_t s, const PcgDev& p, const double* part2_prev,
                        const float* x_prev, const float* r_prev, float* x,
-                       float* r, int* counters, const int* skip);
+                       float* r, int* counters, const int* skip,
+                       double* r0d = nullptr, const double* r0d_prev = nullptr);
 
 /// Appends samples to a data set (feat, clipped dx / sigma) with probability
 /// `rate` per sample (hash of sample and `salt`), up to `cap` samples.

@@ -76,19 +76,22 @@ def contact_keys(contacts: np.ndarray) -> set:
 
 
 def same_decisions(gpu: tuple, ora: tuple, toi_rtol: float = 1e-5) -> bool:
-    """GPU (TOIs, contacts, ADMM iterations) against the oracle's (TOIs,
-    contacts, missing, ADMM iterations): the same number of outer iterations,
-    TOIs equal to FP32 rounding, the same ADMM iteration counts and the same
-    contacts found at the same DCD calls."""
-    gt, gc, ga = gpu[0], gpu[1], gpu[2]
-    ot, oc, oa = ora[0], ora[1], ora[3]
+    """GPU (TOIs, contacts, ADMM iterations, contact info) against the
+    oracle's (TOIs, contacts, missing, ADMM iterations, contact info): the
+    same number of outer iterations, TOIs equal to FP32 rounding, the same
+    ADMM iteration counts, the same contacts found at the same DCD calls and
+    the same colours and solved-vertex masks."""
+    gt, gc, ga, gf = gpu[0], gpu[1], gpu[2], gpu[3]
+    ot, oc, oa, of = ora[0], ora[1], ora[3], ora[4]
     if len(gt) != len(ot):
         return False
     if len(gt) and np.abs(np.asarray(gt, np.float64) - ot).max() > toi_rtol:
         return False
     if not np.array_equal(ga, oa):
         return False
-    return contact_keys(gc) == contact_keys(oc)
+    if contact_keys(gc) != contact_keys(oc):
+        return False
+    return np.array_equal(gf, of)
 
 
 def lockstep_step(w, h, ow, nvx: int, x_prev, v_prev):
@@ -106,7 +109,7 @@ def lockstep_step(w, h, ow, nvx: int, x_prev, v_prev):
     dec_o = ow.decisions()
     dec_g = w.decisions()
     ow.set_state(0, x_prev, v_prev)
-    ow.set_replay(dec_g[0], dec_g[1], dec_g[2])
+    ow.set_replay(dec_g[0], dec_g[1], dec_g[2], dec_g[3])
     rc, _ = ow.step()
     assert rc == 0
     e_rep = float(np.abs(xg - ow.positions(0)).max())

@@ -5,11 +5,13 @@ every PCG solve's iterate meets ||b - A (x0 + dx)|| <= 1e-6 ||b|| (FP64),
 no solve stalls, and after every step no cloth vertex is inside the sphere
 and no cloth edge passes through a cloth triangle.
 
-The CCD runs with SILK_HIP_CCD_ZERO_CONTACT: with the reference's treatment
-of a root at t = 0 (one refinement at ms / 2, then no collision for the
-segment, ccd.cpp:150-180, narrowphase.cu:68) vertices pressed against the
-sphere end a CCD segment within ms / 2 of it and then cross it unchecked in
-this scene (0.1-2.7 mm inside from the 10th step on)."""
+The ADMM loop runs with the reference's defaults (up to 100 iterations,
+stopped by its convergence test).  Cut to 10 iterations (the 80x80 bench
+setting), the contact constraints are too loosely met on this sheet: vertices
+pressed against the sphere come within the CCD's numerical tolerance of it,
+where a root at t = 0 counts as no collision (ccd.cpp:150-180,
+narrowphase.cu:68), and then cross it, 0.1-2.7 mm deep from the 10th step on,
+with either treatment of a root at t = 0."""
 import numpy as np
 import pytest
 
@@ -23,10 +25,9 @@ pytestmark = pytest.mark.gpu
 
 def test_c5_30_steps_converge_without_penetration():
     s = scenes.c2_scene(128, z=0.32)
-    w = gpu_world(acceleration_z=-9.81, dt=1 / 60, max_inner_iteration=10,
-                  max_outer_iteration=100, linear_solver_rel_tol_min=1e-6,
-                  linear_solver_rel_tol_max=1e-6, initial_linear_rel_tol=1e-6,
-                  pcg_criterion=sp.PCG_TRUE, ccd_zero_toi=sp.CCD_ZERO_CONTACT)
+    w = gpu_world(acceleration_z=-9.81, dt=1 / 60, max_outer_iteration=100,
+                  linear_solver_rel_tol_min=1e-6, linear_solver_rel_tol_max=1e-6,
+                  initial_linear_rel_tol=1e-6, pcg_criterion=sp.PCG_TRUE)
     cc = sp.CollisionConfig()
     r, h = w.add_cloth(sp.ClothConfig(), cc, s["V"], s["F"], s["pins"])
     assert r, r.to_string()

@@ -35,18 +35,19 @@ def test_replaying_own_decisions_reproduces_the_step():
         rc, st = ow.step()
         assert rc == 0
         x1 = ow.positions(0)
-        tois, contacts, _, admm = ow.decisions()
+        tois, contacts, _, admm, info = ow.decisions()
         seen += len(contacts)
         ow.set_state(0, x, v)
-        ow.set_replay(tois, contacts, admm)
+        ow.set_replay(tois, contacts, admm, info)
         rc, st2 = ow.step()
         assert rc == 0
         x2 = ow.positions(0)
-        t2, c2, miss, a2 = ow.decisions()
+        t2, c2, miss, a2, f2 = ow.decisions()
         assert miss == 0
         np.testing.assert_array_equal(t2, tois)
         np.testing.assert_array_equal(c2, contacts)
         np.testing.assert_array_equal(a2, admm)
+        np.testing.assert_array_equal(f2, info)
         np.testing.assert_array_equal(x2, x1)
         v = (x1 - x) / np.float32(1 / 60)
         x = x1
@@ -61,7 +62,7 @@ def test_replay_without_the_contacts_differs():
     ow.set_state(0, x, v)
     rc, _ = ow.step()
     x1 = ow.positions(0)
-    tois, contacts, _, admm = ow.decisions()
+    tois, contacts, _, admm, info = ow.decisions()
     assert len(contacts) > 1
     ow.set_state(0, x, v)
     ow.set_replay(tois, contacts[:0])
@@ -98,7 +99,7 @@ def test_replaying_fp32_tois_keeps_the_outer_iterations():
         rc, st = ow.step()
         assert rc == 0
         x1 = ow.positions(0)
-        tois, contacts, _, admm = ow.decisions()
+        tois, contacts, _, admm, info = ow.decisions()
         if len(tois) >= 2:
             multi += 1
             log, rem_f = _fp32_log(tois)
@@ -130,7 +131,7 @@ def test_replayed_admm_counts_are_run_exactly():
     ow.set_state(0, x, v)
     rc, st = ow.step()
     x1 = ow.positions(0)
-    tois, contacts, _, admm = ow.decisions()
+    tois, contacts, _, admm, info = ow.decisions()
     assert len(admm) == len(tois) and int(admm.sum()) == int(st["admm"])
     assert (admm < 20).any()  # the test stopped some outer iteration early
     more = admm.copy()
@@ -141,3 +142,37 @@ def test_replayed_admm_counts_are_run_exactly():
     assert rc == 0
     np.testing.assert_array_equal(ow.decisions()[3], more)
     assert np.abs(ow.positions(0) - x1).max() > 0.0
+
+
+def test_replayed_colours_and_masks_are_used():
+    """A replayed step takes the contacts' colours and solved-vertex masks as
+    given: a reversed colouring is reported back as replayed, empty masks
+    (no vertex in any contact's solve) change the step, and the step's own
+    info reproduces it bit for bit."""
+    ow, V = _world()
+    x = V.astype(np.float64).reshape(-1)
+    v = np.zeros_like(x)
+    v[2::3] = -0.8
+    ow.set_state(0, x, v)
+    rc, _ = ow.step()
+    x1 = ow.positions(0)
+    tois, contacts, _, admm, info = ow.decisions()
+    assert len(info) == len(contacts) > 4 and (info & 15).any()
+    own = info.copy()
+    spread = (len(info) - np.arange(len(info), dtype=np.int32)) << 4 | (info & 15)
+    ow.set_state(0, x, v)
+    ow.set_replay(tois, contacts, admm, spread)
+    rc, _ = ow.step()
+    assert rc == 0
+    np.testing.assert_array_equal(ow.decisions()[4], spread)
+    off = info & ~15
+    ow.set_state(0, x, v)
+    ow.set_replay(tois, contacts, admm, off)
+    rc, _ = ow.step()
+    assert rc == 0
+    np.testing.assert_array_equal(ow.decisions()[4], off)
+    assert np.abs(ow.positions(0) - x1).max() > 0.0
+    ow.set_state(0, x, v)
+    ow.set_replay(tois, contacts, admm, own)
+    rc, _ = ow.step()
+    np.testing.assert_array_equal(ow.positions(0), x1)
