@@ -203,10 +203,12 @@ def pmc_summary(fetch_dir: str, write_dir: str, probe_log: str) -> int:
                            "copy" if "k_copy_unless" in name else None)
                     if key is None:
                         continue
-                    disp = (fn, row.get("Dispatch_Id"))
+                    disp = (fn, int(row.get("Dispatch_Id", 0)))
                     out.setdefault(key, {}).setdefault(disp, 0.0)
                     out[key][disp] += float(row["Counter_Value"])
-        return {k: list(v.values()) for k, v in out.items()}
+        # In dispatch order; the reference copies are the probe's last ones
+        # (the step also runs the copy kernel on small buffers).
+        return {k: [v[d] for d in sorted(v)] for k, v in out.items()}
 
     probe = None
     with open(probe_log) as f:
@@ -220,6 +222,8 @@ def pmc_summary(fetch_dir: str, write_dir: str, probe_log: str) -> int:
         print("pmc summary: missing probe line or counter rows", file=sys.stderr)
         return 1
     copy_kib = probe["copy_bytes"] / 1024.0
+    reps = probe["copy_reps"]
+    fetch["copy"], write["copy"] = fetch["copy"][-reps:], write["copy"][-reps:]
     f_scale = copy_kib / float(np.mean(fetch["copy"]))
     w_scale = copy_kib / float(np.mean(write["copy"]))
     pcg_fetch = float(np.sum(fetch["pcg"])) * f_scale * 1024.0
@@ -402,4 +406,8 @@ def main() -> int:
 
 
 if __name__ == "__main__":
-    sys.exit(main())
+    # A successful run returns normally (the interpreter's own exit), so
+    # that tools attached to the process finish their output first.
+    rc = main()
+    if rc:
+        sys.exit(rc)

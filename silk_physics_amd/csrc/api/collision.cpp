@@ -27,6 +27,7 @@
 
 #include "api/context.hpp"
 #include "hip/collision_kernels.hpp"
+#include "hip/kernels.hpp"
 
 namespace silk::amd {
 
@@ -201,8 +202,10 @@ class HipCollision final : public CollisionWorld {
   std::vector<size_t> point_off_, edge_off_;  // per object, into query_order_
   DevBuf<OrderSeg> order_segs_;
   DevBuf<int> order_off_, order_of_, ovals_in_, ovals_out_;
-  DevBuf<uint32_t> okeys_in_, okeys_out_;
-  DevBuf<float> order_box_;
+  DevBuf<unsigned long long> okeys_in_, okeys_out_;
+  DevBuf<uint32_t> order_box_;
+  DevBuf<int2> order_blk_;  // bounds workgroups: (segment, first item)
+  int order_nblk_ = 0;
   DevBuf<char> order_temp_;
   size_t order_temp_bytes_ = 0;
   void setup_order_rebuild();
@@ -431,18 +434,27 @@ void HipCollision::setup_order_rebuild() {
   order_off_.upload(off, st_);
   order_of_.upload(of, st_);
   order_box_.resize(6 * static_cast<size_t>(order_nseg_));
+  std::vector<int2> blk;
+  for (int k = 0; k < order_nseg_; ++k)
+    for (int i = 0; i < segs[k].n; i += kOrderItemsPerBlock) blk.push_back(int2{k, i});
+  order_nblk_ = static_cast<int>(blk.size());
+  order_blk_.upload(blk, st_);
   okeys_in_.resize(order_total_);
   okeys_out_.resize(order_total_);
   ovals_in_.resize(order_total_);
   ovals_out_.resize(order_total_);
   order_temp_bytes_ = order_sort_temp_bytes(order_total_, order_nseg_);
   order_temp_.resize(std::max<size_t>(1, order_temp_bytes_));
+  // The uploads above read host vectors that end with this function.
+  hip_check(hipStreamSynchronize(st_), "order tables");
 }
 
 void HipCollision::rebuild_order() {
   if (order_total_ == 0) return;
   OrderScratch sc;
   sc.box = order_box_.get();
+  sc.blk = order_blk_.get();
+  sc.nblk = order_nblk_;
   sc.keys_in = okeys_in_.get();
   sc.keys_out = okeys_out_.get();
   sc.vals_in = ovals_in_.get();
@@ -477,11 +489,10 @@ PrimTables HipCollision::tables() const {
 }
 
 void HipCollision::set_cloth(const float* a, const float* b) {
-  const size_t bytes = 3 * static_cast<size_t>(nvc_) * sizeof(float);
-  hip_check(hipMemcpyAsync(p0_.get(), a, bytes, hipMemcpyDeviceToDevice, st_),
-            "collider t0");
-  hip_check(hipMemcpyAsync(p1_.get(), b, bytes, hipMemcpyDeviceToDevice, st_),
-            "collider t1");
+  const int64_t n = 3 * static_cast<int64_t>(nvc_);
+  launch_copy_unless(st_, p0_.get(), a, n, nullptr);
+  launch_copy_unless(st_, p1_.get(), b, n, nullptr);
+  hip_check(hipGetLastError(), "collider positions");
 }
 
 void HipCollision::refit() {
@@ -581,7 +592,7 @@ void HipCollision::build_jobs(const std::vector<float>& pos0) {
 }
 
 void HipCollision::launch_broadphase() {
-  hip_check(hipMemsetAsync(counters_.get(), 0, 4 * sizeof(int), st_), "counters");
+  launch_fill_u32(st_, counters_.get(), 4, 0u);
   PairOut pt{pt_pairs_.get(), counters_.get(), pt_cap_};
   PairOut ee{ee_pairs_.get(), counters_.get() + 1, ee_cap_};
   launch_traverse_jobs(st_, verts(), jobs_.get(), static_cast<int>(jobs_h_.size()),
@@ -864,9 +875,9 @@ void HipCollision::begin_step(const float* outer_x, float* ms, float* dhat,
   max_color_ = 0;
   n_active_ = 0;
   const size_t nc3 = 3 * static_cast<size_t>(nvc_);
-  hip_check(hipMemsetAsync(w_.get(), 0, nvc_ * sizeof(float), st_), "w reset");
+  launch_fill_u32(st_, w_.get(), nvc_, 0u);
   for (DevBuf<float>* b : {&p_, &eta_, &dual_})
-    hip_check(hipMemsetAsync(b->get(), 0, nc3 * sizeof(float), st_), "reset");
+    launch_fill_u32(st_, b->get(), static_cast<int64_t>(nc3), 0u);
   set_cloth(outer_x, outer_x);
   if (rebuild_order_) rebuild_order();
   refit();
@@ -894,11 +905,8 @@ float HipCollision::min_toi(const float* outer_x, const float* inner_x,
     const double one = 1.0;
     unsigned long long bits;
     std::memcpy(&bits, &one, sizeof(bits));
-    // toi bits start at 1.0: write through the pinned staging word.
-    std::memcpy(h_io_ + 6, &bits, sizeof(bits));
-    hip_check(hipMemcpyAsync(toi_bits_.get(), h_io_ + 6, sizeof(bits),
-                             hipMemcpyHostToDevice, st_),
-              "toi init");
+    // toi bits start at 1.0.
+    launch_fill_u64(st_, toi_bits_.get(), 1, bits);
     ctx_.timed_begin(kTimeBroad);
     launch_broadphase();
     ctx_.timed_end();

@@ -1075,8 +1075,7 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
   Scene& S = *scene_;
   ClothDev c = cloth_dev();
   const size_t n3 = 3 * static_cast<size_t>(S.nv);
-  hip_check(hipMemsetAsync(S.pin_lambda.get(), 0, n3 * sizeof(float), stream_),
-            "pin lambda reset");
+  launch_fill_u32(stream_, S.pin_lambda.get(), static_cast<int64_t>(n3), 0u);
   ContactView cv = S.coll ? S.coll->contact_view() : ContactView{};
   const auto& g = global_;
   AdmmCtl* ctl = S.admm_ctl.get();
@@ -1127,7 +1126,7 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
       S.w_coef.resize(6 * static_cast<size_t>(std::max(1, S.nsys)));
     }
     // The first solve of a segment has no previous correction.
-    hip_check(hipMemsetAsync(S.dx.get(), 0, n3 * sizeof(float), stream_), "dx reset");
+    launch_fill_u32(stream_, S.dx.get(), static_cast<int64_t>(n3), 0u);
   }
   WarmFeatArgs wf;
   wf.r = S.r.get();
@@ -1162,20 +1161,16 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
         // Keep the previous iterate and its residual, project the start onto
         // span{network prediction, previous correction}, and let the judge
         // keep whichever of the two starts has the smaller FP64 residual.
-        hip_check(hipMemcpyAsync(S.w_xprev.get(), S.inner_x.get(), n3 * sizeof(float),
-                                 hipMemcpyDeviceToDevice, stream_),
-                  "warm x copy");
-        hip_check(hipMemcpyAsync(S.w_rprev.get(), S.r.get(), n3 * sizeof(float),
-                                 hipMemcpyDeviceToDevice, stream_),
-                  "warm r copy");
+        // (Doubles are copied as pairs of 32-bit words.)
+        const int64_t n3w = static_cast<int64_t>(n3);
+        launch_copy_unless(stream_, S.w_xprev.get(), S.inner_x.get(), n3w, skip);
+        launch_copy_unless(stream_, S.w_rprev.get(), S.r.get(), n3w, skip);
         if (verify)
-          hip_check(hipMemcpyAsync(S.w_r0dprev.get(), S.r0d.get(), n3 * sizeof(double),
-                                   hipMemcpyDeviceToDevice, stream_),
-                    "warm FP64 r copy");
-        hip_check(hipMemcpyAsync(S.w_part2.get(), S.part2.get(),
-                                 2 * static_cast<size_t>(S.nblk) * sizeof(double),
-                                 hipMemcpyDeviceToDevice, stream_),
-                  "warm partials copy");
+          launch_copy_unless(stream_, reinterpret_cast<float*>(S.w_r0dprev.get()),
+                             reinterpret_cast<const float*>(S.r0d.get()), 2 * n3w, skip);
+        launch_copy_unless(stream_, reinterpret_cast<float*>(S.w_part2.get()),
+                           reinterpret_cast<const float*>(S.part2.get()),
+                           4 * static_cast<int64_t>(S.nblk), skip);
         timed_begin(kTimeWarm);
         const float* v1 = nullptr;
         if (learned) {
@@ -1200,9 +1195,8 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
       }
       if (it == 1) launch_mas_factor(stream_, p, S.cd.get(), skip);
       if (keep_x0 && !fold_copy)
-        hip_check(hipMemcpyAsync(S.x0_copy.get(), S.inner_x.get(), n3 * sizeof(float),
-                                 hipMemcpyDeviceToDevice, stream_),
-                  "warm start copy");
+        launch_copy_unless(stream_, S.x0_copy.get(), S.inner_x.get(),
+                           static_cast<int64_t>(n3), nullptr);
       timed_begin(kTimePcg);
       launch_persistent(pa, 0.0f);
       timed_end();
@@ -1295,8 +1289,7 @@ bool Context::admm_solve(silk_hip_step_stats* stats) {
   if (S.use_persistent) return admm_solve_device(stats);
   ClothDev c = cloth_dev();
   const size_t n3 = 3 * static_cast<size_t>(S.nv);
-  hip_check(hipMemsetAsync(S.pin_lambda.get(), 0, n3 * sizeof(float), stream_),
-            "pin lambda reset");
+  launch_fill_u32(stream_, S.pin_lambda.get(), static_cast<int64_t>(n3), 0u);
   ContactView cv = S.coll ? S.coll->contact_view() : ContactView{};
   const int grid = local_step_grid(S.nv);
   double init_primal = 0.0, init_dual = 0.0, ratio = 1.0;
@@ -1385,8 +1378,8 @@ int Context::step(silk_hip_step_stats* stats) {
   launch_step_init(stream_, c, S.x.get(), S.v.get(), S.prev_x.get(),
                    S.prev_v.get(), S.outer_x.get(), S.outer_v.get());
   // Aux variables restart every step (cloth_assembler.cu:137-142).
-  S.z[S.aux_cur].zero(stream_);
-  S.u[S.aux_cur].zero(stream_);
+  launch_fill_u32(stream_, S.z[S.aux_cur].get(), static_cast<int64_t>(S.z[S.aux_cur].size()), 0u);
+  launch_fill_u32(stream_, S.u[S.aux_cur].get(), static_cast<int64_t>(S.u[S.aux_cur].size()), 0u);
 
   float ms = 0.0f, dhat = 0.0f;
   if (S.coll) S.coll->begin_step(S.outer_x.get(), &ms, &dhat, stats);
@@ -1413,10 +1406,8 @@ int Context::step(silk_hip_step_stats* stats) {
     launch_velocity(stream_, static_cast<int>(n3), S.dt, S.outer_x.get(),
                     S.inner_x.get(), S.outer_v.get());
     if (min_toi >= remaining) {
-      hip_check(hipMemcpyAsync(S.outer_x.get(), S.inner_x.get(),
-                               n3 * sizeof(float), hipMemcpyDeviceToDevice,
-                               stream_),
-                "accept");
+      launch_copy_unless(stream_, S.outer_x.get(), S.inner_x.get(),
+                         static_cast<int64_t>(n3), nullptr);
       break;
     }
     const float rollback = 0.8f * min_toi;
@@ -1431,9 +1422,8 @@ int Context::step(silk_hip_step_stats* stats) {
   // Velocity from the whole accepted displacement (main_loop.cu:281-296).
   launch_velocity(stream_, static_cast<int>(n3), S.dt, S.prev_x.get(),
                   S.outer_x.get(), S.v.get());
-  hip_check(hipMemcpyAsync(S.x.get(), S.outer_x.get(), n3 * sizeof(float),
-                           hipMemcpyDeviceToDevice, stream_),
-            "commit");
+  launch_copy_unless(stream_, S.x.get(), S.outer_x.get(), static_cast<int64_t>(n3),
+                     nullptr);
   if (S.coll) S.coll->end_step(stats);
   hip_check(hipEventRecord(ev_[1], stream_), "event");
   hip_check(hipStreamSynchronize(stream_), "step sync");

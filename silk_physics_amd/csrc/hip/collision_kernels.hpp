@@ -56,22 +56,28 @@ struct OrderSeg {
 };
 
 struct OrderScratch {
-  float* box = nullptr;        // 6 per segment
-  uint32_t* keys_in = nullptr; // total each
-  uint32_t* keys_out = nullptr;
+  uint32_t* box = nullptr;     // 6 per segment: centroid bounds, order-preserving
+                               //   integer encodings of the floats
+  const int2* blk = nullptr;   // per bounds workgroup: (segment, first item)
+  int nblk = 0;
+  unsigned long long* keys_in = nullptr;  // total each: segment << 32 | code
+  unsigned long long* keys_out = nullptr;
   int* vals_in = nullptr;
   int* vals_out = nullptr;
   void* temp = nullptr;        // radix sort workspace
   size_t temp_bytes = 0;
 };
 
-/// Radix-sort workspace bytes for `total` keys in `nseg` segments.
+/// Items of one segment per workgroup of the centroid-bounds pass.
+constexpr int kOrderItemsPerBlock = 2048;
+
+/// Radix-sort workspace bytes for `total` keys of `nseg` segments.
 size_t order_sort_temp_bytes(int total, int nseg);
 
 /// Morton codes of every segment's items at positions p (3 per collider
-/// vertex), a stable segmented radix sort, then the new orders written
-/// (oibvh.cuh:521-560,603-633).  seg_off: nseg + 1 key offsets; seg_of: the
-/// segment of every key.
+/// vertex), one stable radix sort of (segment, code) keys, then the new
+/// orders written (oibvh.cuh:521-560,603-633).  seg_off: nseg + 1 key
+/// offsets; seg_of: the segment of every key.
 hipError_t launch_order_rebuild(hipStream_t s, const float* p, const OrderSeg* segs, int nseg,
                                 const int* seg_off, const int* seg_of, int total,
                                 OrderScratch sc);

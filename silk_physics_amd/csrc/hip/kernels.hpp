@@ -265,6 +265,13 @@ void launch_iterate_verify(hipStream_t s, const PcgDev& p, const RhsArgs& ra,
 void launch_copy_unless(hipStream_t s, float* dst, const float* src, int64_t n,
                         const int* skip);
 
+/// p[i] = v for n 32-bit or 64-bit words.  Used instead of hipMemsetAsync /
+/// device-to-device hipMemcpyAsync inside the step: the runtime's blit
+/// kernels leave the device idle for several microseconds around each one,
+/// a kernel of the library's follows the previous one back to back.
+void launch_fill_u32(hipStream_t s, void* p, int64_t n, uint32_t v);
+void launch_fill_u64(hipStream_t s, void* p, int64_t n, unsigned long long v);
+
 /// x0 += dx.
 void launch_pcg_commit(hipStream_t s, const PcgDev& p);
 

@@ -302,6 +302,13 @@ __global__ void k_copy_unless(float* __restrict__ dst, const float* __restrict__
     dst[i] = src[i];
 }
 
+template <class T>
+__global__ void k_fill(T* __restrict__ p, int64_t n, T v) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
 __global__ void k_commit(int n3, float* __restrict__ x,
                          const float* __restrict__ dx) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -687,6 +694,19 @@ void launch_copy_unless(hipStream_t s, float* dst, const float* src, int64_t n,
   if (n <= 0) return;
   const int blocks = static_cast<int>(std::min<int64_t>(1024, (n + kBlock - 1) / kBlock));
   k_copy_unless<<<blocks, kBlock, 0, s>>>(dst, src, n, skip);
+}
+
+void launch_fill_u32(hipStream_t s, void* p, int64_t n, uint32_t v) {
+  if (n <= 0) return;
+  const int blocks = static_cast<int>(std::min<int64_t>(1024, (n + kBlock - 1) / kBlock));
+  k_fill<uint32_t><<<blocks, kBlock, 0, s>>>(static_cast<uint32_t*>(p), n, v);
+}
+
+void launch_fill_u64(hipStream_t s, void* p, int64_t n, unsigned long long v) {
+  if (n <= 0) return;
+  const int blocks = static_cast<int>(std::min<int64_t>(1024, (n + kBlock - 1) / kBlock));
+  k_fill<unsigned long long>
+      <<<blocks, kBlock, 0, s>>>(static_cast<unsigned long long*>(p), n, v);
 }
 
 void launch_pcg_commit(hipStream_t s, const PcgDev& p) {
