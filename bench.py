@@ -6,7 +6,9 @@ the solve's iterate, restarts included), on one GPU.
 
 Prints one JSON line.  Every solver_step call drains the library's stream
 before returning, so host wall time brackets the device work of the timed
-steps.  Step 0 (lazy scene assembly) is the warmup and is reported apart.
+steps.  Step 0 (lazy scene assembly) is the warmup and is reported apart,
+together with the launch autotuning that follows it (broadphase split depth
+and PCG variant, which leave the results unchanged; --no-autotune skips it).
 
 roofline: the kernel with the largest device time over the timed steps,
 timed with HIP events on the library stream around every launch.  Its
@@ -19,7 +21,7 @@ algorithmic bytes per launch follow the reference formulation's byte model
 and, beside it, this library's own bytes per PCG iteration (vertex-scalar
 ELL and 32x32 banks): 8 offdiag + 48 V + 4096 banks (2048 at 16 bits).
 traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE
-counter runs (profiles/pcg_traffic.json, made with --pmc-probe), scaled by
+counter runs (profiles/k_pcg_persistent_c2_traffic.json, made with --pmc-probe), scaled by
 the same counters on the library's copy kernel of known size.
 cpu_baseline: the reference CPU algorithm restated (PD with a sparse
 Cholesky factor, SVD stretch projection, barrier reflection targets, CCD
@@ -44,7 +46,7 @@ from silk_physics_amd import scenes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "sim steps/sec, 80x80 sheet + collision; PCG iters to 1e-6 rel residual"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pcg_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_pcg_persistent_c2_traffic.json")
 CLASSES = {1: "k_pcg_persistent", 2: "k_local_step", 3: "k_contact_update",
            4: "k_broadphase", 5: "k_ccd", 6: "k_residual"}
 
@@ -77,6 +79,72 @@ def build_world(n: int, mas: int = sp.MAS_FP32, batched: int = 0):
         assert r, r.to_string()
         handles.append(h)
     return sheets[0], w, handles[0]
+
+
+def train_warm_net(n: int = 96, steps: int = 10) -> np.ndarray:
+    """The per-vertex warm-start network trained on the GPU from the solves
+    of an n x n drop on the same sphere (samples recorded by the solver)."""
+    s = scenes.c2_scene(n, z=0.32)
+    w = sp.World()
+    assert w.init(sp.Backend.GPU)
+    assert w.set_global_config(sp.GlobalConfig(**global_kw()))
+    cc = sp.CollisionConfig()
+    r, h = w.add_cloth(sp.ClothConfig(), cc, s["V"], s["F"], s["pins"])
+    assert r, r.to_string()
+    r, _ = w.add_obstacle(cc, s["SV"], s["SF"])
+    assert r, r.to_string()
+    assert w.warm_start_record(400_000, 0.1)
+    for _ in range(steps):
+        assert w.solver_step()
+    r, _ = w.warm_start_train(600, 8192, 2e-3, seed=5)
+    assert r, r.to_string()
+    r, net = w.get_warm_start_net()
+    assert r
+    return net
+
+
+def compare_c5(args) -> int:
+    """The 128x128 colliding sheet (reference ADMM defaults: up to 100
+    iterations with the convergence test) in two forms at the same FP64 test
+    of every PCG solve (||b - A x|| <= 1e-6 ||b||): the plain form (FP32 MAS,
+    CG from the previous iterate) and the learned form (BF16 MAS, projected
+    learned start).  One JSON line with each form's steps/s and PCG
+    iterations per solve over the same steps from rest."""
+    net = train_warm_net()
+    out = {}
+    for name, mas, warm in (("plain", sp.MAS_FP32, sp.WARM_PREVIOUS),
+                            ("learned_bf16", sp.MAS_BF16, sp.WARM_LEARNED)):
+        s = scenes.c2_scene(128, z=0.32)
+        w = sp.World()
+        assert w.init(sp.Backend.GPU)
+        kw = global_kw(mas=mas)
+        kw.pop("max_inner_iteration")
+        if warm == sp.WARM_LEARNED:
+            assert w.set_warm_start_net(net)
+        assert w.set_global_config(sp.GlobalConfig(**kw, warm_start=warm))
+        cc = sp.CollisionConfig()
+        r, h = w.add_cloth(sp.ClothConfig(), cc, s["V"], s["F"], s["pins"])
+        assert r, r.to_string()
+        r, _ = w.add_obstacle(cc, s["SV"], s["SF"])
+        assert r, r.to_string()
+        assert w.solver_step()  # scene assembly
+        it = sol = unver = 0
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            assert w.solver_step()
+            st = w.last_stats
+            it += st.pcg_iterations
+            sol += st.pcg_solves
+            unver += st.pcg_unverified_solves
+        dt = time.perf_counter() - t0
+        out[name] = {"steps_per_s": round(args.steps / dt, 3),
+                     "pcg_iters_per_solve": round(it / max(1, sol), 2),
+                     "pcg_solves": sol, "unverified": unver}
+    out["ratio_steps_per_s"] = round(out["learned_bf16"]["steps_per_s"] /
+                                     out["plain"]["steps_per_s"], 3)
+    print(json.dumps({"comparison": "128x128 colliding sheet, learned + BF16 vs plain",
+                      "steps": args.steps, **out}), flush=True)
+    return 0
 
 
 def byte_model(cls: int, info: dict, iters: int, launches: int,
@@ -157,7 +225,7 @@ def cpu_baseline(n: int, budget_s: float, gpu_step_s: list) -> dict:
 
 def traffic_summary() -> dict | None:
     """HBM bytes per launch of the persistent PCG from the committed counter
-    summary (profiles/pcg_traffic.json), or None."""
+    summary (profiles/k_pcg_persistent_c2_traffic.json), or None."""
     try:
         with open(TRAFFIC_FILE) as f:
             return json.load(f)
@@ -184,7 +252,7 @@ def pmc_probe(args) -> int:
 
 
 def pmc_summary(fetch_dir: str, write_dir: str, probe_log: str) -> int:
-    """profiles/pcg_traffic.json from two rocprofv3 --pmc runs of the probe
+    """profiles/k_pcg_persistent_c2_traffic.json from two rocprofv3 --pmc runs of the probe
     (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch): the persistent PCG's
     bytes per iteration, each counter scaled by what it reads for the copy
     kernel's known 256 MiB per launch."""
@@ -269,10 +337,17 @@ def main() -> int:
     ap.add_argument("--batched", type=int, default=0,
                     help="instead of one sheet, this many independent sheets of "
                          "16..128 quads (the batched-instance scene)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes (the broadphase split "
+                         "depth and the PCG variant are otherwise timed after "
+                         "the warmup and the fastest kept)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short run for rocprofv3 --pmc passes (no JSON metric)")
+    ap.add_argument("--compare-c5", action="store_true",
+                    help="the 128x128 sheet in the plain and the learned + BF16 "
+                         "forms (one JSON line, not the metric)")
     ap.add_argument("--pmc-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "PROBE_LOG"),
-                    help="write profiles/pcg_traffic.json from the two counter runs")
+                    help="write profiles/k_pcg_persistent_c2_traffic.json from the two counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the step "
@@ -283,6 +358,8 @@ def main() -> int:
         return pmc_probe(args)
     if args.pmc_summary:
         return pmc_summary(*args.pmc_summary)
+    if args.compare_c5:
+        return compare_c5(args)
     mas = {"fp32": sp.MAS_FP32, "fp16": sp.MAS_FP16, "bf16": sp.MAS_BF16}[args.mas]
 
     s, w, h = build_world(args.n, mas, args.batched)
@@ -291,6 +368,10 @@ def main() -> int:
     t0 = time.perf_counter()
     for _ in range(args.warmup):
         r = w.solver_step()
+        assert r, r.to_string()
+    tune = None
+    if not args.no_autotune:
+        r, tune = w.autotune()
         assert r, r.to_string()
     warm_s = time.perf_counter() - t0
     info = w.scene_info()
@@ -397,6 +478,7 @@ def main() -> int:
         "kernel_ms": {CLASSES[c]: round(v[0], 3) for c, v in kern.items()},
         "kernel_launches": {CLASSES[c]: v[1] for c, v in kern.items()},
         "warmup_s": round(warm_s, 3),
+        "autotune": tune,
         "roofline": roof,
     }
     if not args.no_cpu_baseline:

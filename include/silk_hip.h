@@ -433,6 +433,23 @@ int silk_hip_debug_warm_net(silk_hip_ctx* ctx, int64_t n, const float* feat,
                             const float* target, float* y, float* grad,
                             double* loss);
 
+/* ---- Launch autotuning ----
+ * No counterpart in the reference (its launch shapes are fixed at compile
+ * time).  For callers that step one scene many times: times, on the current
+ * scene, the broadphase at traversal split depths 2..7 (5 launches each) and
+ * the persistent PCG with one and with two workgroups per CU (a solve of the
+ * scene's matrix from x = 0 to 1e-6 on a fixed right-hand side, best of 3),
+ * and keeps the fastest of each until the scene is rebuilt.  Every choice
+ * gives the same results; only the time changes.  Needs one silk_hip_step on
+ * the current scene first (SILK_HIP_ERR_INVALID_CONFIG otherwise).
+ * report (cap >= SILK_HIP_AUTOTUNE_REPORT int32) receives: [0] the chosen
+ * split depth, [1] the chosen PCG variant (0 one workgroup per CU, 1 two,
+ * -1 no persistent PCG), [2..7] microseconds per broadphase launch at depth
+ * 2..7, [8..9] microseconds per PCG solve of variants 0 and 1 (-1 where the
+ * variant does not fit the device). */
+#define SILK_HIP_AUTOTUNE_REPORT 10
+int silk_hip_autotune(silk_hip_ctx* ctx, int32_t* report, int32_t cap);
+
 #ifdef __cplusplus
 }
 #endif

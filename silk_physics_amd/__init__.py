@@ -87,6 +87,7 @@ def load_library():
         "silk_hip_ccd_min_noalloc": (C.c_int, [vp, i64, ip, dp, dp, dp, C.c_double,
                                                C.c_double, i32, dp]),
         "silk_hip_set_warm_start_net": (C.c_int, [vp, fp, i64]),
+        "silk_hip_autotune": (C.c_int, [vp, ip, C.c_int32]),
         "silk_hip_get_warm_start_net": (C.c_int, [vp, fp, i64]),
         "silk_hip_warm_start_record": (C.c_int, [vp, i64, C.c_float]),
         "silk_hip_warm_start_samples": (C.c_int, [vp, C.POINTER(i64)]),
@@ -111,6 +112,7 @@ WARM_PREVIOUS, WARM_LEARNED, WARM_EXTRAPOLATE = 0, 1, 2
 PROJECT_RESIDUAL, PROJECT_GALERKIN = 0, 1
 CCD_ZERO_SKIP, CCD_ZERO_CONTACT = 0, 1
 WARM_NET_PARAMS = 5313  # SILK_HIP_WARM_NET_PARAMS
+AUTOTUNE_REPORT = 10  # SILK_HIP_AUTOTUNE_REPORT
 
 class SilkHipClothParams(C.Structure):
     _fields_ = [("elastic_stiffness", C.c_float), ("bending_stiffness", C.c_float),
@@ -606,6 +608,20 @@ class World:
         if st != 0:
             raise RuntimeError(self._wrap(st).to_string())
         return out.reshape(n, 3)
+
+    # -- launch autotuning (library extension, see include/silk_hip.h)
+    def autotune(self) -> tuple[Result, dict]:
+        """Time the broadphase split depths and the persistent PCG variants
+        on the current scene (after one solver_step) and keep the fastest.
+        Returns the result and the report: chosen depth and PCG variant,
+        microseconds per broadphase launch by depth and per PCG solve by
+        variant (-1: not timed)."""
+        rep = np.full(AUTOTUNE_REPORT, -1, dtype=np.int32)
+        r = self._wrap(load_library().silk_hip_autotune(
+            self._ctx, rep.ctypes.data_as(C.POINTER(C.c_int32)), rep.size))
+        return r, {"split_depth": int(rep[0]), "pcg_variant": int(rep[1]),
+                   "broadphase_us": {d: int(rep[d]) for d in range(2, 8)},
+                   "pcg_us": {0: int(rep[8]), 1: int(rep[9])}}
 
     # -- learned warm start (library extension, see include/silk_hip.h)
     def set_warm_start_net(self, params) -> Result:

@@ -632,6 +632,14 @@ int silk_hip_debug_pcg(silk_hip_ctx* ctx, const float* d, int64_t nv,
   });
 }
 
+int silk_hip_autotune(silk_hip_ctx* ctx, int32_t* report, int32_t cap) {
+  return guarded(ctx, [&](Context& c) {
+    if (report == nullptr || cap < SILK_HIP_AUTOTUNE_REPORT)
+      return int{SILK_HIP_ERR_INVALID_CONFIG};
+    return c.autotune(report);
+  });
+}
+
 int silk_hip_set_warm_start_net(silk_hip_ctx* ctx, const float* params, int64_t n) {
   return guarded(ctx, [&](Context& c) {
     if (params == nullptr || n != SILK_HIP_WARM_NET_PARAMS)

@@ -165,6 +165,9 @@ class HipCollision final : public CollisionWorld {
                  std::vector<int>& recs, std::vector<float>& dist) override;
   const std::vector<int>& decision_log() const override { return log_; }
   const std::vector<int>& decision_info() const override { return info_; }
+  int split_depth() const override { return split_depth_; }
+  void set_split_depth(int depth) override;
+  float time_broadphase(int reps) override;
 
   // Static tables (filled by make_collision_world).
   std::vector<int> h_tri, h_edge, h_tri_obj, h_edge_obj, h_point_obj;
@@ -194,6 +197,8 @@ class HipCollision final : public CollisionWorld {
   std::vector<TraverseJob> jobs_h_;
   DevBuf<TraverseJob> jobs_;
   int job_items_ = 0;
+  std::vector<int> job_height_;  // per job: its tree's height
+  int split_depth_ = kTraverseSplitDepth;
 
   // Per-step Morton re-ordering of tree primitives and query orders
   // (SILK_HIP_BVH_REBUILD=0 keeps the set-up order).
@@ -254,7 +259,10 @@ class HipCollision final : public CollisionWorld {
   void refit();
   void build_jobs(const std::vector<float>& pos0);
   void launch_broadphase();
-  std::vector<DcdRec> dcd(float time, float dhat, silk_hip_step_stats* stats);
+  // reuse_pairs: test the candidate lists of the last broadphase (the CCD
+  // of the segment this DCD's positions lie on) instead of a new one.
+  std::vector<DcdRec> dcd(float time, float dhat, silk_hip_step_stats* stats,
+                          bool reuse_pairs = false);
   void append(std::vector<DcdRec> recs);
   void recolor(int n_old);
   void upload_contacts(int n_old);
@@ -559,13 +567,14 @@ void HipCollision::build_jobs(const std::vector<float>& pos0) {
     j.self = self;
     int height = 0;
     while ((1 << height) < t.P) ++height;
-    j.depth = std::min(kTraverseSplitDepth, height);
+    j.depth = std::min(split_depth_, height);
     j.item0 = items;
     j.nitem = q.n << j.depth;
     j.gate_a = ga;
     j.gate_b = gb;
     items += (j.nitem + kTraverseWg - 1) / kTraverseWg * kTraverseWg;
     jobs_h_.push_back(j);
+    job_height_.push_back(height);
   };
   auto root = [&](const ObjDesc& o) { return trees_[o.tree_t].node + 1; };
   const size_t n = objs_.size();
@@ -591,6 +600,37 @@ void HipCollision::build_jobs(const std::vector<float>& pos0) {
   hip_check(hipStreamSynchronize(st_), "broadphase jobs");
 }
 
+void HipCollision::set_split_depth(int depth) {
+  split_depth_ = std::max(0, depth);
+  int items = 0;
+  for (size_t k = 0; k < jobs_h_.size(); ++k) {
+    TraverseJob& j = jobs_h_[k];
+    j.depth = std::min(split_depth_, job_height_[k]);
+    j.item0 = items;
+    j.nitem = j.q.n << j.depth;
+    items += (j.nitem + kTraverseWg - 1) / kTraverseWg * kTraverseWg;
+  }
+  job_items_ = items;
+  if (!jobs_h_.empty()) jobs_.upload(jobs_h_, st_);
+  hip_check(hipStreamSynchronize(st_), "broadphase jobs");
+}
+
+float HipCollision::time_broadphase(int reps) {
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "event");
+  hip_check(hipEventCreate(&e1), "event");
+  launch_broadphase();  // warm
+  hip_check(hipEventRecord(e0, st_), "event");
+  for (int k = 0; k < reps; ++k) launch_broadphase();
+  hip_check(hipEventRecord(e1, st_), "event");
+  hip_check(hipEventSynchronize(e1), "event sync");
+  float ms = 0.0f;
+  hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+  hip_check(hipEventDestroy(e0), "event");
+  hip_check(hipEventDestroy(e1), "event");
+  return ms / static_cast<float>(std::max(1, reps));
+}
+
 void HipCollision::launch_broadphase() {
   launch_fill_u32(st_, counters_.get(), 4, 0u);
   PairOut pt{pt_pairs_.get(), counters_.get(), pt_cap_};
@@ -612,11 +652,17 @@ void HipCollision::read_io() {
 static int grow(int need) { return std::max(1024, need + need / 2); }
 
 std::vector<DcdRec> HipCollision::dcd(float time, float dhat,
-                                      silk_hip_step_stats* stats) {
+                                      silk_hip_step_stats* stats, bool reuse_pairs) {
   for (;;) {
-    ctx_.timed_begin(kTimeBroad);
-    launch_broadphase();
-    ctx_.timed_end();
+    if (reuse_pairs) {
+      // Record count (and the unused slot after it) back to zero; the pair
+      // counts stay.
+      launch_fill_u32(st_, counters_.get() + 2, 2, 0u);
+    } else {
+      ctx_.timed_begin(kTimeBroad);
+      launch_broadphase();
+      ctx_.timed_end();
+    }
     launch_dcd(st_, verts(), tables(), pt_pairs_.get(), counters_.get(), pt_cap_,
                ee_pairs_.get(), counters_.get() + 1, ee_cap_, time, dhat,
                recs_.get(), counters_.get() + 2, rec_cap_);
@@ -940,11 +986,16 @@ float HipCollision::min_toi(const float* outer_x, const float* inner_x,
   }
 }
 
+// The DCD after a rollback tests the candidates of the CCD that found the
+// rollback's TOI instead of a broadphase of its own (find_collision.cu runs
+// one): the rollback positions lie on that CCD's segment (and obstacles
+// within their frame motion), so every pair whose padded boxes overlap
+// there is in the CCD's list of overlapping swept boxes, and the DCD's
+// distance test selects the same records from the longer list.
 void HipCollision::rollback_contacts(const float* outer_x, float time,
                                      float dhat, silk_hip_step_stats* stats) {
   set_cloth(outer_x, outer_x);
-  refit();
-  append(dcd(time, dhat, stats));
+  append(dcd(time, dhat, stats, /*reuse_pairs=*/true));
 }
 
 void HipCollision::contact_update(const float* x, const int* skip) {

@@ -64,6 +64,14 @@ class CollisionWorld {
   /// take part in the contact solve (nonzero coefficient on a movable
   /// vertex; bit k for vertex k).
   virtual const std::vector<int>& decision_info() const = 0;
+  /// Broadphase work split: each query is walked from every subtree at this
+  /// depth of the trees (capped by a tree's height).  Candidate sets do not
+  /// depend on it, only their order and the launch's speed.
+  virtual int split_depth() const = 0;
+  virtual void set_split_depth(int depth) = 0;
+  /// Mean device time (ms) of `reps` broadphase launches on the current
+  /// collider positions.
+  virtual float time_broadphase(int reps) = 0;
 };
 
 /// nullptr when no object has collision enabled.

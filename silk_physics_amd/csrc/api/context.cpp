@@ -809,7 +809,8 @@ void Context::build_scene() {
     // fit the device that way, the variant that puts two on a CU.
     const size_t lds = pcg_persistent_lds(a);
     auto make_batches = [&](int dense) {
-      S.ps_batches.clear();
+      std::vector<std::pair<int, int>>& out = S.ps_variant[dense];
+      out.clear();
       if (ps_wg <= 0) return false;
       for (int s0 = 0; s0 < S.nsys;) {
         const int w0 = ps_sys[s0].wg0;
@@ -821,19 +822,22 @@ void Context::build_scene() {
           w1 = we;
           ++s1;
         }
-        if (s1 == s0) return false;  // a system alone does not fit
-        S.ps_batches.emplace_back(w0, w1 - w0);
+        if (s1 == s0) {  // a system alone does not fit
+          out.clear();
+          return false;
+        }
+        out.emplace_back(w0, w1 - w0);
         s0 = s1;
       }
       return true;
     };
-    a.dense = 0;
-    bool batches_ok = make_batches(0);
-    if (!batches_ok && !(std::getenv("SILK_HIP_PCG_DENSE") &&
-                         std::strcmp(std::getenv("SILK_HIP_PCG_DENSE"), "0") == 0)) {
-      batches_ok = make_batches(1);
-      a.dense = batches_ok ? 1 : 0;
-    }
+    const bool ok0 = make_batches(0);
+    const char* dense_env = std::getenv("SILK_HIP_PCG_DENSE");
+    const bool ok1 = !(dense_env && std::strcmp(dense_env, "0") == 0) && make_batches(1);
+    if (!ok1) S.ps_variant[1].clear();
+    a.dense = ok0 ? 0 : 1;
+    const bool batches_ok = ok0 || ok1;
+    S.ps_batches = S.ps_variant[a.dense];
     S.use_persistent = want && xch_fits && batches_ok;
   }
   S.coll = make_collision_world(*this, S);
@@ -1425,6 +1429,7 @@ int Context::step(silk_hip_step_stats* stats) {
   launch_copy_unless(stream_, S.x.get(), S.outer_x.get(), static_cast<int64_t>(n3),
                      nullptr);
   if (S.coll) S.coll->end_step(stats);
+  S.stepped = true;
   hip_check(hipEventRecord(ev_[1], stream_), "event");
   hip_check(hipStreamSynchronize(stream_), "step sync");
   float ms_step = 0.0f;
@@ -1444,7 +1449,8 @@ int Context::step(silk_hip_step_stats* stats) {
   return SILK_HIP_OK;
 }
 
-int Context::debug_pcg(const float* d, const float* b, float rel_tol, float* x) {
+int Context::debug_pcg(const float* d, const float* b, float rel_tol, float* x,
+                       float* launch_ms) {
   if (scene_ && scene_->dt != global_.dt) invalidate_scene();
   if (scene_dirty_ || !scene_) build_scene();
   Scene& S = *scene_;
@@ -1485,7 +1491,9 @@ int Context::debug_pcg(const float* d, const float* b, float rel_tol, float* x) 
   if (S.use_persistent) {
     PersistDev a = persist_args(S, global_, pcg_max_iter());
     a.dx_out = nullptr;
+    if (launch_ms) hip_check(hipEventRecord(ev_[0], stream_), "event");
     launch_persistent(a, rel_tol);
+    if (launch_ms) hip_check(hipEventRecord(ev_[1], stream_), "event");
     S.h_state.resize(S.nsys);
     hip_check(hipMemcpyAsync(S.h_state.data(), S.ps_out.get(), S.nsys * sizeof(PcgState),
                              hipMemcpyDeviceToHost, stream_),
@@ -1497,8 +1505,88 @@ int Context::debug_pcg(const float* d, const float* b, float rel_tol, float* x) 
                            stream_),
             "debug x");
   hip_check(hipStreamSynchronize(stream_), "debug pcg");
+  if (launch_ms && S.use_persistent)
+    hip_check(hipEventElapsedTime(launch_ms, ev_[0], ev_[1]), "elapsed");
   for (const auto& st : S.h_state) iters += st.iter;
   return iters;
+}
+
+int Context::autotune(int32_t* report) {
+  for (int k = 0; k < SILK_HIP_AUTOTUNE_REPORT; ++k) report[k] = -1;
+  if (scene_dirty_ || !scene_ || !scene_->stepped) {
+    last_error_ = "autotune needs a solver step on the current scene";
+    return SILK_HIP_ERR_INVALID_CONFIG;
+  }
+  Scene& S = *scene_;
+  // Broadphase: the depth to which every traversal job is split into
+  // independent sub-tree items.  The candidate lists are sorted before use,
+  // so every depth gives the same results.
+  if (S.coll) {
+    int best = S.coll->split_depth();
+    float best_ms = INFINITY;
+    for (int depth = 2; depth <= 7; ++depth) {
+      S.coll->set_split_depth(depth);
+      const float ms = S.coll->time_broadphase(5) / 5.0f;
+      report[2 + depth - 2] = static_cast<int32_t>(std::lround(1000.0f * ms));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = depth;
+      }
+    }
+    S.coll->set_split_depth(best);
+    report[0] = best;
+  }
+  // PCG: one or two persistent workgroups per CU.  Both run the same
+  // arithmetic in the same order; a solve of the scene's matrix from x = 0
+  // to 1e-6 on a fixed right-hand side times each.
+  if (S.use_persistent) {
+    const size_t nv = static_cast<size_t>(S.nv), n3 = 3 * nv;
+    std::vector<float> d(nv), b(n3), x(n3);
+    hip_check(hipMemcpyAsync(d.data(), S.d.get(), nv * sizeof(float),
+                             hipMemcpyDeviceToHost, stream_),
+              "autotune d");
+    hip_check(hipStreamSynchronize(stream_), "autotune d");
+    uint32_t h = 12345u;
+    for (auto& v : b) {
+      h = h * 1664525u + 1013904223u;
+      v = static_cast<float>(h >> 8) * (2.0f / 16777216.0f) - 1.0f;
+    }
+    const int32_t crit = global_.pcg_criterion;
+    global_.pcg_criterion = SILK_HIP_PCG_RHS;
+    const int keep = S.pers.dense;
+    float best_ms = INFINITY;
+    int best = keep;
+    std::vector<float> x_first;
+    bool agree = true;
+    for (int dense = 0; dense < 2; ++dense) {
+      if (S.ps_variant[dense].empty()) continue;
+      S.pers.dense = dense;
+      S.ps_batches = S.ps_variant[dense];
+      float ms = INFINITY;
+      for (int rep = 0; rep < 4; ++rep) {
+        float t = 0.0f;
+        debug_pcg(d.data(), b.data(), 1e-6f, x.data(), &t);
+        if (rep > 0) ms = std::min(ms, t);
+      }
+      // The variants differ only in residency: their solutions agree bit for bit.
+      if (x_first.empty()) x_first = x;
+      else agree = agree && std::memcmp(x_first.data(), x.data(), n3 * sizeof(float)) == 0;
+      report[8 + dense] = static_cast<int32_t>(std::lround(1000.0f * ms));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = dense;
+      }
+    }
+    global_.pcg_criterion = crit;
+    S.pers.dense = best;
+    S.ps_batches = S.ps_variant[best];
+    report[1] = best;
+    if (!agree) {
+      last_error_ = "the persistent PCG variants gave different solutions";
+      return SILK_HIP_ERR_UNKNOWN;
+    }
+  }
+  return SILK_HIP_OK;
 }
 
 void Context::update_pin_targets(const ClothEntity& c) {

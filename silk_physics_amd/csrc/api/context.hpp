@@ -123,6 +123,10 @@ struct Scene {
   bool use_persistent = false;
   PersistDev pers;  // device pointers into the buffers below
   std::vector<std::pair<int, int>> ps_batches;  // (first workgroup, count) per launch
+  // The batch lists of the one- and two-workgroups-per-CU variants (empty
+  // where a system alone does not fit that way); ps_batches is one of them.
+  std::vector<std::pair<int, int>> ps_variant[2];
+  bool stepped = false;  // a step has run on this scene (autotune needs one)
   DevBuf<PcgSysDev> ps_sys;
   DevBuf<int> ps_slot_row, ps_sell_off, ps_sell_width, ps_sell_lcol, ps_halo_ptr,
       ps_halo_slot, ps_wg_n1_ptr, ps_wg_n1, ps_wg_c_ptr, ps_wg_c, ps_slot_cn,
@@ -232,8 +236,14 @@ class Context {
   bool admm_solve(silk_hip_step_stats* st);
   bool admm_solve_device(silk_hip_step_stats* st);
   void get_positions(const ClothEntity& c, float* out) const;
+  /// silk_hip_autotune: times the broadphase split depths and the PCG launch
+  /// variants on the current scene and keeps the fastest (report layout in
+  /// silk_hip.h).
+  int autotune(int32_t* report);
   /// silk_hip_debug_pcg: one solve of K + diag(d) x = b from x = 0.
-  int debug_pcg(const float* d, const float* b, float rel_tol, float* x);
+  /// launch_ms, when given, receives the persistent launch's duration.
+  int debug_pcg(const float* d, const float* b, float rel_tol, float* x,
+                float* launch_ms = nullptr);
   /// Push one cloth's pin targets to the device scene (if built).
   void update_pin_targets(const ClothEntity& c);
 
