@@ -8,16 +8,18 @@
 // (solver/contact_constraints.cu:203-411).
 //
 // Device side: per object one implicit BVH over its triangles and one over
-// its edges (Morton-sorted once, refit every call), point and edge queries
-// traverse them; the CCD kernel and the DCD kernel consume the candidate
-// lists straight from device memory.  Host side: the DCD records are keyed on
-// global vertex ids, sorted, de-duplicated against the contacts of the step,
-// turned into Daviet contacts and greedily coloured in contact order, which
-// makes the contact solve independent of the order in which the GPU found the
-// pairs.
+// its edges (Morton-sorted at every step start, refit every call), point and
+// edge queries traverse them; the CCD kernel and the DCD kernel consume the
+// candidate lists straight from device memory.  Host side: the DCD records
+// are keyed on global vertex ids, sorted, de-duplicated against the contacts
+// of the step and turned into Daviet contacts; the device colours them
+// first-fit in contact order.  Both make the contact solve independent of
+// the order in which the GPU found the pairs.
 #include "api/collision.hpp"
 
 #include <algorithm>
+#include <stdexcept>
+#include <string>
 #include <array>
 #include <chrono>
 #include <cmath>
@@ -233,18 +235,21 @@ class HipCollision final : public CollisionWorld {
   std::vector<HostContact> contacts_;
   std::unordered_set<std::array<int, 5>, KeyHash> keys_;
   std::vector<int> color_;               // per contact, from 1
-  std::vector<std::vector<int>> vinc_;   // per cloth vertex: its contacts
+  std::vector<uint8_t> solved_;          // per cloth vertex: a contact solves for it
   std::vector<int> touched_;             // vertices with contacts
   std::vector<int> cvert_;               // contact vertices, first use order
   std::vector<int> vloc_;                // per cloth vertex: index in cvert_
   int max_color_ = 0;
   PinnedVec<ContactDev> hc_pin_;
-  PinnedVec<int> off_pin_;
+  PinnedVec<int> col_pin_;  // old colours up, colouring results back
   PinnedVec<int> act_pin_;
   PinnedVec<float> w_pin_;
-  std::vector<int> color_off_h_, color_list_h_;
+  std::vector<int> color_off_h_;
   int n_active_ = 0;
-  DevBuf<ContactDev> c_dev_;
+  DevBuf<ContactDev> c_dev_;  // colour order
+  DevBuf<ContactDev> c_in_;   // contact order (colouring input)
+  DevBuf<int> col_dev_, col_head_, col_meta_;
+  DevBuf<unsigned> col_mask_;
   DevBuf<int> color_off_;
   DevBuf<float> corr_;
   int corr_cap_ = 0;
@@ -264,8 +269,7 @@ class HipCollision final : public CollisionWorld {
   std::vector<DcdRec> dcd(float time, float dhat, silk_hip_step_stats* stats,
                           bool reuse_pairs = false);
   void append(std::vector<DcdRec> recs);
-  void recolor(int n_old);
-  void upload_contacts(int n_old);
+  void color_and_upload(int n_old);
   void read_io();
 };
 
@@ -754,72 +758,36 @@ void HipCollision::append(std::vector<DcdRec> recs) {
     contacts_.push_back(c);
   }
   if (static_cast<int>(contacts_.size()) == n_old) return;
-  recolor(n_old);
-  upload_contacts(n_old);
+  color_and_upload(n_old);
 }
 
-// Greedy first-fit colouring in contact order over shared movable vertices
-// (contact_coloring.cu:241-392 with a deterministic schedule).  A contact's
-// colour depends only on the contacts before it, so appended contacts are
-// coloured alone and earlier colours stay.
-void HipCollision::recolor(int n_old) {
+// Contact weights and contact vertices of the contacts appended since n_old,
+// their colours from the device (k_color_contacts: first-fit in contact
+// order over shared movable vertices, the order-independent equivalent of
+// the sequential greedy colouring), and the contacts in colour order.
+void HipCollision::color_and_upload(int n_old) {
   const int n = static_cast<int>(contacts_.size());
-  if (vinc_.size() < static_cast<size_t>(nvc_)) vinc_.resize(nvc_);
+  if (solved_.size() < static_cast<size_t>(nvc_)) solved_.assign(nvc_, 0);
   float* hw = w_pin_.reserve(std::max(1, nvc_));
   if (n_old == 0 && touched_.empty())
     std::fill(hw, hw + std::max(1, nvc_), 0.0f);
-  color_.resize(n, 0);
-  std::vector<char> used;
   for (int c = n_old; c < n; ++c) {
-    used.assign(max_color_ + 2, 0);
     const HostContact& hc = contacts_[c];
     for (int k = 0; k < 4; ++k) {
-      if (hc.gamma[k] == 0.0) continue;
-      for (int o : vinc_[hc.state[k]]) used[color_[o]] = 1;
-    }
-    int col = 1;
-    while (used[col]) ++col;
-    color_[c] = col;
-    max_color_ = std::max(max_color_, col);
-    int mask = 0;
-    for (int k = 0; k < 4; ++k) mask |= hc.gamma[k] != 0.0 ? 1 << k : 0;
-    info_.push_back(col << 4 | mask);
-    for (int k = 0; k < 4; ++k) {
-      if (hc.gamma[k] == 0.0) continue;
-      auto& lst = vinc_[hc.state[k]];
-      if (lst.empty()) {
-        touched_.push_back(hc.state[k]);
-        hw[hc.state[k]] = static_cast<float>(S_.h_contact_w[hc.state[k]]);
+      const int v = hc.state[k];
+      if (hc.gamma[k] != 0.0 && !solved_[v]) {
+        solved_[v] = 1;
+        touched_.push_back(v);
+        hw[v] = static_cast<float>(S_.h_contact_w[v]);
       }
-      if (lst.empty() || lst.back() != c) lst.push_back(c);
-    }
-  }
-  const int maxc = max_color_;
-  color_off_h_.assign(maxc + 1, 0);
-  for (int c = 0; c < n; ++c) color_off_h_[color_[c]] += 1;
-  for (int k = 1; k <= maxc; ++k) color_off_h_[k] += color_off_h_[k - 1];
-  color_list_h_.assign(n, 0);
-  std::vector<int> fill(color_off_h_.begin(), color_off_h_.end() - 1);
-  for (int c = 0; c < n; ++c) color_list_h_[fill[color_[c] - 1]++] = c;
-  n_active_ = static_cast<int>(touched_.size());
-  // Staging buffers are not in flight here: the records just read back
-  // drained the stream.
-  hip_check(hipMemcpyAsync(w_.get(), hw, std::max(1, nvc_) * sizeof(float),
-                           hipMemcpyHostToDevice, st_),
-            "contact weights");
-}
-
-void HipCollision::upload_contacts(int n_old) {
-  const int n = static_cast<int>(contacts_.size());
-  // Contact vertices of the new contacts, numbered in order of first use.
-  for (int i = n_old; i < n; ++i)
-    for (int k = 0; k < 4; ++k) {
-      const int v = contacts_[i].state[k];
+      // Contact vertices, numbered in order of first use.
       if (v >= 0 && vloc_[v] < 0) {
         vloc_[v] = static_cast<int>(cvert_.size());
         cvert_.push_back(v);
       }
     }
+  }
+  n_active_ = static_cast<int>(touched_.size());
   const int nact = static_cast<int>(cvert_.size());
   int* act = act_pin_.reserve(std::max(1, nact));
   std::copy(cvert_.begin(), cvert_.end(), act);
@@ -828,18 +796,20 @@ void HipCollision::upload_contacts(int n_old) {
                            st_),
             "contact vertices");
   pc_.reserve(3 * static_cast<size_t>(std::max(1, nact)));
-  // Device copy in colour order (ids ascending inside a colour), so that a
-  // colour is one contiguous range.
+  // Staging copy in contact order; the colouring kernel writes the colour
+  // ordered one.  (No copy from the pinned buffers is in flight: the last
+  // call ended with a synchronisation.)
   ContactDev* hc = hc_pin_.reserve(std::max(1, n));
-  for (int e = 0; e < n; ++e) {
-    const int i = color_list_h_[e];
+  for (int i = 0; i < n; ++i) {
     const HostContact& c = contacts_[i];
-    ContactDev& d = hc[e];
+    ContactDev& d = hc[i];
     for (int k = 0; k < 4; ++k) {
       d.state[k] = c.state[k] >= 0 ? vloc_[c.state[k]] : -1;
       // A static vertex's term is in affine; its coefficient is 0 here so
       // that the kernels add every term without a test.
       d.coef[k] = c.state[k] >= 0 ? static_cast<float>(c.coef[k]) : 0.0f;
+      // gamma != 0 marks the slots the contact solves for (movable
+      // vertices), also for the colouring.
       d.gamma[k] = static_cast<float>(c.gamma[k]);
     }
     for (int j = 0; j < 3; ++j) {
@@ -849,16 +819,67 @@ void HipCollision::upload_contacts(int n_old) {
     d.mu = static_cast<float>(c.mu);
     d.id = i;
   }
-  c_dev_.reserve(std::max(1, n));
-  hip_check(hipMemcpyAsync(c_dev_.get(), hc, n * sizeof(ContactDev),
+  c_in_.reserve(std::max(1, n));
+  hip_check(hipMemcpyAsync(c_in_.get(), hc, n * sizeof(ContactDev),
                            hipMemcpyHostToDevice, st_),
             "contacts");
-  int* off = off_pin_.reserve(color_off_h_.size());
-  std::copy(color_off_h_.begin(), color_off_h_.end(), off);
-  color_off_.reserve(color_off_h_.size());
-  hip_check(hipMemcpyAsync(color_off_.get(), off, color_off_h_.size() * sizeof(int),
+  const int n_new = n - n_old;
+  constexpr int kBack = 3 + kMaxContactColors + 1;  // meta, then range ends
+  int* hcol = col_pin_.reserve(static_cast<size_t>(n_old) + kBack + n_new);
+  std::copy(color_.begin(), color_.begin() + n_old, hcol);
+  col_dev_.reserve(std::max(1, n));
+  if (n_old > 0)
+    hip_check(hipMemcpyAsync(col_dev_.get(), hcol, n_old * sizeof(int),
+                             hipMemcpyHostToDevice, st_),
+              "colours");
+  col_mask_.reserve(static_cast<size_t>(std::max(1, nact)) * kColorWords);
+  col_head_.reserve(std::max(1, nact));
+  c_dev_.reserve(std::max(1, n));
+  color_off_.reserve(kMaxContactColors + 1);
+  col_meta_.reserve(3);
+  ColorArgs ca;
+  ca.cin = c_in_.get();
+  ca.n_old = n_old;
+  ca.n = n;
+  ca.nact = nact;
+  ca.color = col_dev_.get();
+  ca.mask = col_mask_.get();
+  ca.head = col_head_.get();
+  ca.cout = c_dev_.get();
+  ca.off = color_off_.get();
+  ca.meta = col_meta_.get();
+  hip_check(launch_color_contacts(st_, ca), "contact colouring");
+  int* back = hcol + n_old;
+  hip_check(hipMemcpyAsync(back, col_meta_.get(), 3 * sizeof(int), hipMemcpyDeviceToHost,
+                           st_),
+            "colouring result");
+  hip_check(hipMemcpyAsync(back + 3, color_off_.get(), (kMaxContactColors + 1) * sizeof(int),
+                           hipMemcpyDeviceToHost, st_),
+            "colour ranges");
+  hip_check(hipMemcpyAsync(back + kBack, col_dev_.get() + n_old, n_new * sizeof(int),
+                           hipMemcpyDeviceToHost, st_),
+            "new colours");
+  // Staging buffers are not in flight after this: the weights below are
+  // uploaded after the synchronisation.
+  hip_check(hipStreamSynchronize(st_), "contact colouring");
+  if (back[2] != 0)
+    throw std::runtime_error("contact colouring needs more than " +
+                             std::to_string(kMaxContactColors) + " colours");
+  const int ncol = back[0];
+  max_color_ = ncol;
+  color_off_h_.assign(back + 3, back + 3 + ncol + 1);
+  color_.resize(n);
+  for (int c = n_old; c < n; ++c) {
+    const int col = back[kBack + (c - n_old)];
+    color_[c] = col;
+    const HostContact& h = contacts_[c];
+    int mask = 0;
+    for (int k = 0; k < 4; ++k) mask |= h.gamma[k] != 0.0 ? 1 << k : 0;
+    info_.push_back(col << 4 | mask);
+  }
+  hip_check(hipMemcpyAsync(w_.get(), hw, std::max(1, nvc_) * sizeof(float),
                            hipMemcpyHostToDevice, st_),
-            "colour offsets");
+            "contact weights");
   // Corrections persist across appends; new contacts start at zero.
   if (n > corr_cap_) {
     const int cap = std::max(1024, 2 * n);
@@ -909,10 +930,9 @@ void HipCollision::begin_step(const float* outer_x, float* ms, float* dhat,
   info_.clear();
   dcd_call_ = 0;
   color_off_h_.clear();
-  color_list_h_.clear();
   color_.clear();
   for (int v : touched_) {
-    vinc_[v].clear();
+    solved_[v] = 0;
     w_pin_.get()[v] = 0.0f;
   }
   touched_.clear();

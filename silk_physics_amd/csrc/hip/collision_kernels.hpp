@@ -244,6 +244,24 @@ struct ContactSet {
   int force_variant = -1;           // >= 0: use that variant (tests)
 };
 
+/// Device contact colouring (contact_coloring.cu:69-373): first-fit in
+/// contact order over shared movable vertices, colours 1..kMaxContactColors.
+constexpr int kColorWords = 8;
+constexpr int kMaxContactColors = 32 * kColorWords - 1;
+struct ColorArgs {
+  const ContactDev* cin = nullptr;  // n contacts in contact order
+  int n_old = 0;                    // contacts [0, n_old) keep their colours
+  int n = 0;
+  int nact = 0;                     // contact vertices (ContactDev::state range)
+  int* color = nullptr;             // n: [0, n_old) read, [n_old, n) written
+  unsigned* mask = nullptr;         // scratch: nact * kColorWords
+  int* head = nullptr;              // scratch: nact
+  ContactDev* cout = nullptr;       // n, colour by colour, id = contact index
+  int* off = nullptr;               // kMaxContactColors + 1 range ends
+  int* meta = nullptr;              // colours, rounds, overflow flag
+};
+hipError_t launch_color_contacts(hipStream_t s, const ColorArgs& a);
+
 /// One ADMM contact update (contact_constraints.cu:246-264): single
 /// workgroup, colours separated by workgroup barriers.  p and the
 /// corrections live in LDS for the whole update when they fit.

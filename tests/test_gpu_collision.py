@@ -6,6 +6,7 @@ import pytest
 
 import oracle as O
 import silk_physics_amd as sp
+from silk_physics_amd import scenes
 
 from helpers import (coll_tuple, gpu_world, inside_convex, lockstep_step, oracle_for,
                      self_intersections, sheet_on_sphere)
@@ -438,3 +439,57 @@ def test_device_bvh_reordering_keeps_the_trajectory(monkeypatch):
     print(f"re-ordered vs set-up order: {c1} / {c0} contacts over 40 steps")
     assert c1 > 0 and c1 == c0
     np.testing.assert_array_equal(x1, x0)
+
+
+def _first_fit(recs: np.ndarray, info: np.ndarray) -> np.ndarray:
+    """Sequential first-fit colouring of a step's contacts in contact order:
+    a contact takes the smallest colour >= 1 not held by an earlier contact
+    that shares one of its solved vertices (mask bits of info)."""
+    held: dict[int, set] = {}
+    out = np.zeros(len(recs), np.int64)
+    for c in range(len(recs)):
+        vs = [int(recs[c, 2 + k]) for k in range(4) if info[c] >> k & 1]
+        used = set().union(*(held.get(v, set()) for v in vs)) if vs else set()
+        col = 1
+        while col in used:
+            col += 1
+        out[c] = col
+        for v in vs:
+            held.setdefault(v, set()).add(col)
+    return out
+
+
+@pytest.mark.parametrize("where", ["lds", "global"])
+def test_device_colouring_is_first_fit_in_contact_order(where, monkeypatch):
+    """The device colouring (contact_coloring.cu:69-373 replaced by
+    first-fit Jones-Plassmann rounds with the contact index as priority) equals
+    the sequential first-fit colouring of the step's contacts in contact
+    order, call after call (earlier contacts keep their colours), so no two
+    contacts sharing a solved vertex share a colour; with the vertex masks in
+    LDS and in global memory (SILK_HIP_COLOR_GLOBAL)."""
+    if where == "global":
+        monkeypatch.setenv("SILK_HIP_COLOR_GLOBAL", "1")
+    s = scenes.c2_scene(40, z=0.305)
+    w = gpu_world(acceleration_z=-9.81, dt=1 / 60, max_inner_iteration=10)
+    cc = sp.CollisionConfig()
+    r, h = w.add_cloth(sp.ClothConfig(), cc, s["V"], s["F"], s["pins"])
+    assert r, r.to_string()
+    r, _ = w.add_obstacle(cc, s["SV"], s["SF"])
+    assert r, r.to_string()
+    total = calls = max_col = 0
+    for i in range(24):
+        r = w.solver_step()
+        assert r, f"step {i}: {r.to_string()}"
+        _, recs, _, info = w.decisions()
+        if len(recs) == 0:
+            continue
+        col = info >> 4
+        want = _first_fit(recs, info & 15)
+        assert np.array_equal(col, want), f"step {i}: {np.flatnonzero(col != want)[:8]}"
+        assert w.last_stats.contact_colors == col.max(), f"step {i}"
+        total += len(recs)
+        calls += len(np.unique(recs[:, 0]))
+        max_col = max(max_col, int(col.max()))
+    print(f"device colouring ({where}): {total} contacts over {calls} DCD calls, up to {max_col} "
+          f"colours, all first-fit in contact order")
+    assert total > 1000 and calls > 24
