@@ -223,6 +223,8 @@ class HipCollision final : public CollisionWorld {
   int pt_cap_ = 1 << 16, ee_cap_ = 1 << 16;
   DevBuf<DcdRec> recs_;
   int rec_cap_ = 1 << 12;
+  int rec_guess_ = 256;          // records copied back with the counters
+  PinnedVec<DcdRec> rec_pin_;
   DevBuf<int> counters_;  // pt, ee, dcd, pad (zeroed per broadphase)
   DevBuf<unsigned long long> toi_bits_;
   DevBuf<double> err6_;
@@ -670,6 +672,13 @@ std::vector<DcdRec> HipCollision::dcd(float time, float dhat,
     launch_dcd(st_, verts(), tables(), pt_pairs_.get(), counters_.get(), pt_cap_,
                ee_pairs_.get(), counters_.get() + 1, ee_cap_, time, dhat,
                recs_.get(), counters_.get() + 2, rec_cap_);
+    // The records expected from the last call's count travel with the
+    // counters, which saves a second round trip when they suffice.
+    const int guess = std::min(rec_cap_, rec_guess_);
+    DcdRec* hr = rec_pin_.reserve(static_cast<size_t>(std::max(1, guess)));
+    hip_check(hipMemcpyAsync(hr, recs_.get(), guess * sizeof(DcdRec), hipMemcpyDeviceToHost,
+                             st_),
+              "dcd records");
     read_io();
     ctx_.timed_collect();
     const int npt = h_io_[0], nee = h_io_[1], nrec = h_io_[2];
@@ -692,12 +701,16 @@ std::vector<DcdRec> HipCollision::dcd(float time, float dhat,
     if (again) continue;
     if (stats) stats->dcd_candidates += npt + nee;
     std::vector<DcdRec> out(nrec);
-    if (nrec > 0) {
-      hip_check(hipMemcpyAsync(out.data(), recs_.get(), nrec * sizeof(DcdRec),
+    std::copy(hr, hr + std::min(nrec, guess), out.begin());
+    if (nrec > guess) {
+      DcdRec* rest = rec_pin_.reserve(static_cast<size_t>(nrec - guess));
+      hip_check(hipMemcpyAsync(rest, recs_.get() + guess, (nrec - guess) * sizeof(DcdRec),
                                hipMemcpyDeviceToHost, st_),
                 "dcd records");
       hip_check(hipStreamSynchronize(st_), "dcd sync");
+      std::copy(rest, rest + (nrec - guess), out.begin() + guess);
     }
+    rec_guess_ = std::max(64, nrec + nrec / 4);
     return out;
   }
 }

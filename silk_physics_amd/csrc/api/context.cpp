@@ -1246,6 +1246,19 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
   hip_check(hipMemcpyAsync(S.h_admm_ctl + 1, S.ps_abort.get(), sizeof(int),
                            hipMemcpyDeviceToHost, stream_),
             "pcg abort flag");
+  // Read by admm_device_finish after the next synchronisation of the stream
+  // (the CCD's, when there is collision), so that the CCD is enqueued
+  // behind the ADMM loop without a host round trip between them.
+  admm_pending_ = true;
+  return true;
+}
+
+bool Context::admm_device_finish(silk_hip_step_stats* stats) {
+  if (!admm_pending_) return true;
+  admm_pending_ = false;
+  Scene& S = *scene_;
+  const auto& g = global_;
+  const bool verify = g.pcg_criterion == SILK_HIP_PCG_TRUE;
   hip_check(hipStreamSynchronize(stream_), "admm sync");
   timed_collect();
   const AdmmCtl& hc = S.h_admm_ctl[0];
@@ -1396,14 +1409,19 @@ int Context::step(silk_hip_step_stats* stats) {
   for (; outer < global_.max_outer_iteration; ++outer) {
     launch_predict(stream_, c, S.outer_x.get(), S.outer_v.get(),
                    S.inner_x.get());
-    if (!admm_solve(stats)) {
+    // The device ADMM loop is only enqueued here; its counters are read
+    // after the CCD's synchronisation (a diverged solve is reported then,
+    // and the CCD result on its positions discarded).
+    bool ok = admm_solve(stats);
+    float min_toi = remaining;
+    if (S.coll && ok)
+      min_toi = S.coll->min_toi(S.outer_x.get(), S.inner_x.get(),
+                                1.0f - remaining, remaining, ms, stats);
+    if (!admm_device_finish(stats) || !ok) {
       last_error_ = "solver diverged";
       return SILK_HIP_ERR_UNKNOWN;
     }
-    float min_toi = remaining;
     if (S.coll) {
-      min_toi = S.coll->min_toi(S.outer_x.get(), S.inner_x.get(),
-                                1.0f - remaining, remaining, ms, stats);
       toi_log_.push_back(min_toi);
       admm_log_.push_back(last_admm_iters_);
     }

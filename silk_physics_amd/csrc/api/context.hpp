@@ -235,6 +235,10 @@ class Context {
                  silk_hip_step_stats* st, bool restart = false);
   bool admm_solve(silk_hip_step_stats* st);
   bool admm_solve_device(silk_hip_step_stats* st);
+  // Reads the counters of the last enqueued device ADMM loop (synchronises
+  // the stream; a no-op when none is pending); false when it diverged.
+  bool admm_device_finish(silk_hip_step_stats* st);
+  bool admm_pending_ = false;
   void get_positions(const ClothEntity& c, float* out) const;
   /// silk_hip_autotune: times the broadphase split depths and the PCG launch
   /// variants on the current scene and keeps the fastest (report layout in
