@@ -337,6 +337,15 @@ def main() -> int:
     ap.add_argument("--batched", type=int, default=0,
                     help="instead of one sheet, this many independent sheets of "
                          "16..128 quads (the batched-instance scene)")
+    ap.add_argument("--breakdown-steps", type=int, default=20,
+                    help="steps after the timed region with every kernel class "
+                         "timed by events (the timed region times only the PCG "
+                         "solve kernel: events around every launch cost ~5 %%)")
+    ap.add_argument("--graph", action="store_true",
+                    help="capture each outer iteration's ADMM loop into a hipGraph "
+                         "(admm_graph); the timed region then records no kernel "
+                         "events, and the roofline takes the PCG kernel's time "
+                         "from the breakdown pass")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (the broadphase split "
                          "depth and the PCG variant are otherwise timed after "
@@ -376,9 +385,15 @@ def main() -> int:
     warm_s = time.perf_counter() - t0
     info = w.scene_info()
 
-    w.set_timing(True)
+    if args.graph:
+        assert w.set_global_config(sp.GlobalConfig(**global_kw(False, sp.PCG_TRUE, mas),
+                                                   admm_graph=sp.GRAPH_ON))
+    # Only the PCG solve kernel (class 1) is timed over the timed region, for
+    # the roofline; with the ADMM loop in a graph no class inside it is.
+    w.set_timing(0 if args.graph else 1 << 1)
     for c in range(9):
         w.timing(c, reset=True)
+    graph_launches = 0
     tot = dict(pcg=0, solves=0, restarts=0, admm=0, outer=0, contacts=0, ccd=0,
                unverified=0, stalled=0)
     step_s = []
@@ -398,18 +413,35 @@ def main() -> int:
         tot["ccd"] += st.ccd_candidates
         tot["unverified"] += st.pcg_unverified_solves
         tot["stalled"] += st.pcg_stalled_solves
+        graph_launches += st.graph_launches
     dt_wall = time.perf_counter() - t0
     w.set_timing(False)
-    kern = {}
-    for c, name in CLASSES.items():
-        ms, n = w.timing(c)
-        kern[c] = (ms, n)
+    pcg_timed = w.timing(1)
+    if args.graph:
+        assert w.set_global_config(sp.GlobalConfig(**global_kw(False, sp.PCG_TRUE, mas)))
 
-    # Roofline of the kernel with the most device time.
+    # Per-class device time over a short pass after the timed region, every
+    # class timed by events around its launches.
+    w.set_timing(True)
+    for c in range(9):
+        w.timing(c, reset=True)
+    bd_pcg = 0
+    for _ in range(args.breakdown_steps):
+        assert w.solver_step()
+        bd_pcg += w.last_stats.pcg_iterations
+    w.set_timing(False)
+    kern = {c: w.timing(c) for c in CLASSES}
+
+    # Roofline of the PCG solve kernel, the one with the most device time
+    # (breakdown pass), from its events over the timed region.
     half = mas != sp.MAS_FP32
     dom = max((c for c in kern if c in (1, 2, 6)), key=lambda c: kern[c][0])
-    ms, n = kern[dom]
-    bytes_total = byte_model(dom, info, tot["pcg"], n, half)
+    ms, n = pcg_timed
+    pcg_iters = tot["pcg"]
+    if dom != 1 or n == 0:
+        ms, n = kern[dom]
+        pcg_iters = bd_pcg
+    bytes_total = byte_model(dom, info, pcg_iters, n, half)
     achieved = bytes_total / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     roof = {"bound": "hbm", "kernel": CLASSES[dom],
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -418,17 +450,19 @@ def main() -> int:
             "bytes_per_launch": round(bytes_total / max(1, n), 1),
             "bytes_model": "reference formulation: 40 nnzb + 4 (V+1) + S n_blk + "
                            "120 V per PCG iteration"}
+    roof["timed_over"] = ("the timed region" if pcg_iters == tot["pcg"] else
+                          f"the {args.breakdown_steps}-step breakdown pass")
     if dom == 1 and ms > 0:
-        own = own_pcg_bytes(info, half) * tot["pcg"]
+        own = own_pcg_bytes(info, half) * pcg_iters
         roof["own_bytes_per_launch"] = round(own / max(1, n), 1)
         roof["own_achieved"] = round(own / (ms * 1e-3) / 1e9, 2)
         roof["own_frac"] = round(own / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
-        roof["us_per_iteration"] = round(1e3 * ms / max(1, tot["pcg"]), 3)
+        roof["us_per_iteration"] = round(1e3 * ms / max(1, pcg_iters), 3)
         tr = traffic_summary()
         if tr is not None and tr.get("vertices") == info["vertices"]:
             # Counter bytes per PCG iteration (probe run) times this run's
             # iterations per launch.
-            roof["traffic"] = round(tr["bytes_per_iteration"] * tot["pcg"] / max(1, n), 1)
+            roof["traffic"] = round(tr["bytes_per_iteration"] * pcg_iters / max(1, n), 1)
             roof["traffic_source"] = tr.get("source")
 
     # Verification: the FP64 residual of the stored FP32 x beside the
@@ -475,6 +509,8 @@ def main() -> int:
         "outer_iters_per_step": round(tot["outer"] / args.steps, 2),
         "mean_contacts": round(tot["contacts"] / args.steps, 1),
         "ccd_candidates_per_step": round(tot["ccd"] / args.steps, 1),
+        "admm_graph": bool(args.graph), "graph_launches": graph_launches,
+        "breakdown_steps": args.breakdown_steps,
         "kernel_ms": {CLASSES[c]: round(v[0], 3) for c, v in kern.items()},
         "kernel_launches": {CLASSES[c]: v[1] for c, v in kern.items()},
         "warmup_s": round(warm_s, 3),

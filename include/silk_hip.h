@@ -103,7 +103,16 @@ typedef struct {
                                      SILK_HIP_PROJECT_GALERKIN */
   int32_t ccd_zero_toi;           /* SILK_HIP_CCD_ZERO_SKIP (reference) or
                                      SILK_HIP_CCD_ZERO_CONTACT */
-  int32_t reserved[2];
+  int32_t admm_graph;             /* SILK_HIP_GRAPH_OFF (launches enqueued one
+                                     by one) or SILK_HIP_GRAPH_ON: each outer
+                                     iteration's ADMM loop is captured into a
+                                     hipGraph and launched as one graph (the
+                                     executable graph is updated in place
+                                     while its topology is unchanged); not
+                                     used while per-kernel timing is on or a
+                                     warm-start network or sample recorder is
+                                     active */
+  int32_t reserved;
 } silk_hip_global;
 
 enum {
@@ -152,6 +161,11 @@ enum {
 };
 
 enum {
+  SILK_HIP_GRAPH_OFF = 0,
+  SILK_HIP_GRAPH_ON = 1
+};
+
+enum {
   /* (c1, c2) minimise the FP64 residual norm ||b - A x||. */
   SILK_HIP_PROJECT_RESIDUAL = 0,
   /* (c1, c2) minimise the A-norm of the error (Galerkin). */
@@ -185,6 +199,9 @@ typedef struct {
   int32_t ccd_zero_toi_pairs; /* CCD candidates (summed over segments) whose
                                  last search still put a root at t = 0: no
                                  collision for their segment */
+  int32_t graph_launches;     /* ADMM loops launched as a hipGraph */
+  int32_t graph_instantiations; /* of which needed a new executable graph
+                                   (first capture or changed topology) */
 } silk_hip_step_stats;
 
 /* Number of visible HIP devices (0 when none).  Never fails. */
@@ -266,7 +283,7 @@ int silk_hip_get_partition(const silk_hip_ctx* ctx, uint32_t handle,
 
 /* Elapsed device time (ms) and launch count of a kernel class accumulated
  * since the last call with reset != 0.  Class 0 (whole step) is always
- * recorded; the others only while silk_hip_set_timing(ctx, 1) is on, from
+ * recorded; the others only while silk_hip_set_timing selects them, from
  * HIP events around each launch on the library stream: 1 = PCG solve kernel,
  * 2 = local projection, 3 = contact solve, 4 = broadphase, 5 = CCD,
  * 6 = FP64 residual; 7 = host wall time of contact assembly and colouring
@@ -275,7 +292,10 @@ int silk_hip_get_partition(const silk_hip_ctx* ctx, uint32_t handle,
 int silk_hip_get_timing(silk_hip_ctx* ctx, int32_t cls, double* ms,
                         int64_t* launches, int32_t reset);
 
-/* Turn per-kernel event timing on (1) or off (0). */
+/* Per-kernel event timing: 0 = off, 1 = every class, otherwise a mask of
+ * the classes to time (bit c = class c; e.g. 2 = the PCG solve kernel only).
+ * Events between the ADMM loop's launches keep that loop out of a hipGraph
+ * (admm_graph), so a mask without classes 1, 2, 3, 6 and 8 leaves it on. */
 int silk_hip_set_timing(silk_hip_ctx* ctx, int32_t on);
 
 /* Sizes of the device scene (builds it if needed), in this order:

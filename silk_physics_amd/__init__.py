@@ -111,6 +111,7 @@ PCG_RR0, PCG_RHS, PCG_TRUE = 0, 1, 2
 WARM_PREVIOUS, WARM_LEARNED, WARM_EXTRAPOLATE = 0, 1, 2
 PROJECT_RESIDUAL, PROJECT_GALERKIN = 0, 1
 CCD_ZERO_SKIP, CCD_ZERO_CONTACT = 0, 1
+GRAPH_OFF, GRAPH_ON = 0, 1
 WARM_NET_PARAMS = 5313  # SILK_HIP_WARM_NET_PARAMS
 AUTOTUNE_REPORT = 10  # SILK_HIP_AUTOTUNE_REPORT
 
@@ -141,7 +142,7 @@ class SilkHipGlobal(C.Structure):
                 ("mas_precision", C.c_int32), ("true_residual_stats", C.c_int32),
                 ("pcg_criterion", C.c_int32), ("warm_start", C.c_int32),
                 ("warm_projection", C.c_int32), ("ccd_zero_toi", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("admm_graph", C.c_int32), ("reserved", C.c_int32)]
 
 
 class SilkHipStepStats(C.Structure):
@@ -155,7 +156,8 @@ class SilkHipStepStats(C.Structure):
                 ("pcg_max_solve_iterations", C.c_int32),
                 ("pcg_stalled_solves", C.c_int32), ("warm_start_kept", C.c_int32),
                 ("warm_start_solves", C.c_int32), ("pcg_restarts", C.c_int32),
-                ("pcg_unverified_solves", C.c_int32), ("ccd_zero_toi_pairs", C.c_int32)]
+                ("pcg_unverified_solves", C.c_int32), ("ccd_zero_toi_pairs", C.c_int32),
+                ("graph_launches", C.c_int32), ("graph_instantiations", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -254,6 +256,7 @@ class GlobalConfig:
                                   # WARM_EXTRAPOLATE
     warm_projection: int = 0      # PROJECT_RESIDUAL / PROJECT_GALERKIN
     ccd_zero_toi: int = 0         # CCD_ZERO_SKIP (reference) / CCD_ZERO_CONTACT
+    admm_graph: int = 0           # GRAPH_OFF / GRAPH_ON (ADMM loop as a hipGraph)
 
     def _c(self) -> SilkHipGlobal:
         g = SilkHipGlobal()
@@ -267,7 +270,7 @@ class GlobalConfig:
                      "contact_sweeps", "ccd_max_iter", "ccd_tolerance",
                      "ccd_min_separation_scale", "dcd_activation_scale",
                      "mas_precision", "true_residual_stats", "pcg_criterion",
-                     "warm_start", "warm_projection", "ccd_zero_toi"):
+                     "warm_start", "warm_projection", "ccd_zero_toi", "admm_graph"):
             setattr(g, name, getattr(self, name))
         return g
 
@@ -464,7 +467,9 @@ class World:
                                         offsets.size, None)
         return self._wrap(st), perm, offsets
 
-    def set_timing(self, on: bool) -> Result:
+    def set_timing(self, on) -> Result:
+        """False/True: off/every class; an int other than 0 and 1 is a mask of
+        timing classes (bit c = class c)."""
         return self._wrap(load_library().silk_hip_set_timing(self._ctx, int(on)))
 
     SCENE_INFO_KEYS = ("vertices", "triangles", "offdiag_entries", "mas_banks",

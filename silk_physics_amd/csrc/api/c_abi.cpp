@@ -107,6 +107,8 @@ int silk_hip_set_global(silk_hip_ctx* ctx, const silk_hip_global* g) {
     else if (g->ccd_zero_toi != SILK_HIP_CCD_ZERO_SKIP &&
              g->ccd_zero_toi != SILK_HIP_CCD_ZERO_CONTACT)
       bad = "ccd_zero_toi must be SILK_HIP_CCD_ZERO_SKIP or SILK_HIP_CCD_ZERO_CONTACT";
+    else if (g->admm_graph != SILK_HIP_GRAPH_OFF && g->admm_graph != SILK_HIP_GRAPH_ON)
+      bad = "admm_graph must be SILK_HIP_GRAPH_OFF or SILK_HIP_GRAPH_ON";
     else if (g->warm_start == SILK_HIP_WARM_LEARNED && !c.has_warm_net())
       bad = "warm_start = SILK_HIP_WARM_LEARNED needs a network "
             "(silk_hip_set_warm_start_net)";
@@ -361,7 +363,9 @@ int silk_hip_get_timing(silk_hip_ctx* ctx, int32_t cls, double* ms,
 
 int silk_hip_set_timing(silk_hip_ctx* ctx, int32_t on) {
   return guarded(ctx, [&](Context& c) {
-    c.timing_on_ = on != 0;
+    if (on < 0) return int{SILK_HIP_ERR_INVALID_CONFIG};
+    // 1 times every class; any other value is a mask of classes (bit c).
+    c.timing_mask_ = on == 1 ? ~0u : static_cast<uint32_t>(on);
     return int{SILK_HIP_OK};
   });
 }

@@ -722,7 +722,7 @@ void HipCollision::append(std::vector<DcdRec> recs) {
     Context& c;
     std::chrono::steady_clock::time_point t0;
     ~HostTime() {
-      if (!c.timing_on_) return;
+      if (!c.timing(kTimeHostContacts)) return;
       c.timing_ms_[kTimeHostContacts] +=
           std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
               .count();

@@ -39,6 +39,7 @@ Scene::Scene() {
 }
 
 Scene::~Scene() {
+  if (admm_exec != nullptr) (void)hipGraphExecDestroy(admm_exec);
   if (h_admm_ctl != nullptr) (void)hipHostFree(h_admm_ctl);
 }
 
@@ -58,7 +59,8 @@ Context::Context(int device) : device_(device) {
 }
 
 void Context::timed_begin(int cls) {
-  if (!timing_on_) return;
+  timed_open_ = false;
+  if (!timing(cls)) return;
   const size_t k = ev_cls_.size();
   if (k >= ev_pool_.size()) {
     EvPair e;
@@ -68,10 +70,12 @@ void Context::timed_begin(int cls) {
   }
   ev_cls_.push_back(cls);
   hip_check(hipEventRecord(ev_pool_[k].a, stream_), "event");
+  timed_open_ = true;
 }
 
 void Context::timed_end() {
-  if (!timing_on_ || ev_cls_.empty()) return;
+  if (!timed_open_) return;
+  timed_open_ = false;
   hip_check(hipEventRecord(ev_pool_[ev_cls_.size() - 1].b, stream_), "event");
 }
 
@@ -1077,15 +1081,104 @@ int Context::finish_pcg(int iters, double rel, const RhsArgs& ra,
 // solve reads the counters back.
 bool Context::admm_solve_device(silk_hip_step_stats* stats) {
   Scene& S = *scene_;
+  const auto& g = global_;
+  // Learned warm start, or recording of training samples (from solves that
+  // start at the previous iterate); both need each solve's correction.
+  const bool learned = g.warm_start == SILK_HIP_WARM_LEARNED && has_warm_net();
+  const bool extrap = g.warm_start == SILK_HIP_WARM_EXTRAPOLATE;
+  const bool projected = learned || extrap;
+  const bool record = recording() && g.warm_start == SILK_HIP_WARM_PREVIOUS;
+  // The loop only enqueues kernels (no copies, fills or host waits), so it
+  // can be captured.  Per-kernel timing records events between the launches,
+  // and the warm-start paths advance host-side state per solve, so those run
+  // without the graph.
+  const int nit = g.max_inner_iteration;
+  const bool graph = g.admm_graph == SILK_HIP_GRAPH_ON && !timing_admm() && !projected &&
+                     !record && nit > 1;
+  if (!graph) {
+    admm_enqueue(projected, learned, record, 0, nit);
+  } else {
+    // The first iteration is enqueued eagerly, so the device works on it
+    // (contact solve, local step) while the host captures the rest: the
+    // capture would otherwise leave the device idle after every outer
+    // iteration's synchronisation.
+    admm_enqueue(projected, learned, record, 0, 1);
+    hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed),
+              "ADMM graph capture");
+    try {
+      admm_enqueue(projected, learned, record, 1, nit);
+    } catch (...) {
+      hipGraph_t gr = nullptr;
+      (void)hipStreamEndCapture(stream_, &gr);
+      if (gr != nullptr) (void)hipGraphDestroy(gr);
+      (void)hipGetLastError();
+      throw;
+    }
+    launch_admm_graph(stats);
+  }
+  AdmmCtl* ctl = S.admm_ctl.get();
+  hip_check(hipMemcpyAsync(S.h_admm_ctl, ctl, sizeof(AdmmCtl),
+                           hipMemcpyDeviceToHost, stream_),
+            "admm ctl");
+  hip_check(hipMemcpyAsync(S.h_admm_ctl + 1, S.ps_abort.get(), sizeof(int),
+                           hipMemcpyDeviceToHost, stream_),
+            "pcg abort flag");
+  // Read by admm_device_finish after the next synchronisation of the stream
+  // (the CCD's, when there is collision), so that the CCD is enqueued
+  // behind the ADMM loop without a host round trip between them.
+  admm_pending_ = true;
+  return true;
+}
+
+void Context::launch_admm_graph(silk_hip_step_stats* stats) {
+  Scene& S = *scene_;
+  hipGraph_t gr = nullptr;
+  hip_check(hipStreamEndCapture(stream_, &gr), "ADMM graph capture end");
+  bool fresh = false;
+  if (S.admm_exec != nullptr) {
+    // Kernel arguments (contact counts, colour ranges, the z/u buffers of
+    // the current parity) change between outer iterations; the update keeps
+    // the executable graph when only arguments and launch shapes differ.
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res = hipGraphExecUpdateSuccess;
+    if (hipGraphExecUpdate(S.admm_exec, gr, &bad, &res) != hipSuccess ||
+        res != hipGraphExecUpdateSuccess) {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(S.admm_exec);
+      S.admm_exec = nullptr;
+    }
+  }
+  if (S.admm_exec == nullptr) {
+    const hipError_t e = hipGraphInstantiate(&S.admm_exec, gr, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      S.admm_exec = nullptr;
+      (void)hipGraphDestroy(gr);
+      hip_check(e, "ADMM graph instantiate");
+    }
+    fresh = true;
+  }
+  (void)hipGraphDestroy(gr);
+  hip_check(hipGraphLaunch(S.admm_exec, stream_), "ADMM graph launch");
+  if (stats) {
+    stats->graph_launches += 1;
+    if (fresh) stats->graph_instantiations += 1;
+  }
+}
+
+void Context::admm_enqueue(bool projected, bool learned, bool record, int it0, int it1) {
+  Scene& S = *scene_;
   ClothDev c = cloth_dev();
   const size_t n3 = 3 * static_cast<size_t>(S.nv);
-  launch_fill_u32(stream_, S.pin_lambda.get(), static_cast<int64_t>(n3), 0u);
-  ContactView cv = S.coll ? S.coll->contact_view() : ContactView{};
   const auto& g = global_;
+  const bool first = it0 == 0;
+  const bool last = it1 == g.max_inner_iteration;
+  if (first) launch_fill_u32(stream_, S.pin_lambda.get(), static_cast<int64_t>(n3), 0u);
+  ContactView cv = S.coll ? S.coll->contact_view() : ContactView{};
   AdmmCtl* ctl = S.admm_ctl.get();
   const int* skip = &ctl->done;
-  launch_admm_init(stream_, ctl, g.initial_linear_rel_tol,
-                   g.linear_solver_rel_tol_min, g.linear_solver_rel_tol_max);
+  if (first)
+    launch_admm_init(stream_, ctl, g.initial_linear_rel_tol,
+                     g.linear_solver_rel_tol_min, g.linear_solver_rel_tol_max);
   RhsArgs ra;
   ra.c = c;
   ra.prev_x = S.prev_x.get();
@@ -1109,12 +1202,6 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
     pa.dxc = S.dxc.get();
     pa.max_replace = kPcgRestarts;
   }
-  // Learned warm start, or recording of training samples (from solves that
-  // start at the previous iterate); both need each solve's correction.
-  const bool learned = g.warm_start == SILK_HIP_WARM_LEARNED && has_warm_net();
-  const bool extrap = g.warm_start == SILK_HIP_WARM_EXTRAPOLATE;
-  const bool projected = learned || extrap;
-  const bool record = recording() && g.warm_start == SILK_HIP_WARM_PREVIOUS;
   if (projected || record) {
     pa.dx_out = S.dx.get();
     const size_t ns = n3;
@@ -1130,7 +1217,7 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
       S.w_coef.resize(6 * static_cast<size_t>(std::max(1, S.nsys)));
     }
     // The first solve of a segment has no previous correction.
-    launch_fill_u32(stream_, S.dx.get(), static_cast<int64_t>(n3), 0u);
+    if (first) launch_fill_u32(stream_, S.dx.get(), static_cast<int64_t>(n3), 0u);
   }
   WarmFeatArgs wf;
   wf.r = S.r.get();
@@ -1151,7 +1238,7 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
   ca.lin_min = g.linear_solver_rel_tol_min;
   ca.lin_max = g.linear_solver_rel_tol_max;
   const bool keep_x0 = g.true_residual_stats != 0;
-  for (int it = 0; it < g.max_inner_iteration; ++it) {
+  for (int it = it0; it < it1; ++it) {
     // Without a projected start the residual kernel also keeps the solve's
     // start x for the true-residual test.
     const bool fold_copy = keep_x0 && it != 0 && !projected;
@@ -1239,18 +1326,7 @@ bool Context::admm_solve_device(silk_hip_step_stats* stats) {
     ca.it = it;
     launch_admm_check(stream_, S.admm_part.get(), ca, ctl);
   }
-  launch_enforce(stream_, c, cv.w, cv.p, S.inner_x.get());
-  hip_check(hipMemcpyAsync(S.h_admm_ctl, ctl, sizeof(AdmmCtl),
-                           hipMemcpyDeviceToHost, stream_),
-            "admm ctl");
-  hip_check(hipMemcpyAsync(S.h_admm_ctl + 1, S.ps_abort.get(), sizeof(int),
-                           hipMemcpyDeviceToHost, stream_),
-            "pcg abort flag");
-  // Read by admm_device_finish after the next synchronisation of the stream
-  // (the CCD's, when there is collision), so that the CCD is enqueued
-  // behind the ADMM loop without a host round trip between them.
-  admm_pending_ = true;
-  return true;
+  if (last) launch_enforce(stream_, c, cv.w, cv.p, S.inner_x.get());
 }
 
 bool Context::admm_device_finish(silk_hip_step_stats* stats) {

@@ -137,6 +137,11 @@ struct Scene {
   DevBuf<PcgState> ps_out;
   DevBuf<AdmmCtl> admm_ctl;
   AdmmCtl* h_admm_ctl = nullptr;  // pinned: AdmmCtl, then the abort flag
+  // Executable graph of the device ADMM loop (admm_graph on): captured anew
+  // for every outer iteration and updated in place while the captured
+  // topology is unchanged; it holds this scene's buffer addresses, so it
+  // lives and dies with the scene.
+  hipGraphExec_t admm_exec = nullptr;
   Scene();
   ~Scene();
   // ---- collision
@@ -199,8 +204,16 @@ class Context {
   double timing_ms_[kTimeClasses] = {};
   int64_t timing_launch_[kTimeClasses] = {};
   // Per-kernel event timing (off unless enabled): pairs of events recorded
-  // around launches on stream_, resolved after the next synchronisation.
-  bool timing_on_ = false;
+  // around launches on stream_ of the classes whose bit is set, resolved
+  // after the next synchronisation.
+  uint32_t timing_mask_ = 0;
+  bool timed_open_ = false;  // the last timed_begin recorded its event
+  bool timing(int cls) const { return ((timing_mask_ >> cls) & 1u) != 0; }
+  // A class of the ADMM loop is timed (events between its launches).
+  bool timing_admm() const {
+    return timing(kTimePcg) || timing(kTimeLocal) || timing(kTimeContact) ||
+           timing(kTimeResidual) || timing(kTimeWarm);
+  }
   struct EvPair {
     hipEvent_t a = nullptr, b = nullptr;
   };
@@ -235,10 +248,17 @@ class Context {
                  silk_hip_step_stats* st, bool restart = false);
   bool admm_solve(silk_hip_step_stats* st);
   bool admm_solve_device(silk_hip_step_stats* st);
+  // Enqueues iterations [it0, it1) of one outer iteration's device ADMM loop
+  // on stream_ (kernels only), with the loop's set-up when it0 == 0 and its
+  // final pin enforcement when it1 is the last.
+  void admm_enqueue(bool projected, bool learned, bool record, int it0, int it1);
   // Reads the counters of the last enqueued device ADMM loop (synchronises
   // the stream; a no-op when none is pending); false when it diverged.
   bool admm_device_finish(silk_hip_step_stats* st);
   bool admm_pending_ = false;
+  // Ends the stream capture of the ADMM loop, updates (or instantiates) the
+  // scene's executable graph from it and launches that on stream_.
+  void launch_admm_graph(silk_hip_step_stats* st);
   void get_positions(const ClothEntity& c, float* out) const;
   /// silk_hip_autotune: times the broadphase split depths and the PCG launch
   /// variants on the current scene and keeps the fastest (report layout in

@@ -138,14 +138,15 @@ def test_world_api_host_contract():
 
 
 def test_set_global_rejects_unknown_solver_modes(ctx):
-    """mas_precision, pcg_criterion and warm_start accept only their declared
-    values; a learned warm start needs a loaded network."""
+    """mas_precision, pcg_criterion, warm_start, ccd_zero_toi and admm_graph
+    accept only their declared values; a learned warm start needs a loaded network."""
     lib, h = ctx
     ok = sp.GlobalConfig()._c()
     assert lib.silk_hip_set_global(h, C.byref(ok)) == 0
     for field, value in [("mas_precision", 3), ("mas_precision", -1),
                          ("pcg_criterion", 3), ("warm_start", 5), ("warm_projection", 2),
                          ("ccd_zero_toi", 2), ("ccd_zero_toi", -1),
+                         ("admm_graph", 2), ("admm_graph", -1),
                          ("warm_start", sp.WARM_LEARNED)]:
         g = sp.GlobalConfig()._c()
         setattr(g, field, value)
@@ -155,7 +156,8 @@ def test_set_global_rejects_unknown_solver_modes(ctx):
                          ("pcg_criterion", sp.PCG_RHS), ("pcg_criterion", sp.PCG_TRUE),
                          ("warm_start", sp.WARM_EXTRAPOLATE),
                          ("warm_projection", sp.PROJECT_GALERKIN),
-                         ("ccd_zero_toi", sp.CCD_ZERO_CONTACT)]:
+                         ("ccd_zero_toi", sp.CCD_ZERO_CONTACT),
+                         ("admm_graph", sp.GRAPH_ON)]:
         g = sp.GlobalConfig()._c()
         setattr(g, field, value)
         assert lib.silk_hip_set_global(h, C.byref(g)) == 0, field
