@@ -212,15 +212,30 @@ def cpu_baseline(n: int, budget_s: float, gpu_step_s: list) -> dict:
     k, el = out["all"]
     k1, el1 = out["one"]
     gpu_same = k / max(1e-9, sum(gpu_step_s[:k]))
+    # The library's own CPU backend (Backend.CPU) on the same first k steps.
+    s = scenes.c2_scene(n)
+    wc = sp.World()
+    assert wc.init(sp.Backend.CPU)
+    assert wc.set_global_config(sp.GlobalConfig(acceleration_z=-9.81, dt=1 / 60,
+                                                max_inner_iteration=10))
+    cc = sp.CollisionConfig()
+    assert wc.add_cloth(sp.ClothConfig(), cc, s["V"], s["F"], s["pins"])[0]
+    assert wc.add_obstacle(cc, s["SV"], s["SF"])[0]
+    t0 = time.perf_counter()
+    for _ in range(k):
+        assert wc.solver_step()
+    el_lib = time.perf_counter() - t0
     return {"value": round(k / el, 4), "unit": "steps/s", "cores": hc["nproc"],
             "kind": "port", "one_thread": round(k1 / el1, 4),
+            "library_cpu_backend": round(k / el_lib, 4),
             "host": hc["model"],
             "sample": f"steps 1..{k} of the same 80x80 drop from rest with the "
                       f"reference CPU algorithm restated (PD, envelope Cholesky, "
                       f"SVD projection, barrier targets, FP32 CCD) on "
                       f"{hc['nproc']} threads, {el:.1f} s; one thread: {k1} steps "
-                      f"in {el1:.1f} s; the GPU ran the same first {k} steps at "
-                      f"{gpu_same:.2f} steps/s"}
+                      f"in {el1:.1f} s; the library's CPU backend (Backend.CPU, "
+                      f"its own threads) ran them in {el_lib:.1f} s; the GPU ran "
+                      f"the same first {k} steps at {gpu_same:.2f} steps/s"}
 
 
 def traffic_summary() -> dict | None:

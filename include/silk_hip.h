@@ -48,6 +48,9 @@ enum {
 
 typedef struct silk_hip_ctx silk_hip_ctx;
 
+/* silk_hip_create's device number of the CPU backend. */
+#define SILK_HIP_DEVICE_CPU (-1)
+
 /* silk::ClothConfig (silk/include/silk/silk.hpp:37-47). */
 typedef struct {
   float elastic_stiffness;
@@ -209,7 +212,16 @@ int silk_hip_device_count(int32_t* count);
 
 /* Create a context bound to `device`.  No device call is made until the first
  * silk_hip_step; a context without a usable device reports
- * SILK_HIP_ERR_NO_DEVICE from silk_hip_step. */
+ * SILK_HIP_ERR_NO_DEVICE from silk_hip_step.
+ *
+ * device == SILK_HIP_DEVICE_CPU creates a context of the CPU backend
+ * (silk::Backend::CPU, cpu_backend.cpp:40-45 / pipeline.cpp:45-168): the same
+ * scene calls, stepped by the reference's CPU algorithm (projective dynamics
+ * with a prefactored global system, barrier reflection targets, CCD
+ * rollback) on the host's cores (SILK_CPU_THREADS in the environment sets
+ * their number).  Device-only entry points (timing, autotuning, debug
+ * kernels, warm-start network) return an error on such a context.  In its
+ * step statistics admm_iterations counts the PD iterations. */
 int silk_hip_create(int32_t device, silk_hip_ctx** out);
 int silk_hip_destroy(silk_hip_ctx* ctx);
 

@@ -112,6 +112,7 @@ WARM_PREVIOUS, WARM_LEARNED, WARM_EXTRAPOLATE = 0, 1, 2
 PROJECT_RESIDUAL, PROJECT_GALERKIN = 0, 1
 CCD_ZERO_SKIP, CCD_ZERO_CONTACT = 0, 1
 GRAPH_OFF, GRAPH_ON = 0, 1
+DEVICE_CPU = -1  # silk_hip_create device of the CPU backend
 WARM_NET_PARAMS = 5313  # SILK_HIP_WARM_NET_PARAMS
 AUTOTUNE_REPORT = 10  # SILK_HIP_AUTOTUNE_REPORT
 
@@ -297,8 +298,8 @@ class World:
     def init(self, backend: Backend = Backend.GPU) -> Result:
         lib = load_library()
         if backend == Backend.CPU:
-            return Result(False, ErrorCode.InvalidConfig,
-                          "the CPU backend is not part of this library")
+            # The reference's CPU solver on the host's cores (same C ABI).
+            return self.init_context(DEVICE_CPU)
         n = C.c_int32(0)
         lib.silk_hip_device_count(C.byref(n))
         if n.value <= 0:

@@ -39,7 +39,7 @@ HOST_FLAGS = ["-O3", "-std=c++20", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}"
 
 HIP_SOURCES = sorted((CSRC / "hip").glob("*.hip"))
 RUNTIME_SOURCES = sorted((CSRC / "api").glob("*.cpp"))
-HOST_SOURCES = sorted((CSRC / "host").glob("*.cpp"))
+HOST_SOURCES = sorted((CSRC / "host").glob("*.cpp")) + sorted((CSRC / "cpu").glob("*.cpp"))
 
 
 def _run(cmd: list[str]) -> None:
@@ -75,7 +75,7 @@ def build(verbose: bool = False) -> dict[str, Path]:
     with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, sources))
     hip_so = LIB / "libsilk_hip.so"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o",
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o",
           str(hip_so), *map(str, objs)])
     world_so = LIB / "libsilk.so"
     _run([CXX, *HOST_FLAGS, "-shared", "-o", str(world_so),

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "api/context.hpp"
+#include "cpu/cpu_world.hpp"
 #include "hip/collision_kernels.hpp"
 #include "silk_hip.h"
 
@@ -291,6 +292,7 @@ int silk_hip_step(silk_hip_ctx* ctx, silk_hip_step_stats* stats) {
 
 int silk_hip_reset(silk_hip_ctx* ctx) {
   return guarded(ctx, [&](Context& c) {
+    c.cpu_world_.reset();
     c.scene_.reset();
     c.scene_dirty_ = true;
     for (auto& e : c.cloths_) {

@@ -1,6 +1,8 @@
 // Runtime context: registry, device scene assembly and the PD step.
 #include "api/context.hpp"
 
+#include "cpu/cpu_world.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -43,7 +45,7 @@ Scene::~Scene() {
   if (h_admm_ctl != nullptr) (void)hipHostFree(h_admm_ctl);
 }
 
-Context::Context(int device) : device_(device) {
+Context::Context(int device) : device_(device), cpu_(device == SILK_HIP_DEVICE_CPU) {
   global_.acceleration[0] = 0.0f;
   global_.acceleration[1] = 0.0f;
   global_.acceleration[2] = -10.0f;
@@ -125,6 +127,7 @@ ObstacleEntity* Context::find_obstacle(uint32_t h) {
 
 void Context::ensure_device() {
   if (dev_ready_) return;
+  if (cpu_) throw std::runtime_error("not available on the CPU backend");
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= device_ || device_ < 0) {
     (void)hipGetLastError();
@@ -138,6 +141,14 @@ void Context::ensure_device() {
 }
 
 void Context::invalidate_scene() {
+  if (cpu_world_) {
+    for (size_t ci = 0; ci < cloths_.size(); ++ci) {
+      cloths_[ci].x = cpu_world_->x(static_cast<int>(ci));
+      cloths_[ci].v = cpu_world_->v(static_cast<int>(ci));
+      cloths_[ci].has_state = true;
+    }
+    cpu_world_.reset();
+  }
   if (scene_ && dev_ready_) {
     // Persist cloth states (solver order on device -> caller order on host).
     std::vector<float> hx, hv;
@@ -186,6 +197,74 @@ void Context::ensure_cloth_derived(ClothEntity& c) {
     c.v.assign(c.mesh.V.size(), 0.0f);
     c.has_state = true;
   }
+}
+
+void Context::build_cpu_world() {
+  invalidate_scene();
+  int threads = 0;
+  if (const char* e = std::getenv("SILK_CPU_THREADS")) threads = std::atoi(e);
+  auto w = std::make_unique<cpu::CpuWorld>(global_, threads);
+  for (ClothEntity& c : cloths_) {
+    cpu::ClothInput in;
+    in.mesh = &c.mesh;
+    in.cloth = c.cloth;
+    in.coll = c.coll;
+    in.pins = c.pins;
+    if (c.has_pin_pos) {
+      in.pin_pos = c.pin_curr;
+    } else {
+      for (int p : c.pins)
+        for (int d = 0; d < 3; ++d) in.pin_pos.push_back(c.mesh.V[3 * p + d]);
+    }
+    in.x = c.has_state ? c.x : c.mesh.V;
+    in.v = c.has_state ? c.v : std::vector<float>(c.mesh.V.size(), 0.0f);
+    w->add_cloth(in);
+  }
+  for (ObstacleEntity& o : obstacles_) {
+    cpu::ObstacleInput in;
+    in.mesh = &o.mesh;
+    in.coll = o.coll;
+    w->add_obstacle(in);
+  }
+  cpu_world_ = std::move(w);
+  cpu_global_ = global_;
+  scene_dirty_ = false;
+}
+
+int Context::cpu_step(silk_hip_step_stats* stats) {
+  if (!cpu_world_ || scene_dirty_ ||
+      std::memcmp(&cpu_global_, &global_, sizeof(global_)) != 0) {
+    try {
+      build_cpu_world();
+    } catch (const std::runtime_error& e) {
+      last_error_ = e.what();
+      return SILK_HIP_ERR_CHOLESKY_FAIL;
+    }
+  }
+  for (size_t i = 0; i < obstacles_.size(); ++i) {
+    const ObstacleEntity& o = obstacles_[i];
+    const std::vector<float>& prev = o.has_pos ? o.prev : o.mesh.V;
+    const std::vector<float>& curr = o.has_pos ? o.curr : o.mesh.V;
+    cpu_world_->set_obstacle_frame(static_cast<int>(i), prev.data(), curr.data());
+  }
+  cpu::StepStats st;
+  if (!cpu_world_->step(&st)) {
+    last_error_ = "solver diverged";
+    return SILK_HIP_ERR_UNKNOWN;
+  }
+  for (auto& o : obstacles_) {
+    if (o.has_pos) o.prev = o.curr;
+    if (o.is_static) o.is_static_twice = true;
+    else o.is_static = true;
+  }
+  if (stats) {
+    stats->outer_iterations = st.outer;
+    stats->admm_iterations = st.inner;
+    stats->contacts = st.collisions;
+    stats->ccd_candidates = st.ccd_candidates;
+    stats->min_toi = st.min_toi;
+  }
+  return SILK_HIP_OK;
 }
 
 void Context::build_scene() {
@@ -1461,6 +1540,7 @@ int Context::step(silk_hip_step_stats* stats) {
     last_error_ = "nothing to solve";
     return SILK_HIP_ERR_UNKNOWN;
   }
+  if (cpu_) return cpu_step(stats);
   if (scene_ && scene_->dt != global_.dt) invalidate_scene();
   if (scene_dirty_ || !scene_) build_scene();
   Scene& S = *scene_;
@@ -1684,6 +1764,10 @@ int Context::autotune(int32_t* report) {
 }
 
 void Context::update_pin_targets(const ClothEntity& c) {
+  if (cpu_world_) {
+    cpu_world_->set_pin_targets(static_cast<int>(&c - cloths_.data()), c.pin_curr.data());
+    return;
+  }
   if (!scene_) return;
   const size_t ci = static_cast<size_t>(&c - cloths_.data());
   const int off = scene_->vert_off[ci];
@@ -1701,6 +1785,11 @@ void Context::update_pin_targets(const ClothEntity& c) {
 
 void Context::get_positions(const ClothEntity& c, float* out) const {
   const int nv = c.mesh.nv();
+  if (cpu_world_) {
+    const std::vector<float>& x = cpu_world_->x(static_cast<int>(&c - cloths_.data()));
+    std::memcpy(out, x.data(), x.size() * sizeof(float));
+    return;
+  }
   if (scene_) {
     size_t ci = static_cast<size_t>(&c - cloths_.data());
     const int off = scene_->vert_off[ci];

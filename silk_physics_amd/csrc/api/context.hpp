@@ -148,12 +148,23 @@ struct Scene {
   std::unique_ptr<CollisionWorld> coll;
 };
 
+namespace cpu {
+class CpuWorld;
+}
+
 class Context {
  public:
   explicit Context(int device);
   ~Context();
 
   int device_ = 0;
+  // The CPU backend (device SILK_HIP_DEVICE_CPU): the scene is stepped by
+  // cpu::CpuWorld, built from the entities at the first step after a change.
+  bool cpu_ = false;
+  std::unique_ptr<cpu::CpuWorld> cpu_world_;
+  silk_hip_global cpu_global_{};  // the configuration cpu_world_ was built with
+  void build_cpu_world();
+  int cpu_step(silk_hip_step_stats* stats);
   std::string last_error_;
   silk_hip_global global_{};
 

@@ -2,9 +2,10 @@
 //
 // World keeps the reference pimpl + backend-interface shape
 // (silk/src/common/world.cpp:15-166, common/backend.hpp:9-45): every call
-// checks that a backend exists, then forwards.  The only backend here is
+// checks that a backend exists, then forwards.  Both backends go through
 // HipBackend, a thin mapping from std::span arguments to the C ABI and from
-// C status codes back to silk::ErrorCode.
+// C status codes back to silk::ErrorCode; Backend::CPU opens a context of
+// the library's CPU solver (device SILK_HIP_DEVICE_CPU).
 #include <cstdio>
 #include <memory>
 #include <span>
@@ -114,9 +115,16 @@ World& World::operator=(World&&) = default;
 
 Result World::init(Backend backend) {
   switch (backend) {
-    case Backend::CPU:
-      return Result::error(ErrorCode::InvalidConfig,
-                           "the CPU backend is not part of this library");
+    case Backend::CPU: {
+      // The reference's CPU solver on the host's cores, behind the same C
+      // ABI (cpu_backend.cpp:40-45).
+      silk_hip_ctx* ctx = nullptr;
+      if (silk_hip_create(SILK_HIP_DEVICE_CPU, &ctx) != SILK_HIP_OK) {
+        return Result::error(ErrorCode::Unknown, "context creation failed");
+      }
+      impl_->backend = std::make_unique<HipBackend>(ctx);
+      return Result::ok();
+    }
     case Backend::GPU: {
       int32_t n = 0;
       silk_hip_device_count(&n);
