@@ -102,3 +102,41 @@ def test_trajectory_reader(tmp_path):
     assert tr.objects[0].cloth and not tr.objects[1].moving
     np.testing.assert_array_equal(tr.objects[0].frames[1], rest + 1)
     np.testing.assert_array_equal(tr.objects[1].faces, faces)
+
+
+def test_headless_cpu_backend_trajectory(tmp_path):
+    """--backend cpu steps a scene file on the library's CPU solver; the same
+    scene through the Python World on Backend.CPU gives every frame bit for
+    bit, and no cloth vertex enters the sphere."""
+    V, F = scenes.grid_sheet(10, z=0.36)
+    SV, SF = scenes.icosphere(2, 0.3, (0.5, 0.5, 0.0))
+    write_obj(tmp_path / "sheet.obj", V, F)
+    write_obj(tmp_path / "ball.obj", SV, SF)
+    coll = {"enabled": True, "self_collision": True, "group": 1}
+    scene_json(tmp_path / "scene.json", [
+        {"type": "cloth", "name": "sheet", "mesh": "sheet.obj", "collision": coll},
+        {"type": "obstacle", "name": "ball", "mesh": "ball.obj", "collision": coll}],
+        total_steps=12)
+    out_file = tmp_path / "out.silktraj"
+    r = subprocess.run([EXE, "--config", str(tmp_path / "scene.json"), "--backend", "cpu",
+                        "--output", str(out_file), "--quiet"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    tr = trajectory.read(str(out_file))
+    sheet = tr.objects[0]
+    assert len(sheet.frames) == 13
+    w = sp.World()
+    assert w.init(sp.Backend.CPU)
+    assert w.set_global_config(sp.GlobalConfig(acceleration_z=-9.81, dt=1 / 60,
+                                               max_outer_iteration=100,
+                                               max_inner_iteration=10))
+    cc = sp.CollisionConfig(is_collision_on=True, is_self_collision_on=True, group=1)
+    rr, h = w.add_cloth(sp.ClothConfig(), cc, sheet.rest, F, [])
+    assert rr, rr.to_string()
+    assert w.add_obstacle(cc, tr.objects[1].rest, SF)[0]
+    for k in range(12):
+        assert w.solver_step()
+        _, x = w.get_cloth_position(h, n=V.size)
+        np.testing.assert_array_equal(x.reshape(-1, 3), sheet.frames[k + 1], err_msg=str(k))
+    from helpers import inside_convex
+    assert inside_convex(sheet.frames[-1], SV, SF).min() > 0.0
