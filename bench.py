@@ -60,8 +60,9 @@ def global_kw(true_residual: bool = False, criterion: int = sp.PCG_TRUE,
                 mas_precision=mas)
 
 
-def build_world(n: int, mas: int = sp.MAS_FP32, batched: int = 0):
-    """The C2-style drop of an n x n sheet, or with batched > 0 that many
+def build_world(n: int, mas: int = sp.MAS_FP32, batched: int = 0, jitter_mm: float = 0.0):
+    """The C2-style drop of an n x n sheet (its vertices moved by a seeded
+    uniform +-jitter_mm in every coordinate), or with batched > 0 that many
     independent sheets of 16..128 quads (each over its own sphere, in its own
     collision group).  Returns (scene of the first sheet, world, handle)."""
     w = sp.World()
@@ -70,6 +71,11 @@ def build_world(n: int, mas: int = sp.MAS_FP32, batched: int = 0):
         raise SystemExit(f"GPU init failed: {r.to_string()}")
     assert w.set_global_config(sp.GlobalConfig(**global_kw(mas=mas)))
     sheets = scenes.c4_scene(batched) if batched else [scenes.c2_scene(n)]
+    if jitter_mm > 0 and not batched:
+        rng = np.random.default_rng(0x5117)
+        V = sheets[0]["V"]
+        sheets[0]["V"] = (V + rng.uniform(-1e-3 * jitter_mm, 1e-3 * jitter_mm,
+                                          size=V.shape)).astype(np.float32)
     handles = []
     for s in sheets:
         cc = sp.CollisionConfig(group=s.get("group", 0))  # on, self on, mu 0.3
@@ -349,6 +355,9 @@ def main() -> int:
                     help="steps after the timed region with the FP64 residual of "
                          "the stored x reported, then with the reference's "
                          "stopping rule for its iteration count")
+    ap.add_argument("--jitter", type=float, default=0.0,
+                    help="seeded uniform vertex jitter of the sheet in mm (the "
+                         "mixed-precision workload uses 1 mm at n = 256)")
     ap.add_argument("--batched", type=int, default=0,
                     help="instead of one sheet, this many independent sheets of "
                          "16..128 quads (the batched-instance scene)")
@@ -386,7 +395,7 @@ def main() -> int:
         return compare_c5(args)
     mas = {"fp32": sp.MAS_FP32, "fp16": sp.MAS_FP16, "bf16": sp.MAS_BF16}[args.mas]
 
-    s, w, h = build_world(args.n, mas, args.batched)
+    s, w, h = build_world(args.n, mas, args.batched, args.jitter)
     if args.batched:
         args.no_cpu_baseline = True
     t0 = time.perf_counter()
@@ -506,8 +515,9 @@ def main() -> int:
         "config": {"workload": (f"{args.batched} independent sheets of 16..128 quads "
                                 "(seeded) 2 cm over their own 642-vertex spheres"
                                 if args.batched else
-                                f"{args.n}x{args.n}-quad sheet dropped 0.5 m on a "
-                                "2562-vertex icosphere") +
+                                f"{args.n}x{args.n}-quad sheet"
+                                + (f" (+-{args.jitter:g} mm vertex jitter)" if args.jitter else "")
+                                + " dropped 0.5 m on a 2562-vertex icosphere") +
                                ", collision and self-collision on, dt 1/60, 10 "
                                "ADMM iters, CG to ||b - A x|| <= 1e-6 ||b|| (FP64)",
                    "mas_precision": args.mas, **info},
