@@ -465,7 +465,13 @@ def main() -> int:
     if dom != 1 or n == 0:
         ms, n = kern[dom]
         pcg_iters = bd_pcg
-    bytes_total = byte_model(dom, info, pcg_iters, n, half)
+    # With several systems per launch the iteration total sums every
+    # system's iterations, while the byte model covers all systems at once:
+    # the model is applied to the iterations per system (an average over
+    # systems of uneven size).
+    nsys = max(1, info.get("systems", 1))
+    pcg_iters_model = pcg_iters / nsys
+    bytes_total = byte_model(dom, info, pcg_iters_model, n, half)
     achieved = bytes_total / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     roof = {"bound": "hbm", "kernel": CLASSES[dom],
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -477,16 +483,16 @@ def main() -> int:
     roof["timed_over"] = ("the timed region" if pcg_iters == tot["pcg"] else
                           f"the {args.breakdown_steps}-step breakdown pass")
     if dom == 1 and ms > 0:
-        own = own_pcg_bytes(info, half) * pcg_iters
+        own = own_pcg_bytes(info, half) * pcg_iters_model
         roof["own_bytes_per_launch"] = round(own / max(1, n), 1)
         roof["own_achieved"] = round(own / (ms * 1e-3) / 1e9, 2)
         roof["own_frac"] = round(own / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
-        roof["us_per_iteration"] = round(1e3 * ms / max(1, pcg_iters), 3)
+        roof["us_per_iteration"] = round(1e3 * ms / max(1e-9, pcg_iters_model), 3)
         tr = traffic_summary()
         if tr is not None and tr.get("vertices") == info["vertices"]:
             # Counter bytes per PCG iteration (probe run) times this run's
             # iterations per launch.
-            roof["traffic"] = round(tr["bytes_per_iteration"] * pcg_iters / max(1, n), 1)
+            roof["traffic"] = round(tr["bytes_per_iteration"] * pcg_iters_model / max(1, n), 1)
             roof["traffic_source"] = tr.get("source")
 
     # Verification: the FP64 residual of the stored FP32 x beside the

@@ -205,6 +205,10 @@ typedef struct {
   int32_t graph_launches;     /* ADMM loops launched as a hipGraph */
   int32_t graph_instantiations; /* of which needed a new executable graph
                                    (first capture or changed topology) */
+  int32_t contact_groups;     /* workgroups of the contact update when the
+                                 contacts split into independent groups
+                                 (cloths no contact couples), one group
+                                 each; 0 when one workgroup updates all */
 } silk_hip_step_stats;
 
 /* Number of visible HIP devices (0 when none).  Never fails. */
@@ -316,8 +320,9 @@ int silk_hip_set_timing(silk_hip_ctx* ctx, int32_t on);
  * workgroups over all systems (0 when the multi-launch PCG is used), 6
  * coarse MAS banks, 7 collider vertices, 8 persistent PCG launches per solve
  * (batches of systems whose workgroups fit the device together), 9 PCG
- * systems (cloths).  n: capacity of info (entries beyond it are not
- * written). */
+ * systems (cloths), 10 LDS bytes per workgroup of the persistent PCG (also
+ * when it does not fit the device), 11 its variant (0: one workgroup per
+ * CU, 1: two).  n: capacity of info (entries beyond it are not written). */
 int silk_hip_get_scene_info(silk_hip_ctx* ctx, int64_t* info, int32_t n);
 
 /* Collider vertex count of the current scene: cloth vertices (solver order,

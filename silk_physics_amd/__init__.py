@@ -158,7 +158,8 @@ class SilkHipStepStats(C.Structure):
                 ("pcg_stalled_solves", C.c_int32), ("warm_start_kept", C.c_int32),
                 ("warm_start_solves", C.c_int32), ("pcg_restarts", C.c_int32),
                 ("pcg_unverified_solves", C.c_int32), ("ccd_zero_toi_pairs", C.c_int32),
-                ("graph_launches", C.c_int32), ("graph_instantiations", C.c_int32)]
+                ("graph_launches", C.c_int32), ("graph_instantiations", C.c_int32),
+                ("contact_groups", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -475,11 +476,13 @@ class World:
 
     SCENE_INFO_KEYS = ("vertices", "triangles", "offdiag_entries", "mas_banks",
                        "slots", "pcg_workgroups", "coarse_banks",
-                       "collider_vertices", "pcg_launches", "systems")
+                       "collider_vertices", "pcg_launches", "systems",
+                       "pcg_lds_bytes", "pcg_dense")
 
     def scene_info(self) -> dict:
-        info = (C.c_int64 * 10)()
-        st = load_library().silk_hip_get_scene_info(self._ctx, info, 10)
+        k = len(self.SCENE_INFO_KEYS)
+        info = (C.c_int64 * k)()
+        st = load_library().silk_hip_get_scene_info(self._ctx, info, k)
         if st != 0:
             raise RuntimeError(self._wrap(st).to_string())
         return dict(zip(self.SCENE_INFO_KEYS, [int(v) for v in info]))

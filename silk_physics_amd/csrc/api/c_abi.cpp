@@ -8,6 +8,7 @@
 
 #include "api/context.hpp"
 #include "cpu/cpu_world.hpp"
+#include "hip/kernels.hpp"
 #include "hip/collision_kernels.hpp"
 #include "silk_hip.h"
 
@@ -380,13 +381,15 @@ int silk_hip_get_scene_info(silk_hip_ctx* ctx, int64_t* info, int32_t n) {
     const auto& S = *c.scene_;
     int ncb = 0;
     for (int b = 0; b < S.nbank; ++b) ncb += S.h_bank_level[b] > 0;
-    const int64_t v[10] = {S.nv, S.nf, S.nnz_off, S.nbank,
+    const int64_t v[12] = {S.nv, S.nf, S.nnz_off, S.nbank,
                            S.pers.nslot_total,
-                           S.use_persistent ? S.pers.grid : 0, ncb,
+                           c.persistent_on() ? S.pers.grid : 0, ncb,
                            S.coll ? S.coll->total_vertices() : 0,
-                           S.use_persistent ? static_cast<int64_t>(S.ps_batches.size()) : 0,
-                           S.nsys};
-    for (int i = 0; i < n && i < 10; ++i) info[i] = v[i];
+                           c.persistent_on() ? static_cast<int64_t>(S.ps_batches.size()) : 0,
+                           S.nsys,
+                           static_cast<int64_t>(silk::amd::pcg_persistent_lds(S.pers)),
+                           S.pers.dense};
+    for (int i = 0; i < n && i < 12; ++i) info[i] = v[i];
     return int{SILK_HIP_OK};
   });
 }

@@ -241,6 +241,17 @@ class HipCollision final : public CollisionWorld {
   std::vector<int> touched_;             // vertices with contacts
   std::vector<int> cvert_;               // contact vertices, first use order
   std::vector<int> vloc_;                // per cloth vertex: index in cvert_
+  // Independent groups of contacts (cloths coupled by no contact): contact
+  // vertices numbered group by group (gloc_, the order of gvert_), the
+  // contacts reordered by (group, colour) on the device, one workgroup per
+  // group in the update when every group fits the register variant.
+  std::vector<int> gloc_, gvert_, grp_h_, group_act_h_, group_n_h_;
+  int ngroup_ = 1;
+  bool grouped_ = false;
+  bool groups_on_ = true;  // SILK_HIP_CONTACT_GROUPS=0 turns grouping off
+  int groups_used_ = 0;    // largest grouped launch of the step
+  DevBuf<int> grp_dev_, group_act_, group_off_, group_cnt_;
+  DevBuf<ContactDev> c_grp_;  // (group, colour) order
   int max_color_ = 0;
   PinnedVec<ContactDev> hc_pin_;
   PinnedVec<int> col_pin_;  // old colours up, colouring results back
@@ -384,6 +395,7 @@ void HipCollision::finish_setup(const std::vector<float>& pos0) {
   w_.resize(std::max(1, nvc_));
   for (DevBuf<float>* b : {&p_, &eta_, &dual_}) b->resize(nc3);
   vloc_.assign(std::max(1, nvc_), -1);
+  if (const char* e = std::getenv("SILK_HIP_CONTACT_GROUPS")) groups_on_ = e[0] != '0';
   // SILK_HIP_CONTACT_VARIANT forces a contact-update variant (0..3) where it
   // fits, so that tests can run every one of them.
   if (const char* e = std::getenv("SILK_HIP_CONTACT_VARIANT"))
@@ -802,8 +814,62 @@ void HipCollision::color_and_upload(int n_old) {
   }
   n_active_ = static_cast<int>(touched_.size());
   const int nact = static_cast<int>(cvert_.size());
+  // Groups: cloths joined by a contact (union-find over cloths), numbered
+  // by their lowest cloth; contact vertices renumbered group by group.
+  const int ncl = static_cast<int>(S_.vert_off.size()) - 1;
+  auto cloth_of = [&](int v) {
+    return static_cast<int>(std::upper_bound(S_.vert_off.begin(), S_.vert_off.end(), v) -
+                            S_.vert_off.begin()) - 1;
+  };
+  std::vector<int> uf(std::max(1, ncl));
+  for (int i = 0; i < ncl; ++i) uf[i] = i;
+  auto find = [&](int a) {
+    while (uf[a] != a) a = uf[a] = uf[uf[a]];
+    return a;
+  };
+  for (int c = 0; c < n; ++c) {
+    int first = -1;
+    for (int k = 0; k < 4; ++k) {
+      const int v = contacts_[c].state[k];
+      if (v < 0) continue;
+      const int r = find(cloth_of(v));
+      if (first < 0) first = r;
+      else if (r != first) uf[std::max(r, first)] = std::min(r, first);
+    }
+  }
+  std::vector<int> gid(std::max(1, ncl), -1);
+  ngroup_ = 0;
+  std::vector<int> vgroup(nact);
+  for (int l = 0; l < nact; ++l) {
+    const int r = find(cloth_of(cvert_[l]));
+    if (gid[r] < 0) gid[r] = 0;  // marks a cloth with contacts
+  }
+  for (int r = 0; r < ncl; ++r)
+    if (gid[r] == 0 && find(r) == r) gid[r] = ngroup_++ + 1;
+  for (int l = 0; l < nact; ++l) vgroup[l] = gid[find(cloth_of(cvert_[l]))] - 1;
+  group_act_h_.assign(ngroup_ + 1, 0);
+  for (int l = 0; l < nact; ++l) ++group_act_h_[vgroup[l] + 1];
+  for (int g = 0; g < ngroup_; ++g) group_act_h_[g + 1] += group_act_h_[g];
+  gvert_.assign(nact, 0);
+  if (gloc_.size() < static_cast<size_t>(std::max(1, nvc_))) gloc_.assign(std::max(1, nvc_), -1);
+  {
+    std::vector<int> at(group_act_h_.begin(), group_act_h_.end() - 1);
+    for (int l = 0; l < nact; ++l) {
+      const int q = at[vgroup[l]]++;
+      gvert_[q] = cvert_[l];
+      gloc_[cvert_[l]] = q;
+    }
+  }
+  grp_h_.assign(std::max(1, n), 0);
+  group_n_h_.assign(std::max(1, ngroup_), 0);
+  for (int c = 0; c < n; ++c) {
+    int v = -1;
+    for (int k = 0; k < 4 && v < 0; ++k) v = contacts_[c].state[k];
+    grp_h_[c] = v >= 0 ? gid[find(cloth_of(v))] - 1 : 0;
+    ++group_n_h_[grp_h_[c]];
+  }
   int* act = act_pin_.reserve(std::max(1, nact));
-  std::copy(cvert_.begin(), cvert_.end(), act);
+  std::copy(gvert_.begin(), gvert_.end(), act);
   act_.reserve(std::max(1, nact));
   hip_check(hipMemcpyAsync(act_.get(), act, nact * sizeof(int), hipMemcpyHostToDevice,
                            st_),
@@ -817,7 +883,7 @@ void HipCollision::color_and_upload(int n_old) {
     const HostContact& c = contacts_[i];
     ContactDev& d = hc[i];
     for (int k = 0; k < 4; ++k) {
-      d.state[k] = c.state[k] >= 0 ? vloc_[c.state[k]] : -1;
+      d.state[k] = c.state[k] >= 0 ? gloc_[c.state[k]] : -1;
       // A static vertex's term is in affine; its coefficient is 0 here so
       // that the kernels add every term without a test.
       d.coef[k] = c.state[k] >= 0 ? static_cast<float>(c.coef[k]) : 0.0f;
@@ -862,6 +928,34 @@ void HipCollision::color_and_upload(int n_old) {
   ca.off = color_off_.get();
   ca.meta = col_meta_.get();
   hip_check(launch_color_contacts(st_, ca), "contact colouring");
+  // Reorder by (group, colour) right behind the colouring; whether the
+  // update uses that order is decided once the colour count is known.
+  const bool try_groups = groups_on_ && ngroup_ > 1;
+  if (try_groups) {
+    grp_dev_.reserve(std::max(1, n));
+    hip_check(hipMemcpyAsync(grp_dev_.get(), grp_h_.data(), n * sizeof(int),
+                             hipMemcpyHostToDevice, st_),
+              "contact groups");
+    group_act_.reserve(ngroup_ + 1);
+    hip_check(hipMemcpyAsync(group_act_.get(), group_act_h_.data(),
+                             (ngroup_ + 1) * sizeof(int), hipMemcpyHostToDevice, st_),
+              "group vertices");
+    const size_t nkey = static_cast<size_t>(ngroup_) * kMaxContactColors;
+    group_cnt_.reserve(nkey);
+    group_off_.reserve(nkey + 1);
+    c_grp_.reserve(std::max(1, n));
+    GroupArgs ga;
+    ga.cin = c_dev_.get();
+    ga.n = n;
+    ga.ngroup = ngroup_;
+    ga.group = grp_dev_.get();
+    ga.color = col_dev_.get();
+    ga.meta = col_meta_.get();
+    ga.count = group_cnt_.get();
+    ga.cout = c_grp_.get();
+    ga.off = group_off_.get();
+    hip_check(launch_group_contacts(st_, ga), "contact groups");
+  }
   int* back = hcol + n_old;
   hip_check(hipMemcpyAsync(back, col_meta_.get(), 3 * sizeof(int), hipMemcpyDeviceToHost,
                            st_),
@@ -881,6 +975,12 @@ void HipCollision::color_and_upload(int n_old) {
   const int ncol = back[0];
   max_color_ = ncol;
   color_off_h_.assign(back + 3, back + 3 + ncol + 1);
+  grouped_ = false;
+  if (try_groups) {
+    grouped_ = true;
+    for (int g = 0; g < ngroup_ && grouped_; ++g)
+      grouped_ = contact_group_fits(group_act_h_[g + 1] - group_act_h_[g], group_n_h_[g], ncol);
+  }
   color_.resize(n);
   for (int c = n_old; c < n; ++c) {
     const int col = back[kBack + (c - n_old)];
@@ -1047,6 +1147,17 @@ void HipCollision::contact_update(const float* x, const int* skip) {
   cs.dual = dual_.get();
   cs.pc = pc_.get();
   cs.force_variant = force_variant_;
+  if (grouped_ && force_variant_ < 0) {
+    cs.c = c_grp_.get();
+    cs.ngroup = ngroup_;
+    cs.group_off = group_off_.get();
+    cs.group_act = group_act_.get();
+    cs.group_n_max = *std::max_element(group_n_h_.begin(), group_n_h_.end());
+    int am = 0;
+    for (int g = 0; g < ngroup_; ++g) am = std::max(am, group_act_h_[g + 1] - group_act_h_[g]);
+    cs.group_nact_max = am;
+    groups_used_ = std::max(groups_used_, ngroup_);
+  }
   cs.sweeps = ctx_.contact_sweeps();
   cs.skip = skip;
   launch_contact_update(st_, cs, x);
@@ -1063,7 +1174,10 @@ ContactView HipCollision::contact_view() const {
 }
 
 void HipCollision::end_step(silk_hip_step_stats* stats) {
+  const int used = groups_used_;
+  groups_used_ = 0;
   if (!stats) return;
+  stats->contact_groups = used;
   stats->contacts = static_cast<int>(contacts_.size());
   stats->contact_colors =
       color_off_h_.empty() ? 0 : static_cast<int>(color_off_h_.size()) - 1;

@@ -607,8 +607,11 @@ void Context::build_scene() {
   // in its coarse workgroup's list.
   std::vector<int> ps_cb_rng(2 * ps_cb_member.size() + 2, 0);
   int need_max = 0;
-  int nhalo_max = 0, nwg_max = 0;
+  int nhalo_max = 0, nwg_max = 0, nc_max = 0;
   bool xch_fits = true;
+  // Coarse nodes of every slot's levels as positions in its owner
+  // workgroup's list, and of every halo entry in the reading workgroup's.
+  std::vector<int> ps_slot_cnl(ps_slot_cn.size(), -1), ps_halo_cnl;
   {
     const int nslot = static_cast<int>(ps_slot_row.size());
     std::vector<int> slot_wg(nslot, -1), slot_local(nslot, 0);
@@ -656,6 +659,10 @@ void Context::build_scene() {
             std::lower_bound(hl.begin(), hl.end(), c) - hl.begin());
       }
     });
+    std::vector<std::vector<int>> wg_slots(ps_wg);
+    for (int sl = 0; sl < nslot; ++sl)
+      if (slot_wg[sl] >= 0) wg_slots[slot_wg[sl]].push_back(sl);
+    ps_halo_cnl.assign(4 * ps_halo_slot.size(), -1);
     // Coarse bank cb of a system is solved by wavefront cb / ncw of its
     // coarse workgroup cb % ncw.  Per workgroup: the level-1 nodes whose sums
     // its banks read, member by member (a node read by several members is
@@ -681,11 +688,25 @@ void Context::build_scene() {
             if (ps_slot_cn[4 * static_cast<size_t>(sl) + l] >= 0)
               cs.push_back(ps_slot_cn[4 * static_cast<size_t>(sl) + l]);
         };
-        for (int sl = 0; sl < nslot; ++sl)
-          if (slot_wg[sl] == w && ps_slot_row[sl] >= 0) add_slot(sl);
+        for (int sl : wg_slots[w])
+          if (ps_slot_row[sl] >= 0) add_slot(sl);
         for (int e = ps_halo_ptr[w]; e < ps_halo_ptr[w + 1]; ++e) add_slot(ps_halo_slot[e]);
         std::sort(cs.begin(), cs.end());
         cs.erase(std::unique(cs.begin(), cs.end()), cs.end());
+        nc_max = std::max(nc_max, static_cast<int>(cs.size()));
+        auto pos = [&](int cn) {
+          if (cn < 0) return -1;
+          return static_cast<int>(std::lower_bound(cs.begin(), cs.end(), cn) - cs.begin());
+        };
+        for (int sl : wg_slots[w])
+          if (ps_slot_row[sl] >= 0)
+            for (int l = 0; l < 4; ++l)
+              ps_slot_cnl[4 * static_cast<size_t>(sl) + l] =
+                  pos(ps_slot_cn[4 * static_cast<size_t>(sl) + l]);
+        for (int e = ps_halo_ptr[w]; e < ps_halo_ptr[w + 1]; ++e)
+          for (int l = 0; l < 4; ++l)
+            ps_halo_cnl[4 * static_cast<size_t>(e) + l] =
+                pos(ps_slot_cn[4 * static_cast<size_t>(ps_halo_slot[e]) + l]);
         need_max = std::max(need_max, static_cast<int>(n1s.size()));
         ps_wg_n1.insert(ps_wg_n1.end(), n1s.begin(), n1s.end());
         ps_wg_n1_ptr.push_back(static_cast<int>(ps_wg_n1.size()));
@@ -814,6 +835,9 @@ void Context::build_scene() {
     if (ps_sell_val.empty()) ps_sell_val.push_back(0.0f);
     S.ps_sell_val.upload(ps_sell_val, st);
     up(S.ps_slot_cn, ps_slot_cn);
+    up(S.ps_slot_cnl, ps_slot_cnl);
+    if (ps_halo_cnl.empty()) ps_halo_cnl.push_back(-1);
+    up(S.ps_halo_cnl, ps_halo_cnl);
     up(S.ps_cb_bank, ps_cb_bank);
     up(S.ps_cb_member, ps_cb_member);
     up(S.ps_cb_rng, ps_cb_rng);
@@ -852,6 +876,8 @@ void Context::build_scene() {
     a.wg_c = S.ps_wg_c.get();
     a.sell_val = S.ps_sell_val.get();
     a.slot_cn = S.ps_slot_cn.get();
+    a.slot_cnl = S.ps_slot_cnl.get();
+    a.halo_cnl = S.ps_halo_cnl.get();
     a.cb_bank = S.ps_cb_bank.get();
     a.cb_member = S.ps_cb_member.get();
     a.cb_rng = S.ps_cb_rng.get();
@@ -876,6 +902,7 @@ void Context::build_scene() {
     a.max_iter = pcg_max_iter();
     a.need_max = need_max;
     a.ncn_max = max_ncn;
+    a.nc_max = nc_max;
     a.ncb_max = max_ncb;
     a.nhalo_max = nhalo_max;
     a.nwg_max = nwg_max;
@@ -890,8 +917,10 @@ void Context::build_scene() {
     if (const char* e = std::getenv("SILK_HIP_PCG_BATCH_WG")) cap_wg = std::max(1, std::atoi(e));
     // The one-workgroup-per-CU variant first; when a system alone does not
     // fit the device that way, the variant that puts two on a CU.
-    const size_t lds = pcg_persistent_lds(a);
     auto make_batches = [&](int dense) {
+      PersistDev av = a;
+      av.dense = dense;
+      const size_t lds = pcg_persistent_lds(av);
       std::vector<std::pair<int, int>>& out = S.ps_variant[dense];
       out.clear();
       if (ps_wg <= 0) return false;
@@ -1029,7 +1058,7 @@ int Context::pcg_solve(float rel_tol, bool factor, const RhsArgs& ra,
     hip_check(hipMemcpyAsync(S.x0_copy.get(), S.inner_x.get(), n3 * sizeof(float),
                              hipMemcpyDeviceToDevice, stream_),
               "warm start copy");
-  if (S.use_persistent) {
+  if (persistent_on()) {
     PersistDev a = persist_args(S, global_, pcg_max_iter());
     launch_persistent(a, rel_tol);
     S.h_state.resize(S.nsys);
@@ -1458,7 +1487,7 @@ bool Context::admm_device_finish(silk_hip_step_stats* stats) {
 
 bool Context::admm_solve(silk_hip_step_stats* stats) {
   Scene& S = *scene_;
-  if (S.use_persistent) return admm_solve_device(stats);
+  if (persistent_on()) return admm_solve_device(stats);
   ClothDev c = cloth_dev();
   const size_t n3 = 3 * static_cast<size_t>(S.nv);
   launch_fill_u32(stream_, S.pin_lambda.get(), static_cast<int64_t>(n3), 0u);
@@ -1662,7 +1691,7 @@ int Context::debug_pcg(const float* d, const float* b, float rel_tol, float* x,
   hip_check(hipMemsetAsync(S.inner_x.get(), 0, n3 * sizeof(float), stream_), "x0");
   launch_mas_factor(stream_, p, S.cd.get());
   int iters = 0;
-  if (S.use_persistent) {
+  if (persistent_on()) {
     PersistDev a = persist_args(S, global_, pcg_max_iter());
     a.dx_out = nullptr;
     if (launch_ms) hip_check(hipEventRecord(ev_[0], stream_), "event");
@@ -1679,7 +1708,7 @@ int Context::debug_pcg(const float* d, const float* b, float rel_tol, float* x,
                            stream_),
             "debug x");
   hip_check(hipStreamSynchronize(stream_), "debug pcg");
-  if (launch_ms && S.use_persistent)
+  if (launch_ms && persistent_on())
     hip_check(hipEventElapsedTime(launch_ms, ev_[0], ev_[1]), "elapsed");
   for (const auto& st : S.h_state) iters += st.iter;
   return iters;
@@ -1713,7 +1742,7 @@ int Context::autotune(int32_t* report) {
   // PCG: one or two persistent workgroups per CU.  Both run the same
   // arithmetic in the same order; a solve of the scene's matrix from x = 0
   // to 1e-6 on a fixed right-hand side times each.
-  if (S.use_persistent) {
+  if (persistent_on()) {
     const size_t nv = static_cast<size_t>(S.nv), n3 = 3 * nv;
     std::vector<float> d(nv), b(n3), x(n3);
     hip_check(hipMemcpyAsync(d.data(), S.d.get(), nv * sizeof(float),

@@ -130,7 +130,7 @@ struct Scene {
   DevBuf<PcgSysDev> ps_sys;
   DevBuf<int> ps_slot_row, ps_sell_off, ps_sell_width, ps_sell_lcol, ps_halo_ptr,
       ps_halo_slot, ps_wg_n1_ptr, ps_wg_n1, ps_wg_c_ptr, ps_wg_c, ps_slot_cn,
-      ps_cb_bank, ps_cb_member, ps_cb_rng, ps_abort;
+      ps_cb_bank, ps_cb_member, ps_cb_rng, ps_abort, ps_slot_cnl, ps_halo_cnl;
   DevBuf<float> ps_sell_val;
   DevBuf<unsigned long long> ps_tz, ps_tn, ps_tp, ps_tc, ps_tcp, ps_tdx;
   DevBuf<unsigned> ps_bar;
@@ -267,6 +267,14 @@ class Context {
   // the stream; a no-op when none is pending); false when it diverged.
   bool admm_device_finish(silk_hip_step_stats* st);
   bool admm_pending_ = false;
+  // The persistent PCG serves the scene's solves: it fits the device, and
+  // not BF16 banks in the two-workgroups-per-CU variant, whose solves
+  // diverged on the 256x256 sheet (cause not found; those solves take the
+  // multi-launch PCG instead).
+  bool persistent_on() const {
+    return scene_ && scene_->use_persistent &&
+           !(scene_->pers.dense && global_.mas_precision == SILK_HIP_MAS_BF16);
+  }
   // Ends the stream capture of the ADMM loop, updates (or instantiates) the
   // scene's executable graph from it and launches that on stream_.
   void launch_admm_graph(silk_hip_step_stats* st);

@@ -1270,6 +1270,20 @@ __global__ void __launch_bounds__(kContactRegThreads)
   using namespace dev;
   extern __shared__ float sh[];
   if (cs.skip != nullptr && *cs.skip != 0) return;
+  // One group per workgroup when the contacts are stored by group: its
+  // contacts, colour ranges and contact vertices; state indices are made
+  // relative to the group's first contact vertex when the contacts load.
+  int n = cs.n, cbase = 0, abase = 0;
+  const int* color_off = cs.color_off;
+  if (cs.ngroup > 1) {
+    const int b = static_cast<int>(blockIdx.x), K = cs.ncolor;
+    cbase = cs.group_off[b * K];
+    n = cs.group_off[(b + 1) * K] - cbase;
+    abase = cs.group_act[b];
+    nact = cs.group_act[b + 1] - abase;
+    color_off = cs.group_off + b * K;
+  }
+  const int* act = cs.act + abase;
   float* P = sh;
   int* off = reinterpret_cast<int*>(sh + 3 * static_cast<size_t>(nact));  // ncolor + 1
   float* NM = sh + contact_nm_word(nact, cs.ncolor);  // kNmLds: 4 per contact
@@ -1282,7 +1296,7 @@ __global__ void __launch_bounds__(kContactRegThreads)
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int kl = min(k0 + u * kContactRegThreads, n3 - 1);
-      const int i = cs.act[kl / 3];
+      const int i = act[kl / 3];
       const int k = 3 * i + kl % 3;
       xv[u] = x[k];
       ev[u] = cs.eta[k];
@@ -1294,7 +1308,7 @@ __global__ void __launch_bounds__(kContactRegThreads)
       if (kl < n3) P[kl] = wv[u] > 0.0f ? xv[u] - ev[u] / wv[u] : xv[u];
     }
   }
-  for (int i = tid; i <= cs.ncolor; i += kContactRegThreads) off[i] = cs.color_off[i];
+  for (int i = tid; i <= cs.ncolor; i += kContactRegThreads) off[i] = color_off[i] - cbase;
   __syncthreads();
   ContactDev c[kContactSlots];
   f3 corr[kContactSlots];
@@ -1305,8 +1319,11 @@ __global__ void __launch_bounds__(kContactRegThreads)
     const int e = s * kContactRegThreads + tid;
     color[s] = -1;
     corr[s] = f3{0.0f, 0.0f, 0.0f};
-    if (e < cs.n) {
-      c[s] = cs.c[e];
+    if (e < n) {
+      c[s] = cs.c[cbase + e];
+      if (abase != 0)
+        for (int k = 0; k < 4; ++k)
+          if (c[s].state[k] >= 0) c[s].state[k] -= abase;
       if constexpr (kNmLds)
         *reinterpret_cast<float4*>(NM + 4 * e) =
             float4{c[s].normal[0], c[s].normal[1], c[s].normal[2], c[s].mu};
@@ -1360,7 +1377,7 @@ __global__ void __launch_bounds__(kContactRegThreads)
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int kl = min(k0 + u * kContactRegThreads, n3 - 1);
-      const int i = cs.act[kl / 3];
+      const int i = act[kl / 3];
       kg[u] = 3 * i + kl % 3;
       wv[u] = cs.w[i];
       po[u] = cs.p[kg[u]];
@@ -1508,6 +1525,10 @@ bool contact_variant_fits(int v, int nact, int n, int ncolor) {
 
 }  // namespace
 
+bool contact_group_fits(int nact, int n, int ncolor) {
+  return contact_variant_fits(kContactRegNm, nact, n, ncolor);
+}
+
 int contact_update_variant(int nact, int n, int ncolor) {
   for (int v = kContactRegNm; v < kContactGlobal; ++v)
     if (contact_variant_fits(v, nact, n, ncolor)) return v;
@@ -1524,6 +1545,20 @@ void launch_contact_update(hipStream_t s, const ContactSet& cs,
     variant = cs.force_variant;
   const size_t p_bytes = contact_p_bytes(cs.nact, cs.ncolor);
   const size_t nm_bytes = contact_nm_bytes(cs.nact, cs.n, cs.ncolor);
+  if (cs.ngroup > 1) {
+    // One workgroup per independent group; the caller groups the contacts
+    // only when every group fits the register variant.
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_contact_update_reg<true>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(kContactLdsBytes));
+      attr = true;
+    }
+    const size_t bytes = contact_nm_bytes(cs.group_nact_max, cs.group_n_max, cs.ncolor);
+    k_contact_update_reg<true><<<cs.ngroup, kContactRegThreads, bytes, s>>>(cs, x, cs.nact);
+    return;
+  }
   if (variant == kContactRegNm) {
     static bool attr = false;
     if (!attr) {

@@ -242,6 +242,16 @@ struct ContactSet {
   int sweeps = 10;
   const int* skip = nullptr;        // no-op when *skip != 0
   int force_variant = -1;           // >= 0: use that variant (tests)
+  // Independent groups (no contact of one group shares a contact vertex
+  // with another): with ngroup > 1, c is stored group by group and colour by
+  // colour inside a group, the contact vertices group by group, and one
+  // workgroup updates each group.
+  int ngroup = 1;
+  const int* group_off = nullptr;   // ngroup * ncolor + 1: start of each
+                                    //   (group, colour) range, then the end
+  const int* group_act = nullptr;   // ngroup + 1: contact-vertex range per group
+  int group_n_max = 0;              // largest group: contacts
+  int group_nact_max = 0;           //   and contact vertices
 };
 
 /// Device contact colouring (contact_coloring.cu:69-373): first-fit in
@@ -262,11 +272,28 @@ struct ColorArgs {
 };
 hipError_t launch_color_contacts(hipStream_t s, const ColorArgs& a);
 
+/// Contacts of independent groups: the colour-ordered contacts (cin, ids
+/// set) reordered group by group, colour by colour inside a group.
+struct GroupArgs {
+  const ContactDev* cin = nullptr;  // n, colour by colour
+  int n = 0;
+  int ngroup = 0;
+  const int* group = nullptr;       // per contact id: its group
+  const int* color = nullptr;       // per contact id: its colour (1..)
+  const int* meta = nullptr;        // colouring result (meta[0]: colours)
+  int* count = nullptr;             // scratch: ngroup * kMaxContactColors
+  ContactDev* cout = nullptr;       // n
+  int* off = nullptr;               // ngroup * colours + 1 range starts
+};
+hipError_t launch_group_contacts(hipStream_t s, const GroupArgs& a);
+
 /// One ADMM contact update (contact_constraints.cu:246-264): single
 /// workgroup, colours separated by workgroup barriers.  p and the
 /// corrections live in LDS for the whole update when they fit.
 void launch_contact_update(hipStream_t s, const ContactSet& cs,
                            const float* x);
+/// A group of nact contact vertices and n contacts fits the grouped update.
+bool contact_group_fits(int nact, int n, int ncolor);
 
 /// Variants of launch_contact_update, in the order it prefers them.
 enum : int {

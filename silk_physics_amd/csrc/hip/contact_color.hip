@@ -207,7 +207,62 @@ __global__ void __launch_bounds__(kColorThreads) k_color_contacts(ColorArgs a) {
   }
 }
 
+// Counting sort of the colour-ordered contacts by (group, colour).  Inside
+// one (group, colour) range the order is arbitrary: contacts of one colour
+// share no movable vertex, so the update's result does not depend on it.
+__global__ void __launch_bounds__(kColorThreads) k_group_contacts(GroupArgs a) {
+  __shared__ int s_part[kColorThreads];
+  const int t = static_cast<int>(threadIdx.x);
+  const int ncol = a.meta[0];
+  if (ncol <= 0) return;  // the colouring failed; the caller reports it
+  const int nkey = a.ngroup * ncol;
+  for (int k = t; k < nkey; k += kColorThreads) a.count[k] = 0;
+  phase_end<false>();
+  for (int i = t; i < a.n; i += kColorThreads) {
+    const int id = a.cin[i].id;
+    atomicAdd(&a.count[a.group[id] * ncol + a.color[id] - 1], 1);
+  }
+  phase_end<false>();
+  // Exclusive scan of the counts: a chunk per thread, the chunk totals
+  // scanned in LDS.
+  const int chunk = (nkey + kColorThreads - 1) / kColorThreads;
+  const int k0 = min(nkey, t * chunk), k1 = min(nkey, k0 + chunk);
+  int sum = 0;
+  for (int k = k0; k < k1; ++k) sum += a.count[k];
+  s_part[t] = sum;
+  __syncthreads();
+  if (t == 0) {
+    int run = 0;
+    for (int j = 0; j < kColorThreads; ++j) {
+      const int v = s_part[j];
+      s_part[j] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  int run = s_part[t];
+  for (int k = k0; k < k1; ++k) {
+    const int v = a.count[k];
+    a.off[k] = run;
+    a.count[k] = run;  // cursor
+    run += v;
+  }
+  if (t == 0) a.off[nkey] = a.n;
+  phase_end<false>();
+  for (int i = t; i < a.n; i += kColorThreads) {
+    const ContactDev c = a.cin[i];
+    const int pos = atomicAdd(&a.count[a.group[c.id] * ncol + a.color[c.id] - 1], 1);
+    a.cout[pos] = c;
+  }
+}
+
 }  // namespace
+
+hipError_t launch_group_contacts(hipStream_t s, const GroupArgs& a) {
+  if (a.n <= 0 || a.ngroup <= 0) return hipErrorInvalidValue;
+  k_group_contacts<<<1, kColorThreads, 0, s>>>(a);
+  return hipGetLastError();
+}
 
 hipError_t launch_color_contacts(hipStream_t s, const ColorArgs& a) {
   if (a.n <= 0 || a.n_old < 0 || a.n_old > a.n || a.nact <= 0) return hipErrorInvalidValue;

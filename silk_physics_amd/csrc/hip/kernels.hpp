@@ -331,6 +331,10 @@ struct PersistDev {
   const int* wg_c_ptr = nullptr;    // per workgroup: coarse nodes whose
   const int* wg_c = nullptr;        //   corrections its slots and halo add
   const int* slot_cn = nullptr;     // 4 per slot: coarse node (cn0-relative) per level
+  const int* slot_cnl = nullptr;    // 4 per slot: the same nodes as positions in the
+                                    //   owner workgroup's wg_c list, -1 if absent
+  const int* halo_cnl = nullptr;    // 4 per halo entry: positions in the reading
+                                    //   workgroup's wg_c list, -1 if absent
   const int* cb_bank = nullptr;     // per coarse bank: global bank id (bank_inv)
   const int* cb_member = nullptr;   // 32 per coarse bank: node (cn0-relative), -1
   const int* cb_rng = nullptr;      // 2 per coarse bank member: its nodes'
@@ -375,7 +379,8 @@ struct PersistDev {
   int restart = 0;                  // 1: a restart of unconverged systems
                                     //   (counted in ctl->pcg_restarts)
   int need_max = 0;     // max level-1 nodes listed for one workgroup (LDS)
-  int ncn_max = 0;      // max coarse nodes of a system (LDS)
+  int ncn_max = 0;      // max coarse nodes of a system
+  int nc_max = 0;       // max coarse nodes one workgroup's slots add (LDS)
   int ncb_max = 0;      // max coarse banks of a system (LDS)
   int nhalo_max = 0;    // max halo slots of a workgroup (LDS)
   int nwg_max = 0;      // max workgroups of a system (LDS)

@@ -283,6 +283,21 @@ __device__ __forceinline__ f3 lds4(const float* base, int i) {
   return {v.x, v.y, v.z};
 }
 
+// Records of the level-1 sums: 16 bytes (one LDS read) in the
+// one-workgroup-per-CU variant, 12 bytes in the two-per-CU variant, whose
+// workgroups must fit half of the CU's LDS.
+template <int kWavesPerSimd>
+constexpr int kQsStride = kWavesPerSimd == 2 ? 3 : 4;
+
+template <int kStride>
+__device__ __forceinline__ f3 ldsq(const float* base, int i) {
+  if constexpr (kStride == 4) {
+    return lds4(base, i);
+  } else {
+    return {base[kStride * i], base[kStride * i + 1], base[kStride * i + 2]};
+  }
+}
+
 __device__ __forceinline__ void sts4(float* base, int i, f3 v) {
   base[4 * i] = v.x;
   base[4 * i + 1] = v.y;
@@ -371,11 +386,12 @@ __global__ void __launch_bounds__(kPcgThreads, kWavesPerSimd)
   const int nh = a.halo_ptr[bx + 1] - h0;
   const size_t hl = static_cast<size_t>(kPcgThreads) + a.nhalo_max;
 
-  float* czl = dyn;                                            // 4 x (ncn_max + 1)
-  float* qs = czl + 4 * static_cast<size_t>(a.ncn_max + 1);   // 4 x need_max
+  float* czl = dyn;                                            // 4 x (nc_max + 1)
+  float* qs = czl + 4 * static_cast<size_t>(a.nc_max + 1);    // 4 x need_max
   // Per-wavefront partials as doubles (two floats, partial e = wavefront
   // e % kPcgWaves of workgroup e / kPcgWaves): p.q, r.z0 and r.r.
-  float* ppq = qs + 4 * static_cast<size_t>(a.need_max);      // 8 x nwg_max
+  constexpr int kQs = kQsStride<kWavesPerSimd>;
+  float* ppq = qs + ((kQs * static_cast<size_t>(a.need_max) + 3) & ~size_t{3});  // 8 x nwg_max
   float* prz = ppq + 8 * static_cast<size_t>(a.nwg_max);      // 8 x nwg_max
   float* prr = prz + 8 * static_cast<size_t>(a.nwg_max);      // 8 x nwg_max
   float* cpw = prr + 8 * static_cast<size_t>(a.nwg_max);      // 4 x ncb_max
@@ -407,7 +423,7 @@ __global__ void __launch_bounds__(kPcgThreads, kWavesPerSimd)
   const int re = has_bank ? a.cb_rng[2 * (cbase + (lane & 31)) + 1] : 0;
   const int nm = __popcll(__ballot(lane < kBankSlots && mem_s >= 0));
   const bool deep = __ballot(re - rb > 1) != 0;
-  if (t < 4) czl[4 * a.ncn_max + t] = 0.0f;
+  if (t < 4) czl[4 * a.nc_max + t] = 0.0f;
 
   // This slot's row of its level-0 bank inverse, starting with the entries of
   // the slots in its own row of 16 lanes: winv[k] is entry k ^ (16 * row).
@@ -448,23 +464,27 @@ __global__ void __launch_bounds__(kPcgThreads, kWavesPerSimd)
     nval[k] = v ? a.sell_val[e_off + k * kSlice + e_lane] : 0.0f;
   }
   // Coarse nodes of the slot (and of the halo slots this thread completes)
-  // per level; an absent level points at record ncn_max of czl, which stays
-  // zero, so that the additions need no branch.
-  const int zrec = a.ncn_max;
+  // per level.  czl holds the corrections of the workgroup's own list of
+  // coarse nodes (wg_c), so the additions index it by list position; an
+  // absent level points at record nc_max, which stays zero, so that the
+  // additions need no branch.
+  const int zrec = a.nc_max;
   int scn[4] = {-1, -1, -1, -1};
+  int scz[4] = {zrec, zrec, zrec, zrec};
   if (in_bank && coarse)
-    for (int l = 0; l < 4; ++l) scn[l] = a.slot_cn[4 * slot + l];
-  int scz[4];
-#pragma unroll
-  for (int l = 0; l < 4; ++l) scz[l] = scn[l] >= 0 ? scn[l] : zrec;
+    for (int l = 0; l < 4; ++l) {
+      scn[l] = a.slot_cn[4 * slot + l];
+      const int q = a.slot_cnl[4 * slot + l];
+      scz[l] = q >= 0 ? q : zrec;
+    }
   int hsc[kPcgHaloPerThread][4];
 #pragma unroll
   for (int m = 0; m < kPcgHaloPerThread; ++m) {
     const int hh = t + kPcgThreads * m;
 #pragma unroll
     for (int l = 0; l < 4; ++l) {
-      const int cn = coarse && hh < nh ? a.slot_cn[4 * a.halo_slot[h0 + hh] + l] : -1;
-      hsc[m][l] = cn >= 0 ? cn : zrec;
+      const int q = coarse && hh < nh ? a.halo_cnl[4 * (h0 + hh) + l] : -1;
+      hsc[m][l] = q >= 0 ? q : zrec;
     }
   }
   const float dd = own ? a.d[row] : 0.0f;
@@ -503,7 +523,7 @@ __global__ void __launch_bounds__(kPcgThreads, kWavesPerSimd)
     if (i < nw1) {
       const int n = a.wg_n1[n1_0 + i / 3];
       addr = a.tn + 3 * static_cast<size_t>(n) + i % 3;
-      dest = static_cast<int>(qs - dyn) + 4 * (i / 3) + i % 3;
+      dest = static_cast<int>(qs - dyn) + kQs * (i / 3) + i % 3;
     } else if (i < aw1) {
       const int w = (i - nw1) / kWaveWords, h = (i - nw1) % kWaveWords;
       addr = a.tp + kPcgPartWords * static_cast<size_t>(S.wg0 + w) + h;
@@ -528,9 +548,9 @@ __global__ void __launch_bounds__(kPcgThreads, kWavesPerSimd)
       addr = a.tz + 3 * static_cast<size_t>(a.halo_slot[h0 + hh]) + c;
       dest = static_cast<int>(zl - dyn) + 4 * (kPcgThreads + hh) + c;
     } else {
-      const int n = a.wg_c[c_0 + (i - hw2) / 3], c = (i - hw2) % 3;
-      addr = a.tc + 3 * static_cast<size_t>(S.cn0 + n) + c;
-      dest = static_cast<int>(czl - dyn) + 4 * n + c;
+      const int j = (i - hw2) / 3, c = (i - hw2) % 3;
+      addr = a.tc + 3 * static_cast<size_t>(S.cn0 + a.wg_c[c_0 + j]) + c;
+      dest = static_cast<int>(czl - dyn) + 4 * j + c;
     }
   };
   Share x1, x2;
@@ -597,7 +617,7 @@ __global__ void __launch_bounds__(kPcgThreads, kWavesPerSimd)
   const int grp = nm <= 1 ? 64 : nm <= 2 ? 32 : nm <= 4 ? 16 : nm <= 8 ? 8 : nm <= 16 ? 4 : 2;
   auto sum_nodes = [&](const float* src) -> d3 {
     if (!deep) {
-      const f3 v = rb < re ? lds4(src, rb) : f3{0.0f, 0.0f, 0.0f};
+      const f3 v = rb < re ? ldsq<kQs>(src, rb) : f3{0.0f, 0.0f, 0.0f};
       return {v.x, v.y, v.z};
     }
     const int k = lane / grp, sub = lane % grp;
@@ -607,7 +627,7 @@ __global__ void __launch_bounds__(kPcgThreads, kWavesPerSimd)
     for (int e0 = kb + sub; e0 < ke; e0 += 4 * grp) {
       f3 cv[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) cv[u] = lds4(src, min(e0 + grp * u, ke - 1));
+      for (int u = 0; u < 4; ++u) cv[u] = ldsq<kQs>(src, min(e0 + grp * u, ke - 1));
 #pragma unroll
       for (int u = 0; u < 4; ++u)
         if (e0 + grp * u < ke) v = {v.x + cv[u].x, v.y + cv[u].y, v.z + cv[u].z};
@@ -951,8 +971,9 @@ const void* persistent_fn(int dense) {
 
 size_t pcg_persistent_lds(const PersistDev& a) {
   const size_t hl = static_cast<size_t>(kPcgThreads) + a.nhalo_max;
-  return sizeof(float) * (4 * static_cast<size_t>(a.ncn_max + 1) +
-                          4 * static_cast<size_t>(a.need_max) +
+  const size_t qs_stride = a.dense ? kQsStride<2> : kQsStride<1>;
+  return sizeof(float) * (4 * static_cast<size_t>(a.nc_max + 1) +
+                          ((qs_stride * static_cast<size_t>(a.need_max) + 3) & ~size_t{3}) +
                           24 * static_cast<size_t>(a.nwg_max) +
                           4 * static_cast<size_t>(a.ncb_max) + 8 * hl);
 }
