@@ -5,7 +5,9 @@ mas_preconditioner.cu:657-702,750-831).  The preconditioner's precision may
 change the iteration count, never the answer: each form is held to the FP64
 oracle's PCG solution of the same system, and the whole step to the FP32
 form.  At this size the persistent PCG needs more workgroups than the
-device has CUs: it runs the variant that puts two workgroups on a CU."""
+device has CUs: it runs the variant that puts two workgroups on a CU for
+FP32 and FP16 banks; BF16 banks on that variant take the multi-launch PCG
+(their persistent solve of this sheet diverged)."""
 import numpy as np
 import pytest
 
